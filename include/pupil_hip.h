@@ -1,0 +1,249 @@
+/*
+ * pupil_hip.h -- C ABI of the MI355X-native wavefront MIS path tracer.
+ *
+ * This is the boundary that replaces the OptiX path-tracing pass of
+ * PupilOptixLab.  Every entry point names the reference interface it stands
+ * in for.  Plain C types only: pointers, sizes and POD structs.  All calls
+ * return 0 on success or a negative pt_status; no call aborts on bad input.
+ * A context is not thread-safe: callers serialize, as the reference's single
+ * render thread does (system/system.cpp:93-106).
+ *
+ * Buffer conventions follow the reference pass: float4 RGBA with w = 1,
+ * row 0 = bottom of the image (util/camera.cpp:9-16).
+ */
+#ifndef PUPIL_HIP_H
+#define PUPIL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum pt_status {
+    PT_OK = 0,
+    PT_ERR_INVALID_ARG = -1,
+    PT_ERR_HIP = -2,
+    PT_ERR_NO_SCENE = -3,
+    PT_ERR_CAPACITY = -4,
+    PT_ERR_IO = -5,
+    PT_ERR_PARSE = -6,
+    PT_ERR_OOM = -7,
+    PT_ERR_UNSUPPORTED = -8,
+    PT_ERR_NO_BVH = -9
+};
+
+/* Material types: EMatType order of decl/material_decl.inl:3-9. */
+enum pt_material_type {
+    PT_MAT_DIFFUSE = 1,
+    PT_MAT_DIELECTRIC = 2,
+    PT_MAT_ROUGH_DIELECTRIC = 3,
+    PT_MAT_CONDUCTOR = 4,
+    PT_MAT_ROUGH_CONDUCTOR = 5,
+    PT_MAT_PLASTIC = 6,
+    PT_MAT_ROUGH_PLASTIC = 7
+};
+
+/* Texture kinds of util/texture.h (RGB, checkerboard; bitmap reserved). */
+enum pt_texture_type { PT_TEX_RGB = 0, PT_TEX_CHECKER = 1, PT_TEX_BITMAP = 2 };
+
+/* Shape flags (render/geometry.h:14-21 TriMesh flags, flattened). */
+enum pt_shape_flags {
+    PT_SHAPE_FLIP_NORMALS = 1,
+    PT_SHAPE_FLIP_TEXCOORDS = 2,
+    PT_SHAPE_HAS_NORMALS = 4,
+    PT_SHAPE_HAS_TEXCOORDS = 8,
+    PT_SHAPE_NEG_FACE_NORMAL = 16
+};
+
+enum pt_env_type { PT_ENV_NONE = 0, PT_ENV_CONSTANT = 1 };
+
+/* cuda::Texture (cuda/texture.h:10-27): a color, a checkerboard of two
+ * colors, or a bitmap; uv_row0/uv_row1 transform (u, v, 0, 1). */
+typedef struct pt_texture_desc {
+    uint32_t type;
+    uint32_t bitmap;
+    float color0[3];
+    float color1[3];
+    float uv_row0[4];
+    float uv_row1[4];
+} pt_texture_desc;
+
+/* resource::Material (resource/material.h:59-74).  tex[] slots by type:
+ *   diffuse:          0 reflectance
+ *   dielectric:       0 specular_reflectance, 1 specular_transmittance
+ *   rough dielectric: 0, 1 as dielectric, 2 alpha
+ *   conductor:        0 specular_reflectance, 1 eta, 2 k
+ *   rough conductor:  0, 1, 2 as conductor, 3 alpha
+ *   plastic:          0 diffuse_reflectance, 1 specular_reflectance
+ *   rough plastic:    0, 1 as plastic, 2 alpha                              */
+typedef struct pt_material_desc {
+    uint32_t type;
+    uint32_t twosided;
+    uint32_t nonlinear;
+    float int_ior;
+    float ext_ior;
+    pt_texture_desc tex[4];
+} pt_material_desc;
+
+/* One shape instance flattened to world space (resource::ShapeInstance,
+ * resource/shape.h:87-96 + world::RenderObject): a contiguous triangle
+ * range, its material, and an area emitter when is_emitter != 0. */
+typedef struct pt_shape_desc {
+    uint32_t material;
+    uint32_t flags;
+    uint32_t first_tri;
+    uint32_t num_tris;
+    int32_t is_emitter;
+    pt_texture_desc radiance;
+} pt_shape_desc;
+
+/* optix::Camera (render/camera.h:7-10): row-major 4x4 matrices applied to
+ * column vectors. */
+typedef struct pt_camera_desc {
+    float sample_to_camera[16];
+    float camera_to_world[16];
+} pt_camera_desc;
+
+/* The world handed to the pass on SceneLoad (pt_pass.cpp:127-240):
+ * world-space vertices (normals are M^-T n, not normalized), triangles,
+ * shapes, materials, the environment and the camera. */
+typedef struct pt_scene_desc {
+    uint32_t num_vertices;
+    uint32_t num_triangles;
+    uint32_t num_shapes;
+    uint32_t num_materials;
+    const float *positions;  /* 3 per vertex */
+    const float *normals;    /* 3 per vertex, or NULL */
+    const float *texcoords;  /* 2 per vertex, or NULL */
+    const uint32_t *indices; /* 3 per triangle */
+    const pt_shape_desc *shapes;
+    const pt_material_desc *materials;
+    uint32_t env_type;
+    float env_radiance[3];
+    pt_camera_desc camera;
+    uint32_t width;
+    uint32_t height;
+    uint32_t max_depth;
+} pt_scene_desc;
+
+typedef struct pt_bvh_opts {
+    uint32_t num_bins; /* SAH bins, 0 = default (32) */
+    uint32_t threads;  /* host build threads, 0 = all */
+} pt_bvh_opts;
+
+enum pt_mis_mode { PT_MIS_REFERENCE = 0, PT_MIS_CONSISTENT = 1 };
+
+enum pt_render_flags {
+    PT_RENDER_AOV = 1,        /* write albedo / normal / test / primitive id */
+    PT_RENDER_FP64_ACCUM = 2, /* also keep a double-precision sum */
+    PT_RENDER_TIMING = 4      /* record per-kernel HIP event times */
+};
+
+/* Launch parameters (example/path_tracer/type.h:9-33 config + pass knobs). */
+typedef struct pt_render_params {
+    uint32_t spp;            /* frames (samples per pixel) to add */
+    uint32_t max_depth;      /* 0 = scene value */
+    uint32_t rr_depth;       /* Russian roulette when depth > rr_depth (reference: 2) */
+    uint32_t mis_mode;       /* pt_mis_mode */
+    uint32_t accumulate;     /* running mean over frames (accumulated_flag) */
+    uint32_t flags;          /* pt_render_flags */
+    uint32_t spp_per_launch; /* frames traced concurrently, 0 = automatic */
+    uint32_t block_size;     /* kernel block size, 0 = automatic */
+} pt_render_params;
+
+enum pt_buffer {
+    PT_BUF_FRAME = 0,   /* "final result" float4 */
+    PT_BUF_ACCUM = 1,   /* "pt accum buffer" float4 */
+    PT_BUF_ALBEDO = 2,  /* "albedo" float3 */
+    PT_BUF_NORMAL = 3,  /* "normal" float3 */
+    PT_BUF_TEST = 4,    /* "test" float */
+    PT_BUF_PRIM_ID = 5, /* primary-hit triangle id, uint32 (0xFFFFFFFF = miss) */
+    PT_BUF_ACCUM64 = 6  /* double3 sum of samples (PT_RENDER_FP64_ACCUM) */
+};
+
+typedef struct pt_stats {
+    uint64_t samples;        /* camera samples rendered since the last reset */
+    uint64_t rays_extension; /* camera + bounce rays traced */
+    uint64_t rays_shadow;    /* shadow rays traced */
+    uint64_t node_visits;    /* only with counting enabled */
+    uint64_t tri_tests;
+    double ms_total;         /* wall time of pt_render calls */
+    double ms_raygen, ms_trace, ms_shade, ms_shadow, ms_accumulate; /* HIP event sums */
+    uint64_t launches_trace, launches_shade, launches_shadow;
+    uint32_t bvh_nodes, bvh_leaf_tris, bvh_depth;
+    double bvh_build_ms;
+} pt_stats;
+
+typedef struct pt_ctx pt_ctx;
+
+/* cuda::Context::Init + PTPass ctor (cuda/context.cpp:9-49,
+ * pt_pass.cpp:29-37): bind a device, create the pass stream. */
+int pt_create(int hip_device, pt_ctx **out);
+/* System::Destroy / ~PTPass. */
+void pt_destroy(pt_ctx *ctx);
+/* PTPass::SetScene (pt_pass.cpp:127-240) + World::LoadScene packing
+ * (world/world.cpp:108-146, world/emitter.cpp:169-337): copies the arrays,
+ * builds material and emitter tables, allocates frame buffers. */
+int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *scene);
+/* GASManager/IASManager builds (world/gas_manager.cpp:69-245,
+ * world/ias_manager.cpp:29-114): SAH 8-wide compressed BVH. */
+int pt_build_bvh(pt_ctx *ctx, const pt_bvh_opts *opts);
+/* CameraHelper::GetCudaMemory (world/camera.cpp:72-92) after a camera
+ * change; restarts accumulation like EWorldEvent::CameraChange. */
+int pt_set_camera(pt_ctx *ctx, const float sample_to_camera[16], const float camera_to_world[16]);
+/* m_dirty restart of PTPass::OnRun (pt_pass.cpp:40-49). */
+int pt_reset_accumulation(pt_ctx *ctx);
+/* PTPass::OnRun x spp (pt_pass.cpp:39-57) + optix::Pass::Run
+ * (optix/pass.h:71-89): renders `spp` frames; blocks until done. */
+int pt_render(pt_ctx *ctx, const pt_render_params *params);
+/* BufferManager::GetBuffer + device-to-host copy (system/buffer.cpp:34-99). */
+int pt_read_buffer(pt_ctx *ctx, int which, void *host_dst, size_t bytes);
+int pt_get_stats(pt_ctx *ctx, pt_stats *out);
+/* Node-visit / triangle-test counting in the traversal kernels (off by default). */
+int pt_set_counting(pt_ctx *ctx, int enable);
+/* Last error text of the context (or of the last failed pt_create when ctx is NULL). */
+const char *pt_last_error(const pt_ctx *ctx);
+
+/* Traces caller-supplied rays against the built BVH on the GPU (closest hit
+ * unless any_hit != 0).  rays: 8 floats each (origin xyz, tmin, dir xyz,
+ * tmax).  hits: 4 words each (t, primitive id, b1, b2). */
+int pt_trace_rays(pt_ctx *ctx, const float *rays, uint32_t count, int any_hit, void *hits);
+
+/* ---- host scene helpers (resource::Scene / World, headless) ----------- */
+typedef struct pt_host_scene pt_host_scene;
+
+enum pt_scene_kind {
+    PT_SCENE_CORNELL = 1,   /* closed box, two rotated boxes, one area light */
+    PT_SCENE_TERRAIN = 2,   /* fractal heightfield + icospheres, 7 materials, 4 lights + env */
+    PT_SCENE_DOOR_ROOM = 3  /* room lit through a narrow door gap */
+};
+
+typedef struct pt_gen_params {
+    uint64_t seed;
+    uint32_t width, height; /* 0 = scene default */
+    uint32_t grid;          /* heightfield vertices per side (terrain), 0 = default 1024 */
+    uint32_t spheres;       /* icospheres (terrain), 0 = default 256 */
+    uint32_t max_depth;     /* 0 = scene default */
+} pt_gen_params;
+
+/* Scene::LoadFromXML (resource/scene.cpp:27-56) for the mitsuba subset. */
+int pt_scene_load_xml(const char *path, pt_host_scene **out, char *err, size_t err_len);
+int pt_scene_generate(int kind, const pt_gen_params *params, pt_host_scene **out);
+/* Pointers in the returned desc stay valid until pt_scene_destroy. */
+int pt_scene_get_desc(const pt_host_scene *scene, pt_scene_desc *out);
+void pt_scene_destroy(pt_host_scene *scene);
+/* Writes RGB float EXR (uncompressed scanlines), bottom row first in the
+ * buffer as in util/texture.cpp:23-85. */
+int pt_write_exr(const char *path, const float *rgba, uint32_t width, uint32_t height);
+/* Host build + traversal of the same wide BVH the GPU uses (for checks on
+ * machines without a GPU). */
+int pt_host_trace_rays(const pt_scene_desc *scene, const float *rays, uint32_t count, int any_hit, void *hits,
+                       uint64_t *node_visits, uint64_t *tri_tests);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PUPIL_HIP_H */

@@ -1,0 +1,311 @@
+"""MI355X-native wavefront MIS path tracer (PupilOptixLab's path-tracing pass).
+
+Python mirror of the pass interface over the C ABI in ``include/pupil_hip.h``.
+The compute path is the gfx950 kernels in ``libpupil_hip.so`` (built in-tree by
+``__graft_entry__.build()`` or ``make -C pupiloptixlab_amd``); importing this
+package fails loudly when that library is missing.
+
+``PathTracer`` stands in for ``Pupil::pt::PTPass`` (example/path_tracer/
+pt_pass.h:21-41): ``set_scene`` is ``SetScene`` (pt_pass.cpp:127-240),
+``render`` runs ``OnRun`` once per frame (pt_pass.cpp:39-57) and ``read``
+fetches the named buffers the pass allocates ("final result", "pt accum
+buffer", "albedo", "normal", "test").
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpupil_hip.so")
+
+PT_OK = 0
+STATUS = {
+    0: "PT_OK", -1: "PT_ERR_INVALID_ARG", -2: "PT_ERR_HIP", -3: "PT_ERR_NO_SCENE", -4: "PT_ERR_CAPACITY",
+    -5: "PT_ERR_IO", -6: "PT_ERR_PARSE", -7: "PT_ERR_OOM", -8: "PT_ERR_UNSUPPORTED", -9: "PT_ERR_NO_BVH",
+}
+
+MAT_DIFFUSE, MAT_DIELECTRIC, MAT_ROUGH_DIELECTRIC, MAT_CONDUCTOR, MAT_ROUGH_CONDUCTOR, MAT_PLASTIC, \
+    MAT_ROUGH_PLASTIC = range(1, 8)
+TEX_RGB, TEX_CHECKER, TEX_BITMAP = 0, 1, 2
+SHAPE_FLIP_NORMALS, SHAPE_FLIP_TEXCOORDS, SHAPE_HAS_NORMALS, SHAPE_HAS_TEXCOORDS, SHAPE_NEG_FACE_NORMAL = \
+    1, 2, 4, 8, 16
+ENV_NONE, ENV_CONSTANT = 0, 1
+MIS_REFERENCE, MIS_CONSISTENT = 0, 1
+RENDER_AOV, RENDER_FP64_ACCUM, RENDER_TIMING = 1, 2, 4
+BUF_FRAME, BUF_ACCUM, BUF_ALBEDO, BUF_NORMAL, BUF_TEST, BUF_PRIM_ID, BUF_ACCUM64 = range(7)
+SCENE_CORNELL, SCENE_TERRAIN, SCENE_DOOR_ROOM = 1, 2, 3
+MISS = 0xFFFFFFFF
+
+
+class TextureDesc(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("bitmap", C.c_uint32), ("color0", C.c_float * 3),
+                ("color1", C.c_float * 3), ("uv_row0", C.c_float * 4), ("uv_row1", C.c_float * 4)]
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("twosided", C.c_uint32), ("nonlinear", C.c_uint32),
+                ("int_ior", C.c_float), ("ext_ior", C.c_float), ("tex", TextureDesc * 4)]
+
+
+class ShapeDesc(C.Structure):
+    _fields_ = [("material", C.c_uint32), ("flags", C.c_uint32), ("first_tri", C.c_uint32),
+                ("num_tris", C.c_uint32), ("is_emitter", C.c_int32), ("radiance", TextureDesc)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("sample_to_camera", C.c_float * 16), ("camera_to_world", C.c_float * 16)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("num_shapes", C.c_uint32),
+                ("num_materials", C.c_uint32), ("positions", C.POINTER(C.c_float)),
+                ("normals", C.POINTER(C.c_float)), ("texcoords", C.POINTER(C.c_float)),
+                ("indices", C.POINTER(C.c_uint32)), ("shapes", C.POINTER(ShapeDesc)),
+                ("materials", C.POINTER(MaterialDesc)), ("env_type", C.c_uint32),
+                ("env_radiance", C.c_float * 3), ("camera", CameraDesc), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("max_depth", C.c_uint32)]
+
+
+class BvhOpts(C.Structure):
+    _fields_ = [("num_bins", C.c_uint32), ("threads", C.c_uint32)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("spp", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
+                ("mis_mode", C.c_uint32), ("accumulate", C.c_uint32), ("flags", C.c_uint32),
+                ("spp_per_launch", C.c_uint32), ("block_size", C.c_uint32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64),
+                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("ms_total", C.c_double),
+                ("ms_raygen", C.c_double), ("ms_trace", C.c_double), ("ms_shade", C.c_double),
+                ("ms_shadow", C.c_double), ("ms_accumulate", C.c_double), ("launches_trace", C.c_uint64),
+                ("launches_shade", C.c_uint64), ("launches_shadow", C.c_uint64), ("bvh_nodes", C.c_uint32),
+                ("bvh_leaf_tris", C.c_uint32), ("bvh_depth", C.c_uint32), ("bvh_build_ms", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GenParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32), ("grid", C.c_uint32),
+                ("spheres", C.c_uint32), ("max_depth", C.c_uint32)]
+
+
+# Every symbol the C header declares, with its ctypes signature.
+_SIGNATURES = {
+    "pt_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "pt_destroy": (None, [C.c_void_p]),
+    "pt_set_scene": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
+    "pt_build_bvh": (C.c_int, [C.c_void_p, C.POINTER(BvhOpts)]),
+    "pt_set_camera": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "pt_reset_accumulation": (C.c_int, [C.c_void_p]),
+    "pt_render": (C.c_int, [C.c_void_p, C.POINTER(RenderParams)]),
+    "pt_read_buffer": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
+    "pt_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "pt_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
+    "pt_last_error": (C.c_char_p, [C.c_void_p]),
+    "pt_trace_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),
+    "pt_scene_load_xml": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]),
+    "pt_scene_generate": (C.c_int, [C.c_int, C.POINTER(GenParams), C.POINTER(C.c_void_p)]),
+    "pt_scene_get_desc": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
+    "pt_scene_destroy": (None, [C.c_void_p]),
+    "pt_write_exr": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    "pt_host_trace_rays": (C.c_int, [C.POINTER(SceneDesc), C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
+                                     C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+}
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Loads libpupil_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "or `make -C pupiloptixlab_amd` (there is no CPU fallback)")
+        handle = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+class PupilError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: {STATUS.get(code, code)}")
+        self.code = code
+
+
+def _check(code: int, what: str, ctx=None):
+    if code != PT_OK:
+        msg = lib().pt_last_error(ctx)
+        raise PupilError(code, f"{what} ({msg.decode() if msg else ''})")
+
+
+class HostScene:
+    """A world-space scene on the host (resource::Scene + World packing)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self.desc = SceneDesc()
+        _check(lib().pt_scene_get_desc(self._h, C.byref(self.desc)), "pt_scene_get_desc")
+
+    @classmethod
+    def generate(cls, kind: int, seed: int = 0, width: int = 0, height: int = 0, grid: int = 0,
+                 spheres: int = 0, max_depth: int = 0) -> "HostScene":
+        gp = GenParams(seed, width, height, grid, spheres, max_depth)
+        h = C.c_void_p()
+        _check(lib().pt_scene_generate(kind, C.byref(gp), C.byref(h)), "pt_scene_generate")
+        return cls(h)
+
+    @classmethod
+    def load_xml(cls, path: str) -> "HostScene":
+        h = C.c_void_p()
+        err = C.create_string_buffer(512)
+        code = lib().pt_scene_load_xml(path.encode(), C.byref(h), err, 512)
+        if code != PT_OK:
+            raise PupilError(code, f"pt_scene_load_xml({path}): {err.value.decode()}")
+        return cls(h)
+
+    def close(self):
+        if self._h:
+            lib().pt_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # numpy views (valid while the scene lives)
+    def positions(self):
+        d = self.desc
+        return np.ctypeslib.as_array(d.positions, shape=(d.num_vertices, 3))
+
+    def indices(self):
+        d = self.desc
+        return np.ctypeslib.as_array(d.indices, shape=(d.num_triangles, 3))
+
+    def shapes(self):
+        d = self.desc
+        return [d.shapes[i] for i in range(d.num_shapes)]
+
+    def materials(self):
+        d = self.desc
+        return [d.materials[i] for i in range(d.num_materials)]
+
+    @property
+    def width(self):
+        return self.desc.width
+
+    @property
+    def height(self):
+        return self.desc.height
+
+
+def host_trace_rays(desc: SceneDesc, rays: np.ndarray, any_hit: bool = False):
+    """Traces rays (N x 8: origin, tmin, dir, tmax) on the CPU through the same
+    compressed wide BVH the GPU uses.  Returns (t, prim, b1, b2, node_visits,
+    tri_tests)."""
+    rays = np.ascontiguousarray(rays, dtype=np.float32)
+    n = rays.shape[0]
+    hits = np.zeros((n, 4), dtype=np.uint32)
+    nv, tv = C.c_uint64(), C.c_uint64()
+    _check(lib().pt_host_trace_rays(C.byref(desc), rays.ctypes.data, n, int(any_hit), hits.ctypes.data,
+                                    C.byref(nv), C.byref(tv)), "pt_host_trace_rays")
+    return (hits[:, 0].view(np.float32), hits[:, 1].copy(), hits[:, 2].view(np.float32),
+            hits[:, 3].view(np.float32), nv.value, tv.value)
+
+
+def write_exr(path: str, rgba: np.ndarray):
+    rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+    h, w = rgba.shape[:2]
+    _check(lib().pt_write_exr(path.encode(), rgba.ctypes.data, w, h), "pt_write_exr")
+
+
+class PathTracer:
+    """Headless stand-in for the reference path-tracing pass on one GPU."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(lib().pt_create(device, C.byref(h)), "pt_create")
+        self._ctx = h
+        self.width = 0
+        self.height = 0
+
+    def close(self):
+        if self._ctx:
+            lib().pt_destroy(self._ctx)
+            self._ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_scene(self, scene) -> None:
+        desc = scene.desc if isinstance(scene, HostScene) else scene
+        _check(lib().pt_set_scene(self._ctx, C.byref(desc)), "pt_set_scene", self._ctx)
+        self.width, self.height = desc.width, desc.height
+
+    def build_bvh(self, num_bins: int = 0, threads: int = 0) -> None:
+        _check(lib().pt_build_bvh(self._ctx, C.byref(BvhOpts(num_bins, threads))), "pt_build_bvh", self._ctx)
+
+    def set_camera(self, sample_to_camera, camera_to_world) -> None:
+        a = (C.c_float * 16)(*[float(x) for x in np.asarray(sample_to_camera).ravel()])
+        b = (C.c_float * 16)(*[float(x) for x in np.asarray(camera_to_world).ravel()])
+        _check(lib().pt_set_camera(self._ctx, a, b), "pt_set_camera", self._ctx)
+
+    def reset(self) -> None:
+        _check(lib().pt_reset_accumulation(self._ctx), "pt_reset_accumulation", self._ctx)
+
+    def render(self, spp: int = 1, max_depth: int = 0, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE,
+               accumulate: bool = True, aov: bool = True, fp64: bool = False, timing: bool = False,
+               spp_per_launch: int = 0) -> None:
+        flags = (RENDER_AOV if aov else 0) | (RENDER_FP64_ACCUM if fp64 else 0) | (RENDER_TIMING if timing else 0)
+        p = RenderParams(spp, max_depth, rr_depth, mis_mode, int(accumulate), flags, spp_per_launch, 0)
+        _check(lib().pt_render(self._ctx, C.byref(p)), "pt_render", self._ctx)
+
+    def read(self, which: int) -> np.ndarray:
+        w, h = self.width, self.height
+        shapes = {BUF_FRAME: ((h, w, 4), np.float32), BUF_ACCUM: ((h, w, 4), np.float32),
+                  BUF_ALBEDO: ((h, w, 3), np.float32), BUF_NORMAL: ((h, w, 3), np.float32),
+                  BUF_TEST: ((h, w), np.float32), BUF_PRIM_ID: ((h, w), np.uint32),
+                  BUF_ACCUM64: ((h, w, 3), np.float64)}
+        shape, dt = shapes[which]
+        out = np.empty(shape, dtype=dt)
+        _check(lib().pt_read_buffer(self._ctx, which, out.ctypes.data, out.nbytes), "pt_read_buffer", self._ctx)
+        return out
+
+    def stats(self) -> dict:
+        s = Stats()
+        _check(lib().pt_get_stats(self._ctx, C.byref(s)), "pt_get_stats", self._ctx)
+        return s.as_dict()
+
+    def set_counting(self, enable: bool) -> None:
+        _check(lib().pt_set_counting(self._ctx, int(enable)), "pt_set_counting", self._ctx)
+
+    def trace_rays(self, rays: np.ndarray, any_hit: bool = False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), dtype=np.uint32)
+        _check(lib().pt_trace_rays(self._ctx, rays.ctypes.data, n, int(any_hit), hits.ctypes.data),
+               "pt_trace_rays", self._ctx)
+        return hits[:, 0].view(np.float32), hits[:, 1].copy(), hits[:, 2].view(np.float32), hits[:, 3].view(np.float32)

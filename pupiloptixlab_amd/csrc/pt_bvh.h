@@ -1,0 +1,225 @@
+// Compressed 8-wide BVH (80-byte nodes) and its closest-hit / any-hit
+// traversal, shared by the host (tests, statistics) and the gfx950 kernels.
+//
+// Replaces the OptiX acceleration structures and optixTrace of the reference
+// (world/gas_manager.cpp:69-245, world/ias_manager.cpp:29-114,
+// example/path_tracer/main.cu:80-85,161-166, render/emitter.h:91-100).
+// The scene is flattened to world space, so there is a single level.
+//
+// Result contract (independent of tree shape): the closest hit is the
+// lexicographic minimum of (t, primitive id) over all triangles whose
+// watertight intersection has t in [tmin, tmax]; a shadow query reports
+// whether any such triangle exists.  Node tests are widened by a relative
+// margin so rounding in the slab test never culls a triangle the exact test
+// would accept.
+#pragma once
+
+#include "pt_math.h"
+
+namespace pupil {
+
+struct WideNode {           // 80 bytes, read as five 16-byte loads
+    float px, py, pz;       // quantization origin
+    uint8_t ex, ey, ez;     // per-axis scale exponent (biased by 127)
+    uint8_t imask;          // bit k set: slot k holds an internal child
+    uint32_t child_base;    // index of the first internal child node
+    uint32_t tri_base;      // index of the first leaf triangle
+    uint8_t meta[8];        // per slot: 0 = empty; leaf: (unary count << 5) | tri offset;
+                            // internal: (1 << 5) | (24 + slot)
+    uint8_t qlo_x[8], qlo_y[8], qlo_z[8];
+    uint8_t qhi_x[8], qhi_y[8], qhi_z[8];
+};
+static_assert(sizeof(WideNode) == 80, "wide node must be 80 bytes");
+
+// Leaf triangles in traversal order: three float4 (v0.xyz + primitive id bits,
+// v1.xyz, v2.xyz).
+struct LeafTri {
+    f4 v0;
+    f4 v1;
+    f4 v2;
+};
+
+constexpr float kNodeMargin = 1e-4f;     // relative widening of node intervals
+constexpr uint32_t kMaxBvhDepth = 64;    // builder guarantees depth <= this
+
+struct Hit {
+    float t;
+    uint32_t prim;   // 0xFFFFFFFF = miss
+    float b1, b2;    // barycentric weights of vertex 1 and vertex 2
+};
+
+// Per-ray constants of the watertight test (Woop, Benthin, Wald 2013).
+struct RayPre {
+    int kx, ky, kz;
+    float sx, sy, sz;
+};
+PT_HD float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+PT_HD RayPre ray_precompute(f3 d) {
+    RayPre r;
+    float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    int kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz + 1;
+    if (kx == 3) kx = 0;
+    int ky = kx + 1;
+    if (ky == 3) ky = 0;
+    float dkz = comp(d, kz);
+    if (dkz < 0.f) {
+        int tmp = kx;
+        kx = ky;
+        ky = tmp;
+    }
+    r.kx = kx;
+    r.ky = ky;
+    r.kz = kz;
+    r.sx = comp(d, kx) / dkz;
+    r.sy = comp(d, ky) / dkz;
+    r.sz = 1.f / dkz;
+    return r;
+}
+
+// Watertight ray/triangle test.  Returns true with t = T/det and the
+// barycentric weights of v1 and v2 when the ray crosses the triangle (either
+// facing); the caller applies the t range.
+PT_HD bool intersect_triangle(f3 o, const RayPre &rp, f3 v0, f3 v1, f3 v2, float &t, float &b1, float &b2) {
+    f3 A = v0 - o, B = v1 - o, C = v2 - o;
+    float Akz = comp(A, rp.kz), Bkz = comp(B, rp.kz), Ckz = comp(C, rp.kz);
+    float Ax = comp(A, rp.kx) - rp.sx * Akz;
+    float Ay = comp(A, rp.ky) - rp.sy * Akz;
+    float Bx = comp(B, rp.kx) - rp.sx * Bkz;
+    float By = comp(B, rp.ky) - rp.sy * Bkz;
+    float Cx = comp(C, rp.kx) - rp.sx * Ckz;
+    float Cy = comp(C, rp.ky) - rp.sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.f || V == 0.f || W == 0.f) {
+        double CxBy = (double)Cx * (double)By, CyBx = (double)Cy * (double)Bx;
+        U = (float)(CxBy - CyBx);
+        double AxCy = (double)Ax * (double)Cy, AyCx = (double)Ay * (double)Cx;
+        V = (float)(AxCy - AyCx);
+        double BxAy = (double)Bx * (double)Ay, ByAx = (double)By * (double)Ax;
+        W = (float)(BxAy - ByAx);
+    }
+    if ((U < 0.f || V < 0.f || W < 0.f) && (U > 0.f || V > 0.f || W > 0.f)) return false;
+    float det = U + V + W;
+    if (det == 0.f) return false;
+    float Az = rp.sz * Akz, Bz = rp.sz * Bkz, Cz = rp.sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    t = T / det;
+    b1 = V / det;
+    b2 = W / det;
+    return true;
+}
+
+PT_HD float exp2_from_biased(uint8_t e) { return u2f(((uint32_t)e) << 23); }
+
+struct TraverseStats {
+    uint32_t nodes;
+    uint32_t tris;
+};
+
+// Traversal over the compressed wide BVH.  `Stack` supplies push/pop/empty of
+// 64-bit node groups (x = child_base, y = hit bits << 24 | imask).  Groups
+// hold the not-yet-visited internal children of a node; internal hit bits are
+// stored at position (slot ^ octant) so popping the lowest bit visits slots
+// near-to-far for the ray's direction octant.
+template <bool kAnyHit, typename Stack, typename NodeLoad, typename TriLoad>
+PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float tmin, float tmax,
+                        Stack &stack, Hit &hit, TraverseStats *stats) {
+    const RayPre rp = ray_precompute(d);
+    const float idx = 1.f / (fabsf(d.x) > 1e-12f ? d.x : copysignf(1e-12f, d.x));
+    const float idy = 1.f / (fabsf(d.y) > 1e-12f ? d.y : copysignf(1e-12f, d.y));
+    const float idz = 1.f / (fabsf(d.z) > 1e-12f ? d.z : copysignf(1e-12f, d.z));
+    const uint32_t oct = (d.x < 0.f ? 1u : 0u) | (d.y < 0.f ? 2u : 0u) | (d.z < 0.f ? 4u : 0u);
+
+    hit.t = tmax;
+    hit.prim = 0xFFFFFFFFu;
+    hit.b1 = 0.f;
+    hit.b2 = 0.f;
+    float tbest = tmax;
+
+    // Root group: node 0 sits in slot 0 of a virtual parent.
+    uint32_t g_base = 0, g_bits = (1u << (24 + (0 ^ oct))) | 1u;
+    uint32_t t_base = 0, t_bits = 0;
+    while (true) {
+        if (g_bits & 0xFF000000u) {
+            uint32_t bitpos = __builtin_ctz(g_bits >> 24);
+            g_bits &= ~(1u << (24 + bitpos));
+            uint32_t slot = bitpos ^ oct;
+            uint32_t imask = g_bits & 0xFFu;
+            uint32_t node_index = g_base + __builtin_popcount(imask & ((1u << slot) - 1u));
+            if (g_bits & 0xFF000000u) stack.push(g_base, g_bits);
+
+            WideNode n;
+            load_node(node_index, n);
+            if (stats) stats->nodes++;
+            const float sx = exp2_from_biased(n.ex), sy = exp2_from_biased(n.ey), sz = exp2_from_biased(n.ez);
+            uint32_t new_bits = 0, tri_bits = 0;
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t meta = n.meta[k];
+                if (meta == 0) continue;
+                float lox = n.px + (float)n.qlo_x[k] * sx;
+                float loy = n.py + (float)n.qlo_y[k] * sy;
+                float loz = n.pz + (float)n.qlo_z[k] * sz;
+                float hix = n.px + (float)n.qhi_x[k] * sx;
+                float hiy = n.py + (float)n.qhi_y[k] * sy;
+                float hiz = n.pz + (float)n.qhi_z[k] * sz;
+                float tx0 = (lox - o.x) * idx, tx1 = (hix - o.x) * idx;
+                float ty0 = (loy - o.y) * idy, ty1 = (hiy - o.y) * idy;
+                float tz0 = (loz - o.z) * idz, tz1 = (hiz - o.z) * idz;
+                float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+                float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tbest));
+                if (tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin)) {
+                    if (n.imask & (1u << k)) {
+                        new_bits |= 1u << (24 + (k ^ oct));
+                    } else {
+                        uint32_t cnt_unary = meta >> 5;
+                        uint32_t off = meta & 31u;
+                        tri_bits |= cnt_unary << off;
+                    }
+                }
+            }
+            g_base = n.child_base;
+            g_bits = new_bits | n.imask;
+            t_base = n.tri_base;
+            t_bits = tri_bits;
+        } else {
+            g_bits = 0;
+        }
+
+        while (t_bits) {
+            uint32_t i = __builtin_ctz(t_bits);
+            t_bits &= t_bits - 1u;
+            LeafTri tri;
+            load_tri(t_base + i, tri);
+            if (stats) stats->tris++;
+            float t, b1, b2;
+            if (intersect_triangle(o, rp, f3{tri.v0.x, tri.v0.y, tri.v0.z}, f3{tri.v1.x, tri.v1.y, tri.v1.z},
+                                   f3{tri.v2.x, tri.v2.y, tri.v2.z}, t, b1, b2)) {
+                uint32_t prim = f2u(tri.v0.w);
+                if (t >= tmin && (t < tbest || (t == tbest && prim < hit.prim))) {
+                    if (kAnyHit) {
+                        hit.t = t;
+                        hit.prim = prim;
+                        hit.b1 = b1;
+                        hit.b2 = b2;
+                        return true;
+                    }
+                    tbest = t;
+                    hit.t = t;
+                    hit.prim = prim;
+                    hit.b1 = b1;
+                    hit.b2 = b2;
+                }
+            }
+        }
+
+        if ((g_bits & 0xFF000000u) == 0) {
+            if (stack.empty()) break;
+            stack.pop(g_base, g_bits);
+        }
+    }
+    return hit.prim != 0xFFFFFFFFu;
+}
+
+}  // namespace pupil

@@ -1,0 +1,1072 @@
+// gfx950 kernels of the wavefront path tracer and the C-ABI context that
+// drives them.
+//
+// One frame (= one sample per pixel, as one optixLaunch of the reference,
+// example/path_tracer/pt_pass.cpp:39-57) runs as:
+//   raygen -> { trace -> shade -> shadow } x max_depth -> accumulate
+// over structure-of-arrays path queues.  `spp_per_launch` frames run in the
+// same queues concurrently; accumulation then folds them in frame order, so
+// the result equals frame-by-frame rendering bit for bit.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/pupil_hip.h"
+#include "pt_bvh.h"
+#include "pt_bvh_build.h"
+#include "pt_shade.h"
+
+using namespace pupil;
+
+namespace {
+
+constexpr int kTraceBlock = 128;
+constexpr int kLdsStack = 16;  // stack entries per lane kept in LDS
+constexpr uint32_t kSpillEntries = kMaxBvhDepth + 2 - kLdsStack;
+
+// --- queue layout -------------------------------------------------------------
+struct PathQueue {
+    float4 *ray0;  // origin.xyz, path id bits
+    float4 *ray1;  // direction.xyz, previous BSDF pdf
+    float4 *tl;    // throughput.xyz, radiance.x
+    float4 *lr;    // radiance.y, radiance.z, rng bits, flags bits
+    float4 *hit;   // t, primitive bits, b1, b2
+};
+struct ShadowQueue {
+    float4 *s0;  // origin.xyz, tmax
+    float4 *s1;  // direction.xyz, target bits
+    float4 *s2;  // contribution.xyz
+};
+
+struct DeviceBvh {
+    const WideNode *nodes;
+    const LeafTri *tris;
+    uint2 *spill;
+};
+
+__device__ __forceinline__ void load_node_dev(const WideNode *nodes, uint32_t i, WideNode &n) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(nodes + i);
+    uint4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
+    uint4 *q = reinterpret_cast<uint4 *>(&n);
+    q[0] = a;
+    q[1] = b;
+    q[2] = c;
+    q[3] = d;
+    q[4] = e;
+}
+
+struct LdsStack {
+    uint2 *lds;    // this lane's column: lds[k * stride]
+    uint2 *spill;  // this lane's global overflow area
+    uint32_t stride;
+    uint32_t sp;
+    __device__ void push(uint32_t a, uint32_t b) {
+        if (sp < (uint32_t)kLdsStack) lds[sp * stride] = make_uint2(a, b);
+        else spill[sp - kLdsStack] = make_uint2(a, b);
+        ++sp;
+    }
+    __device__ void pop(uint32_t &a, uint32_t &b) {
+        --sp;
+        uint2 v = sp < (uint32_t)kLdsStack ? lds[sp * stride] : spill[sp - kLdsStack];
+        a = v.x;
+        b = v.y;
+    }
+    __device__ bool empty() const { return sp == 0; }
+};
+
+template <bool kAnyHit, bool kCount>
+__device__ __forceinline__ bool trace_one(const DeviceBvh &bvh, uint2 *lds_col, uint32_t gtid, f3 o, f3 d,
+                                          float tmin, float tmax, Hit &h, TraverseStats &st) {
+    LdsStack stack{lds_col, bvh.spill + (size_t)gtid * kSpillEntries, (uint32_t)kTraceBlock, 0u};
+    auto ln = [&](uint32_t i, WideNode &n) { load_node_dev(bvh.nodes, i, n); };
+    auto lt = [&](uint32_t i, LeafTri &t) {
+        const float4 *p = reinterpret_cast<const float4 *>(bvh.tris + i);
+        float4 a = p[0], b = p[1], c = p[2];
+        t.v0 = f4{a.x, a.y, a.z, a.w};
+        t.v1 = f4{b.x, b.y, b.z, b.w};
+        t.v2 = f4{c.x, c.y, c.z, c.w};
+    };
+    return traverse_bvh<kAnyHit>(ln, lt, o, d, tmin, tmax, stack, h, kCount ? &st : nullptr);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+// Wave-level stream compaction: one atomic per wave reserves slots for the
+// lanes whose predicate holds; each lane gets base + its rank in the ballot.
+__device__ __forceinline__ uint32_t wave_alloc(bool pred, uint32_t *counter) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return 0u;
+    const uint32_t lane = lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned long long lt = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    return base + (uint32_t)__popcll(lt);
+}
+
+__device__ __forceinline__ f3 f3of(float4 v) { return mk3(v.x, v.y, v.z); }
+
+// --- raygen ---------------------------------------------------------------------
+// main.cu:39-78 for every (frame, pixel) of the launch.
+__global__ void k_raygen(PathQueue q, CameraRec cam, uint32_t width, uint32_t height, uint32_t frames,
+                         uint32_t frame_seed0, uint32_t *counters, uint32_t num_counters) {
+    const uint32_t npix = width * height;
+    const uint32_t total = npix * frames;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) {
+        counters[0] = total;
+        for (uint32_t i = 1; i < num_counters; ++i) counters[i] = 0u;
+    }
+    for (uint32_t i = tid; i < total; i += gridDim.x * blockDim.x) {
+        const uint32_t f = i / npix;
+        const uint32_t pixel = i - f * npix;
+        const uint32_t x = pixel % width, y = pixel / width;
+        Rng rng = rng_init(pixel, frame_seed0 + f);
+        const float jx = rng_next(rng);
+        const float jy = rng_next(rng);
+        f3 o, d;
+        camera_ray(cam, x, y, width, height, jx, jy, o, d);
+        q.ray0[i] = make_float4(o.x, o.y, o.z, u2f(i));
+        q.ray1[i] = make_float4(d.x, d.y, d.z, 0.f);
+        q.tl[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+        q.lr[i] = make_float4(0.f, 0.f, u2f(rng.seed), u2f(kPathPrimary));
+    }
+}
+
+// --- closest-hit traversal ------------------------------------------------------
+template <bool kCount>
+__global__ __launch_bounds__(kTraceBlock) void k_trace(PathQueue q, DeviceBvh bvh, const uint32_t *count_ptr,
+                                                       unsigned long long *stat) {
+    __shared__ uint2 lds[kLdsStack * kTraceBlock];
+    const uint32_t count = *count_ptr;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraverseStats st{0, 0};
+    for (uint32_t i = gtid; i < count; i += gridDim.x * blockDim.x) {
+        const float4 r0 = q.ray0[i], r1 = q.ray1[i];
+        Hit h;
+        trace_one<false, kCount>(bvh, lds + threadIdx.x, gtid, f3of(r0), f3of(r1), kExtTmin, kMaxDistance, h, st);
+        q.hit[i] = make_float4(h.t, u2f(h.prim), h.b1, h.b2);
+    }
+    if (kCount) {
+        atomicAdd(stat + 0, (unsigned long long)st.nodes);
+        atomicAdd(stat + 1, (unsigned long long)st.tris);
+    }
+}
+
+// --- shadow (any-hit) traversal ----------------------------------------------------
+template <bool kCount>
+__global__ __launch_bounds__(kTraceBlock) void k_shadow(ShadowQueue sq, DeviceBvh bvh, const uint32_t *count_ptr,
+                                                        PathQueue next, float4 *out_l, unsigned long long *stat) {
+    __shared__ uint2 lds[kLdsStack * kTraceBlock];
+    const uint32_t count = *count_ptr;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraverseStats st{0, 0};
+    for (uint32_t i = gtid; i < count; i += gridDim.x * blockDim.x) {
+        const float4 s0 = sq.s0[i], s1 = sq.s1[i];
+        Hit h;
+        bool occluded =
+            trace_one<true, kCount>(bvh, lds + threadIdx.x, gtid, f3of(s0), f3of(s1), kShadowEps, s0.w, h, st);
+        if (!occluded) {
+            const float4 c = sq.s2[i];
+            const uint32_t target = f2u(s1.w);
+            if (target & 0x80000000u) {
+                const uint32_t p = target & 0x7FFFFFFFu;
+                float4 l = out_l[p];
+                l.x = l.x + c.x;
+                l.y = l.y + c.y;
+                l.z = l.z + c.z;
+                out_l[p] = l;
+            } else {
+                float4 tl = next.tl[target], lr = next.lr[target];
+                tl.w = tl.w + c.x;
+                lr.x = lr.x + c.y;
+                lr.y = lr.y + c.z;
+                next.tl[target] = tl;
+                next.lr[target] = lr;
+            }
+        }
+    }
+    if (kCount) {
+        atomicAdd(stat + 0, (unsigned long long)st.nodes);
+        atomicAdd(stat + 1, (unsigned long long)st.tris);
+    }
+}
+
+// --- shading -----------------------------------------------------------------------
+struct AovBuffers {
+    float *albedo;     // 3 per pixel
+    float *normal;     // 3 per pixel
+    float *test;       // 1 per pixel
+    uint32_t *prim;    // 1 per pixel
+    uint32_t last_frame;  // AOVs are kept from this frame of the launch
+    uint32_t npix;
+};
+
+__global__ __launch_bounds__(256) void k_shade(SceneView scene, ShadeParams prm, PathQueue in,
+                                               const uint32_t *count_in, PathQueue out, uint32_t *count_out,
+                                               ShadowQueue sq, uint32_t *count_shadow, float4 *out_l,
+                                               AovBuffers aov) {
+    const uint32_t count = *count_in;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < count;
+        PathState ps;
+        ShadeOut so;
+        so.cont = so.done = so.shadow = so.wrote_aov = false;
+        uint32_t path = 0, prim = kMissPrim;
+        if (valid) {
+            const float4 r0 = in.ray0[i], r1 = in.ray1[i], tl = in.tl[i], lr = in.lr[i], h = in.hit[i];
+            ps.o = f3of(r0);
+            path = f2u(r0.w);
+            ps.d = f3of(r1);
+            ps.prev_pdf = r1.w;
+            ps.T = f3of(tl);
+            ps.L = mk3(tl.w, lr.x, lr.y);
+            ps.rng.seed = f2u(lr.z);
+            ps.flags = f2u(lr.w);
+            prim = f2u(h.y);
+            shade_path(scene, prm, ps, prim, h.z, h.w, so);
+        }
+        const uint32_t j = wave_alloc(valid && so.cont, count_out);
+        const uint32_t k = wave_alloc(valid && so.shadow, count_shadow);
+        if (!valid) continue;
+        if (so.wrote_aov && aov.albedo) {
+            const uint32_t f = path / aov.npix;
+            if (f == aov.last_frame) {
+                const uint32_t px = path - f * aov.npix;
+                aov.albedo[3 * px + 0] = so.albedo.x;
+                aov.albedo[3 * px + 1] = so.albedo.y;
+                aov.albedo[3 * px + 2] = so.albedo.z;
+                aov.normal[3 * px + 0] = so.normal.x;
+                aov.normal[3 * px + 1] = so.normal.y;
+                aov.normal[3 * px + 2] = so.normal.z;
+                aov.test[px] = so.test;
+                aov.prim[px] = prim;
+            }
+        }
+        if (so.cont) {
+            out.ray0[j] = make_float4(ps.o.x, ps.o.y, ps.o.z, u2f(path));
+            out.ray1[j] = make_float4(ps.d.x, ps.d.y, ps.d.z, ps.prev_pdf);
+            out.tl[j] = make_float4(ps.T.x, ps.T.y, ps.T.z, ps.L.x);
+            out.lr[j] = make_float4(ps.L.y, ps.L.z, u2f(ps.rng.seed), u2f(ps.flags));
+        } else {
+            out_l[path] = make_float4(ps.L.x, ps.L.y, ps.L.z, 1.f);
+        }
+        if (so.shadow) {
+            const uint32_t target = so.cont ? j : (0x80000000u | path);
+            sq.s0[k] = make_float4(so.so.x, so.so.y, so.so.z, so.stmax);
+            sq.s1[k] = make_float4(so.sd.x, so.sd.y, so.sd.z, u2f(target));
+            sq.s2[k] = make_float4(so.contrib.x, so.contrib.y, so.contrib.z, 0.f);
+        }
+    }
+}
+
+// --- accumulation ---------------------------------------------------------------
+// main.cu:190-196: running mean acc = lerp(acc, L, 1 / (n + 1)) per frame.
+__global__ void k_accumulate(const float4 *out_l, float4 *accum, float4 *frame, double *accum64, uint32_t npix,
+                             uint32_t frames, uint32_t sample_cnt0, uint32_t accumulate) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < npix; p += gridDim.x * blockDim.x) {
+        float4 acc = accum[p];
+        f3 a = mk3(acc.x, acc.y, acc.z);
+        uint32_t n = sample_cnt0;
+        double sx = 0, sy = 0, sz = 0;
+        if (accum64) {
+            sx = accum64[3 * (size_t)p];
+            sy = accum64[3 * (size_t)p + 1];
+            sz = accum64[3 * (size_t)p + 2];
+        }
+        for (uint32_t f = 0; f < frames; ++f) {
+            const float4 lv = out_l[(size_t)f * npix + p];
+            f3 L = mk3(lv.x, lv.y, lv.z);
+            if (accumulate && n > 0) {
+                const float t = 1.f / ((float)n + 1.f);
+                L = lerp(a, L, t);
+            }
+            a = L;
+            if (accumulate) n += 1;
+            sx += (double)lv.x;
+            sy += (double)lv.y;
+            sz += (double)lv.z;
+        }
+        const float4 r = make_float4(a.x, a.y, a.z, 1.f);
+        accum[p] = r;
+        frame[p] = r;
+        if (accum64) {
+            accum64[3 * (size_t)p] = sx;
+            accum64[3 * (size_t)p + 1] = sy;
+            accum64[3 * (size_t)p + 2] = sz;
+        }
+    }
+}
+
+// Closest-hit / any-hit over caller rays (pt_trace_rays).
+template <bool kAnyHit>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_rays(const float *rays, uint32_t count, DeviceBvh bvh,
+                                                            uint4 *hits) {
+    __shared__ uint2 lds[kLdsStack * kTraceBlock];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraverseStats st{0, 0};
+    for (uint32_t i = gtid; i < count; i += gridDim.x * blockDim.x) {
+        const float *r = rays + 8 * (size_t)i;
+        Hit h;
+        trace_one<kAnyHit, false>(bvh, lds + threadIdx.x, gtid, mk3(r[0], r[1], r[2]), mk3(r[4], r[5], r[6]), r[3],
+                                  r[7], h, st);
+        hits[i] = make_uint4(f2u(h.t), h.prim, f2u(h.b1), f2u(h.b2));
+    }
+}
+
+}  // namespace
+
+// =====================================================================================
+// Context
+// =====================================================================================
+struct pt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string error;
+    int num_cu = 256;
+
+    // scene (host copies kept for the BVH build)
+    bool has_scene = false;
+    bool has_bvh = false;
+    std::vector<float> h_pos4, h_nrm4, h_uv2;
+    std::vector<uint32_t> h_tri4;
+    std::vector<ShapeRec> h_shapes;
+    std::vector<MaterialRec> h_materials;
+    std::vector<EmitterRec> h_emitters;
+    std::vector<float> h_cdf;
+    EnvRec env{};
+    CameraRec camera{};
+    uint32_t width = 0, height = 0, scene_max_depth = 1;
+    uint32_t num_tris = 0;
+
+    // device scene
+    uint32_t *d_tri = nullptr;
+    float4 *d_pos = nullptr, *d_nrm = nullptr;
+    float2 *d_uv = nullptr;
+    ShapeRec *d_shapes = nullptr;
+    MaterialRec *d_materials = nullptr;
+    EmitterRec *d_emitters = nullptr;
+    float *d_cdf = nullptr;
+    WideNode *d_nodes = nullptr;
+    LeafTri *d_leaf = nullptr;
+    BvhBuildStats bvh_stats{};
+
+    // frame buffers
+    float4 *d_accum = nullptr, *d_frame = nullptr;
+    double *d_accum64 = nullptr;
+    float *d_albedo = nullptr, *d_normal = nullptr, *d_test = nullptr;
+    uint32_t *d_prim = nullptr;
+
+    // queues
+    uint32_t capacity = 0;  // paths
+    PathQueue q[2]{};
+    ShadowQueue sq{};
+    float4 *d_out_l = nullptr;
+    uint32_t *d_counters = nullptr;
+    uint32_t num_counters = 0;
+    uint2 *d_spill = nullptr;
+    uint32_t spill_threads = 0;
+    unsigned long long *d_stat = nullptr;
+    bool counting = false;
+
+    // progressive state (pt_pass.cpp:55-56)
+    uint32_t sample_cnt = 0;
+    uint32_t random_seed = 0;
+    bool accum64_valid = false;
+
+    pt_stats stats{};
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+#define PT_HIP(call)                                                                         \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) {                                                             \
+            ctx->error = std::string(#call) + ": " + hipGetErrorString(e_);                 \
+            return PT_ERR_HIP;                                                              \
+        }                                                                                   \
+    } while (0)
+
+template <typename T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+void free_scene(pt_ctx *ctx) {
+    dfree(ctx->d_tri);
+    dfree(ctx->d_pos);
+    dfree(ctx->d_nrm);
+    dfree(ctx->d_uv);
+    dfree(ctx->d_shapes);
+    dfree(ctx->d_materials);
+    dfree(ctx->d_emitters);
+    dfree(ctx->d_cdf);
+    dfree(ctx->d_nodes);
+    dfree(ctx->d_leaf);
+    dfree(ctx->d_accum);
+    dfree(ctx->d_frame);
+    dfree(ctx->d_accum64);
+    dfree(ctx->d_albedo);
+    dfree(ctx->d_normal);
+    dfree(ctx->d_test);
+    dfree(ctx->d_prim);
+    ctx->has_scene = false;
+    ctx->has_bvh = false;
+}
+
+void free_queues(pt_ctx *ctx) {
+    for (auto &q : ctx->q) {
+        dfree(q.ray0);
+        dfree(q.ray1);
+        dfree(q.tl);
+        dfree(q.lr);
+        dfree(q.hit);
+    }
+    dfree(ctx->sq.s0);
+    dfree(ctx->sq.s1);
+    dfree(ctx->sq.s2);
+    dfree(ctx->d_out_l);
+    dfree(ctx->d_counters);
+    ctx->capacity = 0;
+    ctx->num_counters = 0;
+}
+
+template <typename T>
+int upload(pt_ctx *ctx, T *&dst, const T *src, size_t n) {
+    dfree(dst);
+    if (n == 0) return PT_OK;
+    PT_HIP(hipMalloc((void **)&dst, n * sizeof(T)));
+    PT_HIP(hipMemcpy(dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return PT_OK;
+}
+
+TextureRec to_texture_rec(const pt_texture_desc &t) {
+    TextureRec r;
+    std::memset(&r, 0, sizeof(r));
+    r.type = t.type;
+    r.bitmap = t.bitmap;
+    r.c0 = f4{t.color0[0], t.color0[1], t.color0[2], 0.f};
+    r.c1 = f4{t.color1[0], t.color1[1], t.color1[2], 0.f};
+    r.r0 = f4{t.uv_row0[0], t.uv_row0[1], t.uv_row0[2], t.uv_row0[3]};
+    r.r1 = f4{t.uv_row1[0], t.uv_row1[1], t.uv_row1[2], t.uv_row1[3]};
+    return r;
+}
+
+// Average color of a texture (optix_material.cpp:9-36).
+f3 texture_average(const pt_texture_desc &t) {
+    if (t.type == PT_TEX_CHECKER)
+        return mk3(t.color0[0] + t.color1[0], t.color0[1] + t.color1[1], t.color0[2] + t.color1[2]) * 0.5f;
+    return mk3(t.color0[0], t.color0[1], t.color0[2]);
+}
+
+// Emitter selection weight of a radiance texture (world/emitter.cpp:73-101).
+float texture_weight(const pt_texture_desc &t) {
+    auto mx = [](float r, float g, float b) { return r > g ? (r > b ? r : b) : (g > b ? g : b); };
+    if (t.type == PT_TEX_CHECKER)
+        return (mx(t.color0[0], t.color0[1], t.color0[2]) + mx(t.color1[0], t.color1[1], t.color1[2])) * 0.5f;
+    return mx(t.color0[0], t.color0[1], t.color0[2]);
+}
+
+// optix_material.cpp:41-132 host packing.
+MaterialRec to_material_rec(const pt_material_desc &m) {
+    MaterialRec r;
+    std::memset(&r, 0, sizeof(r));
+    r.type = m.type;
+    r.flags = (m.twosided ? kMatTwosided : 0u) | (m.nonlinear ? kMatNonlinear : 0u);
+    for (int i = 0; i < 4; ++i) r.tex[i] = to_texture_rec(m.tex[i]);
+    r.eta = m.int_ior / m.ext_ior;
+    if (m.type == PT_MAT_PLASTIC || m.type == PT_MAT_ROUGH_PLASTIC) {
+        float diffuse_lum = luminance(texture_average(m.tex[0]));
+        float specular_lum = luminance(texture_average(m.tex[1]));
+        r.spec_weight = specular_lum / (specular_lum + diffuse_lum);
+        r.int_fdr = fresnel_diffuse_reflectance(1.f / r.eta);
+    }
+    return r;
+}
+
+int ensure_queues(pt_ctx *ctx, uint32_t paths, uint32_t max_depth) {
+    uint32_t need_counters = 2 * (max_depth + 2);
+    if (paths <= ctx->capacity && need_counters <= ctx->num_counters) return PT_OK;
+    free_queues(ctx);
+    const size_t n = paths;
+    for (auto &q : ctx->q) {
+        PT_HIP(hipMalloc((void **)&q.ray0, n * sizeof(float4)));
+        PT_HIP(hipMalloc((void **)&q.ray1, n * sizeof(float4)));
+        PT_HIP(hipMalloc((void **)&q.tl, n * sizeof(float4)));
+        PT_HIP(hipMalloc((void **)&q.lr, n * sizeof(float4)));
+        PT_HIP(hipMalloc((void **)&q.hit, n * sizeof(float4)));
+    }
+    PT_HIP(hipMalloc((void **)&ctx->sq.s0, n * sizeof(float4)));
+    PT_HIP(hipMalloc((void **)&ctx->sq.s1, n * sizeof(float4)));
+    PT_HIP(hipMalloc((void **)&ctx->sq.s2, n * sizeof(float4)));
+    PT_HIP(hipMalloc((void **)&ctx->d_out_l, n * sizeof(float4)));
+    PT_HIP(hipMalloc((void **)&ctx->d_counters, need_counters * sizeof(uint32_t)));
+    ctx->capacity = paths;
+    ctx->num_counters = need_counters;
+    return PT_OK;
+}
+
+uint32_t trace_grid(pt_ctx *ctx) { return (uint32_t)ctx->num_cu * 16u; }
+uint32_t shade_grid(pt_ctx *ctx) { return (uint32_t)ctx->num_cu * 8u; }
+
+int ensure_spill(pt_ctx *ctx) {
+    uint32_t threads = trace_grid(ctx) * kTraceBlock;
+    if (ctx->d_spill && ctx->spill_threads >= threads) return PT_OK;
+    dfree(ctx->d_spill);
+    PT_HIP(hipMalloc((void **)&ctx->d_spill, (size_t)threads * kSpillEntries * sizeof(uint2)));
+    ctx->spill_threads = threads;
+    return PT_OK;
+}
+
+DeviceBvh device_bvh(pt_ctx *ctx) { return DeviceBvh{ctx->d_nodes, ctx->d_leaf, ctx->d_spill}; }
+
+SceneView scene_view(pt_ctx *ctx) {
+    SceneView v;
+    v.tri = ctx->d_tri;
+    v.vpos = reinterpret_cast<const f4 *>(ctx->d_pos);
+    v.vnrm = reinterpret_cast<const f4 *>(ctx->d_nrm);
+    v.vuv = reinterpret_cast<const f2 *>(ctx->d_uv);
+    v.shapes = ctx->d_shapes;
+    v.materials = ctx->d_materials;
+    v.emitters = ctx->d_emitters;
+    v.emitter_cdf = ctx->d_cdf;
+    v.num_emitters = (uint32_t)ctx->h_emitters.size();
+    v.pad = 0;
+    v.env = ctx->env;
+    return v;
+}
+
+struct EventTimer {
+    pt_ctx *ctx;
+    bool on;
+    hipEvent_t a = nullptr, b = nullptr;
+    EventTimer(pt_ctx *c, bool enabled) : ctx(c), on(enabled) {
+        if (on) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, ctx->stream);
+        }
+    }
+    void stop(double &acc) {
+        if (!on) return;
+        (void)hipEventRecord(b, ctx->stream);
+        (void)hipEventSynchronize(b);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        acc += ms;
+    }
+    ~EventTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int pt_create(int hip_device, pt_ctx **out) {
+    if (!out) return PT_ERR_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0) {
+        g_create_error = std::string("no HIP device available: ") + hipGetErrorString(e);
+        return PT_ERR_HIP;
+    }
+    if (hip_device < 0 || hip_device >= n) {
+        g_create_error = "device index out of range";
+        return PT_ERR_INVALID_ARG;
+    }
+    pt_ctx *ctx = new (std::nothrow) pt_ctx();
+    if (!ctx) return PT_ERR_OOM;
+    ctx->device = hip_device;
+    if (hipSetDevice(hip_device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "failed to set device / create stream";
+        delete ctx;
+        return PT_ERR_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->num_cu = prop.multiProcessorCount;
+    if (hipMalloc((void **)&ctx->d_stat, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        g_create_error = "hipMalloc failed";
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return PT_ERR_HIP;
+    }
+    *out = ctx;
+    return PT_OK;
+}
+
+void pt_destroy(pt_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_scene(ctx);
+    free_queues(ctx);
+    dfree(ctx->d_spill);
+    dfree(ctx->d_stat);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *pt_last_error(const pt_ctx *ctx) { return ctx ? ctx->error.c_str() : g_create_error.c_str(); }
+
+int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    if (!s || !s->positions || !s->indices || !s->shapes || !s->materials || s->width == 0 || s->height == 0) {
+        ctx->error = "pt_set_scene: missing arrays or zero film size";
+        return PT_ERR_INVALID_ARG;
+    }
+    (void)hipSetDevice(ctx->device);
+    const uint32_t nv = s->num_vertices, nt = s->num_triangles, ns = s->num_shapes, nm = s->num_materials;
+    // Validate ranges first: nothing is uploaded for a malformed description.
+    uint32_t covered = 0;
+    for (uint32_t i = 0; i < ns; ++i) {
+        const pt_shape_desc &sh = s->shapes[i];
+        if (sh.first_tri != covered || sh.material >= nm) {
+            ctx->error = "pt_set_scene: shapes must cover the triangles in order and reference valid materials";
+            return PT_ERR_INVALID_ARG;
+        }
+        covered += sh.num_tris;
+    }
+    if (covered != nt) {
+        ctx->error = "pt_set_scene: shape triangle ranges do not cover all triangles";
+        return PT_ERR_INVALID_ARG;
+    }
+    for (uint32_t i = 0; i < 3 * nt; ++i)
+        if (s->indices[i] >= nv) {
+            ctx->error = "pt_set_scene: vertex index out of range";
+            return PT_ERR_INVALID_ARG;
+        }
+    for (uint32_t i = 0; i < nm; ++i)
+        if (s->materials[i].type < PT_MAT_DIFFUSE || s->materials[i].type > PT_MAT_ROUGH_PLASTIC) {
+            ctx->error = "pt_set_scene: unknown material type";
+            return PT_ERR_UNSUPPORTED;
+        }
+    (void)hipStreamSynchronize(ctx->stream);
+    free_scene(ctx);
+
+    ctx->h_pos4.assign((size_t)nv * 4, 0.f);
+    ctx->h_nrm4.assign(s->normals ? (size_t)nv * 4 : 0, 0.f);
+    ctx->h_uv2.assign(s->texcoords ? (size_t)nv * 2 : 0, 0.f);
+    for (uint32_t i = 0; i < nv; ++i) {
+        for (int k = 0; k < 3; ++k) ctx->h_pos4[4 * i + k] = s->positions[3 * i + k];
+        if (s->normals)
+            for (int k = 0; k < 3; ++k) ctx->h_nrm4[4 * i + k] = s->normals[3 * i + k];
+        if (s->texcoords)
+            for (int k = 0; k < 2; ++k) ctx->h_uv2[2 * i + k] = s->texcoords[2 * i + k];
+    }
+    ctx->h_tri4.assign((size_t)nt * 4, 0u);
+    ctx->h_shapes.resize(ns);
+    // Area emitters: one per triangle of each emitting shape, in shape order
+    // (world/emitter.cpp:169-222, pt_pass.cpp:219-235).
+    ctx->h_emitters.clear();
+    for (uint32_t si = 0; si < ns; ++si) {
+        const pt_shape_desc &sh = s->shapes[si];
+        ShapeRec r;
+        r.material = sh.material;
+        r.flags = sh.flags;
+        if (!s->normals) r.flags &= ~(uint32_t)PT_SHAPE_HAS_NORMALS;
+        if (!s->texcoords) r.flags &= ~(uint32_t)PT_SHAPE_HAS_TEXCOORDS;
+        r.first_tri = sh.first_tri;
+        r.emitter_offset = sh.is_emitter ? (int32_t)ctx->h_emitters.size() : -1;
+        ctx->h_shapes[si] = r;
+        for (uint32_t t = sh.first_tri; t < sh.first_tri + sh.num_tris; ++t) {
+            for (int k = 0; k < 3; ++k) ctx->h_tri4[4 * (size_t)t + k] = s->indices[3 * (size_t)t + k];
+            ctx->h_tri4[4 * (size_t)t + 3] = si;
+            if (!sh.is_emitter) continue;
+            EmitterRec e;
+            std::memset(&e, 0, sizeof(e));
+            e.type = kEmitterTriArea;
+            f3 p[3], n[3];
+            f2 uv[3];
+            for (int k = 0; k < 3; ++k) {
+                uint32_t vi = s->indices[3 * (size_t)t + k];
+                p[k] = mk3(s->positions[3 * vi], s->positions[3 * vi + 1], s->positions[3 * vi + 2]);
+                f3 nn;
+                if (s->normals && (r.flags & kShapeHasNormals)) {
+                    nn = mk3(s->normals[3 * vi], s->normals[3 * vi + 1], s->normals[3 * vi + 2]);
+                } else {
+                    nn = mk3(0.f);
+                }
+                uv[k] = s->texcoords ? mk2(s->texcoords[2 * vi], s->texcoords[2 * vi + 1]) : mk2(0.f, 0.f);
+                n[k] = nn;
+            }
+            for (int k = 0; k < 3; ++k) {
+                f3 nn = n[k];
+                if (!(r.flags & kShapeHasNormals)) {
+                    nn = cross(p[1] - p[0], p[2] - p[0]);
+                    if (r.flags & kShapeNegFaceNormal) nn = -nn;
+                }
+                // util::Transform::TransformNormal normalizes component-wise.
+                float len = sqrtf(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
+                n[k] = mk3(nn.x / len, nn.y / len, nn.z / len);
+            }
+            e.p0 = f4{p[0].x, p[0].y, p[0].z, 0.f};
+            e.p1 = f4{p[1].x, p[1].y, p[1].z, 0.f};
+            e.p2 = f4{p[2].x, p[2].y, p[2].z, 0.f};
+            e.n0 = f4{n[0].x, n[0].y, n[0].z, 0.f};
+            e.n1 = f4{n[1].x, n[1].y, n[1].z, 0.f};
+            e.n2 = f4{n[2].x, n[2].y, n[2].z, 0.f};
+            e.uv01 = f4{uv[0].x, uv[0].y, uv[1].x, uv[1].y};
+            e.uv2 = f4{uv[2].x, uv[2].y, 0.f, 0.f};
+            e.area = length(cross(p[1] - p[0], p[2] - p[0])) * 0.5f;
+            e.radiance = to_texture_rec(sh.radiance);
+            e.weight = texture_weight(sh.radiance) * e.area;
+            ctx->h_emitters.push_back(e);
+        }
+    }
+    // EmitterHelper::ComputeProbability (world/emitter.cpp:321-337).
+    {
+        float sum = 0.f;
+        for (auto &e : ctx->h_emitters) sum += e.weight;
+        const size_t na = ctx->h_emitters.size();
+        const size_t total = na + (s->env_type != PT_ENV_NONE ? 1 : 0);
+        for (auto &e : ctx->h_emitters) {
+            e.select_prob = e.weight / sum * (float)na;
+            e.select_prob = e.select_prob / (float)total;
+        }
+        ctx->h_cdf.resize(na);
+        float acc = 0.f;
+        for (size_t i = 0; i < na; ++i) {
+            acc = acc + ctx->h_emitters[i].select_prob;
+            ctx->h_cdf[i] = acc;
+        }
+        std::memset(&ctx->env, 0, sizeof(ctx->env));
+        if (s->env_type == PT_ENV_CONSTANT) {
+            ctx->env.type = kEmitterConstEnv;
+            ctx->env.select_prob = 1.f / (float)total;
+            ctx->env.color = f4{s->env_radiance[0], s->env_radiance[1], s->env_radiance[2], 0.f};
+        }
+    }
+    ctx->h_materials.resize(nm);
+    for (uint32_t i = 0; i < nm; ++i) ctx->h_materials[i] = to_material_rec(s->materials[i]);
+    for (int r = 0; r < 4; ++r) {
+        ctx->camera.s2c[r] = f4{s->camera.sample_to_camera[4 * r], s->camera.sample_to_camera[4 * r + 1],
+                                s->camera.sample_to_camera[4 * r + 2], s->camera.sample_to_camera[4 * r + 3]};
+        ctx->camera.c2w[r] = f4{s->camera.camera_to_world[4 * r], s->camera.camera_to_world[4 * r + 1],
+                                s->camera.camera_to_world[4 * r + 2], s->camera.camera_to_world[4 * r + 3]};
+    }
+    ctx->width = s->width;
+    ctx->height = s->height;
+    ctx->scene_max_depth = s->max_depth ? s->max_depth : 1;
+    ctx->num_tris = nt;
+
+    int rc;
+    if ((rc = upload(ctx, ctx->d_tri, ctx->h_tri4.data(), ctx->h_tri4.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, ctx->d_pos, reinterpret_cast<const float4 *>(ctx->h_pos4.data()), (size_t)nv)) != PT_OK) return rc;
+    if (s->normals &&
+        (rc = upload(ctx, ctx->d_nrm, reinterpret_cast<const float4 *>(ctx->h_nrm4.data()), (size_t)nv)) != PT_OK)
+        return rc;
+    if (s->texcoords &&
+        (rc = upload(ctx, ctx->d_uv, reinterpret_cast<const float2 *>(ctx->h_uv2.data()), (size_t)nv)) != PT_OK)
+        return rc;
+    if ((rc = upload(ctx, ctx->d_shapes, ctx->h_shapes.data(), ctx->h_shapes.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, ctx->d_materials, ctx->h_materials.data(), ctx->h_materials.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, ctx->d_emitters, ctx->h_emitters.data(), ctx->h_emitters.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, ctx->d_cdf, ctx->h_cdf.data(), ctx->h_cdf.size())) != PT_OK) return rc;
+
+    const size_t npix = (size_t)ctx->width * ctx->height;
+    PT_HIP(hipMalloc((void **)&ctx->d_accum, npix * sizeof(float4)));
+    PT_HIP(hipMalloc((void **)&ctx->d_frame, npix * sizeof(float4)));
+    PT_HIP(hipMalloc((void **)&ctx->d_accum64, npix * 3 * sizeof(double)));
+    PT_HIP(hipMalloc((void **)&ctx->d_albedo, npix * 3 * sizeof(float)));
+    PT_HIP(hipMalloc((void **)&ctx->d_normal, npix * 3 * sizeof(float)));
+    PT_HIP(hipMalloc((void **)&ctx->d_test, npix * sizeof(float)));
+    PT_HIP(hipMalloc((void **)&ctx->d_prim, npix * sizeof(uint32_t)));
+    PT_HIP(hipMemset(ctx->d_accum, 0, npix * sizeof(float4)));
+    PT_HIP(hipMemset(ctx->d_frame, 0, npix * sizeof(float4)));
+    PT_HIP(hipMemset(ctx->d_accum64, 0, npix * 3 * sizeof(double)));
+    PT_HIP(hipMemset(ctx->d_albedo, 0, npix * 3 * sizeof(float)));
+    PT_HIP(hipMemset(ctx->d_normal, 0, npix * 3 * sizeof(float)));
+    PT_HIP(hipMemset(ctx->d_test, 0, npix * sizeof(float)));
+    PT_HIP(hipMemset(ctx->d_prim, 0xFF, npix * sizeof(uint32_t)));
+    ctx->has_scene = true;
+    ctx->has_bvh = false;
+    ctx->sample_cnt = 0;
+    ctx->random_seed = 0;
+    ctx->stats = pt_stats{};
+    return PT_OK;
+}
+
+int pt_build_bvh(pt_ctx *ctx, const pt_bvh_opts *opts) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    if (!ctx->has_scene) {
+        ctx->error = "pt_build_bvh: no scene";
+        return PT_ERR_NO_SCENE;
+    }
+    (void)hipSetDevice(ctx->device);
+    BvhBuildOptions bo;
+    if (opts && opts->num_bins) bo.num_bins = (int)opts->num_bins;
+    if (opts && opts->threads) bo.threads = (int)opts->threads;
+    std::vector<WideNode> nodes;
+    std::vector<LeafTri> tris;
+    std::string err;
+    if (!build_wide_bvh(ctx->h_pos4.data(), (uint32_t)(ctx->h_pos4.size() / 4), ctx->h_tri4.data(), ctx->num_tris, bo,
+                        nodes, tris, ctx->bvh_stats, err)) {
+        ctx->error = "pt_build_bvh: " + err;
+        return PT_ERR_INVALID_ARG;
+    }
+    int rc;
+    if ((rc = upload(ctx, ctx->d_nodes, nodes.data(), nodes.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, ctx->d_leaf, tris.data(), tris.size())) != PT_OK) return rc;
+    if ((rc = ensure_spill(ctx)) != PT_OK) return rc;
+    ctx->has_bvh = true;
+    return PT_OK;
+}
+
+int pt_set_camera(pt_ctx *ctx, const float s2c[16], const float c2w[16]) {
+    if (!ctx || !s2c || !c2w) return PT_ERR_INVALID_ARG;
+    for (int r = 0; r < 4; ++r) {
+        ctx->camera.s2c[r] = f4{s2c[4 * r], s2c[4 * r + 1], s2c[4 * r + 2], s2c[4 * r + 3]};
+        ctx->camera.c2w[r] = f4{c2w[4 * r], c2w[4 * r + 1], c2w[4 * r + 2], c2w[4 * r + 3]};
+    }
+    return pt_reset_accumulation(ctx);
+}
+
+int pt_reset_accumulation(pt_ctx *ctx) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    ctx->sample_cnt = 0;
+    ctx->random_seed = 0;
+    ctx->stats.samples = 0;
+    if (ctx->d_accum64) {
+        (void)hipSetDevice(ctx->device);
+        PT_HIP(hipMemsetAsync(ctx->d_accum64, 0, (size_t)ctx->width * ctx->height * 3 * sizeof(double), ctx->stream));
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    return PT_OK;
+}
+
+int pt_set_counting(pt_ctx *ctx, int enable) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    ctx->counting = enable != 0;
+    return PT_OK;
+}
+
+int pt_render(pt_ctx *ctx, const pt_render_params *p) {
+    if (!ctx || !p) return PT_ERR_INVALID_ARG;
+    if (!ctx->has_scene) {
+        ctx->error = "pt_render: no scene";
+        return PT_ERR_NO_SCENE;
+    }
+    if (!ctx->has_bvh) {
+        ctx->error = "pt_render: BVH not built";
+        return PT_ERR_NO_BVH;
+    }
+    if (p->mis_mode > PT_MIS_CONSISTENT) {
+        ctx->error = "pt_render: unknown MIS mode";
+        return PT_ERR_INVALID_ARG;
+    }
+    (void)hipSetDevice(ctx->device);
+    const auto wall0 = std::chrono::steady_clock::now();
+    const uint32_t npix = ctx->width * ctx->height;
+    const uint32_t max_depth = p->max_depth ? p->max_depth : ctx->scene_max_depth;
+    if (max_depth > 250) {
+        ctx->error = "pt_render: max_depth above 250";
+        return PT_ERR_INVALID_ARG;
+    }
+    uint32_t per_launch = p->spp_per_launch;
+    if (per_launch == 0) per_launch = std::max(1u, std::min(8u, (4u * 1024u * 1024u) / std::max(1u, npix)));
+    per_launch = std::min(per_launch, std::max(1u, p->spp));
+    if ((uint64_t)per_launch * npix > 0x7FFFFFFFull) {
+        ctx->error = "pt_render: spp_per_launch * pixels exceeds the queue index range";
+        return PT_ERR_CAPACITY;
+    }
+    int rc = ensure_queues(ctx, per_launch * npix, max_depth);
+    if (rc != PT_OK) return rc;
+    if (ctx->counting) PT_HIP(hipMemsetAsync(ctx->d_stat, 0, 2 * sizeof(unsigned long long), ctx->stream));
+
+    const bool timing = (p->flags & PT_RENDER_TIMING) != 0;
+    const SceneView sv = scene_view(ctx);
+    const DeviceBvh bvh = device_bvh(ctx);
+    ShadeParams sp;
+    sp.max_depth = max_depth;
+    sp.rr_depth = p->rr_depth;
+    sp.mis_mode = p->mis_mode;
+    sp.aov = (p->flags & PT_RENDER_AOV) ? 1u : 0u;
+    const uint32_t tg = trace_grid(ctx), sg = shade_grid(ctx);
+
+    uint32_t done = 0;
+    while (done < p->spp) {
+        const uint32_t frames = std::min(per_launch, p->spp - done);
+        const uint32_t total = frames * npix;
+        uint32_t *ext = ctx->d_counters;                 // ext[b]
+        uint32_t *shd = ctx->d_counters + (max_depth + 2); // shd[b]
+        {
+            EventTimer t(ctx, timing);
+            const uint32_t grid = std::min<uint32_t>((total + 255) / 256, (uint32_t)ctx->num_cu * 32u);
+            hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, ctx->stream, ctx->q[0], ctx->camera, ctx->width,
+                               ctx->height, frames, ctx->random_seed, ctx->d_counters, ctx->num_counters);
+            t.stop(ctx->stats.ms_raygen);
+        }
+        AovBuffers aov{nullptr, nullptr, nullptr, nullptr, frames - 1, npix};
+        if (sp.aov) {
+            aov.albedo = ctx->d_albedo;
+            aov.normal = ctx->d_normal;
+            aov.test = ctx->d_test;
+            aov.prim = ctx->d_prim;
+        }
+        int cur = 0;
+        for (uint32_t b = 0; b < max_depth; ++b) {
+            const int nxt = cur ^ 1;
+            {
+                EventTimer t(ctx, timing);
+                if (ctx->counting)
+                    hipLaunchKernelGGL(k_trace<true>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->q[cur], bvh,
+                                       ext + b, ctx->d_stat);
+                else
+                    hipLaunchKernelGGL(k_trace<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->q[cur], bvh,
+                                       ext + b, ctx->d_stat);
+                t.stop(ctx->stats.ms_trace);
+                ctx->stats.launches_trace++;
+            }
+            {
+                EventTimer t(ctx, timing);
+                hipLaunchKernelGGL(k_shade, dim3(sg), dim3(256), 0, ctx->stream, sv, sp, ctx->q[cur], ext + b,
+                                   ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov);
+                t.stop(ctx->stats.ms_shade);
+                ctx->stats.launches_shade++;
+            }
+            {
+                EventTimer t(ctx, timing);
+                if (ctx->counting)
+                    hipLaunchKernelGGL(k_shadow<true>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->sq, bvh,
+                                       shd + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat);
+                else
+                    hipLaunchKernelGGL(k_shadow<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->sq, bvh,
+                                       shd + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat);
+                t.stop(ctx->stats.ms_shadow);
+                ctx->stats.launches_shadow++;
+            }
+            cur = nxt;
+        }
+        {
+            EventTimer t(ctx, timing);
+            const uint32_t grid = std::min<uint32_t>((npix + 255) / 256, (uint32_t)ctx->num_cu * 16u);
+            hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(256), 0, ctx->stream, ctx->d_out_l, ctx->d_accum,
+                               ctx->d_frame, (p->flags & PT_RENDER_FP64_ACCUM) ? ctx->d_accum64 : nullptr, npix, frames,
+                               ctx->sample_cnt, p->accumulate);
+            t.stop(ctx->stats.ms_accumulate);
+        }
+        PT_HIP(hipGetLastError());
+        // Ray counts of this launch (read back once per launch).
+        std::vector<uint32_t> counts(ctx->num_counters);
+        PT_HIP(hipMemcpyAsync(counts.data(), ctx->d_counters, counts.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+        for (uint32_t b = 0; b < max_depth; ++b) {
+            ctx->stats.rays_extension += counts[b];
+            ctx->stats.rays_shadow += counts[max_depth + 2 + b];
+        }
+        if (p->accumulate) ctx->sample_cnt += frames;
+        ctx->random_seed += frames;
+        ctx->stats.samples += (uint64_t)frames * npix;
+        done += frames;
+    }
+    if (ctx->counting) {
+        unsigned long long st[2];
+        PT_HIP(hipMemcpy(st, ctx->d_stat, sizeof(st), hipMemcpyDeviceToHost));
+        ctx->stats.node_visits += st[0];
+        ctx->stats.tri_tests += st[1];
+    }
+    ctx->stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    return PT_OK;
+}
+
+int pt_read_buffer(pt_ctx *ctx, int which, void *dst, size_t bytes) {
+    if (!ctx || !dst) return PT_ERR_INVALID_ARG;
+    if (!ctx->has_scene) {
+        ctx->error = "pt_read_buffer: no scene";
+        return PT_ERR_NO_SCENE;
+    }
+    (void)hipSetDevice(ctx->device);
+    const size_t npix = (size_t)ctx->width * ctx->height;
+    const void *src = nullptr;
+    size_t need = 0;
+    switch (which) {
+        case PT_BUF_FRAME: src = ctx->d_frame; need = npix * 16; break;
+        case PT_BUF_ACCUM: src = ctx->d_accum; need = npix * 16; break;
+        case PT_BUF_ALBEDO: src = ctx->d_albedo; need = npix * 12; break;
+        case PT_BUF_NORMAL: src = ctx->d_normal; need = npix * 12; break;
+        case PT_BUF_TEST: src = ctx->d_test; need = npix * 4; break;
+        case PT_BUF_PRIM_ID: src = ctx->d_prim; need = npix * 4; break;
+        case PT_BUF_ACCUM64: src = ctx->d_accum64; need = npix * 24; break;
+        default: ctx->error = "pt_read_buffer: unknown buffer"; return PT_ERR_INVALID_ARG;
+    }
+    if (bytes < need) {
+        ctx->error = "pt_read_buffer: destination too small";
+        return PT_ERR_INVALID_ARG;
+    }
+    PT_HIP(hipStreamSynchronize(ctx->stream));
+    PT_HIP(hipMemcpy(dst, src, need, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+int pt_get_stats(pt_ctx *ctx, pt_stats *out) {
+    if (!ctx || !out) return PT_ERR_INVALID_ARG;
+    *out = ctx->stats;
+    out->bvh_nodes = ctx->bvh_stats.wide_nodes;
+    out->bvh_leaf_tris = ctx->bvh_stats.leaf_tris;
+    out->bvh_depth = ctx->bvh_stats.depth;
+    out->bvh_build_ms = ctx->bvh_stats.build_seconds * 1000.0;
+    return PT_OK;
+}
+
+int pt_trace_rays(pt_ctx *ctx, const float *rays, uint32_t count, int any_hit, void *hits) {
+    if (!ctx || (!rays && count) || (!hits && count)) return PT_ERR_INVALID_ARG;
+    if (!ctx->has_bvh) {
+        ctx->error = "pt_trace_rays: BVH not built";
+        return PT_ERR_NO_BVH;
+    }
+    if (count == 0) return PT_OK;
+    (void)hipSetDevice(ctx->device);
+    float *d_rays = nullptr;
+    uint4 *d_hits = nullptr;
+    PT_HIP(hipMalloc((void **)&d_rays, (size_t)count * 8 * sizeof(float)));
+    if (hipMalloc((void **)&d_hits, (size_t)count * sizeof(uint4)) != hipSuccess) {
+        (void)hipFree(d_rays);
+        ctx->error = "pt_trace_rays: hipMalloc failed";
+        return PT_ERR_HIP;
+    }
+    int rc = PT_OK;
+    do {
+        if (hipMemcpy(d_rays, rays, (size_t)count * 8 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+            rc = PT_ERR_HIP;
+            break;
+        }
+        const DeviceBvh bvh = device_bvh(ctx);
+        const uint32_t grid = std::min<uint32_t>((count + kTraceBlock - 1) / kTraceBlock, trace_grid(ctx));
+        if (any_hit)
+            hipLaunchKernelGGL(k_trace_rays<true>, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, d_rays, count, bvh,
+                               d_hits);
+        else
+            hipLaunchKernelGGL(k_trace_rays<false>, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, d_rays, count, bvh,
+                               d_hits);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            rc = PT_ERR_HIP;
+            break;
+        }
+        if (hipMemcpy(hits, d_hits, (size_t)count * sizeof(uint4), hipMemcpyDeviceToHost) != hipSuccess) rc = PT_ERR_HIP;
+    } while (0);
+    (void)hipFree(d_rays);
+    (void)hipFree(d_hits);
+    if (rc != PT_OK) ctx->error = "pt_trace_rays: HIP call failed";
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,871 @@
+// Scene file input (mitsuba-subset XML, Wavefront OBJ) and EXR output.
+//
+// XML semantics follow resource/xml/visitor.h:18-194 and
+// resource/xml/object.cpp:9-62 (objects vs. properties, <default> with $name
+// substitution, <ref id>), resource/scene.cpp:27-226 (integrator, sensor,
+// film, emitters), resource/shape.cpp:85-266 (shapes) and
+// resource/material.cpp:26-190 (BSDFs).  Transforms compose
+// scale -> rotate -> translate regardless of element order
+// (resource/xml/util_loader.cpp:128-192).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+
+#include "pt_scene.h"
+
+namespace pupil {
+namespace {
+
+// ---------------------------------------------------------------------------
+// Minimal XML DOM.
+struct XmlNode {
+    std::string name;
+    std::vector<std::pair<std::string, std::string>> attrs;
+    std::vector<std::unique_ptr<XmlNode>> children;
+    std::string attr(const std::string &k) const {
+        for (auto &a : attrs)
+            if (a.first == k) return a.second;
+        return "";
+    }
+    bool has_attr(const std::string &k) const {
+        for (auto &a : attrs)
+            if (a.first == k) return true;
+        return false;
+    }
+};
+
+class XmlReader {
+public:
+    explicit XmlReader(const std::string &s) : s_(s) {}
+    std::unique_ptr<XmlNode> parse(std::string &err) {
+        std::unique_ptr<XmlNode> root;
+        skip_misc();
+        if (!element(root, err)) return nullptr;
+        return root;
+    }
+
+private:
+    void skip_ws() {
+        while (i_ < s_.size() && std::isspace((unsigned char)s_[i_])) ++i_;
+    }
+    void skip_misc() {
+        for (;;) {
+            skip_ws();
+            if (s_.compare(i_, 4, "<!--") == 0) {
+                size_t e = s_.find("-->", i_ + 4);
+                i_ = e == std::string::npos ? s_.size() : e + 3;
+            } else if (s_.compare(i_, 2, "<?") == 0) {
+                size_t e = s_.find("?>", i_ + 2);
+                i_ = e == std::string::npos ? s_.size() : e + 2;
+            } else if (s_.compare(i_, 2, "<!") == 0) {
+                size_t e = s_.find('>', i_ + 2);
+                i_ = e == std::string::npos ? s_.size() : e + 1;
+            } else {
+                return;
+            }
+        }
+    }
+    static std::string unescape(const std::string &v) {
+        std::string r;
+        for (size_t k = 0; k < v.size(); ++k) {
+            if (v[k] == '&') {
+                size_t e = v.find(';', k);
+                if (e != std::string::npos) {
+                    std::string ent = v.substr(k + 1, e - k - 1);
+                    if (ent == "lt") r += '<';
+                    else if (ent == "gt") r += '>';
+                    else if (ent == "amp") r += '&';
+                    else if (ent == "quot") r += '"';
+                    else if (ent == "apos") r += '\'';
+                    else r += v.substr(k, e - k + 1);
+                    k = e;
+                    continue;
+                }
+            }
+            r += v[k];
+        }
+        return r;
+    }
+    bool element(std::unique_ptr<XmlNode> &out, std::string &err) {
+        if (i_ >= s_.size() || s_[i_] != '<') {
+            err = "parse: expected '<'";
+            return false;
+        }
+        ++i_;
+        auto node = std::make_unique<XmlNode>();
+        while (i_ < s_.size() && !std::isspace((unsigned char)s_[i_]) && s_[i_] != '>' && s_[i_] != '/')
+            node->name += s_[i_++];
+        for (;;) {
+            skip_ws();
+            if (i_ >= s_.size()) {
+                err = "parse: unterminated tag <" + node->name + ">";
+                return false;
+            }
+            if (s_[i_] == '/') {
+                if (s_.compare(i_, 2, "/>") != 0) {
+                    err = "parse: bad tag end";
+                    return false;
+                }
+                i_ += 2;
+                out = std::move(node);
+                return true;
+            }
+            if (s_[i_] == '>') {
+                ++i_;
+                break;
+            }
+            std::string key;
+            while (i_ < s_.size() && s_[i_] != '=' && !std::isspace((unsigned char)s_[i_])) key += s_[i_++];
+            skip_ws();
+            if (i_ >= s_.size() || s_[i_] != '=') {
+                err = "parse: attribute without value in <" + node->name + ">";
+                return false;
+            }
+            ++i_;
+            skip_ws();
+            if (i_ >= s_.size() || (s_[i_] != '"' && s_[i_] != '\'')) {
+                err = "parse: unquoted attribute";
+                return false;
+            }
+            char q = s_[i_++];
+            size_t e = s_.find(q, i_);
+            if (e == std::string::npos) {
+                err = "parse: unterminated attribute";
+                return false;
+            }
+            node->attrs.emplace_back(key, unescape(s_.substr(i_, e - i_)));
+            i_ = e + 1;
+        }
+        // children until </name>
+        for (;;) {
+            while (i_ < s_.size() && s_[i_] != '<') ++i_;  // text is ignored
+            if (i_ >= s_.size()) {
+                err = "parse: missing </" + node->name + ">";
+                return false;
+            }
+            if (s_.compare(i_, 4, "<!--") == 0 || s_.compare(i_, 2, "<?") == 0) {
+                skip_misc();
+                continue;
+            }
+            if (s_.compare(i_, 2, "</") == 0) {
+                size_t e = s_.find('>', i_);
+                if (e == std::string::npos) {
+                    err = "parse: bad closing tag";
+                    return false;
+                }
+                std::string nm = s_.substr(i_ + 2, e - i_ - 2);
+                while (!nm.empty() && std::isspace((unsigned char)nm.back())) nm.pop_back();
+                if (nm != node->name) {
+                    err = "parse: mismatched </" + nm + "> for <" + node->name + ">";
+                    return false;
+                }
+                i_ = e + 1;
+                out = std::move(node);
+                return true;
+            }
+            std::unique_ptr<XmlNode> child;
+            if (!element(child, err)) return false;
+            node->children.push_back(std::move(child));
+        }
+    }
+    const std::string &s_;
+    size_t i_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Generic object tree of resource/xml/object.h.
+struct Obj {
+    std::string obj_name;   // element name (bsdf, shape, ...)
+    std::string type;       // type attribute
+    std::string id;
+    std::string var_name;   // name attribute
+    std::vector<std::pair<std::string, std::string>> props;
+    std::vector<Obj *> subs;
+    std::string prop(const std::string &k) const {
+        for (auto &p : props)
+            if (p.first == k) return p.second;
+        return "";
+    }
+    Obj *sub(const std::string &n) const {
+        for (Obj *o : subs)
+            if (o->obj_name == n) return o;
+        return nullptr;
+    }
+    Obj *param(const std::string &n) const {
+        for (Obj *o : subs)
+            if (o->var_name == n) return o;
+        return nullptr;
+    }
+};
+
+struct ObjTree {
+    std::vector<std::unique_ptr<Obj>> pool;
+    std::map<std::string, Obj *> ids;
+    std::vector<std::pair<std::string, std::string>> defaults;
+    Obj *root = nullptr;
+
+    std::string subst(const std::string &v) const {
+        if (v.find('$') == std::string::npos) return v;
+        // Longer names first so "$resx" is not consumed by "$res".
+        std::vector<std::pair<std::string, std::string>> d = defaults;
+        std::stable_sort(d.begin(), d.end(), [](const auto &a, const auto &b) { return a.first.size() > b.first.size(); });
+        std::string r = v;
+        for (auto &kv : d) {
+            std::string key = "$" + kv.first;
+            size_t pos = 0;
+            while ((pos = r.find(key, pos)) != std::string::npos) {
+                r.replace(pos, key.size(), kv.second);
+                pos += kv.second.size();
+            }
+        }
+        return r;
+    }
+    std::string a(const XmlNode &n, const std::string &k) const { return subst(n.attr(k)); }
+
+    Obj *make(const std::string &obj_name, const std::string &type) {
+        pool.push_back(std::make_unique<Obj>());
+        Obj *o = pool.back().get();
+        o->obj_name = obj_name;
+        o->type = type;
+        return o;
+    }
+
+    void visit(const XmlNode &n, Obj *cur) {
+        static const char *objects[] = {"bsdf", "emitter", "film", "integrator", "sensor", "shape", "texture", "transform"};
+        static const char *props[] = {"integer", "string", "float", "rgb", "boolean", "matrix"};
+        const std::string &nm = n.name;
+        if (nm == "scene") {
+            root = make("scene", a(n, "version"));
+            for (auto &c : n.children) visit(*c, root);
+            return;
+        }
+        if (nm == "default") {
+            defaults.emplace_back(a(n, "name"), a(n, "value"));
+            return;
+        }
+        if (nm == "ref") {
+            auto it = ids.find(a(n, "id"));
+            if (it != ids.end() && cur) cur->subs.push_back(it->second);
+            return;
+        }
+        if (nm == "lookat") {
+            Obj *o = make("lookat", "");
+            o->props.emplace_back("origin", a(n, "origin"));
+            o->props.emplace_back("target", a(n, "target"));
+            o->props.emplace_back("up", a(n, "up"));
+            if (cur) cur->subs.push_back(o);
+            return;
+        }
+        if (nm == "rotate") {
+            Obj *o = make("rotate", "");
+            std::string axis;
+            if (n.has_attr("value")) axis = a(n, "value");
+            else if (n.has_attr("x")) axis = "1, 0, 0";
+            else if (n.has_attr("y")) axis = "0, 1, 0";
+            else if (n.has_attr("z")) axis = "0, 0, 1";
+            o->props.emplace_back("axis", axis);
+            o->props.emplace_back("angle", a(n, "angle"));
+            if (cur) cur->subs.push_back(o);
+            return;
+        }
+        if (nm == "scale" || nm == "point" || nm == "translate") {
+            std::string name = a(n, "name");
+            if (name.empty()) name = nm;
+            std::string value = a(n, "value");
+            if (value.empty()) {
+                const char *dflt = nm == "scale" ? "1" : "0";
+                std::string x = a(n, "x"), y = a(n, "y"), z = a(n, "z");
+                value = (x.empty() ? dflt : x) + "," + (y.empty() ? dflt : y) + "," + (z.empty() ? dflt : z);
+            }
+            if (cur) cur->props.emplace_back(name, value);
+            return;
+        }
+        for (const char *p : props) {
+            if (nm == p) {
+                std::string name = a(n, "name");
+                if (name.empty()) name = nm;
+                if (cur) cur->props.emplace_back(name, a(n, "value"));
+                return;
+            }
+        }
+        for (const char *ob : objects) {
+            if (nm == ob) {
+                Obj *o = make(nm, a(n, "type"));
+                o->id = a(n, "id");
+                o->var_name = a(n, "name");
+                if (!o->id.empty()) ids[o->id] = o;
+                if (cur) cur->subs.push_back(o);
+                for (auto &c : n.children) visit(*c, o);
+                return;
+            }
+        }
+        // Unknown element: skipped together with its children (visitor.h:81-87).
+    }
+};
+
+std::vector<std::string> split(const std::string &s, char sep) {
+    std::vector<std::string> r;
+    std::string cur;
+    for (char c : s) {
+        if (c == sep) {
+            r.push_back(cur);
+            cur.clear();
+        } else {
+            cur += c;
+        }
+    }
+    r.push_back(cur);
+    return r;
+}
+std::vector<std::string> split_ws(const std::string &s) {
+    std::vector<std::string> r;
+    std::istringstream is(s);
+    std::string t;
+    while (is >> t) r.push_back(t);
+    return r;
+}
+bool to_float(const std::string &s, float &v) {
+    try {
+        size_t k = 0;
+        v = std::stof(s, &k);
+        return true;
+    } catch (...) {
+        return false;
+    }
+}
+bool load_float(const Obj *o, const std::string &k, float &v, float dflt) {
+    std::string s = o->prop(k);
+    if (s.empty() || !to_float(s, v)) {
+        v = dflt;
+        return false;
+    }
+    return true;
+}
+bool load_int(const Obj *o, const std::string &k, int &v, int dflt) {
+    std::string s = o->prop(k);
+    if (s.empty()) {
+        v = dflt;
+        return false;
+    }
+    try {
+        v = std::stoi(s);
+    } catch (...) {
+        v = dflt;
+        return false;
+    }
+    return true;
+}
+bool load_float3(const std::string &s, float v[3], float d0, float d1, float d2, bool allow_scalar = true) {
+    v[0] = d0;
+    v[1] = d1;
+    v[2] = d2;
+    if (s.empty()) return false;
+    auto xyz = split(s, ',');
+    if (xyz.size() == 3) {
+        for (int k = 0; k < 3; ++k)
+            if (!to_float(xyz[k], v[k])) return false;
+        return true;
+    }
+    if (xyz.size() == 1 && allow_scalar) {
+        float f;
+        if (!to_float(xyz[0], f)) return false;
+        v[0] = v[1] = v[2] = f;
+        return true;
+    }
+    return false;
+}
+bool load_bool(const Obj *o, const std::string &k, bool dflt) {
+    std::string s = o->prop(k);
+    if (s == "true") return true;
+    if (s == "false") return false;
+    return dflt;
+}
+
+struct Loader {
+    ObjTree &tree;
+    HostScene &scene;
+    std::string dir;
+    std::string err;
+    std::map<std::string, std::shared_ptr<MeshData>> meshes;
+    // rectangle and cube shapes share one flip_normals flag each: the last
+    // parsed instance wins (resource/shape.cpp:85-111, 457-497).
+    std::vector<std::pair<uint32_t, int>> shared_flip_users;  // (shape index, 0 rect / 1 cube)
+    bool rect_flip = false, cube_flip = false;
+
+    Mat4 load_transform(const Obj *t, bool &is_lookat) {
+        is_lookat = false;
+        Transform tr;
+        if (!t) return tr.m;
+        std::string mv = t->prop("matrix");
+        if (!mv.empty()) {
+            auto e = split_ws(mv);
+            Mat4 m = Mat4::identity();
+            if (e.size() == 16) {
+                for (int i = 0; i < 16; ++i) to_float(e[i], m.e[i]);
+            } else if (e.size() == 9) {
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) to_float(e[3 * r + c], m.e[4 * r + c]);
+            } else {
+                for (size_t i = 0; i < e.size() && i < 16; ++i) to_float(e[i], m.e[i]);
+            }
+            return m;
+        }
+        if (Obj *la = t->sub("lookat")) {
+            float o[3], tg[3], up[3];
+            load_float3(la->prop("origin"), o, 1.f, 0.f, 0.f, false);
+            load_float3(la->prop("target"), tg, 0.f, 0.f, 0.f, false);
+            load_float3(la->prop("up"), up, 0.f, 1.f, 0.f, false);
+            tr.look_at(o, tg, up);
+            for (int r = 0; r < 3; ++r) {  // mitsuba axis convention (util_loader.cpp:153-160)
+                tr.m.e[4 * r + 0] *= -1.f;
+                tr.m.e[4 * r + 2] *= -1.f;
+            }
+            is_lookat = true;
+            return tr.m;
+        }
+        float s[3];
+        if (load_float3(t->prop("scale"), s, 1.f, 1.f, 1.f)) tr.scale(s[0], s[1], s[2]);
+        if (Obj *ro = t->sub("rotate")) {
+            float ax[3], ang;
+            if (load_float3(ro->prop("axis"), ax, 0.f, 0.f, 0.f, false) && load_float(ro, "angle", ang, 0.f))
+                tr.rotate(ax[0], ax[1], ax[2], ang);
+        }
+        float tt[3];
+        if (load_float3(t->prop("translate"), tt, 0.f, 0.f, 0.f, false)) tr.translate(tt[0], tt[1], tt[2]);
+        return tr.m;
+    }
+
+    bool load_texture(const Obj *o, const std::string &name, pt_texture_desc &tex, float d0, float d1, float d2) {
+        Obj *t = o->param(name);
+        if (!t) {
+            float c[3];
+            std::string v = o->prop(name);
+            load_float3(v, c, d0, d1, d2);
+            tex = make_rgb_texture(c[0], c[1], c[2]);
+            return true;
+        }
+        if (t->type == "checkerboard") {
+            float c0[3], c1[3];
+            load_float3(t->prop("color0"), c0, 0.4f, 0.4f, 0.4f);
+            load_float3(t->prop("color1"), c1, 0.2f, 0.2f, 0.2f);
+            tex = make_checker_texture(c0, c1);
+            if (Obj *tr = t->sub("transform")) {
+                if (tr->var_name == "to_uv") {
+                    float s[3];
+                    if (load_float3(tr->prop("scale"), s, 1.f, 1.f, 1.f)) {
+                        tex.uv_row0[0] = s[0];
+                        tex.uv_row1[1] = s[1];
+                    }
+                }
+            }
+            return true;
+        }
+        if (t->type == "bitmap") {
+            err = "unsupported: bitmap textures";
+            return false;
+        }
+        tex = make_rgb_texture(0.f, 0.f, 0.f);  // unknown texture type reads black
+        return true;
+    }
+
+    float dielectric_ior(const Obj *o, const std::string &k, float dflt) {
+        std::string v = o->prop(k);
+        if (v.empty()) return dflt;
+        float f;
+        if (to_float(v, f)) return f;
+        if (lookup_dielectric_ior(v, f)) return f;
+        return dflt;
+    }
+
+    bool load_material(const Obj *o, pt_material_desc &m) {
+        std::memset(&m, 0, sizeof(m));
+        for (int i = 0; i < 4; ++i) m.tex[i] = make_rgb_texture(0.f, 0.f, 0.f);
+        m.int_ior = 1.5046f;
+        m.ext_ior = 1.000277f;
+        if (!o) {  // no bsdf: Material{} of type Unknown; use the loader's diffuse default
+            m.type = PT_MAT_DIFFUSE;
+            m.tex[0] = make_rgb_texture(0.5f, 0.5f, 0.5f);
+            return true;
+        }
+        const std::string &t = o->type;
+        if (t == "twosided") {
+            if (!load_material(o->sub("bsdf"), m)) return false;
+            m.twosided = 1;
+            return true;
+        }
+        if (t == "diffuse") {
+            m.type = PT_MAT_DIFFUSE;
+            return load_texture(o, "reflectance", m.tex[0], 0.5f, 0.5f, 0.5f);
+        }
+        if (t == "dielectric" || t == "roughdielectric") {
+            m.type = t == "dielectric" ? PT_MAT_DIELECTRIC : PT_MAT_ROUGH_DIELECTRIC;
+            m.int_ior = dielectric_ior(o, "int_ior", 1.5046f);
+            m.ext_ior = dielectric_ior(o, "ext_ior", 1.000277f);
+            if (!load_texture(o, "specular_reflectance", m.tex[0], 1.f, 1.f, 1.f)) return false;
+            if (!load_texture(o, "specular_transmittance", m.tex[1], 1.f, 1.f, 1.f)) return false;
+            if (m.type == PT_MAT_ROUGH_DIELECTRIC && !load_texture(o, "alpha", m.tex[2], 0.1f, 0.1f, 0.1f)) return false;
+            return true;
+        }
+        if (t == "conductor" || t == "roughconductor") {
+            m.type = t == "conductor" ? PT_MAT_CONDUCTOR : PT_MAT_ROUGH_CONDUCTOR;
+            float eta[3] = {0.f, 0.f, 0.f}, k[3] = {1.f, 1.f, 1.f};
+            if (!lookup_conductor_ior(o->prop("material"), eta, k)) {
+                eta[0] = eta[1] = eta[2] = 0.f;
+                k[0] = k[1] = k[2] = 1.f;
+            }
+            if (!load_texture(o, "specular_reflectance", m.tex[0], 1.f, 1.f, 1.f)) return false;
+            if (!load_texture(o, "eta", m.tex[1], eta[0], eta[1], eta[2])) return false;
+            if (!load_texture(o, "k", m.tex[2], k[0], k[1], k[2])) return false;
+            if (m.type == PT_MAT_ROUGH_CONDUCTOR && !load_texture(o, "alpha", m.tex[3], 0.1f, 0.1f, 0.1f)) return false;
+            return true;
+        }
+        if (t == "plastic" || t == "roughplastic") {
+            m.type = t == "plastic" ? PT_MAT_PLASTIC : PT_MAT_ROUGH_PLASTIC;
+            m.int_ior = dielectric_ior(o, "int_ior", 1.49f);
+            m.ext_ior = dielectric_ior(o, "ext_ior", 1.000277f);
+            m.nonlinear = o->prop("nonlinear") == "true" ? 1u : 0u;
+            if (!load_texture(o, "diffuse_reflectance", m.tex[0], 0.5f, 0.5f, 0.5f)) return false;
+            if (!load_texture(o, "specular_reflectance", m.tex[1], 1.f, 1.f, 1.f)) return false;
+            if (m.type == PT_MAT_ROUGH_PLASTIC && !load_texture(o, "alpha", m.tex[2], 0.1f, 0.1f, 0.1f)) return false;
+            return true;
+        }
+        // Unknown bsdf: the reference warns and uses an Unknown material that
+        // shades black; map it to a black diffuse surface.
+        m.type = PT_MAT_DIFFUSE;
+        m.tex[0] = make_rgb_texture(0.f, 0.f, 0.f);
+        return true;
+    }
+
+    bool load_shape(const Obj *o) {
+        std::shared_ptr<MeshData> mesh;
+        uint32_t flags = 0;
+        int shared_kind = -1;
+        if (o->type == "rectangle" || o->type == "cube") {
+            shared_kind = o->type == "rectangle" ? 0 : 1;
+            static const MeshData rect = make_rectangle_mesh(), cube = make_cube_mesh();
+            mesh = std::make_shared<MeshData>(shared_kind == 0 ? rect : cube);
+            bool flip = load_bool(o, "flip_normals", false);
+            (shared_kind == 0 ? rect_flip : cube_flip) = flip;
+        } else if (o->type == "obj") {
+            std::string fn = o->prop("filename");
+            std::string path = fn.size() && fn[0] == '/' ? fn : dir + "/" + fn;
+            auto it = meshes.find(path);
+            if (it == meshes.end()) {
+                auto m = std::make_shared<MeshData>();
+                if (!load_obj_mesh(path, *m, err)) return false;
+                meshes[path] = m;
+                mesh = m;
+            } else {
+                mesh = it->second;
+            }
+            if (load_bool(o, "flip_tex_coords", true)) flags |= PT_SHAPE_FLIP_TEXCOORDS;
+            if (load_bool(o, "flip_normals", false)) flags |= PT_SHAPE_FLIP_NORMALS;
+        } else if (o->type == "sphere") {
+            err = "unsupported: analytic sphere shapes";
+            return false;
+        } else {
+            return true;  // unknown shape type: skipped with a warning in the reference
+        }
+        pt_material_desc mat;
+        if (!load_material(o->sub("bsdf"), mat)) return false;
+        uint32_t mi = scene.add_material(mat);
+        bool lookat = false;
+        Mat4 to_world = load_transform(o->sub("transform"), lookat);
+        bool is_emitter = false;
+        pt_texture_desc rad = make_rgb_texture(0.f, 0.f, 0.f);
+        if (Obj *e = o->sub("emitter")) {
+            if (e->type == "area") {
+                is_emitter = true;
+                if (!load_texture(e, "radiance", rad, 0.f, 0.f, 0.f)) return false;
+            }
+        }
+        uint32_t si = scene.add_instance(*mesh, to_world, mi, flags, is_emitter, &rad);
+        if (shared_kind >= 0) shared_flip_users.emplace_back(si, shared_kind);
+        return true;
+    }
+
+    bool run() {
+        Obj *root = tree.root;
+        if (!root) {
+            err = "parse: no <scene> root";
+            return false;
+        }
+        scene.max_depth = 1;
+        int w = 768, h = 576;
+        float fov = 90.f, near_clip = 0.01f, far_clip = 10000.f;
+        Mat4 cam = Mat4::identity();
+        bool have_sensor = false;
+        for (Obj *o : root->subs) {
+            if (o->obj_name == "integrator") {
+                int md;
+                load_int(o, "max_depth", md, 1);
+                scene.max_depth = (uint32_t)std::max(1, md);
+            } else if (o->obj_name == "sensor") {
+                if (o->type != "perspective") continue;
+                have_sensor = true;
+                load_float(o, "fov", fov, 90.f);
+                load_float(o, "near_clip", near_clip, 0.01f);
+                load_float(o, "far_clip", far_clip, 10000.f);
+                if (Obj *film = o->sub("film")) {
+                    if (film->type == "hdrfilm") {
+                        load_int(film, "width", w, 768);
+                        load_int(film, "height", h, 576);
+                    }
+                }
+                std::string axis = o->prop("fov_axis");
+                char fov_axis = 'x';
+                if (axis == "y" || axis == "Y") fov_axis = 'y';
+                if (fov_axis == 'x') {  // resource/scene.cpp:122-127
+                    float aspect = (float)h / (float)w;
+                    float radian = fov * 3.14159265358979323846f / 180.f * 0.5f;
+                    float t = std::tan(radian) * aspect;
+                    fov = 2.f * std::atan(t) * 180.f / 3.14159265358979323846f;
+                }
+                bool lookat = false;
+                cam = load_transform(o->sub("transform"), lookat);
+                for (int r = 0; r < 3; ++r) {  // resource/scene.cpp:132-139
+                    cam.e[4 * r + 0] *= -1.f;
+                    cam.e[4 * r + 2] *= -1.f;
+                }
+            } else if (o->obj_name == "shape") {
+                if (!load_shape(o)) return false;
+            } else if (o->obj_name == "emitter") {
+                if (o->type == "constant") {
+                    float c[3];
+                    load_float3(o->prop("radiance"), c, 0.f, 0.f, 0.f);
+                    scene.env_type = PT_ENV_CONSTANT;
+                    for (int k = 0; k < 3; ++k) scene.env_radiance[k] = c[k];
+                } else if (o->type == "envmap") {
+                    err = "unsupported: envmap emitters";
+                    return false;
+                }
+            }
+        }
+        for (auto &u : shared_flip_users) {
+            bool flip = u.second == 0 ? rect_flip : cube_flip;
+            if (flip) scene.shapes[u.first].flags |= PT_SHAPE_FLIP_NORMALS;
+        }
+        if (w <= 0 || h <= 0) {
+            err = "parse: film size must be positive";
+            return false;
+        }
+        scene.width = (uint32_t)w;
+        scene.height = (uint32_t)h;
+        (void)have_sensor;
+        Mat4 s2c = sample_to_camera_matrix(fov, (float)w / (float)h, near_clip, far_clip);
+        std::memcpy(scene.camera.sample_to_camera, s2c.e, sizeof(s2c.e));
+        std::memcpy(scene.camera.camera_to_world, cam.e, sizeof(cam.e));
+        return true;
+    }
+};
+
+}  // namespace
+
+bool load_xml_scene(const std::string &path, HostScene &out, std::string &err) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) {
+        err = "io: cannot open " + path;
+        return false;
+    }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    std::string text = ss.str();
+    XmlReader rd(text);
+    auto dom = rd.parse(err);
+    if (!dom) return false;
+    if (dom->name != "scene") {
+        err = "parse: root element is <" + dom->name + ">, expected <scene>";
+        return false;
+    }
+    ObjTree tree;
+    tree.visit(*dom, nullptr);
+    out = HostScene();
+    Loader ld{tree, out, "", "", {}, {}, false, false};
+    size_t slash = path.find_last_of('/');
+    ld.dir = slash == std::string::npos ? "." : path.substr(0, slash);
+    if (!ld.run()) {
+        err = ld.err;
+        return false;
+    }
+    return true;
+}
+
+// Wavefront OBJ: v / vt / vn / f (polygons fan-triangulated); one vertex per
+// face corner.
+bool load_obj_mesh(const std::string &path, MeshData &mesh, std::string &err) {
+    std::ifstream f(path);
+    if (!f) {
+        err = "io: cannot open " + path;
+        return false;
+    }
+    std::vector<float> v, vt, vn;
+    mesh = MeshData();
+    bool any_n = false, any_t = false;
+    struct Corner { int v, t, n; };
+    std::vector<Corner> corners;
+    std::vector<uint32_t> idx;
+    std::string line;
+    while (std::getline(f, line)) {
+        std::istringstream is(line);
+        std::string tag;
+        if (!(is >> tag)) continue;
+        if (tag == "v") {
+            float x, y, z;
+            is >> x >> y >> z;
+            v.insert(v.end(), {x, y, z});
+        } else if (tag == "vt") {
+            float a, b;
+            is >> a >> b;
+            vt.insert(vt.end(), {a, b});
+        } else if (tag == "vn") {
+            float x, y, z;
+            is >> x >> y >> z;
+            vn.insert(vn.end(), {x, y, z});
+        } else if (tag == "f") {
+            std::vector<Corner> face;
+            std::string tok;
+            while (is >> tok) {
+                Corner c{0, 0, 0};
+                auto parts = split(tok, '/');
+                auto rd = [](const std::string &s, int count) {
+                    if (s.empty()) return 0;
+                    int i = std::stoi(s);
+                    return i < 0 ? count + i + 1 : i;
+                };
+                try {
+                    c.v = rd(parts[0], (int)v.size() / 3);
+                    if (parts.size() > 1) c.t = rd(parts[1], (int)vt.size() / 2);
+                    if (parts.size() > 2) c.n = rd(parts[2], (int)vn.size() / 3);
+                } catch (...) {
+                    err = "parse: bad face in " + path;
+                    return false;
+                }
+                if (c.v <= 0 || c.v > (int)v.size() / 3) {
+                    err = "parse: vertex index out of range in " + path;
+                    return false;
+                }
+                face.push_back(c);
+            }
+            for (size_t k = 1; k + 1 < face.size(); ++k) {
+                for (const Corner &c : {face[0], face[k], face[k + 1]}) {
+                    idx.push_back((uint32_t)corners.size());
+                    corners.push_back(c);
+                    any_n = any_n || c.n > 0;
+                    any_t = any_t || c.t > 0;
+                }
+            }
+        }
+    }
+    for (const Corner &c : corners) {
+        mesh.positions.insert(mesh.positions.end(), v.begin() + 3 * (c.v - 1), v.begin() + 3 * c.v);
+        if (any_n) {
+            if (c.n > 0 && c.n <= (int)vn.size() / 3)
+                mesh.normals.insert(mesh.normals.end(), vn.begin() + 3 * (c.n - 1), vn.begin() + 3 * c.n);
+            else
+                mesh.normals.insert(mesh.normals.end(), {0.f, 0.f, 0.f});
+        }
+        if (any_t) {
+            if (c.t > 0 && c.t <= (int)vt.size() / 2)
+                mesh.texcoords.insert(mesh.texcoords.end(), vt.begin() + 2 * (c.t - 1), vt.begin() + 2 * c.t);
+            else
+                mesh.texcoords.insert(mesh.texcoords.end(), {0.f, 0.f});
+        }
+    }
+    mesh.indices = idx;
+    return true;
+}
+
+// Single-part scanline EXR, no compression, FLOAT channels B, G, R.  Buffer
+// row 0 is the bottom of the image, so rows are written in reverse
+// (util/texture.cpp:37-44).
+bool write_exr_rgb(const std::string &path, const float *rgba, uint32_t w, uint32_t h, std::string &err) {
+    std::vector<unsigned char> out;
+    auto put = [&](const void *p, size_t n) {
+        const unsigned char *c = static_cast<const unsigned char *>(p);
+        out.insert(out.end(), c, c + n);
+    };
+    auto put_str = [&](const char *s) { put(s, std::strlen(s) + 1); };
+    auto put_i32 = [&](int32_t v) { put(&v, 4); };
+    auto put_f32 = [&](float v) { put(&v, 4); };
+    const unsigned char magic[4] = {0x76, 0x2f, 0x31, 0x01};
+    put(magic, 4);
+    put_i32(2);
+    put_str("channels");
+    put_str("chlist");
+    put_i32(3 * (2 + 16) + 1);
+    for (const char *ch : {"B", "G", "R"}) {
+        put_str(ch);
+        put_i32(2);  // FLOAT
+        const unsigned char z[4] = {0, 0, 0, 0};
+        put(z, 4);
+        put_i32(1);
+        put_i32(1);
+    }
+    out.push_back(0);
+    put_str("compression");
+    put_str("compression");
+    put_i32(1);
+    out.push_back(0);
+    for (const char *nm : {"dataWindow", "displayWindow"}) {
+        put_str(nm);
+        put_str("box2i");
+        put_i32(16);
+        put_i32(0);
+        put_i32(0);
+        put_i32((int32_t)w - 1);
+        put_i32((int32_t)h - 1);
+    }
+    put_str("lineOrder");
+    put_str("lineOrder");
+    put_i32(1);
+    out.push_back(0);
+    put_str("pixelAspectRatio");
+    put_str("float");
+    put_i32(4);
+    put_f32(1.f);
+    put_str("screenWindowCenter");
+    put_str("v2f");
+    put_i32(8);
+    put_f32(0.f);
+    put_f32(0.f);
+    put_str("screenWindowWidth");
+    put_str("float");
+    put_i32(4);
+    put_f32(1.f);
+    out.push_back(0);
+    const uint64_t line_bytes = 8 + (uint64_t)w * 3 * 4;
+    const uint64_t table_start = out.size();
+    for (uint32_t y = 0; y < h; ++y) {
+        uint64_t off = table_start + (uint64_t)h * 8 + (uint64_t)y * line_bytes;
+        put(&off, 8);
+    }
+    std::vector<float> row(w);
+    for (uint32_t y = 0; y < h; ++y) {
+        put_i32((int32_t)y);
+        put_i32((int32_t)(w * 3 * 4));
+        const float *src = rgba + (size_t)(h - 1 - y) * w * 4;
+        for (int ch : {2, 1, 0}) {
+            for (uint32_t x = 0; x < w; ++x) row[x] = src[4 * x + ch];
+            put(row.data(), (size_t)w * 4);
+        }
+    }
+    FILE *fp = std::fopen(path.c_str(), "wb");
+    if (!fp) {
+        err = "io: cannot write " + path;
+        return false;
+    }
+    size_t wr = std::fwrite(out.data(), 1, out.size(), fp);
+    std::fclose(fp);
+    if (wr != out.size()) {
+        err = "io: short write " + path;
+        return false;
+    }
+    return true;
+}
+
+}  // namespace pupil
