@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the plain wavefront MIS path tracer on one MI355X.
+
+Workload: the procedural terrain scene at 1920x1080 -- a 1024x1024-vertex fBm
+heightfield (2,093,058 triangles), 256 subdivision-3 icospheres (327,680
+triangles), four area-light quads, materials drawn from all seven BSDF types,
+constant environment 0.1 -- max depth 16, seed 0.  One step is one frame
+(one sample per pixel); the default run traces 64 frames (the 64-spp render).
+
+Prints one JSON line (rank 0).  `value` is Msamples/s with the scene and BVH
+resident on the GPU before the timed region starts.  The roofline block uses
+the kernel with the largest share of GPU time; its algorithmic bytes come from
+a counting pass over the same frames (identical rays: the tracer is
+deterministic) and its durations from HIP events recorded on the pass stream
+during the timed region.  The CPU baseline is the oracle (scalar C++,
+multithreaded) on a bounded band of rows of the same frame.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per unit ("Roofline accounting").
+RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 28, 16, 80, 36
+SHADOW_RESULT_BYTES = 4
+SHADE_BYTES_PER_VERTEX = 280
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene_desc, target_seconds, threads):
+    import oracle
+
+    w, h = scene_desc.width, scene_desc.height
+    # Calibrate on a band of rows through the middle of the frame, then size
+    # the sample (rows x spp) to the time budget.
+    rows = 32
+    y0 = h // 2 - rows // 2
+    r = oracle.render(scene_desc, spp=1, rows=(y0, y0 + rows), threads=threads)
+    per_row_ms = max(r["counts"]["render_ms"], 1e-3) / rows
+    budget_rows = target_seconds * 1000.0 / per_row_ms
+    spp = int(max(1, min(64, budget_rows // h)))
+    rows = int(max(32, min(h, budget_rows / spp)))
+    y0 = max(0, h // 2 - rows // 2)
+    r = oracle.render(scene_desc, spp=spp, rows=(y0, y0 + rows), threads=threads)
+    c = r["counts"]
+    samples = rows * w * spp
+    secs = c["render_ms"] / 1000.0
+    return {
+        "value": samples / secs / 1e6,
+        "unit": "Msamples/s",
+        "cores": int(c["threads"]),
+        "kind": "port",
+        "sample": (f"oracle (scalar C++, {int(c['threads'])} threads) on the same terrain frame, rows {y0}-"
+                   f"{y0 + rows - 1} of 1080, {spp} spp, depth 16: {samples} samples in {secs:.2f} s, "
+                   f"{(c['rays_extension'] + c['rays_shadow']) / secs / 1e6:.3f} Mrays/s; BVH build excluded"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64, help="frames (samples per pixel) timed")
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--spp-per-launch", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--grid", type=int, default=1024)
+    ap.add_argument("--spheres", type=int, default=256)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU split is described in DESIGN.md).", file=sys.stderr)
+        sys.exit(2)
+
+    import pupiloptixlab_amd as pt
+
+    t0 = time.perf_counter()
+    scene = pt.HostScene.generate(pt.SCENE_TERRAIN, seed=0, grid=args.grid, spheres=args.spheres)
+    d = scene.desc
+    log(f"scene: {d.num_triangles} tris, {d.width}x{d.height}, depth {d.max_depth}, "
+        f"gen {time.perf_counter() - t0:.2f}s")
+    tracer = pt.PathTracer(0)
+    tracer.set_scene(scene)
+    t1 = time.perf_counter()
+    tracer.build_bvh()
+    st = tracer.stats()
+    log(f"bvh: {st['bvh_nodes']} nodes, depth {st['bvh_depth']}, build {time.perf_counter() - t1:.2f}s")
+
+    npix = d.width * d.height
+    if args.warmup:
+        tracer.render(spp=args.warmup, aov=False, spp_per_launch=args.spp_per_launch)
+    # Counting pass over exactly the frames of the timed run (deterministic rays).
+    tracer.reset()
+    tracer.set_counting(True)
+    before = tracer.stats()
+    tracer.render(spp=args.steps, aov=False, spp_per_launch=args.spp_per_launch)
+    after = tracer.stats()
+    counted = {k: after[k] - before[k] for k in ("rays_extension", "rays_shadow", "node_visits", "tri_tests",
+                                                  "shadow_node_visits", "shadow_tri_tests")}
+    tracer.set_counting(False)
+    log(f"counted: {counted['rays_extension']} ext rays, {counted['rays_shadow']} shadow rays, "
+        f"{counted['node_visits']} nodes, {counted['tri_tests']} tris")
+
+    # Timed region: the pass is blocking, so the call brackets all GPU work.
+    tracer.reset()
+    base = tracer.stats()
+    t_start = time.perf_counter()
+    tracer.render(spp=args.steps, aov=False, timing=True, spp_per_launch=args.spp_per_launch)
+    elapsed = time.perf_counter() - t_start
+    s = tracer.stats()
+    ms = {k: s[k] - base[k] for k in ("ms_raygen", "ms_trace", "ms_shade", "ms_shadow", "ms_accumulate")}
+    launches = {k: s[k] - base[k] for k in ("launches_trace", "launches_shade", "launches_shadow")}
+    rays_ext = s["rays_extension"] - base["rays_extension"]
+    rays_shadow = s["rays_shadow"] - base["rays_shadow"]
+    assert rays_ext == counted["rays_extension"], "timed and counting passes diverged"
+
+    samples = args.steps * npix
+    value = samples / elapsed / 1e6
+    kernels = {
+        "trace": (ms["ms_trace"], launches["launches_trace"],
+                  rays_ext * (RAY_BYTES + HIT_BYTES) + counted["node_visits"] * NODE_BYTES
+                  + counted["tri_tests"] * TRI_BYTES),
+        "shadow": (ms["ms_shadow"], launches["launches_shadow"],
+                   rays_shadow * (RAY_BYTES + SHADOW_RESULT_BYTES) + counted["shadow_node_visits"] * NODE_BYTES
+                   + counted["shadow_tri_tests"] * TRI_BYTES),
+        "shade": (ms["ms_shade"], launches["launches_shade"], rays_ext * SHADE_BYTES_PER_VERTEX),
+    }
+    dominant = max(kernels, key=lambda k: kernels[k][0])
+    dms, dlaunch, dbytes = kernels[dominant]
+    achieved = dbytes / (dms / 1000.0) / 1e9 if dms > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_terrain_{dominant}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    per_kernel = {k: {"ms_total": round(v[0], 3), "launches": v[1],
+                      "ms_per_launch": round(v[0] / max(1, v[1]), 4),
+                      "algorithmic_GBps": round(v[2] / (v[0] / 1000.0) / 1e9, 1) if v[0] > 0 else None}
+                  for k, v in kernels.items()}
+    log(f"timed: {elapsed:.3f}s, {value:.2f} Msamples/s, kernels {json.dumps(per_kernel)}")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(d, args.cpu_seconds, threads)
+        log(f"cpu baseline: {cpu}")
+
+    result = {
+        "metric": "Msamples/s at 1080p on 1 GPU; relMSE at equal time, guided vs plain PT",
+        "value": round(value, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1000.0 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": ("terrain: 1024x1024-vertex fBm heightfield + 256 icospheres + 4 area lights, 7 BSDF types, "
+                         "constant env 0.1; 1920x1080, max depth 16, seed 0; plain PT with MIS"),
+            "triangles": d.num_triangles,
+            "width": d.width,
+            "height": d.height,
+            "spp": args.steps,
+            "max_depth": d.max_depth,
+            "parallelism": "single GPU",
+        },
+        "mrays_per_s": round((rays_ext + rays_shadow) / elapsed / 1e6, 2),
+        "kernels": per_kernel,
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": int(dbytes / max(1, dlaunch)),
+            "avg_launch_ms": round(dms / max(1, dlaunch), 4),
+        },
+        "cpu_baseline": cpu,
+        "relmse_guided_vs_plain": None,
+    }
+    tracer.close()
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -167,10 +167,14 @@ typedef struct pt_stats {
     uint64_t samples;        /* camera samples rendered since the last reset */
     uint64_t rays_extension; /* camera + bounce rays traced */
     uint64_t rays_shadow;    /* shadow rays traced */
-    uint64_t node_visits;    /* only with counting enabled */
+    uint64_t node_visits;    /* closest-hit traversal; only with counting enabled */
     uint64_t tri_tests;
+    uint64_t shadow_node_visits; /* any-hit traversal; only with counting enabled */
+    uint64_t shadow_tri_tests;
     double ms_total;         /* wall time of pt_render calls */
-    double ms_raygen, ms_trace, ms_shade, ms_shadow, ms_accumulate; /* HIP event sums */
+    /* Sums of per-launch HIP event times and the number of launches timed,
+     * for launches made with PT_RENDER_TIMING. */
+    double ms_raygen, ms_trace, ms_shade, ms_shadow, ms_accumulate;
     uint64_t launches_trace, launches_shade, launches_shadow;
     uint32_t bvh_nodes, bvh_leaf_tris, bvh_depth;
     double bvh_build_ms;

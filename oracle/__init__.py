@@ -23,7 +23,8 @@ class OrParams(C.Structure):
 
 class OrCounts(C.Structure):
     _fields_ = [("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64), ("node_visits", C.c_uint64),
-                ("tri_tests", C.c_uint64)]
+                ("tri_tests", C.c_uint64), ("build_ms", C.c_double), ("render_ms", C.c_double),
+                ("threads", C.c_uint32)]
 
 
 _lib = None

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <thread>
@@ -962,8 +963,10 @@ int oracle_render(const or_scene *scene, const or_params *params, float *accum, 
                   float *test, uint32_t *prim, double *sum64, or_counts *counts) {
     if (!scene || !params) return -1;
     const or_scene &s = *scene;
+    const auto t_build = std::chrono::steady_clock::now();
     World w;
     build_world(s, w);
+    const auto t_render = std::chrono::steady_clock::now();
     const uint32_t max_depth = params->max_depth ? params->max_depth : (s.max_depth ? s.max_depth : 1);
     const uint32_t y0 = params->row_begin, y1 = params->row_end ? std::min(params->row_end, s.height) : s.height;
     unsigned nthreads = params->threads ? params->threads : std::max(1u, std::thread::hardware_concurrency());
@@ -1006,8 +1009,12 @@ int oracle_render(const or_scene *scene, const or_params *params, float *accum, 
         });
     }
     for (auto &t : pool) t.join();
+    const auto t_end = std::chrono::steady_clock::now();
     if (counts) {
         std::memset(counts, 0, sizeof(*counts));
+        counts->build_ms = std::chrono::duration<double, std::milli>(t_render - t_build).count();
+        counts->render_ms = std::chrono::duration<double, std::milli>(t_end - t_render).count();
+        counts->threads = nthreads;
         for (unsigned ti = 0; ti < nthreads; ++ti) {
             counts->rays_extension += cnt[4 * ti];
             counts->rays_shadow += cnt[4 * ti + 1];

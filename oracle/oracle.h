@@ -77,6 +77,9 @@ typedef struct or_counts {
     uint64_t rays_shadow;
     uint64_t node_visits;
     uint64_t tri_tests;
+    double build_ms;   /* acceleration structure + emitter table */
+    double render_ms;  /* the per-pixel loop only */
+    uint32_t threads;  /* worker threads used */
 } or_counts;
 
 /* Renders params->spp frames from a reset accumulator.  Outputs (any may be

@@ -82,7 +82,8 @@ class RenderParams(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("samples", C.c_uint64), ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64),
-                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("ms_total", C.c_double),
+                ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64), ("shadow_node_visits", C.c_uint64),
+                ("shadow_tri_tests", C.c_uint64), ("ms_total", C.c_double),
                 ("ms_raygen", C.c_double), ("ms_trace", C.c_double), ("ms_shade", C.c_double),
                 ("ms_shadow", C.c_double), ("ms_accumulate", C.c_double), ("launches_trace", C.c_uint64),
                 ("launches_shade", C.c_uint64), ("launches_shadow", C.c_uint64), ("bvh_nodes", C.c_uint32),

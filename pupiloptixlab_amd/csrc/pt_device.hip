@@ -26,6 +26,8 @@ using namespace pupil;
 
 namespace {
 
+enum KernelKind { kKindRaygen = 0, kKindTrace, kKindShade, kKindShadow, kKindAccumulate };
+
 constexpr int kTraceBlock = 128;
 constexpr int kLdsStack = 16;  // stack entries per lane kept in LDS
 constexpr uint32_t kSpillEntries = kMaxBvhDepth + 2 - kLdsStack;
@@ -375,8 +377,19 @@ struct pt_ctx {
     uint32_t num_counters = 0;
     uint2 *d_spill = nullptr;
     uint32_t spill_threads = 0;
-    unsigned long long *d_stat = nullptr;
+    unsigned long long *d_stat = nullptr;  // closest nodes, closest tris, shadow nodes, shadow tris
     bool counting = false;
+    uint32_t *d_count_hist = nullptr;      // per-launch copies of the queue counters
+    uint32_t count_hist_slots = 0;         // capacity in counters
+
+    // kernel timing (PT_RENDER_TIMING)
+    struct Pending {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<hipEvent_t> event_pool;
+    size_t event_next = 0;
+    std::vector<Pending> pending;
 
     // progressive state (pt_pass.cpp:55-56)
     uint32_t sample_cnt = 0;
@@ -549,30 +562,52 @@ SceneView scene_view(pt_ctx *ctx) {
     return v;
 }
 
+hipEvent_t next_event(pt_ctx *ctx) {
+    if (ctx->event_next == ctx->event_pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->event_pool.push_back(e);
+    }
+    return ctx->event_pool[ctx->event_next++];
+}
+
+// Brackets one kernel launch with events on the pass stream; durations are
+// read after the render's final synchronization, so timing adds no host
+// synchronization inside the timed region.
 struct EventTimer {
     pt_ctx *ctx;
     bool on;
-    hipEvent_t a = nullptr, b = nullptr;
+    hipEvent_t a = nullptr;
     EventTimer(pt_ctx *c, bool enabled) : ctx(c), on(enabled) {
         if (on) {
-            (void)hipEventCreate(&a);
-            (void)hipEventCreate(&b);
-            (void)hipEventRecord(a, ctx->stream);
+            a = next_event(ctx);
+            if (a) (void)hipEventRecord(a, ctx->stream);
         }
     }
-    void stop(double &acc) {
-        if (!on) return;
+    void stop(int kind) {
+        if (!on || !a) return;
+        hipEvent_t b = next_event(ctx);
+        if (!b) return;
         (void)hipEventRecord(b, ctx->stream);
-        (void)hipEventSynchronize(b);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, a, b);
-        acc += ms;
-    }
-    ~EventTimer() {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
+        ctx->pending.push_back(pt_ctx::Pending{kind, a, b});
     }
 };
+
+void collect_event_times(pt_ctx *ctx) {
+    for (const auto &p : ctx->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.a, p.b) != hipSuccess) continue;
+        switch (p.kind) {
+            case kKindRaygen: ctx->stats.ms_raygen += ms; break;
+            case kKindTrace: ctx->stats.ms_trace += ms; ctx->stats.launches_trace++; break;
+            case kKindShade: ctx->stats.ms_shade += ms; ctx->stats.launches_shade++; break;
+            case kKindShadow: ctx->stats.ms_shadow += ms; ctx->stats.launches_shadow++; break;
+            default: ctx->stats.ms_accumulate += ms; break;
+        }
+    }
+    ctx->pending.clear();
+    ctx->event_next = 0;
+}
 
 }  // namespace
 
@@ -602,7 +637,7 @@ int pt_create(int hip_device, pt_ctx **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->num_cu = prop.multiProcessorCount;
-    if (hipMalloc((void **)&ctx->d_stat, 2 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc((void **)&ctx->d_stat, 4 * sizeof(unsigned long long)) != hipSuccess) {
         g_create_error = "hipMalloc failed";
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
@@ -620,6 +655,8 @@ void pt_destroy(pt_ctx *ctx) {
     free_queues(ctx);
     dfree(ctx->d_spill);
     dfree(ctx->d_stat);
+    dfree(ctx->d_count_hist);
+    for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -889,7 +926,7 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
     }
     int rc = ensure_queues(ctx, per_launch * npix, max_depth);
     if (rc != PT_OK) return rc;
-    if (ctx->counting) PT_HIP(hipMemsetAsync(ctx->d_stat, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    if (ctx->counting) PT_HIP(hipMemsetAsync(ctx->d_stat, 0, 4 * sizeof(unsigned long long), ctx->stream));
 
     const bool timing = (p->flags & PT_RENDER_TIMING) != 0;
     const SceneView sv = scene_view(ctx);
@@ -901,18 +938,27 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
     sp.aov = (p->flags & PT_RENDER_AOV) ? 1u : 0u;
     const uint32_t tg = trace_grid(ctx), sg = shade_grid(ctx);
 
-    uint32_t done = 0;
+    const uint32_t batches = (p->spp + per_launch - 1) / per_launch;
+    const uint32_t hist_elems = std::max(1u, batches) * ctx->num_counters;
+    if (hist_elems > ctx->count_hist_slots || !ctx->d_count_hist) {
+        dfree(ctx->d_count_hist);
+        PT_HIP(hipMalloc((void **)&ctx->d_count_hist, (size_t)hist_elems * sizeof(uint32_t)));
+        ctx->count_hist_slots = hist_elems;
+    }
+
+    uint32_t done = 0, batch = 0;
+    uint32_t sample_cnt = ctx->sample_cnt, seed = ctx->random_seed;
     while (done < p->spp) {
         const uint32_t frames = std::min(per_launch, p->spp - done);
         const uint32_t total = frames * npix;
-        uint32_t *ext = ctx->d_counters;                 // ext[b]
-        uint32_t *shd = ctx->d_counters + (max_depth + 2); // shd[b]
+        uint32_t *ext = ctx->d_counters;                    // ext[b]
+        uint32_t *shd = ctx->d_counters + (max_depth + 2);  // shd[b]
         {
             EventTimer t(ctx, timing);
             const uint32_t grid = std::min<uint32_t>((total + 255) / 256, (uint32_t)ctx->num_cu * 32u);
             hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, ctx->stream, ctx->q[0], ctx->camera, ctx->width,
-                               ctx->height, frames, ctx->random_seed, ctx->d_counters, ctx->num_counters);
-            t.stop(ctx->stats.ms_raygen);
+                               ctx->height, frames, seed, ctx->d_counters, ctx->num_counters);
+            t.stop(kKindRaygen);
         }
         AovBuffers aov{nullptr, nullptr, nullptr, nullptr, frames - 1, npix};
         if (sp.aov) {
@@ -932,26 +978,23 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
                 else
                     hipLaunchKernelGGL(k_trace<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->q[cur], bvh,
                                        ext + b, ctx->d_stat);
-                t.stop(ctx->stats.ms_trace);
-                ctx->stats.launches_trace++;
+                t.stop(kKindTrace);
             }
             {
                 EventTimer t(ctx, timing);
                 hipLaunchKernelGGL(k_shade, dim3(sg), dim3(256), 0, ctx->stream, sv, sp, ctx->q[cur], ext + b,
                                    ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov);
-                t.stop(ctx->stats.ms_shade);
-                ctx->stats.launches_shade++;
+                t.stop(kKindShade);
             }
             {
                 EventTimer t(ctx, timing);
                 if (ctx->counting)
                     hipLaunchKernelGGL(k_shadow<true>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->sq, bvh,
-                                       shd + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat);
+                                       shd + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2);
                 else
                     hipLaunchKernelGGL(k_shadow<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->sq, bvh,
-                                       shd + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat);
-                t.stop(ctx->stats.ms_shadow);
-                ctx->stats.launches_shadow++;
+                                       shd + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2);
+                t.stop(kKindShadow);
             }
             cur = nxt;
         }
@@ -960,29 +1003,40 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
             const uint32_t grid = std::min<uint32_t>((npix + 255) / 256, (uint32_t)ctx->num_cu * 16u);
             hipLaunchKernelGGL(k_accumulate, dim3(grid), dim3(256), 0, ctx->stream, ctx->d_out_l, ctx->d_accum,
                                ctx->d_frame, (p->flags & PT_RENDER_FP64_ACCUM) ? ctx->d_accum64 : nullptr, npix, frames,
-                               ctx->sample_cnt, p->accumulate);
-            t.stop(ctx->stats.ms_accumulate);
+                               sample_cnt, p->accumulate);
+            t.stop(kKindAccumulate);
         }
         PT_HIP(hipGetLastError());
-        // Ray counts of this launch (read back once per launch).
-        std::vector<uint32_t> counts(ctx->num_counters);
-        PT_HIP(hipMemcpyAsync(counts.data(), ctx->d_counters, counts.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                              ctx->stream));
-        PT_HIP(hipStreamSynchronize(ctx->stream));
-        for (uint32_t b = 0; b < max_depth; ++b) {
-            ctx->stats.rays_extension += counts[b];
-            ctx->stats.rays_shadow += counts[max_depth + 2 + b];
-        }
-        if (p->accumulate) ctx->sample_cnt += frames;
-        ctx->random_seed += frames;
-        ctx->stats.samples += (uint64_t)frames * npix;
+        // Keep this launch's queue counters; read back once after the last launch.
+        PT_HIP(hipMemcpyAsync(ctx->d_count_hist + (size_t)batch * ctx->num_counters, ctx->d_counters,
+                              ctx->num_counters * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+        if (p->accumulate) sample_cnt += frames;
+        seed += frames;
         done += frames;
+        ++batch;
     }
+    std::vector<uint32_t> hist((size_t)batch * ctx->num_counters);
+    PT_HIP(hipMemcpyAsync(hist.data(), ctx->d_count_hist, hist.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    PT_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint32_t k = 0; k < batch; ++k) {
+        const uint32_t *c = hist.data() + (size_t)k * ctx->num_counters;
+        for (uint32_t b = 0; b < max_depth; ++b) {
+            ctx->stats.rays_extension += c[b];
+            ctx->stats.rays_shadow += c[max_depth + 2 + b];
+        }
+    }
+    ctx->sample_cnt = sample_cnt;
+    ctx->random_seed = seed;
+    ctx->stats.samples += (uint64_t)p->spp * npix;
+    if (timing) collect_event_times(ctx);
     if (ctx->counting) {
-        unsigned long long st[2];
+        unsigned long long st[4];
         PT_HIP(hipMemcpy(st, ctx->d_stat, sizeof(st), hipMemcpyDeviceToHost));
         ctx->stats.node_visits += st[0];
         ctx->stats.tri_tests += st[1];
+        ctx->stats.shadow_node_visits += st[2];
+        ctx->stats.shadow_tri_tests += st[3];
     }
     ctx->stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
     return PT_OK;
