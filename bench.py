@@ -64,7 +64,50 @@ def cpu_baseline(scene_desc, target_seconds, threads):
     }
 
 
+COPY_PROBE_BYTES = 1 << 30
+
+
+def summarize_pmc(fetch_dir, write_dir, kernel_prefix, out_path):
+    """Per-dispatch HBM bytes of one kernel from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by the factor that
+    maps the counter onto the known bytes of the copy-probe kernel."""
+    import csv
+    import glob
+
+    def per_kernel(d, counter):
+        vals = {}
+        for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(path) as f:
+                for r in csv.DictReader(f):
+                    if r.get("Counter_Name") != counter:
+                        continue
+                    vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        return vals
+
+    out = {"kernel": kernel_prefix}
+    for counter, d, key in (("FETCH_SIZE", fetch_dir, "fetch"), ("WRITE_SIZE", write_dir, "write")):
+        vals = per_kernel(d, counter)
+        copy = [v for k, vs in vals.items() if "k_copy" in k for v in vs]
+        kern = [v for k, vs in vals.items() if kernel_prefix in k for v in vs]
+        if not copy or not kern:
+            raise SystemExit(f"missing {counter} samples for k_copy or {kernel_prefix} in {d}")
+        copy_kib = sum(copy) / len(copy)
+        factor = (COPY_PROBE_BYTES / 1024.0) / copy_kib
+        out[f"{key}_kib_per_dispatch_raw"] = sum(kern) / len(kern)
+        out[f"{key}_copy_factor"] = factor
+        out[f"{key}_bytes_per_dispatch"] = out[f"{key}_kib_per_dispatch_raw"] * 1024.0 * factor
+        out[f"{key}_dispatches"] = len(kern)
+    out["hbm_bytes_per_launch"] = int(out["fetch_bytes_per_dispatch"] + out["write_bytes_per_dispatch"])
+    with open(out_path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize-pmc":
+        # bench.py --summarize-pmc FETCH_DIR WRITE_DIR KERNEL OUT_JSON
+        summarize_pmc(*sys.argv[2:6])
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64, help="frames (samples per pixel) timed")
@@ -94,6 +137,8 @@ def main():
     tracer.build_bvh()
     st = tracer.stats()
     log(f"bvh: {st['bvh_nodes']} nodes, depth {st['bvh_depth']}, build {time.perf_counter() - t1:.2f}s")
+    copy_gbps = tracer.bandwidth_probe(COPY_PROBE_BYTES, 5)
+    log(f"device copy bandwidth: {copy_gbps:.0f} GB/s (read + write)")
 
     npix = d.width * d.height
     if args.warmup:
@@ -189,6 +234,7 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_launch": int(dbytes / max(1, dlaunch)),
             "avg_launch_ms": round(dms / max(1, dlaunch), 4),
+            "measured_copy_GBps": round(copy_gbps, 1),
         },
         "cpu_baseline": cpu,
         "relmse_guided_vs_plain": None,

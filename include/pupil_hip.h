@@ -210,6 +210,11 @@ int pt_set_counting(pt_ctx *ctx, int enable);
 /* Last error text of the context (or of the last failed pt_create when ctx is NULL). */
 const char *pt_last_error(const pt_ctx *ctx);
 
+/* Device-to-device streaming copy of `bytes` (x iters) on the pass stream;
+ * reports read+write GB/s.  No reference counterpart: used to measure the
+ * achievable HBM bandwidth next to the kernels' roofline figures. */
+int pt_bandwidth_probe(pt_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps);
+
 /* Traces caller-supplied rays against the built BVH on the GPU (closest hit
  * unless any_hit != 0).  rays: 8 floats each (origin xyz, tmin, dir xyz,
  * tmax).  hits: 4 words each (t, primitive id, b1, b2). */
@@ -221,7 +226,8 @@ typedef struct pt_host_scene pt_host_scene;
 enum pt_scene_kind {
     PT_SCENE_CORNELL = 1,   /* closed box, two rotated boxes, one area light */
     PT_SCENE_TERRAIN = 2,   /* fractal heightfield + icospheres, 7 materials, 4 lights + env */
-    PT_SCENE_DOOR_ROOM = 3  /* room lit through a narrow door gap */
+    PT_SCENE_DOOR_ROOM = 3, /* room lit through a narrow door gap */
+    PT_SCENE_FURNACE = 4    /* closed emissive diffuse cube seen from its center */
 };
 
 typedef struct pt_gen_params {
@@ -230,6 +236,8 @@ typedef struct pt_gen_params {
     uint32_t grid;          /* heightfield vertices per side (terrain), 0 = default 1024 */
     uint32_t spheres;       /* icospheres (terrain), 0 = default 256 */
     uint32_t max_depth;     /* 0 = scene default */
+    float albedo;           /* furnace wall albedo, 0 = 0.5 */
+    float radiance;         /* furnace wall emission, 0 = 1 */
 } pt_gen_params;
 
 /* Scene::LoadFromXML (resource/scene.cpp:27-56) for the mitsuba subset. */

@@ -946,7 +946,17 @@ void trace_path(const World &w, const or_params &p, uint32_t max_depth, uint32_t
                 erad = tex_sample(e.radiance, g.uv);
             }
             if (!iszero(epdf)) {
-                float mis = prev_delta ? 1.f : misw(br.pdf, epdf * e.sel);
+                float lpdf = epdf;
+                if (consistent) {
+                    // The pdf next-event estimation would assign to this point:
+                    // it uses the emitter's own vertex normals (area.h:19-33).
+                    float wv = 1.f - h.b1 - h.b2;
+                    V3 en = norm3(add(add(mul(e.n[0], wv), mul(e.n[1], h.b1)), mul(e.n[2], h.b2)));
+                    float nl = dot3(en, dir);
+                    float dist = len3(sub(ray_o, g.pos));
+                    lpdf = nl > 0.f ? dist * dist / (nl * e.area) : 0.f;
+                }
+                float mis = prev_delta ? 1.f : misw(br.pdf, lpdf * e.sel);
                 radiance = add(radiance, mul(mul(throughput, erad), mis));
             }
         }

@@ -37,7 +37,7 @@ ENV_NONE, ENV_CONSTANT = 0, 1
 MIS_REFERENCE, MIS_CONSISTENT = 0, 1
 RENDER_AOV, RENDER_FP64_ACCUM, RENDER_TIMING = 1, 2, 4
 BUF_FRAME, BUF_ACCUM, BUF_ALBEDO, BUF_NORMAL, BUF_TEST, BUF_PRIM_ID, BUF_ACCUM64 = range(7)
-SCENE_CORNELL, SCENE_TERRAIN, SCENE_DOOR_ROOM = 1, 2, 3
+SCENE_CORNELL, SCENE_TERRAIN, SCENE_DOOR_ROOM, SCENE_FURNACE = 1, 2, 3, 4
 MISS = 0xFFFFFFFF
 
 
@@ -95,7 +95,7 @@ class Stats(C.Structure):
 
 class GenParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32), ("grid", C.c_uint32),
-                ("spheres", C.c_uint32), ("max_depth", C.c_uint32)]
+                ("spheres", C.c_uint32), ("max_depth", C.c_uint32), ("albedo", C.c_float), ("radiance", C.c_float)]
 
 
 # Every symbol the C header declares, with its ctypes signature.
@@ -112,6 +112,7 @@ _SIGNATURES = {
     "pt_set_counting": (C.c_int, [C.c_void_p, C.c_int]),
     "pt_last_error": (C.c_char_p, [C.c_void_p]),
     "pt_trace_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),
+    "pt_bandwidth_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
     "pt_scene_load_xml": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]),
     "pt_scene_generate": (C.c_int, [C.c_int, C.POINTER(GenParams), C.POINTER(C.c_void_p)]),
     "pt_scene_get_desc": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
@@ -162,8 +163,8 @@ class HostScene:
 
     @classmethod
     def generate(cls, kind: int, seed: int = 0, width: int = 0, height: int = 0, grid: int = 0,
-                 spheres: int = 0, max_depth: int = 0) -> "HostScene":
-        gp = GenParams(seed, width, height, grid, spheres, max_depth)
+                 spheres: int = 0, max_depth: int = 0, albedo: float = 0.0, radiance: float = 0.0) -> "HostScene":
+        gp = GenParams(seed, width, height, grid, spheres, max_depth, albedo, radiance)
         h = C.c_void_p()
         _check(lib().pt_scene_generate(kind, C.byref(gp), C.byref(h)), "pt_scene_generate")
         return cls(h)
@@ -302,6 +303,12 @@ class PathTracer:
 
     def set_counting(self, enable: bool) -> None:
         _check(lib().pt_set_counting(self._ctx, int(enable)), "pt_set_counting", self._ctx)
+
+    def bandwidth_probe(self, nbytes: int = 1 << 30, iters: int = 10) -> float:
+        """Achievable device copy bandwidth in GB/s (read + write)."""
+        g = C.c_double()
+        _check(lib().pt_bandwidth_probe(self._ctx, nbytes, iters, C.byref(g)), "pt_bandwidth_probe", self._ctx)
+        return g.value
 
     def trace_rays(self, rays: np.ndarray, any_hit: bool = False):
         rays = np.ascontiguousarray(rays, dtype=np.float32)

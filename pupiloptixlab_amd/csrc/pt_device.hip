@@ -29,7 +29,7 @@ namespace {
 enum KernelKind { kKindRaygen = 0, kKindTrace, kKindShade, kKindShadow, kKindAccumulate };
 
 constexpr int kTraceBlock = 128;
-constexpr int kLdsStack = 16;  // stack entries per lane kept in LDS
+constexpr int kLdsStack = 12;  // stack entries per lane kept in LDS
 constexpr uint32_t kSpillEntries = kMaxBvhDepth + 2 - kLdsStack;
 
 // --- queue layout -------------------------------------------------------------
@@ -142,19 +142,34 @@ __global__ void k_raygen(PathQueue q, CameraRec cam, uint32_t width, uint32_t he
     }
 }
 
+// Wave-level work fetch: lane 0 reserves `n` consecutive items for the wave.
+__device__ __forceinline__ uint32_t wave_fetch(uint32_t *work, uint32_t n) {
+    uint32_t base = 0;
+    if (lane_id() == 0) base = atomicAdd(work, n);
+    return __shfl(base, 0);
+}
+
 // --- closest-hit traversal ------------------------------------------------------
+// Persistent grid sized to the resident occupancy; each wave pulls 64 rays at
+// a time, so waves that drew short traversals keep working.
 template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(PathQueue q, DeviceBvh bvh, const uint32_t *count_ptr,
-                                                       unsigned long long *stat) {
+                                                       uint32_t *work, unsigned long long *stat) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
     const uint32_t count = *count_ptr;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraverseStats st{0, 0};
-    for (uint32_t i = gtid; i < count; i += gridDim.x * blockDim.x) {
-        const float4 r0 = q.ray0[i], r1 = q.ray1[i];
-        Hit h;
-        trace_one<false, kCount>(bvh, lds + threadIdx.x, gtid, f3of(r0), f3of(r1), kExtTmin, kMaxDistance, h, st);
-        q.hit[i] = make_float4(h.t, u2f(h.prim), h.b1, h.b2);
+    for (;;) {
+        const uint32_t base = wave_fetch(work, 64u);
+        if (base >= count) break;
+        const uint32_t i = base + lane_id();
+        if (i < count) {
+            const float4 r0 = q.ray0[i], r1 = q.ray1[i];
+            Hit h;
+            trace_one<false, kCount>(bvh, lds + threadIdx.x, gtid, f3of(r0), f3of(r1), kExtTmin, kMaxDistance, h,
+                                     st);
+            q.hit[i] = make_float4(h.t, u2f(h.prim), h.b1, h.b2);
+        }
     }
     if (kCount) {
         atomicAdd(stat + 0, (unsigned long long)st.nodes);
@@ -165,16 +180,25 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(PathQueue q, DeviceBvh bv
 // --- shadow (any-hit) traversal ----------------------------------------------------
 template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(ShadowQueue sq, DeviceBvh bvh, const uint32_t *count_ptr,
-                                                        PathQueue next, float4 *out_l, unsigned long long *stat) {
+                                                        uint32_t *work, PathQueue next, float4 *out_l,
+                                                        unsigned long long *stat) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
     const uint32_t count = *count_ptr;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraverseStats st{0, 0};
-    for (uint32_t i = gtid; i < count; i += gridDim.x * blockDim.x) {
-        const float4 s0 = sq.s0[i], s1 = sq.s1[i];
-        Hit h;
-        bool occluded =
-            trace_one<true, kCount>(bvh, lds + threadIdx.x, gtid, f3of(s0), f3of(s1), kShadowEps, s0.w, h, st);
+    for (;;) {
+        const uint32_t base = wave_fetch(work, 64u);
+        if (base >= count) break;
+        const uint32_t i = base + lane_id();
+        bool occluded = true;
+        float4 s1 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < count) {
+            const float4 s0 = sq.s0[i];
+            s1 = sq.s1[i];
+            Hit h;
+            occluded =
+                trace_one<true, kCount>(bvh, lds + threadIdx.x, gtid, f3of(s0), f3of(s1), kShadowEps, s0.w, h, st);
+        }
         if (!occluded) {
             const float4 c = sq.s2[i];
             const uint32_t target = f2u(s1.w);
@@ -309,6 +333,13 @@ __global__ void k_accumulate(const float4 *out_l, float4 *accum, float4 *frame, 
     }
 }
 
+// Streaming copy used to measure achievable HBM bandwidth and to calibrate
+// hardware byte counters (pt_bandwidth_probe).
+__global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // Closest-hit / any-hit over caller rays (pt_trace_rays).
 template <bool kAnyHit>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(const float *rays, uint32_t count, DeviceBvh bvh,
@@ -335,6 +366,7 @@ struct pt_ctx {
     hipStream_t stream = nullptr;
     std::string error;
     int num_cu = 256;
+    uint32_t trace_blocks_per_cu = 8;
 
     // scene (host copies kept for the BVH build)
     bool has_scene = false;
@@ -511,7 +543,7 @@ MaterialRec to_material_rec(const pt_material_desc &m) {
 }
 
 int ensure_queues(pt_ctx *ctx, uint32_t paths, uint32_t max_depth) {
-    uint32_t need_counters = 2 * (max_depth + 2);
+    uint32_t need_counters = 4 * (max_depth + 2);
     if (paths <= ctx->capacity && need_counters <= ctx->num_counters) return PT_OK;
     free_queues(ctx);
     const size_t n = paths;
@@ -532,7 +564,7 @@ int ensure_queues(pt_ctx *ctx, uint32_t paths, uint32_t max_depth) {
     return PT_OK;
 }
 
-uint32_t trace_grid(pt_ctx *ctx) { return (uint32_t)ctx->num_cu * 16u; }
+uint32_t trace_grid(pt_ctx *ctx) { return (uint32_t)ctx->num_cu * ctx->trace_blocks_per_cu; }
 uint32_t shade_grid(pt_ctx *ctx) { return (uint32_t)ctx->num_cu * 8u; }
 
 int ensure_spill(pt_ctx *ctx) {
@@ -637,6 +669,13 @@ int pt_create(int hip_device, pt_ctx **out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->num_cu = prop.multiProcessorCount;
+    {
+        // Persistent traversal grid: exactly the blocks that can be resident.
+        int a = 0, b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace<false>, kTraceBlock, 0) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false>, kTraceBlock, 0) == hipSuccess)
+            ctx->trace_blocks_per_cu = (uint32_t)std::max(1, std::min(a, b));
+    }
     if (hipMalloc((void **)&ctx->d_stat, 4 * sizeof(unsigned long long)) != hipSuccess) {
         g_create_error = "hipMalloc failed";
         (void)hipStreamDestroy(ctx->stream);
@@ -918,7 +957,9 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
         return PT_ERR_INVALID_ARG;
     }
     uint32_t per_launch = p->spp_per_launch;
-    if (per_launch == 0) per_launch = std::max(1u, std::min(8u, (4u * 1024u * 1024u) / std::max(1u, npix)));
+    // Default launch width: ~64M paths in flight (about 15 GB of queues), which
+    // amortizes the latency-bound tail of the last bounces over many frames.
+    if (per_launch == 0) per_launch = std::max(1u, std::min(64u, (64u * 1024u * 1024u) / std::max(1u, npix)));
     per_launch = std::min(per_launch, std::max(1u, p->spp));
     if ((uint64_t)per_launch * npix > 0x7FFFFFFFull) {
         ctx->error = "pt_render: spp_per_launch * pixels exceeds the queue index range";
@@ -951,8 +992,10 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
     while (done < p->spp) {
         const uint32_t frames = std::min(per_launch, p->spp - done);
         const uint32_t total = frames * npix;
-        uint32_t *ext = ctx->d_counters;                    // ext[b]
-        uint32_t *shd = ctx->d_counters + (max_depth + 2);  // shd[b]
+        uint32_t *ext = ctx->d_counters;                          // extension queue sizes
+        uint32_t *shd = ctx->d_counters + (max_depth + 2);        // shadow queue sizes
+        uint32_t *twork = ctx->d_counters + 2 * (max_depth + 2);  // trace work cursors
+        uint32_t *swork = ctx->d_counters + 3 * (max_depth + 2);  // shadow work cursors
         {
             EventTimer t(ctx, timing);
             const uint32_t grid = std::min<uint32_t>((total + 255) / 256, (uint32_t)ctx->num_cu * 32u);
@@ -974,10 +1017,10 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
                 EventTimer t(ctx, timing);
                 if (ctx->counting)
                     hipLaunchKernelGGL(k_trace<true>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->q[cur], bvh,
-                                       ext + b, ctx->d_stat);
+                                       ext + b, twork + b, ctx->d_stat);
                 else
                     hipLaunchKernelGGL(k_trace<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->q[cur], bvh,
-                                       ext + b, ctx->d_stat);
+                                       ext + b, twork + b, ctx->d_stat);
                 t.stop(kKindTrace);
             }
             {
@@ -990,10 +1033,10 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
                 EventTimer t(ctx, timing);
                 if (ctx->counting)
                     hipLaunchKernelGGL(k_shadow<true>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->sq, bvh,
-                                       shd + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2);
+                                       shd + b, swork + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2);
                 else
                     hipLaunchKernelGGL(k_shadow<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->sq, bvh,
-                                       shd + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2);
+                                       shd + b, swork + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2);
                 t.stop(kKindShadow);
             }
             cur = nxt;
@@ -1079,6 +1122,47 @@ int pt_get_stats(pt_ctx *ctx, pt_stats *out) {
     out->bvh_depth = ctx->bvh_stats.depth;
     out->bvh_build_ms = ctx->bvh_stats.build_seconds * 1000.0;
     return PT_OK;
+}
+
+int pt_bandwidth_probe(pt_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps) {
+    if (!ctx || !gbps || bytes < 4096 || iters == 0) return PT_ERR_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    const size_t n = (size_t)(bytes / sizeof(float4));
+    float4 *a = nullptr, *b = nullptr;
+    PT_HIP(hipMalloc((void **)&a, n * sizeof(float4)));
+    if (hipMalloc((void **)&b, n * sizeof(float4)) != hipSuccess) {
+        (void)hipFree(a);
+        ctx->error = "pt_bandwidth_probe: hipMalloc failed";
+        return PT_ERR_HIP;
+    }
+    int rc = PT_OK;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    do {
+        if (hipMemsetAsync(a, 0, n * sizeof(float4), ctx->stream) != hipSuccess ||
+            hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            rc = PT_ERR_HIP;
+            break;
+        }
+        const uint32_t grid = (uint32_t)ctx->num_cu * 32u;
+        hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, ctx->stream, a, b, n);  // warm
+        (void)hipEventRecord(e0, ctx->stream);
+        for (uint32_t i = 0; i < iters; ++i)
+            hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, ctx->stream, a, b, n);
+        (void)hipEventRecord(e1, ctx->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) {
+            rc = PT_ERR_HIP;
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        *gbps = 2.0 * (double)n * sizeof(float4) * iters / (ms * 1e-3) / 1e9;
+    } while (0);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (rc != PT_OK) ctx->error = "pt_bandwidth_probe: HIP call failed";
+    return rc;
 }
 
 int pt_trace_rays(pt_ctx *ctx, const float *rays, uint32_t count, int any_hit, void *hits) {

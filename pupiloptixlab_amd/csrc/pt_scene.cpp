@@ -608,6 +608,24 @@ void build_door_room(HostScene &s, const pt_gen_params &p) {
     s.set_camera(60.f, cam.m);
 }
 
+// Closed cube [-1,1]^3 of twosided diffuse walls that all emit, seen from the
+// center: every path stays inside, so the expected pixel value is
+// Le * sum_{k < max_depth} albedo^k.
+void build_furnace(HostScene &s, const pt_gen_params &p) {
+    s.width = p.width ? p.width : 32;
+    s.height = p.height ? p.height : 32;
+    s.max_depth = p.max_depth ? p.max_depth : 6;
+    const float albedo = p.albedo > 0.f ? p.albedo : 0.5f;
+    const float le = p.radiance > 0.f ? p.radiance : 1.f;
+    uint32_t wall = s.add_material(make_diffuse(albedo, albedo, albedo, true));
+    pt_texture_desc rad = make_rgb_texture(le, le, le);
+    s.add_instance(make_cube_mesh(), Mat4::identity(), wall, 0, true, &rad);
+    float eye[3] = {0.1f, -0.05f, 0.2f}, target[3] = {0.3f, 0.1f, -1.f}, up[3] = {0.f, 1.f, 0.f};
+    Transform cam;
+    cam.look_at(eye, target, up);
+    s.set_camera(70.f, cam.m);
+}
+
 }  // namespace
 
 bool generate_scene(int kind, const pt_gen_params &params, HostScene &out, std::string &err) {
@@ -616,6 +634,7 @@ bool generate_scene(int kind, const pt_gen_params &params, HostScene &out, std::
         case PT_SCENE_CORNELL: build_cornell(out, params); return true;
         case PT_SCENE_TERRAIN: build_terrain(out, params); return true;
         case PT_SCENE_DOOR_ROOM: build_door_room(out, params); return true;
+        case PT_SCENE_FURNACE: build_furnace(out, params); return true;
         default: err = "unknown scene kind"; return false;
     }
 }

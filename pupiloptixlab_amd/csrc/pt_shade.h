@@ -180,7 +180,19 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
         float epdf;
         area_eval(e, g, ps.o, rad, epdf);
         if (!is_zero(epdf)) {
-            float mis = (ps.flags & kPathPrevDelta) ? 1.f : mis_weight(ps.prev_pdf, epdf * e.select_prob);
+            float lpdf = epdf;
+            if (consistent) {
+                // Weight against the pdf next-event estimation would assign to
+                // this point, which uses the emitter's own vertex normals.
+                const float wv = 1.f - b1 - b2;
+                const f3 en = normalize(mk3(e.n0.x, e.n0.y, e.n0.z) * wv + mk3(e.n1.x, e.n1.y, e.n1.z) * b1 +
+                                        mk3(e.n2.x, e.n2.y, e.n2.z) * b2);
+                const f3 dir = normalize(ps.o - g.position);
+                const float nl = dot(en, dir);
+                const float dist = length(ps.o - g.position);
+                lpdf = nl > 0.f ? dist * dist / (nl * e.area) : 0.f;
+            }
+            float mis = (ps.flags & kPathPrevDelta) ? 1.f : mis_weight(ps.prev_pdf, lpdf * e.select_prob);
             ps.L = ps.L + ps.T * rad * mis;
         }
     }
