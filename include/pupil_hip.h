@@ -31,7 +31,8 @@ enum pt_status {
     PT_ERR_PARSE = -6,
     PT_ERR_OOM = -7,
     PT_ERR_UNSUPPORTED = -8,
-    PT_ERR_NO_BVH = -9
+    PT_ERR_NO_BVH = -9,
+    PT_ERR_NO_GUIDE = -10 /* guide API called before pt_guide_enable */
 };
 
 /* Material types: EMatType order of decl/material_decl.inl:3-9. */
@@ -138,7 +139,10 @@ enum pt_mis_mode { PT_MIS_REFERENCE = 0, PT_MIS_CONSISTENT = 1 };
 enum pt_render_flags {
     PT_RENDER_AOV = 1,        /* write albedo / normal / test / primitive id */
     PT_RENDER_FP64_ACCUM = 2, /* also keep a double-precision sum */
-    PT_RENDER_TIMING = 4      /* record per-kernel HIP event times */
+    PT_RENDER_TIMING = 4,     /* record per-kernel HIP event times */
+    PT_RENDER_GUIDE = 8,      /* sample with the learned guiding proposal (needs
+                                 pt_guide_enable and PT_MIS_CONSISTENT) */
+    PT_RENDER_GUIDE_TRAIN = 16 /* train the guide online on this render's paths */
 };
 
 /* Launch parameters (example/path_tracer/type.h:9-33 config + pass knobs). */
@@ -178,6 +182,9 @@ typedef struct pt_stats {
     uint64_t launches_trace, launches_shade, launches_shadow;
     uint32_t bvh_nodes, bvh_leaf_tris, bvh_depth;
     double bvh_build_ms;
+    double ms_guide;          /* event time of guide training (collect + steps), with timing */
+    uint64_t guide_steps;     /* optimizer steps launched by pt_render / pt_guide_train */
+    uint64_t guide_samples;   /* training samples collected by pt_render */
 } pt_stats;
 
 typedef struct pt_ctx pt_ctx;
@@ -219,6 +226,58 @@ int pt_bandwidth_probe(pt_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps
  * unless any_hit != 0).  rays: 8 floats each (origin xyz, tmin, dir xyz,
  * tmax).  hits: 4 words each (t, primitive id, b1, b2). */
 int pt_trace_rays(pt_ctx *ctx, const float *rays, uint32_t count, int any_hit, void *hits);
+
+/* ---- learned path guiding ---------------------------------------------
+ * No reference counterpart: PupilOptixLab samples directions from the BSDF
+ * only (example/path_tracer/main.cu:146-160).  The guide is a hash-grid
+ * encoding of the hit position followed by an MLP whose 41 outputs define an
+ * 8-lobe von Mises-Fisher mixture and a selection probability c; directions
+ * are drawn from c * guide + (1 - c) * BSDF (one-sample MIS), so renders stay
+ * unbiased whatever the network has learned. */
+typedef struct pt_guide_config {
+    uint32_t log2_table;      /* hash-table entries per level = 2^log2_table, 10..24 (default 19) */
+    uint32_t n_min, n_max;    /* grid resolution of the coarsest / finest level (16, 2048) */
+    uint32_t seed;            /* parameter initialization */
+    float learning_rate;      /* Adam step size (1e-2) */
+    uint32_t batch_size;      /* samples per optimizer step, multiple of 64 (65536) */
+    uint32_t steps_per_frame; /* optimizer steps per frame rendered with PT_RENDER_GUIDE_TRAIN (1) */
+    uint32_t train_stride;    /* one camera path in train_stride records training data (32) */
+} pt_guide_config;
+
+typedef struct pt_guide_info {
+    uint32_t enabled;
+    uint32_t log2_table;
+    uint32_t steps;           /* applied optimizer steps */
+    uint32_t skipped;         /* steps dropped for non-finite gradients */
+    float loss_scale;         /* current dynamic loss scale */
+    float last_loss;          /* mean loss of the last step */
+    uint32_t last_samples;    /* samples in the last step */
+    uint32_t resolution[16];  /* grid resolution per level */
+    uint32_t dense[16];       /* 1 when the level is indexed densely */
+    float bounds_lo[3];       /* position normalization: u = (p - lo) * inv */
+    float bounds_inv[3];
+} pt_guide_info;
+
+/* Allocates and initializes the guide over the bounds of the current scene
+ * (config NULL = defaults).  Re-enabling re-initializes the parameters. */
+int pt_guide_enable(pt_ctx *ctx, const pt_guide_config *config);
+int pt_guide_disable(pt_ctx *ctx);
+int pt_guide_get_info(pt_ctx *ctx, pt_guide_info *out);
+/* Raw network outputs (41 floats per position) for n world-space positions
+ * (3 floats each). */
+int pt_guide_query(pt_ctx *ctx, const float *positions, uint32_t n, float *outputs);
+/* Training sample layout, 12 floats: position xyz, 0, direction xyz (unit),
+ * 0, target radiance (luminance), F/p (luminance of f L cos / pdf), BSDF pdf
+ * of the direction, valid (1 or 0).  fp_norm scales F/p in the selection loss. */
+/* One optimizer step on n samples (n a multiple of 64); *loss = mean loss. */
+int pt_guide_train(pt_ctx *ctx, const float *samples, uint32_t n, float fp_norm, float *loss);
+/* Gradients of the mean loss over n samples, without a step:
+ * table_grad [16][2^log2_table][2], weight_grad [4][64][64] ([in][out]). */
+int pt_guide_gradients(pt_ctx *ctx, const float *samples, uint32_t n, float fp_norm, float *table_grad,
+                       float *weight_grad, float *loss);
+/* FP32 master parameters, same layouts as pt_guide_gradients. */
+int pt_guide_get_params(pt_ctx *ctx, float *table, float *weights);
+int pt_guide_set_params(pt_ctx *ctx, const float *table, const float *weights);
 
 /* ---- host scene helpers (resource::Scene / World, headless) ----------- */
 typedef struct pt_host_scene pt_host_scene;

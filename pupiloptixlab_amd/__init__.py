@@ -26,6 +26,7 @@ PT_OK = 0
 STATUS = {
     0: "PT_OK", -1: "PT_ERR_INVALID_ARG", -2: "PT_ERR_HIP", -3: "PT_ERR_NO_SCENE", -4: "PT_ERR_CAPACITY",
     -5: "PT_ERR_IO", -6: "PT_ERR_PARSE", -7: "PT_ERR_OOM", -8: "PT_ERR_UNSUPPORTED", -9: "PT_ERR_NO_BVH",
+    -10: "PT_ERR_NO_GUIDE",
 }
 
 MAT_DIFFUSE, MAT_DIELECTRIC, MAT_ROUGH_DIELECTRIC, MAT_CONDUCTOR, MAT_ROUGH_CONDUCTOR, MAT_PLASTIC, \
@@ -35,7 +36,8 @@ SHAPE_FLIP_NORMALS, SHAPE_FLIP_TEXCOORDS, SHAPE_HAS_NORMALS, SHAPE_HAS_TEXCOORDS
     1, 2, 4, 8, 16
 ENV_NONE, ENV_CONSTANT = 0, 1
 MIS_REFERENCE, MIS_CONSISTENT = 0, 1
-RENDER_AOV, RENDER_FP64_ACCUM, RENDER_TIMING = 1, 2, 4
+RENDER_AOV, RENDER_FP64_ACCUM, RENDER_TIMING, RENDER_GUIDE, RENDER_GUIDE_TRAIN = 1, 2, 4, 8, 16
+GUIDE_OUTPUTS, GUIDE_LEVELS = 41, 16
 BUF_FRAME, BUF_ACCUM, BUF_ALBEDO, BUF_NORMAL, BUF_TEST, BUF_PRIM_ID, BUF_ACCUM64 = range(7)
 SCENE_CORNELL, SCENE_TERRAIN, SCENE_DOOR_ROOM, SCENE_FURNACE = 1, 2, 3, 4
 MISS = 0xFFFFFFFF
@@ -87,10 +89,31 @@ class Stats(C.Structure):
                 ("ms_raygen", C.c_double), ("ms_trace", C.c_double), ("ms_shade", C.c_double),
                 ("ms_shadow", C.c_double), ("ms_accumulate", C.c_double), ("launches_trace", C.c_uint64),
                 ("launches_shade", C.c_uint64), ("launches_shadow", C.c_uint64), ("bvh_nodes", C.c_uint32),
-                ("bvh_leaf_tris", C.c_uint32), ("bvh_depth", C.c_uint32), ("bvh_build_ms", C.c_double)]
+                ("bvh_leaf_tris", C.c_uint32), ("bvh_depth", C.c_uint32), ("bvh_build_ms", C.c_double),
+                ("ms_guide", C.c_double), ("guide_steps", C.c_uint64), ("guide_samples", C.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GuideConfig(C.Structure):
+    _fields_ = [("log2_table", C.c_uint32), ("n_min", C.c_uint32), ("n_max", C.c_uint32), ("seed", C.c_uint32),
+                ("learning_rate", C.c_float), ("batch_size", C.c_uint32), ("steps_per_frame", C.c_uint32),
+                ("train_stride", C.c_uint32)]
+
+
+class GuideInfo(C.Structure):
+    _fields_ = [("enabled", C.c_uint32), ("log2_table", C.c_uint32), ("steps", C.c_uint32),
+                ("skipped", C.c_uint32), ("loss_scale", C.c_float), ("last_loss", C.c_float),
+                ("last_samples", C.c_uint32), ("resolution", C.c_uint32 * 16), ("dense", C.c_uint32 * 16),
+                ("bounds_lo", C.c_float * 3), ("bounds_inv", C.c_float * 3)]
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if isinstance(v, C.Array) else v
+        return d
 
 
 class GenParams(C.Structure):
@@ -120,6 +143,15 @@ _SIGNATURES = {
     "pt_write_exr": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
     "pt_host_trace_rays": (C.c_int, [C.POINTER(SceneDesc), C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    "pt_guide_enable": (C.c_int, [C.c_void_p, C.POINTER(GuideConfig)]),
+    "pt_guide_disable": (C.c_int, [C.c_void_p]),
+    "pt_guide_get_info": (C.c_int, [C.c_void_p, C.POINTER(GuideInfo)]),
+    "pt_guide_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "pt_guide_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
+    "pt_guide_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_void_p, C.c_void_p,
+                                     C.POINTER(C.c_float)]),
+    "pt_guide_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pt_guide_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -280,8 +312,12 @@ class PathTracer:
 
     def render(self, spp: int = 1, max_depth: int = 0, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE,
                accumulate: bool = True, aov: bool = True, fp64: bool = False, timing: bool = False,
-               spp_per_launch: int = 0) -> None:
+               spp_per_launch: int = 0, guide: bool = False, guide_train: bool = False) -> None:
+        """Renders `spp` frames.  guide: sample with the learned proposal;
+        guide_train: train it on this render's paths (both need
+        guide_enable() and mis_mode=MIS_CONSISTENT)."""
         flags = (RENDER_AOV if aov else 0) | (RENDER_FP64_ACCUM if fp64 else 0) | (RENDER_TIMING if timing else 0)
+        flags |= (RENDER_GUIDE if guide else 0) | (RENDER_GUIDE_TRAIN if guide_train else 0)
         p = RenderParams(spp, max_depth, rr_depth, mis_mode, int(accumulate), flags, spp_per_launch, 0)
         _check(lib().pt_render(self._ctx, C.byref(p)), "pt_render", self._ctx)
 
@@ -317,3 +353,70 @@ class PathTracer:
         _check(lib().pt_trace_rays(self._ctx, rays.ctypes.data, n, int(any_hit), hits.ctypes.data),
                "pt_trace_rays", self._ctx)
         return hits[:, 0].view(np.float32), hits[:, 1].copy(), hits[:, 2].view(np.float32), hits[:, 3].view(np.float32)
+
+    # ---- learned guiding ---------------------------------------------------------
+    def guide_enable(self, log2_table: int = 19, n_min: int = 16, n_max: int = 2048, seed: int = 0,
+                     learning_rate: float = 1e-2, batch_size: int = 65536, steps_per_frame: int = 1,
+                     train_stride: int = 32) -> None:
+        cfg = GuideConfig(log2_table, n_min, n_max, seed, learning_rate, batch_size, steps_per_frame, train_stride)
+        _check(lib().pt_guide_enable(self._ctx, C.byref(cfg)), "pt_guide_enable", self._ctx)
+        self._guide_entries = 1 << log2_table
+
+    def guide_disable(self) -> None:
+        _check(lib().pt_guide_disable(self._ctx), "pt_guide_disable", self._ctx)
+
+    def guide_info(self) -> dict:
+        info = GuideInfo()
+        _check(lib().pt_guide_get_info(self._ctx, C.byref(info)), "pt_guide_get_info", self._ctx)
+        return info.as_dict()
+
+    def guide_query(self, positions: np.ndarray) -> np.ndarray:
+        """Raw network outputs (N x 41) at world-space positions (N x 3)."""
+        pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
+        out = np.zeros((pos.shape[0], GUIDE_OUTPUTS), dtype=np.float32)
+        _check(lib().pt_guide_query(self._ctx, pos.ctypes.data, pos.shape[0], out.ctypes.data), "pt_guide_query",
+               self._ctx)
+        return out
+
+    @staticmethod
+    def guide_samples(pos, wdir, target, fs_over_p, p_bsdf, valid=None) -> np.ndarray:
+        """Packs training samples into the N x 12 float layout of the C ABI."""
+        n = len(target)
+        s = np.zeros((n, 12), dtype=np.float32)
+        s[:, 0:3] = pos
+        s[:, 4:7] = wdir
+        s[:, 8] = target
+        s[:, 9] = fs_over_p
+        s[:, 10] = p_bsdf
+        s[:, 11] = 1.0 if valid is None else valid
+        return s
+
+    def guide_train(self, samples: np.ndarray, fp_norm: float = 1.0) -> float:
+        s = np.ascontiguousarray(samples, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().pt_guide_train(self._ctx, s.ctypes.data, s.shape[0], fp_norm, C.byref(loss)), "pt_guide_train",
+               self._ctx)
+        return loss.value
+
+    def guide_gradients(self, samples: np.ndarray, fp_norm: float = 1.0):
+        """(table_grad [16, T, 2], weight_grad [4, 64, 64], mean loss) without a step."""
+        s = np.ascontiguousarray(samples, dtype=np.float32)
+        tg = np.zeros((GUIDE_LEVELS, self._guide_entries, 2), dtype=np.float32)
+        wg = np.zeros((4, 64, 64), dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().pt_guide_gradients(self._ctx, s.ctypes.data, s.shape[0], fp_norm, tg.ctypes.data,
+                                        wg.ctypes.data, C.byref(loss)), "pt_guide_gradients", self._ctx)
+        return tg, wg, loss.value
+
+    def guide_get_params(self):
+        tb = np.zeros((GUIDE_LEVELS, self._guide_entries, 2), dtype=np.float32)
+        w = np.zeros((4, 64, 64), dtype=np.float32)
+        _check(lib().pt_guide_get_params(self._ctx, tb.ctypes.data, w.ctypes.data), "pt_guide_get_params",
+               self._ctx)
+        return tb, w
+
+    def guide_set_params(self, table=None, weights=None) -> None:
+        tb = None if table is None else np.ascontiguousarray(table, dtype=np.float32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        _check(lib().pt_guide_set_params(self._ctx, None if tb is None else tb.ctypes.data,
+                                         None if w is None else w.ctypes.data), "pt_guide_set_params", self._ctx)

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -20,13 +21,15 @@
 #include "../../include/pupil_hip.h"
 #include "pt_bvh.h"
 #include "pt_bvh_build.h"
+#include "pt_guide_state.h"
 #include "pt_shade.h"
+#include "pt_shade_guided.h"
 
 using namespace pupil;
 
 namespace {
 
-enum KernelKind { kKindRaygen = 0, kKindTrace, kKindShade, kKindShadow, kKindAccumulate };
+enum KernelKind { kKindRaygen = 0, kKindTrace, kKindShade, kKindShadow, kKindAccumulate, kKindGuide };
 
 constexpr int kTraceBlock = 128;
 constexpr int kLdsStack = 12;  // stack entries per lane kept in LDS
@@ -295,6 +298,103 @@ __global__ __launch_bounds__(256) void k_shade(SceneView scene, ShadeParams prm,
     }
 }
 
+// --- guided shading ----------------------------------------------------------------
+struct GuideShadeArgs {
+    GuideView G;
+    RecordSink rs;   // rs.rec null: no training records
+    uint32_t infer;  // sample with the network
+};
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// k_shade with the guided step: every wave that holds at least one
+// guidable vertex evaluates the network for its 64 lanes (one MFMA pass per
+// 16 queries) between the emission/roulette half and the scattering half.
+__global__ __launch_bounds__(256) void k_shade_guided(SceneView scene, ShadeParams prm, PathQueue in,
+                                                      const uint32_t *count_in, PathQueue out, uint32_t *count_out,
+                                                      ShadowQueue sq, uint32_t *count_shadow, float4 *out_l,
+                                                      AovBuffers aov, GuideShadeArgs ga) {
+    __shared__ float stage[4][64 * kGuideUsed];
+    float *st = stage[threadIdx.x >> 6];
+    const uint32_t lane = lane_id();
+    const uint32_t count = *count_in;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < count;
+        PathState ps;
+        ShadeOut so;
+        so.cont = so.done = so.shadow = so.wrote_aov = false;
+        GuidedVertex v;
+        v.guidable = false;
+        uint32_t path = 0, prim = kMissPrim;
+        bool scatter = false;
+        if (valid) {
+            const float4 r0 = in.ray0[i], r1 = in.ray1[i], tl = in.tl[i], lr = in.lr[i], h = in.hit[i];
+            ps.o = f3of(r0);
+            path = f2u(r0.w);
+            ps.d = f3of(r1);
+            ps.prev_pdf = r1.w;
+            ps.T = f3of(tl);
+            ps.L = mk3(tl.w, lr.x, lr.y);
+            ps.rng.seed = f2u(lr.z);
+            ps.flags = f2u(lr.w);
+            prim = f2u(h.y);
+            scatter = guided_begin(scene, prm, ps, prim, h.z, h.w, ga.rs, path, v, so);
+        }
+        bool use_guide = ga.infer != 0u && scatter && v.guidable;
+        GuideMixture m;
+        if (__ballot(use_guide) != 0ull) {
+            guide_infer_wave(ga.G, use_guide ? v.g.position : mk3(0.f), st);
+            wave_lds_sync();
+            if (use_guide) {
+                float o[kGuideUsed];
+#pragma unroll
+                for (int n = 0; n < kGuideUsed; ++n) o[n] = st[lane * kGuideUsed + n];
+                guide_decode(o, m);
+                if (!(m.c >= 0.f && m.c <= 1.f) || !(m.wsum > 0.f) || !isfinite(m.wsum)) use_guide = false;
+            }
+            wave_lds_sync();
+        }
+        if (scatter) guided_scatter(scene, ps, v, use_guide, m, ga.rs, path, so);
+        const uint32_t j = wave_alloc(valid && so.cont, count_out);
+        const uint32_t k = wave_alloc(valid && so.shadow, count_shadow);
+        if (!valid) continue;
+        if (so.wrote_aov && aov.albedo) {
+            const uint32_t f = path / aov.npix;
+            if (f == aov.last_frame) {
+                const uint32_t px = path - f * aov.npix;
+                aov.albedo[3 * px + 0] = so.albedo.x;
+                aov.albedo[3 * px + 1] = so.albedo.y;
+                aov.albedo[3 * px + 2] = so.albedo.z;
+                aov.normal[3 * px + 0] = so.normal.x;
+                aov.normal[3 * px + 1] = so.normal.y;
+                aov.normal[3 * px + 2] = so.normal.z;
+                aov.test[px] = so.test;
+                aov.prim[px] = prim;
+            }
+        }
+        if (so.cont) {
+            out.ray0[j] = make_float4(ps.o.x, ps.o.y, ps.o.z, u2f(path));
+            out.ray1[j] = make_float4(ps.d.x, ps.d.y, ps.d.z, ps.prev_pdf);
+            out.tl[j] = make_float4(ps.T.x, ps.T.y, ps.T.z, ps.L.x);
+            out.lr[j] = make_float4(ps.L.y, ps.L.z, u2f(ps.rng.seed), u2f(ps.flags));
+        } else {
+            out_l[path] = make_float4(ps.L.x, ps.L.y, ps.L.z, 1.f);
+        }
+        if (so.shadow) {
+            const uint32_t target = so.cont ? j : (0x80000000u | path);
+            sq.s0[k] = make_float4(so.so.x, so.so.y, so.so.z, so.stmax);
+            sq.s1[k] = make_float4(so.sd.x, so.sd.y, so.sd.z, u2f(target));
+            sq.s2[k] = make_float4(so.contrib.x, so.contrib.y, so.contrib.z, 0.f);
+        }
+    }
+}
+
 // --- accumulation ---------------------------------------------------------------
 // main.cu:190-196: running mean acc = lerp(acc, L, 1 / (n + 1)) per frame.
 __global__ void k_accumulate(const float4 *out_l, float4 *accum, float4 *frame, double *accum64, uint32_t npix,
@@ -429,6 +529,10 @@ struct pt_ctx {
     bool accum64_valid = false;
 
     pt_stats stats{};
+
+    // learned guiding (pt_guide_*)
+    float scene_lo[3]{}, scene_hi[3]{};
+    GuideState guide;
 };
 
 namespace {
@@ -634,6 +738,7 @@ void collect_event_times(pt_ctx *ctx) {
             case kKindTrace: ctx->stats.ms_trace += ms; ctx->stats.launches_trace++; break;
             case kKindShade: ctx->stats.ms_shade += ms; ctx->stats.launches_shade++; break;
             case kKindShadow: ctx->stats.ms_shadow += ms; ctx->stats.launches_shadow++; break;
+            case kKindGuide: ctx->stats.ms_guide += ms; break;
             default: ctx->stats.ms_accumulate += ms; break;
         }
     }
@@ -690,6 +795,7 @@ void pt_destroy(pt_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->guide.release();
     free_scene(ctx);
     free_queues(ctx);
     dfree(ctx->d_spill);
@@ -736,6 +842,8 @@ int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
         }
     (void)hipStreamSynchronize(ctx->stream);
     free_scene(ctx);
+    // The guide's position normalization follows the scene bounds.
+    ctx->guide.release();
 
     ctx->h_pos4.assign((size_t)nv * 4, 0.f);
     ctx->h_nrm4.assign(s->normals ? (size_t)nv * 4 : 0, 0.f);
@@ -747,6 +855,15 @@ int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
         if (s->texcoords)
             for (int k = 0; k < 2; ++k) ctx->h_uv2[2 * i + k] = s->texcoords[2 * i + k];
     }
+    for (int k = 0; k < 3; ++k) {
+        ctx->scene_lo[k] = nv ? s->positions[k] : 0.f;
+        ctx->scene_hi[k] = ctx->scene_lo[k];
+    }
+    for (uint32_t i = 0; i < nv; ++i)
+        for (int k = 0; k < 3; ++k) {
+            ctx->scene_lo[k] = std::min(ctx->scene_lo[k], s->positions[3 * i + k]);
+            ctx->scene_hi[k] = std::max(ctx->scene_hi[k], s->positions[3 * i + k]);
+        }
     ctx->h_tri4.assign((size_t)nt * 4, 0u);
     ctx->h_shapes.resize(ns);
     // Area emitters: one per triangle of each emitting shape, in shape order
@@ -948,6 +1065,17 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
         ctx->error = "pt_render: unknown MIS mode";
         return PT_ERR_INVALID_ARG;
     }
+    const bool guide_infer = (p->flags & PT_RENDER_GUIDE) != 0;
+    const bool guide_train = (p->flags & PT_RENDER_GUIDE_TRAIN) != 0;
+    const bool guided = guide_infer || guide_train;
+    if (guided && !ctx->guide.enabled) {
+        ctx->error = "pt_render: guided rendering needs pt_guide_enable";
+        return PT_ERR_NO_GUIDE;
+    }
+    if (guided && p->mis_mode != PT_MIS_CONSISTENT) {
+        ctx->error = "pt_render: guided rendering needs PT_MIS_CONSISTENT";
+        return PT_ERR_INVALID_ARG;
+    }
     (void)hipSetDevice(ctx->device);
     const auto wall0 = std::chrono::steady_clock::now();
     const uint32_t npix = ctx->width * ctx->height;
@@ -959,7 +1087,12 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
     uint32_t per_launch = p->spp_per_launch;
     // Default launch width: ~64M paths in flight (about 15 GB of queues), which
     // amortizes the latency-bound tail of the last bounces over many frames.
-    if (per_launch == 0) per_launch = std::max(1u, std::min(64u, (64u * 1024u * 1024u) / std::max(1u, npix)));
+    // While training the guide, launches are kept short (~16M paths) so the
+    // network is updated every few frames.
+    if (per_launch == 0) {
+        const uint32_t paths = guide_train ? 16u * 1024u * 1024u : 64u * 1024u * 1024u;
+        per_launch = std::max(1u, std::min(64u, paths / std::max(1u, npix)));
+    }
     per_launch = std::min(per_launch, std::max(1u, p->spp));
     if ((uint64_t)per_launch * npix > 0x7FFFFFFFull) {
         ctx->error = "pt_render: spp_per_launch * pixels exceeds the queue index range";
@@ -987,6 +1120,25 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
         ctx->count_hist_slots = hist_elems;
     }
 
+    GuideShadeArgs ga;
+    std::memset(&ga, 0, sizeof(ga));
+    if (guided) {
+        ga.G = ctx->guide.view();
+        ga.infer = guide_infer ? 1u : 0u;
+    }
+    if (guide_train) {
+        const pt_guide_config &gc = ctx->guide.cfg;
+        const uint64_t slots = ((uint64_t)per_launch * npix + gc.train_stride - 1) / gc.train_stride;
+        const uint64_t records = slots * max_depth;
+        if (records > 0xFFFFFFFFull / 2) {
+            ctx->error = "pt_render: too many guide training records per launch (raise train_stride)";
+            return PT_ERR_CAPACITY;
+        }
+        if (!ctx->guide.ensure_records((uint32_t)records, ctx->error) ||
+            !ctx->guide.ensure_samples(gc.steps_per_frame * per_launch * gc.batch_size, ctx->error))
+            return PT_ERR_HIP;
+    }
+
     uint32_t done = 0, batch = 0;
     uint32_t sample_cnt = ctx->sample_cnt, seed = ctx->random_seed;
     while (done < p->spp) {
@@ -1010,6 +1162,16 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
             aov.test = ctx->d_test;
             aov.prim = ctx->d_prim;
         }
+        if (guide_train) {
+            GuideState &gs = ctx->guide;
+            gs.epoch += 1u;
+            ga.rs.rec = reinterpret_cast<f4 *>(gs.rec);
+            ga.rs.capacity = gs.rec_cap;
+            ga.rs.stride = gs.cfg.train_stride;
+            ga.rs.phase = gs.phase;
+            ga.rs.max_depth = max_depth;
+            ga.rs.epoch = gs.epoch;
+        }
         int cur = 0;
         for (uint32_t b = 0; b < max_depth; ++b) {
             const int nxt = cur ^ 1;
@@ -1025,8 +1187,12 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
             }
             {
                 EventTimer t(ctx, timing);
-                hipLaunchKernelGGL(k_shade, dim3(sg), dim3(256), 0, ctx->stream, sv, sp, ctx->q[cur], ext + b,
-                                   ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov);
+                if (guided)
+                    hipLaunchKernelGGL(k_shade_guided, dim3(sg), dim3(256), 0, ctx->stream, sv, sp, ctx->q[cur],
+                                       ext + b, ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov, ga);
+                else
+                    hipLaunchKernelGGL(k_shade, dim3(sg), dim3(256), 0, ctx->stream, sv, sp, ctx->q[cur], ext + b,
+                                       ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov);
                 t.stop(kKindShade);
             }
             {
@@ -1049,6 +1215,19 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
                                sample_cnt, p->accumulate);
             t.stop(kKindAccumulate);
         }
+        if (guide_train) {
+            // Training samples from this launch's paths, then the optimizer
+            // steps; the next launch samples with the updated network.
+            EventTimer t(ctx, timing);
+            GuideState &gs = ctx->guide;
+            GuideRecords r{gs.rec, gs.rec_cap, ga.rs.stride, ga.rs.phase, max_depth, gs.epoch};
+            if (!gs.collect(ctx->d_out_l, total, r, ctx->error) ||
+                !gs.train(gs.cfg.steps_per_frame * frames, 0.f, ctx->error))
+                return PT_ERR_HIP;
+            gs.phase = (gs.phase + 1u) % gs.cfg.train_stride;
+            ctx->stats.guide_steps += (uint64_t)gs.cfg.steps_per_frame * frames;
+            t.stop(kKindGuide);
+        }
         PT_HIP(hipGetLastError());
         // Keep this launch's queue counters; read back once after the last launch.
         PT_HIP(hipMemcpyAsync(ctx->d_count_hist + (size_t)batch * ctx->num_counters, ctx->d_counters,
@@ -1068,6 +1247,11 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
             ctx->stats.rays_extension += c[b];
             ctx->stats.rays_shadow += c[max_depth + 2 + b];
         }
+    }
+    if (guide_train) {
+        unsigned long long kept = 0;
+        PT_HIP(hipMemcpy(&kept, &ctx->guide.ctl->total_samples, sizeof(kept), hipMemcpyDeviceToHost));
+        ctx->stats.guide_samples = kept;
     }
     ctx->sample_cnt = sample_cnt;
     ctx->random_seed = seed;
@@ -1207,4 +1391,120 @@ int pt_trace_rays(pt_ctx *ctx, const float *rays, uint32_t count, int any_hit, v
     return rc;
 }
 
+// ---- learned guiding ----------------------------------------------------------
+
+namespace {
+
+int guide_ready(pt_ctx *ctx, const char *who) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    if (!ctx->guide.enabled) {
+        ctx->error = std::string(who) + ": guide not enabled";
+        return PT_ERR_NO_GUIDE;
+    }
+    (void)hipSetDevice(ctx->device);
+    return PT_OK;
+}
+
+}  // namespace
+
+int pt_guide_enable(pt_ctx *ctx, const pt_guide_config *config) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    if (!ctx->has_scene) {
+        ctx->error = "pt_guide_enable: no scene";
+        return PT_ERR_NO_SCENE;
+    }
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    pt_guide_config c;
+    std::memset(&c, 0, sizeof(c));
+    if (config) c = *config;
+    if (!ctx->guide.init(c, ctx->scene_lo, ctx->scene_hi, ctx->stream, ctx->error)) {
+        ctx->guide.release();
+        return ctx->error.rfind("pt_guide_enable", 0) == 0 ? PT_ERR_INVALID_ARG : PT_ERR_HIP;
+    }
+    return PT_OK;
+}
+
+int pt_guide_disable(pt_ctx *ctx) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    ctx->guide.release();
+    return PT_OK;
+}
+
+int pt_guide_get_info(pt_ctx *ctx, pt_guide_info *out) {
+    if (!ctx || !out) return PT_ERR_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    return ctx->guide.info(*out, ctx->error) ? PT_OK : PT_ERR_HIP;
+}
+
+int pt_guide_query(pt_ctx *ctx, const float *positions, uint32_t n, float *outputs) {
+    int rc = guide_ready(ctx, "pt_guide_query");
+    if (rc != PT_OK) return rc;
+    if (n && (!positions || !outputs)) {
+        ctx->error = "pt_guide_query: null buffer";
+        return PT_ERR_INVALID_ARG;
+    }
+    return ctx->guide.query(positions, n, outputs, ctx->error) ? PT_OK : PT_ERR_HIP;
+}
+
+static int guide_check_samples(pt_ctx *ctx, const char *who, const float *samples, uint32_t n) {
+    if (!samples || n == 0 || n % 64u != 0u || n > (1u << 22)) {
+        ctx->error = std::string(who) + ": samples must be a positive multiple of 64 (at most 2^22)";
+        return PT_ERR_INVALID_ARG;
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+        const float *v = samples + 12 * i;
+        for (int k = 0; k < 12; ++k)
+            if (!std::isfinite(v[k])) {
+                ctx->error = std::string(who) + ": non-finite sample value";
+                return PT_ERR_INVALID_ARG;
+            }
+    }
+    return PT_OK;
+}
+
+int pt_guide_train(pt_ctx *ctx, const float *samples, uint32_t n, float fp_norm, float *loss) {
+    int rc = guide_ready(ctx, "pt_guide_train");
+    if (rc != PT_OK) return rc;
+    if ((rc = guide_check_samples(ctx, "pt_guide_train", samples, n)) != PT_OK) return rc;
+    GuideState &g = ctx->guide;
+    const uint32_t saved = g.cfg.batch_size;
+    g.cfg.batch_size = n;
+    bool ok = g.upload_samples(samples, n, ctx->error) && g.train(1, fp_norm > 0.f ? fp_norm : 1.f, ctx->error);
+    g.cfg.batch_size = saved;
+    if (!ok) return PT_ERR_HIP;
+    ctx->stats.guide_steps += 1;
+    pt_guide_info info;
+    if (!g.info(info, ctx->error)) return PT_ERR_HIP;
+    if (loss) *loss = info.last_loss;
+    return PT_OK;
+}
+
+int pt_guide_gradients(pt_ctx *ctx, const float *samples, uint32_t n, float fp_norm, float *table_grad,
+                       float *weight_grad, float *loss) {
+    int rc = guide_ready(ctx, "pt_guide_gradients");
+    if (rc != PT_OK) return rc;
+    if ((rc = guide_check_samples(ctx, "pt_guide_gradients", samples, n)) != PT_OK) return rc;
+    GuideState &g = ctx->guide;
+    if (!g.upload_samples(samples, n, ctx->error)) return PT_ERR_HIP;
+    if (!g.gradients(n, fp_norm > 0.f ? fp_norm : 1.f, table_grad, weight_grad, loss, ctx->error))
+        return ctx->error.find("non-finite") != std::string::npos ? PT_ERR_INVALID_ARG : PT_ERR_HIP;
+    return PT_OK;
+}
+
+int pt_guide_get_params(pt_ctx *ctx, float *table, float *weights) {
+    int rc = guide_ready(ctx, "pt_guide_get_params");
+    if (rc != PT_OK) return rc;
+    return ctx->guide.get_params(table, weights, ctx->error) ? PT_OK : PT_ERR_HIP;
+}
+
+int pt_guide_set_params(pt_ctx *ctx, const float *table, const float *weights) {
+    int rc = guide_ready(ctx, "pt_guide_set_params");
+    if (rc != PT_OK) return rc;
+    return ctx->guide.set_params(table, weights, ctx->error) ? PT_OK : PT_ERR_HIP;
+}
+
 }  // extern "C"
+

@@ -1,0 +1,915 @@
+// Online training of the guiding network on gfx950.
+//
+// One optimizer step over a batch of samples is a chain of kernels on the
+// tracer's stream, all driven by the device control block (GuideCtl):
+//   k_guide_begin  : batch size of this step from the collected sample count
+//   k_guide_train  : per wave, 64 samples: hash encoding, MLP forward (MFMA),
+//                    loss and output gradient, MLP backward (MFMA), hash-table
+//                    gradient scatter; stores every layer's input and output
+//                    gradient (FP16) for the weight gradient
+//   k_guide_dw     : dW_l = X_l^T dY_l per batch slice (MFMA, FP32 partials)
+//   k_guide_mid    : Adam step count and bias corrections, or skip
+//   k_guide_adam_* : Adam on the FP32 masters; refreshes the FP16 table
+//   k_guide_end    : dynamic loss scale, loss bookkeeping
+//   k_guide_pack   : FP32 weights -> pre-permuted FP16 MFMA operands
+// The backward pass runs in FP16 with FP32 accumulation; output gradients are
+// multiplied by a dynamic loss scale, and a step whose gradients are not
+// finite is dropped and the scale halved.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pt_guide_state.h"
+
+using namespace pupil;
+
+namespace {
+
+constexpr int kTrainBlock = 256;  // 4 waves
+
+__device__ __forceinline__ uint16_t hbits(_Float16 h) {
+    union {
+        _Float16 h;
+        uint16_t u;
+    } c;
+    c.h = h;
+    return c.u;
+}
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ void k_guide_begin(GuideCtl *ctl, uint32_t first, uint32_t batch, uint32_t capacity, float fp_explicit) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t n = min(ctl->n_samples, capacity);
+    const uint32_t valid = n > first ? min(batch, n - first) : 0u;
+    ctl->batch_valid = valid;
+    ctl->active = valid > 0u ? 1u : 0u;
+    ctl->loss_sum = 0.f;
+    ctl->loss_cnt = 0.f;
+    float norm = fp_explicit;
+    if (!(norm > 0.f)) norm = (n > 0u && ctl->fp_sum > 0.f) ? ctl->fp_sum / (float)n : 1.f;
+    ctl->fp_norm = norm;
+}
+
+// Loss of one sample and its gradient with respect to the 41 outputs
+// (clipped to kGuideGradClip in max norm).  Returns the loss.
+__device__ float sample_loss_grad(const float *o, f3 w, float target, float fs, float pb, float fp_norm,
+                                  float *go) {
+    GuideMixture m;
+    guide_decode(o, m);
+    float lhat = 0.f, v[kGuideLobes], cosang[kGuideLobes];
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) {
+        cosang[k] = dot(m.mu[k], w);
+        v[k] = vmf_pdf(m.kappa[k], cosang[k]);
+        lhat += m.weight[k] * v[k];
+    }
+    const float q = lhat / m.wsum;
+    const float den = lhat * lhat + kRelEps;
+    float loss = (lhat - target) * (lhat - target) / den;
+    const float dl = 2.f * (lhat - target) / den;
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) {
+        const float wk = m.weight[k], kk = m.kappa[k], vk = v[k], ck = cosang[k];
+        const float lk = o[5 * k + 3], lw = o[5 * k + 4];
+        go[5 * k + 4] = (lw > kWeightMinLog && lw < kWeightMaxLog) ? dl * vk * wk : 0.f;
+        const float dlogv = 1.f / kk - 2.f / expm1f(2.f * kk) + ck - 1.f;
+        go[5 * k + 3] = (lk > kKappaMinLog && lk < kKappaMaxLog) ? dl * wk * vk * dlogv * kk : 0.f;
+        const f3 raw = mk3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
+        const float nr = sqrtf(dot(raw, raw));
+        f3 dr = mk3(0.f);
+        if (nr > 1e-12f) dr = (w - ck * m.mu[k]) * (dl * wk * vk * kk / nr);
+        go[5 * k] = dr.x;
+        go[5 * k + 1] = dr.y;
+        go[5 * k + 2] = dr.z;
+    }
+    const float c = m.c;
+    const float pmix = fmaxf(c * q + (1.f - c) * pb, 1e-30f);
+    const float wgt = fminf(fs / fp_norm, 10.f);
+    loss += -wgt * logf(pmix);
+    const float sig = c / kSelectMax;
+    go[40] = -wgt * (q - pb) / pmix * kSelectMax * sig * (1.f - sig);
+    float mx = 0.f;
+#pragma unroll
+    for (int n = 0; n < kGuideUsed; ++n) mx = fmaxf(mx, fabsf(go[n]));
+    if (mx > kGuideGradClip) {
+        const float s = kGuideGradClip / mx;
+#pragma unroll
+        for (int n = 0; n < kGuideUsed; ++n) go[n] *= s;
+    }
+    return loss;
+}
+
+struct TrainBuffers {
+    uint16_t *xt;       // [layer][k 64][batch] layer inputs (FP16 bits)
+    uint16_t *dyt;      // [layer][n 64][batch] scaled layer output gradients (FP16 bits)
+    float2 *table_grad; // [level][T]
+};
+
+// One wave per 64 samples.  samples: 3 float4 per sample (position,
+// direction, {target, F/p, BSDF pdf, valid}).
+__global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideView G, const float4 *samples, uint32_t batch,
+                                                             GuideCtl *ctl, TrainBuffers B) {
+    __shared__ float stage[kTrainBlock / 64][64 * kGuideUsed];
+    if (!ctl->active) return;
+    const uint32_t nvalid = ctl->batch_valid;
+    const int wave = threadIdx.x >> 6;
+    const int lane = (int)__lane_id();
+    const int g = lane >> 4, i = lane & 15;
+    const uint32_t base = (blockIdx.x * (kTrainBlock / 64) + wave) * 64u;
+    if (base >= nvalid) return;  // wave-uniform; no block barriers below
+    float *st = stage[wave];
+    const float scale = ctl->loss_scale;
+
+    const uint32_t me = base + lane;
+    const bool my_valid = me < nvalid && samples[3 * (size_t)me + 2].w != 0.f;
+    const float4 p4 = samples[3 * (size_t)me];
+    const f3 u = guide_normalize_pos(G, mk3(p4.x, p4.y, p4.z));
+
+    // Forward: encode once (kept for the recomputation in the backward pass),
+    // stage the outputs in LDS.
+    half8 in0[4];
+#pragma unroll
+    for (int qt = 0; qt < 4; ++qt) {
+        const int q = qt * 16 + i;
+        const f3 uq = mk3(__shfl(u.x, q), __shfl(u.y, q), __shfl(u.z, q));
+        in0[qt] = guide_encode(G, uq, g);
+        floatx4 out[4];
+        guide_mlp_tile(G, lane, in0[qt], out, nullptr);
+#pragma unroll
+        for (int rt = 0; rt < 3; ++rt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int n = rt * 16 + 4 * g + j;
+                if (n < kGuideUsed) st[q * kGuideUsed + n] = out[rt][j];
+            }
+    }
+    wave_sync_lds();
+
+    // Loss and output gradient of this lane's sample.
+    {
+        float o[kGuideUsed], go[kGuideUsed];
+#pragma unroll
+        for (int n = 0; n < kGuideUsed; ++n) {
+            o[n] = st[lane * kGuideUsed + n];
+            go[n] = 0.f;
+        }
+        float loss = 0.f;
+        if (my_valid) {
+            const float4 d4 = samples[3 * (size_t)me + 1], aux = samples[3 * (size_t)me + 2];
+            loss = sample_loss_grad(o, mk3(d4.x, d4.y, d4.z), aux.x, aux.y, aux.z, ctl->fp_norm, go);
+        }
+        wave_sync_lds();
+#pragma unroll
+        for (int n = 0; n < kGuideUsed; ++n) st[lane * kGuideUsed + n] = go[n] * scale;
+        float ls = loss, vc = my_valid ? 1.f : 0.f;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            ls += __shfl_xor(ls, off);
+            vc += __shfl_xor(vc, off);
+        }
+        if (lane == 0) {
+            unsafeAtomicAdd(&ctl->loss_sum, ls);
+            unsafeAtomicAdd(&ctl->loss_cnt, vc);
+        }
+    }
+    wave_sync_lds();
+
+    // Backward, tile by tile; the tile's layer inputs are recomputed from
+    // its encoding (MFMA work is cheap next to keeping them in registers).
+    // Gradients are summed over the batch here; the Adam kernels divide by
+    // the number of valid samples.
+    const float inv_scale = 1.f / scale;
+    bool bad = false;
+#pragma unroll
+    for (int qt = 0; qt < 4; ++qt) {
+        const int q = qt * 16 + i;
+        const uint32_t qg = base + q;
+        half8 act[4][2];
+        {
+            floatx4 out[4];
+            guide_mlp_tile(G, lane, in0[qt], out, act);
+        }
+        half8 dy[2];
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const int n = guide_perm(c, g, s);
+                dy[c][s] = (_Float16)(n < kGuideUsed ? st[q * kGuideUsed + n] : 0.f);
+            }
+        const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int l = 3; l >= 0; --l) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+#pragma unroll
+                for (int s = 0; s < 8; ++s)
+                    B.dyt[((size_t)l * 64 + guide_perm(c, g, s)) * batch + qg] = hbits(dy[c][s]);
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                if (l == 0 && c == 1) break;
+#pragma unroll
+                for (int s = 0; s < 8; ++s)
+                    B.xt[((size_t)l * 64 + guide_perm(c, g, s)) * batch + qg] = hbits(act[l][c][s]);
+            }
+            // dX_l^T = W_l dY_l^T
+            floatx4 dx[4];
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) {
+                if (l == 0 && kt >= 2) break;
+                dx[kt] = mfma16(bwd_operand(G, l, kt, 0, lane), dy[0], zero);
+                dx[kt] = mfma16(bwd_operand(G, l, kt, 1, lane), dy[1], dx[kt]);
+            }
+            if (l > 0) {
+                // ReLU of the layer input (the previous layer's output).
+#pragma unroll
+                for (int c = 0; c < 2; ++c)
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) {
+                        const float gv = dx[2 * c + (s >> 2)][s & 3];
+                        dy[c][s] = (_Float16)((float)act[l][c][s] > 0.f ? gv : 0.f);
+                    }
+            } else {
+                // Hash-table gradient of this lane's four levels of query q.
+                const f3 uq = mk3(__shfl(u.x, q), __shfl(u.y, q), __shfl(u.z, q));
+                const bool qvalid = __shfl((int)my_valid, q) != 0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const float g0 = dx[t >> 1][2 * (t & 1)] * inv_scale;
+                    const float g1 = dx[t >> 1][2 * (t & 1) + 1] * inv_scale;
+                    if (!qvalid || (g0 == 0.f && g1 == 0.f)) continue;
+                    if (!isfinite(g0) || !isfinite(g1)) {
+                        bad = true;
+                        continue;
+                    }
+                    const int level = guide_lane_level(g, t);
+                    LevelCorners lc;
+                    level_corners(G, level, uq, lc);
+                    float2 *tg = B.table_grad + ((size_t)level << G.log2t);
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        unsafeAtomicAdd(&tg[lc.idx[k]].x, lc.w[k] * g0);
+                        unsafeAtomicAdd(&tg[lc.idx[k]].y, lc.w[k] * g1);
+                    }
+                }
+            }
+        }
+    }
+    if (bad) atomicOr(&ctl->bad, 1u);
+}
+
+// Partial dW_l[k][n] = sum_q X_l[q][k] dY_l[q][n] over one batch slice; one
+// wave per (slice, layer).  A[i][8g+s] = X_l[q + 8g + s][16 kt + i],
+// B[8g+s][i] = dY_l[q + 8g + s][16 nt + i], D[4g+j][i] = dW[16 kt + 4g + j][16 nt + i].
+__global__ __launch_bounds__(64) void k_guide_dw(const uint16_t *xt, const uint16_t *dyt, uint32_t batch,
+                                                 GuideCtl *ctl, float *w_part) {
+    if (!ctl->active) return;
+    const uint32_t rows = (ctl->batch_valid + 63u) & ~63u;
+    const uint32_t q0 = blockIdx.x * kGuideDwSlice;
+    if (q0 >= rows) return;
+    const uint32_t q1 = min(rows, q0 + kGuideDwSlice);
+    const int layer = blockIdx.y;
+    const int lane = (int)__lane_id();
+    const int g = lane >> 4, i = lane & 15;
+    const int ktiles = layer == 0 ? 2 : 4;
+    floatx4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const uint16_t *X = xt + (size_t)layer * 64 * batch;
+    const uint16_t *Y = dyt + (size_t)layer * 64 * batch;
+    for (uint32_t q = q0; q < q1; q += 32) {
+        half8 bv[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+            bv[nt] = *reinterpret_cast<const half8 *>(Y + (size_t)(nt * 16 + i) * batch + q + 8 * g);
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) {
+            if (kt >= ktiles) break;
+            const half8 av = *reinterpret_cast<const half8 *>(X + (size_t)(kt * 16 + i) * batch + q + 8 * g);
+#pragma unroll
+            for (int nt = 0; nt < 4; ++nt) acc[kt][nt] = mfma16(av, bv[nt], acc[kt][nt]);
+        }
+    }
+    const float s = 1.f / ctl->loss_scale;
+    float *W = w_part + ((size_t)blockIdx.x * 4 + layer) * 64 * 64;
+    bool nonfinite = false;
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt)
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float v = kt < ktiles ? acc[kt][nt][j] * s : 0.f;
+                if (!isfinite(v)) nonfinite = true;
+                W[(kt * 16 + 4 * g + j) * 64 + nt * 16 + i] = v;
+            }
+    if (nonfinite) atomicOr(&ctl->bad, 1u);
+}
+
+__global__ void k_guide_mid(GuideCtl *ctl, float b1, float b2) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!ctl->active || ctl->bad) return;
+    const uint32_t t = ctl->step + 1u;
+    ctl->step = t;
+    ctl->c1 = 1.f - powf(b1, (float)t);
+    ctl->c2 = 1.f - powf(b2, (float)t);
+}
+
+struct AdamParams {
+    float lr, b1, b2, eps;
+};
+
+__device__ __forceinline__ float inv_valid(const GuideCtl *ctl) {
+    return ctl->loss_cnt > 0.f ? 1.f / ctl->loss_cnt : 0.f;
+}
+
+// Adam on the hash table (FP32 master + FP16 copy) with the batch-summed
+// gradient divided by the valid sample count; gradients are cleared.
+// Entries with no gradient and no momentum are left as they are (their
+// update is zero).
+__global__ void k_guide_adam_table(float2 *p, float2 *m, float2 *v, float2 *g, uint32_t *p16, size_t n, AdamParams a,
+                                   const GuideCtl *ctl, uint32_t apply) {
+    if (!ctl->active) return;
+    const bool step = apply && !ctl->bad;
+    const float c1 = ctl->c1, c2 = ctl->c2;
+    const float inv = inv_valid(ctl);
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        float2 gg = g[e];
+        const float2 mm0 = m[e];
+        if (gg.x == 0.f && gg.y == 0.f && mm0.x == 0.f && mm0.y == 0.f) continue;
+        g[e] = make_float2(0.f, 0.f);
+        if (!step) continue;
+        gg.x *= inv;
+        gg.y *= inv;
+        float2 mm = mm0, vv = v[e], pp = p[e];
+        mm.x = a.b1 * mm.x + (1.f - a.b1) * gg.x;
+        mm.y = a.b1 * mm.y + (1.f - a.b1) * gg.y;
+        vv.x = a.b2 * vv.x + (1.f - a.b2) * gg.x * gg.x;
+        vv.y = a.b2 * vv.y + (1.f - a.b2) * gg.y * gg.y;
+        pp.x -= a.lr * (mm.x / c1) / (sqrtf(vv.x / c2) + a.eps);
+        pp.y -= a.lr * (mm.y / c1) / (sqrtf(vv.y / c2) + a.eps);
+        m[e] = mm;
+        v[e] = vv;
+        p[e] = pp;
+        p16[e] = (uint32_t)hbits((_Float16)pp.x) | ((uint32_t)hbits((_Float16)pp.y) << 16);
+    }
+}
+
+// Adam on the MLP weights; the gradient is the sum of the slice partials
+// divided by the valid sample count.
+__global__ void k_guide_adam_weights(float *p, float *m, float *v, const float *part, AdamParams a,
+                                     const GuideCtl *ctl, uint32_t apply, float *grad_out) {
+    if (!ctl->active) return;
+    const uint32_t rows = (ctl->batch_valid + 63u) & ~63u;
+    const uint32_t slices = (rows + kGuideDwSlice - 1) / kGuideDwSlice;
+    const bool step = apply && !ctl->bad;
+    const float c1 = ctl->c1, c2 = ctl->c2;
+    const float inv = inv_valid(ctl);
+    const uint32_t n = 4 * 64 * 64;
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+        float gg = 0.f;
+        for (uint32_t s = 0; s < slices; ++s) gg += part[(size_t)s * n + e];
+        gg *= inv;
+        if (grad_out) grad_out[e] = gg;
+        if (!step) continue;
+        const float mm = a.b1 * m[e] + (1.f - a.b1) * gg;
+        const float vv = a.b2 * v[e] + (1.f - a.b2) * gg * gg;
+        m[e] = mm;
+        v[e] = vv;
+        p[e] -= a.lr * (mm / c1) / (sqrtf(vv / c2) + a.eps);
+    }
+}
+
+__global__ void k_guide_end(GuideCtl *ctl) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (ctl->active) {
+        if (ctl->bad) {
+            ctl->loss_scale = fmaxf(1.f, ctl->loss_scale * 0.5f);
+            ctl->good = 0u;
+            ctl->skipped += 1u;
+        } else if (++ctl->good >= kGuideScaleGrowth) {
+            ctl->loss_scale = fminf(kGuideScaleMax, ctl->loss_scale * 2.f);
+            ctl->good = 0u;
+        }
+        ctl->last_loss = ctl->loss_cnt > 0.f ? ctl->loss_sum / ctl->loss_cnt : 0.f;
+        ctl->last_count = ctl->batch_valid;
+    }
+    ctl->bad = 0u;
+    ctl->active = 0u;
+}
+
+// FP32 weights [layer][in 64][out 64] -> FP16 operands
+// fwd[layer][rt][chunk][lane][s] = W[perm(chunk, g, s)][16 rt + i]
+// bwd[layer][kt][chunk][lane][s] = W[16 kt + i][perm(chunk, g, s)]
+__global__ void k_guide_pack(const float *w, _Float16 *fwd, _Float16 *bwd) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 4 * 4 * 2 * 64 * 8) return;
+    const int s = t & 7, lane = (t >> 3) & 63, chunk = (t >> 9) & 1, tile = (t >> 10) & 3, layer = t >> 12;
+    const int g = lane >> 4, i = lane & 15;
+    const int in_dim = layer == 0 ? kGuideInputs : kGuideHidden;
+    const float *W = w + layer * 64 * 64;
+    const int kf = guide_perm(chunk, g, s);
+    fwd[t] = (_Float16)(kf < in_dim ? W[kf * 64 + tile * 16 + i] : 0.f);
+    const int kb = tile * 16 + i;
+    bwd[t] = (_Float16)(kb < in_dim ? W[kb * 64 + kf] : 0.f);
+}
+
+__global__ void k_guide_fp16_table(const float2 *p, uint32_t *p16, size_t n) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const float2 v = p[e];
+        p16[e] = (uint32_t)hbits((_Float16)v.x) | ((uint32_t)hbits((_Float16)v.y) << 16);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_guide_query(GuideView G, const float *pos, uint32_t n, float *out) {
+    __shared__ float stage[4][64 * kGuideUsed];
+    const int wave = threadIdx.x >> 6;
+    const int lane = (int)__lane_id();
+    const uint32_t base = (blockIdx.x * 4 + wave) * 64;
+    if (base >= n) return;
+    const uint32_t me = min(base + lane, n - 1);
+    guide_infer_wave(G, mk3(pos[3 * me], pos[3 * me + 1], pos[3 * me + 2]), stage[wave]);
+    wave_sync_lds();
+    if (base + lane < n)
+        for (int k = 0; k < kGuideUsed; ++k)
+            out[(size_t)(base + lane) * kGuideUsed + k] = stage[wave][lane * kGuideUsed + k];
+}
+
+__global__ void k_guide_collect_begin(GuideCtl *ctl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->n_samples = 0u;
+        ctl->n_records = 0u;
+        ctl->fp_sum = 0.f;
+    }
+}
+
+__global__ void k_guide_collect_end(GuideCtl *ctl, uint32_t capacity) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctl->total_samples += min(ctl->n_samples, capacity);
+}
+
+__device__ __forceinline__ uint32_t wave_alloc_g(bool pred, uint32_t *counter) {
+    const unsigned long long mask = __ballot(pred);
+    if (mask == 0ull) return 0u;
+    const uint32_t lane = __lane_id();
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned long long lt = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+    return base + (uint32_t)__popcll(lt);
+}
+
+__device__ __forceinline__ bool record_complete(const GuideRecords &r, uint32_t idx, uint32_t total, float4 &r4,
+                                                uint32_t &path) {
+    r4 = r.rec[4 * (size_t)r.capacity + idx];
+    path = (idx / r.max_depth) * r.stride + r.phase;
+    return __float_as_uint(r4.w) == r.epoch && path < total;
+}
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return x;
+}
+
+__global__ __launch_bounds__(256) void k_guide_collect_count(uint32_t total, GuideRecords r, GuideCtl *ctl) {
+    uint32_t mine = 0;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < r.capacity; idx += gridDim.x * blockDim.x) {
+        float4 r4;
+        uint32_t path;
+        if (record_complete(r, idx, total, r4, path)) ++mine;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
+    if (__lane_id() == 0 && mine) atomicAdd(&ctl->n_records, mine);
+}
+
+// Turns the completed records of one launch into training samples, keeping
+// each with probability capacity / completed records so that the kept set is
+// a uniform subsample of the launch.  Incident radiance along the sampled
+// direction = (L_final - L_adj) / T_{k+1} per channel (L_adj already holds
+// the unweighted emission of the next vertex).
+__global__ __launch_bounds__(256) void k_guide_collect(const float4 *out_l, uint32_t total, GuideRecords r,
+                                                       float4 *samples, uint32_t capacity, GuideCtl *ctl) {
+    const uint32_t n = r.capacity;
+    const uint32_t nrec = ctl->n_records;
+    const float keep = nrec > capacity ? (float)capacity / (float)nrec : 1.f;
+    const uint32_t keep_bits = keep >= 1.f ? 0xFFFFFFFFu : (uint32_t)(keep * 4294967040.f);
+    for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {
+        const uint32_t idx = b + threadIdx.x;
+        bool ok = false;
+        float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0, s2 = s0;
+        float fpv = 0.f;
+        float4 r4;
+        uint32_t path;
+        if (idx < n && record_complete(r, idx, total, r4, path) &&
+            mix32(idx * 0x9E3779B1u ^ (r.epoch * 0x85EBCA77u)) <= keep_bits) {
+            const float4 r0 = r.rec[idx], r1 = r.rec[(size_t)n + idx], r2 = r.rec[2 * (size_t)n + idx],
+                         r3 = r.rec[3 * (size_t)n + idx];
+            const float4 lf = out_l[path];
+            const float ix = r2.x > 0.f ? fmaxf(lf.x - r4.x, 0.f) / r2.x : 0.f;
+            const float iy = r2.y > 0.f ? fmaxf(lf.y - r4.y, 0.f) / r2.y : 0.f;
+            const float iz = r2.z > 0.f ? fmaxf(lf.z - r4.z, 0.f) / r2.z : 0.f;
+            const float target = luminance(mk3(ix, iy, iz));
+            fpv = luminance(mk3(r3.x * ix, r3.y * iy, r3.z * iz));
+            if (isfinite(target) && isfinite(fpv)) {
+                ok = true;
+                s0 = make_float4(r0.x, r0.y, r0.z, 0.f);
+                s1 = make_float4(r1.x, r1.y, r1.z, 0.f);
+                s2 = make_float4(target, fpv, r0.w, 1.f);
+            } else {
+                fpv = 0.f;
+            }
+        }
+        const uint32_t k = wave_alloc_g(ok, &ctl->n_samples);
+        float fs = (ok && k < capacity) ? fpv : 0.f;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) fs += __shfl_xor(fs, off);
+        if (__lane_id() == 0 && fs != 0.f) unsafeAtomicAdd(&ctl->fp_sum, fs);
+        if (ok && k < capacity) {
+            samples[3 * (size_t)k] = s0;
+            samples[3 * (size_t)k + 1] = s1;
+            samples[3 * (size_t)k + 2] = s2;
+        }
+    }
+}
+
+uint64_t splitmix(uint64_t &s) {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+float uniform01(uint64_t &s) { return (float)((splitmix(s) >> 40) * (1.0 / 16777216.0)); }
+
+template <typename T>
+void gfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+}  // namespace
+
+namespace pupil {
+
+#define GUIDE_HIP(call)                                              \
+    do {                                                             \
+        hipError_t e_ = (call);                                      \
+        if (e_ != hipSuccess) {                                      \
+            err = std::string(#call) + ": " + hipGetErrorString(e_); \
+            return false;                                            \
+        }                                                            \
+    } while (0)
+
+void GuideState::release() {
+    gfree(table16);
+    gfree(table32);
+    gfree(table_m);
+    gfree(table_v);
+    gfree(table_g);
+    gfree(w32);
+    gfree(w_m);
+    gfree(w_v);
+    gfree(w_part);
+    gfree(fwd_a);
+    gfree(bwd_a);
+    gfree(ctl);
+    gfree(xt);
+    gfree(dyt);
+    gfree(samples);
+    gfree(rec);
+    batch_cap = sample_cap = rec_cap = 0;
+    enabled = false;
+}
+
+bool GuideState::init(const pt_guide_config &c, const float lo[3], const float hi[3], hipStream_t s,
+                      std::string &err) {
+    release();
+    stream = s;
+    cfg = c;
+    if (cfg.log2_table == 0) cfg.log2_table = 19;
+    if (cfg.log2_table < 10 || cfg.log2_table > 24) {
+        err = "pt_guide_enable: log2_table must be in [10, 24]";
+        return false;
+    }
+    if (cfg.n_min == 0) cfg.n_min = 16;
+    if (cfg.n_max == 0) cfg.n_max = 2048;
+    if (cfg.n_max < cfg.n_min) {
+        err = "pt_guide_enable: n_max below n_min";
+        return false;
+    }
+    if (!(cfg.learning_rate > 0.f)) cfg.learning_rate = 1e-2f;
+    if (cfg.batch_size == 0) cfg.batch_size = 65536;
+    if (cfg.batch_size % 64u != 0u || cfg.batch_size > (1u << 22)) {
+        err = "pt_guide_enable: batch_size must be a multiple of 64, at most 2^22";
+        return false;
+    }
+    if (cfg.steps_per_frame == 0) cfg.steps_per_frame = 1;
+    if (cfg.train_stride == 0) cfg.train_stride = 32;
+    // Level resolutions floor(n_min * b^l), b = (n_max / n_min)^(1 / 15), in double.
+    const double b = std::exp((std::log((double)cfg.n_max) - std::log((double)cfg.n_min)) / (kGuideLevels - 1));
+    const uint64_t T = 1ull << cfg.log2_table;
+    for (int l = 0; l < kGuideLevels; ++l) {
+        levels[l].res = (uint32_t)std::floor((double)cfg.n_min * std::pow(b, (double)l));
+        const uint64_t side = levels[l].res + 1ull;
+        levels[l].dense = side * side * side <= T ? 1u : 0u;
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (!(hi[k] >= lo[k])) {
+            err = "pt_guide_enable: invalid scene bounds";
+            return false;
+        }
+        const float ext = hi[k] - lo[k];
+        const float pad = std::max(ext * 1e-3f, 1e-4f);
+        bounds_lo[k] = lo[k] - pad;
+        bounds_inv[k] = 1.f / (ext + 2.f * pad);
+    }
+    const size_t nt = (size_t)kGuideLevels * T;
+    table_entries = nt;
+    const size_t nw = 4 * 64 * 64;
+    GUIDE_HIP(hipMalloc((void **)&table16, nt * sizeof(uint32_t)));
+    GUIDE_HIP(hipMalloc((void **)&table32, nt * sizeof(float2)));
+    GUIDE_HIP(hipMalloc((void **)&table_m, nt * sizeof(float2)));
+    GUIDE_HIP(hipMalloc((void **)&table_v, nt * sizeof(float2)));
+    GUIDE_HIP(hipMalloc((void **)&table_g, nt * sizeof(float2)));
+    GUIDE_HIP(hipMalloc((void **)&w32, nw * sizeof(float)));
+    GUIDE_HIP(hipMalloc((void **)&w_m, nw * sizeof(float)));
+    GUIDE_HIP(hipMalloc((void **)&w_v, nw * sizeof(float)));
+    GUIDE_HIP(hipMalloc((void **)&fwd_a, 4 * 4 * 2 * 64 * 8 * sizeof(_Float16)));
+    GUIDE_HIP(hipMalloc((void **)&bwd_a, 4 * 4 * 2 * 64 * 8 * sizeof(_Float16)));
+    GUIDE_HIP(hipMalloc((void **)&ctl, sizeof(GuideCtl)));
+    // Initialization: table U(-1e-4, 1e-4); weights U(+-sqrt(6 / (fan_in + fan_out)))
+    // on the used part of each matrix, zero elsewhere.
+    uint64_t st = (uint64_t)cfg.seed * 0x2545F4914F6CDD1Dull + 1;
+    std::vector<float> ht(nt * 2);
+    for (auto &x : ht) x = (uniform01(st) * 2.f - 1.f) * 1e-4f;
+    std::vector<float> hw(nw, 0.f);
+    for (int l = 0; l < 4; ++l) {
+        const int in = l == 0 ? kGuideInputs : kGuideHidden;
+        const int out = l == 3 ? kGuideUsed : kGuideHidden;
+        const float a = std::sqrt(6.f / (float)(in + out));
+        for (int k = 0; k < in; ++k)
+            for (int n = 0; n < out; ++n) hw[(size_t)l * 4096 + k * 64 + n] = (uniform01(st) * 2.f - 1.f) * a;
+    }
+    GUIDE_HIP(hipMemcpy(table32, ht.data(), nt * sizeof(float2), hipMemcpyHostToDevice));
+    GUIDE_HIP(hipMemcpy(w32, hw.data(), nw * sizeof(float), hipMemcpyHostToDevice));
+    GUIDE_HIP(hipMemset(table_m, 0, nt * sizeof(float2)));
+    GUIDE_HIP(hipMemset(table_v, 0, nt * sizeof(float2)));
+    GUIDE_HIP(hipMemset(table_g, 0, nt * sizeof(float2)));
+    GUIDE_HIP(hipMemset(w_m, 0, nw * sizeof(float)));
+    GUIDE_HIP(hipMemset(w_v, 0, nw * sizeof(float)));
+    GuideCtl hc;
+    std::memset(&hc, 0, sizeof(hc));
+    hc.loss_scale = kGuideScaleInit;
+    hc.fp_norm = 1.f;
+    GUIDE_HIP(hipMemcpy(ctl, &hc, sizeof(hc), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_guide_fp16_table, dim3(2048), dim3(256), 0, stream, table32, table16, nt);
+    GUIDE_HIP(hipGetLastError());
+    if (!pack(err)) return false;
+    GUIDE_HIP(hipStreamSynchronize(stream));
+    steps_launched = 0;
+    epoch = 0;
+    phase = 0;
+    enabled = true;
+    return true;
+}
+
+bool GuideState::pack(std::string &err) {
+    hipLaunchKernelGGL(k_guide_pack, dim3(64), dim3(256), 0, stream, w32, fwd_a, bwd_a);
+    GUIDE_HIP(hipGetLastError());
+    return true;
+}
+
+GuideView GuideState::view() const {
+    GuideView v;
+    v.table = table16;
+    v.fwd_a = reinterpret_cast<const half8 *>(fwd_a);
+    v.bwd_a = reinterpret_cast<const half8 *>(bwd_a);
+    for (int l = 0; l < kGuideLevels; ++l) v.levels[l] = levels[l];
+    v.mask = (uint32_t)((1ull << cfg.log2_table) - 1ull);
+    v.log2t = cfg.log2_table;
+    v.lo = mk3(bounds_lo[0], bounds_lo[1], bounds_lo[2]);
+    v.inv_ext = mk3(bounds_inv[0], bounds_inv[1], bounds_inv[2]);
+    return v;
+}
+
+bool GuideState::ensure_batch(uint32_t batch, std::string &err) {
+    if (batch <= batch_cap) return true;
+    gfree(xt);
+    gfree(dyt);
+    gfree(w_part);
+    const size_t n = (size_t)4 * 64 * batch;
+    GUIDE_HIP(hipMalloc((void **)&xt, n * sizeof(uint16_t)));
+    GUIDE_HIP(hipMalloc((void **)&dyt, n * sizeof(uint16_t)));
+    GUIDE_HIP(hipMemset(xt, 0, n * sizeof(uint16_t)));
+    GUIDE_HIP(hipMemset(dyt, 0, n * sizeof(uint16_t)));
+    const size_t slices = (batch + kGuideDwSlice - 1) / kGuideDwSlice;
+    GUIDE_HIP(hipMalloc((void **)&w_part, slices * 4 * 64 * 64 * sizeof(float)));
+    batch_cap = batch;
+    return true;
+}
+
+bool GuideState::ensure_samples(uint32_t n, std::string &err) {
+    if (n <= sample_cap) return true;
+    gfree(samples);
+    GUIDE_HIP(hipMalloc((void **)&samples, (size_t)n * 3 * sizeof(float4)));
+    GUIDE_HIP(hipMemset(samples, 0, (size_t)n * 3 * sizeof(float4)));
+    sample_cap = n;
+    return true;
+}
+
+bool GuideState::ensure_records(uint32_t n, std::string &err) {
+    if (n <= rec_cap) return true;
+    gfree(rec);
+    GUIDE_HIP(hipMalloc((void **)&rec, (size_t)n * 5 * sizeof(float4)));
+    GUIDE_HIP(hipMemset(rec, 0, (size_t)n * 5 * sizeof(float4)));
+    rec_cap = n;
+    return true;
+}
+
+bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool apply, std::string &err) {
+    const GuideView G = view();
+    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, first, batch, sample_cap, fp_norm);
+    TrainBuffers B{xt, dyt, table_g};
+    const uint32_t waves = batch / 64;
+    hipLaunchKernelGGL(k_guide_train, dim3((waves + 3) / 4), dim3(kTrainBlock), 0, stream, G,
+                       samples + 3 * (size_t)first, batch, ctl, B);
+    hipLaunchKernelGGL(k_guide_dw, dim3((batch + kGuideDwSlice - 1) / kGuideDwSlice, 4), dim3(64), 0, stream, xt,
+                       dyt, batch, ctl, w_part);
+    const AdamParams at{cfg.learning_rate, 0.9f, 0.99f, 1e-15f};
+    const AdamParams aw{cfg.learning_rate, 0.9f, 0.99f, 1e-8f};
+    if (apply) hipLaunchKernelGGL(k_guide_mid, dim3(1), dim3(64), 0, stream, ctl, at.b1, at.b2);
+    hipLaunchKernelGGL(k_guide_adam_table, dim3(2048), dim3(256), 0, stream, table32, table_m, table_v, table_g,
+                       table16, table_entries, at, ctl, apply ? 1u : 0u);
+    if (apply) {
+        hipLaunchKernelGGL(k_guide_adam_weights, dim3(16), dim3(256), 0, stream, w32, w_m, w_v, w_part, aw, ctl, 1u,
+                           (float *)nullptr);
+        hipLaunchKernelGGL(k_guide_end, dim3(1), dim3(64), 0, stream, ctl);
+        if (!pack(err)) return false;
+    }
+    GUIDE_HIP(hipGetLastError());
+    ++steps_launched;
+    return true;
+}
+
+bool GuideState::collect(const float4 *out_l, uint32_t total_paths, const GuideRecords &r, std::string &err) {
+    hipLaunchKernelGGL(k_guide_collect_begin, dim3(1), dim3(64), 0, stream, ctl);
+    const uint32_t grid = std::max(1u, std::min<uint32_t>((r.capacity + 255) / 256, 4096u));
+    hipLaunchKernelGGL(k_guide_collect_count, dim3(grid), dim3(256), 0, stream, total_paths, r, ctl);
+    hipLaunchKernelGGL(k_guide_collect, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, samples, sample_cap,
+                       ctl);
+    hipLaunchKernelGGL(k_guide_collect_end, dim3(1), dim3(64), 0, stream, ctl, sample_cap);
+    GUIDE_HIP(hipGetLastError());
+    return true;
+}
+
+bool GuideState::train(uint32_t steps, float fp_norm, std::string &err) {
+    const uint32_t B = cfg.batch_size;
+    if (!ensure_batch(B, err)) return false;
+    if ((uint64_t)steps * B > sample_cap) steps = sample_cap / B;
+    for (uint32_t s = 0; s < steps; ++s)
+        if (!launch_step(s * B, B, fp_norm, true, err)) return false;
+    return true;
+}
+
+bool GuideState::upload_samples(const float *host, uint32_t n, std::string &err) {
+    if (!ensure_samples(n, err)) return false;
+    GUIDE_HIP(hipMemcpyAsync(samples, host, (size_t)n * 12 * sizeof(float), hipMemcpyHostToDevice, stream));
+    GUIDE_HIP(hipMemcpyAsync(&ctl->n_samples, &n, sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    GUIDE_HIP(hipStreamSynchronize(stream));
+    return true;
+}
+
+bool GuideState::gradients(uint32_t n, float fp_norm, float *table_grad, float *w_grad, float *loss,
+                           std::string &err) {
+    if (!ensure_batch(n, err)) return false;
+    const GuideView G = view();
+    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, 0u, n, sample_cap, fp_norm);
+    TrainBuffers B{xt, dyt, table_g};
+    hipLaunchKernelGGL(k_guide_train, dim3((n / 64 + 3) / 4), dim3(kTrainBlock), 0, stream, G, samples, n, ctl, B);
+    hipLaunchKernelGGL(k_guide_dw, dim3((n + kGuideDwSlice - 1) / kGuideDwSlice, 4), dim3(64), 0, stream, xt, dyt,
+                       n, ctl, w_part);
+    float *d_wg = nullptr;
+    GUIDE_HIP(hipMalloc((void **)&d_wg, 4 * 64 * 64 * sizeof(float)));
+    const AdamParams aw{cfg.learning_rate, 0.9f, 0.99f, 1e-8f};
+    hipLaunchKernelGGL(k_guide_adam_weights, dim3(16), dim3(256), 0, stream, w32, w_m, w_v, w_part, aw, ctl, 0u,
+                       d_wg);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess && table_grad)
+        e = hipMemcpy(table_grad, table_g, table_entries * sizeof(float2), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && w_grad) e = hipMemcpy(w_grad, d_wg, 4 * 64 * 64 * sizeof(float), hipMemcpyDeviceToHost);
+    GuideCtl hc;
+    std::memset(&hc, 0, sizeof(hc));
+    if (e == hipSuccess) e = hipMemcpy(&hc, ctl, sizeof(hc), hipMemcpyDeviceToHost);
+    (void)hipFree(d_wg);
+    if (e != hipSuccess) {
+        err = std::string("pt_guide_gradients: ") + hipGetErrorString(e);
+        return false;
+    }
+    if (loss) *loss = hc.loss_cnt > 0.f ? hc.loss_sum / hc.loss_cnt : 0.f;
+    if (table_grad && hc.loss_cnt > 0.f) {
+        const float inv = 1.f / hc.loss_cnt;
+        for (size_t k = 0; k < 2 * table_entries; ++k) table_grad[k] *= inv;
+    }
+    // Clear the gradient and the step state; the parameters are untouched.
+    GUIDE_HIP(hipMemsetAsync(table_g, 0, table_entries * sizeof(float2), stream));
+    hipLaunchKernelGGL(k_guide_end, dim3(1), dim3(64), 0, stream, ctl);
+    GUIDE_HIP(hipGetLastError());
+    GUIDE_HIP(hipStreamSynchronize(stream));
+    if (hc.bad) {
+        err = "pt_guide_gradients: non-finite gradient";
+        return false;
+    }
+    return true;
+}
+
+bool GuideState::get_params(float *table, float *weights, std::string &err) {
+    GUIDE_HIP(hipStreamSynchronize(stream));
+    if (table) GUIDE_HIP(hipMemcpy(table, table32, table_entries * sizeof(float2), hipMemcpyDeviceToHost));
+    if (weights) GUIDE_HIP(hipMemcpy(weights, w32, 4 * 64 * 64 * sizeof(float), hipMemcpyDeviceToHost));
+    return true;
+}
+
+bool GuideState::set_params(const float *table, const float *weights, std::string &err) {
+    GUIDE_HIP(hipStreamSynchronize(stream));
+    if (table) {
+        GUIDE_HIP(hipMemcpy(table32, table, table_entries * sizeof(float2), hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_guide_fp16_table, dim3(2048), dim3(256), 0, stream, table32, table16, table_entries);
+    }
+    if (weights) {
+        std::vector<float> w(weights, weights + 4 * 64 * 64);
+        // Keep the padding of the first layer (inputs 32..63) and of the
+        // last layer (outputs 41..63) at zero.
+        for (int k = kGuideInputs; k < 64; ++k)
+            for (int n = 0; n < 64; ++n) w[k * 64 + n] = 0.f;
+        for (int k = 0; k < 64; ++k)
+            for (int n = kGuideUsed; n < 64; ++n) w[3 * 4096 + k * 64 + n] = 0.f;
+        GUIDE_HIP(hipMemcpy(w32, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+        if (!pack(err)) return false;
+    }
+    GUIDE_HIP(hipGetLastError());
+    GUIDE_HIP(hipStreamSynchronize(stream));
+    return true;
+}
+
+bool GuideState::query(const float *pos, uint32_t n, float *out, std::string &err) {
+    if (n == 0) return true;
+    float *d_pos = nullptr, *d_out = nullptr;
+    hipError_t e = hipMalloc((void **)&d_pos, (size_t)n * 3 * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_out, (size_t)n * kGuideUsed * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(d_pos, pos, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_guide_query, dim3((n + 255) / 256), dim3(256), 0, stream, view(), d_pos, n, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, (size_t)n * kGuideUsed * sizeof(float), hipMemcpyDeviceToHost);
+    if (d_pos) (void)hipFree(d_pos);
+    if (d_out) (void)hipFree(d_out);
+    if (e != hipSuccess) {
+        err = std::string("pt_guide_query: ") + hipGetErrorString(e);
+        return false;
+    }
+    return true;
+}
+
+bool GuideState::info(pt_guide_info &o, std::string &err) {
+    std::memset(&o, 0, sizeof(o));
+    o.enabled = enabled ? 1u : 0u;
+    if (!enabled) return true;
+    GUIDE_HIP(hipStreamSynchronize(stream));
+    GuideCtl hc;
+    GUIDE_HIP(hipMemcpy(&hc, ctl, sizeof(hc), hipMemcpyDeviceToHost));
+    o.log2_table = cfg.log2_table;
+    o.steps = hc.step;
+    o.skipped = hc.skipped;
+    o.loss_scale = hc.loss_scale;
+    o.last_loss = hc.last_loss;
+    o.last_samples = hc.last_count;
+    for (int l = 0; l < kGuideLevels; ++l) {
+        o.resolution[l] = levels[l].res;
+        o.dense[l] = levels[l].dense;
+    }
+    for (int k = 0; k < 3; ++k) {
+        o.bounds_lo[k] = bounds_lo[k];
+        o.bounds_inv[k] = bounds_inv[k];
+    }
+    return true;
+}
+
+}  // namespace pupil

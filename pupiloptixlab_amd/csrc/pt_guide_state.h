@@ -1,0 +1,115 @@
+// Device-resident state of the guiding network and its online optimizer.
+//
+// Everything the optimizer decides (whether a step has samples, whether its
+// gradients are finite, the Adam bias corrections, the dynamic loss scale)
+// lives in a device control block, so rendering, sample collection and
+// training steps are queued on one stream without host round trips.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/pupil_hip.h"
+#include "pt_guide_dev.h"
+
+namespace pupil {
+
+constexpr float kGuideGradClip = 100.f;   // max |dLoss/doutput| per sample
+constexpr float kGuideScaleInit = 32.f;   // initial loss scale of the FP16 backward pass
+constexpr float kGuideScaleMax = 32768.f;
+constexpr uint32_t kGuideScaleGrowth = 1000;  // finite steps before the scale doubles
+constexpr uint32_t kGuideDwSlice = 2048;  // samples per weight-gradient partial
+
+struct GuideCtl {
+    float loss_scale;
+    uint32_t step;         // applied Adam steps
+    uint32_t good;         // finite steps since the last scale change
+    uint32_t bad;          // non-finite gradient seen in the current step
+    uint32_t skipped;      // steps dropped for non-finite gradients
+    uint32_t active;       // the current step has samples
+    uint32_t batch_valid;  // samples in the current step
+    uint32_t n_samples;    // samples collected since the last collection reset
+    uint32_t n_records;    // completed records of the last launch
+    unsigned long long total_samples;  // samples kept over all collections
+    float c1, c2;          // Adam bias corrections of the current step
+    float loss_sum, loss_cnt;
+    float last_loss;
+    uint32_t last_count;
+    float fp_sum;          // sum of F/p over collected samples
+    float fp_norm;         // selection-loss normalization of the current step
+};
+
+// Training records written by the guided shading kernel, one per guided
+// vertex of a training path (component-major, 5 float4 per record):
+//   0: position.xyz, BSDF pdf of the sampled direction
+//   1: sampled direction.xyz
+//   2: path throughput after the vertex (T_{k+1}).xyz
+//   3: sample weight f |cos| / p.xyz
+//   4: radiance sum L_after - T_{k+1} Le of the next vertex.xyz, epoch bits
+// Record k of a path is completed by the shading of vertex k + 1.
+struct GuideRecords {
+    float4 *rec;
+    uint32_t capacity;   // records
+    uint32_t stride;     // one camera path in `stride` is a training path
+    uint32_t phase;      // training paths: path % stride == phase
+    uint32_t max_depth;  // records per training path
+    uint32_t epoch;      // launch stamp
+};
+
+class GuideState {
+public:
+    bool enabled = false;
+    pt_guide_config cfg{};
+    GuideLevel levels[kGuideLevels]{};
+    float bounds_lo[3]{}, bounds_inv[3]{};
+    size_t table_entries = 0;
+
+    // parameters and optimizer moments
+    uint32_t *table16 = nullptr;  // FP16 feature pairs used by the kernels
+    float2 *table32 = nullptr, *table_m = nullptr, *table_v = nullptr, *table_g = nullptr;
+    float *w32 = nullptr, *w_m = nullptr, *w_v = nullptr, *w_part = nullptr;
+    _Float16 *fwd_a = nullptr, *bwd_a = nullptr;
+    GuideCtl *ctl = nullptr;
+
+    // per-step scratch
+    uint16_t *xt = nullptr, *dyt = nullptr;
+    uint32_t batch_cap = 0;
+
+    // training samples (3 float4 each) and render-side records
+    float4 *samples = nullptr;
+    uint32_t sample_cap = 0;
+    float4 *rec = nullptr;
+    uint32_t rec_cap = 0;
+    uint32_t epoch = 0;
+    uint32_t phase = 0;
+
+    uint64_t steps_launched = 0;
+    hipStream_t stream = nullptr;
+
+    ~GuideState() { release(); }
+    void release();
+    bool init(const pt_guide_config &c, const float lo[3], const float hi[3], hipStream_t s, std::string &err);
+    GuideView view() const;
+    bool ensure_samples(uint32_t n, std::string &err);
+    bool ensure_records(uint32_t n, std::string &err);
+    // Sample collection from the finished paths of one launch.
+    bool collect(const float4 *out_l, uint32_t total_paths, const GuideRecords &r, std::string &err);
+    // `steps` optimizer steps over the collected samples, batch_size each;
+    // steps without samples are skipped on the device.
+    bool train(uint32_t steps, float fp_norm, std::string &err);
+    // Host-sample entry points (tests / external trainers).
+    bool upload_samples(const float *host, uint32_t n, std::string &err);
+    bool gradients(uint32_t n, float fp_norm, float *table_grad, float *w_grad, float *loss, std::string &err);
+    bool get_params(float *table, float *weights, std::string &err);
+    bool set_params(const float *table, const float *weights, std::string &err);
+    bool query(const float *pos, uint32_t n, float *out, std::string &err);
+    bool info(pt_guide_info &out, std::string &err);
+
+private:
+    bool ensure_batch(uint32_t batch, std::string &err);
+    bool launch_step(uint32_t first, uint32_t batch, float fp_norm, bool apply, std::string &err);
+    bool pack(std::string &err);
+};
+
+}  // namespace pupil
