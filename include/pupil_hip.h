@@ -185,6 +185,7 @@ typedef struct pt_stats {
     double ms_guide;          /* event time of guide training (collect + steps), with timing */
     uint64_t guide_steps;     /* optimizer steps launched by pt_render / pt_guide_train */
     uint64_t guide_samples;   /* training samples collected by pt_render */
+    double ms_guide_infer;    /* event time of the network inference kernels, with timing */
 } pt_stats;
 
 typedef struct pt_ctx pt_ctx;
@@ -239,10 +240,19 @@ typedef struct pt_guide_config {
     uint32_t n_min, n_max;    /* grid resolution of the coarsest / finest level (16, 2048) */
     uint32_t seed;            /* parameter initialization */
     float learning_rate;      /* Adam step size (1e-2) */
-    uint32_t batch_size;      /* samples per optimizer step, multiple of 64 (65536) */
-    uint32_t steps_per_frame; /* optimizer steps per frame rendered with PT_RENDER_GUIDE_TRAIN (1) */
-    uint32_t train_stride;    /* one camera path in train_stride records training data (32) */
+    uint32_t batch_size;      /* samples per optimizer step, multiple of 64 (16384) */
+    uint32_t steps_per_frame; /* optimizer steps per frame rendered with PT_RENDER_GUIDE_TRAIN (4) */
+    uint32_t train_stride;    /* one camera path in train_stride records training data (1) */
+    uint32_t loss_flags;      /* pt_guide_loss_flags: terms left out of the loss (0 = all) */
+    float fixed_selection;    /* > 0: sample the guide with this probability instead of the learned c */
+    float l2_weight;          /* weight of the relative-L2 radiance term (0 = default 1e-3) */
 } pt_guide_config;
+
+enum pt_guide_loss_flags {
+    PT_GUIDE_NO_RADIANCE_L2 = 1, /* relative L2 of the radiance model */
+    PT_GUIDE_NO_GUIDE_CE = 2,    /* cross-entropy of the guide density */
+    PT_GUIDE_NO_MIXTURE_CE = 4   /* cross-entropy of the one-sample mixture (trains c) */
+};
 
 typedef struct pt_guide_info {
     uint32_t enabled;
@@ -267,8 +277,10 @@ int pt_guide_get_info(pt_ctx *ctx, pt_guide_info *out);
  * (3 floats each). */
 int pt_guide_query(pt_ctx *ctx, const float *positions, uint32_t n, float *outputs);
 /* Training sample layout, 12 floats: position xyz, 0, direction xyz (unit),
- * 0, target radiance (luminance), F/p (luminance of f L cos / pdf), BSDF pdf
- * of the direction, valid (1 or 0).  fp_norm scales F/p in the selection loss. */
+ * 0, radiance target (luminance of the incident radiance times |cos|), F/p
+ * (luminance of f L |cos| / pdf), BSDF pdf of the direction, weight (0 =
+ * unused slot; the loss is the weighted mean).  fp_norm scales F/p in the
+ * cross-entropy terms. */
 /* One optimizer step on n samples (n a multiple of 64); *loss = mean loss. */
 int pt_guide_train(pt_ctx *ctx, const float *samples, uint32_t n, float fp_norm, float *loss);
 /* Gradients of the mean loss over n samples, without a step:
@@ -278,6 +290,10 @@ int pt_guide_gradients(pt_ctx *ctx, const float *samples, uint32_t n, float fp_n
 /* FP32 master parameters, same layouts as pt_guide_gradients. */
 int pt_guide_get_params(pt_ctx *ctx, float *table, float *weights);
 int pt_guide_set_params(pt_ctx *ctx, const float *table, const float *weights);
+/* Copies up to max_samples of the training samples collected by the last
+ * training launch of pt_render (12 floats each, layout of pt_guide_train);
+ * *count receives the number copied. */
+int pt_guide_read_samples(pt_ctx *ctx, float *samples, uint32_t max_samples, uint32_t *count);
 
 /* ---- host scene helpers (resource::Scene / World, headless) ----------- */
 typedef struct pt_host_scene pt_host_scene;

@@ -8,19 +8,37 @@ Adam step:
 * multiresolution hash-grid encoding of the hit position (Mueller et al. 2022,
   "Instant neural graphics primitives"): L levels, F features per level, T
   entries per level, resolutions N_l = floor(N_min * b^l), trilinear
-  interpolation, dense indexing while (N_l + 1)^3 <= T, spatial hash
-  (x * 1) ^ (y * 2654435761) ^ (z * 805459861) mod T otherwise;
+  interpolation, dense indexing while (N_l + 1)^3 <= T; above that a
+  spatial hash of 4x4x4-vertex bricks, h = (X * 1) ^ (Y * 2654435761) ^
+  (Z * 805459861) of the brick (X, Y, Z) = (x, y, z) >> 2, entry
+  ((h << 6) | local) mod T with local = (x & 3) | (y & 3) << 2 | (z & 3) << 4,
+  so the 8 corners of a cell mostly share one 64-entry (256-byte) brick;
 * MLP 32 -> 64 -> 64 -> 64 -> 64 (41 outputs used), ReLU, no biases;
 * outputs -> 8 von Mises-Fisher lobes (mean = normalize(o[5k:5k+3]),
   kappa = exp(clamp(o[5k+3])), weight = exp(clamp(o[5k+4]))) and a selection
-  probability c = 0.95 * sigmoid(o[40]);
+  probability c = 0.95 * sigmoid(o[40] - 2) (about 0.11 for a fresh network,
+  so early frames sample mostly from the BSDF);
 * radiance model Lhat(w) = sum_k weight_k vMF(w; mean_k, kappa_k); guiding pdf
   q = Lhat / sum_k weight_k;
-* loss per sample: relative L2 (Lhat - L)^2 / (stopgrad(Lhat)^2 + 0.01) plus
-  the selection term -(F / p_s) * log(c q + (1 - c) p_bsdf) with q held fixed;
-* per-sample output gradients clipped to GRAD_CLIP in max norm (direction
-  kept), the batch mean taken over valid samples only;
-* Adam (beta1 0.9, beta2 0.99, eps 1e-15 on the hash table / 1e-8 on weights).
+* the radiance model factorizes as Lhat = W q, W = sum_k weight_k (the
+  magnitude) and q the normalized guide density; loss per sample, with
+  w = min(F / p_s / norm, 10) and F = f L |cos| of the sample:
+  - relative L2 of the magnitude, l2 (W sg(q) - L)^2 / (sg(Lhat)^2 + 0.01), on
+    the target L (incident radiance times |cos| at the vertex), l2 = L2_WEIGHT
+    by default;
+  - the cross-entropy of the guide, -w log q (shapes the lobes toward the
+    contribution-weighted sample distribution, as in neural importance
+    sampling);
+  - the cross-entropy of the one-sample mixture, -w log(c sg(q) + (1 - c) p_bsdf)
+    (trains the selection probability c);
+  (the relative L2 loss fit to the lobes directly collapses them onto single
+  noisy samples, so it reaches the network only through the magnitude W);
+* per-sample output gradients clipped in max norm (direction kept): the
+  relative-L2 part to L2_CLIP (so single bright samples cannot swamp the
+  shared network), the cross-entropy part to GRAD_CLIP; the batch mean is
+  weighted by the sample weights;
+* Adam (beta1 0.9, beta2 0.99, eps 1e-15 on the hash table / 1e-8 on weights);
+  hash-table entries with a zero gradient in a step are skipped (sparse Adam).
 """
 from __future__ import annotations
 
@@ -31,8 +49,11 @@ PRIMES = (1, 2654435761, 805459861)
 KAPPA_MIN_LOG, KAPPA_MAX_LOG = -5.0, 7.0
 WEIGHT_MIN_LOG, WEIGHT_MAX_LOG = -15.0, 15.0
 C_MAX = 0.95
+C_BIAS = -2.0
 REL_EPS = 0.01
 GRAD_CLIP = 100.0
+L2_CLIP = 1.0
+L2_WEIGHT = 0.001
 
 
 def fp16(x):
@@ -51,9 +72,9 @@ def corner_index(level_res, log2_t, cx, cy, cz):
     side = level_res + 1
     if side ** 3 <= t:
         return (cx + cy * side + cz * side * side) % t
-    h = (np.uint64(cx) * np.uint64(PRIMES[0])) ^ (np.uint64(cy) * np.uint64(PRIMES[1])) ^ \
-        (np.uint64(cz) * np.uint64(PRIMES[2]))
-    return int(h & np.uint64(0xFFFFFFFF)) & (t - 1)
+    h = ((cx >> 2) * PRIMES[0]) ^ (((cy >> 2) * PRIMES[1]) & 0xFFFFFFFF) ^ (((cz >> 2) * PRIMES[2]) & 0xFFFFFFFF)
+    local = (cx & 3) | ((cy & 3) << 2) | ((cz & 3) << 4)
+    return (((h << 6) & 0xFFFFFFFF) | local) & (t - 1)
 
 
 def corner_indices(level_res, log2_t, cx, cy, cz):
@@ -63,10 +84,13 @@ def corner_indices(level_res, log2_t, cx, cy, cz):
     cx, cy, cz = (np.asarray(a, np.int64) for a in (cx, cy, cz))
     if side ** 3 <= t:
         return (cx + cy * side + cz * side * side) % t
+    m32 = np.uint64(0xFFFFFFFF)
     with np.errstate(over="ignore"):
-        h = (cx.astype(np.uint64) * np.uint64(PRIMES[0])) ^ (cy.astype(np.uint64) * np.uint64(PRIMES[1])) ^ \
-            (cz.astype(np.uint64) * np.uint64(PRIMES[2]))
-    return ((h & np.uint64(0xFFFFFFFF)) & np.uint64(t - 1)).astype(np.int64)
+        h = ((cx >> 2).astype(np.uint64) * np.uint64(PRIMES[0]) & m32) ^ \
+            ((cy >> 2).astype(np.uint64) * np.uint64(PRIMES[1]) & m32) ^ \
+            ((cz >> 2).astype(np.uint64) * np.uint64(PRIMES[2]) & m32)
+    local = ((cx & 3) | ((cy & 3) << 2) | ((cz & 3) << 4)).astype(np.uint64)
+    return ((((h << np.uint64(6)) & m32) | local) & np.uint64(t - 1)).astype(np.int64)
 
 
 def encode(table, u, res, log2_t, quantize=False):
@@ -122,7 +146,7 @@ def mixture(o):
     lw = np.stack([o[:, 5 * k + 4] for k in range(LOBES)], 1)
     kappa = np.exp(np.clip(lk, KAPPA_MIN_LOG, KAPPA_MAX_LOG))
     weight = np.exp(np.clip(lw, WEIGHT_MIN_LOG, WEIGHT_MAX_LOG))
-    c = C_MAX / (1.0 + np.exp(-o[:, 40]))
+    c = C_MAX / (1.0 + np.exp(-(o[:, 40] + C_BIAS)))
     return {"raw": raw, "norm": norm, "mu": mu, "lk": lk, "lw": lw, "kappa": kappa, "weight": weight, "c": c}
 
 
@@ -139,45 +163,59 @@ def radiance_and_pdf(m, w):
     return lhat, q, cosang, v
 
 
-def loss_and_output_grad(o, w, target, fs_over_p, p_bsdf, norm_fp=1.0, lam_c=1.0, clip=GRAD_CLIP):
+def vmf_log(kappa, cosang):
+    """log of the vMF density."""
+    return np.log(kappa) - np.log(2 * np.pi) - np.log(-np.expm1(-2 * kappa)) + kappa * (cosang - 1.0)
+
+
+def loss_and_output_grad(o, w, target, fs_over_p, p_bsdf, norm_fp=1.0, lam_c=1.0, clip=GRAD_CLIP, l2=L2_WEIGHT):
     """Per-sample loss and dLoss/do (the mean over the batch is taken by the
-    caller).  target: luminance of the incident radiance sample; fs_over_p:
-    F / p_s (luminance of f L cos / sampling pdf); p_bsdf: BSDF pdf of w.
+    caller).  target: radiance target of the sample; fs_over_p: F / p_s
+    (luminance of f L |cos| / sampling pdf); p_bsdf: BSDF pdf of w.
     Each row of the gradient is scaled down to max-norm `clip` if larger."""
     m = mixture(o)
     lhat, q, cosang, v = radiance_and_pdf(m, w)
+    wsum = np.sum(m["weight"], axis=1)
     den = lhat ** 2 + REL_EPS
-    loss = (lhat - target) ** 2 / den
-    dl = 2 * (lhat - target) / den  # dLoss/dLhat (denominator held fixed)
-    n = o.shape[0]
+    loss = l2 * (lhat - target) ** 2 / den
+    dl = 2 * (lhat - target) / den  # d(relative L2)/dLhat (denominator held fixed), scaled by l2 below
+    wgt = lam_c * np.minimum(fs_over_p / norm_fp, 10.0)
+    # lobe responsibilities r_k = w_k v_k / Lhat, from log-domain terms
+    lt = np.log(m["weight"]) + vmf_log(m["kappa"], cosang)
+    lt = lt - lt.max(axis=1, keepdims=True)
+    r = np.exp(lt)
+    r /= r.sum(axis=1, keepdims=True)
+    c = m["c"]
+    pmix = np.maximum(c * q + (1 - c) * p_bsdf, 1e-30)
+    loss = loss - wgt * np.log(np.maximum(q, 1e-30)) - wgt * np.log(pmix)
+    dc = -wgt * (q - p_bsdf) / pmix
     g = np.zeros_like(o)
+    g2 = np.zeros_like(o)  # relative-L2 part
     for k in range(LOBES):
         wk, kk, vk, ck = m["weight"][:, k], m["kappa"][:, k], v[:, k], cosang[:, k]
-        # weight
+        # relative L2 through W only: d(W q0)/dlw_k = w_k q0
+        # d(-log q)/d(log-term of lobe k) = -(r_k); the weight also enters 1/wsum
+        b = -wgt * r[:, k]
         inside_w = (m["lw"][:, k] > WEIGHT_MIN_LOG) & (m["lw"][:, k] < WEIGHT_MAX_LOG)
-        g[:, 5 * k + 4] = dl * vk * wk * inside_w
+        g2[:, 5 * k + 4] = dl * wk * q * inside_w
+        g[:, 5 * k + 4] = (b + wgt * wk / wsum) * inside_w
         # kappa: d log v / d kappa = 1/kappa - 2/(e^{2 kappa} - 1) + cos - 1
-        dlogv = 1.0 / kk - 2.0 / np.expm1(2 * kk) + ck - 1.0
+        with np.errstate(over="ignore"):
+            dlogv = 1.0 / kk - 2.0 / np.expm1(2 * kk) + ck - 1.0
         inside_k = (m["lk"][:, k] > KAPPA_MIN_LOG) & (m["lk"][:, k] < KAPPA_MAX_LOG)
-        g[:, 5 * k + 3] = dl * wk * vk * dlogv * kk * inside_k
+        g[:, 5 * k + 3] = b * dlogv * kk * inside_k
         # mean: d cos / d raw = (w - cos * mu) / |raw|
-        dcos = dl * wk * vk * kk
         mu = m["mu"][:, k]
         nr = np.maximum(m["norm"][:, k, 0], 1e-30)
         dr = (w - ck[:, None] * mu) / nr[:, None]
-        g[:, 5 * k:5 * k + 3] = (dcos[:, None] * dr) * (m["norm"][:, k, 0] > 1e-12)[:, None]
-    # selection: -(F/p_s) log(c q + (1-c) p_b), q fixed
-    c = m["c"]
-    pmix = c * q + (1 - c) * p_bsdf
-    wgt = np.minimum(fs_over_p / norm_fp, 10.0)
-    loss_c = -wgt * np.log(np.maximum(pmix, 1e-30))
-    dc = -wgt * (q - p_bsdf) / np.maximum(pmix, 1e-30)
+        g[:, 5 * k:5 * k + 3] = ((b * kk)[:, None] * dr) * (m["norm"][:, k, 0] > 1e-12)[:, None]
     sig = c / C_MAX
-    g[:, 40] = lam_c * dc * C_MAX * sig * (1 - sig)
+    g[:, 40] = dc * C_MAX * sig * (1 - sig)
     if clip is not None:
-        mx = np.max(np.abs(g), axis=1, keepdims=True)
-        g = g * np.where(mx > clip, clip / np.maximum(mx, 1e-300), 1.0)
-    return loss + lam_c * loss_c, g
+        for part, bound in ((g, clip), (g2, L2_CLIP)):
+            mx = np.max(np.abs(part), axis=1, keepdims=True)
+            part *= np.where(mx > bound, bound / np.maximum(mx, 1e-300), 1.0)
+    return loss, g + l2 * g2
 
 
 def backward(acts, weights, g_out):
@@ -202,39 +240,47 @@ def encode_backward(table_shape, cache, dfeat):
 
 
 class Adam:
-    def __init__(self, lr=1e-2, b1=0.9, b2=0.99, eps=1e-15):
-        self.lr, self.b1, self.b2, self.eps = lr, b1, b2, eps
+    """Adam; sparse=True skips entries whose gradient is exactly zero (their
+    moments and values stay as they are), as for hash-table entries no sample
+    touched in the step."""
+
+    def __init__(self, lr=1e-2, b1=0.9, b2=0.99, eps=1e-15, sparse=False):
+        self.lr, self.b1, self.b2, self.eps, self.sparse = lr, b1, b2, eps, sparse
 
     def step(self, p, g, m, v, t):
-        m[:] = self.b1 * m + (1 - self.b1) * g
-        v[:] = self.b2 * v + (1 - self.b2) * g * g
-        mh = m / (1 - self.b1 ** t)
-        vh = v / (1 - self.b2 ** t)
-        p[:] = p - self.lr * mh / (np.sqrt(vh) + self.eps)
+        sel = (g != 0) if self.sparse else np.ones(g.shape, bool)
+        m[sel] = self.b1 * m[sel] + (1 - self.b1) * g[sel]
+        v[sel] = self.b2 * v[sel] + (1 - self.b2) * g[sel] * g[sel]
+        mh = m[sel] / (1 - self.b1 ** t)
+        vh = v[sel] / (1 - self.b2 ** t)
+        p[sel] = p[sel] - self.lr * mh / (np.sqrt(vh) + self.eps)
 
 
 def full_loss(table, weights, u, w, target, fs_over_p, p_bsdf, res, log2_t, norm_fp=1.0, clip=GRAD_CLIP,
-              valid=None, quantize=False):
+              valid=None, quantize=False, l2=L2_WEIGHT):
+    """valid: per-sample weights (or booleans); the loss is the weighted mean
+    and g holds the weighted per-sample gradients."""
     feats, cache = encode(table, u, res, log2_t, quantize)
     o, acts = mlp_forward(feats, weights, quantize)
-    loss, g = loss_and_output_grad(o, w, target, fs_over_p, p_bsdf, norm_fp, clip=clip)
-    if valid is not None:
-        loss = loss[valid]
-        g = g * valid[:, None]
-    return float(loss.mean()), (feats, cache, o, acts, g)
+    loss, g = loss_and_output_grad(o, w, target, fs_over_p, p_bsdf, norm_fp, clip=clip, l2=l2)
+    wts = np.ones(len(loss)) if valid is None else np.asarray(valid, np.float64)
+    g = g * wts[:, None]
+    m = wts > 0
+    return float(np.sum(loss[m] * wts[m]) / np.sum(wts[m])), (feats, cache, o, acts, g)
 
 
 def gradients(table, weights, u, w, target, fs_over_p, p_bsdf, res, log2_t, norm_fp=1.0, clip=GRAD_CLIP,
-              valid=None, quantize=False):
-    """Gradients of the mean loss over the valid samples (q and the
-    relative-L2 denominator held fixed, as in training).  quantize: evaluate
-    with half-precision parameters, features and activations as the GPU does."""
+              valid=None, quantize=False, l2=L2_WEIGHT):
+    """Gradients of the weighted mean loss (valid: per-sample weights, 0 for
+    unused samples; q inside the mixture term and the relative-L2
+    denominator held fixed, as in training).  quantize: evaluate with
+    half-precision parameters, features and activations as the GPU does."""
     if quantize:
         table = fp16(table)
         weights = [fp16(x) for x in weights]
     loss, (feats, cache, o, acts, g) = full_loss(table, weights, u, w, target, fs_over_p, p_bsdf, res, log2_t,
-                                                 norm_fp, clip, valid, quantize)
-    n = u.shape[0] if valid is None else int(np.count_nonzero(valid))
+                                                 norm_fp, clip, valid, quantize, l2)
+    n = u.shape[0] if valid is None else float(np.sum(np.asarray(valid, np.float64)))
     dws, dfeat = backward(acts, weights, g / n)
     dtable = encode_backward(table.shape, cache, dfeat)
     return dws, dtable, o, loss

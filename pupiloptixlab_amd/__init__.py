@@ -38,6 +38,7 @@ ENV_NONE, ENV_CONSTANT = 0, 1
 MIS_REFERENCE, MIS_CONSISTENT = 0, 1
 RENDER_AOV, RENDER_FP64_ACCUM, RENDER_TIMING, RENDER_GUIDE, RENDER_GUIDE_TRAIN = 1, 2, 4, 8, 16
 GUIDE_OUTPUTS, GUIDE_LEVELS = 41, 16
+GUIDE_NO_RADIANCE_L2, GUIDE_NO_GUIDE_CE, GUIDE_NO_MIXTURE_CE = 1, 2, 4
 BUF_FRAME, BUF_ACCUM, BUF_ALBEDO, BUF_NORMAL, BUF_TEST, BUF_PRIM_ID, BUF_ACCUM64 = range(7)
 SCENE_CORNELL, SCENE_TERRAIN, SCENE_DOOR_ROOM, SCENE_FURNACE = 1, 2, 3, 4
 MISS = 0xFFFFFFFF
@@ -90,7 +91,8 @@ class Stats(C.Structure):
                 ("ms_shadow", C.c_double), ("ms_accumulate", C.c_double), ("launches_trace", C.c_uint64),
                 ("launches_shade", C.c_uint64), ("launches_shadow", C.c_uint64), ("bvh_nodes", C.c_uint32),
                 ("bvh_leaf_tris", C.c_uint32), ("bvh_depth", C.c_uint32), ("bvh_build_ms", C.c_double),
-                ("ms_guide", C.c_double), ("guide_steps", C.c_uint64), ("guide_samples", C.c_uint64)]
+                ("ms_guide", C.c_double), ("guide_steps", C.c_uint64), ("guide_samples", C.c_uint64),
+                ("ms_guide_infer", C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -99,7 +101,8 @@ class Stats(C.Structure):
 class GuideConfig(C.Structure):
     _fields_ = [("log2_table", C.c_uint32), ("n_min", C.c_uint32), ("n_max", C.c_uint32), ("seed", C.c_uint32),
                 ("learning_rate", C.c_float), ("batch_size", C.c_uint32), ("steps_per_frame", C.c_uint32),
-                ("train_stride", C.c_uint32)]
+                ("train_stride", C.c_uint32), ("loss_flags", C.c_uint32), ("fixed_selection", C.c_float),
+                ("l2_weight", C.c_float)]
 
 
 class GuideInfo(C.Structure):
@@ -152,6 +155,7 @@ _SIGNATURES = {
                                      C.POINTER(C.c_float)]),
     "pt_guide_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pt_guide_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pt_guide_read_samples": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -356,9 +360,11 @@ class PathTracer:
 
     # ---- learned guiding ---------------------------------------------------------
     def guide_enable(self, log2_table: int = 19, n_min: int = 16, n_max: int = 2048, seed: int = 0,
-                     learning_rate: float = 1e-2, batch_size: int = 65536, steps_per_frame: int = 1,
-                     train_stride: int = 32) -> None:
-        cfg = GuideConfig(log2_table, n_min, n_max, seed, learning_rate, batch_size, steps_per_frame, train_stride)
+                     learning_rate: float = 1e-2, batch_size: int = 16384, steps_per_frame: int = 4,
+                     train_stride: int = 1, loss_flags: int = 0, fixed_selection: float = 0.0,
+                     l2_weight: float = 0.0) -> None:
+        cfg = GuideConfig(log2_table, n_min, n_max, seed, learning_rate, batch_size, steps_per_frame, train_stride,
+                          loss_flags, fixed_selection, l2_weight)
         _check(lib().pt_guide_enable(self._ctx, C.byref(cfg)), "pt_guide_enable", self._ctx)
         self._guide_entries = 1 << log2_table
 
@@ -407,6 +413,14 @@ class PathTracer:
         _check(lib().pt_guide_gradients(self._ctx, s.ctypes.data, s.shape[0], fp_norm, tg.ctypes.data,
                                         wg.ctypes.data, C.byref(loss)), "pt_guide_gradients", self._ctx)
         return tg, wg, loss.value
+
+    def guide_read_samples(self, max_samples: int = 1 << 20) -> np.ndarray:
+        """Training samples collected by the last training launch (N x 12)."""
+        out = np.zeros((max_samples, 12), dtype=np.float32)
+        n = C.c_uint32()
+        _check(lib().pt_guide_read_samples(self._ctx, out.ctypes.data, max_samples, C.byref(n)),
+               "pt_guide_read_samples", self._ctx)
+        return out[:n.value]
 
     def guide_get_params(self):
         tb = np.zeros((GUIDE_LEVELS, self._guide_entries, 2), dtype=np.float32)

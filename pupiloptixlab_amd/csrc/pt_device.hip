@@ -29,7 +29,7 @@ using namespace pupil;
 
 namespace {
 
-enum KernelKind { kKindRaygen = 0, kKindTrace, kKindShade, kKindShadow, kKindAccumulate, kKindGuide };
+enum KernelKind { kKindRaygen = 0, kKindTrace, kKindShade, kKindShadow, kKindAccumulate, kKindGuide, kKindGuideInfer };
 
 constexpr int kTraceBlock = 128;
 constexpr int kLdsStack = 12;  // stack entries per lane kept in LDS
@@ -300,9 +300,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneView scene, ShadeParams prm,
 
 // --- guided shading ----------------------------------------------------------------
 struct GuideShadeArgs {
-    GuideView G;
-    RecordSink rs;   // rs.rec null: no training records
-    uint32_t infer;  // sample with the network
+    const uint4 *gout;  // network outputs per queue slot (48 halves, 41 used); null: no guide sampling
+    RecordSink rs;      // rs.rec null: no training records
+    float fixed_c;      // > 0: selection probability used instead of the learned one
 };
 
 __device__ __forceinline__ void wave_lds_sync() {
@@ -311,16 +311,84 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// k_shade with the guided step: every wave that holds at least one
-// guidable vertex evaluates the network for its 64 lanes (one MFMA pass per
-// 16 queries) between the emission/roulette half and the scattering half.
+__device__ __forceinline__ uint32_t pack_half2(float a, float b) {
+    union {
+        _Float16 h[2];
+        uint32_t u;
+    } c;
+    c.h[0] = (_Float16)a;
+    c.h[1] = (_Float16)b;
+    return c.u;
+}
+
+__device__ __forceinline__ void unpack_half2(uint32_t v, float &a, float &b) {
+    union {
+        _Float16 h[2];
+        uint32_t u;
+    } c;
+    c.u = v;
+    a = (float)c.h[0];
+    b = (float)c.h[1];
+}
+
+// Network inference for every hit in the extension queue, ahead of shading.
+// The block keeps the pre-permuted FP16 weight operands in LDS (32 KB); each
+// wave takes 64 queue slots at a time as four 16-query MFMA tiles, queries the
+// hit point o + t d, and stores the 41 outputs (FP16, 48 halves = 96 bytes
+// per slot) straight from the MFMA result registers: lane (g, i) of a tile
+// holds outputs 16 rt + 4 g .. +3 of query i.  Waves stride over the queue;
+// a wave with no hit skips the network.
+constexpr int kInferBlock = 512;
+
+__global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQueue q, const uint32_t *count_ptr,
+                                                             uint2 *gout) {
+    __shared__ half8 wl[4 * 4 * 2 * 64];
+    for (uint32_t t = threadIdx.x; t < 4 * 4 * 2 * 64; t += kInferBlock) wl[t] = G.fwd_a[t];
+    __syncthreads();
+    const uint32_t waves = kInferBlock / 64;
+    const uint32_t wave = threadIdx.x >> 6;
+    const int lane = (int)lane_id();
+    const int g = lane >> 4, il = lane & 15;
+    const uint32_t count = *count_ptr;
+    const uint32_t wstride = gridDim.x * waves * 64u;
+    for (uint32_t base = (blockIdx.x * waves + wave) * 64u; base < count; base += wstride) {
+        const uint32_t i = base + lane;
+        bool valid = i < count;
+        f3 p = mk3(0.f);
+        if (valid) {
+            const float4 r0 = q.ray0[i], r1 = q.ray1[i], h = q.hit[i];
+            valid = f2u(h.y) != kMissPrim;
+            if (valid) p = f3of(r0) + f3of(r1) * h.x;
+        }
+        if (__ballot(valid) == 0ull) continue;
+        const f3 u = guide_normalize_pos(G, p);
+        const int vbit = valid ? 1 : 0;
+#pragma unroll 1
+        for (int qt = 0; qt < 4; ++qt) {
+            // Keep the weight operands in LDS: re-read them per tile instead
+            // of letting them be hoisted into (and occupy) 128 VGPRs.
+            asm volatile("" ::: "memory");
+            const int qi = qt * 16 + il;
+            const f3 uq = mk3(__shfl(u.x, qi), __shfl(u.y, qi), __shfl(u.z, qi));
+            const half8 in0 = guide_encode(G, uq, g);
+            floatx4 out[4];
+            guide_mlp_tile_shared(wl, lane, in0, out);
+            if (__shfl(vbit, qi)) {
+                uint2 *dst = gout + (size_t)(base + qi) * 12u + (uint32_t)g;
+#pragma unroll
+                for (int rt = 0; rt < 3; ++rt)
+                    dst[rt * 4] = make_uint2(pack_half2(out[rt][0], out[rt][1]), pack_half2(out[rt][2], out[rt][3]));
+            }
+        }
+    }
+}
+
+// k_shade with the guided step; the mixture of each guidable vertex comes
+// from k_guide_infer's outputs for the same queue slot.
 __global__ __launch_bounds__(256) void k_shade_guided(SceneView scene, ShadeParams prm, PathQueue in,
                                                       const uint32_t *count_in, PathQueue out, uint32_t *count_out,
                                                       ShadowQueue sq, uint32_t *count_shadow, float4 *out_l,
                                                       AovBuffers aov, GuideShadeArgs ga) {
-    __shared__ float stage[4][64 * kGuideUsed];
-    float *st = stage[threadIdx.x >> 6];
-    const uint32_t lane = lane_id();
     const uint32_t count = *count_in;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
@@ -346,19 +414,21 @@ __global__ __launch_bounds__(256) void k_shade_guided(SceneView scene, ShadePara
             prim = f2u(h.y);
             scatter = guided_begin(scene, prm, ps, prim, h.z, h.w, ga.rs, path, v, so);
         }
-        bool use_guide = ga.infer != 0u && scatter && v.guidable;
+        bool use_guide = ga.gout != nullptr && scatter && v.guidable;
         GuideMixture m;
-        if (__ballot(use_guide) != 0ull) {
-            guide_infer_wave(ga.G, use_guide ? v.g.position : mk3(0.f), st);
-            wave_lds_sync();
-            if (use_guide) {
-                float o[kGuideUsed];
+        if (use_guide) {
+            float o[48];
 #pragma unroll
-                for (int n = 0; n < kGuideUsed; ++n) o[n] = st[lane * kGuideUsed + n];
-                guide_decode(o, m);
-                if (!(m.c >= 0.f && m.c <= 1.f) || !(m.wsum > 0.f) || !isfinite(m.wsum)) use_guide = false;
+            for (int k = 0; k < 6; ++k) {
+                const uint4 u = ga.gout[6 * (size_t)i + k];
+                unpack_half2(u.x, o[8 * k], o[8 * k + 1]);
+                unpack_half2(u.y, o[8 * k + 2], o[8 * k + 3]);
+                unpack_half2(u.z, o[8 * k + 4], o[8 * k + 5]);
+                unpack_half2(u.w, o[8 * k + 6], o[8 * k + 7]);
             }
-            wave_lds_sync();
+            guide_decode(o, m);
+            if (ga.fixed_c > 0.f) m.c = ga.fixed_c;
+            if (!(m.c >= 0.f && m.c <= 1.f) || !(m.wsum > 0.f) || !isfinite(m.wsum)) use_guide = false;
         }
         if (scatter) guided_scatter(scene, ps, v, use_guide, m, ga.rs, path, so);
         const uint32_t j = wave_alloc(valid && so.cont, count_out);
@@ -533,6 +603,9 @@ struct pt_ctx {
     // learned guiding (pt_guide_*)
     float scene_lo[3]{}, scene_hi[3]{};
     GuideState guide;
+    uint4 *d_gout = nullptr;  // network outputs per queue slot
+    uint32_t gout_cap = 0;
+    uint32_t infer_blocks_per_cu = 2;
 };
 
 namespace {
@@ -739,6 +812,7 @@ void collect_event_times(pt_ctx *ctx) {
             case kKindShade: ctx->stats.ms_shade += ms; ctx->stats.launches_shade++; break;
             case kKindShadow: ctx->stats.ms_shadow += ms; ctx->stats.launches_shadow++; break;
             case kKindGuide: ctx->stats.ms_guide += ms; break;
+            case kKindGuideInfer: ctx->stats.ms_guide_infer += ms; break;
             default: ctx->stats.ms_accumulate += ms; break;
         }
     }
@@ -780,6 +854,9 @@ int pt_create(int hip_device, pt_ctx **out) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace<false>, kTraceBlock, 0) == hipSuccess &&
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false>, kTraceBlock, 0) == hipSuccess)
             ctx->trace_blocks_per_cu = (uint32_t)std::max(1, std::min(a, b));
+        int c = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_guide_infer, kInferBlock, 0) == hipSuccess && c > 0)
+            ctx->infer_blocks_per_cu = (uint32_t)c;
     }
     if (hipMalloc((void **)&ctx->d_stat, 4 * sizeof(unsigned long long)) != hipSuccess) {
         g_create_error = "hipMalloc failed";
@@ -798,6 +875,7 @@ void pt_destroy(pt_ctx *ctx) {
     ctx->guide.release();
     free_scene(ctx);
     free_queues(ctx);
+    dfree(ctx->d_gout);
     dfree(ctx->d_spill);
     dfree(ctx->d_stat);
     dfree(ctx->d_count_hist);
@@ -1087,10 +1165,10 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
     uint32_t per_launch = p->spp_per_launch;
     // Default launch width: ~64M paths in flight (about 15 GB of queues), which
     // amortizes the latency-bound tail of the last bounces over many frames.
-    // While training the guide, launches are kept short (~16M paths) so the
-    // network is updated every few frames.
+    // While training the guide, launches are kept short (~2M paths, one
+    // 1080p frame) so the network is updated after every launch.
     if (per_launch == 0) {
-        const uint32_t paths = guide_train ? 16u * 1024u * 1024u : 64u * 1024u * 1024u;
+        const uint32_t paths = guide_train ? 2u * 1024u * 1024u : 64u * 1024u * 1024u;
         per_launch = std::max(1u, std::min(64u, paths / std::max(1u, npix)));
     }
     per_launch = std::min(per_launch, std::max(1u, p->spp));
@@ -1122,9 +1200,19 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
 
     GuideShadeArgs ga;
     std::memset(&ga, 0, sizeof(ga));
+    GuideView gview;
+    std::memset(&gview, 0, sizeof(gview));
     if (guided) {
-        ga.G = ctx->guide.view();
-        ga.infer = guide_infer ? 1u : 0u;
+        gview = ctx->guide.view();
+        ga.fixed_c = std::min(ctx->guide.cfg.fixed_selection, 1.f);
+    }
+    if (guide_infer) {
+        if (ctx->gout_cap < per_launch * npix) {
+            dfree(ctx->d_gout);
+            PT_HIP(hipMalloc((void **)&ctx->d_gout, (size_t)per_launch * npix * 6 * sizeof(uint4)));
+            ctx->gout_cap = per_launch * npix;
+        }
+        ga.gout = ctx->d_gout;
     }
     if (guide_train) {
         const pt_guide_config &gc = ctx->guide.cfg;
@@ -1184,6 +1272,12 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
                     hipLaunchKernelGGL(k_trace<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->q[cur], bvh,
                                        ext + b, twork + b, ctx->d_stat);
                 t.stop(kKindTrace);
+            }
+            if (guide_infer) {
+                EventTimer t(ctx, timing);
+                hipLaunchKernelGGL(k_guide_infer, dim3(ctx->num_cu * ctx->infer_blocks_per_cu), dim3(kInferBlock), 0,
+                                   ctx->stream, gview, ctx->q[cur], ext + b, reinterpret_cast<uint2 *>(ctx->d_gout));
+                t.stop(kKindGuideInfer);
             }
             {
                 EventTimer t(ctx, timing);
@@ -1498,6 +1592,16 @@ int pt_guide_get_params(pt_ctx *ctx, float *table, float *weights) {
     int rc = guide_ready(ctx, "pt_guide_get_params");
     if (rc != PT_OK) return rc;
     return ctx->guide.get_params(table, weights, ctx->error) ? PT_OK : PT_ERR_HIP;
+}
+
+int pt_guide_read_samples(pt_ctx *ctx, float *samples, uint32_t max_samples, uint32_t *count) {
+    int rc = guide_ready(ctx, "pt_guide_read_samples");
+    if (rc != PT_OK) return rc;
+    if (!count || (max_samples && !samples)) {
+        ctx->error = "pt_guide_read_samples: null buffer";
+        return PT_ERR_INVALID_ARG;
+    }
+    return ctx->guide.read_samples(samples, max_samples, *count, ctx->error) ? PT_OK : PT_ERR_HIP;
 }
 
 int pt_guide_set_params(pt_ctx *ctx, const float *table, const float *weights) {

@@ -9,7 +9,8 @@
 //   (mean xyz, log kappa, log weight) and the selection logit.
 // The radiance model is Lhat(w) = sum_k weight_k vMF(w; mean_k, kappa_k); the
 // guiding pdf is q = Lhat / sum_k weight_k; the selection probability is
-// c = 0.95 sigmoid(o40).
+// c = 0.95 sigmoid(o40 - 2), about 0.11 for a fresh network so that early
+// frames sample mostly from the BSDF.
 #pragma once
 
 #include "pt_math.h"
@@ -26,6 +27,7 @@ constexpr int kGuideLobes = 8;
 constexpr float kKappaMinLog = -5.f, kKappaMaxLog = 7.f;
 constexpr float kWeightMinLog = -15.f, kWeightMaxLog = 15.f;
 constexpr float kSelectMax = 0.95f;
+constexpr float kSelectBias = -2.f;
 constexpr float kRelEps = 0.01f;
 
 struct GuideLevel {
@@ -33,13 +35,20 @@ struct GuideLevel {
     uint32_t dense;  // 1 when (res + 1)^3 fits the table
 };
 
+// Table entry of grid vertex (x, y, z): dense while (res + 1)^3 fits the
+// table; above that the hash of the vertex's 4x4x4 brick selects a 64-entry
+// block and the position inside the brick the entry, so the 8 corners of a
+// cell mostly fall into one 256-byte block (one or two cache lines instead of
+// up to eight).
 PT_HD uint32_t guide_corner_index(uint32_t res, uint32_t dense, uint32_t mask, uint32_t x, uint32_t y,
                                   uint32_t z) {
     if (dense) {
         const uint32_t side = res + 1u;
         return (x + y * side + z * side * side) & mask;
     }
-    return (x * 1u ^ y * 2654435761u ^ z * 805459861u) & mask;
+    const uint32_t h = (x >> 2) * 1u ^ (y >> 2) * 2654435761u ^ (z >> 2) * 805459861u;
+    const uint32_t local = (x & 3u) | ((y & 3u) << 2) | ((z & 3u) << 4);
+    return ((h << 6) | local) & mask;
 }
 
 // Feature slot (8g + s) of a lane in MFMA group g holds network input/hidden
@@ -68,7 +77,7 @@ PT_HD void guide_decode(const float *o, GuideMixture &m) {
         m.weight[k] = expf(clampf(o[5 * k + 4], kWeightMinLog, kWeightMaxLog));
         m.wsum += m.weight[k];
     }
-    m.c = kSelectMax / (1.f + expf(-o[40]));
+    m.c = kSelectMax / (1.f + expf(-(o[40] + kSelectBias)));
 }
 
 // vMF density kappa / (2 pi (1 - e^{-2 kappa})) exp(kappa (cos - 1)).

@@ -54,49 +54,86 @@ __global__ void k_guide_begin(GuideCtl *ctl, uint32_t first, uint32_t batch, uin
     ctl->active = valid > 0u ? 1u : 0u;
     ctl->loss_sum = 0.f;
     ctl->loss_cnt = 0.f;
+    // Selection-loss normalization: the mean F/p of the samples that carry
+    // a contribution (given explicitly by host-side training).
     float norm = fp_explicit;
-    if (!(norm > 0.f)) norm = (n > 0u && ctl->fp_sum > 0.f) ? ctl->fp_sum / (float)n : 1.f;
+    if (!(norm > 0.f)) norm = (ctl->fp_cnt > 0.f && ctl->fp_sum > 0.f) ? ctl->fp_sum / ctl->fp_cnt : 1.f;
     ctl->fp_norm = norm;
 }
 
-// Loss of one sample and its gradient with respect to the 41 outputs
-// (clipped to kGuideGradClip in max norm).  Returns the loss.
+// Loss of one sample and its gradient with respect to the 41 outputs.  Each
+// part is clipped in max norm on its own (relative L2 to kGuideL2Clip, so one
+// bright sample cannot swamp the shared network; cross-entropy to
+// kGuideGradClip).  Returns the loss.  The radiance
+// model factorizes as Lhat = W q (W = sum_k w_k, q the normalized guide
+// density).  With w = min(F/p / fp_norm, 10):
+//   relative L2          (W sg(q) - L)^2 / (sg(Lhat)^2 + 0.01)  (trains W)
+//   guide cross-entropy  -w log q                               (shapes the lobes)
+//   mixture              -w log(c sg(q) + (1 - c) p)            (trains c)
+// The guide term is differentiated through the lobe responsibilities
+// r_k = w_k v_k / Lhat, formed in the log domain so directions no lobe covers
+// still get finite gradients.
 __device__ float sample_loss_grad(const float *o, f3 w, float target, float fs, float pb, float fp_norm,
-                                  float *go) {
+                                  uint32_t flags, float l2_weight, float *go) {
     GuideMixture m;
     guide_decode(o, m);
-    float lhat = 0.f, v[kGuideLobes], cosang[kGuideLobes];
+    float lhat = 0.f, v[kGuideLobes], cosang[kGuideLobes], r[kGuideLobes];
+    float lmax = -3.0e38f;
 #pragma unroll
     for (int k = 0; k < kGuideLobes; ++k) {
         cosang[k] = dot(m.mu[k], w);
         v[k] = vmf_pdf(m.kappa[k], cosang[k]);
         lhat += m.weight[k] * v[k];
+        const float lk = clampf(o[5 * k + 3], kKappaMinLog, kKappaMaxLog);
+        const float lw = clampf(o[5 * k + 4], kWeightMinLog, kWeightMaxLog);
+        r[k] = lw + lk - logf(2.f * kPi) - logf(-expm1f(-2.f * m.kappa[k])) + m.kappa[k] * (cosang[k] - 1.f);
+        lmax = fmaxf(lmax, r[k]);
+    }
+    float rsum = 0.f;
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) {
+        r[k] = expf(r[k] - lmax);
+        rsum += r[k];
     }
     const float q = lhat / m.wsum;
     const float den = lhat * lhat + kRelEps;
-    float loss = (lhat - target) * (lhat - target) / den;
+    const float c = m.c;
+    const float pmix = fmaxf(c * q + (1.f - c) * pb, 1e-30f);
+    const float wgt = fminf(fs / fp_norm, 10.f);
+    const float l2w = (flags & PT_GUIDE_NO_RADIANCE_L2) ? 0.f : l2_weight;
+    const float cew = (flags & PT_GUIDE_NO_GUIDE_CE) ? 0.f : wgt;
+    const float mxw = (flags & PT_GUIDE_NO_MIXTURE_CE) ? 0.f : wgt;
     const float dl = 2.f * (lhat - target) / den;
+    const float loss = l2w * (lhat - target) * (lhat - target) / den - cew * logf(fmaxf(q, 1e-30f)) -
+                       mxw * logf(pmix);
+    const float inv_rsum = 1.f / rsum;
+    // Relative-L2 part (log-weights only), clipped on its own.
+    float g2[kGuideLobes], mx2 = 0.f;
 #pragma unroll
     for (int k = 0; k < kGuideLobes; ++k) {
-        const float wk = m.weight[k], kk = m.kappa[k], vk = v[k], ck = cosang[k];
+        const float lw = o[5 * k + 4];
+        g2[k] = (lw > kWeightMinLog && lw < kWeightMaxLog) ? dl * m.weight[k] * q : 0.f;
+        mx2 = fmaxf(mx2, fabsf(g2[k]));
+    }
+    const float s2 = l2w * (mx2 > kGuideL2Clip ? kGuideL2Clip / mx2 : 1.f);
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) {
+        const float wk = m.weight[k], kk = m.kappa[k], ck = cosang[k];
         const float lk = o[5 * k + 3], lw = o[5 * k + 4];
-        go[5 * k + 4] = (lw > kWeightMinLog && lw < kWeightMaxLog) ? dl * vk * wk : 0.f;
+        const float b = -cew * r[k] * inv_rsum;
+        go[5 * k + 4] = (lw > kWeightMinLog && lw < kWeightMaxLog) ? b + cew * wk / m.wsum : 0.f;
         const float dlogv = 1.f / kk - 2.f / expm1f(2.f * kk) + ck - 1.f;
-        go[5 * k + 3] = (lk > kKappaMinLog && lk < kKappaMaxLog) ? dl * wk * vk * dlogv * kk : 0.f;
+        go[5 * k + 3] = (lk > kKappaMinLog && lk < kKappaMaxLog) ? b * dlogv * kk : 0.f;
         const f3 raw = mk3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
         const float nr = sqrtf(dot(raw, raw));
         f3 dr = mk3(0.f);
-        if (nr > 1e-12f) dr = (w - ck * m.mu[k]) * (dl * wk * vk * kk / nr);
+        if (nr > 1e-12f) dr = (w - ck * m.mu[k]) * (b * kk / nr);
         go[5 * k] = dr.x;
         go[5 * k + 1] = dr.y;
         go[5 * k + 2] = dr.z;
     }
-    const float c = m.c;
-    const float pmix = fmaxf(c * q + (1.f - c) * pb, 1e-30f);
-    const float wgt = fminf(fs / fp_norm, 10.f);
-    loss += -wgt * logf(pmix);
     const float sig = c / kSelectMax;
-    go[40] = -wgt * (q - pb) / pmix * kSelectMax * sig * (1.f - sig);
+    go[40] = -mxw * (q - pb) / pmix * kSelectMax * sig * (1.f - sig);
     float mx = 0.f;
 #pragma unroll
     for (int n = 0; n < kGuideUsed; ++n) mx = fmaxf(mx, fabsf(go[n]));
@@ -105,6 +142,8 @@ __device__ float sample_loss_grad(const float *o, f3 w, float target, float fs, 
 #pragma unroll
         for (int n = 0; n < kGuideUsed; ++n) go[n] *= s;
     }
+#pragma unroll
+    for (int k = 0; k < kGuideLobes; ++k) go[5 * k + 4] += g2[k] * s2;
     return loss;
 }
 
@@ -115,9 +154,12 @@ struct TrainBuffers {
 };
 
 // One wave per 64 samples.  samples: 3 float4 per sample (position,
-// direction, {target, F/p, BSDF pdf, valid}).
+// direction, {target, F/p, BSDF pdf, weight}); weight 0 marks an unused
+// slot.  Losses and gradients are weighted sums; the Adam kernels divide by
+// the weight total.
 __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideView G, const float4 *samples, uint32_t batch,
-                                                             GuideCtl *ctl, TrainBuffers B) {
+                                                             GuideCtl *ctl, TrainBuffers B, uint32_t loss_flags,
+                                                             float l2_weight) {
     __shared__ float stage[kTrainBlock / 64][64 * kGuideUsed];
     if (!ctl->active) return;
     const uint32_t nvalid = ctl->batch_valid;
@@ -130,7 +172,8 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideView G, const 
     const float scale = ctl->loss_scale;
 
     const uint32_t me = base + lane;
-    const bool my_valid = me < nvalid && samples[3 * (size_t)me + 2].w != 0.f;
+    const float my_weight = me < nvalid ? samples[3 * (size_t)me + 2].w : 0.f;
+    const bool my_valid = my_weight > 0.f;
     const float4 p4 = samples[3 * (size_t)me];
     const f3 u = guide_normalize_pos(G, mk3(p4.x, p4.y, p4.z));
 
@@ -165,12 +208,14 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideView G, const 
         float loss = 0.f;
         if (my_valid) {
             const float4 d4 = samples[3 * (size_t)me + 1], aux = samples[3 * (size_t)me + 2];
-            loss = sample_loss_grad(o, mk3(d4.x, d4.y, d4.z), aux.x, aux.y, aux.z, ctl->fp_norm, go);
+            loss = sample_loss_grad(o, mk3(d4.x, d4.y, d4.z), aux.x, aux.y, aux.z, ctl->fp_norm, loss_flags,
+                                    l2_weight, go);
         }
         wave_sync_lds();
+        const float gs = my_valid ? my_weight * scale : 0.f;
 #pragma unroll
-        for (int n = 0; n < kGuideUsed; ++n) st[lane * kGuideUsed + n] = go[n] * scale;
-        float ls = loss, vc = my_valid ? 1.f : 0.f;
+        for (int n = 0; n < kGuideUsed; ++n) st[lane * kGuideUsed + n] = go[n] * gs;
+        float ls = my_valid ? loss * my_weight : 0.f, vc = my_valid ? my_weight : 0.f;
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
             ls += __shfl_xor(ls, off);
@@ -330,14 +375,15 @@ struct AdamParams {
     float lr, b1, b2, eps;
 };
 
+// 1 / (sum of sample weights of the step).
 __device__ __forceinline__ float inv_valid(const GuideCtl *ctl) {
     return ctl->loss_cnt > 0.f ? 1.f / ctl->loss_cnt : 0.f;
 }
 
-// Adam on the hash table (FP32 master + FP16 copy) with the batch-summed
-// gradient divided by the valid sample count; gradients are cleared.
-// Entries with no gradient and no momentum are left as they are (their
-// update is zero).
+// Sparse Adam on the hash table (FP32 master + FP16 copy) with the
+// batch-summed gradient divided by the sample-weight total: entries no sample
+// touched (gradient exactly zero) keep their values and moments.  Gradients
+// are cleared.
 __global__ void k_guide_adam_table(float2 *p, float2 *m, float2 *v, float2 *g, uint32_t *p16, size_t n, AdamParams a,
                                    const GuideCtl *ctl, uint32_t apply) {
     if (!ctl->active) return;
@@ -346,13 +392,12 @@ __global__ void k_guide_adam_table(float2 *p, float2 *m, float2 *v, float2 *g, u
     const float inv = inv_valid(ctl);
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
         float2 gg = g[e];
-        const float2 mm0 = m[e];
-        if (gg.x == 0.f && gg.y == 0.f && mm0.x == 0.f && mm0.y == 0.f) continue;
+        if (gg.x == 0.f && gg.y == 0.f) continue;
         g[e] = make_float2(0.f, 0.f);
         if (!step) continue;
         gg.x *= inv;
         gg.y *= inv;
-        float2 mm = mm0, vv = v[e], pp = p[e];
+        float2 mm = m[e], vv = v[e], pp = p[e];
         mm.x = a.b1 * mm.x + (1.f - a.b1) * gg.x;
         mm.y = a.b1 * mm.y + (1.f - a.b1) * gg.y;
         vv.x = a.b2 * vv.x + (1.f - a.b2) * gg.x * gg.x;
@@ -449,8 +494,10 @@ __global__ __launch_bounds__(256) void k_guide_query(GuideView G, const float *p
 __global__ void k_guide_collect_begin(GuideCtl *ctl) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         ctl->n_samples = 0u;
-        ctl->n_records = 0u;
+        ctl->n_nonzero = 0u;
+        ctl->n_zero = 0u;
         ctl->fp_sum = 0.f;
+        ctl->fp_cnt = 0.f;
     }
 }
 
@@ -470,13 +517,6 @@ __device__ __forceinline__ uint32_t wave_alloc_g(bool pred, uint32_t *counter) {
     return base + (uint32_t)__popcll(lt);
 }
 
-__device__ __forceinline__ bool record_complete(const GuideRecords &r, uint32_t idx, uint32_t total, float4 &r4,
-                                                uint32_t &path) {
-    r4 = r.rec[4 * (size_t)r.capacity + idx];
-    path = (idx / r.max_depth) * r.stride + r.phase;
-    return __float_as_uint(r4.w) == r.epoch && path < total;
-}
-
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     x ^= x >> 16;
     x *= 0x7FEB352Du;
@@ -486,60 +526,97 @@ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
     return x;
 }
 
-__global__ __launch_bounds__(256) void k_guide_collect_count(uint32_t total, GuideRecords r, GuideCtl *ctl) {
-    uint32_t mine = 0;
-    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < r.capacity; idx += gridDim.x * blockDim.x) {
-        float4 r4;
-        uint32_t path;
-        if (record_complete(r, idx, total, r4, path)) ++mine;
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) mine += __shfl_xor(mine, off);
-    if (__lane_id() == 0 && mine) atomicAdd(&ctl->n_records, mine);
+// Training sample of record idx, if the record was completed in this launch.
+// Incident radiance along the sampled direction = (L_final - L_adj) / T_{k+1}
+// per channel (L_adj already holds the unweighted emission of the next
+// vertex); the radiance target is its luminance times |cos| (the density a
+// diffuse vertex would ideally sample), and F/p = luminance of
+// (f |cos| / p) * incident radiance.
+__device__ __forceinline__ bool build_sample(const GuideRecords &r, uint32_t idx, uint32_t total, const float4 *out_l,
+                                             float4 &s0, float4 &s1, float &target, float &fpv, float &pb) {
+    const size_t n = r.capacity;
+    const float4 r4 = r.rec[4 * n + idx];
+    const uint32_t path = (idx / r.max_depth) * r.stride + r.phase;
+    if (__float_as_uint(r4.w) != r.epoch || path >= total) return false;
+    const float4 r0 = r.rec[idx], r1 = r.rec[n + idx], r2 = r.rec[2 * n + idx], r3 = r.rec[3 * n + idx];
+    const float4 lf = out_l[path];
+    const float ix = r2.x > 0.f ? fmaxf(lf.x - r4.x, 0.f) / r2.x : 0.f;
+    const float iy = r2.y > 0.f ? fmaxf(lf.y - r4.y, 0.f) / r2.y : 0.f;
+    const float iz = r2.z > 0.f ? fmaxf(lf.z - r4.z, 0.f) / r2.z : 0.f;
+    target = luminance(mk3(ix, iy, iz)) * r1.w;
+    fpv = luminance(mk3(r3.x * ix, r3.y * iy, r3.z * iz));
+    if (!isfinite(target) || !isfinite(fpv)) return false;
+    s0 = make_float4(r0.x, r0.y, r0.z, 0.f);
+    s1 = make_float4(r1.x, r1.y, r1.z, 0.f);
+    pb = r0.w;
+    return true;
 }
 
-// Turns the completed records of one launch into training samples, keeping
-// each with probability capacity / completed records so that the kept set is
-// a uniform subsample of the launch.  Incident radiance along the sampled
-// direction = (L_final - L_adj) / T_{k+1} per channel (L_adj already holds
-// the unweighted emission of the next vertex).
+__global__ __launch_bounds__(256) void k_guide_collect_count(const float4 *out_l, uint32_t total, GuideRecords r,
+                                                             GuideCtl *ctl) {
+    uint32_t nz = 0, z = 0;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < r.capacity; idx += gridDim.x * blockDim.x) {
+        float4 s0, s1;
+        float target, fpv, pb;
+        if (build_sample(r, idx, total, out_l, s0, s1, target, fpv, pb)) {
+            if (target > 0.f || fpv > 0.f) ++nz;
+            else ++z;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        nz += __shfl_xor(nz, off);
+        z += __shfl_xor(z, off);
+    }
+    if (__lane_id() == 0) {
+        if (nz) atomicAdd(&ctl->n_nonzero, nz);
+        if (z) atomicAdd(&ctl->n_zero, z);
+    }
+}
+
+// Turns the completed records of one launch into training samples.  Samples
+// that carry a contribution are kept first (up to 3/4 of the capacity), the
+// rest of the capacity is filled with zero-contribution samples; each class
+// is a uniform subsample, and a kept sample's weight (aux.w) is 1 / its keep
+// probability, so weighted batch means estimate means over all records.
 __global__ __launch_bounds__(256) void k_guide_collect(const float4 *out_l, uint32_t total, GuideRecords r,
                                                        float4 *samples, uint32_t capacity, GuideCtl *ctl) {
     const uint32_t n = r.capacity;
-    const uint32_t nrec = ctl->n_records;
-    const float keep = nrec > capacity ? (float)capacity / (float)nrec : 1.f;
-    const uint32_t keep_bits = keep >= 1.f ? 0xFFFFFFFFu : (uint32_t)(keep * 4294967040.f);
+    const uint32_t nnz = ctl->n_nonzero, nz = ctl->n_zero;
+    const uint32_t cap_nz = capacity - capacity / 4u;
+    const float keep_nz = nnz > cap_nz ? (float)cap_nz / (float)nnz : 1.f;
+    const uint32_t kept_nz = min(nnz, cap_nz);
+    const uint32_t cap_z = capacity - kept_nz;
+    const float keep_z = nz > cap_z ? (float)cap_z / (float)nz : 1.f;
     for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {
         const uint32_t idx = b + threadIdx.x;
         bool ok = false;
         float4 s0 = make_float4(0.f, 0.f, 0.f, 0.f), s1 = s0, s2 = s0;
         float fpv = 0.f;
-        float4 r4;
-        uint32_t path;
-        if (idx < n && record_complete(r, idx, total, r4, path) &&
-            mix32(idx * 0x9E3779B1u ^ (r.epoch * 0x85EBCA77u)) <= keep_bits) {
-            const float4 r0 = r.rec[idx], r1 = r.rec[(size_t)n + idx], r2 = r.rec[2 * (size_t)n + idx],
-                         r3 = r.rec[3 * (size_t)n + idx];
-            const float4 lf = out_l[path];
-            const float ix = r2.x > 0.f ? fmaxf(lf.x - r4.x, 0.f) / r2.x : 0.f;
-            const float iy = r2.y > 0.f ? fmaxf(lf.y - r4.y, 0.f) / r2.y : 0.f;
-            const float iz = r2.z > 0.f ? fmaxf(lf.z - r4.z, 0.f) / r2.z : 0.f;
-            const float target = luminance(mk3(ix, iy, iz));
-            fpv = luminance(mk3(r3.x * ix, r3.y * iy, r3.z * iz));
-            if (isfinite(target) && isfinite(fpv)) {
-                ok = true;
-                s0 = make_float4(r0.x, r0.y, r0.z, 0.f);
-                s1 = make_float4(r1.x, r1.y, r1.z, 0.f);
-                s2 = make_float4(target, fpv, r0.w, 1.f);
-            } else {
-                fpv = 0.f;
+        if (idx < n) {
+            float target, pb;
+            if (build_sample(r, idx, total, out_l, s0, s1, target, fpv, pb)) {
+                const float keep = (target > 0.f || fpv > 0.f) ? keep_nz : keep_z;
+                const float u = (float)(mix32(idx * 0x9E3779B1u ^ (r.epoch * 0x85EBCA77u)) >> 8) * (1.f / 16777216.f);
+                if (u < keep) {
+                    ok = true;
+                    s2 = make_float4(target, fpv, pb, 1.f / keep);
+                }
             }
         }
         const uint32_t k = wave_alloc_g(ok, &ctl->n_samples);
-        float fs = (ok && k < capacity) ? fpv : 0.f;
+        const bool kept = ok && k < capacity;
+        float fs = kept ? fpv * s2.w : 0.f;
+        float fc = (kept && fpv > 0.f) ? s2.w : 0.f;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) fs += __shfl_xor(fs, off);
-        if (__lane_id() == 0 && fs != 0.f) unsafeAtomicAdd(&ctl->fp_sum, fs);
+        for (int off = 32; off > 0; off >>= 1) {
+            fs += __shfl_xor(fs, off);
+            fc += __shfl_xor(fc, off);
+        }
+        if (__lane_id() == 0 && fs != 0.f) {
+            unsafeAtomicAdd(&ctl->fp_sum, fs);
+            unsafeAtomicAdd(&ctl->fp_cnt, fc);
+        }
         if (ok && k < capacity) {
             samples[3 * (size_t)k] = s0;
             samples[3 * (size_t)k + 1] = s1;
@@ -613,13 +690,14 @@ bool GuideState::init(const pt_guide_config &c, const float lo[3], const float h
         return false;
     }
     if (!(cfg.learning_rate > 0.f)) cfg.learning_rate = 1e-2f;
-    if (cfg.batch_size == 0) cfg.batch_size = 65536;
+    if (cfg.batch_size == 0) cfg.batch_size = 16384;
     if (cfg.batch_size % 64u != 0u || cfg.batch_size > (1u << 22)) {
         err = "pt_guide_enable: batch_size must be a multiple of 64, at most 2^22";
         return false;
     }
-    if (cfg.steps_per_frame == 0) cfg.steps_per_frame = 1;
-    if (cfg.train_stride == 0) cfg.train_stride = 32;
+    if (cfg.steps_per_frame == 0) cfg.steps_per_frame = 4;
+    if (cfg.train_stride == 0) cfg.train_stride = 1;
+    if (!(cfg.l2_weight > 0.f)) cfg.l2_weight = 1e-3f;
     // Level resolutions floor(n_min * b^l), b = (n_max / n_min)^(1 / 15), in double.
     const double b = std::exp((std::log((double)cfg.n_max) - std::log((double)cfg.n_min)) / (kGuideLevels - 1));
     const uint64_t T = 1ull << cfg.log2_table;
@@ -747,7 +825,7 @@ bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool
     TrainBuffers B{xt, dyt, table_g};
     const uint32_t waves = batch / 64;
     hipLaunchKernelGGL(k_guide_train, dim3((waves + 3) / 4), dim3(kTrainBlock), 0, stream, G,
-                       samples + 3 * (size_t)first, batch, ctl, B);
+                       samples + 3 * (size_t)first, batch, ctl, B, cfg.loss_flags, cfg.l2_weight);
     hipLaunchKernelGGL(k_guide_dw, dim3((batch + kGuideDwSlice - 1) / kGuideDwSlice, 4), dim3(64), 0, stream, xt,
                        dyt, batch, ctl, w_part);
     const AdamParams at{cfg.learning_rate, 0.9f, 0.99f, 1e-15f};
@@ -769,7 +847,7 @@ bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool
 bool GuideState::collect(const float4 *out_l, uint32_t total_paths, const GuideRecords &r, std::string &err) {
     hipLaunchKernelGGL(k_guide_collect_begin, dim3(1), dim3(64), 0, stream, ctl);
     const uint32_t grid = std::max(1u, std::min<uint32_t>((r.capacity + 255) / 256, 4096u));
-    hipLaunchKernelGGL(k_guide_collect_count, dim3(grid), dim3(256), 0, stream, total_paths, r, ctl);
+    hipLaunchKernelGGL(k_guide_collect_count, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, ctl);
     hipLaunchKernelGGL(k_guide_collect, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, samples, sample_cap,
                        ctl);
     hipLaunchKernelGGL(k_guide_collect_end, dim3(1), dim3(64), 0, stream, ctl, sample_cap);
@@ -800,7 +878,8 @@ bool GuideState::gradients(uint32_t n, float fp_norm, float *table_grad, float *
     const GuideView G = view();
     hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, 0u, n, sample_cap, fp_norm);
     TrainBuffers B{xt, dyt, table_g};
-    hipLaunchKernelGGL(k_guide_train, dim3((n / 64 + 3) / 4), dim3(kTrainBlock), 0, stream, G, samples, n, ctl, B);
+    hipLaunchKernelGGL(k_guide_train, dim3((n / 64 + 3) / 4), dim3(kTrainBlock), 0, stream, G, samples, n, ctl, B,
+                       cfg.loss_flags, cfg.l2_weight);
     hipLaunchKernelGGL(k_guide_dw, dim3((n + kGuideDwSlice - 1) / kGuideDwSlice, 4), dim3(64), 0, stream, xt, dyt,
                        n, ctl, w_part);
     float *d_wg = nullptr;
@@ -885,6 +964,15 @@ bool GuideState::query(const float *pos, uint32_t n, float *out, std::string &er
         err = std::string("pt_guide_query: ") + hipGetErrorString(e);
         return false;
     }
+    return true;
+}
+
+bool GuideState::read_samples(float *out, uint32_t max_samples, uint32_t &count, std::string &err) {
+    GUIDE_HIP(hipStreamSynchronize(stream));
+    uint32_t n = 0;
+    GUIDE_HIP(hipMemcpy(&n, &ctl->n_samples, sizeof(n), hipMemcpyDeviceToHost));
+    count = std::min(std::min(n, sample_cap), max_samples);
+    if (count) GUIDE_HIP(hipMemcpy(out, samples, (size_t)count * 3 * sizeof(float4), hipMemcpyDeviceToHost));
     return true;
 }
 

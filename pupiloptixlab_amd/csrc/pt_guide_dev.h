@@ -139,6 +139,28 @@ __device__ __forceinline__ void guide_mlp_tile(const GuideView &G, int lane, hal
     for (int rt = 0; rt < 4; ++rt) out[rt] = d[rt];
 }
 
+// Forward pass of one 16-query tile with the weight operands read from a
+// shared (LDS) copy of fwd_a (same layout).
+__device__ __forceinline__ void guide_mlp_tile_shared(const half8 *w, int lane, half8 in0, floatx4 out[4]) {
+    floatx4 d[4];
+    const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) d[rt] = mfma16(w[(rt * 2) * 64 + lane], in0, zero);
+    half8 h[2];
+    relu_pack(d, h);
+#pragma unroll
+    for (int l = 1; l < 4; ++l) {
+#pragma unroll
+        for (int rt = 0; rt < 4; ++rt) {
+            d[rt] = mfma16(w[((l * 4 + rt) * 2) * 64 + lane], h[0], zero);
+            d[rt] = mfma16(w[((l * 4 + rt) * 2 + 1) * 64 + lane], h[1], d[rt]);
+        }
+        if (l < 3) relu_pack(d, h);
+    }
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) out[rt] = d[rt];
+}
+
 // Inference for the 64 queries of a wave: lane L supplies the position of
 // query L (any value when `active` is false) and receives its 41 outputs in
 // `stage` (LDS, 64 x 41 floats per wave).  Must be called by all 64 lanes.

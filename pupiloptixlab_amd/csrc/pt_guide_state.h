@@ -15,7 +15,8 @@
 
 namespace pupil {
 
-constexpr float kGuideGradClip = 100.f;   // max |dLoss/doutput| per sample
+constexpr float kGuideGradClip = 100.f;   // max |dLoss/doutput| per sample, cross-entropy part
+constexpr float kGuideL2Clip = 1.f;       // ... relative-L2 part
 constexpr float kGuideScaleInit = 32.f;   // initial loss scale of the FP16 backward pass
 constexpr float kGuideScaleMax = 32768.f;
 constexpr uint32_t kGuideScaleGrowth = 1000;  // finite steps before the scale doubles
@@ -30,24 +31,27 @@ struct GuideCtl {
     uint32_t active;       // the current step has samples
     uint32_t batch_valid;  // samples in the current step
     uint32_t n_samples;    // samples collected since the last collection reset
-    uint32_t n_records;    // completed records of the last launch
+    uint32_t n_nonzero;    // completed records of the last launch with a contribution
+    uint32_t n_zero;       // ... and without
     unsigned long long total_samples;  // samples kept over all collections
     float c1, c2;          // Adam bias corrections of the current step
     float loss_sum, loss_cnt;
     float last_loss;
     uint32_t last_count;
-    float fp_sum;          // sum of F/p over collected samples
+    float fp_sum;          // weighted sum of F/p over collected samples
+    float fp_cnt;          // weighted count of collected samples with F/p > 0
     float fp_norm;         // selection-loss normalization of the current step
 };
 
 // Training records written by the guided shading kernel, one per guided
 // vertex of a training path (component-major, 5 float4 per record):
 //   0: position.xyz, BSDF pdf of the sampled direction
-//   1: sampled direction.xyz
+//   1: sampled direction.xyz, |cos| to the shading normal
 //   2: path throughput after the vertex (T_{k+1}).xyz
 //   3: sample weight f |cos| / p.xyz
 //   4: radiance sum L_after - T_{k+1} Le of the next vertex.xyz, epoch bits
-// Record k of a path is completed by the shading of vertex k + 1.
+// Record k of a path is completed by the shading of vertex k + 1 (at once,
+// with zero throughput, when the sampled direction carries nothing).
 struct GuideRecords {
     float4 *rec;
     uint32_t capacity;   // records
@@ -105,6 +109,7 @@ public:
     bool set_params(const float *table, const float *weights, std::string &err);
     bool query(const float *pos, uint32_t n, float *out, std::string &err);
     bool info(pt_guide_info &out, std::string &err);
+    bool read_samples(float *out, uint32_t max_samples, uint32_t &count, std::string &err);
 
 private:
     bool ensure_batch(uint32_t batch, std::string &err);

@@ -213,7 +213,19 @@ PT_HD void guided_scatter(const SceneView &s, PathState &ps, const GuidedVertex 
     const bool delta = (lobe & kLobeDelta) != 0u;
     float p = pb;
     if (use_guide && !delta) p = c * guide_pdf(m, wi_w) + (1.f - c) * pb;
+    const bool record = v.guidable && !delta && training_path(rs, path);
+    const uint32_t ridx = record ? record_index(rs, path, v.depth - 1u) : 0u;
+    const size_t rn = rs.capacity;
     if (is_zero(f * fabsf(wi_l.z)) || is_zero(p)) {
+        // A direction that carries nothing: recorded complete with a zero
+        // target, so the guide learns to avoid it.
+        if (record && ridx < rs.capacity) {
+            rs.rec[ridx] = f4{g.position.x, g.position.y, g.position.z, pb};
+            rs.rec[rn + ridx] = f4{wi_w.x, wi_w.y, wi_w.z, fabsf(wi_l.z)};
+            rs.rec[2 * rn + ridx] = f4{0.f, 0.f, 0.f, 0.f};
+            rs.rec[3 * rn + ridx] = f4{0.f, 0.f, 0.f, 0.f};
+            rs.rec[4 * rn + ridx] = f4{0.f, 0.f, 0.f, u2f(rs.epoch)};
+        }
         out.done = true;
         return;
     }
@@ -225,16 +237,12 @@ PT_HD void guided_scatter(const SceneView &s, PathState &ps, const GuidedVertex 
     uint32_t fl = v.depth;
     if (delta) fl |= kPathPrevDelta;
     if (dot(g.normal, ps.d) > 0.f) fl |= kPathEnvUpper;
-    if (v.guidable && !delta && training_path(rs, path)) {
-        const uint32_t idx = record_index(rs, path, v.depth - 1u);
-        if (idx < rs.capacity) {
-            const size_t n = rs.capacity;
-            rs.rec[idx] = f4{g.position.x, g.position.y, g.position.z, pb};
-            rs.rec[n + idx] = f4{wi_w.x, wi_w.y, wi_w.z, 0.f};
-            rs.rec[2 * n + idx] = f4{ps.T.x, ps.T.y, ps.T.z, 0.f};
-            rs.rec[3 * n + idx] = f4{weight.x, weight.y, weight.z, 0.f};
-            fl |= kPathRecorded;
-        }
+    if (record && ridx < rs.capacity) {
+        rs.rec[ridx] = f4{g.position.x, g.position.y, g.position.z, pb};
+        rs.rec[rn + ridx] = f4{wi_w.x, wi_w.y, wi_w.z, fabsf(wi_l.z)};
+        rs.rec[2 * rn + ridx] = f4{ps.T.x, ps.T.y, ps.T.z, 0.f};
+        rs.rec[3 * rn + ridx] = f4{weight.x, weight.y, weight.z, 0.f};
+        fl |= kPathRecorded;
     }
     ps.flags = fl;
     out.cont = true;

@@ -56,37 +56,47 @@ def setup_small(seed=1, n=24, log2_t=8):
     return table, weights, u, w, target, fs, pb, res, log2_t
 
 
-def surrogate(table, weights, u, w, target, fs, pb, res, log2_t, frozen):
-    """Batch-mean loss with the relative-L2 denominator and q frozen at `frozen`."""
+def surrogate(table, weights, u, w, target, fs, pb, res, log2_t, frozen, frozen_q=None):
+    """Batch-mean loss with the relative-L2 denominator frozen at `frozen`
+    and q frozen at `frozen_q` inside the magnitude and mixture terms."""
     feats, _ = G.encode(table, u, res, log2_t)
     o, _ = G.mlp_forward(feats, weights)
     m = G.mixture(o)
     lhat, q, _, _ = G.radiance_and_pdf(m, w)
-    den, q0 = frozen
-    loss = (lhat - target) ** 2 / den
+    qf = frozen_q if frozen_q is not None else q
+    wsum = m["weight"].sum(axis=1)
+    loss = (wsum * qf - target) ** 2 / frozen
     c = m["c"]
-    pmix = c * q0 + (1 - c) * pb
-    loss = loss - np.minimum(fs, 10.0) * np.log(pmix)
+    pmix = c * qf + (1 - c) * pb
+    loss = loss - np.minimum(fs, 10.0) * (np.log(q) + np.log(pmix))
     return float(loss.mean())
+
+
+def test_selection_probability_starts_small():
+    """A fresh network (outputs ~ 0) samples the guide with c = 0.95 sigmoid(-2)."""
+    m = G.mixture(np.zeros((1, 64)))
+    assert m["c"][0] == pytest.approx(0.95 / (1 + np.exp(2.0)))
 
 
 def test_output_gradient_matches_finite_differences():
     table, weights, u, w, target, fs, pb, res, log2_t = setup_small()
     feats, _ = G.encode(table, u, res, log2_t)
     o, _ = G.mlp_forward(feats, weights)
-    _, g = G.loss_and_output_grad(o, w, target, fs, pb, clip=None)
+    _, g = G.loss_and_output_grad(o, w, target, fs, pb, clip=None, l2=1.0)
     m = G.mixture(o)
     lhat, q, _, _ = G.radiance_and_pdf(m, w)
     frozen_den, frozen_q = lhat ** 2 + G.REL_EPS, q
 
     def per_sample(oo):
         mm = G.mixture(oo)
-        lh, _, _, _ = G.radiance_and_pdf(mm, w)
+        lh, qq, _, _ = G.radiance_and_pdf(mm, w)
         c = mm["c"]
-        return (lh - target) ** 2 / frozen_den - np.minimum(fs, 10.0) * np.log(c * frozen_q + (1 - c) * pb)
+        wgt = np.minimum(fs, 10.0)
+        mag = mm["weight"].sum(axis=1) * frozen_q
+        return (mag - target) ** 2 / frozen_den - wgt * np.log(qq) - wgt * np.log(c * frozen_q + (1 - c) * pb)
 
     eps = 1e-6
-    for j in list(range(0, 41, 3)) + [40]:
+    for j in range(41):
         op, om = o.copy(), o.copy()
         op[:, j] += eps
         om[:, j] -= eps
@@ -96,11 +106,11 @@ def test_output_gradient_matches_finite_differences():
 
 def test_network_and_table_gradients_match_finite_differences():
     table, weights, u, w, target, fs, pb, res, log2_t = setup_small(n=12)
-    dws, dtable, o, _ = G.gradients(table, weights, u, w, target, fs, pb, res, log2_t, clip=None)
+    dws, dtable, o, _ = G.gradients(table, weights, u, w, target, fs, pb, res, log2_t, clip=None, l2=1.0)
     m = G.mixture(o)
     lhat, q, _, _ = G.radiance_and_pdf(m, w)
-    frozen = (lhat ** 2 + G.REL_EPS, q)
-    f = lambda tb, ws: surrogate(tb, ws, u, w, target, fs, pb, res, log2_t, frozen)
+    frozen = lhat ** 2 + G.REL_EPS
+    f = lambda tb, ws: surrogate(tb, ws, u, w, target, fs, pb, res, log2_t, frozen, q)
     rng = np.random.default_rng(9)
     eps = 1e-6
     for layer in range(4):
@@ -122,11 +132,36 @@ def test_network_and_table_gradients_match_finite_differences():
 
 
 def test_hash_index_known_values():
-    # dense while (res+1)^3 <= T, hashed above; the hash is (x*1)^(y*2654435761)^(z*805459861) mod T
+    # dense while (res+1)^3 <= T
     assert G.corner_index(3, 8, 1, 2, 3) == 1 + 2 * 4 + 3 * 16
-    x, y, z = 5, 7, 11
-    h = (x * 1) ^ ((y * 2654435761) & 0xFFFFFFFF) ^ ((z * 805459861) & 0xFFFFFFFF)
-    assert G.corner_index(100, 10, x, y, z) == h & 1023
+    # hashed: brick (1, 1, 2) of vertex (5, 7, 11), local (1, 3, 3)
+    h = 1 ^ ((1 * 2654435761) & 0xFFFFFFFF) ^ ((2 * 805459861) & 0xFFFFFFFF)
+    assert h == 0xFE33D69A  # 0x9E3779B1 ^ 0x6004AF2A ^ 1
+    local = 1 | (3 << 2) | (3 << 4)
+    assert G.corner_index(100, 10, 5, 7, 11) == (((h << 6) & 0xFFFFFFFF) | local) & 1023
+    assert G.corner_index(200, 20, 5, 7, 11) == (((h << 6) & 0xFFFFFFFF) | local) & ((1 << 20) - 1)
+
+
+def test_hash_bricks_keep_cell_corners_together():
+    """Inside a brick the 8 corners of a cell are distinct entries of one
+    64-entry block; vectorized and scalar indexing agree."""
+    rng = np.random.default_rng(3)
+    for r in (3, 10, 100, 2048):
+        c = rng.integers(0, r + 1, (300, 3))
+        a = G.corner_indices(r, 12, c[:, 0], c[:, 1], c[:, 2])
+        assert np.array_equal(a, [G.corner_index(r, 12, *map(int, x)) for x in c]), r
+    x, y, z = 4 * 7 + 1, 4 * 3 + 2, 4 * 9  # cell (29..30, 14..15, 36..37) inside one brick
+    idx = {G.corner_index(2048, 19, x + dx, y + dy, z + dz) for dx in (0, 1) for dy in (0, 1) for dz in (0, 1)}
+    assert len(idx) == 8 and len({i >> 6 for i in idx}) == 1
+
+
+def test_sparse_adam_skips_untouched_entries():
+    p = np.array([1.0, 2.0, 3.0])
+    m = np.array([0.1, 0.2, 0.0])
+    v = np.array([0.01, 0.04, 0.0])
+    G.Adam(sparse=True).step(p, np.array([0.5, 0.0, -1.0]), m, v, 3)
+    assert p[1] == 2.0 and m[1] == 0.2 and v[1] == 0.04
+    assert p[0] < 1.0 and p[2] > 3.0
 
 
 def test_adam_matches_closed_form_first_step():
@@ -138,11 +173,16 @@ def test_adam_matches_closed_form_first_step():
 
 
 def test_gradient_clip_keeps_direction_and_bounds_max_norm():
+    """Each loss part is clipped on its own: the relative-L2 part (target terms
+    only) to L2_CLIP, the cross-entropy part (F/p terms only) to `clip`."""
     table, weights, u, w, target, fs, pb, res, log2_t = setup_small(n=32)
     feats, _ = G.encode(table, u, res, log2_t)
     o, _ = G.mlp_forward(feats, weights)
-    _, g_raw = G.loss_and_output_grad(o, w, target * 1e4, fs, pb, clip=None)
-    _, g_clip = G.loss_and_output_grad(o, w, target * 1e4, fs, pb, clip=5.0)
+    zero = np.zeros_like(target)
+    # cross-entropy part alone (relative L2 made inert by target = Lhat)
+    lhat = G.radiance_and_pdf(G.mixture(o), w)[0]
+    _, g_raw = G.loss_and_output_grad(o, w, lhat, fs * 1e3, pb, clip=None)
+    _, g_clip = G.loss_and_output_grad(o, w, lhat, fs * 1e3, pb, clip=5.0)
     mx = np.abs(g_raw).max(axis=1)
     assert (mx > 5.0).any()
     assert np.all(np.abs(g_clip).max(axis=1) <= 5.0 + 1e-9)
@@ -150,7 +190,12 @@ def test_gradient_clip_keeps_direction_and_bounds_max_norm():
         if mx[r] > 5.0:
             assert np.allclose(g_clip[r], g_raw[r] * (5.0 / mx[r]))
         else:
-            assert np.array_equal(g_clip[r], g_raw[r])
+            assert np.allclose(g_clip[r], g_raw[r], rtol=1e-12, atol=0)
+    # relative-L2 part alone (F/p = 0): bounded by L2_CLIP
+    _, g2_raw = G.loss_and_output_grad(o, w, target * 1e4, zero, pb, clip=None, l2=1.0)
+    _, g2 = G.loss_and_output_grad(o, w, target * 1e4, zero, pb, l2=1.0)
+    assert np.abs(g2_raw).max() > G.L2_CLIP
+    assert np.all(np.abs(g2).max(axis=1) <= G.L2_CLIP + 1e-12)
 
 
 def test_invalid_samples_do_not_contribute():
@@ -164,3 +209,17 @@ def test_invalid_samples_do_not_contribute():
     for a, b in zip(dws, dws2):
         assert np.allclose(a, b, rtol=1e-12, atol=1e-15)
     assert np.allclose(dt, dt2, rtol=1e-12, atol=1e-15)
+
+
+def test_guide_cross_entropy_gradient_is_stable_far_from_every_lobe():
+    """A direction no lobe covers (q underflows) still gets a finite gradient
+    that moves weight toward the lobe nearest to it."""
+    o = np.zeros((1, 64))
+    for k in range(G.LOBES):
+        o[0, 5 * k:5 * k + 3] = [0.0, 0.0, 1.0]
+        o[0, 5 * k + 3] = 7.0  # kappa = e^7
+    o[0, 0:3] = [0.0, 0.2, 1.0]  # lobe 0 slightly tilted toward +y
+    w = np.array([[0.0, 1.0, 0.0]])
+    _, g = G.loss_and_output_grad(o, w, np.array([0.0]), np.array([1.0]), np.array([0.1]), clip=None)
+    assert np.all(np.isfinite(g))
+    assert g[0, 4] < 0 and np.all(g[0, 9:40:5] > 0)  # raise lobe 0's weight, lower the others
