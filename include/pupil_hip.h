@@ -241,11 +241,13 @@ typedef struct pt_guide_config {
     uint32_t seed;            /* parameter initialization */
     float learning_rate;      /* Adam step size (1e-2) */
     uint32_t batch_size;      /* samples per optimizer step, multiple of 64 (16384) */
-    uint32_t steps_per_frame; /* optimizer steps per frame rendered with PT_RENDER_GUIDE_TRAIN (4) */
+    uint32_t steps_per_frame; /* optimizer steps per frame rendered with PT_RENDER_GUIDE_TRAIN (8) */
     uint32_t train_stride;    /* one camera path in train_stride records training data (1) */
     uint32_t loss_flags;      /* pt_guide_loss_flags: terms left out of the loss (0 = all) */
     float fixed_selection;    /* > 0: sample the guide with this probability instead of the learned c */
     float l2_weight;          /* weight of the relative-L2 radiance term (0 = default 1e-3) */
+    uint32_t guided_depth;    /* path vertices (from the camera) that may sample the guide; deeper
+                                 vertices sample the BSDF only (0 = all) */
 } pt_guide_config;
 
 enum pt_guide_loss_flags {

@@ -102,7 +102,7 @@ class GuideConfig(C.Structure):
     _fields_ = [("log2_table", C.c_uint32), ("n_min", C.c_uint32), ("n_max", C.c_uint32), ("seed", C.c_uint32),
                 ("learning_rate", C.c_float), ("batch_size", C.c_uint32), ("steps_per_frame", C.c_uint32),
                 ("train_stride", C.c_uint32), ("loss_flags", C.c_uint32), ("fixed_selection", C.c_float),
-                ("l2_weight", C.c_float)]
+                ("l2_weight", C.c_float), ("guided_depth", C.c_uint32)]
 
 
 class GuideInfo(C.Structure):
@@ -360,11 +360,11 @@ class PathTracer:
 
     # ---- learned guiding ---------------------------------------------------------
     def guide_enable(self, log2_table: int = 19, n_min: int = 16, n_max: int = 2048, seed: int = 0,
-                     learning_rate: float = 1e-2, batch_size: int = 16384, steps_per_frame: int = 4,
+                     learning_rate: float = 1e-2, batch_size: int = 16384, steps_per_frame: int = 8,
                      train_stride: int = 1, loss_flags: int = 0, fixed_selection: float = 0.0,
-                     l2_weight: float = 0.0) -> None:
+                     l2_weight: float = 0.0, guided_depth: int = 0) -> None:
         cfg = GuideConfig(log2_table, n_min, n_max, seed, learning_rate, batch_size, steps_per_frame, train_stride,
-                          loss_flags, fixed_selection, l2_weight)
+                          loss_flags, fixed_selection, l2_weight, guided_depth)
         _check(lib().pt_guide_enable(self._ctx, C.byref(cfg)), "pt_guide_enable", self._ctx)
         self._guide_entries = 1 << log2_table
 

@@ -1273,7 +1273,12 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
                                        ext + b, twork + b, ctx->d_stat);
                 t.stop(kKindTrace);
             }
-            if (guide_infer) {
+            // The guide is sampled at vertices b < guided_depth; the vertex of the
+            // last bounce never samples a direction.
+            const uint32_t gdepth = ctx->guide.cfg.guided_depth ? ctx->guide.cfg.guided_depth : max_depth;
+            const bool infer_here = guide_infer && b < gdepth && b + 1 < max_depth;
+            ga.gout = infer_here ? ctx->d_gout : nullptr;
+            if (infer_here) {
                 EventTimer t(ctx, timing);
                 hipLaunchKernelGGL(k_guide_infer, dim3(ctx->num_cu * ctx->infer_blocks_per_cu), dim3(kInferBlock), 0,
                                    ctx->stream, gview, ctx->q[cur], ext + b, reinterpret_cast<uint2 *>(ctx->d_gout));

@@ -3,15 +3,14 @@
 // One optimizer step over a batch of samples is a chain of kernels on the
 // tracer's stream, all driven by the device control block (GuideCtl):
 //   k_guide_begin  : batch size of this step from the collected sample count
-//   k_guide_train  : per wave, 64 samples: hash encoding, MLP forward (MFMA),
+//   k_guide_train  : per wave, 16 samples: hash encoding, MLP forward (MFMA),
 //                    loss and output gradient, MLP backward (MFMA), hash-table
 //                    gradient scatter; stores every layer's input and output
 //                    gradient (FP16) for the weight gradient
-//   k_guide_dw     : dW_l = X_l^T dY_l per batch slice (MFMA, FP32 partials)
-//   k_guide_mid    : Adam step count and bias corrections, or skip
-//   k_guide_adam_* : Adam on the FP32 masters; refreshes the FP16 table
-//   k_guide_end    : dynamic loss scale, loss bookkeeping
-//   k_guide_pack   : FP32 weights -> pre-permuted FP16 MFMA operands
+//   k_guide_dw     : dW_l = X_l^T dY_l per batch slice (MFMA, FP32 atomics)
+//   k_guide_adam_table : sparse Adam on the FP32 table; refreshes the FP16 copy
+//   k_guide_finish : Adam on the MLP weights, re-packing of the FP16 MFMA
+//                    operands, step count, dynamic loss scale, bookkeeping
 // The backward pass runs in FP16 with FP32 accumulation; output gradients are
 // multiplied by a dynamic loss scale, and a step whose gradients are not
 // finite is dropped and the scale halved.
@@ -61,250 +60,238 @@ __global__ void k_guide_begin(GuideCtl *ctl, uint32_t first, uint32_t batch, uin
     ctl->fp_norm = norm;
 }
 
-// Loss of one sample and its gradient with respect to the 41 outputs.  Each
-// part is clipped in max norm on its own (relative L2 to kGuideL2Clip, so one
-// bright sample cannot swamp the shared network; cross-entropy to
-// kGuideGradClip).  Returns the loss.  The radiance
-// model factorizes as Lhat = W q (W = sum_k w_k, q the normalized guide
-// density).  With w = min(F/p / fp_norm, 10):
-//   relative L2          (W sg(q) - L)^2 / (sg(Lhat)^2 + 0.01)  (trains W)
-//   guide cross-entropy  -w log q                               (shapes the lobes)
-//   mixture              -w log(c sg(q) + (1 - c) p)            (trains c)
-// The guide term is differentiated through the lobe responsibilities
-// r_k = w_k v_k / Lhat, formed in the log domain so directions no lobe covers
-// still get finite gradients.
-__device__ float sample_loss_grad(const float *o, f3 w, float target, float fs, float pb, float fp_norm,
-                                  uint32_t flags, float l2_weight, float *go) {
-    GuideMixture m;
-    guide_decode(o, m);
-    float lhat = 0.f, v[kGuideLobes], cosang[kGuideLobes], r[kGuideLobes];
-    float lmax = -3.0e38f;
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) {
-        cosang[k] = dot(m.mu[k], w);
-        v[k] = vmf_pdf(m.kappa[k], cosang[k]);
-        lhat += m.weight[k] * v[k];
-        const float lk = clampf(o[5 * k + 3], kKappaMinLog, kKappaMaxLog);
-        const float lw = clampf(o[5 * k + 4], kWeightMinLog, kWeightMaxLog);
-        r[k] = lw + lk - logf(2.f * kPi) - logf(-expm1f(-2.f * m.kappa[k])) + m.kappa[k] * (cosang[k] - 1.f);
-        lmax = fmaxf(lmax, r[k]);
-    }
-    float rsum = 0.f;
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) {
-        r[k] = expf(r[k] - lmax);
-        rsum += r[k];
-    }
-    const float q = lhat / m.wsum;
-    const float den = lhat * lhat + kRelEps;
-    const float c = m.c;
-    const float pmix = fmaxf(c * q + (1.f - c) * pb, 1e-30f);
-    const float wgt = fminf(fs / fp_norm, 10.f);
-    const float l2w = (flags & PT_GUIDE_NO_RADIANCE_L2) ? 0.f : l2_weight;
-    const float cew = (flags & PT_GUIDE_NO_GUIDE_CE) ? 0.f : wgt;
-    const float mxw = (flags & PT_GUIDE_NO_MIXTURE_CE) ? 0.f : wgt;
-    const float dl = 2.f * (lhat - target) / den;
-    const float loss = l2w * (lhat - target) * (lhat - target) / den - cew * logf(fmaxf(q, 1e-30f)) -
-                       mxw * logf(pmix);
-    const float inv_rsum = 1.f / rsum;
-    // Relative-L2 part (log-weights only), clipped on its own.
-    float g2[kGuideLobes], mx2 = 0.f;
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) {
-        const float lw = o[5 * k + 4];
-        g2[k] = (lw > kWeightMinLog && lw < kWeightMaxLog) ? dl * m.weight[k] * q : 0.f;
-        mx2 = fmaxf(mx2, fabsf(g2[k]));
-    }
-    const float s2 = l2w * (mx2 > kGuideL2Clip ? kGuideL2Clip / mx2 : 1.f);
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) {
-        const float wk = m.weight[k], kk = m.kappa[k], ck = cosang[k];
-        const float lk = o[5 * k + 3], lw = o[5 * k + 4];
-        const float b = -cew * r[k] * inv_rsum;
-        go[5 * k + 4] = (lw > kWeightMinLog && lw < kWeightMaxLog) ? b + cew * wk / m.wsum : 0.f;
-        const float dlogv = 1.f / kk - 2.f / expm1f(2.f * kk) + ck - 1.f;
-        go[5 * k + 3] = (lk > kKappaMinLog && lk < kKappaMaxLog) ? b * dlogv * kk : 0.f;
-        const f3 raw = mk3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
-        const float nr = sqrtf(dot(raw, raw));
-        f3 dr = mk3(0.f);
-        if (nr > 1e-12f) dr = (w - ck * m.mu[k]) * (b * kk / nr);
-        go[5 * k] = dr.x;
-        go[5 * k + 1] = dr.y;
-        go[5 * k + 2] = dr.z;
-    }
-    const float sig = c / kSelectMax;
-    go[40] = -mxw * (q - pb) / pmix * kSelectMax * sig * (1.f - sig);
-    float mx = 0.f;
-#pragma unroll
-    for (int n = 0; n < kGuideUsed; ++n) mx = fmaxf(mx, fabsf(go[n]));
-    if (mx > kGuideGradClip) {
-        const float s = kGuideGradClip / mx;
-#pragma unroll
-        for (int n = 0; n < kGuideUsed; ++n) go[n] *= s;
-    }
-#pragma unroll
-    for (int k = 0; k < kGuideLobes; ++k) go[5 * k + 4] += g2[k] * s2;
-    return loss;
-}
-
 struct TrainBuffers {
     uint16_t *xt;       // [layer][k 64][batch] layer inputs (FP16 bits)
     uint16_t *dyt;      // [layer][n 64][batch] scaled layer output gradients (FP16 bits)
     float2 *table_grad; // [level][T]
 };
 
-// One wave per 64 samples.  samples: 3 float4 per sample (position,
-// direction, {target, F/p, BSDF pdf, weight}); weight 0 marks an unused
-// slot.  Losses and gradients are weighted sums; the Adam kernels divide by
-// the weight total.
+struct LossParams {
+    uint32_t flags;     // pt_guide_loss_flags
+    float l2_weight;
+};
+
+__device__ __forceinline__ float xor_sum4(float v) {
+    v += __shfl_xor(v, 16);
+    return v + __shfl_xor(v, 32);
+}
+__device__ __forceinline__ float xor_max4(float v) {
+    v = fmaxf(v, __shfl_xor(v, 16));
+    return fmaxf(v, __shfl_xor(v, 32));
+}
+
+// One optimization pass over 16 samples per wave (one MFMA tile).  Lane
+// (g, i) = 16 g + i belongs to sample i: it encodes levels {2g, 2g+1, 8+2g,
+// 9+2g} of the sample's position, and in the loss it owns lobes 2g and 2g+1;
+// sums over lobes are reduced across the sample's four lanes (xor 16, 32).
+//
+// Loss per sample (w = min(F/p / fp_norm, 10), Lhat = W q, W = sum_k w_k):
+//   relative L2          l2 (W sg(q) - L)^2 / (sg(Lhat)^2 + 0.01)  (trains W)
+//   guide cross-entropy  -w log q                                  (shapes the lobes)
+//   mixture              -w log(c sg(q) + (1 - c) p_bsdf)          (trains c)
+// The guide term is differentiated through the responsibilities
+// r_k = w_k v_k / Lhat, formed in the log domain, so directions no lobe covers
+// still get finite gradients.  Per sample, the relative-L2 part of the output
+// gradient is clipped in max norm to kGuideL2Clip and the cross-entropy part
+// to kGuideGradClip.  Gradients are weighted by the sample weight and summed;
+// the Adam kernels divide by the weight total.
+//
+// samples: 3 float4 per sample (position, direction, {target, F/p, BSDF pdf,
+// weight}); weight 0 marks an unused slot.
 __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideView G, const float4 *samples, uint32_t batch,
-                                                             GuideCtl *ctl, TrainBuffers B, uint32_t loss_flags,
-                                                             float l2_weight) {
-    __shared__ float stage[kTrainBlock / 64][64 * kGuideUsed];
+                                                             GuideCtl *ctl, TrainBuffers B, LossParams lp) {
+    __shared__ float stage[kTrainBlock / 64][16 * kGuideUsed];
     if (!ctl->active) return;
     const uint32_t nvalid = ctl->batch_valid;
     const int wave = threadIdx.x >> 6;
     const int lane = (int)__lane_id();
     const int g = lane >> 4, i = lane & 15;
-    const uint32_t base = (blockIdx.x * (kTrainBlock / 64) + wave) * 64u;
+    const uint32_t base = (blockIdx.x * (kTrainBlock / 64) + wave) * 16u;
     if (base >= nvalid) return;  // wave-uniform; no block barriers below
     float *st = stage[wave];
     const float scale = ctl->loss_scale;
+    const float fp_norm = ctl->fp_norm;
 
-    const uint32_t me = base + lane;
-    const float my_weight = me < nvalid ? samples[3 * (size_t)me + 2].w : 0.f;
-    const bool my_valid = my_weight > 0.f;
+    const uint32_t me = base + i;
+    const bool in_range = me < nvalid;
     const float4 p4 = samples[3 * (size_t)me];
+    const float4 d4 = samples[3 * (size_t)me + 1];
+    const float4 aux = samples[3 * (size_t)me + 2];
+    const float weight = in_range ? aux.w : 0.f;
+    const bool valid = weight > 0.f;
     const f3 u = guide_normalize_pos(G, mk3(p4.x, p4.y, p4.z));
 
-    // Forward: encode once (kept for the recomputation in the backward pass),
-    // stage the outputs in LDS.
-    half8 in0[4];
+    // Forward (inputs of every layer kept for the backward pass).
+    const half8 in0 = guide_encode(G, u, g);
+    half8 act[4][2];
+    floatx4 out[4];
+    guide_mlp_tile(G, lane, in0, out, act);
 #pragma unroll
-    for (int qt = 0; qt < 4; ++qt) {
-        const int q = qt * 16 + i;
-        const f3 uq = mk3(__shfl(u.x, q), __shfl(u.y, q), __shfl(u.z, q));
-        in0[qt] = guide_encode(G, uq, g);
-        floatx4 out[4];
-        guide_mlp_tile(G, lane, in0[qt], out, nullptr);
+    for (int rt = 0; rt < 3; ++rt)
 #pragma unroll
-        for (int rt = 0; rt < 3; ++rt)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int n = rt * 16 + 4 * g + j;
-                if (n < kGuideUsed) st[q * kGuideUsed + n] = out[rt][j];
-            }
-    }
+        for (int j = 0; j < 4; ++j) {
+            const int n = rt * 16 + 4 * g + j;
+            if (n < kGuideUsed) st[i * kGuideUsed + n] = out[rt][j];
+        }
     wave_sync_lds();
 
-    // Loss and output gradient of this lane's sample.
+    // Loss and output gradient; this lane holds lobes 2g and 2g + 1.
     {
-        float o[kGuideUsed], go[kGuideUsed];
+        const float *o = st + i * kGuideUsed;
+        const f3 w = mk3(d4.x, d4.y, d4.z);
+        float lt[2], kap[2], wt[2], cs[2], lwv[2], lkv[2];
+        f3 mu[2];
+        float lhat_p = 0.f, wsum_p = 0.f, lmax_p = -3.0e38f;
 #pragma unroll
-        for (int n = 0; n < kGuideUsed; ++n) {
-            o[n] = st[lane * kGuideUsed + n];
-            go[n] = 0.f;
+        for (int t = 0; t < 2; ++t) {
+            const int k = 2 * g + t;
+            const f3 raw = mk3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
+            const float n2 = dot(raw, raw);
+            mu[t] = n2 > 1e-24f ? raw * (1.f / sqrtf(n2)) : mk3(0.f, 0.f, 1.f);
+            lkv[t] = o[5 * k + 3];
+            lwv[t] = o[5 * k + 4];
+            const float lkc = clampf(lkv[t], kKappaMinLog, kKappaMaxLog);
+            const float lwc = clampf(lwv[t], kWeightMinLog, kWeightMaxLog);
+            kap[t] = expf(lkc);
+            wt[t] = expf(lwc);
+            cs[t] = dot(mu[t], w);
+            lhat_p += wt[t] * vmf_pdf(kap[t], cs[t]);
+            wsum_p += wt[t];
+            lt[t] = lwc + lkc - logf(2.f * kPi) - logf(-expm1f(-2.f * kap[t])) + kap[t] * (cs[t] - 1.f);
+            lmax_p = fmaxf(lmax_p, lt[t]);
         }
-        float loss = 0.f;
-        if (my_valid) {
-            const float4 d4 = samples[3 * (size_t)me + 1], aux = samples[3 * (size_t)me + 2];
-            loss = sample_loss_grad(o, mk3(d4.x, d4.y, d4.z), aux.x, aux.y, aux.z, ctl->fp_norm, loss_flags,
-                                    l2_weight, go);
+        const float lhat = xor_sum4(lhat_p), wsum = xor_sum4(wsum_p), lmax = xor_max4(lmax_p);
+        const float r0 = expf(lt[0] - lmax), r1 = expf(lt[1] - lmax);
+        const float rsum = xor_sum4(r0 + r1);
+        const float target = aux.x, fs = aux.y, pb = aux.z;
+        const float q = lhat / wsum;
+        const float den = lhat * lhat + kRelEps;
+        const float c = kSelectMax / (1.f + expf(-(o[40] + kSelectBias)));
+        const float pmix = fmaxf(c * q + (1.f - c) * pb, 1e-30f);
+        const float wgt = fminf(fs / fp_norm, 10.f);
+        const float l2w = (lp.flags & PT_GUIDE_NO_RADIANCE_L2) ? 0.f : lp.l2_weight;
+        const float cew = (lp.flags & PT_GUIDE_NO_GUIDE_CE) ? 0.f : wgt;
+        const float mxw = (lp.flags & PT_GUIDE_NO_MIXTURE_CE) ? 0.f : wgt;
+        const float dl = 2.f * (lhat - target) / den;
+        const float loss = l2w * (lhat - target) * (lhat - target) / den - cew * logf(fmaxf(q, 1e-30f)) -
+                           mxw * logf(pmix);
+        const float rr[2] = {r0 / rsum, r1 / rsum};
+        float ge[2][5], g2[2];
+        float mx = 0.f, mx2 = 0.f;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const bool in_w = lwv[t] > kWeightMinLog && lwv[t] < kWeightMaxLog;
+            const bool in_k = lkv[t] > kKappaMinLog && lkv[t] < kKappaMaxLog;
+            g2[t] = in_w ? dl * wt[t] * q : 0.f;
+            const float b = -cew * rr[t];
+            ge[t][4] = in_w ? b + cew * wt[t] / wsum : 0.f;
+            const float dlogv = 1.f / kap[t] - 2.f / expm1f(2.f * kap[t]) + cs[t] - 1.f;
+            ge[t][3] = in_k ? b * dlogv * kap[t] : 0.f;
+            const int k = 2 * g + t;
+            const f3 raw = mk3(o[5 * k], o[5 * k + 1], o[5 * k + 2]);
+            const float nr = sqrtf(dot(raw, raw));
+            f3 dr = mk3(0.f);
+            if (nr > 1e-12f) dr = (w - cs[t] * mu[t]) * (b * kap[t] / nr);
+            ge[t][0] = dr.x;
+            ge[t][1] = dr.y;
+            ge[t][2] = dr.z;
+#pragma unroll
+            for (int e = 0; e < 5; ++e) mx = fmaxf(mx, fabsf(ge[t][e]));
+            mx2 = fmaxf(mx2, fabsf(g2[t]));
         }
+        const float sig = c / kSelectMax;
+        const float g40 = -mxw * (q - pb) / pmix * kSelectMax * sig * (1.f - sig);
+        mx = fmaxf(xor_max4(mx), fabsf(g40));
+        mx2 = xor_max4(mx2);
+        const float s1 = mx > kGuideGradClip ? kGuideGradClip / mx : 1.f;
+        const float s2 = l2w * (mx2 > kGuideL2Clip ? kGuideL2Clip / mx2 : 1.f);
+        const float ws = valid ? weight * scale : 0.f;
         wave_sync_lds();
-        const float gs = my_valid ? my_weight * scale : 0.f;
 #pragma unroll
-        for (int n = 0; n < kGuideUsed; ++n) st[lane * kGuideUsed + n] = go[n] * gs;
-        float ls = my_valid ? loss * my_weight : 0.f, vc = my_valid ? my_weight : 0.f;
+        for (int t = 0; t < 2; ++t) {
+            const int k = 2 * g + t;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
+            for (int e = 0; e < 4; ++e) st[i * kGuideUsed + 5 * k + e] = ge[t][e] * s1 * ws;
+            st[i * kGuideUsed + 5 * k + 4] = (ge[t][4] * s1 + g2[t] * s2) * ws;
+        }
+        if (g == 0) st[i * kGuideUsed + 40] = g40 * s1 * ws;
+        // weighted loss and weight total (counted once per sample, by g == 0)
+        float ls = (valid && g == 0) ? loss * weight : 0.f, vc = (valid && g == 0) ? weight : 0.f;
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) {
             ls += __shfl_xor(ls, off);
             vc += __shfl_xor(vc, off);
         }
-        if (lane == 0) {
+        if (lane == 0 && vc > 0.f) {
             unsafeAtomicAdd(&ctl->loss_sum, ls);
             unsafeAtomicAdd(&ctl->loss_cnt, vc);
         }
     }
     wave_sync_lds();
 
-    // Backward, tile by tile; the tile's layer inputs are recomputed from
-    // its encoding (MFMA work is cheap next to keeping them in registers).
-    // Gradients are summed over the batch here; the Adam kernels divide by
-    // the number of valid samples.
+    // Backward.
     const float inv_scale = 1.f / scale;
+    const uint32_t qg = base + i;
+    half8 dy[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int n = guide_perm(c, g, s);
+            dy[c][s] = (_Float16)(n < kGuideUsed ? st[i * kGuideUsed + n] : 0.f);
+        }
+    const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
     bool bad = false;
 #pragma unroll
-    for (int qt = 0; qt < 4; ++qt) {
-        const int q = qt * 16 + i;
-        const uint32_t qg = base + q;
-        half8 act[4][2];
-        {
-            floatx4 out[4];
-            guide_mlp_tile(G, lane, in0[qt], out, act);
-        }
-        half8 dy[2];
+    for (int l = 3; l >= 0; --l) {
 #pragma unroll
         for (int c = 0; c < 2; ++c)
 #pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                const int n = guide_perm(c, g, s);
-                dy[c][s] = (_Float16)(n < kGuideUsed ? st[q * kGuideUsed + n] : 0.f);
-            }
-        const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+            for (int s = 0; s < 8; ++s)
+                B.dyt[((size_t)l * 64 + guide_perm(c, g, s)) * batch + qg] = hbits(dy[c][s]);
 #pragma unroll
-        for (int l = 3; l >= 0; --l) {
+        for (int c = 0; c < 2; ++c) {
+            if (l == 0 && c == 1) break;
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+                B.xt[((size_t)l * 64 + guide_perm(c, g, s)) * batch + qg] = hbits(act[l][c][s]);
+        }
+        // dX_l^T = W_l dY_l^T
+        floatx4 dx[4];
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) {
+            if (l == 0 && kt >= 2) break;
+            dx[kt] = mfma16(bwd_operand(G, l, kt, 0, lane), dy[0], zero);
+            dx[kt] = mfma16(bwd_operand(G, l, kt, 1, lane), dy[1], dx[kt]);
+        }
+        if (l > 0) {
+            // ReLU of the layer input (the previous layer's output).
 #pragma unroll
             for (int c = 0; c < 2; ++c)
 #pragma unroll
-                for (int s = 0; s < 8; ++s)
-                    B.dyt[((size_t)l * 64 + guide_perm(c, g, s)) * batch + qg] = hbits(dy[c][s]);
+                for (int s = 0; s < 8; ++s) {
+                    const float gv = dx[2 * c + (s >> 2)][s & 3];
+                    dy[c][s] = (_Float16)((float)act[l][c][s] > 0.f ? gv : 0.f);
+                }
+        } else if (valid) {
+            // Hash-table gradient of this lane's four levels.
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                if (l == 0 && c == 1) break;
+            for (int t = 0; t < 4; ++t) {
+                const float g0 = dx[t >> 1][2 * (t & 1)] * inv_scale;
+                const float g1 = dx[t >> 1][2 * (t & 1) + 1] * inv_scale;
+                if (g0 == 0.f && g1 == 0.f) continue;
+                if (!isfinite(g0) || !isfinite(g1)) {
+                    bad = true;
+                    continue;
+                }
+                const int level = guide_lane_level(g, t);
+                LevelCorners lc;
+                level_corners(G, level, u, lc);
+                float2 *tg = B.table_grad + ((size_t)level << G.log2t);
 #pragma unroll
-                for (int s = 0; s < 8; ++s)
-                    B.xt[((size_t)l * 64 + guide_perm(c, g, s)) * batch + qg] = hbits(act[l][c][s]);
-            }
-            // dX_l^T = W_l dY_l^T
-            floatx4 dx[4];
-#pragma unroll
-            for (int kt = 0; kt < 4; ++kt) {
-                if (l == 0 && kt >= 2) break;
-                dx[kt] = mfma16(bwd_operand(G, l, kt, 0, lane), dy[0], zero);
-                dx[kt] = mfma16(bwd_operand(G, l, kt, 1, lane), dy[1], dx[kt]);
-            }
-            if (l > 0) {
-                // ReLU of the layer input (the previous layer's output).
-#pragma unroll
-                for (int c = 0; c < 2; ++c)
-#pragma unroll
-                    for (int s = 0; s < 8; ++s) {
-                        const float gv = dx[2 * c + (s >> 2)][s & 3];
-                        dy[c][s] = (_Float16)((float)act[l][c][s] > 0.f ? gv : 0.f);
-                    }
-            } else {
-                // Hash-table gradient of this lane's four levels of query q.
-                const f3 uq = mk3(__shfl(u.x, q), __shfl(u.y, q), __shfl(u.z, q));
-                const bool qvalid = __shfl((int)my_valid, q) != 0;
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const float g0 = dx[t >> 1][2 * (t & 1)] * inv_scale;
-                    const float g1 = dx[t >> 1][2 * (t & 1) + 1] * inv_scale;
-                    if (!qvalid || (g0 == 0.f && g1 == 0.f)) continue;
-                    if (!isfinite(g0) || !isfinite(g1)) {
-                        bad = true;
-                        continue;
-                    }
-                    const int level = guide_lane_level(g, t);
-                    LevelCorners lc;
-                    level_corners(G, level, uq, lc);
-                    float2 *tg = B.table_grad + ((size_t)level << G.log2t);
-#pragma unroll
-                    for (int k = 0; k < 8; ++k) {
-                        unsafeAtomicAdd(&tg[lc.idx[k]].x, lc.w[k] * g0);
-                        unsafeAtomicAdd(&tg[lc.idx[k]].y, lc.w[k] * g1);
-                    }
+                for (int k = 0; k < 8; ++k) {
+                    unsafeAtomicAdd(&tg[lc.idx[k]].x, lc.w[k] * g0);
+                    unsafeAtomicAdd(&tg[lc.idx[k]].y, lc.w[k] * g1);
                 }
             }
         }
@@ -312,11 +299,12 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideView G, const 
     if (bad) atomicOr(&ctl->bad, 1u);
 }
 
-// Partial dW_l[k][n] = sum_q X_l[q][k] dY_l[q][n] over one batch slice; one
-// wave per (slice, layer).  A[i][8g+s] = X_l[q + 8g + s][16 kt + i],
-// B[8g+s][i] = dY_l[q + 8g + s][16 nt + i], D[4g+j][i] = dW[16 kt + 4g + j][16 nt + i].
+// dW_l[k][n] += sum_q X_l[q][k] dY_l[q][n] over one batch slice (kGuideDwSlice
+// samples), one wave per (slice, layer); partial sums are added atomically.
+// A[i][8g+s] = X_l[q + 8g + s][16 kt + i], B[8g+s][i] = dY_l[q + 8g + s][16 nt + i],
+// D[4g+j][i] = dW[16 kt + 4g + j][16 nt + i].
 __global__ __launch_bounds__(64) void k_guide_dw(const uint16_t *xt, const uint16_t *dyt, uint32_t batch,
-                                                 GuideCtl *ctl, float *w_part) {
+                                                 GuideCtl *ctl, float *w_grad) {
     if (!ctl->active) return;
     const uint32_t rows = (ctl->batch_valid + 63u) & ~63u;
     const uint32_t q0 = blockIdx.x * kGuideDwSlice;
@@ -347,28 +335,24 @@ __global__ __launch_bounds__(64) void k_guide_dw(const uint16_t *xt, const uint1
         }
     }
     const float s = 1.f / ctl->loss_scale;
-    float *W = w_part + ((size_t)blockIdx.x * 4 + layer) * 64 * 64;
+    float *W = w_grad + (size_t)layer * 64 * 64;
     bool nonfinite = false;
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt)
+    for (int kt = 0; kt < 4; ++kt) {
+        if (kt >= ktiles) break;
 #pragma unroll
         for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float v = kt < ktiles ? acc[kt][nt][j] * s : 0.f;
-                if (!isfinite(v)) nonfinite = true;
-                W[(kt * 16 + 4 * g + j) * 64 + nt * 16 + i] = v;
+                const float v = acc[kt][nt][j] * s;
+                if (!isfinite(v)) {
+                    nonfinite = true;
+                    continue;
+                }
+                if (v != 0.f) unsafeAtomicAdd(&W[(kt * 16 + 4 * g + j) * 64 + nt * 16 + i], v);
             }
+    }
     if (nonfinite) atomicOr(&ctl->bad, 1u);
-}
-
-__global__ void k_guide_mid(GuideCtl *ctl, float b1, float b2) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (!ctl->active || ctl->bad) return;
-    const uint32_t t = ctl->step + 1u;
-    ctl->step = t;
-    ctl->c1 = 1.f - powf(b1, (float)t);
-    ctl->c2 = 1.f - powf(b2, (float)t);
 }
 
 struct AdamParams {
@@ -380,6 +364,16 @@ __device__ __forceinline__ float inv_valid(const GuideCtl *ctl) {
     return ctl->loss_cnt > 0.f ? 1.f / ctl->loss_cnt : 0.f;
 }
 
+// Whether this step applies, and its Adam bias corrections (step count t =
+// ctl->step + 1; the count itself is advanced by k_guide_finish).
+__device__ __forceinline__ bool adam_step_of(const GuideCtl *ctl, uint32_t apply, const AdamParams &a, float &c1,
+                                            float &c2) {
+    const float t = (float)(ctl->step + 1u);
+    c1 = 1.f - powf(a.b1, t);
+    c2 = 1.f - powf(a.b2, t);
+    return apply && !ctl->bad;
+}
+
 // Sparse Adam on the hash table (FP32 master + FP16 copy) with the
 // batch-summed gradient divided by the sample-weight total: entries no sample
 // touched (gradient exactly zero) keep their values and moments.  Gradients
@@ -387,8 +381,8 @@ __device__ __forceinline__ float inv_valid(const GuideCtl *ctl) {
 __global__ void k_guide_adam_table(float2 *p, float2 *m, float2 *v, float2 *g, uint32_t *p16, size_t n, AdamParams a,
                                    const GuideCtl *ctl, uint32_t apply) {
     if (!ctl->active) return;
-    const bool step = apply && !ctl->bad;
-    const float c1 = ctl->c1, c2 = ctl->c2;
+    float c1, c2;
+    const bool step = adam_step_of(ctl, apply, a, c1, c2);
     const float inv = inv_valid(ctl);
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
         float2 gg = g[e];
@@ -411,52 +405,73 @@ __global__ void k_guide_adam_table(float2 *p, float2 *m, float2 *v, float2 *g, u
     }
 }
 
-// Adam on the MLP weights; the gradient is the sum of the slice partials
-// divided by the valid sample count.
-__global__ void k_guide_adam_weights(float *p, float *m, float *v, const float *part, AdamParams a,
-                                     const GuideCtl *ctl, uint32_t apply, float *grad_out) {
-    if (!ctl->active) return;
-    const uint32_t rows = (ctl->batch_valid + 63u) & ~63u;
-    const uint32_t slices = (rows + kGuideDwSlice - 1) / kGuideDwSlice;
-    const bool step = apply && !ctl->bad;
-    const float c1 = ctl->c1, c2 = ctl->c2;
+constexpr int kFinishBlock = 1024;
+
+// Single block: Adam on the MLP weights (gradient = w_grad / weight total;
+// w_grad is cleared), then the FP16 MFMA operands are re-packed and the step
+// is closed (step count, dynamic loss scale, loss bookkeeping).
+//   fwd[layer][rt][chunk][lane][s] = W[perm(chunk, g, s)][16 rt + i]
+//   bwd[layer][kt][chunk][lane][s] = W[16 kt + i][perm(chunk, g, s)]
+// apply = 0 only reports the gradient (grad_out) and closes the step.
+__global__ __launch_bounds__(kFinishBlock) void k_guide_finish(float *p, float *m, float *v, float *w_grad,
+                                                              AdamParams a, GuideCtl *ctl, uint32_t apply,
+                                                              float *grad_out, _Float16 *fwd, _Float16 *bwd) {
+    const bool active = ctl->active != 0u;
+    float c1, c2;
+    const bool step = active && adam_step_of(ctl, apply, a, c1, c2);
     const float inv = inv_valid(ctl);
     const uint32_t n = 4 * 64 * 64;
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
-        float gg = 0.f;
-        for (uint32_t s = 0; s < slices; ++s) gg += part[(size_t)s * n + e];
-        gg *= inv;
-        if (grad_out) grad_out[e] = gg;
-        if (!step) continue;
-        const float mm = a.b1 * m[e] + (1.f - a.b1) * gg;
-        const float vv = a.b2 * v[e] + (1.f - a.b2) * gg * gg;
-        m[e] = mm;
-        v[e] = vv;
-        p[e] -= a.lr * (mm / c1) / (sqrtf(vv / c2) + a.eps);
-    }
-}
-
-__global__ void k_guide_end(GuideCtl *ctl) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (ctl->active) {
-        if (ctl->bad) {
-            ctl->loss_scale = fmaxf(1.f, ctl->loss_scale * 0.5f);
-            ctl->good = 0u;
-            ctl->skipped += 1u;
-        } else if (++ctl->good >= kGuideScaleGrowth) {
-            ctl->loss_scale = fminf(kGuideScaleMax, ctl->loss_scale * 2.f);
-            ctl->good = 0u;
+    if (active) {
+        for (uint32_t e = threadIdx.x; e < n; e += kFinishBlock) {
+            const float gg = w_grad[e] * inv;
+            w_grad[e] = 0.f;
+            if (grad_out) grad_out[e] = gg;
+            if (!step) continue;
+            const float mm = a.b1 * m[e] + (1.f - a.b1) * gg;
+            const float vv = a.b2 * v[e] + (1.f - a.b2) * gg * gg;
+            m[e] = mm;
+            v[e] = vv;
+            p[e] -= a.lr * (mm / c1) / (sqrtf(vv / c2) + a.eps);
         }
-        ctl->last_loss = ctl->loss_cnt > 0.f ? ctl->loss_sum / ctl->loss_cnt : 0.f;
-        ctl->last_count = ctl->batch_valid;
     }
-    ctl->bad = 0u;
-    ctl->active = 0u;
+    __syncthreads();
+    if (step) {
+        for (uint32_t t = threadIdx.x; t < 4 * 4 * 2 * 64 * 8; t += kFinishBlock) {
+            const int s = t & 7, lane = (t >> 3) & 63, chunk = (t >> 9) & 1, tile = (t >> 10) & 3, layer = t >> 12;
+            const int g = lane >> 4, i = lane & 15;
+            const int in_dim = layer == 0 ? kGuideInputs : kGuideHidden;
+            const float *W = p + layer * 64 * 64;
+            const int kf = guide_perm(chunk, g, s);
+            fwd[t] = (_Float16)(kf < in_dim ? W[kf * 64 + tile * 16 + i] : 0.f);
+            const int kb = tile * 16 + i;
+            bwd[t] = (_Float16)(kb < in_dim ? W[kb * 64 + kf] : 0.f);
+        }
+    }
+    if (threadIdx.x == 0) {
+        if (active && apply) {
+            if (ctl->bad) {
+                ctl->loss_scale = fmaxf(1.f, ctl->loss_scale * 0.5f);
+                ctl->good = 0u;
+                ctl->skipped += 1u;
+            } else {
+                ctl->step += 1u;
+                if (++ctl->good >= kGuideScaleGrowth) {
+                    ctl->loss_scale = fminf(kGuideScaleMax, ctl->loss_scale * 2.f);
+                    ctl->good = 0u;
+                }
+            }
+        }
+        if (active) {
+            ctl->last_loss = ctl->loss_cnt > 0.f ? ctl->loss_sum / ctl->loss_cnt : 0.f;
+            ctl->last_count = ctl->batch_valid;
+        }
+        ctl->bad = 0u;
+        ctl->active = 0u;
+    }
 }
 
-// FP32 weights [layer][in 64][out 64] -> FP16 operands
-// fwd[layer][rt][chunk][lane][s] = W[perm(chunk, g, s)][16 rt + i]
-// bwd[layer][kt][chunk][lane][s] = W[16 kt + i][perm(chunk, g, s)]
+// Stand-alone packing of the FP16 operands (parameter initialization and
+// pt_guide_set_params).
 __global__ void k_guide_pack(const float *w, _Float16 *fwd, _Float16 *bwd) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= 4 * 4 * 2 * 64 * 8) return;
@@ -503,18 +518,6 @@ __global__ void k_guide_collect_begin(GuideCtl *ctl) {
 
 __global__ void k_guide_collect_end(GuideCtl *ctl, uint32_t capacity) {
     if (threadIdx.x == 0 && blockIdx.x == 0) ctl->total_samples += min(ctl->n_samples, capacity);
-}
-
-__device__ __forceinline__ uint32_t wave_alloc_g(bool pred, uint32_t *counter) {
-    const unsigned long long mask = __ballot(pred);
-    if (mask == 0ull) return 0u;
-    const uint32_t lane = __lane_id();
-    const int leader = __ffsll((long long)mask) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    const unsigned long long lt = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
-    return base + (uint32_t)__popcll(lt);
 }
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
@@ -581,6 +584,7 @@ __global__ __launch_bounds__(256) void k_guide_collect_count(const float4 *out_l
 // probability, so weighted batch means estimate means over all records.
 __global__ __launch_bounds__(256) void k_guide_collect(const float4 *out_l, uint32_t total, GuideRecords r,
                                                        float4 *samples, uint32_t capacity, GuideCtl *ctl) {
+    __shared__ uint32_t wcount[4], wbase[4];
     const uint32_t n = r.capacity;
     const uint32_t nnz = ctl->n_nonzero, nz = ctl->n_zero;
     const uint32_t cap_nz = capacity - capacity / 4u;
@@ -588,6 +592,7 @@ __global__ __launch_bounds__(256) void k_guide_collect(const float4 *out_l, uint
     const uint32_t kept_nz = min(nnz, cap_nz);
     const uint32_t cap_z = capacity - kept_nz;
     const float keep_z = nz > cap_z ? (float)cap_z / (float)nz : 1.f;
+    float fs_acc = 0.f, fc_acc = 0.f;  // summed per lane, one atomic per wave at the end
     for (uint32_t b = blockIdx.x * blockDim.x; b < n; b += gridDim.x * blockDim.x) {
         const uint32_t idx = b + threadIdx.x;
         bool ok = false;
@@ -604,24 +609,40 @@ __global__ __launch_bounds__(256) void k_guide_collect(const float4 *out_l, uint
                 }
             }
         }
-        const uint32_t k = wave_alloc_g(ok, &ctl->n_samples);
+        // Slots for the block's kept samples: one global atomic per block
+        // and iteration (wave counts combined in LDS).
+        const unsigned long long mask = __ballot(ok);
+        const uint32_t wave = threadIdx.x >> 6, lane = __lane_id();
+        if (lane == 0) wcount[wave] = (uint32_t)__popcll(mask);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t t = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+            const uint32_t b0 = t ? atomicAdd(&ctl->n_samples, t) : 0u;
+            wbase[0] = b0;
+            wbase[1] = b0 + wcount[0];
+            wbase[2] = b0 + wcount[0] + wcount[1];
+            wbase[3] = b0 + wcount[0] + wcount[1] + wcount[2];
+        }
+        __syncthreads();
+        const unsigned long long below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+        const uint32_t k = wbase[wave] + (uint32_t)__popcll(below);
         const bool kept = ok && k < capacity;
-        float fs = kept ? fpv * s2.w : 0.f;
-        float fc = (kept && fpv > 0.f) ? s2.w : 0.f;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-            fs += __shfl_xor(fs, off);
-            fc += __shfl_xor(fc, off);
-        }
-        if (__lane_id() == 0 && fs != 0.f) {
-            unsafeAtomicAdd(&ctl->fp_sum, fs);
-            unsafeAtomicAdd(&ctl->fp_cnt, fc);
-        }
+        fs_acc += kept ? fpv * s2.w : 0.f;
+        fc_acc += (kept && fpv > 0.f) ? s2.w : 0.f;
         if (ok && k < capacity) {
             samples[3 * (size_t)k] = s0;
             samples[3 * (size_t)k + 1] = s1;
             samples[3 * (size_t)k + 2] = s2;
         }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        fs_acc += __shfl_xor(fs_acc, off);
+        fc_acc += __shfl_xor(fc_acc, off);
+    }
+    if (__lane_id() == 0 && fs_acc != 0.f) {
+        unsafeAtomicAdd(&ctl->fp_sum, fs_acc);
+        unsafeAtomicAdd(&ctl->fp_cnt, fc_acc);
     }
 }
 
@@ -661,7 +682,7 @@ void GuideState::release() {
     gfree(w32);
     gfree(w_m);
     gfree(w_v);
-    gfree(w_part);
+    gfree(w_grad);
     gfree(fwd_a);
     gfree(bwd_a);
     gfree(ctl);
@@ -695,7 +716,7 @@ bool GuideState::init(const pt_guide_config &c, const float lo[3], const float h
         err = "pt_guide_enable: batch_size must be a multiple of 64, at most 2^22";
         return false;
     }
-    if (cfg.steps_per_frame == 0) cfg.steps_per_frame = 4;
+    if (cfg.steps_per_frame == 0) cfg.steps_per_frame = 8;
     if (cfg.train_stride == 0) cfg.train_stride = 1;
     if (!(cfg.l2_weight > 0.f)) cfg.l2_weight = 1e-3f;
     // Level resolutions floor(n_min * b^l), b = (n_max / n_min)^(1 / 15), in double.
@@ -727,6 +748,7 @@ bool GuideState::init(const pt_guide_config &c, const float lo[3], const float h
     GUIDE_HIP(hipMalloc((void **)&w32, nw * sizeof(float)));
     GUIDE_HIP(hipMalloc((void **)&w_m, nw * sizeof(float)));
     GUIDE_HIP(hipMalloc((void **)&w_v, nw * sizeof(float)));
+    GUIDE_HIP(hipMalloc((void **)&w_grad, nw * sizeof(float)));
     GUIDE_HIP(hipMalloc((void **)&fwd_a, 4 * 4 * 2 * 64 * 8 * sizeof(_Float16)));
     GUIDE_HIP(hipMalloc((void **)&bwd_a, 4 * 4 * 2 * 64 * 8 * sizeof(_Float16)));
     GUIDE_HIP(hipMalloc((void **)&ctl, sizeof(GuideCtl)));
@@ -750,6 +772,7 @@ bool GuideState::init(const pt_guide_config &c, const float lo[3], const float h
     GUIDE_HIP(hipMemset(table_g, 0, nt * sizeof(float2)));
     GUIDE_HIP(hipMemset(w_m, 0, nw * sizeof(float)));
     GUIDE_HIP(hipMemset(w_v, 0, nw * sizeof(float)));
+    GUIDE_HIP(hipMemset(w_grad, 0, nw * sizeof(float)));
     GuideCtl hc;
     std::memset(&hc, 0, sizeof(hc));
     hc.loss_scale = kGuideScaleInit;
@@ -789,14 +812,11 @@ bool GuideState::ensure_batch(uint32_t batch, std::string &err) {
     if (batch <= batch_cap) return true;
     gfree(xt);
     gfree(dyt);
-    gfree(w_part);
     const size_t n = (size_t)4 * 64 * batch;
     GUIDE_HIP(hipMalloc((void **)&xt, n * sizeof(uint16_t)));
     GUIDE_HIP(hipMalloc((void **)&dyt, n * sizeof(uint16_t)));
     GUIDE_HIP(hipMemset(xt, 0, n * sizeof(uint16_t)));
     GUIDE_HIP(hipMemset(dyt, 0, n * sizeof(uint16_t)));
-    const size_t slices = (batch + kGuideDwSlice - 1) / kGuideDwSlice;
-    GUIDE_HIP(hipMalloc((void **)&w_part, slices * 4 * 64 * 64 * sizeof(float)));
     batch_cap = batch;
     return true;
 }
@@ -823,22 +843,17 @@ bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool
     const GuideView G = view();
     hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, first, batch, sample_cap, fp_norm);
     TrainBuffers B{xt, dyt, table_g};
-    const uint32_t waves = batch / 64;
+    const uint32_t waves = batch / 16;
     hipLaunchKernelGGL(k_guide_train, dim3((waves + 3) / 4), dim3(kTrainBlock), 0, stream, G,
-                       samples + 3 * (size_t)first, batch, ctl, B, cfg.loss_flags, cfg.l2_weight);
+                       samples + 3 * (size_t)first, batch, ctl, B, LossParams{cfg.loss_flags, cfg.l2_weight});
     hipLaunchKernelGGL(k_guide_dw, dim3((batch + kGuideDwSlice - 1) / kGuideDwSlice, 4), dim3(64), 0, stream, xt,
-                       dyt, batch, ctl, w_part);
+                       dyt, batch, ctl, w_grad);
     const AdamParams at{cfg.learning_rate, 0.9f, 0.99f, 1e-15f};
     const AdamParams aw{cfg.learning_rate, 0.9f, 0.99f, 1e-8f};
-    if (apply) hipLaunchKernelGGL(k_guide_mid, dim3(1), dim3(64), 0, stream, ctl, at.b1, at.b2);
     hipLaunchKernelGGL(k_guide_adam_table, dim3(2048), dim3(256), 0, stream, table32, table_m, table_v, table_g,
                        table16, table_entries, at, ctl, apply ? 1u : 0u);
-    if (apply) {
-        hipLaunchKernelGGL(k_guide_adam_weights, dim3(16), dim3(256), 0, stream, w32, w_m, w_v, w_part, aw, ctl, 1u,
-                           (float *)nullptr);
-        hipLaunchKernelGGL(k_guide_end, dim3(1), dim3(64), 0, stream, ctl);
-        if (!pack(err)) return false;
-    }
+    hipLaunchKernelGGL(k_guide_finish, dim3(1), dim3(kFinishBlock), 0, stream, w32, w_m, w_v, w_grad, aw, ctl,
+                       apply ? 1u : 0u, (float *)nullptr, fwd_a, bwd_a);
     GUIDE_HIP(hipGetLastError());
     ++steps_launched;
     return true;
@@ -872,44 +887,43 @@ bool GuideState::upload_samples(const float *host, uint32_t n, std::string &err)
     return true;
 }
 
-bool GuideState::gradients(uint32_t n, float fp_norm, float *table_grad, float *w_grad, float *loss,
+bool GuideState::gradients(uint32_t n, float fp_norm, float *table_grad, float *weight_grad, float *loss,
                            std::string &err) {
     if (!ensure_batch(n, err)) return false;
     const GuideView G = view();
     hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, 0u, n, sample_cap, fp_norm);
     TrainBuffers B{xt, dyt, table_g};
-    hipLaunchKernelGGL(k_guide_train, dim3((n / 64 + 3) / 4), dim3(kTrainBlock), 0, stream, G, samples, n, ctl, B,
-                       cfg.loss_flags, cfg.l2_weight);
+    hipLaunchKernelGGL(k_guide_train, dim3((n / 16 + 3) / 4), dim3(kTrainBlock), 0, stream, G, samples, n, ctl, B,
+                       LossParams{cfg.loss_flags, cfg.l2_weight});
     hipLaunchKernelGGL(k_guide_dw, dim3((n + kGuideDwSlice - 1) / kGuideDwSlice, 4), dim3(64), 0, stream, xt, dyt,
-                       n, ctl, w_part);
+                       n, ctl, w_grad);
+    GUIDE_HIP(hipGetLastError());
+    GUIDE_HIP(hipStreamSynchronize(stream));
+    GuideCtl hc;
+    GUIDE_HIP(hipMemcpy(&hc, ctl, sizeof(hc), hipMemcpyDeviceToHost));
+    if (table_grad) {
+        GUIDE_HIP(hipMemcpy(table_grad, table_g, table_entries * sizeof(float2), hipMemcpyDeviceToHost));
+        const float inv = hc.loss_cnt > 0.f ? 1.f / hc.loss_cnt : 0.f;
+        for (size_t k = 0; k < 2 * table_entries; ++k) table_grad[k] *= inv;
+    }
+    if (loss) *loss = hc.loss_cnt > 0.f ? hc.loss_sum / hc.loss_cnt : 0.f;
+    // Report the weight gradient and close the step without applying it;
+    // the table gradient is cleared, the parameters are untouched.
     float *d_wg = nullptr;
     GUIDE_HIP(hipMalloc((void **)&d_wg, 4 * 64 * 64 * sizeof(float)));
     const AdamParams aw{cfg.learning_rate, 0.9f, 0.99f, 1e-8f};
-    hipLaunchKernelGGL(k_guide_adam_weights, dim3(16), dim3(256), 0, stream, w32, w_m, w_v, w_part, aw, ctl, 0u,
-                       d_wg);
+    hipLaunchKernelGGL(k_guide_finish, dim3(1), dim3(kFinishBlock), 0, stream, w32, w_m, w_v, w_grad, aw, ctl, 0u,
+                       d_wg, fwd_a, bwd_a);
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemsetAsync(table_g, 0, table_entries * sizeof(float2), stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
-    if (e == hipSuccess && table_grad)
-        e = hipMemcpy(table_grad, table_g, table_entries * sizeof(float2), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && w_grad) e = hipMemcpy(w_grad, d_wg, 4 * 64 * 64 * sizeof(float), hipMemcpyDeviceToHost);
-    GuideCtl hc;
-    std::memset(&hc, 0, sizeof(hc));
-    if (e == hipSuccess) e = hipMemcpy(&hc, ctl, sizeof(hc), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && weight_grad)
+        e = hipMemcpy(weight_grad, d_wg, 4 * 64 * 64 * sizeof(float), hipMemcpyDeviceToHost);
     (void)hipFree(d_wg);
     if (e != hipSuccess) {
         err = std::string("pt_guide_gradients: ") + hipGetErrorString(e);
         return false;
     }
-    if (loss) *loss = hc.loss_cnt > 0.f ? hc.loss_sum / hc.loss_cnt : 0.f;
-    if (table_grad && hc.loss_cnt > 0.f) {
-        const float inv = 1.f / hc.loss_cnt;
-        for (size_t k = 0; k < 2 * table_entries; ++k) table_grad[k] *= inv;
-    }
-    // Clear the gradient and the step state; the parameters are untouched.
-    GUIDE_HIP(hipMemsetAsync(table_g, 0, table_entries * sizeof(float2), stream));
-    hipLaunchKernelGGL(k_guide_end, dim3(1), dim3(64), 0, stream, ctl);
-    GUIDE_HIP(hipGetLastError());
-    GUIDE_HIP(hipStreamSynchronize(stream));
     if (hc.bad) {
         err = "pt_guide_gradients: non-finite gradient";
         return false;

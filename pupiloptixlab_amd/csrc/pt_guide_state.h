@@ -20,7 +20,7 @@ constexpr float kGuideL2Clip = 1.f;       // ... relative-L2 part
 constexpr float kGuideScaleInit = 32.f;   // initial loss scale of the FP16 backward pass
 constexpr float kGuideScaleMax = 32768.f;
 constexpr uint32_t kGuideScaleGrowth = 1000;  // finite steps before the scale doubles
-constexpr uint32_t kGuideDwSlice = 2048;  // samples per weight-gradient partial
+constexpr uint32_t kGuideDwSlice = 512;   // samples per weight-gradient partial
 
 struct GuideCtl {
     float loss_scale;
@@ -72,7 +72,7 @@ public:
     // parameters and optimizer moments
     uint32_t *table16 = nullptr;  // FP16 feature pairs used by the kernels
     float2 *table32 = nullptr, *table_m = nullptr, *table_v = nullptr, *table_g = nullptr;
-    float *w32 = nullptr, *w_m = nullptr, *w_v = nullptr, *w_part = nullptr;
+    float *w32 = nullptr, *w_m = nullptr, *w_v = nullptr, *w_grad = nullptr;
     _Float16 *fwd_a = nullptr, *bwd_a = nullptr;
     GuideCtl *ctl = nullptr;
 
@@ -104,7 +104,7 @@ public:
     bool train(uint32_t steps, float fp_norm, std::string &err);
     // Host-sample entry points (tests / external trainers).
     bool upload_samples(const float *host, uint32_t n, std::string &err);
-    bool gradients(uint32_t n, float fp_norm, float *table_grad, float *w_grad, float *loss, std::string &err);
+    bool gradients(uint32_t n, float fp_norm, float *table_grad, float *weight_grad, float *loss, std::string &err);
     bool get_params(float *table, float *weights, std::string &err);
     bool set_params(const float *table, const float *weights, std::string &err);
     bool query(const float *pos, uint32_t n, float *out, std::string &err);

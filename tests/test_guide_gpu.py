@@ -259,3 +259,35 @@ def test_guided_errors(tracer):
         tracer.guide_train(np.zeros((100, 12), np.float32))
     with pytest.raises(pt.PupilError):
         tracer.guide_enable(log2_table=30)
+
+
+def test_guided_door_room_gains_over_plain(tracer):
+    """Hard indirect lighting (the room lit through a 5 cm door gap), reduced
+    to 480x270: after 64 training frames the guided estimator reaches less
+    than half the relMSE of plain BSDF sampling at the same sample count
+    (measured against a 4096-spp FP64 plain render).  Training samples are
+    well formed."""
+    print("case: door room 480x270, plain 256 spp vs guided (train 64) 256 spp, ref 4096", flush=True)
+    scene = pt.HostScene.generate(pt.SCENE_DOOR_ROOM, width=480, height=270)
+    tracer.set_scene(scene)
+    tracer.build_bvh()
+    tracer.render(spp=4096, mis_mode=pt.MIS_CONSISTENT, fp64=True, aov=False)
+    ref = tracer.read(pt.BUF_ACCUM64) / 4096
+    tracer.reset()
+    tracer.render(spp=256, mis_mode=pt.MIS_CONSISTENT, aov=False)
+    plain = relmse(tracer.read(pt.BUF_ACCUM)[..., :3].astype(np.float64), ref)
+    tracer.reset()
+    tracer.guide_enable(seed=1)
+    tracer.render(spp=64, mis_mode=pt.MIS_CONSISTENT, aov=False, guide=True, guide_train=True)
+    s = tracer.guide_read_samples()
+    assert len(s) > 0
+    assert np.all(np.isfinite(s)) and np.all(s[:, 8] >= 0) and np.all(s[:, 9] >= 0) and np.all(s[:, 11] > 0)
+    assert np.allclose(np.linalg.norm(s[:, 4:7], axis=1), 1.0, atol=1e-5)
+    assert np.mean(s[:, 9] > 0) > 0.5  # contribution-carrying samples are kept first
+    tracer.render(spp=192, mis_mode=pt.MIS_CONSISTENT, aov=False, guide=True)
+    img = tracer.read(pt.BUF_ACCUM)[..., :3].astype(np.float64)
+    guided = relmse(img, ref)
+    info = tracer.guide_info()
+    print(f"relmse plain {plain:.4f} guided {guided:.4f} ratio {plain / guided:.2f} steps {info['steps']}")
+    assert abs(img.mean() / ref.mean() - 1) < 0.02
+    assert guided < 0.5 * plain
