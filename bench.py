@@ -8,7 +8,10 @@ constant environment 0.1 -- max depth 16, seed 0.  One step is one frame
 (one sample per pixel); the default run traces 64 frames (the 64-spp render).
 
 Prints one JSON line (rank 0).  `value` is Msamples/s with the scene and BVH
-resident on the GPU before the timed region starts.  The roofline block uses
+resident on the GPU before the timed region starts.  `relmse_guided_vs_plain`
+is the second half of the metric: relMSE of the plain tracer divided by that
+of the guided tracer at equal wall-clock on the door-room scene (1080p, depth
+16, against a 4096-spp FP64 plain reference; see guided_experiment).  The roofline block uses
 the kernel with the largest share of GPU time; its algorithmic bytes come from
 a counting pass over the same frames (identical rays: the tracer is
 deterministic) and its durations from HIP events recorded on the pass stream
@@ -65,6 +68,89 @@ def cpu_baseline(scene_desc, target_seconds, threads):
 
 
 COPY_PROBE_BYTES = 1 << 30
+# Multiply-adds of one network evaluation as executed on MFMA: 32x64 + 3 x 64x64.
+GUIDE_FLOPS_PER_QUERY = 2 * (32 * 64 + 3 * 64 * 64)
+MFMA_FP16_PEAK_TFLOPS = 2500.0
+
+
+def relmse(img, ref):
+    import numpy as np
+
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def guided_experiment(pt, ref_spp, plain_spp, train_spp):
+    """relMSE at equal wall-clock, guided vs plain, on the door room (room lit
+    through a 5 cm door gap, ~500K triangles, 1920x1080, max depth 16).
+
+    Reference: ref_spp plain frames accumulated in FP64.  Plain: plain_spp
+    frames, timed.  Guided: trained online on its own paths during the first
+    train_spp frames, then rendering continues in 8-frame calls until the plain
+    run's time is used up; the guided time includes training.  Both use
+    balance-heuristic MIS with the emitter selection probability on both sides."""
+    import numpy as np
+
+    scene = pt.HostScene.generate(pt.SCENE_DOOR_ROOM, seed=0)
+    d = scene.desc
+    t = pt.PathTracer(0)
+    t.set_scene(scene)
+    t.build_bvh()
+    mis = pt.MIS_CONSISTENT
+    t0 = time.perf_counter()
+    t.render(spp=ref_spp, mis_mode=mis, fp64=True, aov=False)
+    ref = t.read(pt.BUF_ACCUM64) / ref_spp
+    t_ref = time.perf_counter() - t0
+    log(f"guided experiment: reference {ref_spp} spp in {t_ref:.1f}s")
+    t.reset()
+    t.render(spp=4, mis_mode=mis, aov=False)  # warm-up
+    t.reset()
+    t0 = time.perf_counter()
+    t.render(spp=plain_spp, mis_mode=mis, aov=False)
+    t_plain = time.perf_counter() - t0
+    e_plain = relmse(t.read(pt.BUF_ACCUM)[..., :3].astype(np.float64), ref)
+    t.reset()
+    t.guide_enable()
+    s0 = t.stats()
+    t0 = time.perf_counter()
+    t.render(spp=train_spp, mis_mode=mis, aov=False, guide=True, guide_train=True, timing=True)
+    t_train = time.perf_counter() - t0
+    n = train_spp
+    while time.perf_counter() - t0 < t_plain:
+        t.render(spp=8, mis_mode=mis, aov=False, guide=True, timing=True)
+        n += 8
+    t_guided = time.perf_counter() - t0
+    img = t.read(pt.BUF_ACCUM)[..., :3].astype(np.float64)
+    e_guided = relmse(img, ref)
+    s1 = t.stats()
+    info = t.guide_info()
+    q = s1["guide_queries"] - s0["guide_queries"]
+    ms_infer = s1["ms_guide_infer"] - s0["ms_guide_infer"]
+    infer_tflops = q * GUIDE_FLOPS_PER_QUERY / (ms_infer / 1000.0) / 1e12 if ms_infer > 0 else None
+    t.close()
+    out = {
+        "scene": (f"door room: 4x6x3 m room, walls ~500K tris diffuse 0.6, emitter (radiance 200) behind a 5 cm "
+                  f"door gap; {d.width}x{d.height}, max depth {d.max_depth}, {d.num_triangles} tris"),
+        "reference_spp": ref_spp,
+        "plain_spp": plain_spp,
+        "plain_seconds": round(t_plain, 3),
+        "relmse_plain": round(e_plain, 5),
+        "guided_spp": n,
+        "guided_train_spp": train_spp,
+        "guided_train_seconds": round(t_train, 3),
+        "guided_seconds": round(t_guided, 3),
+        "relmse_guided": round(e_guided, 5),
+        "ratio": round(e_plain / e_guided, 3),
+        "mean_rel_diff": round(float(img.mean() / ref.mean() - 1.0), 5),
+        "guide_steps": info["steps"],
+        "guide_skipped_steps": info["skipped"],
+        "guide_queries": q,
+        "ms_guide_infer": round(ms_infer, 2),
+        "ms_guide_train": round(s1["ms_guide"] - s0["ms_guide"], 2),
+        "infer_TFLOPs": round(infer_tflops, 2) if infer_tflops else None,
+        "infer_mfma_frac": round(infer_tflops / MFMA_FP16_PEAK_TFLOPS, 4) if infer_tflops else None,
+    }
+    log(f"guided experiment: {json.dumps(out)}")
+    return out
 
 
 def summarize_pmc(fetch_dir, write_dir, kernel_prefix, out_path):
@@ -117,6 +203,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--spheres", type=int, default=256)
+    ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain relMSE experiment")
+    ap.add_argument("--guided-ref-spp", type=int, default=4096)
+    ap.add_argument("--guided-plain-spp", type=int, default=1024)
+    ap.add_argument("--guided-train-spp", type=int, default=256)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -199,6 +289,11 @@ def main():
         cpu = cpu_baseline(d, args.cpu_seconds, threads)
         log(f"cpu baseline: {cpu}")
 
+    tracer.close()
+    guided = None
+    if not args.no_guided:
+        guided = guided_experiment(pt, args.guided_ref_spp, args.guided_plain_spp, args.guided_train_spp)
+
     result = {
         "metric": "Msamples/s at 1080p on 1 GPU; relMSE at equal time, guided vs plain PT",
         "value": round(value, 3),
@@ -237,9 +332,9 @@ def main():
             "measured_copy_GBps": round(copy_gbps, 1),
         },
         "cpu_baseline": cpu,
-        "relmse_guided_vs_plain": None,
+        "relmse_guided_vs_plain": guided["ratio"] if guided else None,
+        "guided": guided,
     }
-    tracer.close()
     print(json.dumps(result), flush=True)
 
 

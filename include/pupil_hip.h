@@ -186,6 +186,7 @@ typedef struct pt_stats {
     uint64_t guide_steps;     /* optimizer steps launched by pt_render / pt_guide_train */
     uint64_t guide_samples;   /* training samples collected by pt_render */
     double ms_guide_infer;    /* event time of the network inference kernels, with timing */
+    uint64_t guide_queries;   /* network evaluations made for sampling */
 } pt_stats;
 
 typedef struct pt_ctx pt_ctx;

@@ -92,7 +92,7 @@ class Stats(C.Structure):
                 ("launches_shade", C.c_uint64), ("launches_shadow", C.c_uint64), ("bvh_nodes", C.c_uint32),
                 ("bvh_leaf_tris", C.c_uint32), ("bvh_depth", C.c_uint32), ("bvh_build_ms", C.c_double),
                 ("ms_guide", C.c_double), ("guide_steps", C.c_uint64), ("guide_samples", C.c_uint64),
-                ("ms_guide_infer", C.c_double)]
+                ("ms_guide_infer", C.c_double), ("guide_queries", C.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -341,7 +341,7 @@ __device__ __forceinline__ void unpack_half2(uint32_t v, float &a, float &b) {
 constexpr int kInferBlock = 512;
 
 __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQueue q, const uint32_t *count_ptr,
-                                                             uint2 *gout) {
+                                                             uint2 *gout, unsigned long long *queries) {
     __shared__ half8 wl[4 * 4 * 2 * 64];
     for (uint32_t t = threadIdx.x; t < 4 * 4 * 2 * 64; t += kInferBlock) wl[t] = G.fwd_a[t];
     __syncthreads();
@@ -351,6 +351,7 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQu
     const int g = lane >> 4, il = lane & 15;
     const uint32_t count = *count_ptr;
     const uint32_t wstride = gridDim.x * waves * 64u;
+    uint32_t nq = 0;  // queries of this wave (counted by lane 0)
     for (uint32_t base = (blockIdx.x * waves + wave) * 64u; base < count; base += wstride) {
         const uint32_t i = base + lane;
         bool valid = i < count;
@@ -360,7 +361,9 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQu
             valid = f2u(h.y) != kMissPrim;
             if (valid) p = f3of(r0) + f3of(r1) * h.x;
         }
-        if (__ballot(valid) == 0ull) continue;
+        const unsigned long long vmask = __ballot(valid);
+        if (vmask == 0ull) continue;
+        nq += (uint32_t)__popcll(vmask);
         const f3 u = guide_normalize_pos(G, p);
         const int vbit = valid ? 1 : 0;
 #pragma unroll 1
@@ -381,6 +384,7 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQu
             }
         }
     }
+    if (lane == 0 && nq) atomicAdd(queries, (unsigned long long)nq);
 }
 
 // k_shade with the guided step; the mixture of each guidable vertex comes
@@ -579,7 +583,7 @@ struct pt_ctx {
     uint32_t num_counters = 0;
     uint2 *d_spill = nullptr;
     uint32_t spill_threads = 0;
-    unsigned long long *d_stat = nullptr;  // closest nodes, closest tris, shadow nodes, shadow tris
+    unsigned long long *d_stat = nullptr;  // closest nodes, closest tris, shadow nodes, shadow tris, guide queries
     bool counting = false;
     uint32_t *d_count_hist = nullptr;      // per-launch copies of the queue counters
     uint32_t count_hist_slots = 0;         // capacity in counters
@@ -858,7 +862,8 @@ int pt_create(int hip_device, pt_ctx **out) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_guide_infer, kInferBlock, 0) == hipSuccess && c > 0)
             ctx->infer_blocks_per_cu = (uint32_t)c;
     }
-    if (hipMalloc((void **)&ctx->d_stat, 4 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc((void **)&ctx->d_stat, 5 * sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(ctx->d_stat, 0, 5 * sizeof(unsigned long long)) != hipSuccess) {
         g_create_error = "hipMalloc failed";
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
@@ -1281,7 +1286,8 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
             if (infer_here) {
                 EventTimer t(ctx, timing);
                 hipLaunchKernelGGL(k_guide_infer, dim3(ctx->num_cu * ctx->infer_blocks_per_cu), dim3(kInferBlock), 0,
-                                   ctx->stream, gview, ctx->q[cur], ext + b, reinterpret_cast<uint2 *>(ctx->d_gout));
+                                   ctx->stream, gview, ctx->q[cur], ext + b, reinterpret_cast<uint2 *>(ctx->d_gout),
+                                   ctx->d_stat + 4);
                 t.stop(kKindGuideInfer);
             }
             {
@@ -1346,6 +1352,11 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
             ctx->stats.rays_extension += c[b];
             ctx->stats.rays_shadow += c[max_depth + 2 + b];
         }
+    }
+    if (guide_infer) {
+        unsigned long long nq = 0;
+        PT_HIP(hipMemcpy(&nq, ctx->d_stat + 4, sizeof(nq), hipMemcpyDeviceToHost));
+        ctx->stats.guide_queries = nq;
     }
     if (guide_train) {
         unsigned long long kept = 0;
