@@ -207,6 +207,8 @@ def main():
     ap.add_argument("--guided-ref-spp", type=int, default=4096)
     ap.add_argument("--guided-plain-spp", type=int, default=1024)
     ap.add_argument("--guided-train-spp", type=int, default=256)
+    ap.add_argument("--guided-only", action="store_true",
+                    help="run only the guided experiment and print its JSON (profiling runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -215,6 +217,11 @@ def main():
         sys.exit(2)
 
     import pupiloptixlab_amd as pt
+
+    if args.guided_only:
+        print(json.dumps(guided_experiment(pt, args.guided_ref_spp, args.guided_plain_spp, args.guided_train_spp)),
+              flush=True)
+        return
 
     t0 = time.perf_counter()
     scene = pt.HostScene.generate(pt.SCENE_TERRAIN, seed=0, grid=args.grid, spheres=args.spheres)
