@@ -298,6 +298,62 @@ int pt_guide_set_params(pt_ctx *ctx, const float *table, const float *weights);
  * *count receives the number copied. */
 int pt_guide_read_samples(pt_ctx *ctx, float *samples, uint32_t max_samples, uint32_t *count);
 
+/* ---- batched multi-view rendering ----------------------------------------
+ * No counterpart in the reference (one scene and one camera per pass).  Views
+ * -- each a scene with its own camera, film, depth and frame count -- are
+ * added to the context's batch; pt_batch_render traces all of them in one
+ * wavefront loop: every launch packs frames of several views into the same
+ * queues, so paths of different scenes and lengths share the per-bounce
+ * kernels.  Each view renders exactly what pt_render would for its scene
+ * (same seeds, same bits). */
+enum pt_batch_flags {
+    PT_BATCH_GRAPH = 1,    /* capture one launch's kernels in a hipGraph and replay it */
+    PT_BATCH_AUTOTUNE = 2  /* pick the launch shape by timing candidates first (pt_batch_autotune) */
+};
+
+typedef struct pt_batch_params {
+    uint32_t rr_depth;           /* Russian roulette when depth > rr_depth */
+    uint32_t mis_mode;           /* pt_mis_mode */
+    uint32_t render_flags;       /* PT_RENDER_FP64_ACCUM, PT_RENDER_TIMING */
+    uint32_t batch_flags;        /* pt_batch_flags */
+    uint32_t paths_per_launch;   /* queue capacity, 0 = tuned or default (32M) */
+    uint32_t shade_block;        /* shading block size (64, 128, 256), 0 = tuned or 256 */
+    uint32_t shade_blocks_per_cu;/* 0 = tuned or 8 */
+    uint32_t trace_blocks_per_cu;/* traversal blocks per CU, 0 = tuned or resident maximum */
+} pt_batch_params;
+
+typedef struct pt_batch_tuning {
+    uint32_t paths_per_launch, shade_block, shade_blocks_per_cu, trace_blocks_per_cu;
+    uint32_t candidates;         /* launch shapes timed */
+    uint32_t probe_paths;        /* paths per timed probe */
+    double best_ns_per_path;
+    double default_ns_per_path;  /* the default shape on the same probe */
+} pt_batch_tuning;
+
+typedef struct pt_batch_stats {
+    uint32_t views, launches;
+    uint64_t paths, rays_extension, rays_shadow, triangles;
+    double ms_total, build_ms;
+} pt_batch_stats;
+
+/* Removes all views and frees their device data. */
+int pt_batch_clear(pt_ctx *ctx);
+/* Adds a view: converts the scene, builds its BVH (one tree per view in a
+ * shared node array), allocates its film.  spp = frames rendered per call
+ * of pt_batch_render. */
+int pt_batch_add_view(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uint32_t *view_index);
+/* Renders every view's spp frames; blocks until done. */
+int pt_batch_render(pt_ctx *ctx, const pt_batch_params *params);
+/* PT_BUF_FRAME, PT_BUF_ACCUM (float4) or PT_BUF_ACCUM64 (double3) of a view. */
+int pt_batch_read(pt_ctx *ctx, uint32_t view, int which, void *host_dst, size_t bytes);
+/* Restarts accumulation of every view. */
+int pt_batch_reset(pt_ctx *ctx);
+/* Times launch shapes (queue capacity, shading and traversal grids) on the
+ * batch's own frames and keeps the fastest for later renders.  Accumulation
+ * is restarted afterwards. */
+int pt_batch_autotune(pt_ctx *ctx, const pt_batch_params *params, pt_batch_tuning *out);
+int pt_batch_get_stats(pt_ctx *ctx, pt_batch_stats *out);
+
 /* ---- host scene helpers (resource::Scene / World, headless) ----------- */
 typedef struct pt_host_scene pt_host_scene;
 
@@ -305,7 +361,8 @@ enum pt_scene_kind {
     PT_SCENE_CORNELL = 1,   /* closed box, two rotated boxes, one area light */
     PT_SCENE_TERRAIN = 2,   /* fractal heightfield + icospheres, 7 materials, 4 lights + env */
     PT_SCENE_DOOR_ROOM = 3, /* room lit through a narrow door gap */
-    PT_SCENE_FURNACE = 4    /* closed emissive diffuse cube seen from its center */
+    PT_SCENE_FURNACE = 4,   /* closed emissive diffuse cube seen from its center */
+    PT_SCENE_CORNELL_VARIANT = 5 /* Cornell box tessellated to 50K-2M tris, random camera and film size */
 };
 
 typedef struct pt_gen_params {
@@ -316,6 +373,7 @@ typedef struct pt_gen_params {
     uint32_t max_depth;     /* 0 = scene default */
     float albedo;           /* furnace wall albedo, 0 = 0.5 */
     float radiance;         /* furnace wall emission, 0 = 1 */
+    uint32_t triangles;     /* Cornell variant: target triangle count, 0 = drawn from the seed */
 } pt_gen_params;
 
 /* Scene::LoadFromXML (resource/scene.cpp:27-56) for the mitsuba subset. */

@@ -39,8 +39,9 @@ MIS_REFERENCE, MIS_CONSISTENT = 0, 1
 RENDER_AOV, RENDER_FP64_ACCUM, RENDER_TIMING, RENDER_GUIDE, RENDER_GUIDE_TRAIN = 1, 2, 4, 8, 16
 GUIDE_OUTPUTS, GUIDE_LEVELS = 41, 16
 GUIDE_NO_RADIANCE_L2, GUIDE_NO_GUIDE_CE, GUIDE_NO_MIXTURE_CE = 1, 2, 4
+BATCH_GRAPH, BATCH_AUTOTUNE = 1, 2
 BUF_FRAME, BUF_ACCUM, BUF_ALBEDO, BUF_NORMAL, BUF_TEST, BUF_PRIM_ID, BUF_ACCUM64 = range(7)
-SCENE_CORNELL, SCENE_TERRAIN, SCENE_DOOR_ROOM, SCENE_FURNACE = 1, 2, 3, 4
+SCENE_CORNELL, SCENE_TERRAIN, SCENE_DOOR_ROOM, SCENE_FURNACE, SCENE_CORNELL_VARIANT = 1, 2, 3, 4, 5
 MISS = 0xFFFFFFFF
 
 
@@ -119,9 +120,34 @@ class GuideInfo(C.Structure):
         return d
 
 
+class BatchParams(C.Structure):
+    _fields_ = [("rr_depth", C.c_uint32), ("mis_mode", C.c_uint32), ("render_flags", C.c_uint32),
+                ("batch_flags", C.c_uint32), ("paths_per_launch", C.c_uint32), ("shade_block", C.c_uint32),
+                ("shade_blocks_per_cu", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("paths_per_launch", C.c_uint32), ("shade_block", C.c_uint32), ("shade_blocks_per_cu", C.c_uint32),
+                ("trace_blocks_per_cu", C.c_uint32), ("candidates", C.c_uint32), ("probe_paths", C.c_uint32),
+                ("best_ns_per_path", C.c_double), ("default_ns_per_path", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("views", C.c_uint32), ("launches", C.c_uint32), ("paths", C.c_uint64),
+                ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64), ("triangles", C.c_uint64),
+                ("ms_total", C.c_double), ("build_ms", C.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class GenParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32), ("grid", C.c_uint32),
-                ("spheres", C.c_uint32), ("max_depth", C.c_uint32), ("albedo", C.c_float), ("radiance", C.c_float)]
+                ("spheres", C.c_uint32), ("max_depth", C.c_uint32), ("albedo", C.c_float), ("radiance", C.c_float),
+                ("triangles", C.c_uint32)]
 
 
 # Every symbol the C header declares, with its ctypes signature.
@@ -156,6 +182,13 @@ _SIGNATURES = {
     "pt_guide_get_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pt_guide_set_params": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pt_guide_read_samples": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "pt_batch_clear": (C.c_int, [C.c_void_p]),
+    "pt_batch_add_view": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc), C.c_uint32, C.POINTER(C.c_uint32)]),
+    "pt_batch_render": (C.c_int, [C.c_void_p, C.POINTER(BatchParams)]),
+    "pt_batch_read": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, C.c_void_p, C.c_size_t]),
+    "pt_batch_reset": (C.c_int, [C.c_void_p]),
+    "pt_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(BatchParams), C.POINTER(BatchTuning)]),
+    "pt_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -199,8 +232,9 @@ class HostScene:
 
     @classmethod
     def generate(cls, kind: int, seed: int = 0, width: int = 0, height: int = 0, grid: int = 0,
-                 spheres: int = 0, max_depth: int = 0, albedo: float = 0.0, radiance: float = 0.0) -> "HostScene":
-        gp = GenParams(seed, width, height, grid, spheres, max_depth, albedo, radiance)
+                 spheres: int = 0, max_depth: int = 0, albedo: float = 0.0, radiance: float = 0.0,
+                 triangles: int = 0) -> "HostScene":
+        gp = GenParams(seed, width, height, grid, spheres, max_depth, albedo, radiance, triangles)
         h = C.c_void_p()
         _check(lib().pt_scene_generate(kind, C.byref(gp), C.byref(h)), "pt_scene_generate")
         return cls(h)
@@ -434,3 +468,56 @@ class PathTracer:
         w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
         _check(lib().pt_guide_set_params(self._ctx, None if tb is None else tb.ctypes.data,
                                          None if w is None else w.ctypes.data), "pt_guide_set_params", self._ctx)
+
+    # ---- batched views -------------------------------------------------------------
+    def batch_clear(self) -> None:
+        _check(lib().pt_batch_clear(self._ctx), "pt_batch_clear", self._ctx)
+        self._batch_films = []
+
+    def batch_add_view(self, scene, spp: int) -> int:
+        desc = scene.desc if isinstance(scene, HostScene) else scene
+        idx = C.c_uint32()
+        _check(lib().pt_batch_add_view(self._ctx, C.byref(desc), spp, C.byref(idx)), "pt_batch_add_view",
+               self._ctx)
+        if not hasattr(self, "_batch_films"):
+            self._batch_films = []
+        self._batch_films.append((desc.width, desc.height))
+        return idx.value
+
+    @staticmethod
+    def _batch_params(rr_depth, mis_mode, fp64, timing, graph, autotune, paths_per_launch, shade_block,
+                      shade_blocks_per_cu, trace_blocks_per_cu):
+        rf = (RENDER_FP64_ACCUM if fp64 else 0) | (RENDER_TIMING if timing else 0)
+        bf = (BATCH_GRAPH if graph else 0) | (BATCH_AUTOTUNE if autotune else 0)
+        return BatchParams(rr_depth, mis_mode, rf, bf, paths_per_launch, shade_block, shade_blocks_per_cu,
+                           trace_blocks_per_cu)
+
+    def batch_render(self, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE, fp64: bool = False,
+                     timing: bool = False, graph: bool = False, autotune: bool = False, paths_per_launch: int = 0,
+                     shade_block: int = 0, shade_blocks_per_cu: int = 0, trace_blocks_per_cu: int = 0) -> None:
+        p = self._batch_params(rr_depth, mis_mode, fp64, timing, graph, autotune, paths_per_launch, shade_block,
+                               shade_blocks_per_cu, trace_blocks_per_cu)
+        _check(lib().pt_batch_render(self._ctx, C.byref(p)), "pt_batch_render", self._ctx)
+
+    def batch_autotune(self, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE, graph: bool = False) -> dict:
+        p = self._batch_params(rr_depth, mis_mode, False, False, graph, False, 0, 0, 0, 0)
+        t = BatchTuning()
+        _check(lib().pt_batch_autotune(self._ctx, C.byref(p), C.byref(t)), "pt_batch_autotune", self._ctx)
+        return t.as_dict()
+
+    def batch_reset(self) -> None:
+        _check(lib().pt_batch_reset(self._ctx), "pt_batch_reset", self._ctx)
+
+    def batch_read(self, view: int, which: int = BUF_ACCUM) -> np.ndarray:
+        w, h = self._batch_films[view]
+        if which == BUF_ACCUM64:
+            out = np.empty((h, w, 3), dtype=np.float64)
+        else:
+            out = np.empty((h, w, 4), dtype=np.float32)
+        _check(lib().pt_batch_read(self._ctx, view, which, out.ctypes.data, out.nbytes), "pt_batch_read", self._ctx)
+        return out
+
+    def batch_stats(self) -> dict:
+        s = BatchStats()
+        _check(lib().pt_batch_get_stats(self._ctx, C.byref(s)), "pt_batch_get_stats", self._ctx)
+        return s.as_dict()

@@ -123,9 +123,11 @@ struct TraverseStats {
 // hold the not-yet-visited internal children of a node; internal hit bits are
 // stored at position (slot ^ octant) so popping the lowest bit visits slots
 // near-to-far for the ray's direction octant.
+// `root` is the index of the tree's root node (several trees can share one
+// node array).
 template <bool kAnyHit, typename Stack, typename NodeLoad, typename TriLoad>
 PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float tmin, float tmax,
-                        Stack &stack, Hit &hit, TraverseStats *stats) {
+                        Stack &stack, Hit &hit, TraverseStats *stats, uint32_t root = 0u) {
     const RayPre rp = ray_precompute(d);
     const float idx = 1.f / (fabsf(d.x) > 1e-12f ? d.x : copysignf(1e-12f, d.x));
     const float idy = 1.f / (fabsf(d.y) > 1e-12f ? d.y : copysignf(1e-12f, d.y));
@@ -138,8 +140,8 @@ PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float 
     hit.b2 = 0.f;
     float tbest = tmax;
 
-    // Root group: node 0 sits in slot 0 of a virtual parent.
-    uint32_t g_base = 0, g_bits = (1u << (24 + (0 ^ oct))) | 1u;
+    // Root group: the root sits in slot 0 of a virtual parent.
+    uint32_t g_base = root, g_bits = (1u << (24 + (0 ^ oct))) | 1u;
     uint32_t t_base = 0, t_bits = 0;
     while (true) {
         if (g_bits & 0xFF000000u) {

@@ -87,7 +87,7 @@ struct LdsStack {
 
 template <bool kAnyHit, bool kCount>
 __device__ __forceinline__ bool trace_one(const DeviceBvh &bvh, uint2 *lds_col, uint32_t gtid, f3 o, f3 d,
-                                          float tmin, float tmax, Hit &h, TraverseStats &st) {
+                                          float tmin, float tmax, Hit &h, TraverseStats &st, uint32_t root = 0u) {
     LdsStack stack{lds_col, bvh.spill + (size_t)gtid * kSpillEntries, (uint32_t)kTraceBlock, 0u};
     auto ln = [&](uint32_t i, WideNode &n) { load_node_dev(bvh.nodes, i, n); };
     auto lt = [&](uint32_t i, LeafTri &t) {
@@ -97,7 +97,7 @@ __device__ __forceinline__ bool trace_one(const DeviceBvh &bvh, uint2 *lds_col, 
         t.v1 = f4{b.x, b.y, b.z, b.w};
         t.v2 = f4{c.x, c.y, c.z, c.w};
     };
-    return traverse_bvh<kAnyHit>(ln, lt, o, d, tmin, tmax, stack, h, kCount ? &st : nullptr);
+    return traverse_bvh<kAnyHit>(ln, lt, o, d, tmin, tmax, stack, h, kCount ? &st : nullptr, root);
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -155,9 +155,13 @@ __device__ __forceinline__ uint32_t wave_fetch(uint32_t *work, uint32_t n) {
 // --- closest-hit traversal ------------------------------------------------------
 // Persistent grid sized to the resident occupancy; each wave pulls 64 rays at
 // a time, so waves that drew short traversals keep working.
+// Several scenes can share the queues (batched views): the view of a path
+// sits above bit `path_bits` of its id and selects its tree's root in `roots`
+// (null for a single scene, where path_bits is 31).
 template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(PathQueue q, DeviceBvh bvh, const uint32_t *count_ptr,
-                                                       uint32_t *work, unsigned long long *stat) {
+                                                       uint32_t *work, unsigned long long *stat,
+                                                       const uint32_t *roots, uint32_t path_bits) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
     const uint32_t count = *count_ptr;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -168,9 +172,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(PathQueue q, DeviceBvh bv
         const uint32_t i = base + lane_id();
         if (i < count) {
             const float4 r0 = q.ray0[i], r1 = q.ray1[i];
+            const uint32_t root = roots ? roots[f2u(r0.w) >> path_bits] : 0u;
             Hit h;
             trace_one<false, kCount>(bvh, lds + threadIdx.x, gtid, f3of(r0), f3of(r1), kExtTmin, kMaxDistance, h,
-                                     st);
+                                     st, root);
             q.hit[i] = make_float4(h.t, u2f(h.prim), h.b1, h.b2);
         }
     }
@@ -181,10 +186,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(PathQueue q, DeviceBvh bv
 }
 
 // --- shadow (any-hit) traversal ----------------------------------------------------
+// Shadow-ray targets: bit 31 set = output slot, else next-queue slot; the
+// view sits in bits [path_bits, 31), the slot below.
 template <bool kCount>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(ShadowQueue sq, DeviceBvh bvh, const uint32_t *count_ptr,
                                                         uint32_t *work, PathQueue next, float4 *out_l,
-                                                        unsigned long long *stat) {
+                                                        unsigned long long *stat, const uint32_t *roots,
+                                                        uint32_t path_bits) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
     const uint32_t count = *count_ptr;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -198,27 +206,29 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(ShadowQueue sq, DeviceBv
         if (i < count) {
             const float4 s0 = sq.s0[i];
             s1 = sq.s1[i];
+            const uint32_t root = roots ? roots[(f2u(s1.w) & 0x7FFFFFFFu) >> path_bits] : 0u;
             Hit h;
-            occluded =
-                trace_one<true, kCount>(bvh, lds + threadIdx.x, gtid, f3of(s0), f3of(s1), kShadowEps, s0.w, h, st);
+            occluded = trace_one<true, kCount>(bvh, lds + threadIdx.x, gtid, f3of(s0), f3of(s1), kShadowEps, s0.w,
+                                               h, st, root);
         }
         if (!occluded) {
             const float4 c = sq.s2[i];
             const uint32_t target = f2u(s1.w);
+            const uint32_t slot = target & ((1u << path_bits) - 1u);
             if (target & 0x80000000u) {
-                const uint32_t p = target & 0x7FFFFFFFu;
+                const uint32_t p = slot;
                 float4 l = out_l[p];
                 l.x = l.x + c.x;
                 l.y = l.y + c.y;
                 l.z = l.z + c.z;
                 out_l[p] = l;
             } else {
-                float4 tl = next.tl[target], lr = next.lr[target];
+                float4 tl = next.tl[slot], lr = next.lr[slot];
                 tl.w = tl.w + c.x;
                 lr.x = lr.x + c.y;
                 lr.y = lr.y + c.z;
-                next.tl[target] = tl;
-                next.lr[target] = lr;
+                next.tl[slot] = tl;
+                next.lr[slot] = lr;
             }
         }
     }
@@ -238,10 +248,24 @@ struct AovBuffers {
     uint32_t npix;
 };
 
+// Per-view records of a batched render (k_raygen_batch, k_shade,
+// k_accumulate_batch).
+struct BatchViewDev {
+    CameraRec camera;
+    EnvRec env;
+    uint32_t width, height, max_depth, root;
+    uint32_t emitter_begin, emitter_count, pad0, pad1;
+    float4 *accum, *frame;
+    double *accum64;
+    void *pad2;
+};
+
+// With `views` set (batched render) the path id carries the view above bit
+// `path_bits`; the view selects its emitters, environment and depth.
 __global__ __launch_bounds__(256) void k_shade(SceneView scene, ShadeParams prm, PathQueue in,
                                                const uint32_t *count_in, PathQueue out, uint32_t *count_out,
                                                ShadowQueue sq, uint32_t *count_shadow, float4 *out_l,
-                                               AovBuffers aov) {
+                                               AovBuffers aov, const BatchViewDev *views, uint32_t path_bits) {
     const uint32_t count = *count_in;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
@@ -250,11 +274,11 @@ __global__ __launch_bounds__(256) void k_shade(SceneView scene, ShadeParams prm,
         PathState ps;
         ShadeOut so;
         so.cont = so.done = so.shadow = so.wrote_aov = false;
-        uint32_t path = 0, prim = kMissPrim;
+        uint32_t path = 0, packed = 0, vbits = 0, prim = kMissPrim;
         if (valid) {
             const float4 r0 = in.ray0[i], r1 = in.ray1[i], tl = in.tl[i], lr = in.lr[i], h = in.hit[i];
             ps.o = f3of(r0);
-            path = f2u(r0.w);
+            packed = path = f2u(r0.w);
             ps.d = f3of(r1);
             ps.prev_pdf = r1.w;
             ps.T = f3of(tl);
@@ -262,7 +286,22 @@ __global__ __launch_bounds__(256) void k_shade(SceneView scene, ShadeParams prm,
             ps.rng.seed = f2u(lr.z);
             ps.flags = f2u(lr.w);
             prim = f2u(h.y);
-            shade_path(scene, prm, ps, prim, h.z, h.w, so);
+            if (views) {
+                const uint32_t view = packed >> path_bits;
+                path = packed & ((1u << path_bits) - 1u);
+                vbits = view << path_bits;
+                const BatchViewDev &bv = views[view];
+                SceneView sv = scene;
+                sv.emitters += bv.emitter_begin;
+                sv.emitter_cdf += bv.emitter_begin;
+                sv.num_emitters = bv.emitter_count;
+                sv.env = bv.env;
+                ShadeParams pv = prm;
+                pv.max_depth = bv.max_depth;
+                shade_path(sv, pv, ps, prim, h.z, h.w, so);
+            } else {
+                shade_path(scene, prm, ps, prim, h.z, h.w, so);
+            }
         }
         const uint32_t j = wave_alloc(valid && so.cont, count_out);
         const uint32_t k = wave_alloc(valid && so.shadow, count_shadow);
@@ -282,7 +321,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView scene, ShadeParams prm,
             }
         }
         if (so.cont) {
-            out.ray0[j] = make_float4(ps.o.x, ps.o.y, ps.o.z, u2f(path));
+            out.ray0[j] = make_float4(ps.o.x, ps.o.y, ps.o.z, u2f(packed));
             out.ray1[j] = make_float4(ps.d.x, ps.d.y, ps.d.z, ps.prev_pdf);
             out.tl[j] = make_float4(ps.T.x, ps.T.y, ps.T.z, ps.L.x);
             out.lr[j] = make_float4(ps.L.y, ps.L.z, u2f(ps.rng.seed), u2f(ps.flags));
@@ -290,7 +329,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView scene, ShadeParams prm,
             out_l[path] = make_float4(ps.L.x, ps.L.y, ps.L.z, 1.f);
         }
         if (so.shadow) {
-            const uint32_t target = so.cont ? j : (0x80000000u | path);
+            const uint32_t target = so.cont ? (vbits | j) : (0x80000000u | vbits | path);
             sq.s0[k] = make_float4(so.so.x, so.so.y, so.so.z, so.stmax);
             sq.s1[k] = make_float4(so.sd.x, so.sd.y, so.sd.z, u2f(target));
             sq.s2[k] = make_float4(so.contrib.x, so.contrib.y, so.contrib.z, 0.f);
@@ -507,6 +546,122 @@ __global__ void k_accumulate(const float4 *out_l, float4 *accum, float4 *frame, 
     }
 }
 
+// --- batched views ----------------------------------------------------------------
+constexpr uint32_t kMaxLaunchEntries = 1024;  // frames per launch
+constexpr uint32_t kMaxLaunchSegs = 256;      // runs of consecutive frames of one view per launch
+
+// Frame `frame_seed` (RNG frame index) of a view occupies the launch's paths
+// [path_begin, path_begin + pixels).
+struct LaunchEntry {
+    uint32_t view, frame_seed, path_begin, pixels;
+};
+// Consecutive frames [first_entry, first_entry + frames) of one view, folded
+// into its film in frame order; its pixels are threads [pix_begin, +pixels).
+struct LaunchSeg {
+    uint32_t view, first_entry, frames, pix_begin;
+    uint32_t sample_cnt0, accumulate, pad0, pad1;
+};
+struct LaunchDesc {
+    uint32_t total_paths, n_entries, n_segs, seg_pixels;
+    LaunchEntry e[kMaxLaunchEntries];
+    LaunchSeg g[kMaxLaunchSegs];
+};
+
+// main.cu:39-78 for every (view, frame, pixel) of a batched launch.
+__global__ void k_raygen_batch(PathQueue q, const LaunchDesc *ld, const BatchViewDev *views, uint32_t path_bits,
+                               uint32_t *counters, uint32_t num_counters) {
+    const uint32_t total = ld->total_paths;
+    const uint32_t ne = ld->n_entries;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid == 0) {
+        counters[0] = total;
+        for (uint32_t i = 1; i < num_counters; ++i) counters[i] = 0u;
+    }
+    for (uint32_t i = tid; i < total; i += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = ne - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1u) >> 1;
+            if (ld->e[mid].path_begin <= i) lo = mid;
+            else hi = mid - 1u;
+        }
+        const LaunchEntry en = ld->e[lo];
+        const BatchViewDev &v = views[en.view];
+        const uint32_t pixel = i - en.path_begin;
+        const uint32_t x = pixel % v.width, y = pixel / v.width;
+        Rng rng = rng_init(pixel, en.frame_seed);
+        const float jx = rng_next(rng);
+        const float jy = rng_next(rng);
+        f3 o, d;
+        camera_ray(v.camera, x, y, v.width, v.height, jx, jy, o, d);
+        q.ray0[i] = make_float4(o.x, o.y, o.z, u2f((en.view << path_bits) | i));
+        q.ray1[i] = make_float4(d.x, d.y, d.z, 0.f);
+        q.tl[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+        q.lr[i] = make_float4(0.f, 0.f, u2f(rng.seed), u2f(kPathPrimary));
+    }
+}
+
+// k_accumulate per segment: the running mean of main.cu:190-196 over the
+// segment's frames in order, into its view's film.
+__global__ void k_accumulate_batch(const float4 *out_l, const LaunchDesc *ld, const BatchViewDev *views,
+                                   uint32_t fp64) {
+    const uint32_t total = ld->seg_pixels;
+    const uint32_t ns = ld->n_segs;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = ns - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1u) >> 1;
+            if (ld->g[mid].pix_begin <= t) lo = mid;
+            else hi = mid - 1u;
+        }
+        const LaunchSeg sg = ld->g[lo];
+        const BatchViewDev &v = views[sg.view];
+        const uint32_t p = t - sg.pix_begin;
+        const float4 acc = v.accum[p];
+        f3 a = mk3(acc.x, acc.y, acc.z);
+        uint32_t n = sg.sample_cnt0;
+        double sx = 0, sy = 0, sz = 0;
+        if (fp64) {
+            sx = v.accum64[3 * (size_t)p];
+            sy = v.accum64[3 * (size_t)p + 1];
+            sz = v.accum64[3 * (size_t)p + 2];
+        }
+        for (uint32_t f = 0; f < sg.frames; ++f) {
+            const float4 lv = out_l[ld->e[sg.first_entry + f].path_begin + p];
+            f3 L = mk3(lv.x, lv.y, lv.z);
+            if (sg.accumulate && n > 0) {
+                const float w = 1.f / ((float)n + 1.f);
+                L = lerp(a, L, w);
+            }
+            a = L;
+            if (sg.accumulate) n += 1;
+            sx += (double)lv.x;
+            sy += (double)lv.y;
+            sz += (double)lv.z;
+        }
+        const float4 r = make_float4(a.x, a.y, a.z, 1.f);
+        v.accum[p] = r;
+        v.frame[p] = r;
+        if (fp64) {
+            v.accum64[3 * (size_t)p] = sx;
+            v.accum64[3 * (size_t)p + 1] = sy;
+            v.accum64[3 * (size_t)p + 2] = sz;
+        }
+    }
+}
+
+// Adds one launch's queue sizes to running ray counts (graph-capturable; no
+// host read-back per launch).
+__global__ void k_batch_count(const uint32_t *counters, uint32_t max_depth, unsigned long long *rays) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    unsigned long long e = 0, sh = 0;
+    for (uint32_t b = 0; b < max_depth; ++b) {
+        e += counters[b];
+        sh += counters[max_depth + 2 + b];
+    }
+    rays[0] += e;
+    rays[1] += sh;
+}
+
 // Streaming copy used to measure achievable HBM bandwidth and to calibrate
 // hardware byte counters (pt_bandwidth_probe).
 __global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst, size_t n) {
@@ -535,6 +690,50 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(const float *rays, u
 // =====================================================================================
 // Context
 // =====================================================================================
+// One view of a batched render (host side).
+struct BatchView {
+    uint32_t width = 0, height = 0, spp = 0, max_depth = 1;
+    CameraRec camera{};
+    EnvRec env{};
+    uint32_t root = 0, emitter_begin = 0, emitter_count = 0, tri_count = 0;
+    float4 *d_accum = nullptr, *d_frame = nullptr;
+    double *d_accum64 = nullptr;
+    uint32_t sample_cnt = 0, random_seed = 0;  // progressive state, as pt_pass.cpp:55-56 per view
+};
+
+struct BatchState {
+    std::vector<BatchView> views;
+    // concatenated scene (global vertex / triangle / shape / material / emitter indices)
+    std::vector<float> pos4, nrm4, uv2;
+    std::vector<uint32_t> tri4;
+    std::vector<ShapeRec> shapes;
+    std::vector<MaterialRec> materials;
+    std::vector<EmitterRec> emitters;
+    std::vector<float> cdf;
+    std::vector<WideNode> nodes;
+    std::vector<LeafTri> leaf;
+    bool any_normals = false, any_texcoords = false;
+    bool dirty = true;
+    uint32_t *d_tri = nullptr;
+    float4 *d_pos = nullptr, *d_nrm = nullptr;
+    float2 *d_uv = nullptr;
+    ShapeRec *d_shapes = nullptr;
+    MaterialRec *d_materials = nullptr;
+    EmitterRec *d_emitters = nullptr;
+    float *d_cdf = nullptr;
+    WideNode *d_nodes = nullptr;
+    LeafTri *d_leaf = nullptr;
+    BatchViewDev *d_views = nullptr;
+    uint32_t *d_roots = nullptr;
+    LaunchDesc *d_launch = nullptr;
+    LaunchDesc *h_launch = nullptr;  // pinned ring, one slot per launch of a render
+    uint32_t h_launch_slots = 0;
+    unsigned long long *d_rays = nullptr;  // extension, shadow
+    bool tuned = false;
+    pt_batch_tuning tuning{};
+    pt_batch_stats stats{};
+};
+
 struct pt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -610,6 +809,9 @@ struct pt_ctx {
     uint4 *d_gout = nullptr;  // network outputs per queue slot
     uint32_t gout_cap = 0;
     uint32_t infer_blocks_per_cu = 2;
+
+    // batched views (pt_batch_*)
+    BatchState batch;
 };
 
 namespace {
@@ -824,6 +1026,469 @@ void collect_event_times(pt_ctx *ctx) {
     ctx->event_next = 0;
 }
 
+// A scene description converted to the device record layout (host copies).
+struct SceneHostData {
+    std::vector<float> h_pos4, h_nrm4, h_uv2;
+    std::vector<uint32_t> h_tri4;
+    std::vector<ShapeRec> h_shapes;
+    std::vector<MaterialRec> h_materials;
+    std::vector<EmitterRec> h_emitters;
+    std::vector<float> h_cdf;
+    EnvRec env{};
+    CameraRec camera{};
+    uint32_t width = 0, height = 0, max_depth = 1, num_tris = 0;
+    float lo[3]{}, hi[3]{};
+    bool has_normals = false, has_texcoords = false;
+};
+
+// Validation and packing of PTPass::SetScene + World::LoadScene
+// (pt_pass.cpp:127-240, world/world.cpp:108-146, world/emitter.cpp:169-337).
+// Nothing is produced for a malformed description.
+int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, std::string &err) {
+    if (!s || !s->positions || !s->indices || !s->shapes || !s->materials || s->width == 0 || s->height == 0) {
+        err = std::string(who) + ": missing arrays or zero film size";
+        return PT_ERR_INVALID_ARG;
+    }
+    const uint32_t nv = s->num_vertices, nt = s->num_triangles, ns = s->num_shapes, nm = s->num_materials;
+    // Validate ranges first: nothing is uploaded for a malformed description.
+    uint32_t covered = 0;
+    for (uint32_t i = 0; i < ns; ++i) {
+        const pt_shape_desc &sh = s->shapes[i];
+        if (sh.first_tri != covered || sh.material >= nm) {
+            err = std::string(who) + ": shapes must cover the triangles in order and reference valid materials";
+            return PT_ERR_INVALID_ARG;
+        }
+        covered += sh.num_tris;
+    }
+    if (covered != nt) {
+        err = std::string(who) + ": shape triangle ranges do not cover all triangles";
+        return PT_ERR_INVALID_ARG;
+    }
+    for (uint32_t i = 0; i < 3 * nt; ++i)
+        if (s->indices[i] >= nv) {
+            err = std::string(who) + ": vertex index out of range";
+            return PT_ERR_INVALID_ARG;
+        }
+    for (uint32_t i = 0; i < nm; ++i)
+        if (s->materials[i].type < PT_MAT_DIFFUSE || s->materials[i].type > PT_MAT_ROUGH_PLASTIC) {
+            err = std::string(who) + ": unknown material type";
+            return PT_ERR_UNSUPPORTED;
+        }
+    out.h_pos4.assign((size_t)nv * 4, 0.f);
+    out.h_nrm4.assign(s->normals ? (size_t)nv * 4 : 0, 0.f);
+    out.h_uv2.assign(s->texcoords ? (size_t)nv * 2 : 0, 0.f);
+    for (uint32_t i = 0; i < nv; ++i) {
+        for (int k = 0; k < 3; ++k) out.h_pos4[4 * i + k] = s->positions[3 * i + k];
+        if (s->normals)
+            for (int k = 0; k < 3; ++k) out.h_nrm4[4 * i + k] = s->normals[3 * i + k];
+        if (s->texcoords)
+            for (int k = 0; k < 2; ++k) out.h_uv2[2 * i + k] = s->texcoords[2 * i + k];
+    }
+    for (int k = 0; k < 3; ++k) {
+        out.lo[k] = nv ? s->positions[k] : 0.f;
+        out.hi[k] = out.lo[k];
+    }
+    for (uint32_t i = 0; i < nv; ++i)
+        for (int k = 0; k < 3; ++k) {
+            out.lo[k] = std::min(out.lo[k], s->positions[3 * i + k]);
+            out.hi[k] = std::max(out.hi[k], s->positions[3 * i + k]);
+        }
+    out.h_tri4.assign((size_t)nt * 4, 0u);
+    out.h_shapes.resize(ns);
+    // Area emitters: one per triangle of each emitting shape, in shape order
+    // (world/emitter.cpp:169-222, pt_pass.cpp:219-235).
+    out.h_emitters.clear();
+    for (uint32_t si = 0; si < ns; ++si) {
+        const pt_shape_desc &sh = s->shapes[si];
+        ShapeRec r;
+        r.material = sh.material;
+        r.flags = sh.flags;
+        if (!s->normals) r.flags &= ~(uint32_t)PT_SHAPE_HAS_NORMALS;
+        if (!s->texcoords) r.flags &= ~(uint32_t)PT_SHAPE_HAS_TEXCOORDS;
+        r.first_tri = sh.first_tri;
+        r.emitter_offset = sh.is_emitter ? (int32_t)out.h_emitters.size() : -1;
+        out.h_shapes[si] = r;
+        for (uint32_t t = sh.first_tri; t < sh.first_tri + sh.num_tris; ++t) {
+            for (int k = 0; k < 3; ++k) out.h_tri4[4 * (size_t)t + k] = s->indices[3 * (size_t)t + k];
+            out.h_tri4[4 * (size_t)t + 3] = si;
+            if (!sh.is_emitter) continue;
+            EmitterRec e;
+            std::memset(&e, 0, sizeof(e));
+            e.type = kEmitterTriArea;
+            f3 p[3], n[3];
+            f2 uv[3];
+            for (int k = 0; k < 3; ++k) {
+                uint32_t vi = s->indices[3 * (size_t)t + k];
+                p[k] = mk3(s->positions[3 * vi], s->positions[3 * vi + 1], s->positions[3 * vi + 2]);
+                f3 nn;
+                if (s->normals && (r.flags & kShapeHasNormals)) {
+                    nn = mk3(s->normals[3 * vi], s->normals[3 * vi + 1], s->normals[3 * vi + 2]);
+                } else {
+                    nn = mk3(0.f);
+                }
+                uv[k] = s->texcoords ? mk2(s->texcoords[2 * vi], s->texcoords[2 * vi + 1]) : mk2(0.f, 0.f);
+                n[k] = nn;
+            }
+            for (int k = 0; k < 3; ++k) {
+                f3 nn = n[k];
+                if (!(r.flags & kShapeHasNormals)) {
+                    nn = cross(p[1] - p[0], p[2] - p[0]);
+                    if (r.flags & kShapeNegFaceNormal) nn = -nn;
+                }
+                // util::Transform::TransformNormal normalizes component-wise.
+                float len = sqrtf(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
+                n[k] = mk3(nn.x / len, nn.y / len, nn.z / len);
+            }
+            e.p0 = f4{p[0].x, p[0].y, p[0].z, 0.f};
+            e.p1 = f4{p[1].x, p[1].y, p[1].z, 0.f};
+            e.p2 = f4{p[2].x, p[2].y, p[2].z, 0.f};
+            e.n0 = f4{n[0].x, n[0].y, n[0].z, 0.f};
+            e.n1 = f4{n[1].x, n[1].y, n[1].z, 0.f};
+            e.n2 = f4{n[2].x, n[2].y, n[2].z, 0.f};
+            e.uv01 = f4{uv[0].x, uv[0].y, uv[1].x, uv[1].y};
+            e.uv2 = f4{uv[2].x, uv[2].y, 0.f, 0.f};
+            e.area = length(cross(p[1] - p[0], p[2] - p[0])) * 0.5f;
+            e.radiance = to_texture_rec(sh.radiance);
+            e.weight = texture_weight(sh.radiance) * e.area;
+            out.h_emitters.push_back(e);
+        }
+    }
+    // EmitterHelper::ComputeProbability (world/emitter.cpp:321-337).
+    {
+        float sum = 0.f;
+        for (auto &e : out.h_emitters) sum += e.weight;
+        const size_t na = out.h_emitters.size();
+        const size_t total = na + (s->env_type != PT_ENV_NONE ? 1 : 0);
+        for (auto &e : out.h_emitters) {
+            e.select_prob = e.weight / sum * (float)na;
+            e.select_prob = e.select_prob / (float)total;
+        }
+        out.h_cdf.resize(na);
+        float acc = 0.f;
+        for (size_t i = 0; i < na; ++i) {
+            acc = acc + out.h_emitters[i].select_prob;
+            out.h_cdf[i] = acc;
+        }
+        std::memset(&out.env, 0, sizeof(out.env));
+        if (s->env_type == PT_ENV_CONSTANT) {
+            out.env.type = kEmitterConstEnv;
+            out.env.select_prob = 1.f / (float)total;
+            out.env.color = f4{s->env_radiance[0], s->env_radiance[1], s->env_radiance[2], 0.f};
+        }
+    }
+    out.h_materials.resize(nm);
+    for (uint32_t i = 0; i < nm; ++i) out.h_materials[i] = to_material_rec(s->materials[i]);
+    for (int r = 0; r < 4; ++r) {
+        out.camera.s2c[r] = f4{s->camera.sample_to_camera[4 * r], s->camera.sample_to_camera[4 * r + 1],
+                                s->camera.sample_to_camera[4 * r + 2], s->camera.sample_to_camera[4 * r + 3]};
+        out.camera.c2w[r] = f4{s->camera.camera_to_world[4 * r], s->camera.camera_to_world[4 * r + 1],
+                                s->camera.camera_to_world[4 * r + 2], s->camera.camera_to_world[4 * r + 3]};
+    }
+    out.width = s->width;
+    out.height = s->height;
+    out.max_depth = s->max_depth ? s->max_depth : 1;
+    out.num_tris = nt;
+
+    out.has_normals = s->normals != nullptr;
+    out.has_texcoords = s->texcoords != nullptr;
+    return PT_OK;
+}
+
+// --- batched views: host side ------------------------------------------------------
+constexpr uint32_t kBatchDefaultPaths = 32u << 20;
+
+uint32_t batch_path_bits(size_t views) {
+    uint32_t vb = 0;
+    while (((size_t)1 << vb) < views) ++vb;
+    return 31u - vb;
+}
+
+void batch_free_device(pt_ctx *ctx) {
+    BatchState &b = ctx->batch;
+    dfree(b.d_tri);
+    dfree(b.d_pos);
+    dfree(b.d_nrm);
+    dfree(b.d_uv);
+    dfree(b.d_shapes);
+    dfree(b.d_materials);
+    dfree(b.d_emitters);
+    dfree(b.d_cdf);
+    dfree(b.d_nodes);
+    dfree(b.d_leaf);
+    dfree(b.d_views);
+    dfree(b.d_roots);
+    b.dirty = true;
+}
+
+void batch_free_all(pt_ctx *ctx) {
+    BatchState &b = ctx->batch;
+    batch_free_device(ctx);
+    for (auto &v : b.views) {
+        dfree(v.d_accum);
+        dfree(v.d_frame);
+        dfree(v.d_accum64);
+    }
+    dfree(b.d_launch);
+    dfree(b.d_rays);
+    if (b.h_launch) (void)hipHostFree(b.h_launch);
+    b.h_launch = nullptr;
+    b.h_launch_slots = 0;
+    b = BatchState();
+}
+
+int batch_upload(pt_ctx *ctx) {
+    BatchState &b = ctx->batch;
+    if (!b.dirty) return PT_OK;
+    int rc;
+    const size_t nv = b.pos4.size() / 4;
+    if ((rc = upload(ctx, b.d_tri, b.tri4.data(), b.tri4.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, b.d_pos, reinterpret_cast<const float4 *>(b.pos4.data()), nv)) != PT_OK) return rc;
+    if (b.any_normals &&
+        (rc = upload(ctx, b.d_nrm, reinterpret_cast<const float4 *>(b.nrm4.data()), nv)) != PT_OK)
+        return rc;
+    if (b.any_texcoords &&
+        (rc = upload(ctx, b.d_uv, reinterpret_cast<const float2 *>(b.uv2.data()), nv)) != PT_OK)
+        return rc;
+    if ((rc = upload(ctx, b.d_shapes, b.shapes.data(), b.shapes.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, b.d_materials, b.materials.data(), b.materials.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, b.d_emitters, b.emitters.data(), b.emitters.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, b.d_cdf, b.cdf.data(), b.cdf.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, b.d_nodes, b.nodes.data(), b.nodes.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, b.d_leaf, b.leaf.data(), b.leaf.size())) != PT_OK) return rc;
+    std::vector<BatchViewDev> vd(b.views.size());
+    std::vector<uint32_t> roots(b.views.size());
+    for (size_t i = 0; i < b.views.size(); ++i) {
+        const BatchView &v = b.views[i];
+        BatchViewDev &d = vd[i];
+        std::memset(&d, 0, sizeof(d));
+        d.camera = v.camera;
+        d.env = v.env;
+        d.width = v.width;
+        d.height = v.height;
+        d.max_depth = v.max_depth;
+        d.root = v.root;
+        d.emitter_begin = v.emitter_begin;
+        d.emitter_count = v.emitter_count;
+        d.accum = v.d_accum;
+        d.frame = v.d_frame;
+        d.accum64 = v.d_accum64;
+        roots[i] = v.root;
+    }
+    if ((rc = upload(ctx, b.d_views, vd.data(), vd.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, b.d_roots, roots.data(), roots.size())) != PT_OK) return rc;
+    if (!b.d_launch) PT_HIP(hipMalloc((void **)&b.d_launch, sizeof(LaunchDesc)));
+    if (!b.d_rays) {
+        PT_HIP(hipMalloc((void **)&b.d_rays, 2 * sizeof(unsigned long long)));
+        PT_HIP(hipMemset(b.d_rays, 0, 2 * sizeof(unsigned long long)));
+    }
+    b.dirty = false;
+    return PT_OK;
+}
+
+// Packs every view's frames (views in order, frames in order) into launches
+// of at most `capacity` paths.
+void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::vector<LaunchDesc> &plan) {
+    plan.clear();
+    LaunchDesc cur;
+    std::memset(&cur, 0, sizeof(cur));
+    auto flush = [&]() {
+        if (cur.n_entries) plan.push_back(cur);
+        std::memset(&cur, 0, sizeof(cur));
+    };
+    for (uint32_t vi = 0; vi < (uint32_t)b.views.size(); ++vi) {
+        const BatchView &v = b.views[vi];
+        const uint32_t npix = v.width * v.height;
+        for (uint32_t f = 0; f < v.spp; ++f) {
+            const bool new_seg = cur.n_segs == 0 || cur.g[cur.n_segs - 1].view != vi;
+            if ((uint64_t)cur.total_paths + npix > capacity || cur.n_entries == kMaxLaunchEntries ||
+                (new_seg && cur.n_segs == kMaxLaunchSegs))
+                flush();
+            const bool seg_start = cur.n_segs == 0 || cur.g[cur.n_segs - 1].view != vi;
+            LaunchEntry &e = cur.e[cur.n_entries];
+            e.view = vi;
+            e.frame_seed = v.random_seed + f;
+            e.path_begin = cur.total_paths;
+            e.pixels = npix;
+            if (seg_start) {
+                LaunchSeg &g = cur.g[cur.n_segs++];
+                g.view = vi;
+                g.first_entry = cur.n_entries;
+                g.frames = 0;
+                g.pix_begin = cur.seg_pixels;
+                g.sample_cnt0 = accumulate ? v.sample_cnt + f : 0u;
+                g.accumulate = accumulate ? 1u : 0u;
+                cur.seg_pixels += npix;
+            }
+            cur.g[cur.n_segs - 1].frames += 1;
+            cur.n_entries += 1;
+            cur.total_paths += npix;
+        }
+    }
+    flush();
+}
+
+struct BatchShape {
+    uint32_t capacity, shade_block, shade_bpc, trace_bpc;
+};
+
+// The kernels of one launch (the sequence a hipGraph captures).
+void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth, const ShadeParams &sp,
+                          uint32_t path_bits, bool fp64) {
+    BatchState &b = ctx->batch;
+    SceneView sv;
+    sv.tri = b.d_tri;
+    sv.vpos = reinterpret_cast<const f4 *>(b.d_pos);
+    sv.vnrm = reinterpret_cast<const f4 *>(b.d_nrm);
+    sv.vuv = reinterpret_cast<const f2 *>(b.d_uv);
+    sv.shapes = b.d_shapes;
+    sv.materials = b.d_materials;
+    sv.emitters = b.d_emitters;
+    sv.emitter_cdf = b.d_cdf;
+    sv.num_emitters = 0;
+    sv.pad = 0;
+    std::memset(&sv.env, 0, sizeof(sv.env));
+    const DeviceBvh bvh{b.d_nodes, b.d_leaf, ctx->d_spill};
+    AovBuffers aov{nullptr, nullptr, nullptr, nullptr, 0u, 1u};
+    uint32_t *ext = ctx->d_counters;
+    uint32_t *shd = ctx->d_counters + (max_depth + 2);
+    uint32_t *twork = ctx->d_counters + 2 * (max_depth + 2);
+    uint32_t *swork = ctx->d_counters + 3 * (max_depth + 2);
+    const uint32_t tg = (uint32_t)ctx->num_cu * sh.trace_bpc;
+    const uint32_t sg = (uint32_t)ctx->num_cu * sh.shade_bpc;
+    hipLaunchKernelGGL(k_raygen_batch, dim3((uint32_t)ctx->num_cu * 32u), dim3(256), 0, ctx->stream, ctx->q[0],
+                       b.d_launch, b.d_views, path_bits, ctx->d_counters, ctx->num_counters);
+    int cur = 0;
+    for (uint32_t bb = 0; bb < max_depth; ++bb) {
+        const int nxt = cur ^ 1;
+        hipLaunchKernelGGL(k_trace<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->q[cur], bvh, ext + bb,
+                           twork + bb, ctx->d_stat, (const uint32_t *)b.d_roots, path_bits);
+        hipLaunchKernelGGL(k_shade, dim3(sg), dim3(sh.shade_block), 0, ctx->stream, sv, sp, ctx->q[cur], ext + bb,
+                           ctx->q[nxt], ext + bb + 1, ctx->sq, shd + bb, ctx->d_out_l, aov,
+                           (const BatchViewDev *)b.d_views, path_bits);
+        hipLaunchKernelGGL(k_shadow<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->sq, bvh, shd + bb,
+                           swork + bb, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2, (const uint32_t *)b.d_roots,
+                           path_bits);
+        cur = nxt;
+    }
+    hipLaunchKernelGGL(k_accumulate_batch, dim3((uint32_t)ctx->num_cu * 16u), dim3(256), 0, ctx->stream,
+                       ctx->d_out_l, b.d_launch, b.d_views, fp64 ? 1u : 0u);
+    hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters, max_depth, b.d_rays);
+}
+
+// Renders the batch's frames with one launch shape.  Returns PT_OK and the
+// wall time in *ms.
+int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool graph, double *ms,
+              uint32_t *launches) {
+    BatchState &b = ctx->batch;
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = batch_upload(ctx);
+    if (rc != PT_OK) return rc;
+    const uint32_t path_bits = batch_path_bits(b.views.size());
+    uint32_t max_depth = 1, max_npix = 0;
+    for (const auto &v : b.views) {
+        max_depth = std::max(max_depth, v.max_depth);
+        max_npix = std::max(max_npix, v.width * v.height);
+    }
+    if (sh.capacity < max_npix || sh.capacity > (1u << path_bits)) {
+        ctx->error = "pt_batch_render: paths_per_launch must hold one frame of the largest view and fit the "
+                     "path-id range";
+        return PT_ERR_CAPACITY;
+    }
+    if ((rc = ensure_queues(ctx, sh.capacity, max_depth)) != PT_OK) return rc;
+    if ((rc = ensure_spill(ctx)) != PT_OK) return rc;
+    std::vector<LaunchDesc> plan;
+    batch_plan(b, sh.capacity, true, plan);
+    if (plan.size() > b.h_launch_slots) {
+        if (b.h_launch) (void)hipHostFree(b.h_launch);
+        b.h_launch = nullptr;
+        PT_HIP(hipHostMalloc((void **)&b.h_launch, plan.size() * sizeof(LaunchDesc), hipHostMallocDefault));
+        b.h_launch_slots = (uint32_t)plan.size();
+    }
+    std::memcpy(b.h_launch, plan.data(), plan.size() * sizeof(LaunchDesc));
+    ShadeParams spar;
+    spar.max_depth = max_depth;
+    spar.rr_depth = p.rr_depth;
+    spar.mis_mode = p.mis_mode;
+    spar.aov = 0;
+    const bool fp64 = (p.render_flags & PT_RENDER_FP64_ACCUM) != 0;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    if (graph) {
+        PT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        batch_enqueue_launch(ctx, sh, max_depth, spar, path_bits, fp64);
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            ctx->error = std::string("pt_batch_render: graph capture failed: ") + hipGetErrorString(e);
+            return PT_ERR_HIP;
+        }
+    }
+    rc = PT_OK;
+    for (size_t k = 0; k < plan.size() && rc == PT_OK; ++k) {
+        hipError_t e = hipMemcpyAsync(b.d_launch, b.h_launch + k, sizeof(LaunchDesc), hipMemcpyHostToDevice,
+                                      ctx->stream);
+        if (e == hipSuccess) {
+            if (graph) e = hipGraphLaunch(ge, ctx->stream);
+            else {
+                batch_enqueue_launch(ctx, sh, max_depth, spar, path_bits, fp64);
+                e = hipGetLastError();
+            }
+        }
+        if (e != hipSuccess) {
+            ctx->error = std::string("pt_batch_render: ") + hipGetErrorString(e);
+            rc = PT_ERR_HIP;
+        }
+    }
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+    if (rc != PT_OK) return rc;
+    if (e != hipSuccess) {
+        ctx->error = std::string("pt_batch_render: ") + hipGetErrorString(e);
+        return PT_ERR_HIP;
+    }
+    uint64_t paths = 0;
+    for (auto &v : b.views) {
+        paths += (uint64_t)v.width * v.height * v.spp;
+        v.sample_cnt += v.spp;
+        v.random_seed += v.spp;
+    }
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (launches) *launches = (uint32_t)plan.size();
+    b.stats.paths += paths;
+    return PT_OK;
+}
+
+int batch_reset_films(pt_ctx *ctx) {
+    for (auto &v : ctx->batch.views) {
+        const size_t npix = (size_t)v.width * v.height;
+        v.sample_cnt = 0;
+        v.random_seed = 0;
+        PT_HIP(hipMemsetAsync(v.d_accum, 0, npix * sizeof(float4), ctx->stream));
+        PT_HIP(hipMemsetAsync(v.d_frame, 0, npix * sizeof(float4), ctx->stream));
+        PT_HIP(hipMemsetAsync(v.d_accum64, 0, npix * 3 * sizeof(double), ctx->stream));
+    }
+    PT_HIP(hipStreamSynchronize(ctx->stream));
+    return PT_OK;
+}
+
+BatchShape batch_shape(pt_ctx *ctx, const pt_batch_params &p) {
+    const BatchState &b = ctx->batch;
+    BatchShape sh;
+    sh.capacity = p.paths_per_launch ? p.paths_per_launch : (b.tuned ? b.tuning.paths_per_launch : kBatchDefaultPaths);
+    sh.shade_block = p.shade_block ? p.shade_block : (b.tuned ? b.tuning.shade_block : 256u);
+    sh.shade_bpc = p.shade_blocks_per_cu ? p.shade_blocks_per_cu : (b.tuned ? b.tuning.shade_blocks_per_cu : 8u);
+    sh.trace_bpc = p.trace_blocks_per_cu ? p.trace_blocks_per_cu
+                                         : (b.tuned ? b.tuning.trace_blocks_per_cu : ctx->trace_blocks_per_cu);
+    const uint32_t path_bits = batch_path_bits(b.views.size());
+    sh.capacity = std::min(sh.capacity, 1u << path_bits);
+    // The traversal spill area is sized for the resident grid.
+    sh.trace_bpc = std::max(1u, std::min(sh.trace_bpc, ctx->trace_blocks_per_cu));
+    sh.shade_bpc = std::max(1u, sh.shade_bpc);
+    return sh;
+}
+
 }  // namespace
 
 extern "C" {
@@ -878,6 +1543,7 @@ void pt_destroy(pt_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->guide.release();
+    batch_free_all(ctx);
     free_scene(ctx);
     free_queues(ctx);
     dfree(ctx->d_gout);
@@ -893,157 +1559,34 @@ const char *pt_last_error(const pt_ctx *ctx) { return ctx ? ctx->error.c_str() :
 
 int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
     if (!ctx) return PT_ERR_INVALID_ARG;
-    if (!s || !s->positions || !s->indices || !s->shapes || !s->materials || s->width == 0 || s->height == 0) {
-        ctx->error = "pt_set_scene: missing arrays or zero film size";
-        return PT_ERR_INVALID_ARG;
-    }
+    SceneHostData d;
+    int rc = convert_scene(s, "pt_set_scene", d, ctx->error);
+    if (rc != PT_OK) return rc;
     (void)hipSetDevice(ctx->device);
-    const uint32_t nv = s->num_vertices, nt = s->num_triangles, ns = s->num_shapes, nm = s->num_materials;
-    // Validate ranges first: nothing is uploaded for a malformed description.
-    uint32_t covered = 0;
-    for (uint32_t i = 0; i < ns; ++i) {
-        const pt_shape_desc &sh = s->shapes[i];
-        if (sh.first_tri != covered || sh.material >= nm) {
-            ctx->error = "pt_set_scene: shapes must cover the triangles in order and reference valid materials";
-            return PT_ERR_INVALID_ARG;
-        }
-        covered += sh.num_tris;
-    }
-    if (covered != nt) {
-        ctx->error = "pt_set_scene: shape triangle ranges do not cover all triangles";
-        return PT_ERR_INVALID_ARG;
-    }
-    for (uint32_t i = 0; i < 3 * nt; ++i)
-        if (s->indices[i] >= nv) {
-            ctx->error = "pt_set_scene: vertex index out of range";
-            return PT_ERR_INVALID_ARG;
-        }
-    for (uint32_t i = 0; i < nm; ++i)
-        if (s->materials[i].type < PT_MAT_DIFFUSE || s->materials[i].type > PT_MAT_ROUGH_PLASTIC) {
-            ctx->error = "pt_set_scene: unknown material type";
-            return PT_ERR_UNSUPPORTED;
-        }
     (void)hipStreamSynchronize(ctx->stream);
     free_scene(ctx);
     // The guide's position normalization follows the scene bounds.
     ctx->guide.release();
-
-    ctx->h_pos4.assign((size_t)nv * 4, 0.f);
-    ctx->h_nrm4.assign(s->normals ? (size_t)nv * 4 : 0, 0.f);
-    ctx->h_uv2.assign(s->texcoords ? (size_t)nv * 2 : 0, 0.f);
-    for (uint32_t i = 0; i < nv; ++i) {
-        for (int k = 0; k < 3; ++k) ctx->h_pos4[4 * i + k] = s->positions[3 * i + k];
-        if (s->normals)
-            for (int k = 0; k < 3; ++k) ctx->h_nrm4[4 * i + k] = s->normals[3 * i + k];
-        if (s->texcoords)
-            for (int k = 0; k < 2; ++k) ctx->h_uv2[2 * i + k] = s->texcoords[2 * i + k];
-    }
+    ctx->h_pos4.swap(d.h_pos4);
+    ctx->h_nrm4.swap(d.h_nrm4);
+    ctx->h_uv2.swap(d.h_uv2);
+    ctx->h_tri4.swap(d.h_tri4);
+    ctx->h_shapes.swap(d.h_shapes);
+    ctx->h_materials.swap(d.h_materials);
+    ctx->h_emitters.swap(d.h_emitters);
+    ctx->h_cdf.swap(d.h_cdf);
+    ctx->env = d.env;
+    ctx->camera = d.camera;
+    ctx->width = d.width;
+    ctx->height = d.height;
+    ctx->scene_max_depth = d.max_depth;
+    ctx->num_tris = d.num_tris;
     for (int k = 0; k < 3; ++k) {
-        ctx->scene_lo[k] = nv ? s->positions[k] : 0.f;
-        ctx->scene_hi[k] = ctx->scene_lo[k];
+        ctx->scene_lo[k] = d.lo[k];
+        ctx->scene_hi[k] = d.hi[k];
     }
-    for (uint32_t i = 0; i < nv; ++i)
-        for (int k = 0; k < 3; ++k) {
-            ctx->scene_lo[k] = std::min(ctx->scene_lo[k], s->positions[3 * i + k]);
-            ctx->scene_hi[k] = std::max(ctx->scene_hi[k], s->positions[3 * i + k]);
-        }
-    ctx->h_tri4.assign((size_t)nt * 4, 0u);
-    ctx->h_shapes.resize(ns);
-    // Area emitters: one per triangle of each emitting shape, in shape order
-    // (world/emitter.cpp:169-222, pt_pass.cpp:219-235).
-    ctx->h_emitters.clear();
-    for (uint32_t si = 0; si < ns; ++si) {
-        const pt_shape_desc &sh = s->shapes[si];
-        ShapeRec r;
-        r.material = sh.material;
-        r.flags = sh.flags;
-        if (!s->normals) r.flags &= ~(uint32_t)PT_SHAPE_HAS_NORMALS;
-        if (!s->texcoords) r.flags &= ~(uint32_t)PT_SHAPE_HAS_TEXCOORDS;
-        r.first_tri = sh.first_tri;
-        r.emitter_offset = sh.is_emitter ? (int32_t)ctx->h_emitters.size() : -1;
-        ctx->h_shapes[si] = r;
-        for (uint32_t t = sh.first_tri; t < sh.first_tri + sh.num_tris; ++t) {
-            for (int k = 0; k < 3; ++k) ctx->h_tri4[4 * (size_t)t + k] = s->indices[3 * (size_t)t + k];
-            ctx->h_tri4[4 * (size_t)t + 3] = si;
-            if (!sh.is_emitter) continue;
-            EmitterRec e;
-            std::memset(&e, 0, sizeof(e));
-            e.type = kEmitterTriArea;
-            f3 p[3], n[3];
-            f2 uv[3];
-            for (int k = 0; k < 3; ++k) {
-                uint32_t vi = s->indices[3 * (size_t)t + k];
-                p[k] = mk3(s->positions[3 * vi], s->positions[3 * vi + 1], s->positions[3 * vi + 2]);
-                f3 nn;
-                if (s->normals && (r.flags & kShapeHasNormals)) {
-                    nn = mk3(s->normals[3 * vi], s->normals[3 * vi + 1], s->normals[3 * vi + 2]);
-                } else {
-                    nn = mk3(0.f);
-                }
-                uv[k] = s->texcoords ? mk2(s->texcoords[2 * vi], s->texcoords[2 * vi + 1]) : mk2(0.f, 0.f);
-                n[k] = nn;
-            }
-            for (int k = 0; k < 3; ++k) {
-                f3 nn = n[k];
-                if (!(r.flags & kShapeHasNormals)) {
-                    nn = cross(p[1] - p[0], p[2] - p[0]);
-                    if (r.flags & kShapeNegFaceNormal) nn = -nn;
-                }
-                // util::Transform::TransformNormal normalizes component-wise.
-                float len = sqrtf(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
-                n[k] = mk3(nn.x / len, nn.y / len, nn.z / len);
-            }
-            e.p0 = f4{p[0].x, p[0].y, p[0].z, 0.f};
-            e.p1 = f4{p[1].x, p[1].y, p[1].z, 0.f};
-            e.p2 = f4{p[2].x, p[2].y, p[2].z, 0.f};
-            e.n0 = f4{n[0].x, n[0].y, n[0].z, 0.f};
-            e.n1 = f4{n[1].x, n[1].y, n[1].z, 0.f};
-            e.n2 = f4{n[2].x, n[2].y, n[2].z, 0.f};
-            e.uv01 = f4{uv[0].x, uv[0].y, uv[1].x, uv[1].y};
-            e.uv2 = f4{uv[2].x, uv[2].y, 0.f, 0.f};
-            e.area = length(cross(p[1] - p[0], p[2] - p[0])) * 0.5f;
-            e.radiance = to_texture_rec(sh.radiance);
-            e.weight = texture_weight(sh.radiance) * e.area;
-            ctx->h_emitters.push_back(e);
-        }
-    }
-    // EmitterHelper::ComputeProbability (world/emitter.cpp:321-337).
-    {
-        float sum = 0.f;
-        for (auto &e : ctx->h_emitters) sum += e.weight;
-        const size_t na = ctx->h_emitters.size();
-        const size_t total = na + (s->env_type != PT_ENV_NONE ? 1 : 0);
-        for (auto &e : ctx->h_emitters) {
-            e.select_prob = e.weight / sum * (float)na;
-            e.select_prob = e.select_prob / (float)total;
-        }
-        ctx->h_cdf.resize(na);
-        float acc = 0.f;
-        for (size_t i = 0; i < na; ++i) {
-            acc = acc + ctx->h_emitters[i].select_prob;
-            ctx->h_cdf[i] = acc;
-        }
-        std::memset(&ctx->env, 0, sizeof(ctx->env));
-        if (s->env_type == PT_ENV_CONSTANT) {
-            ctx->env.type = kEmitterConstEnv;
-            ctx->env.select_prob = 1.f / (float)total;
-            ctx->env.color = f4{s->env_radiance[0], s->env_radiance[1], s->env_radiance[2], 0.f};
-        }
-    }
-    ctx->h_materials.resize(nm);
-    for (uint32_t i = 0; i < nm; ++i) ctx->h_materials[i] = to_material_rec(s->materials[i]);
-    for (int r = 0; r < 4; ++r) {
-        ctx->camera.s2c[r] = f4{s->camera.sample_to_camera[4 * r], s->camera.sample_to_camera[4 * r + 1],
-                                s->camera.sample_to_camera[4 * r + 2], s->camera.sample_to_camera[4 * r + 3]};
-        ctx->camera.c2w[r] = f4{s->camera.camera_to_world[4 * r], s->camera.camera_to_world[4 * r + 1],
-                                s->camera.camera_to_world[4 * r + 2], s->camera.camera_to_world[4 * r + 3]};
-    }
-    ctx->width = s->width;
-    ctx->height = s->height;
-    ctx->scene_max_depth = s->max_depth ? s->max_depth : 1;
-    ctx->num_tris = nt;
+    const uint32_t nv = s->num_vertices;
 
-    int rc;
     if ((rc = upload(ctx, ctx->d_tri, ctx->h_tri4.data(), ctx->h_tri4.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, ctx->d_pos, reinterpret_cast<const float4 *>(ctx->h_pos4.data()), (size_t)nv)) != PT_OK) return rc;
     if (s->normals &&
@@ -1272,10 +1815,10 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
                 EventTimer t(ctx, timing);
                 if (ctx->counting)
                     hipLaunchKernelGGL(k_trace<true>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->q[cur], bvh,
-                                       ext + b, twork + b, ctx->d_stat);
+                                       ext + b, twork + b, ctx->d_stat, (const uint32_t *)nullptr, 31u);
                 else
                     hipLaunchKernelGGL(k_trace<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->q[cur], bvh,
-                                       ext + b, twork + b, ctx->d_stat);
+                                       ext + b, twork + b, ctx->d_stat, (const uint32_t *)nullptr, 31u);
                 t.stop(kKindTrace);
             }
             // The guide is sampled at vertices b < guided_depth; the vertex of the
@@ -1297,17 +1840,20 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
                                        ext + b, ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov, ga);
                 else
                     hipLaunchKernelGGL(k_shade, dim3(sg), dim3(256), 0, ctx->stream, sv, sp, ctx->q[cur], ext + b,
-                                       ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov);
+                                       ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov,
+                                       (const BatchViewDev *)nullptr, 31u);
                 t.stop(kKindShade);
             }
             {
                 EventTimer t(ctx, timing);
                 if (ctx->counting)
                     hipLaunchKernelGGL(k_shadow<true>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->sq, bvh,
-                                       shd + b, swork + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2);
+                                       shd + b, swork + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2,
+                                       (const uint32_t *)nullptr, 31u);
                 else
                     hipLaunchKernelGGL(k_shadow<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->sq, bvh,
-                                       shd + b, swork + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2);
+                                       shd + b, swork + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2,
+                                       (const uint32_t *)nullptr, 31u);
                 t.stop(kKindShadow);
             }
             cur = nxt;
@@ -1624,6 +2170,283 @@ int pt_guide_set_params(pt_ctx *ctx, const float *table, const float *weights) {
     int rc = guide_ready(ctx, "pt_guide_set_params");
     if (rc != PT_OK) return rc;
     return ctx->guide.set_params(table, weights, ctx->error) ? PT_OK : PT_ERR_HIP;
+}
+
+// ---- batched views ------------------------------------------------------------
+
+int pt_batch_clear(pt_ctx *ctx) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    batch_free_all(ctx);
+    return PT_OK;
+}
+
+int pt_batch_add_view(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uint32_t *view_index) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    if (spp == 0) {
+        ctx->error = "pt_batch_add_view: spp must be positive";
+        return PT_ERR_INVALID_ARG;
+    }
+    BatchState &b = ctx->batch;
+    if (b.views.size() >= 1024) {
+        ctx->error = "pt_batch_add_view: at most 1024 views";
+        return PT_ERR_CAPACITY;
+    }
+    SceneHostData d;
+    int rc = convert_scene(scene, "pt_batch_add_view", d, ctx->error);
+    if (rc != PT_OK) return rc;
+    (void)hipSetDevice(ctx->device);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<WideNode> nodes;
+    std::vector<LeafTri> leaf;
+    BvhBuildStats bst;
+    std::string err;
+    if (!build_wide_bvh(d.h_pos4.data(), (uint32_t)(d.h_pos4.size() / 4), d.h_tri4.data(), d.num_tris,
+                        BvhBuildOptions(), nodes, leaf, bst, err)) {
+        ctx->error = "pt_batch_add_view: " + err;
+        return PT_ERR_INVALID_ARG;
+    }
+    const uint64_t nv0 = b.pos4.size() / 4, nt0 = b.tri4.size() / 4;
+    const uint64_t node0 = b.nodes.size(), leaf0 = b.leaf.size();
+    const uint64_t nv = d.h_pos4.size() / 4;
+    if (nv0 + nv > 0xFFFFFFFFull || nt0 + d.num_tris > 0x7FFFFFFFull || node0 + nodes.size() > 0xFFFFFFFFull ||
+        leaf0 + leaf.size() > 0xFFFFFFFFull) {
+        ctx->error = "pt_batch_add_view: batch exceeds 32-bit index ranges";
+        return PT_ERR_CAPACITY;
+    }
+    const uint32_t shape0 = (uint32_t)b.shapes.size(), mat0 = (uint32_t)b.materials.size();
+    const uint32_t em0 = (uint32_t)b.emitters.size();
+    // Relocate into the shared arrays.
+    for (auto &n : nodes) {
+        n.child_base += (uint32_t)node0;
+        n.tri_base += (uint32_t)leaf0;
+    }
+    for (auto &t : leaf) t.v0.w = u2f(f2u(t.v0.w) + (uint32_t)nt0);
+    b.nodes.insert(b.nodes.end(), nodes.begin(), nodes.end());
+    b.leaf.insert(b.leaf.end(), leaf.begin(), leaf.end());
+    b.pos4.insert(b.pos4.end(), d.h_pos4.begin(), d.h_pos4.end());
+    if (d.has_normals && !b.any_normals) b.nrm4.assign(nv0 * 4, 0.f);
+    if (d.has_texcoords && !b.any_texcoords) b.uv2.assign(nv0 * 2, 0.f);
+    b.any_normals = b.any_normals || d.has_normals;
+    b.any_texcoords = b.any_texcoords || d.has_texcoords;
+    if (b.any_normals) {
+        if (d.has_normals) b.nrm4.insert(b.nrm4.end(), d.h_nrm4.begin(), d.h_nrm4.end());
+        else b.nrm4.resize(b.nrm4.size() + nv * 4, 0.f);
+    }
+    if (b.any_texcoords) {
+        if (d.has_texcoords) b.uv2.insert(b.uv2.end(), d.h_uv2.begin(), d.h_uv2.end());
+        else b.uv2.resize(b.uv2.size() + nv * 2, 0.f);
+    }
+    for (size_t t = 0; t < d.num_tris; ++t) {
+        for (int k = 0; k < 3; ++k) b.tri4.push_back(d.h_tri4[4 * t + k] + (uint32_t)nv0);
+        b.tri4.push_back(d.h_tri4[4 * t + 3] + shape0);
+    }
+    for (ShapeRec r : d.h_shapes) {
+        r.material += mat0;
+        r.first_tri += (uint32_t)nt0;  // emitter offsets stay view-local
+        b.shapes.push_back(r);
+    }
+    b.materials.insert(b.materials.end(), d.h_materials.begin(), d.h_materials.end());
+    b.emitters.insert(b.emitters.end(), d.h_emitters.begin(), d.h_emitters.end());
+    b.cdf.insert(b.cdf.end(), d.h_cdf.begin(), d.h_cdf.end());
+    BatchView v;
+    v.width = d.width;
+    v.height = d.height;
+    v.spp = spp;
+    v.max_depth = d.max_depth;
+    v.camera = d.camera;
+    v.env = d.env;
+    v.root = (uint32_t)node0;
+    v.emitter_begin = em0;
+    v.emitter_count = (uint32_t)d.h_emitters.size();
+    v.tri_count = d.num_tris;
+    const size_t npix = (size_t)v.width * v.height;
+    PT_HIP(hipMalloc((void **)&v.d_accum, npix * sizeof(float4)));
+    PT_HIP(hipMalloc((void **)&v.d_frame, npix * sizeof(float4)));
+    PT_HIP(hipMalloc((void **)&v.d_accum64, npix * 3 * sizeof(double)));
+    PT_HIP(hipMemset(v.d_accum, 0, npix * sizeof(float4)));
+    PT_HIP(hipMemset(v.d_frame, 0, npix * sizeof(float4)));
+    PT_HIP(hipMemset(v.d_accum64, 0, npix * 3 * sizeof(double)));
+    b.views.push_back(v);
+    b.dirty = true;
+    b.stats.views = (uint32_t)b.views.size();
+    b.stats.triangles += d.num_tris;
+    b.stats.build_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (view_index) *view_index = (uint32_t)b.views.size() - 1u;
+    return PT_OK;
+}
+
+int pt_batch_render(pt_ctx *ctx, const pt_batch_params *params) {
+    if (!ctx || !params) return PT_ERR_INVALID_ARG;
+    BatchState &b = ctx->batch;
+    if (b.views.empty()) {
+        ctx->error = "pt_batch_render: no views";
+        return PT_ERR_NO_SCENE;
+    }
+    if (params->mis_mode > PT_MIS_CONSISTENT || (params->shade_block != 0 && params->shade_block != 64 &&
+                                                  params->shade_block != 128 && params->shade_block != 256)) {
+        ctx->error = "pt_batch_render: bad mis_mode or shade_block";
+        return PT_ERR_INVALID_ARG;
+    }
+    (void)hipSetDevice(ctx->device);
+    if ((params->batch_flags & PT_BATCH_AUTOTUNE) && !b.tuned) {
+        pt_batch_tuning t;
+        int rc = pt_batch_autotune(ctx, params, &t);
+        if (rc != PT_OK) return rc;
+    }
+    const BatchShape sh = batch_shape(ctx, *params);
+    double ms = 0.0;
+    uint32_t launches = 0;
+    int rc = batch_run(ctx, *params, sh, (params->batch_flags & PT_BATCH_GRAPH) != 0, &ms, &launches);
+    if (rc != PT_OK) return rc;
+    b.stats.launches += launches;
+    b.stats.ms_total += ms;
+    return PT_OK;
+}
+
+int pt_batch_reset(pt_ctx *ctx) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    (void)hipSetDevice(ctx->device);
+    return batch_reset_films(ctx);
+}
+
+int pt_batch_read(pt_ctx *ctx, uint32_t view, int which, void *dst, size_t bytes) {
+    if (!ctx || !dst) return PT_ERR_INVALID_ARG;
+    BatchState &b = ctx->batch;
+    if (view >= b.views.size()) {
+        ctx->error = "pt_batch_read: no such view";
+        return PT_ERR_INVALID_ARG;
+    }
+    const BatchView &v = b.views[view];
+    const size_t npix = (size_t)v.width * v.height;
+    const void *src = nullptr;
+    size_t need = 0;
+    switch (which) {
+        case PT_BUF_FRAME: src = v.d_frame; need = npix * 16; break;
+        case PT_BUF_ACCUM: src = v.d_accum; need = npix * 16; break;
+        case PT_BUF_ACCUM64: src = v.d_accum64; need = npix * 24; break;
+        default: ctx->error = "pt_batch_read: unsupported buffer"; return PT_ERR_INVALID_ARG;
+    }
+    if (bytes < need) {
+        ctx->error = "pt_batch_read: destination too small";
+        return PT_ERR_INVALID_ARG;
+    }
+    (void)hipSetDevice(ctx->device);
+    PT_HIP(hipStreamSynchronize(ctx->stream));
+    PT_HIP(hipMemcpy(dst, src, need, hipMemcpyDeviceToHost));
+    return PT_OK;
+}
+
+int pt_batch_autotune(pt_ctx *ctx, const pt_batch_params *params, pt_batch_tuning *out) {
+    if (!ctx || !params) return PT_ERR_INVALID_ARG;
+    BatchState &b = ctx->batch;
+    if (b.views.empty()) {
+        ctx->error = "pt_batch_autotune: no views";
+        return PT_ERR_NO_SCENE;
+    }
+    (void)hipSetDevice(ctx->device);
+    // Probe: every view with a share of its frames, ~96M paths in all.
+    uint64_t total = 0;
+    uint32_t max_npix = 0;
+    for (const auto &v : b.views) {
+        total += (uint64_t)v.width * v.height * v.spp;
+        max_npix = std::max(max_npix, v.width * v.height);
+    }
+    const double share = std::min(1.0, (96.0 * (1 << 20)) / (double)std::max<uint64_t>(total, 1));
+    std::vector<uint32_t> saved(b.views.size());
+    uint64_t probe_paths = 0;
+    for (size_t i = 0; i < b.views.size(); ++i) {
+        saved[i] = b.views[i].spp;
+        b.views[i].spp = std::max(1u, (uint32_t)std::lround(share * saved[i]));
+        probe_paths += (uint64_t)b.views[i].width * b.views[i].height * b.views[i].spp;
+    }
+    const uint32_t path_bits = batch_path_bits(b.views.size());
+    pt_batch_params p = *params;
+    p.render_flags &= ~(uint32_t)PT_RENDER_FP64_ACCUM;
+    uint32_t candidates = 0;
+    int rc = PT_OK;
+    auto time_shape = [&](const BatchShape &sh, double &ns) -> int {
+        double ms = 0.0;
+        int r = batch_run(ctx, p, sh, (params->batch_flags & PT_BATCH_GRAPH) != 0, &ms, nullptr);
+        if (r != PT_OK) return r;
+        ++candidates;
+        ns = ms * 1e6 / (double)probe_paths;
+        return PT_OK;
+    };
+    BatchShape best{kBatchDefaultPaths, 256u, 8u, ctx->trace_blocks_per_cu};
+    best.capacity = std::min(best.capacity, 1u << path_bits);
+    best.capacity = std::max(best.capacity, max_npix);
+    double best_ns = 0.0, dummy = 0.0, default_ns = 0.0;
+    // warm-up (allocations, code loading), then the default shape
+    if ((rc = time_shape(best, dummy)) == PT_OK) rc = time_shape(best, best_ns);
+    default_ns = best_ns;
+    const uint32_t caps[] = {4u << 20, 8u << 20, 16u << 20, 32u << 20, 64u << 20};
+    for (uint32_t c : caps) {
+        if (rc != PT_OK) break;
+        if (c < max_npix || c > (1u << path_bits) || c == best.capacity) continue;
+        BatchShape sh = best;
+        sh.capacity = c;
+        double ns = 0.0;
+        if ((rc = time_shape(sh, ns)) == PT_OK && ns < best_ns) {
+            best_ns = ns;
+            best = sh;
+        }
+    }
+    const uint32_t shades[][2] = {{256, 4}, {256, 16}, {128, 8}, {128, 16}, {64, 16}};
+    for (auto &sb : shades) {
+        if (rc != PT_OK) break;
+        BatchShape sh = best;
+        sh.shade_block = sb[0];
+        sh.shade_bpc = sb[1];
+        double ns = 0.0;
+        if ((rc = time_shape(sh, ns)) == PT_OK && ns < best_ns) {
+            best_ns = ns;
+            best = sh;
+        }
+    }
+    for (uint32_t tb : {ctx->trace_blocks_per_cu / 2u, (ctx->trace_blocks_per_cu * 3u) / 4u}) {
+        if (rc != PT_OK) break;
+        if (tb == 0 || tb == best.trace_bpc) continue;
+        BatchShape sh = best;
+        sh.trace_bpc = tb;
+        double ns = 0.0;
+        if ((rc = time_shape(sh, ns)) == PT_OK && ns < best_ns) {
+            best_ns = ns;
+            best = sh;
+        }
+    }
+    for (size_t i = 0; i < b.views.size(); ++i) b.views[i].spp = saved[i];
+    if (rc != PT_OK) return rc;
+    if ((rc = batch_reset_films(ctx)) != PT_OK) return rc;
+    b.stats.paths = 0;
+    b.tuned = true;
+    b.tuning.paths_per_launch = best.capacity;
+    b.tuning.shade_block = best.shade_block;
+    b.tuning.shade_blocks_per_cu = best.shade_bpc;
+    b.tuning.trace_blocks_per_cu = best.trace_bpc;
+    b.tuning.candidates = candidates;
+    b.tuning.probe_paths = (uint32_t)std::min<uint64_t>(probe_paths, 0xFFFFFFFFull);
+    b.tuning.best_ns_per_path = best_ns;
+    b.tuning.default_ns_per_path = default_ns;
+    if (out) *out = b.tuning;
+    return PT_OK;
+}
+
+int pt_batch_get_stats(pt_ctx *ctx, pt_batch_stats *out) {
+    if (!ctx || !out) return PT_ERR_INVALID_ARG;
+    BatchState &b = ctx->batch;
+    *out = b.stats;
+    out->views = (uint32_t)b.views.size();
+    if (b.d_rays) {
+        (void)hipSetDevice(ctx->device);
+        unsigned long long r[2] = {0, 0};
+        PT_HIP(hipStreamSynchronize(ctx->stream));
+        PT_HIP(hipMemcpy(r, b.d_rays, sizeof(r), hipMemcpyDeviceToHost));
+        out->rays_extension = r[0];
+        out->rays_shadow = r[1];
+    }
+    return PT_OK;
 }
 
 }  // extern "C"

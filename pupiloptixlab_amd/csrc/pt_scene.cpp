@@ -608,6 +608,93 @@ void build_door_room(HostScene &s, const pt_gen_params &p) {
     s.set_camera(60.f, cam.m);
 }
 
+// Rectangle [-1,1]^2 (z = 0, normal +z) split into n x n cells.
+MeshData make_grid_rectangle_mesh(int n) {
+    MeshData m;
+    for (int j = 0; j <= n; ++j)
+        for (int i = 0; i <= n; ++i) {
+            const float u = (float)i / (float)n, v = (float)j / (float)n;
+            m.positions.insert(m.positions.end(), {2.f * u - 1.f, 2.f * v - 1.f, 0.f});
+            m.normals.insert(m.normals.end(), {0.f, 0.f, 1.f});
+            m.texcoords.insert(m.texcoords.end(), {u, v});
+        }
+    for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+            uint32_t a = (uint32_t)(j * (n + 1) + i), b = a + 1, c = a + (uint32_t)(n + 1), d = c + 1;
+            m.indices.insert(m.indices.end(), {a, b, d, a, d, c});
+        }
+    return m;
+}
+
+// Cube [-1,1]^3 with every face split into n x n cells (outward normals; the
+// face layout and winding of make_cube_mesh).
+MeshData make_grid_cube_mesh(int n) {
+    MeshData m;
+    const int axes[6] = {0, 0, 1, 1, 2, 2};
+    const float signs[6] = {-1.f, 1.f, -1.f, 1.f, -1.f, 1.f};
+    for (int f = 0; f < 6; ++f) {
+        const int a = axes[f];
+        const float sgn = signs[f];
+        const int u = (a + 1) % 3, v = (a + 2) % 3;
+        const uint32_t base = (uint32_t)(m.positions.size() / 3);
+        for (int j = 0; j <= n; ++j)
+            for (int i = 0; i <= n; ++i) {
+                float p[3], nn[3] = {0.f, 0.f, 0.f};
+                p[a] = sgn;
+                p[u] = (2.f * (float)i / (float)n - 1.f) * (sgn > 0.f ? 1.f : -1.f);
+                p[v] = 2.f * (float)j / (float)n - 1.f;
+                nn[a] = sgn;
+                m.positions.insert(m.positions.end(), p, p + 3);
+                m.normals.insert(m.normals.end(), nn, nn + 3);
+                m.texcoords.insert(m.texcoords.end(), {(float)i / (float)n, (float)j / (float)n});
+            }
+        for (int j = 0; j < n; ++j)
+            for (int i = 0; i < n; ++i) {
+                const uint32_t a0 = base + (uint32_t)(j * (n + 1) + i), b0 = a0 + 1, c0 = a0 + (uint32_t)(n + 1),
+                               d0 = c0 + 1;
+                m.indices.insert(m.indices.end(), {a0, b0, d0, a0, d0, c0});
+            }
+    }
+    return m;
+}
+
+// Seeded variant of the Cornell box: the walls and both boxes tessellated so
+// the scene has about `triangles` triangles (log-uniform in [50K, 2M] when 0),
+// a random camera in front of / inside the open box, and a film size drawn from
+// {256^2, 512^2, 768x432, 1024^2, 1920x1080}.  Max depth 8.
+void build_cornell_variant(HostScene &s, const pt_gen_params &p) {
+    SplitMix rng{p.seed * 0x9E3779B97F4A7C15ull + 0x5851F42D4C957F2Dull};
+    static const uint32_t sizes[5][2] = {{256, 256}, {512, 512}, {768, 432}, {1024, 1024}, {1920, 1080}};
+    const uint32_t pick = (uint32_t)(rng.next() % 5);
+    s.width = p.width ? p.width : sizes[pick][0];
+    s.height = p.height ? p.height : sizes[pick][1];
+    s.max_depth = p.max_depth ? p.max_depth : 8;
+    double target = p.triangles;
+    if (target == 0.0) target = std::exp(std::log(5.0e4) + (std::log(2.0e6) - std::log(5.0e4)) * rng.uniform());
+    // 5 walls + 12 box faces of 2 n^2 triangles each, plus the 2-triangle light.
+    const int n = std::max(1, (int)std::lround(std::sqrt(std::max(0.0, target - 2.0) / 34.0)));
+    uint32_t white = s.add_material(make_diffuse(0.73f, 0.73f, 0.73f, true));
+    uint32_t red = s.add_material(make_diffuse(0.65f, 0.05f, 0.05f, true));
+    uint32_t green = s.add_material(make_diffuse(0.12f, 0.45f, 0.15f, true));
+    uint32_t black = s.add_material(make_diffuse(0.f, 0.f, 0.f, true));
+    MeshData rect = make_grid_rectangle_mesh(n), cube = make_grid_cube_mesh(n), light = make_rectangle_mesh();
+    s.add_instance(rect, trs(1, 1, 1, 1, 0, 0, -90.f, 0, 0, 0), white, 0, false, nullptr);
+    s.add_instance(rect, trs(1, 1, 1, 1, 0, 0, 90.f, 0, 2, 0), white, 0, false, nullptr);
+    s.add_instance(rect, trs(1, 1, 1, 0, 0, 0, 0.f, 0, 1, -1), white, 0, false, nullptr);
+    s.add_instance(rect, trs(1, 1, 1, 0, 1, 0, 90.f, -1, 1, 0), red, 0, false, nullptr);
+    s.add_instance(rect, trs(1, 1, 1, 0, 1, 0, -90.f, 1, 1, 0), green, 0, false, nullptr);
+    s.add_instance(cube, trs(0.3f, 0.3f, 0.3f, 0, 1, 0, -17.f, 0.33f, 0.3f, 0.37f), white, 0, false, nullptr);
+    s.add_instance(cube, trs(0.3f, 0.6f, 0.3f, 0, 1, 0, 17.f, -0.335f, 0.6f, -0.29f), white, 0, false, nullptr);
+    pt_texture_desc le = make_rgb_texture(17.f, 12.f, 4.f);
+    s.add_instance(light, trs(0.235f, 0.19f, 1, 1, 0, 0, 90.f, -0.005f, 1.98f, -0.03f), black, 0, true, &le);
+    float eye[3] = {rng.range(-0.6f, 0.6f), rng.range(0.4f, 1.6f), rng.range(1.2f, 3.9f)};
+    float tgt[3] = {rng.range(-0.4f, 0.4f), rng.range(0.4f, 1.4f), rng.range(-0.9f, 0.f)};
+    float up[3] = {0.f, 1.f, 0.f};
+    Transform cam;
+    cam.look_at(eye, tgt, up);
+    s.set_camera(rng.range(30.f, 60.f), cam.m);
+}
+
 // Closed cube [-1,1]^3 of twosided diffuse walls that all emit, seen from the
 // center: every path stays inside, so the expected pixel value is
 // Le * sum_{k < max_depth} albedo^k.
@@ -635,6 +722,7 @@ bool generate_scene(int kind, const pt_gen_params &params, HostScene &out, std::
         case PT_SCENE_TERRAIN: build_terrain(out, params); return true;
         case PT_SCENE_DOOR_ROOM: build_door_room(out, params); return true;
         case PT_SCENE_FURNACE: build_furnace(out, params); return true;
+        case PT_SCENE_CORNELL_VARIANT: build_cornell_variant(out, params); return true;
         default: err = "unknown scene kind"; return false;
     }
 }
