@@ -79,6 +79,63 @@ def relmse(img, ref):
     return float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
 
 
+def batched_experiment(pt, views, seed):
+    """Config 3: `views` independent views in one wavefront loop -- seeded
+    Cornell-box variants (triangles log-uniform in [50K, 2M], random camera,
+    film from {256^2, 512^2, 768x432, 1024^2, 1920x1080}, spp from {4, 16, 64},
+    depth 8).  The launch shape comes from the autotuner; one launch's kernels
+    are replayed from a hipGraph.  Reports whole-batch Msamples/s with and
+    without the graph, and the tuner's gain over the default shape."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    t = pt.PathTracer(0)
+    t0 = time.perf_counter()
+    tris = 0
+    samples = 0
+    for v in range(views):
+        scene = pt.HostScene.generate(pt.SCENE_CORNELL_VARIANT, seed=seed * 1000 + v)
+        spp = int(rng.choice([4, 16, 64]))
+        t.batch_add_view(scene, spp)
+        tris += scene.desc.num_triangles
+        samples += scene.width * scene.height * spp
+        scene.close()
+    t_build = time.perf_counter() - t0
+    log(f"batched: {views} views, {tris} tris, {samples} samples per render, scenes + BVHs in {t_build:.1f}s")
+    tuning = t.batch_autotune(graph=True)
+    log(f"batched: tuning {tuning}")
+    t.batch_render(graph=True)  # warm-up
+    t.batch_reset()
+    t0 = time.perf_counter()
+    t.batch_render(graph=True)
+    t_graph = time.perf_counter() - t0
+    t.batch_reset()
+    t0 = time.perf_counter()
+    t.batch_render(graph=False)
+    t_direct = time.perf_counter() - t0
+    st = t.batch_stats()
+    finite = all(bool(np.all(np.isfinite(t.batch_read(v)))) for v in range(views))
+    t.close()
+    out = {
+        "workload": (f"{views} seeded Cornell-box variants (50K-2M tris each, random camera, film from "
+                     "{256^2, 512^2, 768x432, 1024^2, 1920x1080}, spp from {4, 16, 64}, depth 8) in one wavefront "
+                     "loop"),
+        "views": views,
+        "triangles": tris,
+        "samples": samples,
+        "build_seconds": round(t_build, 2),
+        "msamples_per_s_graph": round(samples / t_graph / 1e6, 2),
+        "msamples_per_s_direct": round(samples / t_direct / 1e6, 2),
+        "seconds_graph": round(t_graph, 3),
+        "seconds_direct": round(t_direct, 3),
+        "launches_per_render": st["launches"] // 3 if st["launches"] else None,
+        "tuning": tuning,
+        "all_finite": finite,
+    }
+    log(f"batched: {json.dumps(out)}")
+    return out
+
+
 def guided_experiment(pt, ref_spp, plain_spp, train_spp):
     """relMSE at equal wall-clock, guided vs plain, on the door room (room lit
     through a 5 cm door gap, ~500K triangles, 1920x1080, max depth 16).
@@ -209,6 +266,10 @@ def main():
     ap.add_argument("--guided-train-spp", type=int, default=256)
     ap.add_argument("--guided-only", action="store_true",
                     help="run only the guided experiment and print its JSON (profiling runs)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched multi-view experiment")
+    ap.add_argument("--batched-views", type=int, default=64)
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched experiment and print its JSON (profiling runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -218,6 +279,9 @@ def main():
 
     import pupiloptixlab_amd as pt
 
+    if args.batched_only:
+        print(json.dumps(batched_experiment(pt, args.batched_views, 0)), flush=True)
+        return
     if args.guided_only:
         print(json.dumps(guided_experiment(pt, args.guided_ref_spp, args.guided_plain_spp, args.guided_train_spp)),
               flush=True)
@@ -300,6 +364,9 @@ def main():
     guided = None
     if not args.no_guided:
         guided = guided_experiment(pt, args.guided_ref_spp, args.guided_plain_spp, args.guided_train_spp)
+    batched = None
+    if not args.no_batched:
+        batched = batched_experiment(pt, args.batched_views, 0)
 
     result = {
         "metric": "Msamples/s at 1080p on 1 GPU; relMSE at equal time, guided vs plain PT",
@@ -341,6 +408,7 @@ def main():
         "cpu_baseline": cpu,
         "relmse_guided_vs_plain": guided["ratio"] if guided else None,
         "guided": guided,
+        "batched": batched,
     }
     print(json.dumps(result), flush=True)
 
