@@ -155,20 +155,20 @@ PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float 
             WideNode n;
             load_node(node_index, n);
             if (stats) stats->nodes++;
-            const float sx = exp2_from_biased(n.ex), sy = exp2_from_biased(n.ey), sz = exp2_from_biased(n.ez);
+            // Slab distances of the quantized child boxes: with a = 2^e / d and
+            // b = (origin - o) / d per axis, plane q is at t = q a + b (one FMA).
+            // Rounding differs from decoding the box first by a few ulps, which
+            // the relative node margin covers; triangle tests stay exact.
+            const float ax = exp2_from_biased(n.ex) * idx, bx = (n.px - o.x) * idx;
+            const float ay = exp2_from_biased(n.ey) * idy, by = (n.py - o.y) * idy;
+            const float az = exp2_from_biased(n.ez) * idz, bz = (n.pz - o.z) * idz;
             uint32_t new_bits = 0, tri_bits = 0;
             for (int k = 0; k < 8; ++k) {
                 const uint32_t meta = n.meta[k];
                 if (meta == 0) continue;
-                float lox = n.px + (float)n.qlo_x[k] * sx;
-                float loy = n.py + (float)n.qlo_y[k] * sy;
-                float loz = n.pz + (float)n.qlo_z[k] * sz;
-                float hix = n.px + (float)n.qhi_x[k] * sx;
-                float hiy = n.py + (float)n.qhi_y[k] * sy;
-                float hiz = n.pz + (float)n.qhi_z[k] * sz;
-                float tx0 = (lox - o.x) * idx, tx1 = (hix - o.x) * idx;
-                float ty0 = (loy - o.y) * idy, ty1 = (hiy - o.y) * idy;
-                float tz0 = (loz - o.z) * idz, tz1 = (hiz - o.z) * idz;
+                const float tx0 = fmaf((float)n.qlo_x[k], ax, bx), tx1 = fmaf((float)n.qhi_x[k], ax, bx);
+                const float ty0 = fmaf((float)n.qlo_y[k], ay, by), ty1 = fmaf((float)n.qhi_y[k], ay, by);
+                const float tz0 = fmaf((float)n.qlo_z[k], az, bz), tz1 = fmaf((float)n.qhi_z[k], az, bz);
                 float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
                 float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tbest));
                 if (tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin)) {
