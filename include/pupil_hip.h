@@ -49,13 +49,14 @@ enum pt_material_type {
 /* Texture kinds of util/texture.h (RGB, checkerboard; bitmap reserved). */
 enum pt_texture_type { PT_TEX_RGB = 0, PT_TEX_CHECKER = 1, PT_TEX_BITMAP = 2 };
 
-/* Shape flags (render/geometry.h:14-21 TriMesh flags, flattened). */
+/* Shape flags (render/geometry.h:14-27 TriMesh and Sphere flags, flattened). */
 enum pt_shape_flags {
     PT_SHAPE_FLIP_NORMALS = 1,
     PT_SHAPE_FLIP_TEXCOORDS = 2,
     PT_SHAPE_HAS_NORMALS = 4,
     PT_SHAPE_HAS_TEXCOORDS = 8,
-    PT_SHAPE_NEG_FACE_NORMAL = 16
+    PT_SHAPE_NEG_FACE_NORMAL = 16,
+    PT_SHAPE_SPHERE = 32 /* analytic spheres: first_tri/num_tris index spheres[] */
 };
 
 enum pt_env_type { PT_ENV_NONE = 0, PT_ENV_CONSTANT = 1 };
@@ -90,7 +91,9 @@ typedef struct pt_material_desc {
 
 /* One shape instance flattened to world space (resource::ShapeInstance,
  * resource/shape.h:87-96 + world::RenderObject): a contiguous triangle
- * range, its material, and an area emitter when is_emitter != 0. */
+ * range (or, with PT_SHAPE_SPHERE, a contiguous range of spheres[]), its
+ * material, and an area emitter when is_emitter != 0 (one emitter per
+ * triangle or sphere, in shape order). */
 typedef struct pt_shape_desc {
     uint32_t material;
     uint32_t flags;
@@ -99,6 +102,16 @@ typedef struct pt_shape_desc {
     int32_t is_emitter;
     pt_texture_desc radiance;
 } pt_shape_desc;
+
+/* Analytic sphere (ShapeLoader<_sphere>, resource/shape.cpp:113-132 and
+ * :245-247; render/geometry.h:23-27): the unit sphere at the origin mapped
+ * by to_world (3x4 rows, object -> world; for the XML loader
+ * to_world = transform * T(center) * S(radius)).  to_world must be a
+ * similarity (rotation, uniform scale, translation), as the reference's
+ * sphere emitter assumes (world/emitter.cpp:224-241). */
+typedef struct pt_sphere_desc {
+    float to_world[12];
+} pt_sphere_desc;
 
 /* optix::Camera (render/camera.h:7-10): row-major 4x4 matrices applied to
  * column vectors. */
@@ -127,6 +140,8 @@ typedef struct pt_scene_desc {
     uint32_t width;
     uint32_t height;
     uint32_t max_depth;
+    uint32_t num_spheres;
+    const pt_sphere_desc *spheres; /* num_spheres entries, or NULL */
 } pt_scene_desc;
 
 typedef struct pt_bvh_opts {

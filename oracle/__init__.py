@@ -68,6 +68,15 @@ def lib():
         L.oracle_ggx_sample.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_void_p]
         L.oracle_intersect.restype = C.c_int
         L.oracle_intersect.argtypes = [C.c_void_p] * 5 + [C.c_void_p]
+        L.oracle_atan2.restype = C.c_float
+        L.oracle_atan2.argtypes = [C.c_float, C.c_float]
+        L.oracle_acos.restype = C.c_float
+        L.oracle_acos.argtypes = [C.c_float]
+        L.oracle_sphere_texcoord.restype = None
+        L.oracle_sphere_texcoord.argtypes = [C.c_void_p, C.c_void_p]
+        L.oracle_intersect_sphere.restype = C.c_int
+        L.oracle_intersect_sphere.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_float,
+                                              _fp]
         L.oracle_emitter_table.restype = C.c_int
         L.oracle_emitter_table.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                            C.POINTER(C.c_uint32), _fp]
@@ -170,6 +179,27 @@ def intersect(o, d, v0, v1, v2):
     hit = lib().oracle_intersect(_f32(o).ctypes.data, _f32(d).ctypes.data, _f32(v0).ctypes.data,
                                  _f32(v1).ctypes.data, _f32(v2).ctypes.data, out.ctypes.data)
     return bool(hit), out
+
+
+def atan2(y, x):
+    return lib().oracle_atan2(float(y), float(x))
+
+
+def acos(z):
+    return lib().oracle_acos(float(z))
+
+
+def sphere_texcoord(d):
+    out = np.zeros(2, np.float32)
+    lib().oracle_sphere_texcoord(_f32(d).ctypes.data, out.ctypes.data)
+    return out
+
+
+def intersect_sphere(o, d, center, radius, tmin=0.0, tmax=np.inf):
+    t = C.c_float()
+    hit = lib().oracle_intersect_sphere(_f32(o).ctypes.data, _f32(d).ctypes.data, _f32(center).ctypes.data,
+                                        float(radius), float(tmin), float(tmax), C.byref(t))
+    return bool(hit), t.value
 
 
 def emitter_table(desc, capacity=1 << 20):

@@ -11,6 +11,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -110,6 +111,44 @@ V3 uni_hemi(float u1, float u2) {
     return v3(st * c, st * s, fabsf(z));
 }
 float uni_hemi_pdf(V3 v) { return v.z > 0.f ? INV_PI_F * 0.5f : 0.f; }
+V3 uni_sphere(float u1, float u2) {  // UniformSampleSphere (optix/util.h:38-43)
+    float z = 1.f - 2.f * u1, st = sqrtf(fmaxf(0.f, 1.f - z * z)), phi = 2.0f * PI_F * u2, s, c;
+    sincos_or(phi, s, c);
+    return v3(st * c, st * s, z);
+}
+
+// atan on [0, 1] from the Taylor series of atan (through u^19) after the
+// reduction atan(a) = pi/4 + atan((a-1)/(a+1)) for a > tan(pi/8); atan2 by
+// octant symmetry; acos(z) = atan2(sqrt((1-z)(1+z)), z).
+// Pinned by: tests against numpy float64 arctan2/arccos (a few ulp).
+float atan01(float a) {
+    float base = 0.f, u = a;
+    if (a > 0.41421356f) { u = (a - 1.f) / (a + 1.f); base = 0.785398163397448f; }
+    const float u2 = u * u;
+    const float coef[8] = {1.f / 17.f, -1.f / 15.f, 1.f / 13.f, -1.f / 11.f, 1.f / 9.f, -1.f / 7.f, 1.f / 5.f, -1.f / 3.f};
+    float p = -1.f / 19.f;
+    for (int i = 0; i < 8; ++i) p = coef[i] + u2 * p;
+    return base + (u + u * (u2 * p));
+}
+float atan2_or(float y, float x) {
+    float ax = fabsf(x), ay = fabsf(y);
+    float hi = fmaxf(ax, ay), lo = fminf(ax, ay);
+    if (hi == 0.f) return 0.f;
+    float r = atan01(lo / hi);
+    if (ay > ax) r = 1.57079632679490f - r;
+    if (x < 0.f) r = 3.14159265358979f - r;
+    return y < 0.f ? -r : r;
+}
+float acos_or(float z) {
+    z = fminf(fmaxf(z, -1.f), 1.f);
+    return atan2_or(sqrtf((1.f - z) * (1.f + z)), z);
+}
+V2 sphere_uv(V3 p) {  // GetSphereTexcoord (optix/util.h:117-128)
+    float phi = atan2_or(p.y, p.x);
+    if (phi < 0.f) phi = phi + PI_F * 2.f;
+    float theta = acos_or(p.z);
+    return V2{phi * INV_PI_F * 0.5f, theta * INV_PI_F};
+}
 V3 reflect_local(V3 v) { return v3(-v.x, -v.y, v.z); }
 V3 reflect_about(V3 v, V3 n) { return add(neg(v), mul(2.f * dot3(v, n), n)); }
 V3 refract_local(V3 v, float ct, float eta) {
@@ -515,19 +554,74 @@ bool tri_hit(V3 o, const RayK &k, V3 v0, V3 v1, V3 v2, float &t, float &b1, floa
     return true;
 }
 
+// Ray / sphere (Haines et al., Ray Tracing Gems 2019, ch. 7): discriminant
+// from the squared distance of the center to the ray line, roots c/q and q/a.
+// Nearest root in [tmin, tmax].  Pinned by: float64 closed-form roots and
+// brute-force agreement (tests).
+bool sph_hit(V3 o, V3 d, V3 c, float r, float tmin, float tmax, float &t) {
+    V3 f = sub(o, c);
+    float a = dot3(d, d), b = dot3(f, d);
+    V3 l = sub(f, mul(b / a, d));
+    float r2 = r * r;
+    float disc = a * (r2 - dot3(l, l));
+    if (!(disc >= 0.f)) return false;
+    float sq = sqrtf(disc);
+    float q = b >= 0.f ? -(b + sq) : -(b - sq);
+    if (q == 0.f) return false;
+    float t0 = (dot3(f, f) - r2) / q, t1 = q / a;
+    if (t0 > t1) std::swap(t0, t1);
+    if (t0 >= tmin && t0 <= tmax) { t = t0; return true; }
+    if (t1 >= tmin && t1 <= tmax) { t = t1; return true; }
+    return false;
+}
+
+const uint32_t SPHERE_BIT = 1u << 30;
+
+// Sphere placement as in SetSphereAreaEmitter (world/emitter.cpp:224-241):
+// center = M (0,0,0), radius = |M (1,0,0) - center|.
+void sphere_cr(const float *m, V3 &c, float &r) {
+    c = v3(m[3], m[7], m[11]);
+    V3 px = v3(m[0] + m[3], m[4] + m[7], m[8] + m[11]);
+    r = len3(sub(c, px));
+}
+
 // Binary BVH over float boxes (median split on the widest centroid axis).
 struct BNode { float lo[3], hi[3]; uint32_t left_or_first, count; };  // count > 0: leaf
 
 struct Accel {
     std::vector<BNode> nodes;
-    std::vector<uint32_t> prims;
+    std::vector<uint32_t> prims;  // primitive ids (spheres carry SPHERE_BIT)
     std::vector<V3> p0, p1, p2;  // per triangle vertices
+    std::vector<V3> sc;          // per sphere center
+    std::vector<float> sr;       // per sphere radius
 };
 
+// Closest-hit candidate test shared by the BVH walk and the brute force.
+inline bool prim_hit(const Accel &acc, uint32_t p, V3 o, V3 d, const RayK &k, float tmin, float tmax, float &t,
+                     float &b1, float &b2) {
+    b1 = b2 = 0.f;
+    if (p & SPHERE_BIT) {
+        const uint32_t i = p & (SPHERE_BIT - 1u);
+        return sph_hit(o, d, acc.sc[i], acc.sr[i], tmin, tmax, t);
+    }
+    return tri_hit(o, k, acc.p0[p], acc.p1[p], acc.p2[p], t, b1, b2);
+}
+
 void build_accel(const or_scene &s, Accel &acc) {
-    const uint32_t nt = s.num_triangles;
+    const uint32_t nt = s.num_triangles, ns = s.num_spheres, np = nt + ns;
     acc.p0.resize(nt); acc.p1.resize(nt); acc.p2.resize(nt);
-    std::vector<float> cen(3 * (size_t)nt), blo(3 * (size_t)nt), bhi(3 * (size_t)nt);
+    acc.sc.resize(ns); acc.sr.resize(ns);
+    std::vector<float> cen(3 * (size_t)np), blo(3 * (size_t)np), bhi(3 * (size_t)np);
+    for (uint32_t i = 0; i < ns; ++i) {
+        sphere_cr(s.spheres + 12 * (size_t)i, acc.sc[i], acc.sr[i]);
+        const float cc[3] = {acc.sc[i].x, acc.sc[i].y, acc.sc[i].z};
+        const size_t j = nt + (size_t)i;
+        for (int k = 0; k < 3; ++k) {
+            blo[3 * j + k] = cc[k] - acc.sr[i];
+            bhi[3 * j + k] = cc[k] + acc.sr[i];
+            cen[3 * j + k] = cc[k];
+        }
+    }
     for (uint32_t t = 0; t < nt; ++t) {
         const uint32_t *ix = s.indices + 3 * (size_t)t;
         const float *a = s.positions + 3 * (size_t)ix[0], *b = s.positions + 3 * (size_t)ix[1], *c = s.positions + 3 * (size_t)ix[2];
@@ -540,14 +634,15 @@ void build_accel(const or_scene &s, Accel &acc) {
             cen[3 * t + k] = 0.5f * (blo[3 * t + k] + bhi[3 * t + k]);
         }
     }
-    acc.prims.resize(nt);
-    for (uint32_t i = 0; i < nt; ++i) acc.prims[i] = i;
+    // Work on slots 0..np-1 (triangles, then spheres); prims[] maps them to ids.
+    std::vector<uint32_t> slot(np);
+    for (uint32_t i = 0; i < np; ++i) slot[i] = i;
     acc.nodes.clear();
-    acc.nodes.reserve(2 * (size_t)nt + 1);
+    acc.nodes.reserve(2 * (size_t)np + 1);
     struct Item { uint32_t node, first, count; };
     std::vector<Item> work;
     acc.nodes.push_back(BNode{});
-    work.push_back(Item{0, 0, nt});
+    work.push_back(Item{0, 0, np});
     while (!work.empty()) {
         Item it = work.back();
         work.pop_back();
@@ -555,7 +650,7 @@ void build_accel(const or_scene &s, Accel &acc) {
         float clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (int k = 0; k < 3; ++k) { n.lo[k] = INFINITY; n.hi[k] = -INFINITY; }
         for (uint32_t i = it.first; i < it.first + it.count; ++i) {
-            uint32_t p = acc.prims[i];
+            uint32_t p = slot[i];
             for (int k = 0; k < 3; ++k) {
                 n.lo[k] = std::min(n.lo[k], blo[3 * p + k]);
                 n.hi[k] = std::max(n.hi[k], bhi[3 * p + k]);
@@ -572,7 +667,7 @@ void build_accel(const or_scene &s, Accel &acc) {
         int axis = 0;
         for (int k = 1; k < 3; ++k) if (chi[k] - clo[k] > chi[axis] - clo[axis]) axis = k;
         uint32_t mid = it.first + it.count / 2;
-        std::nth_element(acc.prims.begin() + it.first, acc.prims.begin() + mid, acc.prims.begin() + it.first + it.count,
+        std::nth_element(slot.begin() + it.first, slot.begin() + mid, slot.begin() + it.first + it.count,
                          [&](uint32_t a, uint32_t b) { return cen[3 * a + axis] < cen[3 * b + axis]; });
         uint32_t l = (uint32_t)acc.nodes.size();
         acc.nodes.push_back(BNode{});
@@ -583,6 +678,8 @@ void build_accel(const or_scene &s, Accel &acc) {
         work.push_back(Item{l, it.first, mid - it.first});
         work.push_back(Item{l + 1, mid, it.first + it.count - mid});
     }
+    acc.prims.resize(np);
+    for (uint32_t i = 0; i < np; ++i) acc.prims[i] = slot[i] < nt ? slot[i] : (SPHERE_BIT | (slot[i] - nt));
 }
 
 struct HitRec { float t; uint32_t prim; float b1, b2; };
@@ -624,7 +721,7 @@ bool accel_trace(const Accel &acc, V3 o, V3 d, float tmin, float tmax, bool any,
             uint32_t p = acc.prims[i];
             if (tris_v) (*tris_v)++;
             float t, b1, b2;
-            if (!tri_hit(o, k, acc.p0[p], acc.p1[p], acc.p2[p], t, b1, b2)) continue;
+            if (!prim_hit(acc, p, o, d, k, tmin, tmax, t, b1, b2)) continue;
             if (t >= tmin && (t < tbest || (t == tbest && p < h.prim))) {
                 h.t = t; h.prim = p; h.b1 = b1; h.b2 = b2;
                 tbest = t;
@@ -640,9 +737,11 @@ bool brute_trace(const Accel &acc, V3 o, V3 d, float tmin, float tmax, bool any,
     h.prim = 0xFFFFFFFFu;
     h.b1 = h.b2 = 0.f;
     RayK k = ray_k(d);
-    for (uint32_t p = 0; p < acc.p0.size(); ++p) {
+    const uint32_t nt = (uint32_t)acc.p0.size(), np = nt + (uint32_t)acc.sc.size();
+    for (uint32_t i = 0; i < np; ++i) {
+        const uint32_t p = i < nt ? i : (SPHERE_BIT | (i - nt));
         float t, b1, b2;
-        if (!tri_hit(o, k, acc.p0[p], acc.p1[p], acc.p2[p], t, b1, b2)) continue;
+        if (!prim_hit(acc, p, o, d, k, tmin, tmax, t, b1, b2)) continue;
         if (t >= tmin && (t < h.t || (t == h.t && p < h.prim))) {
             h.t = t; h.prim = p; h.b1 = b1; h.b2 = b2;
             if (any) return true;
@@ -655,8 +754,10 @@ bool brute_trace(const Accel &acc, V3 o, V3 d, float tmin, float tmax, bool any,
 // World: per-triangle area emitters and selection probabilities
 // (world/emitter.cpp:169-222, 321-337), camera, environment.
 struct AreaEmitter {
+    bool sphere;     // SphereEmitter (sphere.h): center p[0], radius rad
     V3 p[3], n[3];
     V2 uv[3];
+    float rad;
     float area, weight, sel;
     or_texture radiance;
 };
@@ -665,6 +766,8 @@ struct World {
     const or_scene *s;
     Accel acc;
     std::vector<int32_t> tri_shape;
+    std::vector<int32_t> sph_shape;
+    std::vector<std::array<float, 12>> sph_w2o;  // world -> object rows
     std::vector<int32_t> shape_emitter_offset;
     std::vector<AreaEmitter> emitters;
     std::vector<float> cdf;  // running sums of selection probabilities
@@ -683,15 +786,52 @@ void build_world(const or_scene &s, World &w) {
     w.s = &s;
     build_accel(s, w.acc);
     w.tri_shape.assign(s.num_triangles, 0);
+    w.sph_shape.assign(s.num_spheres, 0);
+    w.sph_w2o.resize(s.num_spheres);
+    for (uint32_t k = 0; k < s.num_spheres; ++k) {
+        // Inverse of the affine [A | t] in double precision: [A^-1 | -A^-1 t].
+        const float *m = s.spheres + 12 * (size_t)k;
+        double A[3][3], inv[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) A[i][j] = m[4 * i + j];
+        double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                     A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                // cofactor of A[j][i] / det
+                int r0 = (j + 1) % 3, r1 = (j + 2) % 3, c0 = (i + 1) % 3, c1 = (i + 2) % 3;
+                inv[i][j] = (A[r0][c0] * A[r1][c1] - A[r0][c1] * A[r1][c0]) / det;
+            }
+        for (int i = 0; i < 3; ++i) {
+            double tt = -(inv[i][0] * m[3] + inv[i][1] * m[7] + inv[i][2] * m[11]);
+            for (int j = 0; j < 3; ++j) w.sph_w2o[k][4 * i + j] = (float)inv[i][j];
+            w.sph_w2o[k][4 * i + 3] = (float)tt;
+        }
+    }
     w.shape_emitter_offset.assign(s.num_shapes, -1);
     w.emitters.clear();
     for (uint32_t si = 0; si < s.num_shapes; ++si) {
         const or_shape &sh = s.shapes[si];
         if (sh.is_emitter) w.shape_emitter_offset[si] = (int32_t)w.emitters.size();
+        if (sh.flags & 32u) {
+            for (uint32_t k = sh.first_tri; k < sh.first_tri + sh.num_tris; ++k) {
+                w.sph_shape[k] = (int32_t)si;
+                if (!sh.is_emitter) continue;
+                AreaEmitter e{};
+                e.sphere = true;
+                e.p[0] = w.acc.sc[k];
+                e.rad = w.acc.sr[k];
+                e.area = 4 * 3.14159265358979323846f * e.rad * e.rad;  // emitter.cpp:238
+                e.radiance = sh.radiance;
+                e.weight = tex_weight(sh.radiance) * e.area;
+                w.emitters.push_back(e);
+            }
+            continue;
+        }
         for (uint32_t t = sh.first_tri; t < sh.first_tri + sh.num_tris; ++t) {
             w.tri_shape[t] = (int32_t)si;
             if (!sh.is_emitter) continue;
-            AreaEmitter e;
+            AreaEmitter e{};
             for (int k = 0; k < 3; ++k) {
                 uint32_t vi = s.indices[3 * (size_t)t + k];
                 e.p[k] = v3(s.positions[3 * vi], s.positions[3 * vi + 1], s.positions[3 * vi + 2]);
@@ -744,9 +884,32 @@ int select_emitter(const World &w, float p) {
 
 struct Geo { V3 pos, n; V2 uv; };
 
-// Geometry::GetHitLocalGeometry (render/geometry.h:60-93, 176-180).
-void hit_geo(const World &w, uint32_t prim, float b1, float b2, V3 ray_dir, Geo &g, int &mat, int &emitter) {
+// Geometry::GetHitLocalGeometry (render/geometry.h:60-100, 176-180).
+void hit_geo(const World &w, uint32_t prim, float t_hit, float b1, float b2, V3 ray_o, V3 ray_dir, Geo &g, int &mat,
+             int &emitter) {
     const or_scene &s = *w.s;
+    if (prim & SPHERE_BIT) {
+        // Sphere: position o + t d; object-space point -> texcoord; normal by
+        // the inverse transpose of object -> world (transposed w2o).
+        const uint32_t k = prim & (SPHERE_BIT - 1u);
+        const int si = w.sph_shape[k];
+        const or_shape &sh = s.shapes[si];
+        const float *M = w.sph_w2o[k].data();
+        g.pos = add(ray_o, mul(t_hit, ray_dir));
+        V4 hp{g.pos.x, g.pos.y, g.pos.z, 1.f};
+        V3 lp = v3(dot4(V4{M[0], M[1], M[2], M[3]}, hp), dot4(V4{M[4], M[5], M[6], M[7]}, hp),
+                   dot4(V4{M[8], M[9], M[10], M[11]}, hp));
+        g.uv = sphere_uv(norm3(lp));
+        V3 n = v3(M[0] * lp.x + M[4] * lp.y + M[8] * lp.z, M[1] * lp.x + M[5] * lp.y + M[9] * lp.z,
+                  M[2] * lp.x + M[6] * lp.y + M[10] * lp.z);
+        n = norm3(n);
+        if (sh.flags & 1u) n = mul(n, -1.f);
+        mat = (int)sh.material;
+        if (dot3(neg(ray_dir), n) < 0.f && s.materials[sh.material].twosided) n = neg(n);
+        g.n = n;
+        emitter = w.shape_emitter_offset[si] >= 0 ? w.shape_emitter_offset[si] + (int)(k - sh.first_tri) : -1;
+        return;
+    }
     const uint32_t *ix = s.indices + 3 * (size_t)prim;
     const int si = w.tri_shape[prim];
     const or_shape &sh = s.shapes[si];
@@ -779,6 +942,22 @@ void hit_geo(const World &w, uint32_t prim, float b1, float b2, V3 ray_dir, Geo 
 struct ESample { V3 rad, wi; float dist, pdf; };
 
 void area_sample(const AreaEmitter &e, const Geo &hit, V2 xi, ESample &r) {  // area.h:17-35
+    if (e.sphere) {  // sphere.h:14-33
+        V3 t = uni_sphere(xi.x, xi.y);
+        V3 pos = add(mul(t, e.rad), e.p[0]);
+        V3 nrm = norm3(t);
+        r.rad = tex_sample(e.radiance, sphere_uv(t));
+        r.wi = norm3(sub(pos, hit.pos));
+        float nol = dot3(hit.n, r.wi), lnol = dot3(nrm, neg(r.wi));
+        r.pdf = 0.f;
+        r.dist = 0.f;
+        if (nol > 0.f && lnol > 0.f) {
+            float dist = len3(sub(pos, hit.pos));
+            r.pdf = dist * dist / (lnol * e.area);
+            r.dist = dist;
+        }
+        return;
+    }
     V3 t = sample_tri(xi.x, xi.y);
     V3 pos = add(add(mul(e.p[0], t.x), mul(e.p[1], t.y)), mul(e.p[2], t.z));
     V3 nrm = norm3(add(add(mul(e.n[0], t.x), mul(e.n[1], t.y)), mul(e.n[2], t.z)));
@@ -845,7 +1024,7 @@ void trace_path(const World &w, const or_params &p, uint32_t max_depth, uint32_t
         out.albedo = v3s(0.f);
         out.normal = v3s(0.f);
     } else {
-        hit_geo(w, h.prim, h.b1, h.b2, ray_d, g, mat, emitter);
+        hit_geo(w, h.prim, h.t, h.b1, h.b2, ray_o, ray_d, g, mat, emitter);
         bsdf = make_bsdf(s.materials[mat], g.uv);
         if (emitter >= 0) radiance = add(radiance, tex_sample(w.emitters[emitter].radiance, g.uv));
         out.albedo = albedo_of(bsdf);
@@ -931,7 +1110,7 @@ void trace_path(const World &w, const or_params &p, uint32_t max_depth, uint32_t
             env_radiance = mul(env_radiance, mul(throughput, mis));
             break;
         }
-        hit_geo(w, h.prim, h.b1, h.b2, ray_d, g, mat, emitter);
+        hit_geo(w, h.prim, h.t, h.b1, h.b2, ray_o, ray_d, g, mat, emitter);
         bsdf = make_bsdf(s.materials[mat], g.uv);
         if (emitter >= 0) {
             const AreaEmitter &e = w.emitters[emitter];
@@ -949,9 +1128,11 @@ void trace_path(const World &w, const or_params &p, uint32_t max_depth, uint32_t
                 float lpdf = epdf;
                 if (consistent) {
                     // The pdf next-event estimation would assign to this point:
-                    // it uses the emitter's own vertex normals (area.h:19-33).
+                    // it uses the emitter's own normals (area.h:19-33,
+                    // sphere.h:14-33).
                     float wv = 1.f - h.b1 - h.b2;
-                    V3 en = norm3(add(add(mul(e.n[0], wv), mul(e.n[1], h.b1)), mul(e.n[2], h.b2)));
+                    V3 en = e.sphere ? norm3(sub(g.pos, e.p[0]))
+                                     : norm3(add(add(mul(e.n[0], wv), mul(e.n[1], h.b1)), mul(e.n[2], h.b2)));
                     float nl = dot3(en, dir);
                     float dist = len3(sub(ray_o, g.pos));
                     lpdf = nl > 0.f ? dist * dist / (nl * e.area) : 0.f;
@@ -1110,6 +1291,19 @@ int oracle_intersect(const float *o, const float *d, const float *v0, const floa
         return 0;
     out[0] = t; out[1] = b1; out[2] = b2;
     return 1;
+}
+
+float oracle_atan2(float y, float x) { return atan2_or(y, x); }
+float oracle_acos(float z) { return acos_or(z); }
+void oracle_sphere_texcoord(const float *dir, float *uv) {
+    V2 t = sphere_uv(v3(dir[0], dir[1], dir[2]));
+    uv[0] = t.x;
+    uv[1] = t.y;
+}
+int oracle_intersect_sphere(const float *o, const float *d, const float *center, float radius, float tmin,
+                            float tmax, float *t) {
+    return sph_hit(v3(o[0], o[1], o[2]), v3(d[0], d[1], d[2]), v3(center[0], center[1], center[2]), radius, tmin,
+                   tmax, *t) ? 1 : 0;
 }
 
 int oracle_emitter_table(const or_scene *scene, float *sel, float *area, uint32_t capacity, uint32_t *count,

@@ -7,7 +7,9 @@
  * The product (pupiloptixlab_amd) never includes, links or calls it.
  *
  * Scene input: a struct with the byte layout of pt_scene_desc (declared here
- * independently as or_scene).  All functions are thread-safe.
+ * independently as or_scene).  Shapes whose flags hold 32 are analytic
+ * spheres: first_tri/num_tris then index spheres[], and sphere k has
+ * primitive id (1 << 30) | k.  All functions are thread-safe.
  */
 #ifndef PUPIL_ORACLE_H
 #define PUPIL_ORACLE_H
@@ -59,6 +61,8 @@ typedef struct or_scene {
     float sample_to_camera[16];
     float camera_to_world[16];
     uint32_t width, height, max_depth;
+    uint32_t num_spheres;
+    const float *spheres; /* 12 floats each: object -> world 3x4 rows of the unit sphere */
 } or_scene;
 
 typedef struct or_params {
@@ -106,6 +110,14 @@ float oracle_fresnel_conductor(float eta, float k, float cos_i);
 float oracle_ggx_d(const float *wh, float alpha);
 float oracle_ggx_pdf(const float *wo, const float *wh, float alpha);
 void oracle_ggx_sample(const float *wo, float alpha, float u1, float u2, float *wh);
+/* Sphere unit functions: atan2 and acos as the oracle evaluates them, the
+ * sphere texcoord (phi / 2pi, theta / pi) of a unit direction, and the ray /
+ * sphere test (nearest root in [tmin, tmax]; returns 1 on hit). */
+float oracle_atan2(float y, float x);
+float oracle_acos(float z);
+void oracle_sphere_texcoord(const float *dir, float *uv);
+int oracle_intersect_sphere(const float *o, const float *d, const float *center, float radius, float tmin,
+                            float tmax, float *t);
 /* Intersection of one ray with one triangle (watertight); returns 1 on hit. */
 int oracle_intersect(const float *o, const float *d, const float *v0, const float *v1, const float *v2,
                      float *t_b1_b2);

@@ -32,8 +32,9 @@ STATUS = {
 MAT_DIFFUSE, MAT_DIELECTRIC, MAT_ROUGH_DIELECTRIC, MAT_CONDUCTOR, MAT_ROUGH_CONDUCTOR, MAT_PLASTIC, \
     MAT_ROUGH_PLASTIC = range(1, 8)
 TEX_RGB, TEX_CHECKER, TEX_BITMAP = 0, 1, 2
-SHAPE_FLIP_NORMALS, SHAPE_FLIP_TEXCOORDS, SHAPE_HAS_NORMALS, SHAPE_HAS_TEXCOORDS, SHAPE_NEG_FACE_NORMAL = \
-    1, 2, 4, 8, 16
+SHAPE_FLIP_NORMALS, SHAPE_FLIP_TEXCOORDS, SHAPE_HAS_NORMALS, SHAPE_HAS_TEXCOORDS, SHAPE_NEG_FACE_NORMAL, \
+    SHAPE_SPHERE = 1, 2, 4, 8, 16, 32
+PRIM_SPHERE = 1 << 30  # primitive ids of spheres: PRIM_SPHERE | sphere index
 ENV_NONE, ENV_CONSTANT = 0, 1
 MIS_REFERENCE, MIS_CONSISTENT = 0, 1
 RENDER_AOV, RENDER_FP64_ACCUM, RENDER_TIMING, RENDER_GUIDE, RENDER_GUIDE_TRAIN = 1, 2, 4, 8, 16
@@ -64,6 +65,10 @@ class CameraDesc(C.Structure):
     _fields_ = [("sample_to_camera", C.c_float * 16), ("camera_to_world", C.c_float * 16)]
 
 
+class SphereDesc(C.Structure):
+    _fields_ = [("to_world", C.c_float * 12)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("num_shapes", C.c_uint32),
                 ("num_materials", C.c_uint32), ("positions", C.POINTER(C.c_float)),
@@ -71,7 +76,8 @@ class SceneDesc(C.Structure):
                 ("indices", C.POINTER(C.c_uint32)), ("shapes", C.POINTER(ShapeDesc)),
                 ("materials", C.POINTER(MaterialDesc)), ("env_type", C.c_uint32),
                 ("env_radiance", C.c_float * 3), ("camera", CameraDesc), ("width", C.c_uint32),
-                ("height", C.c_uint32), ("max_depth", C.c_uint32)]
+                ("height", C.c_uint32), ("max_depth", C.c_uint32), ("num_spheres", C.c_uint32),
+                ("spheres", C.POINTER(SphereDesc))]
 
 
 class BvhOpts(C.Structure):
@@ -271,6 +277,13 @@ class HostScene:
     def shapes(self):
         d = self.desc
         return [d.shapes[i] for i in range(d.num_shapes)]
+
+    def spheres(self):
+        """Object -> world 3x4 matrices of the analytic spheres (N x 3 x 4)."""
+        d = self.desc
+        if d.num_spheres == 0:
+            return np.zeros((0, 3, 4), dtype=np.float32)
+        return np.ctypeslib.as_array(C.cast(d.spheres, C.POINTER(C.c_float)), shape=(d.num_spheres, 3, 4))
 
     def materials(self):
         d = self.desc

@@ -8,10 +8,11 @@
 //
 // Result contract (independent of tree shape): the closest hit is the
 // lexicographic minimum of (t, primitive id) over all triangles whose
-// watertight intersection has t in [tmin, tmax]; a shadow query reports
-// whether any such triangle exists.  Node tests are widened by a relative
-// margin so rounding in the slab test never culls a triangle the exact test
-// would accept.
+// watertight intersection has t in [tmin, tmax] and all spheres with a root
+// in [tmin, tmax] (the nearest such root); a shadow query reports whether
+// any such primitive exists.  Node tests are widened by a relative margin so
+// rounding in the slab test never culls a primitive the exact test would
+// accept.
 #pragma once
 
 #include "pt_math.h"
@@ -31,13 +32,20 @@ struct WideNode {           // 80 bytes, read as five 16-byte loads
 };
 static_assert(sizeof(WideNode) == 80, "wide node must be 80 bytes");
 
-// Leaf triangles in traversal order: three float4 (v0.xyz + primitive id bits,
-// v1.xyz, v2.xyz).
+// Leaf primitives in traversal order, three float4 each.  Triangle: v0.xyz +
+// primitive id bits, v1.xyz, v2.xyz (v1.w = 0).  Sphere: v0 = (center,
+// primitive id bits), v1 = (radius, 0, 0, tag bits 1), v2 unused.
 struct LeafTri {
     f4 v0;
     f4 v1;
     f4 v2;
 };
+
+// Primitive ids: triangles are numbered from 0; sphere k has id
+// kPrimSphere | k.
+constexpr uint32_t kPrimSphere = 1u << 30;
+constexpr uint32_t kPrimIndexMask = kPrimSphere - 1u;
+PT_HD bool leaf_is_sphere(const LeafTri &l) { return f2u(l.v1.w) != 0u; }
 
 constexpr float kNodeMargin = 1e-4f;     // relative widening of node intervals
 constexpr uint32_t kMaxBvhDepth = 64;    // builder guarantees depth <= this
@@ -111,6 +119,42 @@ PT_HD bool intersect_triangle(f3 o, const RayPre &rp, f3 v0, f3 v1, f3 v2, float
     return true;
 }
 
+// Ray/sphere test after Haines et al., "Precision Improvements for
+// Ray/Sphere Intersection" (Ray Tracing Gems, 2019, ch. 7): the discriminant
+// from the squared distance of the center to the ray line, and the roots as
+// c/q and q/a with q = -(b + sign(b) sqrt(disc)), which avoids cancellation.
+// Returns the nearest root in [tmin, tmax].  With t >= tmin a ray
+// leaving the surface (root near 0) finds the far side only when it points
+// inward, as a surface-bounded sphere intersector does.
+PT_HD bool intersect_sphere(f3 o, f3 d, f3 c, float r, float tmin, float tmax, float &t) {
+    const f3 f = o - c;
+    const float a = dot(d, d);
+    const float b = dot(f, d);  // half the linear coefficient
+    const f3 l = f - (b / a) * d;
+    const float r2 = r * r;
+    const float disc = a * (r2 - dot(l, l));
+    if (!(disc >= 0.f)) return false;
+    const float sq = sqrtf(disc);
+    const float q = b >= 0.f ? -(b + sq) : -(b - sq);
+    if (q == 0.f) return false;
+    const float cc = dot(f, f) - r2;
+    float t0 = cc / q, t1 = q / a;
+    if (t0 > t1) {
+        const float tmp = t0;
+        t0 = t1;
+        t1 = tmp;
+    }
+    if (t0 >= tmin && t0 <= tmax) {
+        t = t0;
+        return true;
+    }
+    if (t1 >= tmin && t1 <= tmax) {
+        t = t1;
+        return true;
+    }
+    return false;
+}
+
 PT_HD float exp2_from_biased(uint8_t e) { return u2f(((uint32_t)e) << 23); }
 
 struct TraverseStats {
@@ -124,8 +168,9 @@ struct TraverseStats {
 // stored at position (slot ^ octant) so popping the lowest bit visits slots
 // near-to-far for the ray's direction octant.
 // `root` is the index of the tree's root node (several trees can share one
-// node array).
-template <bool kAnyHit, typename Stack, typename NodeLoad, typename TriLoad>
+// node array).  With kSpheres false the sphere test is compiled out (for
+// trees without sphere leaves).
+template <bool kAnyHit, bool kSpheres = true, typename Stack, typename NodeLoad, typename TriLoad>
 PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float tmin, float tmax,
                         Stack &stack, Hit &hit, TraverseStats *stats, uint32_t root = 0u) {
     const RayPre rp = ray_precompute(d);
@@ -195,9 +240,16 @@ PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float 
             LeafTri tri;
             load_tri(t_base + i, tri);
             if (stats) stats->tris++;
-            float t, b1, b2;
-            if (intersect_triangle(o, rp, f3{tri.v0.x, tri.v0.y, tri.v0.z}, f3{tri.v1.x, tri.v1.y, tri.v1.z},
-                                   f3{tri.v2.x, tri.v2.y, tri.v2.z}, t, b1, b2)) {
+            float t, b1 = 0.f, b2 = 0.f;
+            bool hit_prim;
+            if (kSpheres && leaf_is_sphere(tri)) {
+                hit_prim = intersect_sphere(o, d, f3{tri.v0.x, tri.v0.y, tri.v0.z}, tri.v1.x, tmin, tmax, t);
+            } else {
+                hit_prim = intersect_triangle(o, rp, f3{tri.v0.x, tri.v0.y, tri.v0.z},
+                                              f3{tri.v1.x, tri.v1.y, tri.v1.z}, f3{tri.v2.x, tri.v2.y, tri.v2.z},
+                                              t, b1, b2);
+            }
+            if (hit_prim) {
                 uint32_t prim = f2u(tri.v0.w);
                 if (t >= tmin && (t < tbest || (t == tbest && prim < hit.prim))) {
                     if (kAnyHit) {

@@ -164,8 +164,9 @@ private:
 class WideCollapser {
 public:
     WideCollapser(const std::vector<BNode> &bn, const std::vector<PrimRef> &refs, const float *pos4,
-                  const uint32_t *idx4, std::vector<WideNode> &nodes, std::vector<LeafTri> &tris)
-        : bn_(bn), refs_(refs), pos4_(pos4), idx4_(idx4), nodes_(nodes), tris_(tris) {}
+                  const uint32_t *idx4, const float *spheres4, std::vector<WideNode> &nodes,
+                  std::vector<LeafTri> &tris)
+        : bn_(bn), refs_(refs), pos4_(pos4), idx4_(idx4), sph4_(spheres4), nodes_(nodes), tris_(tris) {}
 
     bool run(uint32_t root, std::string &err) {
         nodes_.clear();
@@ -329,6 +330,15 @@ private:
                 n.meta[s] = (uint8_t)((unary[c.count] << 5) | tri_off);
                 for (uint32_t k = 0; k < c.count; ++k) {
                     uint32_t prim = refs_[c.first + k].prim;
+                    if (prim & kPrimSphere) {
+                        const float *q = sph4_ + 4 * (size_t)(prim & kPrimIndexMask);
+                        LeafTri ls;
+                        ls.v0 = f4{q[0], q[1], q[2], u2f(prim)};
+                        ls.v1 = f4{q[3], 0.f, 0.f, u2f(1u)};
+                        ls.v2 = f4{0.f, 0.f, 0.f, 0.f};
+                        tris_.push_back(ls);
+                        continue;
+                    }
                     const uint32_t *ix = idx4_ + 4 * (size_t)prim;
                     const float *p0 = pos4_ + 4 * (size_t)ix[0];
                     const float *p1 = pos4_ + 4 * (size_t)ix[1];
@@ -375,6 +385,7 @@ private:
     const std::vector<PrimRef> &refs_;
     const float *pos4_;
     const uint32_t *idx4_;
+    const float *sph4_;
     std::vector<WideNode> &nodes_;
     std::vector<LeafTri> &tris_;
 };
@@ -382,20 +393,41 @@ private:
 }  // namespace
 
 bool build_wide_bvh(const float *positions4, uint32_t num_vertices, const uint32_t *indices4, uint32_t num_tris,
-                    const BvhBuildOptions &opt, std::vector<WideNode> &nodes, std::vector<LeafTri> &tris,
-                    BvhBuildStats &stats, std::string &err) {
+                    const float *spheres4, uint32_t num_spheres, const BvhBuildOptions &opt,
+                    std::vector<WideNode> &nodes, std::vector<LeafTri> &tris, BvhBuildStats &stats,
+                    std::string &err) {
     auto t0 = std::chrono::steady_clock::now();
     nodes.clear();
     tris.clear();
     stats = BvhBuildStats{};
-    if (num_tris == 0) {
+    if (num_tris >= kPrimSphere || num_spheres >= kPrimSphere) {
+        err = "too many primitives";
+        return false;
+    }
+    if (num_tris + (uint64_t)num_spheres == 0) {
         WideNode empty;
         std::memset(&empty, 0, sizeof(empty));
         empty.ex = empty.ey = empty.ez = 127;
         nodes.push_back(empty);
         return true;
     }
-    std::vector<PrimRef> refs(num_tris);
+    std::vector<PrimRef> refs((size_t)num_tris + num_spheres);
+    for (uint32_t k = 0; k < num_spheres; ++k) {
+        const float *q = spheres4 + 4 * (size_t)k;
+        if (!std::isfinite(q[0]) || !std::isfinite(q[1]) || !std::isfinite(q[2]) || !std::isfinite(q[3]) ||
+            !(q[3] > 0.f)) {
+            err = "invalid sphere";
+            return false;
+        }
+        PrimRef r;
+        for (int a = 0; a < 3; ++a) {
+            r.box.lo[a] = q[a] - q[3];
+            r.box.hi[a] = q[a] + q[3];
+            r.c[a] = q[a];
+        }
+        r.prim = kPrimSphere | k;
+        refs[(size_t)num_tris + k] = r;
+    }
     for (uint32_t t = 0; t < num_tris; ++t) {
         PrimRef r;
         for (int k = 0; k < 3; ++k) {
@@ -419,7 +451,7 @@ bool build_wide_bvh(const float *positions4, uint32_t num_vertices, const uint32
     BinaryBuilder bb(refs, std::min(kMaxBins, std::max(4, opt.num_bins)), threads);
     uint32_t root = bb.build();
     stats.binary_nodes = (uint32_t)bb.nodes().size();
-    WideCollapser wc(bb.nodes(), refs, positions4, indices4, nodes, tris);
+    WideCollapser wc(bb.nodes(), refs, positions4, indices4, spheres4, nodes, tris);
     if (!wc.run(root, err)) return false;
     stats.wide_nodes = (uint32_t)nodes.size();
     stats.leaf_tris = (uint32_t)tris.size();

@@ -88,6 +88,9 @@ int pt_host_trace_rays(const pt_scene_desc *s, const float *rays, uint32_t count
                        uint64_t *node_visits, uint64_t *tri_tests) {
     if (!s || (!rays && count) || (!hits && count)) return PT_ERR_INVALID_ARG;
     if (s->num_triangles && (!s->positions || !s->indices)) return PT_ERR_INVALID_ARG;
+    if (s->num_spheres && !s->spheres) return PT_ERR_INVALID_ARG;
+    std::vector<float> sph4((size_t)s->num_spheres * 4, 0.f);
+    for (uint32_t k = 0; k < s->num_spheres; ++k) pupil::sphere_center_radius(s->spheres[k].to_world, &sph4[4 * k]);
     std::vector<float> pos4((size_t)s->num_vertices * 4, 0.f);
     for (uint32_t i = 0; i < s->num_vertices; ++i)
         for (int k = 0; k < 3; ++k) pos4[4 * i + k] = s->positions[3 * i + k];
@@ -98,8 +101,8 @@ int pt_host_trace_rays(const pt_scene_desc *s, const float *rays, uint32_t count
     std::vector<pupil::LeafTri> leaf;
     pupil::BvhBuildStats st;
     std::string err;
-    if (!pupil::build_wide_bvh(pos4.data(), s->num_vertices, tri4.data(), s->num_triangles, pupil::BvhBuildOptions{},
-                               nodes, leaf, st, err))
+    if (!pupil::build_wide_bvh(pos4.data(), s->num_vertices, tri4.data(), s->num_triangles, sph4.data(),
+                               s->num_spheres, pupil::BvhBuildOptions{}, nodes, leaf, st, err))
         return PT_ERR_INVALID_ARG;
     uint32_t *h = static_cast<uint32_t *>(hits);
     uint64_t nv = 0, tv = 0;

@@ -85,7 +85,7 @@ struct LdsStack {
     __device__ bool empty() const { return sp == 0; }
 };
 
-template <bool kAnyHit, bool kCount>
+template <bool kAnyHit, bool kCount, bool kSph>
 __device__ __forceinline__ bool trace_one(const DeviceBvh &bvh, uint2 *lds_col, uint32_t gtid, f3 o, f3 d,
                                           float tmin, float tmax, Hit &h, TraverseStats &st, uint32_t root = 0u) {
     LdsStack stack{lds_col, bvh.spill + (size_t)gtid * kSpillEntries, (uint32_t)kTraceBlock, 0u};
@@ -97,7 +97,7 @@ __device__ __forceinline__ bool trace_one(const DeviceBvh &bvh, uint2 *lds_col, 
         t.v1 = f4{b.x, b.y, b.z, b.w};
         t.v2 = f4{c.x, c.y, c.z, c.w};
     };
-    return traverse_bvh<kAnyHit>(ln, lt, o, d, tmin, tmax, stack, h, kCount ? &st : nullptr, root);
+    return traverse_bvh<kAnyHit, kSph>(ln, lt, o, d, tmin, tmax, stack, h, kCount ? &st : nullptr, root);
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
@@ -158,7 +158,7 @@ __device__ __forceinline__ uint32_t wave_fetch(uint32_t *work, uint32_t n) {
 // Several scenes can share the queues (batched views): the view of a path
 // sits above bit `path_bits` of its id and selects its tree's root in `roots`
 // (null for a single scene, where path_bits is 31).
-template <bool kCount>
+template <bool kCount, bool kSph>
 __global__ __launch_bounds__(kTraceBlock) void k_trace(PathQueue q, DeviceBvh bvh, const uint32_t *count_ptr,
                                                        uint32_t *work, unsigned long long *stat,
                                                        const uint32_t *roots, uint32_t path_bits) {
@@ -174,8 +174,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(PathQueue q, DeviceBvh bv
             const float4 r0 = q.ray0[i], r1 = q.ray1[i];
             const uint32_t root = roots ? roots[f2u(r0.w) >> path_bits] : 0u;
             Hit h;
-            trace_one<false, kCount>(bvh, lds + threadIdx.x, gtid, f3of(r0), f3of(r1), kExtTmin, kMaxDistance, h,
-                                     st, root);
+            trace_one<false, kCount, kSph>(bvh, lds + threadIdx.x, gtid, f3of(r0), f3of(r1), kExtTmin,
+                                           kMaxDistance, h, st, root);
             q.hit[i] = make_float4(h.t, u2f(h.prim), h.b1, h.b2);
         }
     }
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(PathQueue q, DeviceBvh bv
 // --- shadow (any-hit) traversal ----------------------------------------------------
 // Shadow-ray targets: bit 31 set = output slot, else next-queue slot; the
 // view sits in bits [path_bits, 31), the slot below.
-template <bool kCount>
+template <bool kCount, bool kSph>
 __global__ __launch_bounds__(kTraceBlock) void k_shadow(ShadowQueue sq, DeviceBvh bvh, const uint32_t *count_ptr,
                                                         uint32_t *work, PathQueue next, float4 *out_l,
                                                         unsigned long long *stat, const uint32_t *roots,
@@ -208,8 +208,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(ShadowQueue sq, DeviceBv
             s1 = sq.s1[i];
             const uint32_t root = roots ? roots[(f2u(s1.w) & 0x7FFFFFFFu) >> path_bits] : 0u;
             Hit h;
-            occluded = trace_one<true, kCount>(bvh, lds + threadIdx.x, gtid, f3of(s0), f3of(s1), kShadowEps, s0.w,
-                                               h, st, root);
+            occluded = trace_one<true, kCount, kSph>(bvh, lds + threadIdx.x, gtid, f3of(s0), f3of(s1), kShadowEps,
+                                                     s0.w, h, st, root);
         }
         if (!occluded) {
             const float4 c = sq.s2[i];
@@ -298,9 +298,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneView scene, ShadeParams prm,
                 sv.env = bv.env;
                 ShadeParams pv = prm;
                 pv.max_depth = bv.max_depth;
-                shade_path(sv, pv, ps, prim, h.z, h.w, so);
+                shade_path(sv, pv, ps, prim, h.x, h.z, h.w, so);
             } else {
-                shade_path(scene, prm, ps, prim, h.z, h.w, so);
+                shade_path(scene, prm, ps, prim, h.x, h.z, h.w, so);
             }
         }
         const uint32_t j = wave_alloc(valid && so.cont, count_out);
@@ -455,7 +455,7 @@ __global__ __launch_bounds__(256) void k_shade_guided(SceneView scene, ShadePara
             ps.rng.seed = f2u(lr.z);
             ps.flags = f2u(lr.w);
             prim = f2u(h.y);
-            scatter = guided_begin(scene, prm, ps, prim, h.z, h.w, ga.rs, path, v, so);
+            scatter = guided_begin(scene, prm, ps, prim, h.x, h.z, h.w, ga.rs, path, v, so);
         }
         bool use_guide = ga.gout != nullptr && scatter && v.guidable;
         GuideMixture m;
@@ -670,7 +670,7 @@ __global__ void k_copy(const float4 *__restrict__ src, float4 *__restrict__ dst,
 }
 
 // Closest-hit / any-hit over caller rays (pt_trace_rays).
-template <bool kAnyHit>
+template <bool kAnyHit, bool kSph>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(const float *rays, uint32_t count, DeviceBvh bvh,
                                                             uint4 *hits) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
@@ -679,10 +679,31 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(const float *rays, u
     for (uint32_t i = gtid; i < count; i += gridDim.x * blockDim.x) {
         const float *r = rays + 8 * (size_t)i;
         Hit h;
-        trace_one<kAnyHit, false>(bvh, lds + threadIdx.x, gtid, mk3(r[0], r[1], r[2]), mk3(r[4], r[5], r[6]), r[3],
-                                  r[7], h, st);
+        trace_one<kAnyHit, false, kSph>(bvh, lds + threadIdx.x, gtid, mk3(r[0], r[1], r[2]), mk3(r[4], r[5], r[6]),
+                                        r[3], r[7], h, st);
         hits[i] = make_uint4(f2u(h.t), h.prim, f2u(h.b1), f2u(h.b2));
     }
+}
+
+// Traversal variants: counting on/off x sphere leaves present/absent (a
+// triangle-only tree runs without the sphere test, like an OptiX pipeline
+// built without the sphere primitive flag, optix/pipeline.cpp:14-28).
+using TraceKernel = void (*)(PathQueue, DeviceBvh, const uint32_t *, uint32_t *, unsigned long long *,
+                             const uint32_t *, uint32_t);
+using ShadowKernel = void (*)(ShadowQueue, DeviceBvh, const uint32_t *, uint32_t *, PathQueue, float4 *,
+                              unsigned long long *, const uint32_t *, uint32_t);
+using TraceRaysKernel = void (*)(const float *, uint32_t, DeviceBvh, uint4 *);
+TraceKernel trace_kernel(bool count, bool sph) {
+    return count ? (sph ? k_trace<true, true> : k_trace<true, false>)
+                 : (sph ? k_trace<false, true> : k_trace<false, false>);
+}
+ShadowKernel shadow_kernel(bool count, bool sph) {
+    return count ? (sph ? k_shadow<true, true> : k_shadow<true, false>)
+                 : (sph ? k_shadow<false, true> : k_shadow<false, false>);
+}
+TraceRaysKernel trace_rays_kernel(bool any_hit, bool sph) {
+    return any_hit ? (sph ? k_trace_rays<true, true> : k_trace_rays<true, false>)
+                   : (sph ? k_trace_rays<false, true> : k_trace_rays<false, false>);
 }
 
 }  // namespace
@@ -707,6 +728,7 @@ struct BatchState {
     std::vector<float> pos4, nrm4, uv2;
     std::vector<uint32_t> tri4;
     std::vector<ShapeRec> shapes;
+    std::vector<SphereRec> spheres;
     std::vector<MaterialRec> materials;
     std::vector<EmitterRec> emitters;
     std::vector<float> cdf;
@@ -718,6 +740,7 @@ struct BatchState {
     float4 *d_pos = nullptr, *d_nrm = nullptr;
     float2 *d_uv = nullptr;
     ShapeRec *d_shapes = nullptr;
+    SphereRec *d_spheres = nullptr;
     MaterialRec *d_materials = nullptr;
     EmitterRec *d_emitters = nullptr;
     float *d_cdf = nullptr;
@@ -747,19 +770,22 @@ struct pt_ctx {
     std::vector<float> h_pos4, h_nrm4, h_uv2;
     std::vector<uint32_t> h_tri4;
     std::vector<ShapeRec> h_shapes;
+    std::vector<SphereRec> h_spheres;
+    std::vector<float> h_sph4;  // world center xyz + radius per sphere
     std::vector<MaterialRec> h_materials;
     std::vector<EmitterRec> h_emitters;
     std::vector<float> h_cdf;
     EnvRec env{};
     CameraRec camera{};
     uint32_t width = 0, height = 0, scene_max_depth = 1;
-    uint32_t num_tris = 0;
+    uint32_t num_tris = 0, num_spheres = 0;
 
     // device scene
     uint32_t *d_tri = nullptr;
     float4 *d_pos = nullptr, *d_nrm = nullptr;
     float2 *d_uv = nullptr;
     ShapeRec *d_shapes = nullptr;
+    SphereRec *d_spheres = nullptr;
     MaterialRec *d_materials = nullptr;
     EmitterRec *d_emitters = nullptr;
     float *d_cdf = nullptr;
@@ -839,6 +865,7 @@ void free_scene(pt_ctx *ctx) {
     dfree(ctx->d_nrm);
     dfree(ctx->d_uv);
     dfree(ctx->d_shapes);
+    dfree(ctx->d_spheres);
     dfree(ctx->d_materials);
     dfree(ctx->d_emitters);
     dfree(ctx->d_cdf);
@@ -968,6 +995,7 @@ SceneView scene_view(pt_ctx *ctx) {
     v.vnrm = reinterpret_cast<const f4 *>(ctx->d_nrm);
     v.vuv = reinterpret_cast<const f2 *>(ctx->d_uv);
     v.shapes = ctx->d_shapes;
+    v.spheres = ctx->d_spheres;
     v.materials = ctx->d_materials;
     v.emitters = ctx->d_emitters;
     v.emitter_cdf = ctx->d_cdf;
@@ -1031,12 +1059,14 @@ struct SceneHostData {
     std::vector<float> h_pos4, h_nrm4, h_uv2;
     std::vector<uint32_t> h_tri4;
     std::vector<ShapeRec> h_shapes;
+    std::vector<SphereRec> h_spheres;
+    std::vector<float> h_sph4;
     std::vector<MaterialRec> h_materials;
     std::vector<EmitterRec> h_emitters;
     std::vector<float> h_cdf;
     EnvRec env{};
     CameraRec camera{};
-    uint32_t width = 0, height = 0, max_depth = 1, num_tris = 0;
+    uint32_t width = 0, height = 0, max_depth = 1, num_tris = 0, num_spheres = 0;
     float lo[3]{}, hi[3]{};
     bool has_normals = false, has_texcoords = false;
 };
@@ -1045,24 +1075,49 @@ struct SceneHostData {
 // (pt_pass.cpp:127-240, world/world.cpp:108-146, world/emitter.cpp:169-337).
 // Nothing is produced for a malformed description.
 int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, std::string &err) {
-    if (!s || !s->positions || !s->indices || !s->shapes || !s->materials || s->width == 0 || s->height == 0) {
+    if (!s || (s->num_vertices && !s->positions) || (s->num_triangles && !s->indices) ||
+        (s->num_shapes && !s->shapes) || (s->num_materials && !s->materials) || (s->num_spheres && !s->spheres) ||
+        s->width == 0 || s->height == 0) {
         err = std::string(who) + ": missing arrays or zero film size";
         return PT_ERR_INVALID_ARG;
     }
     const uint32_t nv = s->num_vertices, nt = s->num_triangles, ns = s->num_shapes, nm = s->num_materials;
+    const uint32_t nsph = s->num_spheres;
+    if (nt >= kPrimSphere || nsph >= kPrimSphere) {
+        err = std::string(who) + ": too many primitives";
+        return PT_ERR_CAPACITY;
+    }
     // Validate ranges first: nothing is uploaded for a malformed description.
-    uint32_t covered = 0;
+    uint32_t covered = 0, covered_sph = 0;
     for (uint32_t i = 0; i < ns; ++i) {
         const pt_shape_desc &sh = s->shapes[i];
-        if (sh.first_tri != covered || sh.material >= nm) {
-            err = std::string(who) + ": shapes must cover the triangles in order and reference valid materials";
+        uint32_t &cov = (sh.flags & PT_SHAPE_SPHERE) ? covered_sph : covered;
+        if (sh.first_tri != cov || sh.material >= nm) {
+            err = std::string(who) +
+                  ": shapes must cover the triangles (and spheres) in order and reference valid materials";
             return PT_ERR_INVALID_ARG;
         }
-        covered += sh.num_tris;
+        cov += sh.num_tris;
     }
-    if (covered != nt) {
-        err = std::string(who) + ": shape triangle ranges do not cover all triangles";
+    if (covered != nt || covered_sph != nsph) {
+        err = std::string(who) + ": shape ranges do not cover all triangles and spheres";
         return PT_ERR_INVALID_ARG;
+    }
+    for (uint32_t k = 0; k < nsph; ++k) {
+        const float *m = s->spheres[k].to_world;
+        bool finite = true;
+        for (int i = 0; i < 12; ++i) finite = finite && std::isfinite(m[i]);
+        double c[3][3];
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) c[j][i] = m[4 * i + j];  // column j
+        auto dotc = [&](int a, int b) { return c[a][0] * c[b][0] + c[a][1] * c[b][1] + c[a][2] * c[b][2]; };
+        const double n0 = dotc(0, 0);
+        const double tol = 1e-4 * n0;
+        if (!finite || !(n0 > 0.0) || std::fabs(dotc(1, 1) - n0) > tol || std::fabs(dotc(2, 2) - n0) > tol ||
+            std::fabs(dotc(0, 1)) > tol || std::fabs(dotc(0, 2)) > tol || std::fabs(dotc(1, 2)) > tol) {
+            err = std::string(who) + ": sphere to_world must be a finite similarity transform (uniform scale)";
+            return PT_ERR_UNSUPPORTED;
+        }
     }
     for (uint32_t i = 0; i < 3 * nt; ++i)
         if (s->indices[i] >= nv) {
@@ -1085,14 +1140,46 @@ int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, s
             for (int k = 0; k < 2; ++k) out.h_uv2[2 * i + k] = s->texcoords[2 * i + k];
     }
     for (int k = 0; k < 3; ++k) {
-        out.lo[k] = nv ? s->positions[k] : 0.f;
-        out.hi[k] = out.lo[k];
+        out.lo[k] = INFINITY;
+        out.hi[k] = -INFINITY;
     }
     for (uint32_t i = 0; i < nv; ++i)
         for (int k = 0; k < 3; ++k) {
             out.lo[k] = std::min(out.lo[k], s->positions[3 * i + k]);
             out.hi[k] = std::max(out.hi[k], s->positions[3 * i + k]);
         }
+    // Spheres: world center = to_world * (0, 0, 0), radius = |to_world * (1,
+    // 0, 0) - center| (world/emitter.cpp:224-241); rows of the inverse
+    // transform for shading.
+    out.h_sph4.assign((size_t)nsph * 4, 0.f);
+    out.h_spheres.assign(nsph, SphereRec{});
+    for (uint32_t k = 0; k < nsph; ++k) {
+        const float *m = s->spheres[k].to_world;
+        float *q = &out.h_sph4[4 * (size_t)k];
+        sphere_center_radius(m, q);
+        for (int a = 0; a < 3; ++a) {
+            out.lo[a] = std::min(out.lo[a], q[a] - q[3]);
+            out.hi[a] = std::max(out.hi[a], q[a] + q[3]);
+        }
+        // Inverse of [A | t] in double: [A^-1 | -A^-1 t].
+        const double a00 = m[0], a01 = m[1], a02 = m[2], a10 = m[4], a11 = m[5], a12 = m[6];
+        const double a20 = m[8], a21 = m[9], a22 = m[10];
+        const double c00 = a11 * a22 - a12 * a21, c01 = a02 * a21 - a01 * a22, c02 = a01 * a12 - a02 * a11;
+        const double c10 = a12 * a20 - a10 * a22, c11 = a00 * a22 - a02 * a20, c12 = a02 * a10 - a00 * a12;
+        const double c20 = a10 * a21 - a11 * a20, c21 = a01 * a20 - a00 * a21, c22 = a00 * a11 - a01 * a10;
+        const double det = a00 * c00 + a01 * c10 + a02 * c20;
+        const double inv[3][3] = {{c00 / det, c01 / det, c02 / det},
+                                  {c10 / det, c11 / det, c12 / det},
+                                  {c20 / det, c21 / det, c22 / det}};
+        const double t[3] = {m[3], m[7], m[11]};
+        SphereRec &rec = out.h_spheres[k];
+        for (int i = 0; i < 3; ++i) {
+            const double ti = -(inv[i][0] * t[0] + inv[i][1] * t[1] + inv[i][2] * t[2]);
+            rec.w2o[i] = f4{(float)inv[i][0], (float)inv[i][1], (float)inv[i][2], (float)ti};
+        }
+    }
+    for (int k = 0; k < 3; ++k)
+        if (!(out.lo[k] <= out.hi[k])) out.lo[k] = out.hi[k] = 0.f;
     out.h_tri4.assign((size_t)nt * 4, 0u);
     out.h_shapes.resize(ns);
     // Area emitters: one per triangle of each emitting shape, in shape order
@@ -1107,6 +1194,25 @@ int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, s
         if (!s->texcoords) r.flags &= ~(uint32_t)PT_SHAPE_HAS_TEXCOORDS;
         r.first_tri = sh.first_tri;
         r.emitter_offset = sh.is_emitter ? (int32_t)out.h_emitters.size() : -1;
+        if (sh.flags & PT_SHAPE_SPHERE) {
+            r.flags = sh.flags & (uint32_t)(PT_SHAPE_SPHERE | PT_SHAPE_FLIP_NORMALS);
+            out.h_shapes[si] = r;
+            for (uint32_t k = sh.first_tri; k < sh.first_tri + sh.num_tris; ++k) {
+                out.h_spheres[k].shape = si;
+                if (!sh.is_emitter) continue;
+                // SetSphereAreaEmitter (world/emitter.cpp:224-241).
+                EmitterRec e;
+                std::memset(&e, 0, sizeof(e));
+                e.type = kEmitterSphere;
+                const float rad = out.h_sph4[4 * k + 3];
+                e.p0 = f4{out.h_sph4[4 * k], out.h_sph4[4 * k + 1], out.h_sph4[4 * k + 2], rad};
+                e.area = 4.f * 3.14159265358979323846f * rad * rad;
+                e.radiance = to_texture_rec(sh.radiance);
+                e.weight = texture_weight(sh.radiance) * e.area;
+                out.h_emitters.push_back(e);
+            }
+            continue;
+        }
         out.h_shapes[si] = r;
         for (uint32_t t = sh.first_tri; t < sh.first_tri + sh.num_tris; ++t) {
             for (int k = 0; k < 3; ++k) out.h_tri4[4 * (size_t)t + k] = s->indices[3 * (size_t)t + k];
@@ -1188,6 +1294,7 @@ int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, s
     out.height = s->height;
     out.max_depth = s->max_depth ? s->max_depth : 1;
     out.num_tris = nt;
+    out.num_spheres = nsph;
 
     out.has_normals = s->normals != nullptr;
     out.has_texcoords = s->texcoords != nullptr;
@@ -1210,6 +1317,7 @@ void batch_free_device(pt_ctx *ctx) {
     dfree(b.d_nrm);
     dfree(b.d_uv);
     dfree(b.d_shapes);
+    dfree(b.d_spheres);
     dfree(b.d_materials);
     dfree(b.d_emitters);
     dfree(b.d_cdf);
@@ -1250,6 +1358,7 @@ int batch_upload(pt_ctx *ctx) {
         (rc = upload(ctx, b.d_uv, reinterpret_cast<const float2 *>(b.uv2.data()), nv)) != PT_OK)
         return rc;
     if ((rc = upload(ctx, b.d_shapes, b.shapes.data(), b.shapes.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, b.d_spheres, b.spheres.data(), b.spheres.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, b.d_materials, b.materials.data(), b.materials.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, b.d_emitters, b.emitters.data(), b.emitters.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, b.d_cdf, b.cdf.data(), b.cdf.size())) != PT_OK) return rc;
@@ -1341,6 +1450,7 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
     sv.vnrm = reinterpret_cast<const f4 *>(b.d_nrm);
     sv.vuv = reinterpret_cast<const f2 *>(b.d_uv);
     sv.shapes = b.d_shapes;
+    sv.spheres = b.d_spheres;
     sv.materials = b.d_materials;
     sv.emitters = b.d_emitters;
     sv.emitter_cdf = b.d_cdf;
@@ -1360,14 +1470,15 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
     int cur = 0;
     for (uint32_t bb = 0; bb < max_depth; ++bb) {
         const int nxt = cur ^ 1;
-        hipLaunchKernelGGL(k_trace<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->q[cur], bvh, ext + bb,
-                           twork + bb, ctx->d_stat, (const uint32_t *)b.d_roots, path_bits);
+        hipLaunchKernelGGL(trace_kernel(false, !b.spheres.empty()), dim3(tg), dim3(kTraceBlock), 0, ctx->stream,
+                           ctx->q[cur], bvh, ext + bb, twork + bb, ctx->d_stat, (const uint32_t *)b.d_roots,
+                           path_bits);
         hipLaunchKernelGGL(k_shade, dim3(sg), dim3(sh.shade_block), 0, ctx->stream, sv, sp, ctx->q[cur], ext + bb,
                            ctx->q[nxt], ext + bb + 1, ctx->sq, shd + bb, ctx->d_out_l, aov,
                            (const BatchViewDev *)b.d_views, path_bits);
-        hipLaunchKernelGGL(k_shadow<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->sq, bvh, shd + bb,
-                           swork + bb, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2, (const uint32_t *)b.d_roots,
-                           path_bits);
+        hipLaunchKernelGGL(shadow_kernel(false, !b.spheres.empty()), dim3(tg), dim3(kTraceBlock), 0, ctx->stream,
+                           ctx->sq, bvh, shd + bb, swork + bb, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2,
+                           (const uint32_t *)b.d_roots, path_bits);
         cur = nxt;
     }
     hipLaunchKernelGGL(k_accumulate_batch, dim3((uint32_t)ctx->num_cu * 16u), dim3(256), 0, ctx->stream,
@@ -1520,8 +1631,8 @@ int pt_create(int hip_device, pt_ctx **out) {
     {
         // Persistent traversal grid: exactly the blocks that can be resident.
         int a = 0, b = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace<false>, kTraceBlock, 0) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false>, kTraceBlock, 0) == hipSuccess)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace<false, true>, kTraceBlock, 0) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false, true>, kTraceBlock, 0) == hipSuccess)
             ctx->trace_blocks_per_cu = (uint32_t)std::max(1, std::min(a, b));
         int c = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_guide_infer, kInferBlock, 0) == hipSuccess && c > 0)
@@ -1572,6 +1683,8 @@ int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
     ctx->h_uv2.swap(d.h_uv2);
     ctx->h_tri4.swap(d.h_tri4);
     ctx->h_shapes.swap(d.h_shapes);
+    ctx->h_spheres.swap(d.h_spheres);
+    ctx->h_sph4.swap(d.h_sph4);
     ctx->h_materials.swap(d.h_materials);
     ctx->h_emitters.swap(d.h_emitters);
     ctx->h_cdf.swap(d.h_cdf);
@@ -1581,6 +1694,7 @@ int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
     ctx->height = d.height;
     ctx->scene_max_depth = d.max_depth;
     ctx->num_tris = d.num_tris;
+    ctx->num_spheres = d.num_spheres;
     for (int k = 0; k < 3; ++k) {
         ctx->scene_lo[k] = d.lo[k];
         ctx->scene_hi[k] = d.hi[k];
@@ -1596,6 +1710,7 @@ int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
         (rc = upload(ctx, ctx->d_uv, reinterpret_cast<const float2 *>(ctx->h_uv2.data()), (size_t)nv)) != PT_OK)
         return rc;
     if ((rc = upload(ctx, ctx->d_shapes, ctx->h_shapes.data(), ctx->h_shapes.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, ctx->d_spheres, ctx->h_spheres.data(), ctx->h_spheres.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, ctx->d_materials, ctx->h_materials.data(), ctx->h_materials.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, ctx->d_emitters, ctx->h_emitters.data(), ctx->h_emitters.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, ctx->d_cdf, ctx->h_cdf.data(), ctx->h_cdf.size())) != PT_OK) return rc;
@@ -1636,8 +1751,8 @@ int pt_build_bvh(pt_ctx *ctx, const pt_bvh_opts *opts) {
     std::vector<WideNode> nodes;
     std::vector<LeafTri> tris;
     std::string err;
-    if (!build_wide_bvh(ctx->h_pos4.data(), (uint32_t)(ctx->h_pos4.size() / 4), ctx->h_tri4.data(), ctx->num_tris, bo,
-                        nodes, tris, ctx->bvh_stats, err)) {
+    if (!build_wide_bvh(ctx->h_pos4.data(), (uint32_t)(ctx->h_pos4.size() / 4), ctx->h_tri4.data(), ctx->num_tris,
+                        ctx->h_sph4.data(), ctx->num_spheres, bo, nodes, tris, ctx->bvh_stats, err)) {
         ctx->error = "pt_build_bvh: " + err;
         return PT_ERR_INVALID_ARG;
     }
@@ -1813,12 +1928,9 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
             const int nxt = cur ^ 1;
             {
                 EventTimer t(ctx, timing);
-                if (ctx->counting)
-                    hipLaunchKernelGGL(k_trace<true>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->q[cur], bvh,
-                                       ext + b, twork + b, ctx->d_stat, (const uint32_t *)nullptr, 31u);
-                else
-                    hipLaunchKernelGGL(k_trace<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->q[cur], bvh,
-                                       ext + b, twork + b, ctx->d_stat, (const uint32_t *)nullptr, 31u);
+                hipLaunchKernelGGL(trace_kernel(ctx->counting, ctx->num_spheres > 0), dim3(tg), dim3(kTraceBlock), 0,
+                                   ctx->stream, ctx->q[cur], bvh, ext + b, twork + b, ctx->d_stat,
+                                   (const uint32_t *)nullptr, 31u);
                 t.stop(kKindTrace);
             }
             // The guide is sampled at vertices b < guided_depth; the vertex of the
@@ -1846,14 +1958,9 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
             }
             {
                 EventTimer t(ctx, timing);
-                if (ctx->counting)
-                    hipLaunchKernelGGL(k_shadow<true>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->sq, bvh,
-                                       shd + b, swork + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2,
-                                       (const uint32_t *)nullptr, 31u);
-                else
-                    hipLaunchKernelGGL(k_shadow<false>, dim3(tg), dim3(kTraceBlock), 0, ctx->stream, ctx->sq, bvh,
-                                       shd + b, swork + b, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2,
-                                       (const uint32_t *)nullptr, 31u);
+                hipLaunchKernelGGL(shadow_kernel(ctx->counting, ctx->num_spheres > 0), dim3(tg), dim3(kTraceBlock), 0,
+                                   ctx->stream, ctx->sq, bvh, shd + b, swork + b, ctx->q[nxt], ctx->d_out_l,
+                                   ctx->d_stat + 2, (const uint32_t *)nullptr, 31u);
                 t.stop(kKindShadow);
             }
             cur = nxt;
@@ -2029,12 +2136,8 @@ int pt_trace_rays(pt_ctx *ctx, const float *rays, uint32_t count, int any_hit, v
         }
         const DeviceBvh bvh = device_bvh(ctx);
         const uint32_t grid = std::min<uint32_t>((count + kTraceBlock - 1) / kTraceBlock, trace_grid(ctx));
-        if (any_hit)
-            hipLaunchKernelGGL(k_trace_rays<true>, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, d_rays, count, bvh,
-                               d_hits);
-        else
-            hipLaunchKernelGGL(k_trace_rays<false>, dim3(grid), dim3(kTraceBlock), 0, ctx->stream, d_rays, count, bvh,
-                               d_hits);
+        hipLaunchKernelGGL(trace_rays_kernel(any_hit != 0, ctx->num_spheres > 0), dim3(grid), dim3(kTraceBlock), 0,
+                           ctx->stream, d_rays, count, bvh, d_hits);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
             rc = PT_ERR_HIP;
             break;
@@ -2203,15 +2306,15 @@ int pt_batch_add_view(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uin
     BvhBuildStats bst;
     std::string err;
     if (!build_wide_bvh(d.h_pos4.data(), (uint32_t)(d.h_pos4.size() / 4), d.h_tri4.data(), d.num_tris,
-                        BvhBuildOptions(), nodes, leaf, bst, err)) {
+                        d.h_sph4.data(), d.num_spheres, BvhBuildOptions(), nodes, leaf, bst, err)) {
         ctx->error = "pt_batch_add_view: " + err;
         return PT_ERR_INVALID_ARG;
     }
-    const uint64_t nv0 = b.pos4.size() / 4, nt0 = b.tri4.size() / 4;
+    const uint64_t nv0 = b.pos4.size() / 4, nt0 = b.tri4.size() / 4, nsph0 = b.spheres.size();
     const uint64_t node0 = b.nodes.size(), leaf0 = b.leaf.size();
     const uint64_t nv = d.h_pos4.size() / 4;
-    if (nv0 + nv > 0xFFFFFFFFull || nt0 + d.num_tris > 0x7FFFFFFFull || node0 + nodes.size() > 0xFFFFFFFFull ||
-        leaf0 + leaf.size() > 0xFFFFFFFFull) {
+    if (nv0 + nv > 0xFFFFFFFFull || nt0 + d.num_tris >= kPrimSphere || nsph0 + d.num_spheres >= kPrimSphere ||
+        node0 + nodes.size() > 0xFFFFFFFFull || leaf0 + leaf.size() > 0xFFFFFFFFull) {
         ctx->error = "pt_batch_add_view: batch exceeds 32-bit index ranges";
         return PT_ERR_CAPACITY;
     }
@@ -2222,7 +2325,8 @@ int pt_batch_add_view(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uin
         n.child_base += (uint32_t)node0;
         n.tri_base += (uint32_t)leaf0;
     }
-    for (auto &t : leaf) t.v0.w = u2f(f2u(t.v0.w) + (uint32_t)nt0);
+    for (auto &t : leaf)
+        t.v0.w = u2f(f2u(t.v0.w) + (uint32_t)(leaf_is_sphere(t) ? nsph0 : nt0));
     b.nodes.insert(b.nodes.end(), nodes.begin(), nodes.end());
     b.leaf.insert(b.leaf.end(), leaf.begin(), leaf.end());
     b.pos4.insert(b.pos4.end(), d.h_pos4.begin(), d.h_pos4.end());
@@ -2244,8 +2348,12 @@ int pt_batch_add_view(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uin
     }
     for (ShapeRec r : d.h_shapes) {
         r.material += mat0;
-        r.first_tri += (uint32_t)nt0;  // emitter offsets stay view-local
+        r.first_tri += (uint32_t)((r.flags & PT_SHAPE_SPHERE) ? nsph0 : nt0);  // emitter offsets stay view-local
         b.shapes.push_back(r);
+    }
+    for (SphereRec sp : d.h_spheres) {
+        sp.shape += shape0;
+        b.spheres.push_back(sp);
     }
     b.materials.insert(b.materials.end(), d.h_materials.begin(), d.h_materials.end());
     b.emitters.insert(b.emitters.end(), d.h_emitters.begin(), d.h_emitters.end());

@@ -392,10 +392,10 @@ struct Loader {
     std::string dir;
     std::string err;
     std::map<std::string, std::shared_ptr<MeshData>> meshes;
-    // rectangle and cube shapes share one flip_normals flag each: the last
-    // parsed instance wins (resource/shape.cpp:85-111, 457-497).
-    std::vector<std::pair<uint32_t, int>> shared_flip_users;  // (shape index, 0 rect / 1 cube)
-    bool rect_flip = false, cube_flip = false;
+    // rectangle, cube and sphere shapes share one flip_normals flag per kind:
+    // the last parsed instance wins (resource/shape.cpp:85-132, 441-497).
+    std::vector<std::pair<uint32_t, int>> shared_flip_users;  // (shape index, 0 rect / 1 cube / 2 sphere)
+    bool shared_flip[3] = {false, false, false};
 
     Mat4 load_transform(const Obj *t, bool &is_lookat) {
         is_lookat = false;
@@ -549,8 +549,7 @@ struct Loader {
             shared_kind = o->type == "rectangle" ? 0 : 1;
             static const MeshData rect = make_rectangle_mesh(), cube = make_cube_mesh();
             mesh = std::make_shared<MeshData>(shared_kind == 0 ? rect : cube);
-            bool flip = load_bool(o, "flip_normals", false);
-            (shared_kind == 0 ? rect_flip : cube_flip) = flip;
+            shared_flip[shared_kind] = load_bool(o, "flip_normals", false);
         } else if (o->type == "obj") {
             std::string fn = o->prop("filename");
             std::string path = fn.size() && fn[0] == '/' ? fn : dir + "/" + fn;
@@ -566,8 +565,8 @@ struct Loader {
             if (load_bool(o, "flip_tex_coords", true)) flags |= PT_SHAPE_FLIP_TEXCOORDS;
             if (load_bool(o, "flip_normals", false)) flags |= PT_SHAPE_FLIP_NORMALS;
         } else if (o->type == "sphere") {
-            err = "unsupported: analytic sphere shapes";
-            return false;
+            shared_kind = 2;
+            shared_flip[2] = load_bool(o, "flip_normals", false);
         } else {
             return true;  // unknown shape type: skipped with a warning in the reference
         }
@@ -584,7 +583,20 @@ struct Loader {
                 if (!load_texture(e, "radiance", rad, 0.f, 0.f, 0.f)) return false;
             }
         }
-        uint32_t si = scene.add_instance(*mesh, to_world, mi, flags, is_emitter, &rad);
+        uint32_t si;
+        if (shared_kind == 2) {
+            // ShapeLoader<_sphere>: to_world * T(center) * S(radius) applied to
+            // the unit sphere.
+            float center[3], radius;
+            load_float3(o->prop("center"), center, 0.f, 0.f, 0.f, false);
+            load_float(o, "radius", radius, 1.f);
+            Transform local;
+            local.scale(radius, radius, radius);
+            local.translate(center[0], center[1], center[2]);
+            si = scene.add_sphere(to_world * local.m, mi, 0u, is_emitter, &rad);
+        } else {
+            si = scene.add_instance(*mesh, to_world, mi, flags, is_emitter, &rad);
+        }
         if (shared_kind >= 0) shared_flip_users.emplace_back(si, shared_kind);
         return true;
     }
@@ -647,8 +659,7 @@ struct Loader {
             }
         }
         for (auto &u : shared_flip_users) {
-            bool flip = u.second == 0 ? rect_flip : cube_flip;
-            if (flip) scene.shapes[u.first].flags |= PT_SHAPE_FLIP_NORMALS;
+            if (shared_flip[u.second]) scene.shapes[u.first].flags |= PT_SHAPE_FLIP_NORMALS;
         }
         if (w <= 0 || h <= 0) {
             err = "parse: film size must be positive";

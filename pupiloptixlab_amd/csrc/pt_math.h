@@ -172,6 +172,61 @@ PT_HD f3 uniform_sample_hemisphere(float u1, float u2) {
     return f3{sin_theta * c, sin_theta * s, fabsf(z)};
 }
 PT_HD float uniform_hemisphere_pdf(f3 v) { return v.z > 0.f ? kInvPi * 0.5f : 0.f; }
+// UniformSampleSphere (optix/util.h:38-43).
+PT_HD f3 uniform_sample_sphere(float u1, float u2) {
+    float z = 1.f - 2.f * u1;
+    float sin_theta = sqrtf(fmaxf(0.f, 1.f - z * z));
+    float phi = 2.0f * kPi * u2;
+    float s, c;
+    sincos_pt(phi, &s, &c);
+    return f3{sin_theta * c, sin_theta * s, z};
+}
+
+// atan on [0, 1]: for a > tan(pi/8) the identity atan(a) = pi/4 +
+// atan((a - 1) / (a + 1)) moves the argument into |u| <= tan(pi/8), where the
+// Taylor series u - u^3/3 + u^5/5 - ... up to u^19 leaves a truncation error
+// below 5e-10.  Like sincos_pt, one operation order on host and GPU.
+PT_HD float atan_unit_pt(float a) {
+    float base = 0.f, u = a;
+    if (a > 0.41421356f) {
+        u = (a - 1.f) / (a + 1.f);
+        base = 0.785398163397448f;
+    }
+    const float u2 = u * u;
+    float p = -1.f / 19.f;
+    p = 1.f / 17.f + u2 * p;
+    p = -1.f / 15.f + u2 * p;
+    p = 1.f / 13.f + u2 * p;
+    p = -1.f / 11.f + u2 * p;
+    p = 1.f / 9.f + u2 * p;
+    p = -1.f / 7.f + u2 * p;
+    p = 1.f / 5.f + u2 * p;
+    p = -1.f / 3.f + u2 * p;
+    return base + (u + u * (u2 * p));
+}
+// Four-quadrant arctangent in (-pi, pi]; atan2(0, 0) = 0.
+PT_HD float atan2_pt(float y, float x) {
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    if (mx == 0.f) return 0.f;
+    float r = atan_unit_pt(mn / mx);
+    if (ay > ax) r = 1.57079632679490f - r;
+    if (x < 0.f) r = 3.14159265358979f - r;
+    return y < 0.f ? -r : r;
+}
+// acos(z) = atan2(sqrt((1 - z)(1 + z)), z) for z clamped to [-1, 1].
+PT_HD float acos_pt(float z) {
+    z = fminf(fmaxf(z, -1.f), 1.f);
+    return atan2_pt(sqrtf((1.f - z) * (1.f + z)), z);
+}
+// GetSphereTexcoord (optix/util.h:117-128): (phi / 2pi, theta / pi) of a unit
+// direction in the sphere's object frame, phi in [0, 2pi).
+PT_HD f2 sphere_texcoord(f3 p) {
+    float phi = atan2_pt(p.y, p.x);
+    if (phi < 0.f) phi = phi + kPi * 2.f;
+    const float theta = acos_pt(p.z);
+    return f2{phi * kInvPi * 0.5f, theta * kInvPi};
+}
 PT_HD f3 reflect_z(f3 v) { return f3{-v.x, -v.y, v.z}; }
 PT_HD f3 reflect_n(f3 v, f3 n) { return -v + (2.f * dot(v, n)) * n; }
 PT_HD f3 refract_z(f3 v, float cos_theta_t, float eta) {

@@ -4,7 +4,8 @@
 //
 // References: cuda/texture.h:33-57, render/material/optix_material.h:119-131,
 // render/material/bsdf/*.h GetLocal, render/emitter.h:110-136,
-// render/emitter/area.h:17-45, render/emitter/env.h:67-86,
+// render/emitter/area.h:17-45, render/emitter/sphere.h:14-43,
+// render/emitter/env.h:67-86,
 // render/camera.h:7-10, example/path_tracer/main.cu:58-78.
 #pragma once
 
@@ -52,9 +53,11 @@ struct ShapeRec {
     uint32_t first_tri;
 };
 
-enum EmitterType : uint32_t { kEmitterNone = 0, kEmitterTriArea = 1, kEmitterConstEnv = 3 };
+// EEmitterType (render/emitter/types.h:7-15).
+enum EmitterType : uint32_t { kEmitterNone = 0, kEmitterTriArea = 1, kEmitterSphere = 2, kEmitterConstEnv = 3 };
 
-// One emitting triangle (TriAreaEmitter, area.h:4-15) in world space.
+// One emitting triangle (TriAreaEmitter, area.h:4-15) or sphere
+// (SphereEmitter, sphere.h:4-12: p0 = center xyz + radius) in world space.
 struct EmitterRec {
     uint32_t type;
     float select_prob;
@@ -65,6 +68,15 @@ struct EmitterRec {
     f4 uv01;         // uv0.xy, uv1.xy
     f4 uv2;          // uv2.xy
     TextureRec radiance;
+};
+
+// Shading record of an analytic sphere (Geometry::Sphere, render/geometry.h:
+// 23-27, as instanced by world/render_object.cpp:30-35): the world -> object
+// transform of its unit sphere and the owning shape.
+struct SphereRec {
+    f4 w2o[3];       // rows of the 3x4 world -> object transform
+    uint32_t shape;
+    uint32_t pad0, pad1, pad2;
 };
 
 struct EnvRec {
@@ -176,7 +188,52 @@ PT_HD void area_sample_direct(const EmitterRec &e, const LocalGeometry &hit, f2 
     }
 }
 
-// TriAreaEmitter::Eval (area.h:37-45); pdf stays 0 when the hit faces away.
+// SphereEmitter::SampleDirect (sphere.h:14-33): a uniform point on the whole
+// sphere, texcoord from the world-frame direction.
+PT_HD void sphere_sample_direct(const EmitterRec &e, const LocalGeometry &hit, f2 xi, EmitterSample &ret) {
+    const f3 t = uniform_sample_sphere(xi.x, xi.y);
+    const f3 position = t * e.p0.w + mk3(e.p0.x, e.p0.y, e.p0.z);
+    const f3 normal = normalize(t);
+    ret.radiance = texture_sample(e.radiance, sphere_texcoord(t));
+    ret.wi = normalize(position - hit.position);
+    const float nol = dot(hit.normal, ret.wi);
+    const float lnol = dot(normal, -ret.wi);
+    ret.pdf = 0.f;
+    ret.distance = 0.f;
+    if (nol > 0.f && lnol > 0.f) {
+        const float distance = length(position - hit.position);
+        ret.pdf = distance * distance / (lnol * e.area);
+        ret.distance = distance;
+    }
+}
+
+// Emitter::SampleDirect (render/emitter.h:74-90) for the area kinds.
+PT_HD void emitter_sample_direct(const EmitterRec &e, const LocalGeometry &hit, f2 xi, EmitterSample &ret) {
+    if (e.type == kEmitterSphere) sphere_sample_direct(e, hit, xi, ret);
+    else area_sample_direct(e, hit, xi, ret);
+}
+
+// Solid-angle density with which emitter_sample_direct produces the emitter
+// point `p` (barycentrics b1, b2 on a triangle) seen from `o`: the emitter's
+// own outward normal, as in SampleDirect, not the shading normal of the hit.
+PT_HD float emitter_nee_pdf(const EmitterRec &e, f3 p, float b1, float b2, f3 o) {
+    f3 en;
+    if (e.type == kEmitterSphere) {
+        en = normalize(p - mk3(e.p0.x, e.p0.y, e.p0.z));
+    } else {
+        const float wv = 1.f - b1 - b2;
+        en = normalize(mk3(e.n0.x, e.n0.y, e.n0.z) * wv + mk3(e.n1.x, e.n1.y, e.n1.z) * b1 +
+                       mk3(e.n2.x, e.n2.y, e.n2.z) * b2);
+    }
+    const f3 dir = normalize(o - p);
+    const float nl = dot(en, dir);
+    const float dist = length(o - p);
+    return nl > 0.f ? dist * dist / (nl * e.area) : 0.f;
+}
+
+// TriAreaEmitter::Eval (area.h:37-45) and SphereEmitter::Eval (sphere.h:
+// 35-43), which are the same function of the hit geometry; pdf stays 0 when
+// the hit faces away.
 PT_HD void area_eval(const EmitterRec &e, const LocalGeometry &emit_geo, f3 scatter_pos, f3 &radiance, float &pdf) {
     f3 dir = normalize(scatter_pos - emit_geo.position);
     float lnol = dot(emit_geo.normal, dir);

@@ -324,6 +324,24 @@ uint32_t HostScene::add_instance(const MeshData &mesh, const Mat4 &to_world, uin
     return (uint32_t)shapes.size() - 1;
 }
 
+uint32_t HostScene::add_sphere(const Mat4 &to_world, uint32_t material, uint32_t flags, bool is_emitter,
+                               const pt_texture_desc *radiance) {
+    pt_sphere_desc sp;
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 4; ++c) sp.to_world[4 * r + c] = to_world.e[4 * r + c];
+    pt_shape_desc s;
+    std::memset(&s, 0, sizeof(s));
+    s.material = material;
+    s.flags = (flags & PT_SHAPE_FLIP_NORMALS) | PT_SHAPE_SPHERE;
+    s.first_tri = (uint32_t)spheres.size();
+    s.num_tris = 1;
+    s.is_emitter = is_emitter ? 1 : 0;
+    s.radiance = radiance ? *radiance : make_rgb_texture(0.f, 0.f, 0.f);
+    spheres.push_back(sp);
+    shapes.push_back(s);
+    return (uint32_t)shapes.size() - 1;
+}
+
 void HostScene::set_camera(float fov_y_deg, const Mat4 &c2w) {
     Mat4 s2c = sample_to_camera_matrix(fov_y_deg, (float)width / (float)height, 0.01f, 10000.f);
     std::memcpy(camera.sample_to_camera, s2c.e, sizeof(s2c.e));
@@ -349,6 +367,8 @@ pt_scene_desc HostScene::desc() const {
     d.width = width;
     d.height = height;
     d.max_depth = max_depth;
+    d.num_spheres = (uint32_t)spheres.size();
+    d.spheres = spheres.empty() ? nullptr : spheres.data();
     return d;
 }
 

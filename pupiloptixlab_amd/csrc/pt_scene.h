@@ -56,6 +56,7 @@ public:
     std::vector<uint32_t> indices;
     std::vector<pt_shape_desc> shapes;
     std::vector<pt_material_desc> materials;
+    std::vector<pt_sphere_desc> spheres;
     uint32_t env_type = PT_ENV_NONE;
     float env_radiance[3] = {0.f, 0.f, 0.f};
     pt_camera_desc camera{};
@@ -67,6 +68,10 @@ public:
     // transpose (not normalized).  Returns the shape index.
     uint32_t add_instance(const MeshData &mesh, const Mat4 &to_world, uint32_t material, uint32_t flags,
                           bool is_emitter, const pt_texture_desc *radiance);
+    // Appends an analytic sphere shape: the unit sphere mapped by `to_world`
+    // (only PT_SHAPE_FLIP_NORMALS of `flags` applies).  Returns the shape index.
+    uint32_t add_sphere(const Mat4 &to_world, uint32_t material, uint32_t flags, bool is_emitter,
+                        const pt_texture_desc *radiance);
     void set_camera(float fov_y_deg, const Mat4 &camera_to_world);
     pt_scene_desc desc() const;
 };

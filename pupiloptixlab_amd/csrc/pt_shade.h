@@ -12,6 +12,7 @@
 // in the reference order.
 #pragma once
 
+#include "pt_bvh.h"
 #include "pt_records.h"
 
 namespace pupil {
@@ -33,6 +34,7 @@ struct SceneView {
     const f4 *vnrm;             // may be null
     const f2 *vuv;              // may be null
     const ShapeRec *shapes;
+    const SphereRec *spheres;   // may be null without spheres
     const MaterialRec *materials;
     const EmitterRec *emitters;
     const float *emitter_cdf;
@@ -69,9 +71,37 @@ struct ShadeOut {
     float test;
 };
 
-// Reconstructs the hit (render/geometry.h:60-93, 176-180) in world space.
-PT_HD void hit_geometry(const SceneView &s, uint32_t prim, float b1, float b2, f3 ray_dir, LocalGeometry &g,
-                        uint32_t &material, int32_t &emitter_index) {
+// Sphere hit (render/geometry.h:94-100): position o + t d, texcoord and
+// normal from the object-space point (the normal through the transposed
+// world -> object matrix, i.e. the inverse transpose of object -> world).
+PT_HD void sphere_hit_geometry(const SceneView &s, uint32_t k, float t, f3 ray_o, f3 ray_dir, LocalGeometry &g,
+                               uint32_t &material, int32_t &emitter_index, uint32_t &flags) {
+    const SphereRec &sp = s.spheres[k];
+    const ShapeRec sh = s.shapes[sp.shape];
+    g.position = ray_o + t * ray_dir;
+    const f4 p = f4{g.position.x, g.position.y, g.position.z, 1.f};
+    const f3 local = mk3(dot(sp.w2o[0], p), dot(sp.w2o[1], p), dot(sp.w2o[2], p));
+    g.texcoord = sphere_texcoord(normalize(local));
+    const f3 n = mk3(sp.w2o[0].x * local.x + sp.w2o[1].x * local.y + sp.w2o[2].x * local.z,
+                     sp.w2o[0].y * local.x + sp.w2o[1].y * local.y + sp.w2o[2].y * local.z,
+                     sp.w2o[0].z * local.x + sp.w2o[1].z * local.y + sp.w2o[2].z * local.z);
+    g.normal = normalize(n);
+    material = sh.material;
+    flags = sh.flags;
+    emitter_index = sh.emitter_offset >= 0 ? sh.emitter_offset + (int32_t)(k - sh.first_tri) : -1;
+}
+
+// Reconstructs the hit (render/geometry.h:60-100, 176-180) in world space.
+// Triangles use the barycentrics (b1, b2), spheres the ray and its t.
+PT_HD void hit_geometry(const SceneView &s, uint32_t prim, float t_hit, float b1, float b2, f3 ray_o, f3 ray_dir,
+                        LocalGeometry &g, uint32_t &material, int32_t &emitter_index) {
+    if (prim & kPrimSphere) {
+        uint32_t flags;
+        sphere_hit_geometry(s, prim & kPrimIndexMask, t_hit, ray_o, ray_dir, g, material, emitter_index, flags);
+        if (flags & kShapeFlipNormals) g.normal = g.normal * -1.f;
+        if (dot(-ray_dir, g.normal) < 0.f && (s.materials[material].flags & kMatTwosided)) g.normal = -g.normal;
+        return;
+    }
     const uint32_t *t = s.tri + 4 * (size_t)prim;
     const uint32_t i0 = t[0], i1 = t[1], i2 = t[2];
     const ShapeRec sh = s.shapes[t[3]];
@@ -119,8 +149,8 @@ PT_HD int32_t select_emitter(const SceneView &s, float p) {
     return -2;
 }
 
-PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps, uint32_t prim, float b1, float b2,
-                      ShadeOut &out) {
+PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps, uint32_t prim, float t_hit,
+                      float b1, float b2, ShadeOut &out) {
     out.cont = false;
     out.done = false;
     out.shadow = false;
@@ -162,7 +192,7 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
     LocalGeometry g;
     uint32_t mat_index;
     int32_t emitter_index;
-    hit_geometry(s, prim, b1, b2, ps.d, g, mat_index, emitter_index);
+    hit_geometry(s, prim, t_hit, b1, b2, ps.o, ps.d, g, mat_index, emitter_index);
     const MaterialRec &mat = s.materials[mat_index];
     const LocalBsdf bsdf = material_local_bsdf(mat, g.texcoord);
 
@@ -180,18 +210,9 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
         float epdf;
         area_eval(e, g, ps.o, rad, epdf);
         if (!is_zero(epdf)) {
-            float lpdf = epdf;
-            if (consistent) {
-                // Weight against the pdf next-event estimation would assign to
-                // this point, which uses the emitter's own vertex normals.
-                const float wv = 1.f - b1 - b2;
-                const f3 en = normalize(mk3(e.n0.x, e.n0.y, e.n0.z) * wv + mk3(e.n1.x, e.n1.y, e.n1.z) * b1 +
-                                        mk3(e.n2.x, e.n2.y, e.n2.z) * b2);
-                const f3 dir = normalize(ps.o - g.position);
-                const float nl = dot(en, dir);
-                const float dist = length(ps.o - g.position);
-                lpdf = nl > 0.f ? dist * dist / (nl * e.area) : 0.f;
-            }
+            // Consistent mode weighs against the pdf next-event estimation
+            // would assign to this point, which uses the emitter's own normals.
+            const float lpdf = consistent ? emitter_nee_pdf(e, g.position, b1, b2, ps.o) : epdf;
             float mis = (ps.flags & kPathPrevDelta) ? 1.f : mis_weight(ps.prev_pdf, lpdf * e.select_prob);
             ps.L = ps.L + ps.T * rad * mis;
         }
@@ -220,7 +241,7 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
         es.pdf = 0.f;
         float sel_prob = 0.f;
         if (sel >= 0) {
-            area_sample_direct(s.emitters[sel], g, mk2(x, y), es);
+            emitter_sample_direct(s.emitters[sel], g, mk2(x, y), es);
             sel_prob = s.emitters[sel].select_prob;
         } else if (sel == -1) {
             const_env_sample_direct(s.env, g, mk2(x, y), es);

@@ -62,8 +62,8 @@ PT_HD void complete_record(const RecordSink &r, uint32_t path, const PathState &
 }
 
 // Returns true when the path scatters at this vertex (guided_scatter follows).
-PT_HD bool guided_begin(const SceneView &s, const ShadeParams &prm, PathState &ps, uint32_t prim, float b1, float b2,
-                        const RecordSink &rs, uint32_t path, GuidedVertex &v, ShadeOut &out) {
+PT_HD bool guided_begin(const SceneView &s, const ShadeParams &prm, PathState &ps, uint32_t prim, float t_hit,
+                        float b1, float b2, const RecordSink &rs, uint32_t path, GuidedVertex &v, ShadeOut &out) {
     out.cont = false;
     out.done = false;
     out.shadow = false;
@@ -95,7 +95,7 @@ PT_HD bool guided_begin(const SceneView &s, const ShadeParams &prm, PathState &p
 
     uint32_t mat_index;
     int32_t emitter_index;
-    hit_geometry(s, prim, b1, b2, ps.d, v.g, mat_index, emitter_index);
+    hit_geometry(s, prim, t_hit, b1, b2, ps.o, ps.d, v.g, mat_index, emitter_index);
     const MaterialRec &mat = s.materials[mat_index];
     v.bsdf = material_local_bsdf(mat, v.g.texcoord);
     v.guidable = guidable_material(v.bsdf.type);
@@ -115,13 +115,7 @@ PT_HD bool guided_begin(const SceneView &s, const ShadeParams &prm, PathState &p
         float epdf;
         area_eval(e, v.g, ps.o, rad, epdf);
         if (!is_zero(epdf)) {
-            const float wv = 1.f - b1 - b2;
-            const f3 en = normalize(mk3(e.n0.x, e.n0.y, e.n0.z) * wv + mk3(e.n1.x, e.n1.y, e.n1.z) * b1 +
-                                    mk3(e.n2.x, e.n2.y, e.n2.z) * b2);
-            const f3 dir = normalize(ps.o - v.g.position);
-            const float nl = dot(en, dir);
-            const float dist = length(ps.o - v.g.position);
-            const float lpdf = nl > 0.f ? dist * dist / (nl * e.area) : 0.f;
+            const float lpdf = emitter_nee_pdf(e, v.g.position, b1, b2, ps.o);
             const float mis = (ps.flags & kPathPrevDelta) ? 1.f : mis_weight(ps.prev_pdf, lpdf * e.select_prob);
             ps.L = ps.L + ps.T * rad * mis;
             le = rad;
@@ -162,7 +156,7 @@ PT_HD void guided_scatter(const SceneView &s, PathState &ps, const GuidedVertex 
         es.pdf = 0.f;
         float sel_prob = 0.f;
         if (sel >= 0) {
-            area_sample_direct(s.emitters[sel], g, mk2(x, y), es);
+            emitter_sample_direct(s.emitters[sel], g, mk2(x, y), es);
             sel_prob = s.emitters[sel].select_prob;
         } else if (sel == -1) {
             const_env_sample_direct(s.env, g, mk2(x, y), es);

@@ -35,7 +35,10 @@ def test_struct_layouts_match_header():
     assert C.sizeof(pt.TextureDesc) == 64
     assert C.sizeof(pt.MaterialDesc) == 20 + 4 * 64
     assert pt.SceneDesc.width.offset == 208
-    assert C.sizeof(pt.SceneDesc) == 224
+    assert pt.SceneDesc.num_spheres.offset == 220
+    assert pt.SceneDesc.spheres.offset == 224
+    assert C.sizeof(pt.SceneDesc) == 232
+    assert C.sizeof(pt.SphereDesc) == 48
 
 
 def test_create_without_device_fails_cleanly():
@@ -262,7 +265,7 @@ def test_xml_obj_shape(tmp_path):
 @pytest.mark.parametrize("text,code", [
     ("<scene><shape type='cube'>", pt.STATUS and -6),
     ("<notscene/>", -6),
-    ("<scene><shape type='sphere'><float name='radius' value='1'/></shape></scene>", -6),
+    ("<scene><emitter type='envmap'><string name='filename' value='sky.exr'/></emitter></scene>", -6),
     ("<scene><shape type='obj'><string name='filename' value='missing.obj'/></shape></scene>", -5),
 ])
 def test_xml_errors(tmp_path, text, code):
