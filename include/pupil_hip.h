@@ -46,7 +46,7 @@ enum pt_material_type {
     PT_MAT_ROUGH_PLASTIC = 7
 };
 
-/* Texture kinds of util/texture.h (RGB, checkerboard; bitmap reserved). */
+/* Texture kinds of util/texture.h (RGB, checkerboard, bitmap). */
 enum pt_texture_type { PT_TEX_RGB = 0, PT_TEX_CHECKER = 1, PT_TEX_BITMAP = 2 };
 
 /* Shape flags (render/geometry.h:14-27 TriMesh and Sphere flags, flattened). */
@@ -59,7 +59,23 @@ enum pt_shape_flags {
     PT_SHAPE_SPHERE = 32 /* analytic spheres: first_tri/num_tris index spheres[] */
 };
 
-enum pt_env_type { PT_ENV_NONE = 0, PT_ENV_CONSTANT = 1 };
+enum pt_env_type { PT_ENV_NONE = 0, PT_ENV_CONSTANT = 1, PT_ENV_MAP = 2 };
+
+/* Bitmap sampling modes (util/texture.h:10-20, applied through the CUDA
+ * texture object of cuda/texture.cpp:60-100). */
+enum pt_bitmap_filter { PT_FILTER_POINT = 0, PT_FILTER_LINEAR = 1 };
+enum pt_bitmap_wrap { PT_WRAP_REPEAT = 0, PT_WRAP_CLAMP = 1, PT_WRAP_MIRROR = 2, PT_WRAP_BORDER = 3 };
+
+/* util::BitmapTexture (util/texture.h:38-50): RGBA float texels, row 0 is
+ * the first (top) row of the image file and is sampled at v = 0.  Textures
+ * reference a bitmap by index (pt_texture_desc.bitmap with PT_TEX_BITMAP). */
+typedef struct pt_bitmap_desc {
+    uint32_t width;
+    uint32_t height;
+    uint32_t filter;   /* pt_bitmap_filter */
+    uint32_t wrap;     /* pt_bitmap_wrap */
+    const float *rgba; /* width * height * 4 floats */
+} pt_bitmap_desc;
 
 /* cuda::Texture (cuda/texture.h:10-27): a color, a checkerboard of two
  * colors, or a bitmap; uv_row0/uv_row1 transform (u, v, 0, 1). */
@@ -142,6 +158,15 @@ typedef struct pt_scene_desc {
     uint32_t max_depth;
     uint32_t num_spheres;
     const pt_sphere_desc *spheres; /* num_spheres entries, or NULL */
+    uint32_t num_bitmaps;
+    const pt_bitmap_desc *bitmaps; /* num_bitmaps entries, or NULL */
+    /* PT_ENV_MAP (resource::EnvMap, resource/scene.cpp:209-222, and
+     * EnvMapEmitter, render/emitter/env.h:6-65): radiance = bitmaps[env_bitmap]
+     * (bilinear, repeat) x env_scale; env_to_world = the 3x3 part of the
+     * emitter's to_world (rows). */
+    uint32_t env_bitmap;
+    float env_scale;
+    float env_to_world[9];
 } pt_scene_desc;
 
 typedef struct pt_bvh_opts {
@@ -397,6 +422,12 @@ int pt_scene_generate(int kind, const pt_gen_params *params, pt_host_scene **out
 /* Pointers in the returned desc stay valid until pt_scene_destroy. */
 int pt_scene_get_desc(const pt_host_scene *scene, pt_scene_desc *out);
 void pt_scene_destroy(pt_host_scene *scene);
+/* util::BitmapTexture::Load (util/texture.cpp:87-174): reads an EXR
+ * (NONE/RLE/ZIPS/ZIP), Radiance HDR or PNG file as RGBA float, row 0 = top
+ * row, 8-bit channels linearized with (v/255)^2.2.  Call with rgba = NULL to
+ * get the size; capacity is in floats. */
+int pt_load_image(const char *path, uint32_t *width, uint32_t *height, float *rgba, size_t capacity,
+                  char *err, size_t err_len);
 /* Writes RGB float EXR (uncompressed scanlines), bottom row first in the
  * buffer as in util/texture.cpp:23-85. */
 int pt_write_exr(const char *path, const float *rgba, uint32_t width, uint32_t height);

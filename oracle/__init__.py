@@ -68,6 +68,8 @@ def lib():
         L.oracle_ggx_sample.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_void_p]
         L.oracle_intersect.restype = C.c_int
         L.oracle_intersect.argtypes = [C.c_void_p] * 5 + [C.c_void_p]
+        L.oracle_bitmap_sample.restype = None
+        L.oracle_bitmap_sample.argtypes = [C.c_void_p, C.c_float, C.c_float, C.c_void_p]
         L.oracle_atan2.restype = C.c_float
         L.oracle_atan2.argtypes = [C.c_float, C.c_float]
         L.oracle_acos.restype = C.c_float
@@ -179,6 +181,20 @@ def intersect(o, d, v0, v1, v2):
     hit = lib().oracle_intersect(_f32(o).ctypes.data, _f32(d).ctypes.data, _f32(v0).ctypes.data,
                                  _f32(v1).ctypes.data, _f32(v2).ctypes.data, out.ctypes.data)
     return bool(hit), out
+
+
+class OrBitmap(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("filter", C.c_uint32), ("wrap", C.c_uint32),
+                ("rgba", C.POINTER(C.c_float))]
+
+
+def bitmap_sample(rgba, u, v, filter=1, wrap=0):
+    """Lookup in an (h, w, 4) float image at texture coordinates (u, v)."""
+    img = np.ascontiguousarray(rgba, np.float32)
+    b = OrBitmap(img.shape[1], img.shape[0], filter, wrap, img.ctypes.data_as(C.POINTER(C.c_float)))
+    out = np.zeros(3, np.float32)
+    lib().oracle_bitmap_sample(C.byref(b), float(u), float(v), out.ctypes.data)
+    return out
 
 
 def atan2(y, x):

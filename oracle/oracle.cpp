@@ -233,7 +233,49 @@ V3 ggx_sample(V3 wo, float a, V2 xi) {
 enum { DIFFUSE = 1, DIELECTRIC, ROUGH_DIELECTRIC, CONDUCTOR, ROUGH_CONDUCTOR, PLASTIC, ROUGH_PLASTIC };
 enum { LOBE_DIFFUSE_R = 1 << 1, LOBE_GLOSSY_R = 1 << 3, LOBE_GLOSSY_T = 1 << 4, LOBE_DELTA_R = 1 << 5, LOBE_DELTA_T = 1 << 6 };
 
-V3 tex_sample(const or_texture &t, V2 uv) {
+// Bitmap lookup with the CUDA texture-object semantics of cuda/texture.cpp:
+// 60-100 (normalized coordinates; point = texel floor(u w); bilinear = the
+// four texels around (u w - 1/2, v h - 1/2) with float weights; repeat /
+// clamp / mirror / border addressing, border color (1, 0, 0)).  The
+// coordinate is first reduced (fractional part, modulo 2, or clamped to
+// [-1, 2]) so texel indices fit an int.  Pinned by: texel-center, edge and
+// wrap closed forms (tests).
+int wrap_idx(int i, int n, uint32_t mode, bool &border) {
+    switch (mode) {
+        case 1: return std::min(std::max(i, 0), n - 1);
+        case 3: if (i < 0 || i >= n) { border = true; return 0; } return i;
+        case 2: { int m = ((i % (2 * n)) + 2 * n) % (2 * n); return m < n ? m : 2 * n - 1 - m; }
+        default: return ((i % n) + n) % n;
+    }
+}
+V4 texel(const or_bitmap &b, int x, int y) {
+    bool border = false;
+    int ix = wrap_idx(x, (int)b.width, b.wrap, border), iy = wrap_idx(y, (int)b.height, b.wrap, border);
+    if (border) return V4{1.f, 0.f, 0.f, 0.f};
+    const float *p = b.rgba + 4 * ((size_t)iy * b.width + (size_t)ix);
+    return V4{p[0], p[1], p[2], p[3]};
+}
+float reduce_coord(float u, uint32_t mode) {
+    if (!(u >= -3.0e38f && u <= 3.0e38f)) return 0.f;
+    if (mode == 0) return u - floorf(u);
+    if (mode == 2) return u - 2.f * floorf(u * 0.5f);
+    return fminf(fmaxf(u, -1.f), 2.f);
+}
+V3 bitmap_lookup(const or_bitmap &b, float u, float v) {
+    const float fu = reduce_coord(u, b.wrap) * (float)b.width, fv = reduce_coord(v, b.wrap) * (float)b.height;
+    if (b.filter == 0) {
+        V4 t = texel(b, (int)floorf(fu), (int)floorf(fv));
+        return v3(t.x, t.y, t.z);
+    }
+    const float x = fu - 0.5f, y = fv - 0.5f, x0 = floorf(x), y0 = floorf(y), a = x - x0, c = y - y0;
+    const int i = (int)x0, j = (int)y0;
+    V4 t00 = texel(b, i, j), t10 = texel(b, i + 1, j), t01 = texel(b, i, j + 1), t11 = texel(b, i + 1, j + 1);
+    float w00 = (1.f - a) * (1.f - c), w10 = a * (1.f - c), w01 = (1.f - a) * c, w11 = a * c;
+    return v3(w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x, w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
+              w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z);
+}
+
+V3 tex_sample(const or_texture &t, V2 uv, const or_bitmap *bm) {
     if (t.type == 0) return v3(t.color0[0], t.color0[1], t.color0[2]);
     V4 tc = V4{uv.x, uv.y, 0.f, 1.f};
     float tx = dot4(V4{t.uv_row0[0], t.uv_row0[1], t.uv_row0[2], t.uv_row0[3]}, tc);
@@ -247,10 +289,20 @@ V3 tex_sample(const or_texture &t, V2 uv) {
         if (tx > 0.5f) return ty > 0.5f ? p1 : p2;
         return ty > 0.5f ? p2 : p1;
     }
-    return v3s(0.f);
+    return bitmap_lookup(bm[t.bitmap], tx, ty);
 }
-V3 tex_average(const or_texture &t) {
+V3 tex_average(const or_texture &t, const or_bitmap *bm) {  // optix_material.cpp:9-36
     if (t.type == 1) return mul(v3(t.color0[0] + t.color1[0], t.color0[1] + t.color1[1], t.color0[2] + t.color1[2]), 0.5f);
+    if (t.type == 2) {
+        const or_bitmap &b = bm[t.bitmap];
+        float r = 0.f, g = 0.f, bl = 0.f;
+        for (size_t i = 0; i < (size_t)b.width * b.height; ++i) {
+            r += b.rgba[4 * i];
+            g += b.rgba[4 * i + 1];
+            bl += b.rgba[4 * i + 2];
+        }
+        return divs(v3(r, g, bl), 1.f * (float)b.height * (float)b.width);
+    }
     return v3(t.color0[0], t.color0[1], t.color0[2]);
 }
 
@@ -262,40 +314,40 @@ struct Bsdf {
 };
 
 // Material::GetLocalBsdf with host precompute of optix_material.cpp:88-119.
-Bsdf make_bsdf(const or_material &m, V2 uv) {
+Bsdf make_bsdf(const or_material &m, V2 uv, const or_bitmap *bm) {
     Bsdf b;
     std::memset(&b, 0, sizeof(b));
     b.type = (int)m.type;
     b.nonlinear = m.nonlinear != 0;
     b.eta = m.int_ior / m.ext_ior;
     switch (m.type) {
-        case DIFFUSE: b.diffuse_refl = tex_sample(m.tex[0], uv); break;
+        case DIFFUSE: b.diffuse_refl = tex_sample(m.tex[0], uv, bm); break;
         case DIELECTRIC:
-            b.spec_refl = tex_sample(m.tex[0], uv);
-            b.spec_trans = tex_sample(m.tex[1], uv);
+            b.spec_refl = tex_sample(m.tex[0], uv, bm);
+            b.spec_trans = tex_sample(m.tex[1], uv, bm);
             break;
         case ROUGH_DIELECTRIC:
-            b.spec_refl = tex_sample(m.tex[0], uv);
-            b.spec_trans = tex_sample(m.tex[1], uv);
-            b.alpha = tex_sample(m.tex[2], uv).x;
+            b.spec_refl = tex_sample(m.tex[0], uv, bm);
+            b.spec_trans = tex_sample(m.tex[1], uv, bm);
+            b.alpha = tex_sample(m.tex[2], uv, bm).x;
             break;
         case CONDUCTOR:
-            b.spec_refl = tex_sample(m.tex[0], uv);
-            b.cond_eta = tex_sample(m.tex[1], uv);
-            b.cond_k = tex_sample(m.tex[2], uv);
+            b.spec_refl = tex_sample(m.tex[0], uv, bm);
+            b.cond_eta = tex_sample(m.tex[1], uv, bm);
+            b.cond_k = tex_sample(m.tex[2], uv, bm);
             break;
         case ROUGH_CONDUCTOR:
-            b.spec_refl = tex_sample(m.tex[0], uv);
-            b.cond_eta = tex_sample(m.tex[1], uv);
-            b.cond_k = tex_sample(m.tex[2], uv);
-            b.alpha = tex_sample(m.tex[3], uv).x;
+            b.spec_refl = tex_sample(m.tex[0], uv, bm);
+            b.cond_eta = tex_sample(m.tex[1], uv, bm);
+            b.cond_k = tex_sample(m.tex[2], uv, bm);
+            b.alpha = tex_sample(m.tex[3], uv, bm).x;
             break;
         case PLASTIC:
         case ROUGH_PLASTIC: {
-            b.diffuse_refl = tex_sample(m.tex[0], uv);
-            b.spec_refl = tex_sample(m.tex[1], uv);
-            if (m.type == ROUGH_PLASTIC) b.alpha = tex_sample(m.tex[2], uv).x;
-            float dl = lum(tex_average(m.tex[0])), sl = lum(tex_average(m.tex[1]));
+            b.diffuse_refl = tex_sample(m.tex[0], uv, bm);
+            b.spec_refl = tex_sample(m.tex[1], uv, bm);
+            if (m.type == ROUGH_PLASTIC) b.alpha = tex_sample(m.tex[2], uv, bm).x;
+            float dl = lum(tex_average(m.tex[0], bm)), sl = lum(tex_average(m.tex[1], bm));
             b.spec_w = sl / (sl + dl);
             b.int_fdr = fr_diffuse(1.f / b.eta);
         } break;
@@ -774,11 +826,26 @@ struct World {
     bool has_env;
     float env_sel;
     V3 env_color;
+    // EnvMapEmitter (env.h:6-65) when s->env_type == 2
+    bool env_map;
+    const or_bitmap *env_bm;
+    float env_scale, env_norm;
+    uint32_t env_w, env_h;
+    std::vector<float> env_row_cdf, env_col_cdf, env_row_w;
+    float env_to_world[9], env_to_local[9];
 };
 
+void build_env_tables(const or_scene &s, World &w);
+
 float max3f(float r, float g, float b) { return r > g ? (r > b ? r : b) : (g > b ? g : b); }
-float tex_weight(const or_texture &t) {
+float tex_weight(const or_texture &t, const or_bitmap *bm) {  // GetWeight (world/emitter.cpp:73-101)
     if (t.type == 1) return (max3f(t.color0[0], t.color0[1], t.color0[2]) + max3f(t.color1[0], t.color1[1], t.color1[2])) * 0.5f;
+    if (t.type == 2) {
+        const or_bitmap &b = bm[t.bitmap];
+        float acc = 0.f;
+        for (size_t i = 0; i < (size_t)b.width * b.height; ++i) acc += max3f(b.rgba[4 * i], b.rgba[4 * i + 1], b.rgba[4 * i + 2]);
+        return acc / (1.f * (float)b.width * (float)b.height);
+    }
     return max3f(t.color0[0], t.color0[1], t.color0[2]);
 }
 
@@ -823,7 +890,7 @@ void build_world(const or_scene &s, World &w) {
                 e.rad = w.acc.sr[k];
                 e.area = 4 * 3.14159265358979323846f * e.rad * e.rad;  // emitter.cpp:238
                 e.radiance = sh.radiance;
-                e.weight = tex_weight(sh.radiance) * e.area;
+                e.weight = tex_weight(sh.radiance, s.bitmaps) * e.area;
                 w.emitters.push_back(e);
             }
             continue;
@@ -851,12 +918,14 @@ void build_world(const or_scene &s, World &w) {
             }
             e.area = len3(cross3(sub(e.p[1], e.p[0]), sub(e.p[2], e.p[0]))) * 0.5f;
             e.radiance = sh.radiance;
-            e.weight = tex_weight(sh.radiance) * e.area;
+            e.weight = tex_weight(sh.radiance, s.bitmaps) * e.area;
             w.emitters.push_back(e);
         }
     }
     w.has_env = s.env_type != 0;
     w.env_color = v3(s.env_radiance[0], s.env_radiance[1], s.env_radiance[2]);
+    w.env_map = s.env_type == 2;
+    if (w.env_map) build_env_tables(s, w);
     float sum = 0.f;
     for (auto &e : w.emitters) sum += e.weight;
     size_t na = w.emitters.size(), total = na + (w.has_env ? 1 : 0);
@@ -868,6 +937,126 @@ void build_world(const or_scene &s, World &w) {
     w.cdf.resize(na);
     float run = 0.f;
     for (size_t i = 0; i < na; ++i) { run = run + w.emitters[i].sel; w.cdf[i] = run; }
+}
+
+// BuildEnvMapCdfTable (world/emitter.cpp:107-149) and the frame of
+// world/emitter.cpp:293-312.  Float sums in the reference order; a black row
+// keeps its 0/0 column entries; an all-black map gets normalization 0 (the
+// reference divides by zero).  Pinned by: uniform-map and single-texel
+// closed forms of the rendered irradiance (tests).
+void build_env_tables(const or_scene &s, World &w) {
+    const or_bitmap &b = s.bitmaps[s.env_bitmap];
+    w.env_bm = &b;
+    w.env_scale = s.env_scale;
+    const uint32_t W = b.width, H = b.height;
+    w.env_w = W;
+    w.env_h = H;
+    w.env_row_cdf.assign(H + 1, 0.f);
+    w.env_col_cdf.assign((size_t)(W + 1) * H, 0.f);
+    w.env_row_w.assign(H, 0.f);
+    float total = 0.f;
+    for (uint32_t y = 0; y < H; ++y) {
+        float *cc = &w.env_col_cdf[(size_t)y * (W + 1)];
+        float run = 0.f;
+        for (uint32_t x = 0; x < W; ++x) {
+            const float *p = b.rgba + 4 * ((size_t)y * W + x);
+            run += lum(v3(p[0], p[1], p[2]));
+            cc[x + 1] = run;
+        }
+        for (uint32_t x = 1; x < W; ++x) cc[x] /= run;
+        cc[W] = 1.f;
+        w.env_row_w[y] = std::sin(((float)y + 0.5f) * PI_F / (float)H);
+        total += run * w.env_row_w[y];
+        w.env_row_cdf[y + 1] = total;
+    }
+    for (uint32_t y = 1; y < H; ++y) w.env_row_cdf[y] /= total;
+    w.env_row_cdf[H] = 1.f;
+    w.env_norm = total == 0.f ? 0.f : 1.f / (total * (2.f * PI_F / (float)W) * (PI_F / (float)H));
+    const float *m = s.env_to_world;
+    double det = (double)m[0] * ((double)m[4] * m[8] - (double)m[5] * m[7]) - (double)m[1] * ((double)m[3] * m[8] - (double)m[5] * m[6]) +
+                 (double)m[2] * ((double)m[3] * m[7] - (double)m[4] * m[6]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            int r0 = (j + 1) % 3, r1 = (j + 2) % 3, c0 = (i + 1) % 3, c1 = (i + 2) % 3;
+            w.env_to_local[3 * i + j] = (float)(((double)m[3 * r0 + c0] * m[3 * r1 + c1] - (double)m[3 * r0 + c1] * m[3 * r1 + c0]) / det);
+            w.env_to_world[3 * i + j] = m[3 * i + j];
+        }
+}
+
+V3 rows3(const float *r, V3 v) {
+    return v3(r[0] * v.x + r[1] * v.y + r[2] * v.z, r[3] * v.x + r[4] * v.y + r[5] * v.z, r[6] * v.x + r[7] * v.y + r[8] * v.z);
+}
+// Local direction of map angles (env.h:36): (sin t sin(pi - p), cos t, sin t cos(pi - p)).
+V3 env_dir(float phi, float theta) {
+    float st, ct, sp, cp;
+    sincos_or(theta, st, ct);
+    sincos_or(PI_F - phi, sp, cp);
+    return v3(st * sp, ct, st * cp);
+}
+V3 env_lookup(const World &w, V2 tex) { return mul(bitmap_lookup(*w.env_bm, tex.x, tex.y), w.env_scale); }
+V2 env_uv(const World &w, V3 dir, float &theta) {  // env.h:52-56
+    V3 d = rows3(w.env_to_local, dir);
+    float phi = PI_F - atan2_or(d.x, d.z);
+    theta = acos_or(d.y);
+    return V2{phi * 0.5f * INV_PI_F, theta * INV_PI_F};
+}
+// Density of the continuous sampler: texel share over its (phi, theta) cell.
+float env_pdf_cell(const World &w, V2 tex, float theta) {
+    uint32_t x = (uint32_t)fminf(fmaxf(tex.x * (float)w.env_w, 0.f), (float)(w.env_w - 1));
+    uint32_t y = (uint32_t)fminf(fmaxf(tex.y * (float)w.env_h, 0.f), (float)(w.env_h - 1));
+    const float *p = w.env_bm->rgba + 4 * ((size_t)y * w.env_bm->width + x);
+    float st, ct;
+    sincos_or(theta, st, ct);
+    if (!(st > 0.f)) return 0.f;
+    return lum(v3(p[0], p[1], p[2])) * w.env_row_w[y] * w.env_norm / st;
+}
+// Eval (env.h:51-63) or, with exact, the continuous sampler's density.
+void env_eval(const World &w, V3 dir, bool exact, V3 &rad, float &pdf) {
+    float theta;
+    V2 tex = env_uv(w, dir, theta);
+    rad = env_lookup(w, tex);
+    if (exact) { pdf = env_pdf_cell(w, tex, theta); return; }
+    uint32_t row = (uint32_t)(tex.y * (float)w.env_h);
+    row = std::min(row, w.env_h - 2);
+    float t = tex.y * (float)w.env_h - 1.f * (float)row;
+    float wr = w.env_row_w[row] + t * (w.env_row_w[row + 1] - w.env_row_w[row]);
+    float st, ct;
+    sincos_or(theta, st, ct);
+    pdf = lum(rad) * wr * w.env_norm / fmaxf(1e-4f, fabsf(st));
+}
+// SampleDirect (env.h:23-49): scans for the first entry not below the draw;
+// direction = that entry's texel corner.  The row is clamped to the last one
+// when the draw exceeds every row entry (the reference reads past its table).
+void env_sample_ref(const World &w, V2 xi, V3 &wi, V3 &rad, float &pdf) {
+    uint32_t row = 0;
+    while (row < w.env_h && !(xi.x <= w.env_row_cdf[row])) ++row;
+    row = std::min(row, w.env_h - 1);
+    const float *cc = &w.env_col_cdf[(size_t)row * (w.env_w + 1)];
+    uint32_t col = 0;
+    while (col < w.env_w - 1 && !(xi.y <= cc[col])) ++col;
+    float phi = (float)col * PI_F * 2.f / (float)w.env_w, theta = (float)row * PI_F / (float)w.env_h;
+    wi = rows3(w.env_to_world, env_dir(phi, theta));
+    rad = env_lookup(w, V2{phi * 0.5f * INV_PI_F, theta * INV_PI_F});
+    float st, ct;
+    sincos_or(theta, st, ct);
+    pdf = lum(rad) * w.env_row_w[row] * w.env_norm / fmaxf(1e-4f, fabsf(st));
+    if (pdf < 0.f) pdf = 0.f;
+}
+// Continuous sampling of the same tables: the cdf interval of the row and of
+// the column, the in-texel position from the rest of each draw.
+void env_sample_exact(const World &w, V2 xi, V3 &wi, V3 &rad, float &pdf) {
+    uint32_t y = 0;
+    while (y + 1 < w.env_h && !(xi.x <= w.env_row_cdf[y + 1])) ++y;
+    float r0 = w.env_row_cdf[y], r1 = w.env_row_cdf[y + 1];
+    float fy = r1 > r0 ? fminf(fmaxf((xi.x - r0) / (r1 - r0), 0.f), 1.f) : 0.5f;
+    const float *cc = &w.env_col_cdf[(size_t)y * (w.env_w + 1)];
+    uint32_t x = 0;
+    while (x + 1 < w.env_w && !(xi.y <= cc[x + 1])) ++x;
+    float c0 = cc[x], c1 = cc[x + 1];
+    float fx = c1 > c0 ? fminf(fmaxf((xi.y - c0) / (c1 - c0), 0.f), 1.f) : 0.5f;
+    float phi = ((float)x + fx) * PI_F * 2.f / (float)w.env_w, theta = ((float)y + fy) * PI_F / (float)w.env_h;
+    wi = rows3(w.env_to_world, env_dir(phi, theta));
+    env_eval(w, wi, true, rad, pdf);
 }
 
 // SelectOneEmiiter (render/emitter.h:110-136): linear scan, `p <= sum`.
@@ -941,12 +1130,12 @@ void hit_geo(const World &w, uint32_t prim, float t_hit, float b1, float b2, V3 
 
 struct ESample { V3 rad, wi; float dist, pdf; };
 
-void area_sample(const AreaEmitter &e, const Geo &hit, V2 xi, ESample &r) {  // area.h:17-35
+void area_sample(const AreaEmitter &e, const Geo &hit, V2 xi, ESample &r, const or_bitmap *bm) {  // area.h:17-35
     if (e.sphere) {  // sphere.h:14-33
         V3 t = uni_sphere(xi.x, xi.y);
         V3 pos = add(mul(t, e.rad), e.p[0]);
         V3 nrm = norm3(t);
-        r.rad = tex_sample(e.radiance, sphere_uv(t));
+        r.rad = tex_sample(e.radiance, sphere_uv(t), bm);
         r.wi = norm3(sub(pos, hit.pos));
         float nol = dot3(hit.n, r.wi), lnol = dot3(nrm, neg(r.wi));
         r.pdf = 0.f;
@@ -962,7 +1151,7 @@ void area_sample(const AreaEmitter &e, const Geo &hit, V2 xi, ESample &r) {  // 
     V3 pos = add(add(mul(e.p[0], t.x), mul(e.p[1], t.y)), mul(e.p[2], t.z));
     V3 nrm = norm3(add(add(mul(e.n[0], t.x), mul(e.n[1], t.y)), mul(e.n[2], t.z)));
     V2 tex = add2(add2(mul2(e.uv[0], t.x), mul2(e.uv[1], t.y)), mul2(e.uv[2], t.z));
-    r.rad = tex_sample(e.radiance, tex);
+    r.rad = tex_sample(e.radiance, tex, bm);
     r.wi = norm3(sub(pos, hit.pos));
     float nol = dot3(hit.n, r.wi), lnol = dot3(nrm, neg(r.wi));
     r.pdf = 0.f;
@@ -1020,13 +1209,19 @@ void trace_path(const World &w, const or_params &p, uint32_t max_depth, uint32_t
     std::memset(&bsdf, 0, sizeof(bsdf));
     out.prim = h.prim;
     if (done) {
-        if (w.has_env) { env_radiance = w.env_color; env_pdf = 0.25f * INV_PI_F; }
+        if (w.env_map) {
+            if (consistent) env_eval(w, ray_d, true, env_radiance, env_pdf);
+            else env_eval(w, norm3(sub(add(ray_o, ray_d), ray_o)), false, env_radiance, env_pdf);
+        } else if (w.has_env) {
+            env_radiance = w.env_color;
+            env_pdf = 0.25f * INV_PI_F;
+        }
         out.albedo = v3s(0.f);
         out.normal = v3s(0.f);
     } else {
         hit_geo(w, h.prim, h.t, h.b1, h.b2, ray_o, ray_d, g, mat, emitter);
-        bsdf = make_bsdf(s.materials[mat], g.uv);
-        if (emitter >= 0) radiance = add(radiance, tex_sample(w.emitters[emitter].radiance, g.uv));
+        bsdf = make_bsdf(s.materials[mat], g.uv, s.bitmaps);
+        if (emitter >= 0) radiance = add(radiance, tex_sample(w.emitters[emitter].radiance, g.uv, s.bitmaps));
         out.albedo = albedo_of(bsdf);
         out.normal = g.n;
     }
@@ -1050,8 +1245,13 @@ void trace_path(const World &w, const or_params &p, uint32_t max_depth, uint32_t
             es.dist = 0.f;
             float sel_prob = 0.f;
             if (sel >= 0) {
-                area_sample(w.emitters[sel], g, V2{x, y}, es);
+                area_sample(w.emitters[sel], g, V2{x, y}, es, s.bitmaps);
                 sel_prob = w.emitters[sel].sel;
+            } else if (sel == -1 && w.env_map) {  // EnvMapEmitter::SampleDirect
+                if (consistent) env_sample_exact(w, V2{x, y}, es.wi, es.rad, es.pdf);
+                else env_sample_ref(w, V2{x, y}, es.wi, es.rad, es.pdf);
+                es.dist = 1e16f;
+                sel_prob = w.env_sel;
             } else if (sel == -1) {  // ConstEnvEmitter::SampleDirect (env.h:70-80)
                 V3 lw = uni_hemi(x, y);
                 es.wi = to_world(lw, g.n);
@@ -1099,10 +1299,22 @@ void trace_path(const World &w, const or_params &p, uint32_t max_depth, uint32_t
         counts[0]++;
         bool h2 = accel_trace(w.acc, ray_o, ray_d, 0.001f, 1e16f, false, h, counts + 2, counts + 3);
         if (!h2) {
-            if (w.has_env) { env_radiance = w.env_color; env_pdf = 0.25f * INV_PI_F; }
+            float lp = 0.f;  // selection x next-event density of this direction
+            if (w.env_map) {
+                if (consistent) {
+                    float p;
+                    env_eval(w, ray_d, true, env_radiance, p);
+                    lp = w.env_sel * p;
+                } else {  // __miss__default evaluates the direction (o + d) - o
+                    env_eval(w, norm3(sub(add(ray_o, ray_d), ray_o)), false, env_radiance, env_pdf);
+                }
+            } else if (w.has_env) {
+                env_radiance = w.env_color;
+                env_pdf = 0.25f * INV_PI_F;
+                lp = prev_upper ? w.env_sel * (INV_PI_F * 0.5f) : 0.f;
+            }
             float mis;
             if (consistent) {
-                float lp = prev_upper ? w.env_sel * (INV_PI_F * 0.5f) : 0.f;
                 mis = prev_delta ? 1.f : misw(br.pdf, lp);
             } else {
                 mis = misw(br.pdf, env_pdf);
@@ -1111,7 +1323,7 @@ void trace_path(const World &w, const or_params &p, uint32_t max_depth, uint32_t
             break;
         }
         hit_geo(w, h.prim, h.t, h.b1, h.b2, ray_o, ray_d, g, mat, emitter);
-        bsdf = make_bsdf(s.materials[mat], g.uv);
+        bsdf = make_bsdf(s.materials[mat], g.uv, s.bitmaps);
         if (emitter >= 0) {
             const AreaEmitter &e = w.emitters[emitter];
             // TriAreaEmitter::Eval (area.h:37-45)
@@ -1122,7 +1334,7 @@ void trace_path(const World &w, const or_params &p, uint32_t max_depth, uint32_t
             if (lnol > 0.f) {
                 float dist = len3(sub(ray_o, g.pos));
                 epdf = dist * dist / (lnol * e.area);
-                erad = tex_sample(e.radiance, g.uv);
+                erad = tex_sample(e.radiance, g.uv, s.bitmaps);
             }
             if (!iszero(epdf)) {
                 float lpdf = epdf;
@@ -1252,7 +1464,7 @@ void oracle_camera_ray(const or_scene *scene, uint32_t x, uint32_t y, float jx, 
 }
 
 void oracle_bsdf_eval(const or_material *m, const float *wo, const float *wi, float *out) {
-    Bsdf b = make_bsdf(*m, V2{0.f, 0.f});
+    Bsdf b = make_bsdf(*m, V2{0.f, 0.f}, nullptr);
     BRec r;
     r.wo = v3(wo[0], wo[1], wo[2]);
     r.wi = v3(wi[0], wi[1], wi[2]);
@@ -1261,7 +1473,7 @@ void oracle_bsdf_eval(const or_material *m, const float *wo, const float *wi, fl
 }
 
 void oracle_bsdf_sample(const or_material *m, const float *wo, uint32_t *seed, float *out) {
-    Bsdf b = make_bsdf(*m, V2{0.f, 0.f});
+    Bsdf b = make_bsdf(*m, V2{0.f, 0.f}, nullptr);
     Random rng;
     rng.seed = *seed;
     BRec r;
@@ -1293,6 +1505,12 @@ int oracle_intersect(const float *o, const float *d, const float *v0, const floa
     return 1;
 }
 
+void oracle_bitmap_sample(const or_bitmap *b, float u, float v, float *out3) {
+    V3 c = bitmap_lookup(*b, u, v);
+    out3[0] = c.x;
+    out3[1] = c.y;
+    out3[2] = c.z;
+}
 float oracle_atan2(float y, float x) { return atan2_or(y, x); }
 float oracle_acos(float z) { return acos_or(z); }
 void oracle_sphere_texcoord(const float *dir, float *uv) {

@@ -22,13 +22,20 @@ extern "C" {
 #endif
 
 typedef struct or_texture {
-    uint32_t type; /* 0 rgb, 1 checkerboard */
+    uint32_t type; /* 0 rgb, 1 checkerboard, 2 bitmap */
     uint32_t bitmap;
     float color0[3];
     float color1[3];
     float uv_row0[4];
     float uv_row1[4];
 } or_texture;
+
+typedef struct or_bitmap {
+    uint32_t width, height;
+    uint32_t filter; /* 0 point, 1 bilinear */
+    uint32_t wrap;   /* 0 repeat, 1 clamp, 2 mirror, 3 border */
+    const float *rgba; /* row 0 sampled at v = 0 */
+} or_bitmap;
 
 typedef struct or_material {
     uint32_t type;
@@ -63,6 +70,11 @@ typedef struct or_scene {
     uint32_t width, height, max_depth;
     uint32_t num_spheres;
     const float *spheres; /* 12 floats each: object -> world 3x4 rows of the unit sphere */
+    uint32_t num_bitmaps;
+    const or_bitmap *bitmaps;
+    uint32_t env_bitmap;  /* env_type 2: environment map */
+    float env_scale;
+    float env_to_world[9];
 } or_scene;
 
 typedef struct or_params {
@@ -113,6 +125,9 @@ void oracle_ggx_sample(const float *wo, float alpha, float u1, float u2, float *
 /* Sphere unit functions: atan2 and acos as the oracle evaluates them, the
  * sphere texcoord (phi / 2pi, theta / pi) of a unit direction, and the ray /
  * sphere test (nearest root in [tmin, tmax]; returns 1 on hit). */
+/* Bitmap lookup (point / bilinear, repeat / clamp / mirror / border) at
+ * texture coordinates (u, v). */
+void oracle_bitmap_sample(const or_bitmap *b, float u, float v, float *out3);
 float oracle_atan2(float y, float x);
 float oracle_acos(float z);
 void oracle_sphere_texcoord(const float *dir, float *uv);

@@ -35,7 +35,9 @@ TEX_RGB, TEX_CHECKER, TEX_BITMAP = 0, 1, 2
 SHAPE_FLIP_NORMALS, SHAPE_FLIP_TEXCOORDS, SHAPE_HAS_NORMALS, SHAPE_HAS_TEXCOORDS, SHAPE_NEG_FACE_NORMAL, \
     SHAPE_SPHERE = 1, 2, 4, 8, 16, 32
 PRIM_SPHERE = 1 << 30  # primitive ids of spheres: PRIM_SPHERE | sphere index
-ENV_NONE, ENV_CONSTANT = 0, 1
+ENV_NONE, ENV_CONSTANT, ENV_MAP = 0, 1, 2
+FILTER_POINT, FILTER_LINEAR = 0, 1
+WRAP_REPEAT, WRAP_CLAMP, WRAP_MIRROR, WRAP_BORDER = 0, 1, 2, 3
 MIS_REFERENCE, MIS_CONSISTENT = 0, 1
 RENDER_AOV, RENDER_FP64_ACCUM, RENDER_TIMING, RENDER_GUIDE, RENDER_GUIDE_TRAIN = 1, 2, 4, 8, 16
 GUIDE_OUTPUTS, GUIDE_LEVELS = 41, 16
@@ -69,6 +71,11 @@ class SphereDesc(C.Structure):
     _fields_ = [("to_world", C.c_float * 12)]
 
 
+class BitmapDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("filter", C.c_uint32), ("wrap", C.c_uint32),
+                ("rgba", C.POINTER(C.c_float))]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("num_shapes", C.c_uint32),
                 ("num_materials", C.c_uint32), ("positions", C.POINTER(C.c_float)),
@@ -77,7 +84,8 @@ class SceneDesc(C.Structure):
                 ("materials", C.POINTER(MaterialDesc)), ("env_type", C.c_uint32),
                 ("env_radiance", C.c_float * 3), ("camera", CameraDesc), ("width", C.c_uint32),
                 ("height", C.c_uint32), ("max_depth", C.c_uint32), ("num_spheres", C.c_uint32),
-                ("spheres", C.POINTER(SphereDesc))]
+                ("spheres", C.POINTER(SphereDesc)), ("num_bitmaps", C.c_uint32), ("bitmaps", C.POINTER(BitmapDesc)),
+                ("env_bitmap", C.c_uint32), ("env_scale", C.c_float), ("env_to_world", C.c_float * 9)]
 
 
 class BvhOpts(C.Structure):
@@ -176,6 +184,8 @@ _SIGNATURES = {
     "pt_scene_get_desc": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
     "pt_scene_destroy": (None, [C.c_void_p]),
     "pt_write_exr": (C.c_int, [C.c_char_p, C.c_void_p, C.c_uint32, C.c_uint32]),
+    "pt_load_image": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.c_void_p, C.c_size_t,
+                                C.c_char_p, C.c_size_t]),
     "pt_host_trace_rays": (C.c_int, [C.POINTER(SceneDesc), C.c_void_p, C.c_uint32, C.c_int, C.c_void_p,
                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "pt_guide_enable": (C.c_int, [C.c_void_p, C.POINTER(GuideConfig)]),
@@ -278,6 +288,16 @@ class HostScene:
         d = self.desc
         return [d.shapes[i] for i in range(d.num_shapes)]
 
+    def bitmaps(self):
+        """(width, height, filter, wrap, rgba array) of each bitmap."""
+        d = self.desc
+        out = []
+        for i in range(d.num_bitmaps):
+            b = d.bitmaps[i]
+            out.append((b.width, b.height, b.filter, b.wrap,
+                        np.ctypeslib.as_array(b.rgba, shape=(b.height, b.width, 4))))
+        return out
+
     def spheres(self):
         """Object -> world 3x4 matrices of the analytic spheres (N x 3 x 4)."""
         d = self.desc
@@ -310,6 +330,20 @@ def host_trace_rays(desc: SceneDesc, rays: np.ndarray, any_hit: bool = False):
                                     C.byref(nv), C.byref(tv)), "pt_host_trace_rays")
     return (hits[:, 0].view(np.float32), hits[:, 1].copy(), hits[:, 2].view(np.float32),
             hits[:, 3].view(np.float32), nv.value, tv.value)
+
+
+def load_image(path: str) -> np.ndarray:
+    """util::BitmapTexture::Load: EXR / Radiance HDR / PNG as (h, w, 4) float32,
+    row 0 = top row of the file."""
+    w, h = C.c_uint32(), C.c_uint32()
+    err = C.create_string_buffer(512)
+    code = lib().pt_load_image(path.encode(), C.byref(w), C.byref(h), None, 0, err, 512)
+    if code != PT_OK:
+        raise PupilError(code, f"pt_load_image({path}): {err.value.decode()}")
+    out = np.zeros((h.value, w.value, 4), np.float32)
+    _check(lib().pt_load_image(path.encode(), C.byref(w), C.byref(h), out.ctypes.data, out.size, err, 512),
+           "pt_load_image")
+    return out
 
 
 def write_exr(path: str, rgba: np.ndarray):

@@ -78,6 +78,25 @@ int pt_scene_get_desc(const pt_host_scene *scene, pt_scene_desc *out) {
 
 void pt_scene_destroy(pt_host_scene *scene) { delete scene; }
 
+int pt_load_image(const char *path, uint32_t *width, uint32_t *height, float *rgba, size_t capacity, char *err,
+                  size_t err_len) {
+    if (!path || !width || !height) return PT_ERR_INVALID_ARG;
+    uint32_t w = 0, h = 0;
+    std::vector<float> data;
+    std::string e;
+    if (!pupil::load_image(path, w, h, data, e)) {
+        copy_err(e, err, err_len);
+        return e.rfind("io:", 0) == 0 ? PT_ERR_IO : (e.rfind("unsupported", 0) == 0 ? PT_ERR_UNSUPPORTED : PT_ERR_PARSE);
+    }
+    *width = w;
+    *height = h;
+    if (rgba) {
+        if (capacity < data.size()) return PT_ERR_CAPACITY;
+        std::memcpy(rgba, data.data(), data.size() * sizeof(float));
+    }
+    return PT_OK;
+}
+
 int pt_write_exr(const char *path, const float *rgba, uint32_t width, uint32_t height) {
     if (!path || !rgba || width == 0 || height == 0) return PT_ERR_INVALID_ARG;
     std::string e;

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <map>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -717,6 +718,7 @@ struct BatchView {
     CameraRec camera{};
     EnvRec env{};
     uint32_t root = 0, emitter_begin = 0, emitter_count = 0, tri_count = 0;
+    size_t env_tables = 0;  // offset of this view's env-map tables in BatchState::env_tables
     float4 *d_accum = nullptr, *d_frame = nullptr;
     double *d_accum64 = nullptr;
     uint32_t sample_cnt = 0, random_seed = 0;  // progressive state, as pt_pass.cpp:55-56 per view
@@ -732,6 +734,9 @@ struct BatchState {
     std::vector<MaterialRec> materials;
     std::vector<EmitterRec> emitters;
     std::vector<float> cdf;
+    std::vector<BitmapRec> bitmaps;
+    std::vector<float> texels;
+    std::vector<float> env_tables;
     std::vector<WideNode> nodes;
     std::vector<LeafTri> leaf;
     bool any_normals = false, any_texcoords = false;
@@ -744,6 +749,9 @@ struct BatchState {
     MaterialRec *d_materials = nullptr;
     EmitterRec *d_emitters = nullptr;
     float *d_cdf = nullptr;
+    BitmapRec *d_bitmaps = nullptr;
+    float4 *d_texels = nullptr;
+    float *d_env_tables = nullptr;
     WideNode *d_nodes = nullptr;
     LeafTri *d_leaf = nullptr;
     BatchViewDev *d_views = nullptr;
@@ -789,6 +797,9 @@ struct pt_ctx {
     MaterialRec *d_materials = nullptr;
     EmitterRec *d_emitters = nullptr;
     float *d_cdf = nullptr;
+    BitmapRec *d_bitmaps = nullptr;
+    float4 *d_texels = nullptr;
+    float *d_env_tables = nullptr;
     WideNode *d_nodes = nullptr;
     LeafTri *d_leaf = nullptr;
     BvhBuildStats bvh_stats{};
@@ -869,6 +880,9 @@ void free_scene(pt_ctx *ctx) {
     dfree(ctx->d_materials);
     dfree(ctx->d_emitters);
     dfree(ctx->d_cdf);
+    dfree(ctx->d_bitmaps);
+    dfree(ctx->d_texels);
+    dfree(ctx->d_env_tables);
     dfree(ctx->d_nodes);
     dfree(ctx->d_leaf);
     dfree(ctx->d_accum);
@@ -920,23 +934,44 @@ TextureRec to_texture_rec(const pt_texture_desc &t) {
     return r;
 }
 
-// Average color of a texture (optix_material.cpp:9-36).
-f3 texture_average(const pt_texture_desc &t) {
+// Average color of a texture (GetPixelAverage, optix_material.cpp:9-36); a
+// bitmap averages its texels in row order.
+f3 texture_average(const pt_texture_desc &t, const pt_bitmap_desc *bitmaps) {
     if (t.type == PT_TEX_CHECKER)
         return mk3(t.color0[0] + t.color1[0], t.color0[1] + t.color1[1], t.color0[2] + t.color1[2]) * 0.5f;
+    if (t.type == PT_TEX_BITMAP) {
+        const pt_bitmap_desc &b = bitmaps[t.bitmap];
+        float r = 0.f, g = 0.f, bl = 0.f;
+        const size_t n = (size_t)b.width * b.height;
+        for (size_t i = 0; i < n; ++i) {
+            r += b.rgba[4 * i];
+            g += b.rgba[4 * i + 1];
+            bl += b.rgba[4 * i + 2];
+        }
+        return mk3(r, g, bl) / (1.f * (float)b.height * (float)b.width);
+    }
     return mk3(t.color0[0], t.color0[1], t.color0[2]);
 }
 
-// Emitter selection weight of a radiance texture (world/emitter.cpp:73-101).
-float texture_weight(const pt_texture_desc &t) {
+// Emitter selection weight of a radiance texture (GetWeight, world/emitter.cpp:
+// 73-101): the largest channel, averaged over a bitmap's texels (all texels;
+// world/emitter.cpp's loop indexes a square image's texels).
+float texture_weight(const pt_texture_desc &t, const pt_bitmap_desc *bitmaps) {
     auto mx = [](float r, float g, float b) { return r > g ? (r > b ? r : b) : (g > b ? g : b); };
     if (t.type == PT_TEX_CHECKER)
         return (mx(t.color0[0], t.color0[1], t.color0[2]) + mx(t.color1[0], t.color1[1], t.color1[2])) * 0.5f;
+    if (t.type == PT_TEX_BITMAP) {
+        const pt_bitmap_desc &b = bitmaps[t.bitmap];
+        float w = 0.f;
+        const size_t n = (size_t)b.width * b.height;
+        for (size_t i = 0; i < n; ++i) w += mx(b.rgba[4 * i], b.rgba[4 * i + 1], b.rgba[4 * i + 2]);
+        return w / (1.f * (float)b.width * (float)b.height);
+    }
     return mx(t.color0[0], t.color0[1], t.color0[2]);
 }
 
 // optix_material.cpp:41-132 host packing.
-MaterialRec to_material_rec(const pt_material_desc &m) {
+MaterialRec to_material_rec(const pt_material_desc &m, const pt_bitmap_desc *bitmaps) {
     MaterialRec r;
     std::memset(&r, 0, sizeof(r));
     r.type = m.type;
@@ -944,8 +979,8 @@ MaterialRec to_material_rec(const pt_material_desc &m) {
     for (int i = 0; i < 4; ++i) r.tex[i] = to_texture_rec(m.tex[i]);
     r.eta = m.int_ior / m.ext_ior;
     if (m.type == PT_MAT_PLASTIC || m.type == PT_MAT_ROUGH_PLASTIC) {
-        float diffuse_lum = luminance(texture_average(m.tex[0]));
-        float specular_lum = luminance(texture_average(m.tex[1]));
+        float diffuse_lum = luminance(texture_average(m.tex[0], bitmaps));
+        float specular_lum = luminance(texture_average(m.tex[1], bitmaps));
         r.spec_weight = specular_lum / (specular_lum + diffuse_lum);
         r.int_fdr = fresnel_diffuse_reflectance(1.f / r.eta);
     }
@@ -1001,6 +1036,7 @@ SceneView scene_view(pt_ctx *ctx) {
     v.emitter_cdf = ctx->d_cdf;
     v.num_emitters = (uint32_t)ctx->h_emitters.size();
     v.pad = 0;
+    v.tex = TexArrays{ctx->d_bitmaps, reinterpret_cast<const f4 *>(ctx->d_texels)};
     v.env = ctx->env;
     return v;
 }
@@ -1064,12 +1100,83 @@ struct SceneHostData {
     std::vector<MaterialRec> h_materials;
     std::vector<EmitterRec> h_emitters;
     std::vector<float> h_cdf;
+    std::vector<BitmapRec> h_bitmaps;
+    std::vector<float> h_texels;     // 4 floats per texel
+    std::vector<float> h_env_tables; // env map: row cdf (h + 1) | column cdfs ((w + 1) h) | row weights (h)
     EnvRec env{};
     CameraRec camera{};
     uint32_t width = 0, height = 0, max_depth = 1, num_tris = 0, num_spheres = 0;
     float lo[3]{}, hi[3]{};
     bool has_normals = false, has_texcoords = false;
 };
+
+// EnvMapEmitter set-up (world/emitter.cpp:107-149 BuildEnvMapCdfTable and
+// :293-312): per row the running luminance sums normalized to a column cdf,
+// per row the weight sin((y + 1/2) pi / h), the row cdf over the weighted row
+// sums, and the normalization 1 / (sum (2 pi / w) (pi / h)).  Sums run in
+// float in the same order.  A black row keeps the 0/0 entries the reference
+// table holds.  Frame: to_world = the 3x3 of the emitter transform, to_local
+// its inverse.
+void build_env_map(const pt_scene_desc *s, SceneHostData &out) {
+    const pt_bitmap_desc &bm = s->bitmaps[s->env_bitmap];
+    const size_t w = bm.width, h = bm.height;
+    std::vector<float> &t = out.h_env_tables;
+    t.assign((h + 1) + (w + 1) * h + h, 0.f);
+    float *row_cdf = t.data(), *col_cdf = row_cdf + (h + 1), *row_weight = col_cdf + (w + 1) * h;
+    size_t ci = 0, ri = 0;
+    float row_sum = 0.f;
+    row_cdf[ri++] = 0.f;
+    for (size_t y = 0; y < h; ++y) {
+        float col_sum = 0.f;
+        col_cdf[ci++] = 0.f;
+        for (size_t x = 0; x < w; ++x) {
+            const float *p = bm.rgba + 4 * (y * w + x);
+            col_sum += luminance(mk3(p[0], p[1], p[2]));
+            col_cdf[ci++] = col_sum;
+        }
+        for (size_t x = 1; x < w; ++x) col_cdf[ci - x - 1] /= col_sum;
+        col_cdf[ci - 1] = 1.f;
+        const float weight = std::sin(((float)y + 0.5f) * kPi / (float)h);
+        row_weight[y] = weight;
+        row_sum += col_sum * weight;
+        row_cdf[ri++] = row_sum;
+    }
+    for (size_t y = 1; y < h; ++y) row_cdf[ri - y - 1] /= row_sum;
+    row_cdf[ri - 1] = 1.f;
+    EnvRec &e = out.env;
+    e.bitmap = s->env_bitmap;
+    e.scale = s->env_scale;
+    e.w = (uint32_t)w;
+    e.h = (uint32_t)h;
+    e.normalization = 1.f / (row_sum * (2.f * kPi / (float)w) * (kPi / (float)h));
+    const float *m = s->env_to_world;
+    double a[3][3], inv[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) a[i][j] = m[3 * i + j];
+    const double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+                       a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                       a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const int r0 = (j + 1) % 3, r1 = (j + 2) % 3, c0 = (i + 1) % 3, c1 = (i + 2) % 3;
+            inv[i][j] = (a[r0][c0] * a[r1][c1] - a[r0][c1] * a[r1][c0]) / det;
+        }
+    for (int i = 0; i < 3; ++i) {
+        e.to_world[i] = f4{m[3 * i], m[3 * i + 1], m[3 * i + 2], 0.f};
+        e.to_local[i] = f4{(float)inv[i][0], (float)inv[i][1], (float)inv[i][2], 0.f};
+    }
+}
+
+// Device pointers of an env map's tables inside one uploaded array.
+void env_table_pointers(EnvRec &e, const float *base) {
+    if (e.type != kEmitterEnvMap || !base) {
+        e.row_cdf = e.col_cdf = e.row_weight = nullptr;
+        return;
+    }
+    e.row_cdf = base;
+    e.col_cdf = base + (e.h + 1u);
+    e.row_weight = e.col_cdf + (size_t)(e.w + 1u) * e.h;
+}
 
 // Validation and packing of PTPass::SetScene + World::LoadScene
 // (pt_pass.cpp:127-240, world/world.cpp:108-146, world/emitter.cpp:169-337).
@@ -1118,6 +1225,39 @@ int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, s
             err = std::string(who) + ": sphere to_world must be a finite similarity transform (uniform scale)";
             return PT_ERR_UNSUPPORTED;
         }
+    }
+    // Bitmaps and the textures that name them.
+    const uint32_t nb = s->num_bitmaps;
+    if (nb && !s->bitmaps) {
+        err = std::string(who) + ": missing bitmap array";
+        return PT_ERR_INVALID_ARG;
+    }
+    for (uint32_t i = 0; i < nb; ++i) {
+        const pt_bitmap_desc &b = s->bitmaps[i];
+        if (!b.rgba || b.width == 0 || b.height == 0 || b.width > 65536 || b.height > 65536 ||
+            b.filter > PT_FILTER_LINEAR || b.wrap > PT_WRAP_BORDER) {
+            err = std::string(who) + ": invalid bitmap " + std::to_string(i);
+            return PT_ERR_INVALID_ARG;
+        }
+    }
+    auto tex_ok = [&](const pt_texture_desc &t) {
+        return t.type == PT_TEX_RGB || t.type == PT_TEX_CHECKER || (t.type == PT_TEX_BITMAP && t.bitmap < nb);
+    };
+    for (uint32_t i = 0; i < nm; ++i)
+        for (int k = 0; k < 4; ++k)
+            if (!tex_ok(s->materials[i].tex[k])) {
+                err = std::string(who) + ": material texture names an unknown type or bitmap";
+                return PT_ERR_INVALID_ARG;
+            }
+    for (uint32_t i = 0; i < ns; ++i)
+        if (s->shapes[i].is_emitter && !tex_ok(s->shapes[i].radiance)) {
+            err = std::string(who) + ": emitter radiance names an unknown texture type or bitmap";
+            return PT_ERR_INVALID_ARG;
+        }
+    if (s->env_type > PT_ENV_MAP ||
+        (s->env_type == PT_ENV_MAP && (s->env_bitmap >= nb || s->bitmaps[s->env_bitmap].height < 2))) {
+        err = std::string(who) + ": invalid environment (an env map needs a bitmap of height >= 2)";
+        return PT_ERR_INVALID_ARG;
     }
     for (uint32_t i = 0; i < 3 * nt; ++i)
         if (s->indices[i] >= nv) {
@@ -1208,7 +1348,7 @@ int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, s
                 e.p0 = f4{out.h_sph4[4 * k], out.h_sph4[4 * k + 1], out.h_sph4[4 * k + 2], rad};
                 e.area = 4.f * 3.14159265358979323846f * rad * rad;
                 e.radiance = to_texture_rec(sh.radiance);
-                e.weight = texture_weight(sh.radiance) * e.area;
+                e.weight = texture_weight(sh.radiance, s->bitmaps) * e.area;
                 out.h_emitters.push_back(e);
             }
             continue;
@@ -1255,7 +1395,7 @@ int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, s
             e.uv2 = f4{uv[2].x, uv[2].y, 0.f, 0.f};
             e.area = length(cross(p[1] - p[0], p[2] - p[0])) * 0.5f;
             e.radiance = to_texture_rec(sh.radiance);
-            e.weight = texture_weight(sh.radiance) * e.area;
+            e.weight = texture_weight(sh.radiance, s->bitmaps) * e.area;
             out.h_emitters.push_back(e);
         }
     }
@@ -1280,10 +1420,42 @@ int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, s
             out.env.type = kEmitterConstEnv;
             out.env.select_prob = 1.f / (float)total;
             out.env.color = f4{s->env_radiance[0], s->env_radiance[1], s->env_radiance[2], 0.f};
+        } else if (s->env_type == PT_ENV_MAP) {
+            out.env.type = kEmitterEnvMap;
+            out.env.select_prob = 1.f / (float)total;
         }
     }
     out.h_materials.resize(nm);
-    for (uint32_t i = 0; i < nm; ++i) out.h_materials[i] = to_material_rec(s->materials[i]);
+    for (uint32_t i = 0; i < nm; ++i) out.h_materials[i] = to_material_rec(s->materials[i], s->bitmaps);
+    {
+        // Texels are stored once per image: bitmaps that sample the same
+        // texel array in other modes share it.
+        std::map<const float *, size_t> stored;
+        out.h_bitmaps.resize(nb);
+        out.h_texels.clear();
+        for (uint32_t i = 0; i < nb; ++i) {
+            const pt_bitmap_desc &b = s->bitmaps[i];
+            const size_t n = (size_t)b.width * b.height;
+            auto it = stored.find(b.rgba);
+            size_t off;
+            if (it != stored.end()) {
+                off = it->second;
+            } else {
+                off = out.h_texels.size() / 4;
+                out.h_texels.insert(out.h_texels.end(), b.rgba, b.rgba + 4 * n);
+                stored[b.rgba] = off;
+            }
+            BitmapRec r;
+            std::memset(&r, 0, sizeof(r));
+            r.offset = off;
+            r.w = b.width;
+            r.h = b.height;
+            r.filter = b.filter;
+            r.wrap = b.wrap;
+            out.h_bitmaps[i] = r;
+        }
+    }
+    if (s->env_type == PT_ENV_MAP) build_env_map(s, out);
     for (int r = 0; r < 4; ++r) {
         out.camera.s2c[r] = f4{s->camera.sample_to_camera[4 * r], s->camera.sample_to_camera[4 * r + 1],
                                 s->camera.sample_to_camera[4 * r + 2], s->camera.sample_to_camera[4 * r + 3]};
@@ -1321,6 +1493,9 @@ void batch_free_device(pt_ctx *ctx) {
     dfree(b.d_materials);
     dfree(b.d_emitters);
     dfree(b.d_cdf);
+    dfree(b.d_bitmaps);
+    dfree(b.d_texels);
+    dfree(b.d_env_tables);
     dfree(b.d_nodes);
     dfree(b.d_leaf);
     dfree(b.d_views);
@@ -1362,6 +1537,10 @@ int batch_upload(pt_ctx *ctx) {
     if ((rc = upload(ctx, b.d_materials, b.materials.data(), b.materials.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, b.d_emitters, b.emitters.data(), b.emitters.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, b.d_cdf, b.cdf.data(), b.cdf.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, b.d_bitmaps, b.bitmaps.data(), b.bitmaps.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, b.d_texels, reinterpret_cast<const float4 *>(b.texels.data()), b.texels.size() / 4)) != PT_OK)
+        return rc;
+    if ((rc = upload(ctx, b.d_env_tables, b.env_tables.data(), b.env_tables.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, b.d_nodes, b.nodes.data(), b.nodes.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, b.d_leaf, b.leaf.data(), b.leaf.size())) != PT_OK) return rc;
     std::vector<BatchViewDev> vd(b.views.size());
@@ -1372,6 +1551,7 @@ int batch_upload(pt_ctx *ctx) {
         std::memset(&d, 0, sizeof(d));
         d.camera = v.camera;
         d.env = v.env;
+        env_table_pointers(d.env, b.d_env_tables ? b.d_env_tables + v.env_tables : nullptr);
         d.width = v.width;
         d.height = v.height;
         d.max_depth = v.max_depth;
@@ -1456,6 +1636,7 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
     sv.emitter_cdf = b.d_cdf;
     sv.num_emitters = 0;
     sv.pad = 0;
+    sv.tex = TexArrays{b.d_bitmaps, reinterpret_cast<const f4 *>(b.d_texels)};
     std::memset(&sv.env, 0, sizeof(sv.env));
     const DeviceBvh bvh{b.d_nodes, b.d_leaf, ctx->d_spill};
     AovBuffers aov{nullptr, nullptr, nullptr, nullptr, 0u, 1u};
@@ -1714,6 +1895,12 @@ int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
     if ((rc = upload(ctx, ctx->d_materials, ctx->h_materials.data(), ctx->h_materials.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, ctx->d_emitters, ctx->h_emitters.data(), ctx->h_emitters.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, ctx->d_cdf, ctx->h_cdf.data(), ctx->h_cdf.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, ctx->d_bitmaps, d.h_bitmaps.data(), d.h_bitmaps.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, ctx->d_texels, reinterpret_cast<const float4 *>(d.h_texels.data()), d.h_texels.size() / 4)) !=
+        PT_OK)
+        return rc;
+    if ((rc = upload(ctx, ctx->d_env_tables, d.h_env_tables.data(), d.h_env_tables.size())) != PT_OK) return rc;
+    env_table_pointers(ctx->env, ctx->d_env_tables);
 
     const size_t npix = (size_t)ctx->width * ctx->height;
     PT_HIP(hipMalloc((void **)&ctx->d_accum, npix * sizeof(float4)));
@@ -2355,6 +2542,20 @@ int pt_batch_add_view(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uin
         sp.shape += shape0;
         b.spheres.push_back(sp);
     }
+    // Bitmap indices and texel offsets become batch-global.
+    const uint32_t bmp0 = (uint32_t)b.bitmaps.size();
+    const uint64_t texel0 = b.texels.size() / 4;
+    for (MaterialRec &m : d.h_materials)
+        for (TextureRec &t : m.tex)
+            if (t.type == kTexBitmap) t.bitmap += bmp0;
+    for (EmitterRec &e : d.h_emitters)
+        if (e.radiance.type == kTexBitmap) e.radiance.bitmap += bmp0;
+    for (BitmapRec &r : d.h_bitmaps) r.offset += texel0;
+    if (d.env.type == kEmitterEnvMap) d.env.bitmap += bmp0;
+    const size_t env0 = b.env_tables.size();
+    b.bitmaps.insert(b.bitmaps.end(), d.h_bitmaps.begin(), d.h_bitmaps.end());
+    b.texels.insert(b.texels.end(), d.h_texels.begin(), d.h_texels.end());
+    b.env_tables.insert(b.env_tables.end(), d.h_env_tables.begin(), d.h_env_tables.end());
     b.materials.insert(b.materials.end(), d.h_materials.begin(), d.h_materials.end());
     b.emitters.insert(b.emitters.end(), d.h_emitters.begin(), d.h_emitters.end());
     b.cdf.insert(b.cdf.end(), d.h_cdf.begin(), d.h_cdf.end());
@@ -2365,6 +2566,7 @@ int pt_batch_add_view(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uin
     v.max_depth = d.max_depth;
     v.camera = d.camera;
     v.env = d.env;
+    v.env_tables = env0;
     v.root = (uint32_t)node0;
     v.emitter_begin = em0;
     v.emitter_count = (uint32_t)d.h_emitters.size();

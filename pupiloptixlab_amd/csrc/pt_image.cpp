@@ -1,0 +1,562 @@
+// Bitmap loading for textures and environment maps: util::BitmapTexture::Load
+// (util/texture.cpp:87-174), which reads EXR through tinyexr and everything
+// else through stb_image.  Written here from the file-format specifications:
+//   OpenEXR  scanline files, compression NONE / RLE / ZIPS / ZIP, pixel types
+//            UINT / HALF / FLOAT (channels R, G, B, A or Y);
+//   Radiance RGBE (.hdr / .pic), flat and new run-length scanlines;
+//   PNG      8- and 16-bit (and 1/2/4-bit gray / palette), non-interlaced.
+// Output: RGBA float, row 0 = the first (top) row of the file, as both
+// loaders of the reference return it.  8-bit data is linearized with
+// (v / 255)^2.2 as in util/texture.cpp:106-111 (16-bit PNG channels keep their
+// high byte first, as stb_image's 8-bit path does); HDR and EXR stay linear.
+#include <zlib.h>
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pt_scene.h"
+
+namespace pupil {
+namespace {
+
+bool read_file(const std::string &path, std::vector<unsigned char> &out, std::string &err) {
+    FILE *fp = std::fopen(path.c_str(), "rb");
+    if (!fp) {
+        err = "io: cannot open image " + path;
+        return false;
+    }
+    std::fseek(fp, 0, SEEK_END);
+    long n = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    if (n < 0) {
+        std::fclose(fp);
+        err = "io: cannot size image " + path;
+        return false;
+    }
+    out.resize((size_t)n);
+    size_t rd = n ? std::fread(out.data(), 1, (size_t)n, fp) : 0;
+    std::fclose(fp);
+    if (rd != (size_t)n) {
+        err = "io: short read " + path;
+        return false;
+    }
+    return true;
+}
+
+std::string lower_ext(const std::string &path) {
+    size_t dot = path.find_last_of('.');
+    size_t slash = path.find_last_of("/\\");
+    if (dot == std::string::npos || (slash != std::string::npos && dot < slash)) return "";
+    std::string e = path.substr(dot + 1);
+    for (char &c : e) c = (char)std::tolower((unsigned char)c);
+    return e;
+}
+
+float half_to_float(uint16_t h) {
+    const uint32_t s = (uint32_t)(h >> 15) << 31;
+    const uint32_t e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+    uint32_t bits;
+    if (e == 0) {
+        if (m == 0) {
+            bits = s;
+        } else {  // subnormal: m * 2^-24
+            float f = (float)m * 5.9604644775390625e-8f;
+            std::memcpy(&bits, &f, 4);
+            bits |= s;
+        }
+    } else if (e == 31) {
+        bits = s | 0x7F800000u | (m << 13);
+    } else {
+        bits = s | ((e + 112u) << 23) | (m << 13);
+    }
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+
+// ------------------------------------------------------------------ OpenEXR
+struct ExrChannel {
+    std::string name;
+    int32_t type;  // 0 uint, 1 half, 2 float
+    int32_t xs, ys;
+};
+
+bool exr_undo_predictor_and_reorder(std::vector<unsigned char> &buf) {
+    for (size_t i = 1; i < buf.size(); ++i) buf[i] = (unsigned char)(buf[i - 1] + buf[i] - 128);
+    std::vector<unsigned char> out(buf.size());
+    const size_t half = (buf.size() + 1) / 2;
+    for (size_t i = 0, a = 0, b = half; i < buf.size(); ++i) out[i] = (i & 1) ? buf[b++] : buf[a++];
+    buf.swap(out);
+    return true;
+}
+
+bool exr_rle_decode(const unsigned char *in, size_t n, size_t expect, std::vector<unsigned char> &out) {
+    out.clear();
+    size_t i = 0;
+    while (i < n) {
+        const int8_t c = (int8_t)in[i++];
+        if (c < 0) {
+            const size_t run = (size_t)(-(int)c);
+            if (i + run > n) return false;
+            out.insert(out.end(), in + i, in + i + run);
+            i += run;
+        } else {
+            if (i >= n) return false;
+            out.insert(out.end(), (size_t)c + 1, in[i++]);
+        }
+        if (out.size() > expect) return false;
+    }
+    return out.size() == expect;
+}
+
+bool load_exr(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std::vector<float> &rgba,
+              std::string &err) {
+    size_t p = 0;
+    auto need = [&](size_t k) { return p + k <= f.size(); };
+    auto rd_i32 = [&](int32_t &v) {
+        if (!need(4)) return false;
+        std::memcpy(&v, &f[p], 4);
+        p += 4;
+        return true;
+    };
+    auto rd_str = [&](std::string &s) {
+        size_t e = p;
+        while (e < f.size() && f[e] != 0) ++e;
+        if (e >= f.size()) return false;
+        s.assign(reinterpret_cast<const char *>(&f[p]), e - p);
+        p = e + 1;
+        return true;
+    };
+    int32_t magic = 0, version = 0;
+    if (!rd_i32(magic) || magic != 20000630 || !rd_i32(version)) {
+        err = "parse: not an OpenEXR file";
+        return false;
+    }
+    // version flags: 0x200 tiled, 0x800 deep data, 0x1000 multipart
+    if ((version & 0xFF) != 2 || (version & 0x200) || (version & 0x800) || (version & 0x1000)) {
+        err = "unsupported: EXR files other than single-part scanline images";
+        return false;
+    }
+    std::vector<ExrChannel> chans;
+    int compression = -1, line_order = 0;
+    int32_t dw[4] = {0, 0, -1, -1};
+    for (;;) {
+        std::string name, type;
+        if (!rd_str(name)) {
+            err = "parse: truncated EXR header";
+            return false;
+        }
+        if (name.empty()) break;
+        int32_t size = 0;
+        if (!rd_str(type) || !rd_i32(size) || size < 0 || !need((size_t)size)) {
+            err = "parse: truncated EXR header";
+            return false;
+        }
+        const size_t end = p + (size_t)size;
+        if (name == "channels" && type == "chlist") {
+            while (p < end && f[p] != 0) {
+                ExrChannel c;
+                int32_t dummy;
+                if (!rd_str(c.name) || !rd_i32(c.type) || !rd_i32(dummy) || !rd_i32(c.xs) || !rd_i32(c.ys)) {
+                    err = "parse: bad EXR channel list";
+                    return false;
+                }
+                chans.push_back(c);
+            }
+        } else if (name == "compression") {
+            compression = f[p];
+        } else if (name == "dataWindow" && size == 16) {
+            std::memcpy(dw, &f[p], 16);
+        } else if (name == "lineOrder") {
+            line_order = f[p];
+        }
+        p = end;
+    }
+    (void)line_order;  // chunks carry their own y; the order does not matter here
+    if (dw[2] < dw[0] || dw[3] < dw[1] || chans.empty()) {
+        err = "parse: EXR without data window or channels";
+        return false;
+    }
+    const int64_t W = (int64_t)dw[2] - dw[0] + 1, H = (int64_t)dw[3] - dw[1] + 1;
+    if (W > 65536 || H > 65536) {
+        err = "unsupported: EXR image larger than 65536 pixels per side";
+        return false;
+    }
+    int lines = 0;
+    switch (compression) {
+        case 0: case 1: case 2: lines = 1; break;
+        case 3: lines = 16; break;
+        default:
+            err = "unsupported: EXR compression " + std::to_string(compression) + " (NONE, RLE, ZIPS and ZIP are read)";
+            return false;
+    }
+    size_t pix_bytes = 0;
+    for (auto &c : chans) {
+        if (c.xs != 1 || c.ys != 1 || c.type < 0 || c.type > 2) {
+            err = "unsupported: subsampled or unknown EXR channel " + c.name;
+            return false;
+        }
+        pix_bytes += c.type == 1 ? 2 : 4;
+    }
+    int ci[4] = {-1, -1, -1, -1};  // R G B A
+    int cy = -1;
+    for (size_t i = 0; i < chans.size(); ++i) {
+        const std::string &n = chans[i].name;
+        if (n == "R") ci[0] = (int)i;
+        else if (n == "G") ci[1] = (int)i;
+        else if (n == "B") ci[2] = (int)i;
+        else if (n == "A") ci[3] = (int)i;
+        else if (n == "Y") cy = (int)i;
+    }
+    if ((ci[0] < 0 || ci[1] < 0 || ci[2] < 0) && cy < 0) {
+        err = "unsupported: EXR without R, G, B (or Y) channels";
+        return false;
+    }
+    w = (uint32_t)W;
+    h = (uint32_t)H;
+    rgba.assign((size_t)W * H * 4, 0.f);
+    for (size_t i = 0; i < (size_t)W * H; ++i) rgba[4 * i + 3] = 1.f;
+    const int64_t chunks = (H + lines - 1) / lines;
+    if (!need((size_t)chunks * 8)) {
+        err = "parse: truncated EXR offset table";
+        return false;
+    }
+    std::vector<uint64_t> offs((size_t)chunks);
+    std::memcpy(offs.data(), &f[p], (size_t)chunks * 8);
+    std::vector<unsigned char> buf;
+    for (int64_t k = 0; k < chunks; ++k) {
+        p = (size_t)offs[(size_t)k];
+        int32_t y0 = 0, size = 0;
+        if (offs[(size_t)k] >= f.size() || !rd_i32(y0) || !rd_i32(size) || size < 0 || !need((size_t)size)) {
+            err = "parse: bad EXR chunk";
+            return false;
+        }
+        const int64_t ys = (int64_t)y0 - dw[1];
+        if (ys < 0 || ys >= H) {
+            err = "parse: EXR chunk outside the data window";
+            return false;
+        }
+        const int64_t ny = std::min<int64_t>(lines, H - ys);
+        const size_t expect = (size_t)ny * (size_t)W * pix_bytes;
+        const unsigned char *src = &f[p];
+        if ((size_t)size == expect) {  // stored uncompressed (always for NONE)
+            buf.assign(src, src + expect);
+        } else if (compression == 1) {
+            if (!exr_rle_decode(src, (size_t)size, expect, buf)) {
+                err = "parse: bad EXR RLE chunk";
+                return false;
+            }
+            exr_undo_predictor_and_reorder(buf);
+        } else if (compression == 2 || compression == 3) {
+            buf.resize(expect);
+            uLongf dl = (uLongf)expect;
+            if (uncompress(buf.data(), &dl, src, (uLong)size) != Z_OK || dl != expect) {
+                err = "parse: bad EXR ZIP chunk";
+                return false;
+            }
+            exr_undo_predictor_and_reorder(buf);
+        } else {
+            err = "parse: EXR chunk size mismatch";
+            return false;
+        }
+        size_t q = 0;
+        for (int64_t yy = 0; yy < ny; ++yy) {
+            float *row = &rgba[(size_t)(ys + yy) * W * 4];
+            for (size_t c = 0; c < chans.size(); ++c) {
+                int slot = -1;
+                for (int s = 0; s < 4; ++s)
+                    if (ci[s] == (int)c) slot = s;
+                const bool gray = (int)c == cy && (ci[0] < 0 || ci[1] < 0 || ci[2] < 0);
+                for (int64_t x = 0; x < W; ++x) {
+                    float v;
+                    if (chans[c].type == 1) {
+                        uint16_t hv;
+                        std::memcpy(&hv, &buf[q], 2);
+                        v = half_to_float(hv);
+                        q += 2;
+                    } else if (chans[c].type == 2) {
+                        std::memcpy(&v, &buf[q], 4);
+                        q += 4;
+                    } else {
+                        uint32_t u;
+                        std::memcpy(&u, &buf[q], 4);
+                        v = (float)u;
+                        q += 4;
+                    }
+                    if (slot >= 0) row[4 * x + slot] = v;
+                    if (gray) row[4 * x] = row[4 * x + 1] = row[4 * x + 2] = v;
+                }
+            }
+        }
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ Radiance RGBE
+bool load_hdr(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std::vector<float> &rgba,
+              std::string &err) {
+    size_t p = 0;
+    auto line = [&](std::string &s) {
+        s.clear();
+        while (p < f.size() && f[p] != '\n') s.push_back((char)f[p++]);
+        if (p >= f.size()) return false;
+        ++p;
+        return true;
+    };
+    std::string s;
+    if (!line(s) || (s.rfind("#?RADIANCE", 0) != 0 && s.rfind("#?RGBE", 0) != 0)) {
+        err = "parse: not a Radiance HDR file";
+        return false;
+    }
+    bool fmt_ok = true;
+    for (;;) {
+        if (!line(s)) {
+            err = "parse: truncated HDR header";
+            return false;
+        }
+        if (s.empty()) break;
+        if (s.rfind("FORMAT=", 0) == 0 && s != "FORMAT=32-bit_rle_rgbe") fmt_ok = false;
+    }
+    if (!fmt_ok) {
+        err = "unsupported: HDR pixel format other than 32-bit_rle_rgbe";
+        return false;
+    }
+    if (!line(s)) {
+        err = "parse: missing HDR resolution";
+        return false;
+    }
+    char ya[3] = {0}, xa[3] = {0};
+    int H = 0, W = 0;
+    if (std::sscanf(s.c_str(), "%2s %d %2s %d", ya, &H, xa, &W) != 4 || std::string(xa) != "+X" ||
+        (std::string(ya) != "-Y" && std::string(ya) != "+Y") || W <= 0 || H <= 0 || W > 65536 || H > 65536) {
+        err = "unsupported: HDR orientation other than -Y H +X W / +Y H +X W";
+        return false;
+    }
+    const bool bottom_up = std::string(ya) == "+Y";
+    w = (uint32_t)W;
+    h = (uint32_t)H;
+    rgba.assign((size_t)W * H * 4, 0.f);
+    std::vector<unsigned char> scan((size_t)W * 4);
+    for (int y = 0; y < H; ++y) {
+        if (p + 4 > f.size()) {
+            err = "parse: truncated HDR data";
+            return false;
+        }
+        const bool rle = W >= 8 && W < 32768 && f[p] == 2 && f[p + 1] == 2 && ((f[p + 2] << 8) | f[p + 3]) == W &&
+                         !(f[p + 2] & 0x80);
+        if (rle) {
+            p += 4;
+            for (int c = 0; c < 4; ++c) {
+                int x = 0;
+                while (x < W) {
+                    if (p >= f.size()) {
+                        err = "parse: truncated HDR run";
+                        return false;
+                    }
+                    int n = f[p++];
+                    if (n > 128) {
+                        n -= 128;
+                        if (x + n > W || p >= f.size()) {
+                            err = "parse: bad HDR run";
+                            return false;
+                        }
+                        const unsigned char v = f[p++];
+                        for (int k = 0; k < n; ++k) scan[4 * (size_t)(x++) + c] = v;
+                    } else {
+                        if (n == 0 || x + n > W || p + (size_t)n > f.size()) {
+                            err = "parse: bad HDR run";
+                            return false;
+                        }
+                        for (int k = 0; k < n; ++k) scan[4 * (size_t)(x++) + c] = f[p++];
+                    }
+                }
+            }
+        } else {
+            if (p + (size_t)W * 4 > f.size()) {
+                err = "parse: truncated HDR data";
+                return false;
+            }
+            std::memcpy(scan.data(), &f[p], (size_t)W * 4);
+            p += (size_t)W * 4;
+        }
+        float *row = &rgba[(size_t)(bottom_up ? H - 1 - y : y) * W * 4];
+        for (int x = 0; x < W; ++x) {
+            const unsigned char *e = &scan[4 * (size_t)x];
+            const float m = e[3] ? std::ldexp(1.f, (int)e[3] - 136) : 0.f;
+            row[4 * x + 0] = e[0] * m;
+            row[4 * x + 1] = e[1] * m;
+            row[4 * x + 2] = e[2] * m;
+            row[4 * x + 3] = 1.f;
+        }
+    }
+    return true;
+}
+
+// ------------------------------------------------------------------ PNG
+uint32_t be32(const unsigned char *p) { return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | p[3]; }
+
+bool load_png(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std::vector<float> &rgba,
+              std::string &err) {
+    static const unsigned char sig[8] = {0x89, 'P', 'N', 'G', 0x0D, 0x0A, 0x1A, 0x0A};
+    if (f.size() < 8 || std::memcmp(f.data(), sig, 8) != 0) {
+        err = "parse: not a PNG file";
+        return false;
+    }
+    size_t p = 8;
+    uint32_t W = 0, H = 0;
+    int depth = 0, ctype = -1, interlace = 0;
+    std::vector<unsigned char> idat, plte;
+    bool have_ihdr = false;
+    while (p + 8 <= f.size()) {
+        const uint32_t len = be32(&f[p]);
+        const std::string type(reinterpret_cast<const char *>(&f[p + 4]), 4);
+        p += 8;
+        if (p + (size_t)len + 4 > f.size()) {
+            err = "parse: truncated PNG chunk";
+            return false;
+        }
+        const unsigned char *d = &f[p];
+        if (type == "IHDR" && len >= 13) {
+            W = be32(d);
+            H = be32(d + 4);
+            depth = d[8];
+            ctype = d[9];
+            interlace = d[12];
+            have_ihdr = true;
+        } else if (type == "PLTE") {
+            plte.assign(d, d + len);
+        } else if (type == "IDAT") {
+            idat.insert(idat.end(), d, d + len);
+        } else if (type == "IEND") {
+            break;
+        }
+        p += (size_t)len + 4;  // data + CRC
+    }
+    if (!have_ihdr || W == 0 || H == 0 || W > 65536 || H > 65536) {
+        err = "parse: PNG without a valid IHDR";
+        return false;
+    }
+    if (interlace != 0) {
+        err = "unsupported: interlaced PNG";
+        return false;
+    }
+    int chans;
+    switch (ctype) {
+        case 0: chans = 1; break;
+        case 2: chans = 3; break;
+        case 3: chans = 1; break;
+        case 4: chans = 2; break;
+        case 6: chans = 4; break;
+        default:
+            err = "parse: bad PNG color type";
+            return false;
+    }
+    const bool depth_ok = depth == 8 || (depth == 16 && ctype != 3) || ((ctype == 0 || ctype == 3) && depth < 8 &&
+                                                                         (depth == 1 || depth == 2 || depth == 4));
+    if (!depth_ok || (ctype == 3 && plte.size() < 3)) {
+        err = "parse: bad PNG bit depth or palette";
+        return false;
+    }
+    const size_t bits_pp = (size_t)chans * depth;
+    const size_t row_bytes = ((size_t)W * bits_pp + 7) / 8;
+    const size_t bpp = std::max<size_t>(1, bits_pp / 8);
+    std::vector<unsigned char> raw((row_bytes + 1) * H);
+    uLongf dl = (uLongf)raw.size();
+    if (idat.empty() || uncompress(raw.data(), &dl, idat.data(), (uLong)idat.size()) != Z_OK || dl != raw.size()) {
+        err = "parse: bad PNG image data";
+        return false;
+    }
+    std::vector<unsigned char> prev(row_bytes, 0), cur(row_bytes);
+    w = W;
+    h = H;
+    rgba.assign((size_t)W * H * 4, 0.f);
+    float lut[256];
+    for (int i = 0; i < 256; ++i) lut[i] = std::pow(i * 1.f / 255.f, 2.2f);
+    for (uint32_t y = 0; y < H; ++y) {
+        const unsigned char *src = &raw[(row_bytes + 1) * y];
+        const int ft = src[0];
+        for (size_t i = 0; i < row_bytes; ++i) {
+            const int a = i >= bpp ? cur[i - bpp] : 0, b = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
+            int v = src[1 + i];
+            switch (ft) {
+                case 0: break;
+                case 1: v += a; break;
+                case 2: v += b; break;
+                case 3: v += (a + b) / 2; break;
+                case 4: {
+                    const int pp = a + b - c, pa = std::abs(pp - a), pb = std::abs(pp - b), pc = std::abs(pp - c);
+                    v += (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
+                } break;
+                default:
+                    err = "parse: bad PNG filter";
+                    return false;
+            }
+            cur[i] = (unsigned char)v;
+        }
+        float *row = &rgba[(size_t)y * W * 4];
+        for (uint32_t x = 0; x < W; ++x) {
+            unsigned char px[4] = {0, 0, 0, 255};
+            if (depth < 8) {
+                const size_t bit = (size_t)x * depth;
+                const int v = (cur[bit / 8] >> (8 - depth - (int)(bit % 8))) & ((1 << depth) - 1);
+                if (ctype == 3) {
+                    if ((size_t)v * 3 + 2 >= plte.size()) {
+                        err = "parse: PNG palette index out of range";
+                        return false;
+                    }
+                    for (int k = 0; k < 3; ++k) px[k] = plte[3 * v + k];
+                } else {
+                    const int scale = depth == 1 ? 255 : (depth == 2 ? 0x55 : 0x11);
+                    px[0] = px[1] = px[2] = (unsigned char)(v * scale);
+                }
+            } else {
+                const size_t step = depth / 8;
+                unsigned char s[4];
+                for (int k = 0; k < chans; ++k) s[k] = cur[((size_t)x * chans + k) * step];  // high byte
+                if (ctype == 3) {
+                    if ((size_t)s[0] * 3 + 2 >= plte.size()) {
+                        err = "parse: PNG palette index out of range";
+                        return false;
+                    }
+                    for (int k = 0; k < 3; ++k) px[k] = plte[3 * (size_t)s[0] + k];
+                } else if (chans <= 2) {
+                    px[0] = px[1] = px[2] = s[0];
+                    if (chans == 2) px[3] = s[1];
+                } else {
+                    for (int k = 0; k < chans; ++k) px[k] = s[k];
+                }
+            }
+            row[4 * x + 0] = lut[px[0]];
+            row[4 * x + 1] = lut[px[1]];
+            row[4 * x + 2] = lut[px[2]];
+            row[4 * x + 3] = px[3] * 1.f / 255.f;
+        }
+        prev.swap(cur);
+    }
+    return true;
+}
+
+}  // namespace
+
+bool load_image(const std::string &path, uint32_t &w, uint32_t &h, std::vector<float> &rgba, std::string &err) {
+    std::vector<unsigned char> f;
+    if (!read_file(path, f, err)) return false;
+    const std::string e = lower_ext(path);
+    bool ok;
+    if (e == "exr") ok = load_exr(f, w, h, rgba, err);
+    else if (e == "hdr" || e == "pic") ok = load_hdr(f, w, h, rgba, err);
+    else if (e == "png") ok = load_png(f, w, h, rgba, err);
+    else {
+        err = "unsupported: image format ." + e + " (EXR, HDR and PNG are read)";
+        return false;
+    }
+    if (!ok && err.find(path) == std::string::npos) err += " (" + path + ")";
+    return ok;
+}
+
+}  // namespace pupil

@@ -392,6 +392,8 @@ struct Loader {
     std::string dir;
     std::string err;
     std::map<std::string, std::shared_ptr<MeshData>> meshes;
+    std::map<std::string, std::shared_ptr<BitmapData>> images;
+    std::map<std::string, uint32_t> bitmap_index;
     // rectangle, cube and sphere shapes share one flip_normals flag per kind:
     // the last parsed instance wins (resource/shape.cpp:85-132, 441-497).
     std::vector<std::pair<uint32_t, int>> shared_flip_users;  // (shape index, 0 rect / 1 cube / 2 sphere)
@@ -454,22 +456,49 @@ struct Loader {
             load_float3(t->prop("color0"), c0, 0.4f, 0.4f, 0.4f);
             load_float3(t->prop("color1"), c1, 0.2f, 0.2f, 0.2f);
             tex = make_checker_texture(c0, c1);
-            if (Obj *tr = t->sub("transform")) {
-                if (tr->var_name == "to_uv") {
-                    float s[3];
-                    if (load_float3(tr->prop("scale"), s, 1.f, 1.f, 1.f)) {
-                        tex.uv_row0[0] = s[0];
-                        tex.uv_row1[1] = s[1];
-                    }
-                }
-            }
+        } else if (t->type == "bitmap") {
+            // resource/scene.cpp:144-165: filter "bilinear" or point (also when
+            // absent), wrap "repeat" (default), "mirror" or "clamp".
+            const uint32_t filter = t->prop("filter_type") == "bilinear" ? PT_FILTER_LINEAR : PT_FILTER_POINT;
+            const std::string wm = t->prop("wrap_mode");
+            const uint32_t wrap = wm == "mirror" ? PT_WRAP_MIRROR : (wm == "clamp" ? PT_WRAP_CLAMP : PT_WRAP_REPEAT);
+            uint32_t index;
+            if (!bitmap(t->prop("filename"), filter, wrap, index)) return false;
+            tex = make_bitmap_texture(index);
+        } else {
+            tex = make_rgb_texture(0.f, 0.f, 0.f);  // unknown texture type reads black
             return true;
         }
-        if (t->type == "bitmap") {
-            err = "unsupported: bitmap textures";
-            return false;
+        // Texture transform (Texture::Sample applies rows 0 and 1 to (u, v, 0, 1)).
+        if (Obj *tr = t->sub("transform")) {
+            bool la = false;
+            const Mat4 m = load_transform(tr, la);
+            for (int k = 0; k < 4; ++k) {
+                tex.uv_row0[k] = m.e[k];
+                tex.uv_row1[k] = m.e[4 + k];
+            }
         }
-        tex = make_rgb_texture(0.f, 0.f, 0.f);  // unknown texture type reads black
+        return true;
+    }
+
+    // util::BitmapTexture loads through the texture manager's cache
+    // (resource/texture.cpp): one image per file, one bitmap per sampling mode.
+    bool bitmap(const std::string &fn, uint32_t filter, uint32_t wrap, uint32_t &index) {
+        const std::string path = fn.size() && fn[0] == '/' ? fn : dir + "/" + fn;
+        const std::string key = path + "|" + std::to_string(filter) + "|" + std::to_string(wrap);
+        auto it = bitmap_index.find(key);
+        if (it != bitmap_index.end()) {
+            index = it->second;
+            return true;
+        }
+        auto img = images.find(path);
+        if (img == images.end()) {
+            auto data = std::make_shared<BitmapData>();
+            if (!load_image(path, data->width, data->height, data->rgba, err)) return false;
+            img = images.emplace(path, data).first;
+        }
+        index = scene.add_bitmap(img->second, filter, wrap);
+        bitmap_index[key] = index;
         return true;
     }
 
@@ -653,8 +682,19 @@ struct Loader {
                     scene.env_type = PT_ENV_CONSTANT;
                     for (int k = 0; k < 3; ++k) scene.env_radiance[k] = c[k];
                 } else if (o->type == "envmap") {
-                    err = "unsupported: envmap emitters";
-                    return false;
+                    // resource/scene.cpp:209-222: bilinear, repeat; scale; the
+                    // 3x3 part of to_world orients the map.
+                    uint32_t index;
+                    if (!bitmap(o->prop("filename"), PT_FILTER_LINEAR, PT_WRAP_REPEAT, index)) return false;
+                    float sc;
+                    load_float(o, "scale", sc, 1.f);
+                    bool la = false;
+                    const Mat4 m = load_transform(o->sub("transform"), la);
+                    scene.env_type = PT_ENV_MAP;
+                    scene.env_bitmap = index;
+                    scene.env_scale = sc;
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) scene.env_to_world[3 * r + c] = m.e[4 * r + c];
                 }
             }
         }
@@ -696,7 +736,7 @@ bool load_xml_scene(const std::string &path, HostScene &out, std::string &err) {
     ObjTree tree;
     tree.visit(*dom, nullptr);
     out = HostScene();
-    Loader ld{tree, out, "", "", {}, {}, false, false};
+    Loader ld{tree, out, "", ""};
     size_t slash = path.find_last_of('/');
     ld.dir = slash == std::string::npos ? "." : path.substr(0, slash);
     if (!ld.run()) {

@@ -174,6 +174,12 @@ pt_texture_desc make_rgb_texture(float r, float g, float b) {
     t.uv_row1[1] = 1.f;
     return t;
 }
+pt_texture_desc make_bitmap_texture(uint32_t bitmap) {
+    pt_texture_desc t = make_rgb_texture(0.f, 0.f, 0.f);
+    t.type = PT_TEX_BITMAP;
+    t.bitmap = bitmap;
+    return t;
+}
 pt_texture_desc make_checker_texture(const float c0[3], const float c1[3]) {
     pt_texture_desc t = make_rgb_texture(c0[0], c0[1], c0[2]);
     t.type = PT_TEX_CHECKER;
@@ -324,6 +330,19 @@ uint32_t HostScene::add_instance(const MeshData &mesh, const Mat4 &to_world, uin
     return (uint32_t)shapes.size() - 1;
 }
 
+uint32_t HostScene::add_bitmap(std::shared_ptr<BitmapData> data, uint32_t filter, uint32_t wrap) {
+    pt_bitmap_desc b;
+    std::memset(&b, 0, sizeof(b));
+    b.width = data->width;
+    b.height = data->height;
+    b.filter = filter;
+    b.wrap = wrap;
+    b.rgba = data->rgba.data();  // owned by the shared image; the buffer does not move
+    bitmap_data.push_back(std::move(data));
+    bitmaps.push_back(b);
+    return (uint32_t)bitmaps.size() - 1;
+}
+
 uint32_t HostScene::add_sphere(const Mat4 &to_world, uint32_t material, uint32_t flags, bool is_emitter,
                                const pt_texture_desc *radiance) {
     pt_sphere_desc sp;
@@ -369,6 +388,11 @@ pt_scene_desc HostScene::desc() const {
     d.max_depth = max_depth;
     d.num_spheres = (uint32_t)spheres.size();
     d.spheres = spheres.empty() ? nullptr : spheres.data();
+    d.num_bitmaps = (uint32_t)bitmaps.size();
+    d.bitmaps = bitmaps.empty() ? nullptr : bitmaps.data();
+    d.env_bitmap = env_bitmap;
+    d.env_scale = env_scale;
+    std::memcpy(d.env_to_world, env_to_world, sizeof(env_to_world));
     return d;
 }
 

@@ -7,6 +7,7 @@
 
 #include <stdint.h>
 
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -47,6 +48,13 @@ struct MeshData {
 };
 
 pt_texture_desc make_rgb_texture(float r, float g, float b);
+pt_texture_desc make_bitmap_texture(uint32_t bitmap);
+
+// Decoded image shared by the bitmaps that sample it in different modes.
+struct BitmapData {
+    uint32_t width = 0, height = 0;
+    std::vector<float> rgba;
+};
 pt_texture_desc make_checker_texture(const float c0[3], const float c1[3]);
 pt_material_desc make_diffuse(float r, float g, float b, bool twosided);
 
@@ -57,8 +65,13 @@ public:
     std::vector<pt_shape_desc> shapes;
     std::vector<pt_material_desc> materials;
     std::vector<pt_sphere_desc> spheres;
+    std::vector<std::shared_ptr<BitmapData>> bitmap_data;  // per bitmap
+    std::vector<pt_bitmap_desc> bitmaps;                   // rgba points into bitmap_data
     uint32_t env_type = PT_ENV_NONE;
     float env_radiance[3] = {0.f, 0.f, 0.f};
+    uint32_t env_bitmap = 0;
+    float env_scale = 1.f;
+    float env_to_world[9] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f};
     pt_camera_desc camera{};
     uint32_t width = 768, height = 576, max_depth = 1;
     bool any_normals = false, any_texcoords = false;
@@ -72,6 +85,7 @@ public:
     // (only PT_SHAPE_FLIP_NORMALS of `flags` applies).  Returns the shape index.
     uint32_t add_sphere(const Mat4 &to_world, uint32_t material, uint32_t flags, bool is_emitter,
                         const pt_texture_desc *radiance);
+    uint32_t add_bitmap(std::shared_ptr<BitmapData> data, uint32_t filter, uint32_t wrap);
     void set_camera(float fov_y_deg, const Mat4 &camera_to_world);
     pt_scene_desc desc() const;
 };
@@ -83,6 +97,9 @@ MeshData make_icosphere_mesh(int subdivisions);
 bool generate_scene(int kind, const pt_gen_params &params, HostScene &out, std::string &err);
 bool load_xml_scene(const std::string &path, HostScene &out, std::string &err);
 bool load_obj_mesh(const std::string &path, MeshData &mesh, std::string &err);
+// util::BitmapTexture::Load: EXR / Radiance HDR / PNG as RGBA float, row 0 =
+// top row; `err` starts with "io:" for file errors.
+bool load_image(const std::string &path, uint32_t &w, uint32_t &h, std::vector<float> &rgba, std::string &err);
 bool write_exr_rgb(const std::string &path, const float *rgba, uint32_t width, uint32_t height, std::string &err);
 
 // Dielectric IOR names (Hecht, Optics, 4th ed.) and conductor eta/k tables

@@ -40,6 +40,7 @@ struct SceneView {
     const float *emitter_cdf;
     uint32_t num_emitters;
     uint32_t pad;
+    TexArrays tex;              // bitmaps (may be null without bitmap textures)
     EnvRec env;
 };
 
@@ -162,10 +163,21 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
     if (prim == kMissPrim) {
         // __miss__default (main.cu:199-215) and the escape branch (main.cu:168-172).
         f3 env_rad = mk3(0.f);
-        float env_pdf = 0.f;
-        if (s.env.type != kEmitterNone) {
+        float env_pdf = 0.f;    // the environment's Eval pdf (reference weighting)
+        float light_pdf = 0.f;  // selection x next-event density (consistent weighting)
+        if (s.env.type == kEmitterConstEnv) {
             env_rad = mk3(s.env.color.x, s.env.color.y, s.env.color.z);
             env_pdf = 0.25f * kInvPi;
+            light_pdf = (ps.flags & kPathEnvUpper) ? s.env.select_prob * (kInvPi * 0.5f) : 0.f;
+        } else if (s.env.type == kEmitterEnvMap) {
+            if (consistent) {
+                float p;
+                env_map_eval(s.env, ps.d, true, s.tex, env_rad, p);
+                light_pdf = s.env.select_prob * p;
+            } else {
+                // The miss program evaluates the direction (o + d) - o.
+                env_map_eval(s.env, normalize((ps.o + ps.d) - ps.o), false, s.tex, env_rad, env_pdf);
+            }
         }
         if (primary) {
             if (prm.aov) {
@@ -177,7 +189,6 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
         } else {
             float mis;
             if (consistent) {
-                float light_pdf = (ps.flags & kPathEnvUpper) ? s.env.select_prob * (kInvPi * 0.5f) : 0.f;
                 mis = (ps.flags & kPathPrevDelta) ? 1.f : mis_weight(ps.prev_pdf, light_pdf);
             } else {
                 mis = mis_weight(ps.prev_pdf, env_pdf);
@@ -194,10 +205,10 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
     int32_t emitter_index;
     hit_geometry(s, prim, t_hit, b1, b2, ps.o, ps.d, g, mat_index, emitter_index);
     const MaterialRec &mat = s.materials[mat_index];
-    const LocalBsdf bsdf = material_local_bsdf(mat, g.texcoord);
+    const LocalBsdf bsdf = material_local_bsdf(mat, g.texcoord, s.tex);
 
     if (primary) {
-        if (emitter_index >= 0) ps.L = ps.L + texture_sample(s.emitters[emitter_index].radiance, g.texcoord);
+        if (emitter_index >= 0) ps.L = ps.L + texture_sample(s.emitters[emitter_index].radiance, g.texcoord, s.tex);
         if (prm.aov) {
             out.wrote_aov = true;
             out.albedo = bsdf.c0;
@@ -208,7 +219,7 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
         const EmitterRec &e = s.emitters[emitter_index];
         f3 rad;
         float epdf;
-        area_eval(e, g, ps.o, rad, epdf);
+        area_eval(e, g, ps.o, rad, epdf, s.tex);
         if (!is_zero(epdf)) {
             // Consistent mode weighs against the pdf next-event estimation
             // would assign to this point, which uses the emitter's own normals.
@@ -241,10 +252,10 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
         es.pdf = 0.f;
         float sel_prob = 0.f;
         if (sel >= 0) {
-            emitter_sample_direct(s.emitters[sel], g, mk2(x, y), es);
+            emitter_sample_direct(s.emitters[sel], g, mk2(x, y), es, s.tex);
             sel_prob = s.emitters[sel].select_prob;
         } else if (sel == -1) {
-            const_env_sample_direct(s.env, g, mk2(x, y), es);
+            env_sample_direct(s.env, g, mk2(x, y), consistent, es, s.tex);
             sel_prob = s.env.select_prob;
         }
         if (es.pdf != 0.f) {

@@ -74,7 +74,15 @@ PT_HD bool guided_begin(const SceneView &s, const ShadeParams &prm, PathState &p
 
     if (prim == kMissPrim) {
         f3 env_rad = mk3(0.f);
-        if (s.env.type != kEmitterNone) env_rad = mk3(s.env.color.x, s.env.color.y, s.env.color.z);
+        float light_pdf = 0.f;
+        if (s.env.type == kEmitterConstEnv) {
+            env_rad = mk3(s.env.color.x, s.env.color.y, s.env.color.z);
+            light_pdf = (ps.flags & kPathEnvUpper) ? s.env.select_prob * (kInvPi * 0.5f) : 0.f;
+        } else if (s.env.type == kEmitterEnvMap) {
+            float p;
+            env_map_eval(s.env, ps.d, true, s.tex, env_rad, p);
+            light_pdf = s.env.select_prob * p;
+        }
         if (primary) {
             if (prm.aov) {
                 out.wrote_aov = true;
@@ -84,7 +92,6 @@ PT_HD bool guided_begin(const SceneView &s, const ShadeParams &prm, PathState &p
             out.test = rng_next(ps.rng);
             ps.L = ps.L + env_rad;
         } else {
-            const float light_pdf = (ps.flags & kPathEnvUpper) ? s.env.select_prob * (kInvPi * 0.5f) : 0.f;
             const float mis = (ps.flags & kPathPrevDelta) ? 1.f : mis_weight(ps.prev_pdf, light_pdf);
             ps.L = ps.L + env_rad * (ps.T * mis);
             if (train) complete_record(rs, path, ps, env_rad);
@@ -97,12 +104,13 @@ PT_HD bool guided_begin(const SceneView &s, const ShadeParams &prm, PathState &p
     int32_t emitter_index;
     hit_geometry(s, prim, t_hit, b1, b2, ps.o, ps.d, v.g, mat_index, emitter_index);
     const MaterialRec &mat = s.materials[mat_index];
-    v.bsdf = material_local_bsdf(mat, v.g.texcoord);
+    v.bsdf = material_local_bsdf(mat, v.g.texcoord, s.tex);
     v.guidable = guidable_material(v.bsdf.type);
 
     f3 le = mk3(0.f);
     if (primary) {
-        if (emitter_index >= 0) ps.L = ps.L + texture_sample(s.emitters[emitter_index].radiance, v.g.texcoord);
+        if (emitter_index >= 0)
+            ps.L = ps.L + texture_sample(s.emitters[emitter_index].radiance, v.g.texcoord, s.tex);
         if (prm.aov) {
             out.wrote_aov = true;
             out.albedo = v.bsdf.c0;
@@ -113,7 +121,7 @@ PT_HD bool guided_begin(const SceneView &s, const ShadeParams &prm, PathState &p
         const EmitterRec &e = s.emitters[emitter_index];
         f3 rad;
         float epdf;
-        area_eval(e, v.g, ps.o, rad, epdf);
+        area_eval(e, v.g, ps.o, rad, epdf, s.tex);
         if (!is_zero(epdf)) {
             const float lpdf = emitter_nee_pdf(e, v.g.position, b1, b2, ps.o);
             const float mis = (ps.flags & kPathPrevDelta) ? 1.f : mis_weight(ps.prev_pdf, lpdf * e.select_prob);
@@ -156,10 +164,10 @@ PT_HD void guided_scatter(const SceneView &s, PathState &ps, const GuidedVertex 
         es.pdf = 0.f;
         float sel_prob = 0.f;
         if (sel >= 0) {
-            emitter_sample_direct(s.emitters[sel], g, mk2(x, y), es);
+            emitter_sample_direct(s.emitters[sel], g, mk2(x, y), es, s.tex);
             sel_prob = s.emitters[sel].select_prob;
         } else if (sel == -1) {
-            const_env_sample_direct(s.env, g, mk2(x, y), es);
+            env_sample_direct(s.env, g, mk2(x, y), true, es, s.tex);
             sel_prob = s.env.select_prob;
         }
         if (es.pdf != 0.f) {

@@ -37,8 +37,9 @@ def test_struct_layouts_match_header():
     assert pt.SceneDesc.width.offset == 208
     assert pt.SceneDesc.num_spheres.offset == 220
     assert pt.SceneDesc.spheres.offset == 224
-    assert C.sizeof(pt.SceneDesc) == 232
-    assert C.sizeof(pt.SphereDesc) == 48
+    assert pt.SceneDesc.bitmaps.offset == 240 and pt.SceneDesc.env_to_world.offset == 256
+    assert C.sizeof(pt.SceneDesc) == 296
+    assert C.sizeof(pt.SphereDesc) == 48 and C.sizeof(pt.BitmapDesc) == 24
 
 
 def test_create_without_device_fails_cleanly():
