@@ -15,6 +15,8 @@
 // accept.
 #pragma once
 
+#include <cstring>
+
 #include "pt_math.h"
 
 namespace pupil {
@@ -156,6 +158,14 @@ PT_HD bool intersect_sphere(f3 o, f3 d, f3 c, float r, float tmin, float tmax, f
 }
 
 PT_HD float exp2_from_biased(uint8_t e) { return u2f(((uint32_t)e) << 23); }
+PT_HD float qbyte(const uint32_t *w, int k) { return (float)((w[k >> 2] >> (8 * (k & 3))) & 0xFFu); }
+PT_HD void swap_words(uint32_t *a, uint32_t *b) {
+    for (int i = 0; i < 2; ++i) {
+        const uint32_t t = a[i];
+        a[i] = b[i];
+        b[i] = t;
+    }
+}
 
 struct TraverseStats {
     uint32_t nodes;
@@ -178,6 +188,7 @@ PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float 
     const float idy = 1.f / (fabsf(d.y) > 1e-12f ? d.y : copysignf(1e-12f, d.y));
     const float idz = 1.f / (fabsf(d.z) > 1e-12f ? d.z : copysignf(1e-12f, d.z));
     const uint32_t oct = (d.x < 0.f ? 1u : 0u) | (d.y < 0.f ? 2u : 0u) | (d.z < 0.f ? 4u : 0u);
+    const bool neg_x = idx < 0.f, neg_y = idy < 0.f, neg_z = idz < 0.f;  // sign of each slab slope
 
     hit.t = tmax;
     hit.prim = 0xFFFFFFFFu;
@@ -204,18 +215,33 @@ PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float 
             // b = (origin - o) / d per axis, plane q is at t = q a + b (one FMA).
             // Rounding differs from decoding the box first by a few ulps, which
             // the relative node margin covers; triangle tests stay exact.
+            // The near plane of an axis is the low bound when the slope a is
+            // positive and the high bound otherwise (fma is monotone in q), so
+            // the byte arrays are swapped once per node instead of taking a
+            // min and a max per child (Ylitie et al. 2017); the distances are
+            // the same as min(t0, t1) / max(t0, t1).
             const float ax = exp2_from_biased(n.ex) * idx, bx = (n.px - o.x) * idx;
             const float ay = exp2_from_biased(n.ey) * idy, by = (n.py - o.y) * idy;
             const float az = exp2_from_biased(n.ez) * idz, bz = (n.pz - o.z) * idz;
+            uint32_t nqx[2], fqx[2], nqy[2], fqy[2], nqz[2], fqz[2];
+            std::memcpy(nqx, n.qlo_x, 8);
+            std::memcpy(fqx, n.qhi_x, 8);
+            std::memcpy(nqy, n.qlo_y, 8);
+            std::memcpy(fqy, n.qhi_y, 8);
+            std::memcpy(nqz, n.qlo_z, 8);
+            std::memcpy(fqz, n.qhi_z, 8);
+            if (neg_x) swap_words(nqx, fqx);
+            if (neg_y) swap_words(nqy, fqy);
+            if (neg_z) swap_words(nqz, fqz);
             uint32_t new_bits = 0, tri_bits = 0;
             for (int k = 0; k < 8; ++k) {
                 const uint32_t meta = n.meta[k];
                 if (meta == 0) continue;
-                const float tx0 = fmaf((float)n.qlo_x[k], ax, bx), tx1 = fmaf((float)n.qhi_x[k], ax, bx);
-                const float ty0 = fmaf((float)n.qlo_y[k], ay, by), ty1 = fmaf((float)n.qhi_y[k], ay, by);
-                const float tz0 = fmaf((float)n.qlo_z[k], az, bz), tz1 = fmaf((float)n.qhi_z[k], az, bz);
-                float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
-                float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tbest));
+                const float txn = fmaf(qbyte(nqx, k), ax, bx), txf = fmaf(qbyte(fqx, k), ax, bx);
+                const float tyn = fmaf(qbyte(nqy, k), ay, by), tyf = fmaf(qbyte(fqy, k), ay, by);
+                const float tzn = fmaf(qbyte(nqz, k), az, bz), tzf = fmaf(qbyte(fqz, k), az, bz);
+                float tn = fmaxf(fmaxf(txn, tyn), fmaxf(tzn, tmin));
+                float tf = fminf(fminf(txf, tyf), fminf(tzf, tbest));
                 if (tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin)) {
                     if (n.imask & (1u << k)) {
                         new_bits |= 1u << (24 + (k ^ oct));

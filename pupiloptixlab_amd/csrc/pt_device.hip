@@ -770,7 +770,8 @@ struct pt_ctx {
     hipStream_t stream = nullptr;
     std::string error;
     int num_cu = 256;
-    uint32_t trace_blocks_per_cu = 8;
+    uint32_t trace_blocks_per_cu = 8;      // resident traversal blocks per CU, sphere variant
+    uint32_t trace_blocks_per_cu_tri = 8;  // triangle-only variant
 
     // scene (host copies kept for the BVH build)
     bool has_scene = false;
@@ -1009,11 +1010,14 @@ int ensure_queues(pt_ctx *ctx, uint32_t paths, uint32_t max_depth) {
     return PT_OK;
 }
 
-uint32_t trace_grid(pt_ctx *ctx) { return (uint32_t)ctx->num_cu * ctx->trace_blocks_per_cu; }
+uint32_t trace_grid(pt_ctx *ctx) {
+    return (uint32_t)ctx->num_cu * (ctx->num_spheres > 0 ? ctx->trace_blocks_per_cu : ctx->trace_blocks_per_cu_tri);
+}
 uint32_t shade_grid(pt_ctx *ctx) { return (uint32_t)ctx->num_cu * 8u; }
 
 int ensure_spill(pt_ctx *ctx) {
-    uint32_t threads = trace_grid(ctx) * kTraceBlock;
+    const uint32_t threads =
+        (uint32_t)ctx->num_cu * std::max(ctx->trace_blocks_per_cu, ctx->trace_blocks_per_cu_tri) * kTraceBlock;
     if (ctx->d_spill && ctx->spill_threads >= threads) return PT_OK;
     dfree(ctx->d_spill);
     PT_HIP(hipMalloc((void **)&ctx->d_spill, (size_t)threads * kSpillEntries * sizeof(uint2)));
@@ -1815,6 +1819,10 @@ int pt_create(int hip_device, pt_ctx **out) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace<false, true>, kTraceBlock, 0) == hipSuccess &&
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false, true>, kTraceBlock, 0) == hipSuccess)
             ctx->trace_blocks_per_cu = (uint32_t)std::max(1, std::min(a, b));
+        a = b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace<false, false>, kTraceBlock, 0) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false, false>, kTraceBlock, 0) == hipSuccess)
+            ctx->trace_blocks_per_cu_tri = (uint32_t)std::max(1, std::min(a, b));
         int c = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_guide_infer, kInferBlock, 0) == hipSuccess && c > 0)
             ctx->infer_blocks_per_cu = (uint32_t)c;
