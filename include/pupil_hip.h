@@ -227,6 +227,8 @@ typedef struct pt_stats {
     uint64_t guide_samples;   /* training samples collected by pt_render */
     double ms_guide_infer;    /* event time of the network inference kernels, with timing */
     uint64_t guide_queries;   /* network evaluations made for sampling */
+    double ms_refit;          /* wall time of BVH refits after shape updates */
+    uint64_t refits;
 } pt_stats;
 
 typedef struct pt_ctx pt_ctx;
@@ -248,6 +250,20 @@ int pt_build_bvh(pt_ctx *ctx, const pt_bvh_opts *opts);
 int pt_set_camera(pt_ctx *ctx, const float sample_to_camera[16], const float camera_to_world[16]);
 /* m_dirty restart of PTPass::OnRun (pt_pass.cpp:40-49). */
 int pt_reset_accumulation(pt_ctx *ctx);
+/* RenderObject::UpdateTransform (world/render_object.cpp:72-75) with the
+ * world's RenderInstanceTransform / RenderInstanceUpdate handling
+ * (world/world.cpp:45-59): the shape's geometry becomes `transform` (3x4
+ * rows, affine) applied to the geometry the shape had in pt_set_scene; its
+ * area emitters and the emitter selection table are rebuilt
+ * (EmitterHelper::ResetAreaEmitter + ComputeProbability) and accumulation
+ * restarts (pt_pass.cpp:242-249).  The BVH keeps its topology and is refit
+ * on the GPU before the next render or trace, as IASManager::GetIASHandle
+ * updates a dirty IAS (world/ias_manager.cpp:187-205).  Triangle shapes must
+ * not share vertices with other shapes; spheres must stay similarity
+ * transforms (PT_ERR_UNSUPPORTED otherwise; nothing changes then). */
+int pt_set_shape_transform(pt_ctx *ctx, uint32_t shape, const float transform[12]);
+/* Refits the BVH now if shape updates are pending (otherwise no-op). */
+int pt_refit_bvh(pt_ctx *ctx);
 /* PTPass::OnRun x spp (pt_pass.cpp:39-57) + optix::Pass::Run
  * (optix/pass.h:71-89): renders `spp` frames; blocks until done. */
 int pt_render(pt_ctx *ctx, const pt_render_params *params);

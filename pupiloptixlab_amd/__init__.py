@@ -107,7 +107,8 @@ class Stats(C.Structure):
                 ("launches_shade", C.c_uint64), ("launches_shadow", C.c_uint64), ("bvh_nodes", C.c_uint32),
                 ("bvh_leaf_tris", C.c_uint32), ("bvh_depth", C.c_uint32), ("bvh_build_ms", C.c_double),
                 ("ms_guide", C.c_double), ("guide_steps", C.c_uint64), ("guide_samples", C.c_uint64),
-                ("ms_guide_infer", C.c_double), ("guide_queries", C.c_uint64)]
+                ("ms_guide_infer", C.c_double), ("guide_queries", C.c_uint64), ("ms_refit", C.c_double),
+                ("refits", C.c_uint64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -172,6 +173,8 @@ _SIGNATURES = {
     "pt_build_bvh": (C.c_int, [C.c_void_p, C.POINTER(BvhOpts)]),
     "pt_set_camera": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "pt_reset_accumulation": (C.c_int, [C.c_void_p]),
+    "pt_set_shape_transform": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "pt_refit_bvh": (C.c_int, [C.c_void_p]),
     "pt_render": (C.c_int, [C.c_void_p, C.POINTER(RenderParams)]),
     "pt_read_buffer": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "pt_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
@@ -394,6 +397,15 @@ class PathTracer:
 
     def reset(self) -> None:
         _check(lib().pt_reset_accumulation(self._ctx), "pt_reset_accumulation", self._ctx)
+
+    def set_shape_transform(self, shape: int, transform) -> None:
+        """RenderObject::UpdateTransform: the shape's pt_set_scene geometry under
+        `transform` (3x4, or 4x4 whose last row is ignored)."""
+        m = np.ascontiguousarray(np.asarray(transform, np.float32).reshape(-1)[:12])
+        _check(lib().pt_set_shape_transform(self._ctx, shape, m.ctypes.data), "pt_set_shape_transform", self._ctx)
+
+    def refit_bvh(self) -> None:
+        _check(lib().pt_refit_bvh(self._ctx), "pt_refit_bvh", self._ctx)
 
     def render(self, spp: int = 1, max_depth: int = 0, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE,
                accumulate: bool = True, aov: bool = True, fp64: bool = False, timing: bool = False,

@@ -167,6 +167,53 @@ PT_HD void swap_words(uint32_t *a, uint32_t *b) {
     }
 }
 
+// Exponent n with x = m 2^n, m in [0.5, 1), for positive finite x (frexp);
+// subnormal x reports -1022.
+PT_HD int frexp_exponent(double x) {
+    union { double d; uint64_t u; } v;
+    v.d = x;
+    const int e = (int)((v.u >> 52) & 0x7FFu);
+    return e == 0 ? -1022 : e - 1022;
+}
+
+// Quantization of one axis of a wide node: the smallest power-of-two step
+// 2^e (e >= -126) for which 8-bit codes exist whose decode p + q 2^e (in
+// float) contains every child interval [clo, chi].  Returns false when no
+// exponent up to 127 works.  The same rule serves the host builder and the
+// GPU refit.
+PT_HD bool quantize_node_axis(float p, float node_hi, const float *clo, const float *chi, int n, uint8_t &e_out,
+                              uint8_t *qlo, uint8_t *qhi) {
+    const double extent = (double)node_hi - (double)p;
+    int e = -126;
+    if (extent > 0.0) {
+        const int ex = frexp_exponent(extent / 255.0);
+        e = ex > -126 ? ex : -126;
+    }
+    for (; e <= 127; ++e) {
+        const float sc = u2f((uint32_t)(e + 127) << 23);
+        bool ok = true;
+        for (int i = 0; i < n && ok; ++i) {
+            const double lo = floor(((double)clo[i] - (double)p) / (double)sc);
+            const double hi = ceil(((double)chi[i] - (double)p) / (double)sc);
+            int ql = (int)fmax(0.0, fmin(255.0, lo));
+            int qh = (int)fmax(0.0, fmin(255.0, hi));
+            while (ql > 0 && !(p + (float)ql * sc <= clo[i])) --ql;
+            if (!(p + (float)ql * sc <= clo[i])) ok = false;
+            while (ok && qh < 255 && !(p + (float)qh * sc >= chi[i])) ++qh;
+            if (ok && !(p + (float)qh * sc >= chi[i])) ok = false;
+            if (ok) {
+                qlo[i] = (uint8_t)ql;
+                qhi[i] = (uint8_t)qh;
+            }
+        }
+        if (ok) {
+            e_out = (uint8_t)(e + 127);
+            return true;
+        }
+    }
+    return false;
+}
+
 struct TraverseStats {
     uint32_t nodes;
     uint32_t tris;

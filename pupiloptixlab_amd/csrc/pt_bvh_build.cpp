@@ -178,43 +178,6 @@ public:
     double sah = 0.0;
 
 private:
-    static bool decode_le(float p, uint32_t q, float s, float v) { return p + (float)q * s <= v; }
-    static bool decode_ge(float p, uint32_t q, float s, float v) { return p + (float)q * s >= v; }
-
-    // Chooses exponent and per-child codes so the float decode used by the
-    // traversal contains every child box.
-    static bool quantize_axis(float p, float node_hi, const float *clo, const float *chi, int n, uint8_t &e_out,
-                              uint8_t *qlo, uint8_t *qhi) {
-        double extent = (double)node_hi - (double)p;
-        int e = -126;
-        if (extent > 0.0) {
-            int ex;
-            std::frexp(extent / 255.0, &ex);  // extent/255 < 2^ex
-            e = std::max(ex, -126);
-        }
-        for (; e <= 127; ++e) {
-            float s = std::ldexp(1.0f, e);
-            bool ok = true;
-            for (int i = 0; i < n && ok; ++i) {
-                double lo = std::floor(((double)clo[i] - (double)p) / (double)s);
-                double hi = std::ceil(((double)chi[i] - (double)p) / (double)s);
-                int ql = (int)std::max(0.0, std::min(255.0, lo));
-                int qh = (int)std::max(0.0, std::min(255.0, hi));
-                while (ql > 0 && !decode_le(p, (uint32_t)ql, s, clo[i])) --ql;
-                if (!decode_le(p, (uint32_t)ql, s, clo[i])) { ok = false; break; }
-                while (qh < 255 && !decode_ge(p, (uint32_t)qh, s, chi[i])) ++qh;
-                if (!decode_ge(p, (uint32_t)qh, s, chi[i])) { ok = false; break; }
-                qlo[i] = (uint8_t)ql;
-                qhi[i] = (uint8_t)qh;
-            }
-            if (ok) {
-                e_out = (uint8_t)(e + 127);
-                return true;
-            }
-        }
-        return false;
-    }
-
     bool emit(uint32_t bi, uint32_t wi, uint32_t depth, std::string &err) {
         max_depth = std::max(max_depth, depth);
         if (depth > kMaxBvhDepth) {
@@ -356,7 +319,7 @@ private:
         const float np[3] = {n.px, n.py, n.pz};
         uint8_t qlo[3][8], qhi[3][8], e[3];
         for (int a = 0; a < 3; ++a) {
-            if (!quantize_axis(np[a], nhi[a], clo[a], chi[a], nslots, e[a], qlo[a], qhi[a])) {
+            if (!quantize_node_axis(np[a], nhi[a], clo[a], chi[a], nslots, e[a], qlo[a], qhi[a])) {
                 err = "failed to quantize a BVH node";
                 return false;
             }

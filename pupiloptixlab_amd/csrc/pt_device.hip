@@ -707,6 +707,114 @@ TraceRaysKernel trace_rays_kernel(bool any_hit, bool sph) {
                    : (sph ? k_trace_rays<false, true> : k_trace_rays<false, false>);
 }
 
+// --- BVH refit after shape updates ------------------------------------------------
+// Leaf records copy their primitive's current geometry (the primitive id and
+// the sphere tag stay).
+__global__ void k_refit_leaves(LeafTri *leaf, uint32_t n, const uint32_t *tri, const float4 *pos,
+                               const float4 *sph4) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float4 *p = reinterpret_cast<float4 *>(leaf + i);
+        const float4 a = p[0], b = p[1];
+        const uint32_t prim = f2u(a.w);
+        if (f2u(b.w) != 0u) {
+            const float4 q = sph4[prim & kPrimIndexMask];
+            p[0] = make_float4(q.x, q.y, q.z, a.w);
+            p[1] = make_float4(q.w, 0.f, 0.f, b.w);
+        } else {
+            const uint4 t = *reinterpret_cast<const uint4 *>(tri + 4 * (size_t)prim);
+            const float4 p0 = pos[t.x], p1 = pos[t.y], p2 = pos[t.z];
+            p[0] = make_float4(p0.x, p0.y, p0.z, a.w);
+            p[1] = make_float4(p1.x, p1.y, p1.z, 0.f);
+            p[2] = make_float4(p2.x, p2.y, p2.z, 0.f);
+        }
+    }
+}
+
+// One tree level, bottom-up: each node's children boxes (child nodes from
+// the level below, leaves from their primitives), the node box, and the
+// quantized child boxes against the new box (the builder's rule).  Boxes
+// are kept per node for the parents; `failed` is set when an axis cannot be
+// quantized.
+__global__ void k_refit_nodes(WideNode *nodes, float4 *boxes, const LeafTri *leaf, const uint32_t *list,
+                              uint32_t count, uint32_t *failed) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const uint32_t idx = list[t];
+    WideNode n;
+    load_node_dev(nodes, idx, n);
+    float clo[3][8], chi[3][8];
+    int slot[8];
+    int ns = 0;
+    float nlo[3] = {INFINITY, INFINITY, INFINITY}, nhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t meta = n.meta[k];
+        if (meta == 0) continue;
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        if (n.imask & (1u << k)) {
+            const uint32_t c = n.child_base + __builtin_popcount((uint32_t)n.imask & ((1u << k) - 1u));
+            const float4 a = boxes[2 * (size_t)c], b = boxes[2 * (size_t)c + 1];
+            lo[0] = a.x; lo[1] = a.y; lo[2] = a.z;
+            hi[0] = b.x; hi[1] = b.y; hi[2] = b.z;
+        } else {
+            const uint32_t cnt = __builtin_popcount(meta >> 5), off = meta & 31u;
+            for (uint32_t j = 0; j < cnt; ++j) {
+                const float4 *p = reinterpret_cast<const float4 *>(leaf + n.tri_base + off + j);
+                const float4 a = p[0], b = p[1];
+                if (f2u(b.w) != 0u) {
+                    const float c3[3] = {a.x, a.y, a.z};
+                    for (int ax = 0; ax < 3; ++ax) {
+                        lo[ax] = fminf(lo[ax], c3[ax] - b.x);
+                        hi[ax] = fmaxf(hi[ax], c3[ax] + b.x);
+                    }
+                } else {
+                    const float4 c = p[2];
+                    const float v[3][3] = {{a.x, a.y, a.z}, {b.x, b.y, b.z}, {c.x, c.y, c.z}};
+                    for (int q = 0; q < 3; ++q)
+                        for (int ax = 0; ax < 3; ++ax) {
+                            lo[ax] = fminf(lo[ax], v[q][ax]);
+                            hi[ax] = fmaxf(hi[ax], v[q][ax]);
+                        }
+                }
+            }
+        }
+        for (int ax = 0; ax < 3; ++ax) {
+            clo[ax][ns] = lo[ax];
+            chi[ax][ns] = hi[ax];
+            nlo[ax] = fminf(nlo[ax], lo[ax]);
+            nhi[ax] = fmaxf(nhi[ax], hi[ax]);
+        }
+        slot[ns++] = k;
+    }
+    if (ns == 0) return;
+    uint8_t e[3], ql[3][8], qh[3][8];
+    const float np[3] = {nlo[0], nlo[1], nlo[2]};
+    for (int ax = 0; ax < 3; ++ax)
+        if (!quantize_node_axis(np[ax], nhi[ax], clo[ax], chi[ax], ns, e[ax], ql[ax], qh[ax])) {
+            atomicOr(failed, 1u);
+            return;
+        }
+    n.px = np[0];
+    n.py = np[1];
+    n.pz = np[2];
+    n.ex = e[0];
+    n.ey = e[1];
+    n.ez = e[2];
+    for (int j = 0; j < ns; ++j) {
+        const int k = slot[j];
+        n.qlo_x[k] = ql[0][j];
+        n.qlo_y[k] = ql[1][j];
+        n.qlo_z[k] = ql[2][j];
+        n.qhi_x[k] = qh[0][j];
+        n.qhi_y[k] = qh[1][j];
+        n.qhi_z[k] = qh[2][j];
+    }
+    const uint4 *src = reinterpret_cast<const uint4 *>(&n);
+    uint4 *dst = reinterpret_cast<uint4 *>(nodes + idx);
+    for (int q = 0; q < 5; ++q) dst[q] = src[q];
+    boxes[2 * (size_t)idx] = make_float4(nlo[0], nlo[1], nlo[2], 0.f);
+    boxes[2 * (size_t)idx + 1] = make_float4(nhi[0], nhi[1], nhi[2], 0.f);
+}
+
 }  // namespace
 
 // =====================================================================================
@@ -765,6 +873,17 @@ struct BatchState {
     pt_batch_stats stats{};
 };
 
+// Host facts of a shape for dynamic updates.
+constexpr uint32_t kVertexUnused = 0xFFFFFFFFu, kVertexShared = 0xFFFFFFFEu;
+struct ShapeHost {
+    uint32_t flags0 = 0;        // ShapeRec flags at pt_set_scene
+    uint32_t first = 0, count = 0;
+    uint32_t vmin = 0, vmax = 0;  // vertex range used (triangle shapes)
+    bool sphere = false, emitter = false, shared = false;
+    TextureRec radiance{};
+    float radiance_weight = 0.f;
+};
+
 struct pt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -788,6 +907,12 @@ struct pt_ctx {
     CameraRec camera{};
     uint32_t width = 0, height = 0, scene_max_depth = 1;
     uint32_t num_tris = 0, num_spheres = 0;
+    // dynamic updates: geometry as given to pt_set_scene, per-shape facts
+    std::vector<float> h_pos0, h_nrm0, h_sph_m0;  // 3 / 3 per vertex, 12 per sphere
+    std::vector<ShapeHost> shape_host;
+    std::vector<uint32_t> vert_owner;             // shape of each vertex, kVertexUnused / kVertexShared
+    bool has_env = false;
+    bool bvh_dirty = false;
 
     // device scene
     uint32_t *d_tri = nullptr;
@@ -803,6 +928,12 @@ struct pt_ctx {
     float *d_env_tables = nullptr;
     WideNode *d_nodes = nullptr;
     LeafTri *d_leaf = nullptr;
+    float4 *d_sph4 = nullptr;       // sphere center + radius (refit)
+    uint32_t *d_levels = nullptr;   // node indices level by level (refit)
+    float4 *d_node_box = nullptr;   // refit: per node box lo, hi
+    uint32_t *d_refit_err = nullptr;
+    std::vector<uint32_t> level_off;
+    uint32_t num_nodes = 0, num_leaf = 0;
     BvhBuildStats bvh_stats{};
 
     // frame buffers
@@ -886,6 +1017,9 @@ void free_scene(pt_ctx *ctx) {
     dfree(ctx->d_env_tables);
     dfree(ctx->d_nodes);
     dfree(ctx->d_leaf);
+    dfree(ctx->d_sph4);
+    dfree(ctx->d_levels);
+    dfree(ctx->d_node_box);
     dfree(ctx->d_accum);
     dfree(ctx->d_frame);
     dfree(ctx->d_accum64);
@@ -1107,6 +1241,8 @@ struct SceneHostData {
     std::vector<BitmapRec> h_bitmaps;
     std::vector<float> h_texels;     // 4 floats per texel
     std::vector<float> h_env_tables; // env map: row cdf (h + 1) | column cdfs ((w + 1) h) | row weights (h)
+    std::vector<ShapeHost> shape_host;
+    std::vector<uint32_t> vert_owner;
     EnvRec env{};
     CameraRec camera{};
     uint32_t width = 0, height = 0, max_depth = 1, num_tris = 0, num_spheres = 0;
@@ -1182,6 +1318,102 @@ void env_table_pointers(EnvRec &e, const float *base) {
     e.row_weight = e.col_cdf + (size_t)(e.w + 1u) * e.h;
 }
 
+// One emitting triangle (SetMeshAreaEmitter, world/emitter.cpp:169-222):
+// vertex normals (normalized component-wise as util::Transform::
+// TransformNormal does) or the face normal.
+EmitterRec tri_emitter(const f3 p[3], const f3 nrm[3], bool has_normals, bool neg_face, const f2 uv[3],
+                       const TextureRec &radiance, float radiance_weight) {
+    EmitterRec e;
+    std::memset(&e, 0, sizeof(e));
+    e.type = kEmitterTriArea;
+    f3 n[3];
+    for (int k = 0; k < 3; ++k) {
+        f3 nn = has_normals ? nrm[k] : mk3(0.f);
+        if (!has_normals) {
+            nn = cross(p[1] - p[0], p[2] - p[0]);
+            if (neg_face) nn = -nn;
+        }
+        float len = sqrtf(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
+        n[k] = mk3(nn.x / len, nn.y / len, nn.z / len);
+    }
+    e.p0 = f4{p[0].x, p[0].y, p[0].z, 0.f};
+    e.p1 = f4{p[1].x, p[1].y, p[1].z, 0.f};
+    e.p2 = f4{p[2].x, p[2].y, p[2].z, 0.f};
+    e.n0 = f4{n[0].x, n[0].y, n[0].z, 0.f};
+    e.n1 = f4{n[1].x, n[1].y, n[1].z, 0.f};
+    e.n2 = f4{n[2].x, n[2].y, n[2].z, 0.f};
+    e.uv01 = f4{uv[0].x, uv[0].y, uv[1].x, uv[1].y};
+    e.uv2 = f4{uv[2].x, uv[2].y, 0.f, 0.f};
+    e.area = length(cross(p[1] - p[0], p[2] - p[0])) * 0.5f;
+    e.radiance = radiance;
+    e.weight = radiance_weight * e.area;
+    return e;
+}
+
+// SetSphereAreaEmitter (world/emitter.cpp:224-241); sph4 = center, radius.
+EmitterRec sphere_emitter(const float *sph4, const TextureRec &radiance, float radiance_weight) {
+    EmitterRec e;
+    std::memset(&e, 0, sizeof(e));
+    e.type = kEmitterSphere;
+    const float rad = sph4[3];
+    e.p0 = f4{sph4[0], sph4[1], sph4[2], rad};
+    e.area = 4.f * 3.14159265358979323846f * rad * rad;
+    e.radiance = radiance;
+    e.weight = radiance_weight * e.area;
+    return e;
+}
+
+// EmitterHelper::ComputeProbability (world/emitter.cpp:321-337) and the
+// running-sum table select_emitter searches.
+void emitter_probabilities(std::vector<EmitterRec> &em, bool has_env, std::vector<float> &cdf, float &env_prob) {
+    float sum = 0.f;
+    for (auto &e : em) sum += e.weight;
+    const size_t na = em.size();
+    const size_t total = na + (has_env ? 1 : 0);
+    for (auto &e : em) {
+        e.select_prob = e.weight / sum * (float)na;
+        e.select_prob = e.select_prob / (float)total;
+    }
+    cdf.resize(na);
+    float acc = 0.f;
+    for (size_t i = 0; i < na; ++i) {
+        acc = acc + em[i].select_prob;
+        cdf[i] = acc;
+    }
+    env_prob = has_env ? 1.f / (float)total : 0.f;
+}
+
+// Rows of the world -> object transform of a sphere placed by `m` (3x4).
+void sphere_inverse_rows(const float *m, SphereRec &rec) {
+    const double a00 = m[0], a01 = m[1], a02 = m[2], a10 = m[4], a11 = m[5], a12 = m[6];
+    const double a20 = m[8], a21 = m[9], a22 = m[10];
+    const double c00 = a11 * a22 - a12 * a21, c01 = a02 * a21 - a01 * a22, c02 = a01 * a12 - a02 * a11;
+    const double c10 = a12 * a20 - a10 * a22, c11 = a00 * a22 - a02 * a20, c12 = a02 * a10 - a00 * a12;
+    const double c20 = a10 * a21 - a11 * a20, c21 = a01 * a20 - a00 * a21, c22 = a00 * a11 - a01 * a10;
+    const double det = a00 * c00 + a01 * c10 + a02 * c20;
+    const double inv[3][3] = {{c00 / det, c01 / det, c02 / det},
+                              {c10 / det, c11 / det, c12 / det},
+                              {c20 / det, c21 / det, c22 / det}};
+    const double t[3] = {m[3], m[7], m[11]};
+    for (int i = 0; i < 3; ++i) {
+        const double ti = -(inv[i][0] * t[0] + inv[i][1] * t[1] + inv[i][2] * t[2]);
+        rec.w2o[i] = f4{(float)inv[i][0], (float)inv[i][1], (float)inv[i][2], (float)ti};
+    }
+}
+
+// Whether a 3x4 placement is a finite similarity (uniform scale).
+bool is_similarity(const float *m) {
+    for (int i = 0; i < 12; ++i)
+        if (!std::isfinite(m[i])) return false;
+    double c[3][3];
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i) c[j][i] = m[4 * i + j];
+    auto dotc = [&](int a, int b) { return c[a][0] * c[b][0] + c[a][1] * c[b][1] + c[a][2] * c[b][2]; };
+    const double n0 = dotc(0, 0), tol = 1e-4 * n0;
+    return n0 > 0.0 && std::fabs(dotc(1, 1) - n0) <= tol && std::fabs(dotc(2, 2) - n0) <= tol &&
+           std::fabs(dotc(0, 1)) <= tol && std::fabs(dotc(0, 2)) <= tol && std::fabs(dotc(1, 2)) <= tol;
+}
+
 // Validation and packing of PTPass::SetScene + World::LoadScene
 // (pt_pass.cpp:127-240, world/world.cpp:108-146, world/emitter.cpp:169-337).
 // Nothing is produced for a malformed description.
@@ -1215,17 +1447,7 @@ int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, s
         return PT_ERR_INVALID_ARG;
     }
     for (uint32_t k = 0; k < nsph; ++k) {
-        const float *m = s->spheres[k].to_world;
-        bool finite = true;
-        for (int i = 0; i < 12; ++i) finite = finite && std::isfinite(m[i]);
-        double c[3][3];
-        for (int j = 0; j < 3; ++j)
-            for (int i = 0; i < 3; ++i) c[j][i] = m[4 * i + j];  // column j
-        auto dotc = [&](int a, int b) { return c[a][0] * c[b][0] + c[a][1] * c[b][1] + c[a][2] * c[b][2]; };
-        const double n0 = dotc(0, 0);
-        const double tol = 1e-4 * n0;
-        if (!finite || !(n0 > 0.0) || std::fabs(dotc(1, 1) - n0) > tol || std::fabs(dotc(2, 2) - n0) > tol ||
-            std::fabs(dotc(0, 1)) > tol || std::fabs(dotc(0, 2)) > tol || std::fabs(dotc(1, 2)) > tol) {
+        if (!is_similarity(s->spheres[k].to_world)) {
             err = std::string(who) + ": sphere to_world must be a finite similarity transform (uniform scale)";
             return PT_ERR_UNSUPPORTED;
         }
@@ -1305,27 +1527,41 @@ int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, s
             out.lo[a] = std::min(out.lo[a], q[a] - q[3]);
             out.hi[a] = std::max(out.hi[a], q[a] + q[3]);
         }
-        // Inverse of [A | t] in double: [A^-1 | -A^-1 t].
-        const double a00 = m[0], a01 = m[1], a02 = m[2], a10 = m[4], a11 = m[5], a12 = m[6];
-        const double a20 = m[8], a21 = m[9], a22 = m[10];
-        const double c00 = a11 * a22 - a12 * a21, c01 = a02 * a21 - a01 * a22, c02 = a01 * a12 - a02 * a11;
-        const double c10 = a12 * a20 - a10 * a22, c11 = a00 * a22 - a02 * a20, c12 = a02 * a10 - a00 * a12;
-        const double c20 = a10 * a21 - a11 * a20, c21 = a01 * a20 - a00 * a21, c22 = a00 * a11 - a01 * a10;
-        const double det = a00 * c00 + a01 * c10 + a02 * c20;
-        const double inv[3][3] = {{c00 / det, c01 / det, c02 / det},
-                                  {c10 / det, c11 / det, c12 / det},
-                                  {c20 / det, c21 / det, c22 / det}};
-        const double t[3] = {m[3], m[7], m[11]};
-        SphereRec &rec = out.h_spheres[k];
-        for (int i = 0; i < 3; ++i) {
-            const double ti = -(inv[i][0] * t[0] + inv[i][1] * t[1] + inv[i][2] * t[2]);
-            rec.w2o[i] = f4{(float)inv[i][0], (float)inv[i][1], (float)inv[i][2], (float)ti};
-        }
+        sphere_inverse_rows(m, out.h_spheres[k]);
     }
     for (int k = 0; k < 3; ++k)
         if (!(out.lo[k] <= out.hi[k])) out.lo[k] = out.hi[k] = 0.f;
     out.h_tri4.assign((size_t)nt * 4, 0u);
     out.h_shapes.resize(ns);
+    out.shape_host.assign(ns, ShapeHost());
+    out.vert_owner.assign(nv, kVertexUnused);
+    for (uint32_t si = 0; si < ns; ++si) {
+        const pt_shape_desc &sh = s->shapes[si];
+        ShapeHost &h = out.shape_host[si];
+        h.first = sh.first_tri;
+        h.count = sh.num_tris;
+        h.sphere = (sh.flags & PT_SHAPE_SPHERE) != 0;
+        h.emitter = sh.is_emitter != 0;
+        h.radiance = to_texture_rec(sh.radiance);
+        h.radiance_weight = texture_weight(sh.radiance, s->bitmaps);
+        if (h.sphere) continue;
+        h.vmin = 0xFFFFFFFFu;
+        h.vmax = 0;
+        for (uint64_t i = 3ull * sh.first_tri; i < 3ull * ((uint64_t)sh.first_tri + sh.num_tris); ++i) {
+            const uint32_t v = s->indices[i];
+            h.vmin = std::min(h.vmin, v);
+            h.vmax = std::max(h.vmax, v);
+            uint32_t &o = out.vert_owner[v];
+            if (o == kVertexUnused) {
+                o = si;
+            } else if (o != si) {
+                if (o != kVertexShared) out.shape_host[o].shared = true;
+                h.shared = true;
+                o = kVertexShared;
+            }
+        }
+        if (sh.num_tris == 0) h.vmin = h.vmax = 0;
+    }
     // Area emitters: one per triangle of each emitting shape, in shape order
     // (world/emitter.cpp:169-222, pt_pass.cpp:219-235).
     out.h_emitters.clear();
@@ -1341,92 +1577,46 @@ int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, s
         if (sh.flags & PT_SHAPE_SPHERE) {
             r.flags = sh.flags & (uint32_t)(PT_SHAPE_SPHERE | PT_SHAPE_FLIP_NORMALS);
             out.h_shapes[si] = r;
+            out.shape_host[si].flags0 = r.flags;
             for (uint32_t k = sh.first_tri; k < sh.first_tri + sh.num_tris; ++k) {
                 out.h_spheres[k].shape = si;
                 if (!sh.is_emitter) continue;
-                // SetSphereAreaEmitter (world/emitter.cpp:224-241).
-                EmitterRec e;
-                std::memset(&e, 0, sizeof(e));
-                e.type = kEmitterSphere;
-                const float rad = out.h_sph4[4 * k + 3];
-                e.p0 = f4{out.h_sph4[4 * k], out.h_sph4[4 * k + 1], out.h_sph4[4 * k + 2], rad};
-                e.area = 4.f * 3.14159265358979323846f * rad * rad;
-                e.radiance = to_texture_rec(sh.radiance);
-                e.weight = texture_weight(sh.radiance, s->bitmaps) * e.area;
-                out.h_emitters.push_back(e);
+                out.h_emitters.push_back(sphere_emitter(&out.h_sph4[4 * (size_t)k], to_texture_rec(sh.radiance),
+                                                        texture_weight(sh.radiance, s->bitmaps)));
             }
             continue;
         }
         out.h_shapes[si] = r;
+        out.shape_host[si].flags0 = r.flags;
         for (uint32_t t = sh.first_tri; t < sh.first_tri + sh.num_tris; ++t) {
             for (int k = 0; k < 3; ++k) out.h_tri4[4 * (size_t)t + k] = s->indices[3 * (size_t)t + k];
             out.h_tri4[4 * (size_t)t + 3] = si;
             if (!sh.is_emitter) continue;
-            EmitterRec e;
-            std::memset(&e, 0, sizeof(e));
-            e.type = kEmitterTriArea;
             f3 p[3], n[3];
             f2 uv[3];
+            const bool has_n = s->normals && (r.flags & kShapeHasNormals);
             for (int k = 0; k < 3; ++k) {
                 uint32_t vi = s->indices[3 * (size_t)t + k];
                 p[k] = mk3(s->positions[3 * vi], s->positions[3 * vi + 1], s->positions[3 * vi + 2]);
-                f3 nn;
-                if (s->normals && (r.flags & kShapeHasNormals)) {
-                    nn = mk3(s->normals[3 * vi], s->normals[3 * vi + 1], s->normals[3 * vi + 2]);
-                } else {
-                    nn = mk3(0.f);
-                }
+                n[k] = has_n ? mk3(s->normals[3 * vi], s->normals[3 * vi + 1], s->normals[3 * vi + 2]) : mk3(0.f);
                 uv[k] = s->texcoords ? mk2(s->texcoords[2 * vi], s->texcoords[2 * vi + 1]) : mk2(0.f, 0.f);
-                n[k] = nn;
             }
-            for (int k = 0; k < 3; ++k) {
-                f3 nn = n[k];
-                if (!(r.flags & kShapeHasNormals)) {
-                    nn = cross(p[1] - p[0], p[2] - p[0]);
-                    if (r.flags & kShapeNegFaceNormal) nn = -nn;
-                }
-                // util::Transform::TransformNormal normalizes component-wise.
-                float len = sqrtf(nn.x * nn.x + nn.y * nn.y + nn.z * nn.z);
-                n[k] = mk3(nn.x / len, nn.y / len, nn.z / len);
-            }
-            e.p0 = f4{p[0].x, p[0].y, p[0].z, 0.f};
-            e.p1 = f4{p[1].x, p[1].y, p[1].z, 0.f};
-            e.p2 = f4{p[2].x, p[2].y, p[2].z, 0.f};
-            e.n0 = f4{n[0].x, n[0].y, n[0].z, 0.f};
-            e.n1 = f4{n[1].x, n[1].y, n[1].z, 0.f};
-            e.n2 = f4{n[2].x, n[2].y, n[2].z, 0.f};
-            e.uv01 = f4{uv[0].x, uv[0].y, uv[1].x, uv[1].y};
-            e.uv2 = f4{uv[2].x, uv[2].y, 0.f, 0.f};
-            e.area = length(cross(p[1] - p[0], p[2] - p[0])) * 0.5f;
-            e.radiance = to_texture_rec(sh.radiance);
-            e.weight = texture_weight(sh.radiance, s->bitmaps) * e.area;
-            out.h_emitters.push_back(e);
+            out.h_emitters.push_back(tri_emitter(p, n, has_n, (r.flags & kShapeNegFaceNormal) != 0, uv,
+                                                 to_texture_rec(sh.radiance), texture_weight(sh.radiance, s->bitmaps)));
         }
     }
-    // EmitterHelper::ComputeProbability (world/emitter.cpp:321-337).
     {
-        float sum = 0.f;
-        for (auto &e : out.h_emitters) sum += e.weight;
-        const size_t na = out.h_emitters.size();
-        const size_t total = na + (s->env_type != PT_ENV_NONE ? 1 : 0);
-        for (auto &e : out.h_emitters) {
-            e.select_prob = e.weight / sum * (float)na;
-            e.select_prob = e.select_prob / (float)total;
-        }
-        out.h_cdf.resize(na);
-        float acc = 0.f;
-        for (size_t i = 0; i < na; ++i) {
-            acc = acc + out.h_emitters[i].select_prob;
-            out.h_cdf[i] = acc;
-        }
+        float env_prob;
+        emitter_probabilities(out.h_emitters, s->env_type != PT_ENV_NONE, out.h_cdf, env_prob);
         std::memset(&out.env, 0, sizeof(out.env));
+        const float total_inv = env_prob;
         if (s->env_type == PT_ENV_CONSTANT) {
             out.env.type = kEmitterConstEnv;
-            out.env.select_prob = 1.f / (float)total;
+            out.env.select_prob = total_inv;
             out.env.color = f4{s->env_radiance[0], s->env_radiance[1], s->env_radiance[2], 0.f};
         } else if (s->env_type == PT_ENV_MAP) {
             out.env.type = kEmitterEnvMap;
-            out.env.select_prob = 1.f / (float)total;
+            out.env.select_prob = total_inv;
         }
     }
     out.h_materials.resize(nm);
@@ -1618,6 +1808,39 @@ void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::ve
         }
     }
     flush();
+}
+
+// Brings the BVH up to date with pending shape updates: leaf records from
+// the current geometry, then node boxes level by level from the deepest.
+int bvh_refit(pt_ctx *ctx) {
+    if (!ctx->bvh_dirty || !ctx->has_bvh) return PT_OK;
+    const auto t0 = std::chrono::steady_clock::now();
+    (void)hipSetDevice(ctx->device);
+    PT_HIP(hipMemsetAsync(ctx->d_refit_err, 0, sizeof(uint32_t), ctx->stream));
+    if (ctx->num_leaf) {
+        const uint32_t grid = std::min<uint32_t>((ctx->num_leaf + 255u) / 256u, (uint32_t)ctx->num_cu * 16u);
+        hipLaunchKernelGGL(k_refit_leaves, dim3(grid), dim3(256), 0, ctx->stream, ctx->d_leaf, ctx->num_leaf,
+                           ctx->d_tri, ctx->d_pos, ctx->d_sph4);
+    }
+    for (size_t l = ctx->level_off.size() - 1; l-- > 0;) {
+        const uint32_t begin = ctx->level_off[l], count = ctx->level_off[l + 1] - begin;
+        if (count == 0) continue;
+        hipLaunchKernelGGL(k_refit_nodes, dim3((count + 63u) / 64u), dim3(64), 0, ctx->stream, ctx->d_nodes,
+                           ctx->d_node_box, ctx->d_leaf, ctx->d_levels + begin, count, ctx->d_refit_err);
+    }
+    uint32_t failed = 0;
+    PT_HIP(hipGetLastError());
+    PT_HIP(hipMemcpyAsync(&failed, ctx->d_refit_err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    PT_HIP(hipStreamSynchronize(ctx->stream));
+    if (failed) {
+        ctx->error = "BVH refit: a node box cannot be quantized; rebuild with pt_build_bvh";
+        ctx->has_bvh = false;
+        return PT_ERR_CAPACITY;
+    }
+    ctx->bvh_dirty = false;
+    ctx->stats.refits += 1;
+    ctx->stats.ms_refit += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return PT_OK;
 }
 
 struct BatchShape {
@@ -1849,6 +2072,7 @@ void pt_destroy(pt_ctx *ctx) {
     dfree(ctx->d_gout);
     dfree(ctx->d_spill);
     dfree(ctx->d_stat);
+    dfree(ctx->d_refit_err);
     dfree(ctx->d_count_hist);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -1877,6 +2101,16 @@ int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
     ctx->h_materials.swap(d.h_materials);
     ctx->h_emitters.swap(d.h_emitters);
     ctx->h_cdf.swap(d.h_cdf);
+    ctx->shape_host.swap(d.shape_host);
+    ctx->vert_owner.swap(d.vert_owner);
+    ctx->h_pos0.assign(s->positions, s->positions + 3 * (size_t)s->num_vertices);
+    if (s->normals) ctx->h_nrm0.assign(s->normals, s->normals + 3 * (size_t)s->num_vertices);
+    else ctx->h_nrm0.clear();
+    ctx->h_sph_m0.resize(12 * (size_t)s->num_spheres);
+    for (uint32_t k = 0; k < s->num_spheres; ++k)
+        std::memcpy(&ctx->h_sph_m0[12 * (size_t)k], s->spheres[k].to_world, 12 * sizeof(float));
+    ctx->has_env = s->env_type != PT_ENV_NONE;
+    ctx->bvh_dirty = false;
     ctx->env = d.env;
     ctx->camera = d.camera;
     ctx->width = d.width;
@@ -1900,6 +2134,9 @@ int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
         return rc;
     if ((rc = upload(ctx, ctx->d_shapes, ctx->h_shapes.data(), ctx->h_shapes.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, ctx->d_spheres, ctx->h_spheres.data(), ctx->h_spheres.size())) != PT_OK) return rc;
+    if ((rc = upload(ctx, ctx->d_sph4, reinterpret_cast<const float4 *>(ctx->h_sph4.data()), ctx->h_sph4.size() / 4)) !=
+        PT_OK)
+        return rc;
     if ((rc = upload(ctx, ctx->d_materials, ctx->h_materials.data(), ctx->h_materials.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, ctx->d_emitters, ctx->h_emitters.data(), ctx->h_emitters.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, ctx->d_cdf, ctx->h_cdf.data(), ctx->h_cdf.size())) != PT_OK) return rc;
@@ -1955,6 +2192,29 @@ int pt_build_bvh(pt_ctx *ctx, const pt_bvh_opts *opts) {
     if ((rc = upload(ctx, ctx->d_nodes, nodes.data(), nodes.size())) != PT_OK) return rc;
     if ((rc = upload(ctx, ctx->d_leaf, tris.data(), tris.size())) != PT_OK) return rc;
     if ((rc = ensure_spill(ctx)) != PT_OK) return rc;
+    // Node levels for refits: internal children of node i are child_base ..
+    // child_base + popcount(imask) - 1, one level below.
+    {
+        std::vector<uint32_t> order, next, cur{0u};
+        ctx->level_off.assign(1, 0u);
+        while (!cur.empty()) {
+            next.clear();
+            for (uint32_t i : cur) {
+                order.push_back(i);
+                const uint32_t ni = (uint32_t)__builtin_popcount(nodes[i].imask);
+                for (uint32_t j = 0; j < ni; ++j) next.push_back(nodes[i].child_base + j);
+            }
+            ctx->level_off.push_back((uint32_t)order.size());
+            cur.swap(next);
+        }
+        if ((rc = upload(ctx, ctx->d_levels, order.data(), order.size())) != PT_OK) return rc;
+        dfree(ctx->d_node_box);
+        PT_HIP(hipMalloc((void **)&ctx->d_node_box, nodes.size() * 2 * sizeof(float4)));
+        if (!ctx->d_refit_err) PT_HIP(hipMalloc((void **)&ctx->d_refit_err, sizeof(uint32_t)));
+    }
+    ctx->num_nodes = (uint32_t)nodes.size();
+    ctx->num_leaf = (uint32_t)tris.size();
+    ctx->bvh_dirty = false;
     ctx->has_bvh = true;
     return PT_OK;
 }
@@ -1981,6 +2241,143 @@ int pt_reset_accumulation(pt_ctx *ctx) {
     return PT_OK;
 }
 
+int pt_set_shape_transform(pt_ctx *ctx, uint32_t shape, const float m[12]) {
+    if (!ctx || !m) return PT_ERR_INVALID_ARG;
+    if (!ctx->has_scene) {
+        ctx->error = "pt_set_shape_transform: no scene";
+        return PT_ERR_NO_SCENE;
+    }
+    if (shape >= ctx->shape_host.size()) {
+        ctx->error = "pt_set_shape_transform: shape index out of range";
+        return PT_ERR_INVALID_ARG;
+    }
+    for (int i = 0; i < 12; ++i)
+        if (!std::isfinite(m[i])) {
+            ctx->error = "pt_set_shape_transform: non-finite transform";
+            return PT_ERR_INVALID_ARG;
+        }
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    const ShapeHost &sh = ctx->shape_host[shape];
+    int rc;
+    if (sh.sphere) {
+        // New placement m * M0 of each sphere (3x4 composition, float).
+        std::vector<float> placed(12 * (size_t)sh.count);
+        for (uint32_t j = 0; j < sh.count; ++j) {
+            const float *a = &ctx->h_sph_m0[12 * (size_t)(sh.first + j)];
+            float *o = &placed[12 * (size_t)j];
+            for (int r = 0; r < 3; ++r) {
+                for (int c = 0; c < 3; ++c)
+                    o[4 * r + c] = m[4 * r] * a[c] + m[4 * r + 1] * a[4 + c] + m[4 * r + 2] * a[8 + c];
+                o[4 * r + 3] = m[4 * r] * a[3] + m[4 * r + 1] * a[7] + m[4 * r + 2] * a[11] + m[4 * r + 3];
+            }
+            if (!is_similarity(o)) {
+                ctx->error = "pt_set_shape_transform: a sphere must stay a similarity transform";
+                return PT_ERR_UNSUPPORTED;
+            }
+        }
+        for (uint32_t j = 0; j < sh.count; ++j) {
+            const uint32_t k = sh.first + j;
+            sphere_center_radius(&placed[12 * (size_t)j], &ctx->h_sph4[4 * (size_t)k]);
+            sphere_inverse_rows(&placed[12 * (size_t)j], ctx->h_spheres[k]);
+            if (sh.emitter) {
+                const int32_t ei = ctx->h_shapes[shape].emitter_offset + (int32_t)j;
+                ctx->h_emitters[ei] = sphere_emitter(&ctx->h_sph4[4 * (size_t)k], sh.radiance, sh.radiance_weight);
+            }
+        }
+        if (sh.count) {
+            PT_HIP(hipMemcpy(ctx->d_spheres + sh.first, &ctx->h_spheres[sh.first], sh.count * sizeof(SphereRec),
+                             hipMemcpyHostToDevice));
+            PT_HIP(hipMemcpy(ctx->d_sph4 + sh.first, &ctx->h_sph4[4 * (size_t)sh.first], sh.count * sizeof(float4),
+                             hipMemcpyHostToDevice));
+        }
+    } else {
+        if (sh.shared) {
+            ctx->error = "pt_set_shape_transform: the shape shares vertices with another shape";
+            return PT_ERR_UNSUPPORTED;
+        }
+        // Normals by the inverse transpose of the 3x3 part (cofactors / det).
+        const double a00 = m[0], a01 = m[1], a02 = m[2], a10 = m[4], a11 = m[5], a12 = m[6];
+        const double a20 = m[8], a21 = m[9], a22 = m[10];
+        const double c00 = a11 * a22 - a12 * a21, c01 = a02 * a21 - a01 * a22, c02 = a01 * a12 - a02 * a11;
+        const double c10 = a12 * a20 - a10 * a22, c11 = a00 * a22 - a02 * a20, c12 = a02 * a10 - a00 * a12;
+        const double c20 = a10 * a21 - a11 * a20, c21 = a01 * a20 - a00 * a21, c22 = a00 * a11 - a01 * a10;
+        const double det = a00 * c00 + a01 * c10 + a02 * c20;
+        if (!(det != 0.0)) {
+            ctx->error = "pt_set_shape_transform: singular transform";
+            return PT_ERR_INVALID_ARG;
+        }
+        // (A^-1)^T: row i = column i of A^-1 = (c0i, c1i, c2i) / det.
+        const float it[9] = {(float)(c00 / det), (float)(c10 / det), (float)(c20 / det),
+                             (float)(c01 / det), (float)(c11 / det), (float)(c21 / det),
+                             (float)(c02 / det), (float)(c12 / det), (float)(c22 / det)};
+        const bool normals = !ctx->h_nrm0.empty();
+        for (uint32_t v = sh.vmin; sh.count && v <= sh.vmax; ++v) {
+            if (ctx->vert_owner[v] != shape) continue;
+            const float *p = &ctx->h_pos0[3 * (size_t)v];
+            float *q = &ctx->h_pos4[4 * (size_t)v];
+            q[0] = m[0] * p[0] + m[1] * p[1] + m[2] * p[2] + m[3];
+            q[1] = m[4] * p[0] + m[5] * p[1] + m[6] * p[2] + m[7];
+            q[2] = m[8] * p[0] + m[9] * p[1] + m[10] * p[2] + m[11];
+            if (normals) {
+                const float *n = &ctx->h_nrm0[3 * (size_t)v];
+                float *o = &ctx->h_nrm4[4 * (size_t)v];
+                o[0] = it[0] * n[0] + it[1] * n[1] + it[2] * n[2];
+                o[1] = it[3] * n[0] + it[4] * n[1] + it[5] * n[2];
+                o[2] = it[6] * n[0] + it[7] * n[1] + it[8] * n[2];
+            }
+        }
+        ShapeRec &rec = ctx->h_shapes[shape];
+        rec.flags = sh.flags0 ^ (det < 0.0 ? (uint32_t)kShapeNegFaceNormal : 0u);
+        if (sh.count) {
+            const size_t nv = (size_t)sh.vmax - sh.vmin + 1;
+            PT_HIP(hipMemcpy(ctx->d_pos + sh.vmin, &ctx->h_pos4[4 * (size_t)sh.vmin], nv * sizeof(float4),
+                             hipMemcpyHostToDevice));
+            if (normals && ctx->d_nrm)
+                PT_HIP(hipMemcpy(ctx->d_nrm + sh.vmin, &ctx->h_nrm4[4 * (size_t)sh.vmin], nv * sizeof(float4),
+                                 hipMemcpyHostToDevice));
+        }
+        PT_HIP(hipMemcpy(ctx->d_shapes + shape, &rec, sizeof(ShapeRec), hipMemcpyHostToDevice));
+        if (sh.emitter) {
+            const bool has_n = normals && (rec.flags & kShapeHasNormals);
+            for (uint32_t j = 0; j < sh.count; ++j) {
+                const uint32_t *t = &ctx->h_tri4[4 * (size_t)(sh.first + j)];
+                f3 p[3], n[3];
+                f2 uv[3];
+                for (int k = 0; k < 3; ++k) {
+                    const float *q = &ctx->h_pos4[4 * (size_t)t[k]];
+                    p[k] = mk3(q[0], q[1], q[2]);
+                    n[k] = has_n ? mk3(ctx->h_nrm4[4 * (size_t)t[k]], ctx->h_nrm4[4 * (size_t)t[k] + 1],
+                                       ctx->h_nrm4[4 * (size_t)t[k] + 2])
+                                 : mk3(0.f);
+                    uv[k] = ctx->h_uv2.empty() ? mk2(0.f, 0.f)
+                                               : mk2(ctx->h_uv2[2 * (size_t)t[k]], ctx->h_uv2[2 * (size_t)t[k] + 1]);
+                }
+                ctx->h_emitters[rec.emitter_offset + (int32_t)j] =
+                    tri_emitter(p, n, has_n, (rec.flags & kShapeNegFaceNormal) != 0, uv, sh.radiance,
+                                sh.radiance_weight);
+            }
+        }
+    }
+    if (sh.emitter) {
+        float env_prob;
+        emitter_probabilities(ctx->h_emitters, ctx->has_env, ctx->h_cdf, env_prob);
+        if ((rc = upload(ctx, ctx->d_emitters, ctx->h_emitters.data(), ctx->h_emitters.size())) != PT_OK) return rc;
+        if ((rc = upload(ctx, ctx->d_cdf, ctx->h_cdf.data(), ctx->h_cdf.size())) != PT_OK) return rc;
+    }
+    ctx->bvh_dirty = true;
+    return pt_reset_accumulation(ctx);
+}
+
+int pt_refit_bvh(pt_ctx *ctx) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    if (!ctx->has_bvh) {
+        ctx->error = "pt_refit_bvh: BVH not built";
+        return PT_ERR_NO_BVH;
+    }
+    return bvh_refit(ctx);
+}
+
 int pt_set_counting(pt_ctx *ctx, int enable) {
     if (!ctx) return PT_ERR_INVALID_ARG;
     ctx->counting = enable != 0;
@@ -1996,6 +2393,10 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
     if (!ctx->has_bvh) {
         ctx->error = "pt_render: BVH not built";
         return PT_ERR_NO_BVH;
+    }
+    {
+        const int rr = bvh_refit(ctx);
+        if (rr != PT_OK) return rr;
     }
     if (p->mis_mode > PT_MIS_CONSISTENT) {
         ctx->error = "pt_render: unknown MIS mode";
@@ -2312,6 +2713,10 @@ int pt_trace_rays(pt_ctx *ctx, const float *rays, uint32_t count, int any_hit, v
     if (!ctx->has_bvh) {
         ctx->error = "pt_trace_rays: BVH not built";
         return PT_ERR_NO_BVH;
+    }
+    {
+        const int rr = bvh_refit(ctx);
+        if (rr != PT_OK) return rr;
     }
     if (count == 0) return PT_OK;
     (void)hipSetDevice(ctx->device);
