@@ -285,6 +285,32 @@ int pt_bandwidth_probe(pt_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps
  * tmax).  hits: 4 words each (t, primitive id, b1, b2). */
 int pt_trace_rays(pt_ctx *ctx, const float *rays, uint32_t count, int any_hit, void *hits);
 
+/* ---- denoising ---------------------------------------------------------
+ * DenoisePass (system/denoise_pass.cpp:31-93) runs optix::Denoiser
+ * (optix/denoiser.h:9-38) on the noisy buffer with optional albedo / normal
+ * guides (EMode UseAlbedo = 1, UseNormal = 2) and writes "final result".  The
+ * OptiX model is closed; this build filters with an edge-avoiding a-trous
+ * wavelet transform (Dammertz et al. 2010) with variance-guided luminance
+ * stopping (Schied et al. 2017) on albedo-demodulated color. */
+enum pt_denoise_flags { PT_DENOISE_ALBEDO = 1, PT_DENOISE_NORMAL = 2 };
+
+typedef struct pt_denoise_params {
+    uint32_t flags;        /* pt_denoise_flags */
+    uint32_t iterations;   /* a-trous levels, 0 = 5 (strides 1..16) */
+    float sigma_luminance; /* luminance stopping in local standard deviations, 0 = 4 */
+    float sigma_normal;    /* normal weight exponent, 0 = 64 */
+    float sigma_albedo;    /* albedo distance scale, 0 = 0.1 */
+} pt_denoise_params;
+
+/* Denoises the context's "pt accum buffer" (PT_BUF_ACCUM) with the AOVs of
+ * the last rendered frame as guides into "final result" (PT_BUF_FRAME). */
+int pt_denoise(pt_ctx *ctx, const pt_denoise_params *params);
+/* Same filter on caller images (host memory): rgba 4 floats per pixel,
+ * albedo / normal 3 floats per pixel or NULL (flags then ignored for them),
+ * out 4 floats per pixel. */
+int pt_denoise_image(pt_ctx *ctx, uint32_t width, uint32_t height, const float *rgba, const float *albedo,
+                     const float *normal, float *out, const pt_denoise_params *params);
+
 /* ---- learned path guiding ---------------------------------------------
  * No reference counterpart: PupilOptixLab samples directions from the BSDF
  * only (example/path_tracer/main.cu:146-160).  The guide is a hash-grid

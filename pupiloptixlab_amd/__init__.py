@@ -40,6 +40,7 @@ FILTER_POINT, FILTER_LINEAR = 0, 1
 WRAP_REPEAT, WRAP_CLAMP, WRAP_MIRROR, WRAP_BORDER = 0, 1, 2, 3
 MIS_REFERENCE, MIS_CONSISTENT = 0, 1
 RENDER_AOV, RENDER_FP64_ACCUM, RENDER_TIMING, RENDER_GUIDE, RENDER_GUIDE_TRAIN = 1, 2, 4, 8, 16
+DENOISE_ALBEDO, DENOISE_NORMAL = 1, 2
 GUIDE_OUTPUTS, GUIDE_LEVELS = 41, 16
 GUIDE_NO_RADIANCE_L2, GUIDE_NO_GUIDE_CE, GUIDE_NO_MIXTURE_CE = 1, 2, 4
 BATCH_GRAPH, BATCH_AUTOTUNE = 1, 2
@@ -65,6 +66,11 @@ class ShapeDesc(C.Structure):
 
 class CameraDesc(C.Structure):
     _fields_ = [("sample_to_camera", C.c_float * 16), ("camera_to_world", C.c_float * 16)]
+
+
+class DenoiseParams(C.Structure):
+    _fields_ = [("flags", C.c_uint32), ("iterations", C.c_uint32), ("sigma_luminance", C.c_float),
+                ("sigma_normal", C.c_float), ("sigma_albedo", C.c_float)]
 
 
 class SphereDesc(C.Structure):
@@ -174,6 +180,9 @@ _SIGNATURES = {
     "pt_set_camera": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "pt_reset_accumulation": (C.c_int, [C.c_void_p]),
     "pt_set_shape_transform": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "pt_denoise": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pt_denoise_image": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p]),
     "pt_refit_bvh": (C.c_int, [C.c_void_p]),
     "pt_render": (C.c_int, [C.c_void_p, C.POINTER(RenderParams)]),
     "pt_read_buffer": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
@@ -406,6 +415,29 @@ class PathTracer:
 
     def refit_bvh(self) -> None:
         _check(lib().pt_refit_bvh(self._ctx), "pt_refit_bvh", self._ctx)
+
+    # ---- denoising (DenoisePass) ---------------------------------------------------
+    def denoise(self, albedo: bool = True, normal: bool = True, iterations: int = 0, sigma_luminance: float = 0.0,
+                sigma_normal: float = 0.0, sigma_albedo: float = 0.0) -> np.ndarray:
+        """Denoises the accumulation buffer into the frame buffer and returns it."""
+        p = DenoiseParams((DENOISE_ALBEDO if albedo else 0) | (DENOISE_NORMAL if normal else 0), iterations,
+                          sigma_luminance, sigma_normal, sigma_albedo)
+        _check(lib().pt_denoise(self._ctx, C.byref(p)), "pt_denoise", self._ctx)
+        return self.read(BUF_FRAME)
+
+    def denoise_image(self, rgba, albedo=None, normal=None, iterations: int = 0, sigma_luminance: float = 0.0,
+                      sigma_normal: float = 0.0, sigma_albedo: float = 0.0) -> np.ndarray:
+        rgba = np.ascontiguousarray(rgba, np.float32)
+        h, w = rgba.shape[:2]
+        a = None if albedo is None else np.ascontiguousarray(albedo, np.float32)
+        n = None if normal is None else np.ascontiguousarray(normal, np.float32)
+        flags = (DENOISE_ALBEDO if a is not None else 0) | (DENOISE_NORMAL if n is not None else 0)
+        p = DenoiseParams(flags, iterations, sigma_luminance, sigma_normal, sigma_albedo)
+        out = np.zeros((h, w, 4), np.float32)
+        _check(lib().pt_denoise_image(self._ctx, w, h, rgba.ctypes.data, None if a is None else a.ctypes.data,
+                                      None if n is None else n.ctypes.data, out.ctypes.data, C.byref(p)),
+               "pt_denoise_image", self._ctx)
+        return out
 
     def render(self, spp: int = 1, max_depth: int = 0, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE,
                accumulate: bool = True, aov: bool = True, fp64: bool = False, timing: bool = False,

@@ -22,6 +22,7 @@
 #include "../../include/pupil_hip.h"
 #include "pt_bvh.h"
 #include "pt_bvh_build.h"
+#include "pt_denoise.h"
 #include "pt_guide_state.h"
 #include "pt_shade.h"
 #include "pt_shade_guided.h"
@@ -941,6 +942,8 @@ struct pt_ctx {
     double *d_accum64 = nullptr;
     float *d_albedo = nullptr, *d_normal = nullptr, *d_test = nullptr;
     uint32_t *d_prim = nullptr;
+    float4 *d_dn_scratch = nullptr;  // denoiser ping-pong, 2 w h
+    size_t dn_scratch_px = 0;
 
     // queues
     uint32_t capacity = 0;  // paths
@@ -1027,6 +1030,8 @@ void free_scene(pt_ctx *ctx) {
     dfree(ctx->d_normal);
     dfree(ctx->d_test);
     dfree(ctx->d_prim);
+    dfree(ctx->d_dn_scratch);
+    ctx->dn_scratch_px = 0;
     ctx->has_scene = false;
     ctx->has_bvh = false;
 }
@@ -2367,6 +2372,89 @@ int pt_set_shape_transform(pt_ctx *ctx, uint32_t shape, const float m[12]) {
     }
     ctx->bvh_dirty = true;
     return pt_reset_accumulation(ctx);
+}
+
+namespace {
+int denoise_params(const pt_denoise_params *p, DenoiseParams &out, uint32_t &flags, std::string &err) {
+    pt_denoise_params d{};
+    if (p) d = *p;
+    if (d.iterations > 10 || !(d.sigma_luminance >= 0.f) || !(d.sigma_normal >= 0.f) || !(d.sigma_albedo >= 0.f)) {
+        err = "denoise: iterations must be <= 10 and sigmas non-negative";
+        return PT_ERR_INVALID_ARG;
+    }
+    out.iterations = d.iterations ? d.iterations : 5u;
+    out.sigma_luminance = d.sigma_luminance > 0.f ? d.sigma_luminance : 4.f;
+    out.sigma_normal = d.sigma_normal > 0.f ? d.sigma_normal : 64.f;
+    out.sigma_albedo = d.sigma_albedo > 0.f ? d.sigma_albedo : 0.1f;
+    flags = d.flags;
+    return PT_OK;
+}
+
+int ensure_dn_scratch(pt_ctx *ctx, size_t px) {
+    if (ctx->d_dn_scratch && ctx->dn_scratch_px >= px) return PT_OK;
+    dfree(ctx->d_dn_scratch);
+    PT_HIP(hipMalloc((void **)&ctx->d_dn_scratch, 2 * px * sizeof(float4)));
+    ctx->dn_scratch_px = px;
+    return PT_OK;
+}
+}  // namespace
+
+int pt_denoise(pt_ctx *ctx, const pt_denoise_params *params) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    if (!ctx->has_scene) {
+        ctx->error = "pt_denoise: no scene";
+        return PT_ERR_NO_SCENE;
+    }
+    DenoiseParams prm;
+    uint32_t flags;
+    int rc = denoise_params(params, prm, flags, ctx->error);
+    if (rc != PT_OK) return rc;
+    (void)hipSetDevice(ctx->device);
+    const size_t px = (size_t)ctx->width * ctx->height;
+    if ((rc = ensure_dn_scratch(ctx, px)) != PT_OK) return rc;
+    PT_HIP(denoise_launch(ctx->stream, ctx->width, ctx->height, ctx->d_accum,
+                          (flags & PT_DENOISE_ALBEDO) ? ctx->d_albedo : nullptr,
+                          (flags & PT_DENOISE_NORMAL) ? ctx->d_normal : nullptr, ctx->d_frame, ctx->d_dn_scratch, prm));
+    PT_HIP(hipStreamSynchronize(ctx->stream));
+    return PT_OK;
+}
+
+int pt_denoise_image(pt_ctx *ctx, uint32_t w, uint32_t h, const float *rgba, const float *albedo, const float *normal,
+                     float *out, const pt_denoise_params *params) {
+    if (!ctx || !rgba || !out || w == 0 || h == 0 || (uint64_t)w * h > (1ull << 28)) return PT_ERR_INVALID_ARG;
+    DenoiseParams prm;
+    uint32_t flags;
+    int rc = denoise_params(params, prm, flags, ctx->error);
+    if (rc != PT_OK) return rc;
+    (void)hipSetDevice(ctx->device);
+    const size_t px = (size_t)w * h;
+    float4 *d_in = nullptr, *d_out = nullptr, *d_scr = nullptr;
+    float *d_a = nullptr, *d_n = nullptr;
+    const bool ua = albedo && (flags & PT_DENOISE_ALBEDO), un = normal && (flags & PT_DENOISE_NORMAL);
+    auto cleanup = [&]() {
+        dfree(d_in);
+        dfree(d_out);
+        dfree(d_scr);
+        dfree(d_a);
+        dfree(d_n);
+    };
+    hipError_t e = hipMalloc((void **)&d_in, px * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_out, px * sizeof(float4));
+    if (e == hipSuccess) e = hipMalloc((void **)&d_scr, 2 * px * sizeof(float4));
+    if (e == hipSuccess && ua) e = hipMalloc((void **)&d_a, px * 3 * sizeof(float));
+    if (e == hipSuccess && un) e = hipMalloc((void **)&d_n, px * 3 * sizeof(float));
+    if (e == hipSuccess) e = hipMemcpy(d_in, rgba, px * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess && ua) e = hipMemcpy(d_a, albedo, px * 3 * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess && un) e = hipMemcpy(d_n, normal, px * 3 * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = denoise_launch(ctx->stream, w, h, d_in, d_a, d_n, d_out, d_scr, prm);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(out, d_out, px * sizeof(float4), hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) {
+        ctx->error = std::string("pt_denoise_image: ") + hipGetErrorString(e);
+        return PT_ERR_HIP;
+    }
+    return PT_OK;
 }
 
 int pt_refit_bvh(pt_ctx *ctx) {

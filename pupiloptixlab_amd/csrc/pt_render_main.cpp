@@ -1,10 +1,11 @@
 // pt_render: headless counterpart of example/path_tracer/main.cpp:5-21.
 //
 //   pt_render --scene file.xml | --cornell | --terrain [--seed N]
-//             [--spp N] [--max-depth N] [--out image.exr]
+//             [--spp N] [--max-depth N] [--denoise] [--out image.exr]
 //
-// Builds the system, adds the path-tracing pass, loads the scene, runs one
-// frame per sample and writes the accumulated image as EXR.
+// Builds the system, adds the path-tracing pass (and, with --denoise, the
+// denoising pass after it, as main.cpp adds DenoisePass), loads the scene,
+// runs one frame per sample and writes the final image as EXR.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -18,6 +19,7 @@ int main(int argc, char **argv) {
     int kind = PT_SCENE_CORNELL;
     uint32_t spp = 16, seed = 0;
     int max_depth = 0;
+    bool denoise = false;
     for (int i = 1; i < argc; ++i) {
         std::string a = argv[i];
         auto next = [&](const char *what) -> const char * {
@@ -35,6 +37,7 @@ int main(int argc, char **argv) {
         else if (a == "--seed") seed = (uint32_t)std::atoi(next("--seed"));
         else if (a == "--max-depth") max_depth = std::atoi(next("--max-depth"));
         else if (a == "--out") out = next("--out");
+        else if (a == "--denoise") denoise = true;
         else {
             std::fprintf(stderr, "unknown argument %s\n", a.c_str());
             return 2;
@@ -47,6 +50,8 @@ int main(int argc, char **argv) {
     }
     Pupil::pt::PTPass pass(&system);
     system.AddPass(&pass);
+    Pupil::DenoisePass dn_pass(&system, Pupil::DenoisePass::Config{});
+    if (denoise) system.AddPass(&dn_pass);
     bool ok;
     if (!xml.empty()) {
         ok = system.SetScene(xml);
@@ -70,7 +75,9 @@ int main(int argc, char **argv) {
         system.Destroy();
         return 1;
     }
-    std::printf("%s\nwrote %s (%ux%u, %u spp)\n", pass.Inspector().c_str(), out.c_str(), d->width, d->height, spp);
+    std::printf("%s\n", pass.Inspector().c_str());
+    if (denoise) std::printf("%s\n", dn_pass.Inspector().c_str());
+    std::printf("wrote %s (%ux%u, %u spp)\n", out.c_str(), d->width, d->height, spp);
     system.Destroy();
     return 0;
 }

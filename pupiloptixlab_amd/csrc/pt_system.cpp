@@ -157,4 +157,25 @@ bool PTPass::ReadBuffer(int which, void *dst, size_t bytes) noexcept {
 }
 
 }  // namespace pt
+
+DenoisePass::DenoisePass(System *system, const Config &config, std::string_view name) noexcept
+    : Pass(name), m_system(system), m_config(config), m_enabled(config.default_enable) {}
+
+// DenoisePass::OnRun (system/denoise_pass.cpp:49-71).
+void DenoisePass::OnRun() noexcept {
+    if (!m_enabled) return;
+    pt_ctx *ctx = m_system ? m_system->Context() : nullptr;
+    if (!ctx) return;
+    pt_denoise_params p{};
+    p.flags = (m_config.use_albedo ? PT_DENOISE_ALBEDO : 0u) | (m_config.use_normal ? PT_DENOISE_NORMAL : 0u);
+    if (pt_denoise(ctx, &p) != PT_OK) m_error = pt_last_error(ctx);
+}
+
+std::string DenoisePass::Inspector() noexcept {
+    char buf[128];
+    std::snprintf(buf, sizeof(buf), "%s: %s, %.3f ms, albedo %s, normal %s", name.c_str(), m_enabled ? "on" : "off",
+                  m_last_ms, m_config.use_albedo ? "on" : "off", m_config.use_normal ? "on" : "off");
+    return buf;
+}
+
 }  // namespace Pupil

@@ -101,4 +101,28 @@ private:
 };
 
 }  // namespace pt
+
+// DenoisePass (system/denoise_pass.h:14-43): filters the path tracer's
+// accumulation buffer into "final result" each frame it is enabled, with the
+// albedo / normal AOVs as guides when the config asks for them.
+class DenoisePass : public Pass {
+public:
+    struct Config {
+        bool default_enable = true;
+        bool use_albedo = true;
+        bool use_normal = true;
+    };
+    explicit DenoisePass(System *system, const Config &config, std::string_view name = "Denoise") noexcept;
+    void OnRun() noexcept override;
+    std::string Inspector() noexcept override;
+    void SetEnabled(bool on) noexcept { m_enabled = on; }
+    const std::string &LastError() const noexcept { return m_error; }
+
+private:
+    System *m_system;
+    Config m_config;
+    bool m_enabled;
+    std::string m_error;
+};
+
 }  // namespace Pupil
