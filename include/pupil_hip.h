@@ -349,8 +349,9 @@ typedef struct pt_guide_info {
     uint32_t last_samples;    /* samples in the last step */
     uint32_t resolution[16];  /* grid resolution per level */
     uint32_t dense[16];       /* 1 when the level is indexed densely */
-    float bounds_lo[3];       /* position normalization: u = (p - lo) * inv */
+    float bounds_lo[3];       /* position normalization of grid 0: u = (p - lo) * inv */
     float bounds_inv[3];
+    uint32_t grids;           /* hash grids (1, or one per batch view) */
 } pt_guide_info;
 
 /* Allocates and initializes the guide over the bounds of the current scene
@@ -361,7 +362,18 @@ int pt_guide_get_info(pt_ctx *ctx, pt_guide_info *out);
 /* Raw network outputs (41 floats per position) for n world-space positions
  * (3 floats each). */
 int pt_guide_query(pt_ctx *ctx, const float *positions, uint32_t n, float *outputs);
-/* Training sample layout, 12 floats: position xyz, 0, direction xyz (unit),
+/* Guide for the batch (pt_batch_*): one hash grid per batch view, over that
+ * view's scene bounds, all feeding one shared MLP; pt_batch_render then takes
+ * PT_RENDER_GUIDE / PT_RENDER_GUIDE_TRAIN.  Adding or clearing views
+ * disables it.  Table layouts of the calls below become
+ * [views][16][2^log2_table][2]. */
+int pt_guide_enable_batch(pt_ctx *ctx, const pt_guide_config *config);
+/* pt_guide_query on hash grid `grid` (view index of a batch guide). */
+int pt_guide_query_grid(pt_ctx *ctx, uint32_t grid, const float *positions, uint32_t n, float *outputs);
+/* Position normalization of hash grid `grid`: u = (p - lo) * inv. */
+int pt_guide_grid_bounds(pt_ctx *ctx, uint32_t grid, float lo[3], float inv[3]);
+/* Training sample layout, 12 floats: position xyz, hash grid (0 with one
+ * grid; with several, a sample naming no grid counts as unused), direction xyz (unit),
  * 0, radiance target (luminance of the incident radiance times |cos|), F/p
  * (luminance of f L |cos| / pdf), BSDF pdf of the direction, weight (0 =
  * unused slot; the loss is the weighted mean).  fp_norm scales F/p in the
@@ -396,12 +408,15 @@ enum pt_batch_flags {
 typedef struct pt_batch_params {
     uint32_t rr_depth;           /* Russian roulette when depth > rr_depth */
     uint32_t mis_mode;           /* pt_mis_mode */
-    uint32_t render_flags;       /* PT_RENDER_FP64_ACCUM, PT_RENDER_TIMING */
+    uint32_t render_flags;       /* PT_RENDER_FP64_ACCUM, PT_RENDER_TIMING; PT_RENDER_GUIDE and
+                                    PT_RENDER_GUIDE_TRAIN after pt_guide_enable_batch */
     uint32_t batch_flags;        /* pt_batch_flags */
     uint32_t paths_per_launch;   /* queue capacity, 0 = tuned or default (32M) */
     uint32_t shade_block;        /* shading block size (64, 128, 256), 0 = tuned or 256 */
     uint32_t shade_blocks_per_cu;/* 0 = tuned or 8 */
     uint32_t trace_blocks_per_cu;/* traversal blocks per CU, 0 = tuned or resident maximum */
+    uint32_t guide_train_spp;    /* with PT_RENDER_GUIDE_TRAIN: each view trains the guide on its first
+                                    guide_train_spp frames of the call (0 = all frames) */
 } pt_batch_params;
 
 typedef struct pt_batch_tuning {
@@ -442,7 +457,7 @@ typedef struct pt_host_scene pt_host_scene;
 enum pt_scene_kind {
     PT_SCENE_CORNELL = 1,   /* closed box, two rotated boxes, one area light */
     PT_SCENE_TERRAIN = 2,   /* fractal heightfield + icospheres, 7 materials, 4 lights + env */
-    PT_SCENE_DOOR_ROOM = 3, /* room lit through a narrow door gap */
+    PT_SCENE_DOOR_ROOM = 3, /* room lit through a narrow door gap (seed > 0: door and camera moved) */
     PT_SCENE_FURNACE = 4,   /* closed emissive diffuse cube seen from its center */
     PT_SCENE_CORNELL_VARIANT = 5 /* Cornell box tessellated to 50K-2M tris, random camera and film size */
 };

@@ -131,7 +131,7 @@ class GuideInfo(C.Structure):
     _fields_ = [("enabled", C.c_uint32), ("log2_table", C.c_uint32), ("steps", C.c_uint32),
                 ("skipped", C.c_uint32), ("loss_scale", C.c_float), ("last_loss", C.c_float),
                 ("last_samples", C.c_uint32), ("resolution", C.c_uint32 * 16), ("dense", C.c_uint32 * 16),
-                ("bounds_lo", C.c_float * 3), ("bounds_inv", C.c_float * 3)]
+                ("bounds_lo", C.c_float * 3), ("bounds_inv", C.c_float * 3), ("grids", C.c_uint32)]
 
     def as_dict(self):
         d = {}
@@ -144,7 +144,8 @@ class GuideInfo(C.Structure):
 class BatchParams(C.Structure):
     _fields_ = [("rr_depth", C.c_uint32), ("mis_mode", C.c_uint32), ("render_flags", C.c_uint32),
                 ("batch_flags", C.c_uint32), ("paths_per_launch", C.c_uint32), ("shade_block", C.c_uint32),
-                ("shade_blocks_per_cu", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32)]
+                ("shade_blocks_per_cu", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
+                ("guide_train_spp", C.c_uint32)]
 
 
 class BatchTuning(C.Structure):
@@ -204,6 +205,9 @@ _SIGNATURES = {
     "pt_guide_disable": (C.c_int, [C.c_void_p]),
     "pt_guide_get_info": (C.c_int, [C.c_void_p, C.POINTER(GuideInfo)]),
     "pt_guide_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "pt_guide_enable_batch": (C.c_int, [C.c_void_p, C.POINTER(GuideConfig)]),
+    "pt_guide_query_grid": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "pt_guide_grid_bounds": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "pt_guide_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
     "pt_guide_gradients": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_void_p, C.c_void_p,
                                      C.POINTER(C.c_float)]),
@@ -492,6 +496,25 @@ class PathTracer:
                           loss_flags, fixed_selection, l2_weight, guided_depth)
         _check(lib().pt_guide_enable(self._ctx, C.byref(cfg)), "pt_guide_enable", self._ctx)
         self._guide_entries = 1 << log2_table
+        self._guide_grids = 1
+
+    def guide_enable_batch(self, log2_table: int = 19, n_min: int = 16, n_max: int = 2048, seed: int = 0,
+                           learning_rate: float = 1e-2, batch_size: int = 16384, steps_per_frame: int = 8,
+                           train_stride: int = 1, loss_flags: int = 0, fixed_selection: float = 0.0,
+                           l2_weight: float = 0.0, guided_depth: int = 0) -> None:
+        """Guide for the batch: one hash grid per batch view, one shared MLP."""
+        cfg = GuideConfig(log2_table, n_min, n_max, seed, learning_rate, batch_size, steps_per_frame, train_stride,
+                          loss_flags, fixed_selection, l2_weight, guided_depth)
+        _check(lib().pt_guide_enable_batch(self._ctx, C.byref(cfg)), "pt_guide_enable_batch", self._ctx)
+        self._guide_entries = 1 << log2_table
+        self._guide_grids = self.guide_info()["grids"]
+
+    def guide_grid_bounds(self, grid: int):
+        """(lo, inv) of hash grid `grid`: u = (p - lo) * inv."""
+        lo = (C.c_float * 3)()
+        inv = (C.c_float * 3)()
+        _check(lib().pt_guide_grid_bounds(self._ctx, grid, lo, inv), "pt_guide_grid_bounds", self._ctx)
+        return np.array(list(lo), np.float32), np.array(list(inv), np.float32)
 
     def guide_disable(self) -> None:
         _check(lib().pt_guide_disable(self._ctx), "pt_guide_disable", self._ctx)
@@ -501,20 +524,27 @@ class PathTracer:
         _check(lib().pt_guide_get_info(self._ctx, C.byref(info)), "pt_guide_get_info", self._ctx)
         return info.as_dict()
 
-    def guide_query(self, positions: np.ndarray) -> np.ndarray:
-        """Raw network outputs (N x 41) at world-space positions (N x 3)."""
+    def guide_query(self, positions: np.ndarray, grid=None) -> np.ndarray:
+        """Raw network outputs (N x 41) at world-space positions (N x 3), in
+        hash grid `grid` (None: pt_guide_query, grid 0)."""
         pos = np.ascontiguousarray(positions, dtype=np.float32).reshape(-1, 3)
         out = np.zeros((pos.shape[0], GUIDE_OUTPUTS), dtype=np.float32)
-        _check(lib().pt_guide_query(self._ctx, pos.ctypes.data, pos.shape[0], out.ctypes.data), "pt_guide_query",
-               self._ctx)
+        if grid is None:
+            _check(lib().pt_guide_query(self._ctx, pos.ctypes.data, pos.shape[0], out.ctypes.data),
+                   "pt_guide_query", self._ctx)
+        else:
+            _check(lib().pt_guide_query_grid(self._ctx, grid, pos.ctypes.data, pos.shape[0], out.ctypes.data),
+                   "pt_guide_query_grid", self._ctx)
         return out
 
     @staticmethod
-    def guide_samples(pos, wdir, target, fs_over_p, p_bsdf, valid=None) -> np.ndarray:
+    def guide_samples(pos, wdir, target, fs_over_p, p_bsdf, valid=None, grid=None) -> np.ndarray:
         """Packs training samples into the N x 12 float layout of the C ABI."""
         n = len(target)
         s = np.zeros((n, 12), dtype=np.float32)
         s[:, 0:3] = pos
+        if grid is not None:
+            s[:, 3] = grid
         s[:, 4:7] = wdir
         s[:, 8] = target
         s[:, 9] = fs_over_p
@@ -532,12 +562,16 @@ class PathTracer:
     def guide_gradients(self, samples: np.ndarray, fp_norm: float = 1.0):
         """(table_grad [16, T, 2], weight_grad [4, 64, 64], mean loss) without a step."""
         s = np.ascontiguousarray(samples, dtype=np.float32)
-        tg = np.zeros((GUIDE_LEVELS, self._guide_entries, 2), dtype=np.float32)
+        tg = np.zeros(self._guide_table_shape(), dtype=np.float32)
         wg = np.zeros((4, 64, 64), dtype=np.float32)
         loss = C.c_float()
         _check(lib().pt_guide_gradients(self._ctx, s.ctypes.data, s.shape[0], fp_norm, tg.ctypes.data,
                                         wg.ctypes.data, C.byref(loss)), "pt_guide_gradients", self._ctx)
         return tg, wg, loss.value
+
+    def _guide_table_shape(self):
+        g = getattr(self, "_guide_grids", 1)
+        return (GUIDE_LEVELS, self._guide_entries, 2) if g == 1 else (g, GUIDE_LEVELS, self._guide_entries, 2)
 
     def guide_read_samples(self, max_samples: int = 1 << 20) -> np.ndarray:
         """Training samples collected by the last training launch (N x 12)."""
@@ -548,7 +582,7 @@ class PathTracer:
         return out[:n.value]
 
     def guide_get_params(self):
-        tb = np.zeros((GUIDE_LEVELS, self._guide_entries, 2), dtype=np.float32)
+        tb = np.zeros(self._guide_table_shape(), dtype=np.float32)
         w = np.zeros((4, 64, 64), dtype=np.float32)
         _check(lib().pt_guide_get_params(self._ctx, tb.ctypes.data, w.ctypes.data), "pt_guide_get_params",
                self._ctx)
@@ -577,21 +611,27 @@ class PathTracer:
 
     @staticmethod
     def _batch_params(rr_depth, mis_mode, fp64, timing, graph, autotune, paths_per_launch, shade_block,
-                      shade_blocks_per_cu, trace_blocks_per_cu):
+                      shade_blocks_per_cu, trace_blocks_per_cu, guide=False, guide_train=False, guide_train_spp=0):
         rf = (RENDER_FP64_ACCUM if fp64 else 0) | (RENDER_TIMING if timing else 0)
+        rf |= (RENDER_GUIDE if guide else 0) | (RENDER_GUIDE_TRAIN if guide_train else 0)
         bf = (BATCH_GRAPH if graph else 0) | (BATCH_AUTOTUNE if autotune else 0)
         return BatchParams(rr_depth, mis_mode, rf, bf, paths_per_launch, shade_block, shade_blocks_per_cu,
-                           trace_blocks_per_cu)
+                           trace_blocks_per_cu, guide_train_spp)
 
     def batch_render(self, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE, fp64: bool = False,
                      timing: bool = False, graph: bool = False, autotune: bool = False, paths_per_launch: int = 0,
-                     shade_block: int = 0, shade_blocks_per_cu: int = 0, trace_blocks_per_cu: int = 0) -> None:
+                     shade_block: int = 0, shade_blocks_per_cu: int = 0, trace_blocks_per_cu: int = 0,
+                     guide: bool = False, guide_train: bool = False, guide_train_spp: int = 0) -> None:
+        """Renders every view's frames.  guide / guide_train (after
+        guide_enable_batch, MIS_CONSISTENT): sample the batch guide / train it
+        on the first guide_train_spp frames of each view (0 = all)."""
         p = self._batch_params(rr_depth, mis_mode, fp64, timing, graph, autotune, paths_per_launch, shade_block,
-                               shade_blocks_per_cu, trace_blocks_per_cu)
+                               shade_blocks_per_cu, trace_blocks_per_cu, guide, guide_train, guide_train_spp)
         _check(lib().pt_batch_render(self._ctx, C.byref(p)), "pt_batch_render", self._ctx)
 
-    def batch_autotune(self, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE, graph: bool = False) -> dict:
-        p = self._batch_params(rr_depth, mis_mode, False, False, graph, False, 0, 0, 0, 0)
+    def batch_autotune(self, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE, graph: bool = False,
+                       guide: bool = False) -> dict:
+        p = self._batch_params(rr_depth, mis_mode, False, False, graph, False, 0, 0, 0, 0, guide)
         t = BatchTuning()
         _check(lib().pt_batch_autotune(self._ctx, C.byref(p), C.byref(t)), "pt_batch_autotune", self._ctx)
         return t.as_dict()

@@ -262,6 +262,30 @@ struct BatchViewDev {
     void *pad2;
 };
 
+// Launch layout of a batched render (one LaunchDesc per launch, copied to
+// the device before the launch's kernels run).
+constexpr uint32_t kMaxLaunchEntries = 1024;  // frames per launch
+constexpr uint32_t kMaxLaunchSegs = 256;      // runs of consecutive frames of one view per launch
+constexpr uint32_t kEntryTrain = 0x80000000u; // LaunchEntry::view flag: the frame trains the guide
+
+// Frame `frame_seed` (RNG frame index) of a view occupies the launch's paths
+// [path_begin, path_begin + pixels).
+struct LaunchEntry {
+    uint32_t view, frame_seed, path_begin, pixels;
+};
+// Consecutive frames [first_entry, first_entry + frames) of one view, folded
+// into its film in frame order; its pixels are threads [pix_begin, +pixels).
+struct LaunchSeg {
+    uint32_t view, first_entry, frames, pix_begin;
+    uint32_t sample_cnt0, accumulate, pad0, pad1;
+};
+struct LaunchDesc {
+    uint32_t total_paths, n_entries, n_segs, seg_pixels;
+    uint32_t epoch, phase, train_frames, pad;  // guide training records of this launch
+    LaunchEntry e[kMaxLaunchEntries];
+    LaunchSeg g[kMaxLaunchSegs];
+};
+
 // With `views` set (batched render) the path id carries the view above bit
 // `path_bits`; the view selects its emitters, environment and depth.
 __global__ __launch_bounds__(256) void k_shade(SceneView scene, ShadeParams prm, PathQueue in,
@@ -378,11 +402,13 @@ __device__ __forceinline__ void unpack_half2(uint32_t v, float &a, float &b) {
 // hit point o + t d, and stores the 41 outputs (FP16, 48 halves = 96 bytes
 // per slot) straight from the MFMA result registers: lane (g, i) of a tile
 // holds outputs 16 rt + 4 g .. +3 of query i.  Waves stride over the queue;
-// a wave with no hit skips the network.
+// a wave with no hit skips the network.  In a batched render the path id's
+// view bits (above `path_bits`) select the query's hash grid.
 constexpr int kInferBlock = 512;
 
 __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQueue q, const uint32_t *count_ptr,
-                                                             uint2 *gout, unsigned long long *queries) {
+                                                             uint2 *gout, unsigned long long *queries,
+                                                             uint32_t path_bits) {
     __shared__ half8 wl[4 * 4 * 2 * 64];
     for (uint32_t t = threadIdx.x; t < 4 * 4 * 2 * 64; t += kInferBlock) wl[t] = G.fwd_a[t];
     __syncthreads();
@@ -397,15 +423,17 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQu
         const uint32_t i = base + lane;
         bool valid = i < count;
         f3 p = mk3(0.f);
+        uint32_t grid = 0u;
         if (valid) {
             const float4 r0 = q.ray0[i], r1 = q.ray1[i], h = q.hit[i];
             valid = f2u(h.y) != kMissPrim;
             if (valid) p = f3of(r0) + f3of(r1) * h.x;
+            grid = min(f2u(r0.w) >> path_bits, G.grids - 1u);
         }
         const unsigned long long vmask = __ballot(valid);
         if (vmask == 0ull) continue;
         nq += (uint32_t)__popcll(vmask);
-        const f3 u = guide_normalize_pos(G, p);
+        const f3 u = guide_grid_pos(G, grid, p);
         const int vbit = valid ? 1 : 0;
 #pragma unroll 1
         for (int qt = 0; qt < 4; ++qt) {
@@ -414,7 +442,7 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQu
             asm volatile("" ::: "memory");
             const int qi = qt * 16 + il;
             const f3 uq = mk3(__shfl(u.x, qi), __shfl(u.y, qi), __shfl(u.z, qi));
-            const half8 in0 = guide_encode(G, uq, g);
+            const half8 in0 = guide_encode(G, uq, g, (uint32_t)__shfl((int)grid, qi));
             floatx4 out[4];
             guide_mlp_tile_shared(wl, lane, in0, out);
             if (__shfl(vbit, qi)) {
@@ -429,13 +457,24 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQu
 }
 
 // k_shade with the guided step; the mixture of each guidable vertex comes
-// from k_guide_infer's outputs for the same queue slot.
+// from k_guide_infer's outputs for the same queue slot.  With `views` set
+// (batched render) the path id carries the view above bit `path_bits`, as in
+// k_shade, the view is also the hash grid its records name, and the launch's
+// record epoch and phase come from `ld` (so a captured launch replays with
+// the values of each launch).
+template <bool kBatch>
 __global__ __launch_bounds__(256) void k_shade_guided(SceneView scene, ShadeParams prm, PathQueue in,
                                                       const uint32_t *count_in, PathQueue out, uint32_t *count_out,
                                                       ShadowQueue sq, uint32_t *count_shadow, float4 *out_l,
-                                                      AovBuffers aov, GuideShadeArgs ga) {
+                                                      AovBuffers aov, GuideShadeArgs ga, const BatchViewDev *views,
+                                                      uint32_t path_bits, const LaunchDesc *ld) {
     const uint32_t count = *count_in;
     const uint32_t stride = gridDim.x * blockDim.x;
+    RecordSink rs = ga.rs;
+    if constexpr (kBatch) {
+        rs.epoch = ld->epoch;
+        rs.phase = ld->phase;
+    }
     for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < count;
@@ -444,12 +483,13 @@ __global__ __launch_bounds__(256) void k_shade_guided(SceneView scene, ShadePara
         so.cont = so.done = so.shadow = so.wrote_aov = false;
         GuidedVertex v;
         v.guidable = false;
-        uint32_t path = 0, prim = kMissPrim;
+        uint32_t path = 0, packed = 0, vbits = 0, view = 0, prim = kMissPrim;
         bool scatter = false;
+        SceneView sv = scene;
         if (valid) {
             const float4 r0 = in.ray0[i], r1 = in.ray1[i], tl = in.tl[i], lr = in.lr[i], h = in.hit[i];
             ps.o = f3of(r0);
-            path = f2u(r0.w);
+            packed = path = f2u(r0.w);
             ps.d = f3of(r1);
             ps.prev_pdf = r1.w;
             ps.T = f3of(tl);
@@ -457,7 +497,19 @@ __global__ __launch_bounds__(256) void k_shade_guided(SceneView scene, ShadePara
             ps.rng.seed = f2u(lr.z);
             ps.flags = f2u(lr.w);
             prim = f2u(h.y);
-            scatter = guided_begin(scene, prm, ps, prim, h.x, h.z, h.w, ga.rs, path, v, so);
+            ShadeParams pv = prm;
+            if constexpr (kBatch) {
+                view = packed >> path_bits;
+                path = packed & ((1u << path_bits) - 1u);
+                vbits = view << path_bits;
+                const BatchViewDev &bv = views[view];
+                sv.emitters += bv.emitter_begin;
+                sv.emitter_cdf += bv.emitter_begin;
+                sv.num_emitters = bv.emitter_count;
+                sv.env = bv.env;
+                pv.max_depth = bv.max_depth;
+            }
+            scatter = guided_begin(sv, pv, ps, prim, h.x, h.z, h.w, rs, path, v, so);
         }
         bool use_guide = ga.gout != nullptr && scatter && v.guidable;
         GuideMixture m;
@@ -475,7 +527,7 @@ __global__ __launch_bounds__(256) void k_shade_guided(SceneView scene, ShadePara
             if (ga.fixed_c > 0.f) m.c = ga.fixed_c;
             if (!(m.c >= 0.f && m.c <= 1.f) || !(m.wsum > 0.f) || !isfinite(m.wsum)) use_guide = false;
         }
-        if (scatter) guided_scatter(scene, ps, v, use_guide, m, ga.rs, path, so);
+        if (scatter) guided_scatter(sv, ps, v, use_guide, m, rs, path, so, view);
         const uint32_t j = wave_alloc(valid && so.cont, count_out);
         const uint32_t k = wave_alloc(valid && so.shadow, count_shadow);
         if (!valid) continue;
@@ -494,7 +546,7 @@ __global__ __launch_bounds__(256) void k_shade_guided(SceneView scene, ShadePara
             }
         }
         if (so.cont) {
-            out.ray0[j] = make_float4(ps.o.x, ps.o.y, ps.o.z, u2f(path));
+            out.ray0[j] = make_float4(ps.o.x, ps.o.y, ps.o.z, u2f(packed));
             out.ray1[j] = make_float4(ps.d.x, ps.d.y, ps.d.z, ps.prev_pdf);
             out.tl[j] = make_float4(ps.T.x, ps.T.y, ps.T.z, ps.L.x);
             out.lr[j] = make_float4(ps.L.y, ps.L.z, u2f(ps.rng.seed), u2f(ps.flags));
@@ -502,7 +554,7 @@ __global__ __launch_bounds__(256) void k_shade_guided(SceneView scene, ShadePara
             out_l[path] = make_float4(ps.L.x, ps.L.y, ps.L.z, 1.f);
         }
         if (so.shadow) {
-            const uint32_t target = so.cont ? j : (0x80000000u | path);
+            const uint32_t target = so.cont ? (vbits | j) : (0x80000000u | vbits | path);
             sq.s0[k] = make_float4(so.so.x, so.so.y, so.so.z, so.stmax);
             sq.s1[k] = make_float4(so.sd.x, so.sd.y, so.sd.z, u2f(target));
             sq.s2[k] = make_float4(so.contrib.x, so.contrib.y, so.contrib.z, 0.f);
@@ -549,26 +601,6 @@ __global__ void k_accumulate(const float4 *out_l, float4 *accum, float4 *frame, 
 }
 
 // --- batched views ----------------------------------------------------------------
-constexpr uint32_t kMaxLaunchEntries = 1024;  // frames per launch
-constexpr uint32_t kMaxLaunchSegs = 256;      // runs of consecutive frames of one view per launch
-
-// Frame `frame_seed` (RNG frame index) of a view occupies the launch's paths
-// [path_begin, path_begin + pixels).
-struct LaunchEntry {
-    uint32_t view, frame_seed, path_begin, pixels;
-};
-// Consecutive frames [first_entry, first_entry + frames) of one view, folded
-// into its film in frame order; its pixels are threads [pix_begin, +pixels).
-struct LaunchSeg {
-    uint32_t view, first_entry, frames, pix_begin;
-    uint32_t sample_cnt0, accumulate, pad0, pad1;
-};
-struct LaunchDesc {
-    uint32_t total_paths, n_entries, n_segs, seg_pixels;
-    LaunchEntry e[kMaxLaunchEntries];
-    LaunchSeg g[kMaxLaunchSegs];
-};
-
 // main.cu:39-78 for every (view, frame, pixel) of a batched launch.
 __global__ void k_raygen_batch(PathQueue q, const LaunchDesc *ld, const BatchViewDev *views, uint32_t path_bits,
                                uint32_t *counters, uint32_t num_counters) {
@@ -587,7 +619,8 @@ __global__ void k_raygen_batch(PathQueue q, const LaunchDesc *ld, const BatchVie
             else hi = mid - 1u;
         }
         const LaunchEntry en = ld->e[lo];
-        const BatchViewDev &v = views[en.view];
+        const uint32_t view = en.view & ~kEntryTrain;
+        const BatchViewDev &v = views[view];
         const uint32_t pixel = i - en.path_begin;
         const uint32_t x = pixel % v.width, y = pixel / v.width;
         Rng rng = rng_init(pixel, en.frame_seed);
@@ -595,10 +628,11 @@ __global__ void k_raygen_batch(PathQueue q, const LaunchDesc *ld, const BatchVie
         const float jy = rng_next(rng);
         f3 o, d;
         camera_ray(v.camera, x, y, v.width, v.height, jx, jy, o, d);
-        q.ray0[i] = make_float4(o.x, o.y, o.z, u2f((en.view << path_bits) | i));
+        const uint32_t flags = kPathPrimary | ((en.view & kEntryTrain) ? 0u : kPathNoTrain);
+        q.ray0[i] = make_float4(o.x, o.y, o.z, u2f((view << path_bits) | i));
         q.ray1[i] = make_float4(d.x, d.y, d.z, 0.f);
         q.tl[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-        q.lr[i] = make_float4(0.f, 0.f, u2f(rng.seed), u2f(kPathPrimary));
+        q.lr[i] = make_float4(0.f, 0.f, u2f(rng.seed), u2f(flags));
     }
 }
 
@@ -824,6 +858,7 @@ __global__ void k_refit_nodes(WideNode *nodes, float4 *boxes, const LeafTri *lea
 // One view of a batched render (host side).
 struct BatchView {
     uint32_t width = 0, height = 0, spp = 0, max_depth = 1;
+    float lo[3]{}, hi[3]{};  // scene bounds (the view's guide hash grid covers them)
     CameraRec camera{};
     EnvRec env{};
     uint32_t root = 0, emitter_begin = 0, emitter_count = 0, tri_count = 0;
@@ -1774,8 +1809,10 @@ int batch_upload(pt_ctx *ctx) {
 }
 
 // Packs every view's frames (views in order, frames in order) into launches
-// of at most `capacity` paths.
-void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::vector<LaunchDesc> &plan) {
+// of at most `capacity` paths.  With `train`, frame f of a view trains the
+// guide when train_spp == 0 or f < train_spp.
+void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::vector<LaunchDesc> &plan,
+                bool train = false, uint32_t train_spp = 0) {
     plan.clear();
     LaunchDesc cur;
     std::memset(&cur, 0, sizeof(cur));
@@ -1793,7 +1830,9 @@ void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::ve
                 flush();
             const bool seg_start = cur.n_segs == 0 || cur.g[cur.n_segs - 1].view != vi;
             LaunchEntry &e = cur.e[cur.n_entries];
-            e.view = vi;
+            const bool train_frame = train && (train_spp == 0 || f < train_spp);
+            e.view = vi | (train_frame ? kEntryTrain : 0u);
+            if (train_frame) cur.train_frames += 1;
             e.frame_seed = v.random_seed + f;
             e.path_begin = cur.total_paths;
             e.pixels = npix;
@@ -1852,9 +1891,24 @@ struct BatchShape {
     uint32_t capacity, shade_block, shade_bpc, trace_bpc;
 };
 
+// Training samples for `steps` optimizer steps of `batch` samples (capped at
+// 2^28 samples, 12 GB; the optimizer then runs fewer steps).
+uint32_t guide_sample_count(uint64_t steps, uint32_t batch) {
+    return (uint32_t)std::min<uint64_t>(steps * batch, 1ull << 28);
+}
+
+// Guided batched render: network outputs per queue slot and the record sink
+// (epoch and phase of each launch are read from its LaunchDesc).
+struct BatchGuide {
+    bool infer = false, train = false;
+    GuideView view{};
+    GuideShadeArgs ga{};
+    uint32_t gdepth = 0;  // vertices below this depth may sample the guide
+};
+
 // The kernels of one launch (the sequence a hipGraph captures).
 void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth, const ShadeParams &sp,
-                          uint32_t path_bits, bool fp64) {
+                          uint32_t path_bits, bool fp64, const BatchGuide &bg) {
     BatchState &b = ctx->batch;
     SceneView sv;
     sv.tri = b.d_tri;
@@ -1886,9 +1940,23 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
         hipLaunchKernelGGL(trace_kernel(false, !b.spheres.empty()), dim3(tg), dim3(kTraceBlock), 0, ctx->stream,
                            ctx->q[cur], bvh, ext + bb, twork + bb, ctx->d_stat, (const uint32_t *)b.d_roots,
                            path_bits);
-        hipLaunchKernelGGL(k_shade, dim3(sg), dim3(sh.shade_block), 0, ctx->stream, sv, sp, ctx->q[cur], ext + bb,
-                           ctx->q[nxt], ext + bb + 1, ctx->sq, shd + bb, ctx->d_out_l, aov,
-                           (const BatchViewDev *)b.d_views, path_bits);
+        if (bg.infer || bg.train) {
+            GuideShadeArgs ga = bg.ga;
+            const bool infer_here = bg.infer && bb < bg.gdepth && bb + 1 < max_depth;
+            if (!infer_here) ga.gout = nullptr;
+            if (infer_here)
+                hipLaunchKernelGGL(k_guide_infer, dim3(ctx->num_cu * ctx->infer_blocks_per_cu), dim3(kInferBlock), 0,
+                                   ctx->stream, bg.view, ctx->q[cur], ext + bb,
+                                   reinterpret_cast<uint2 *>(ctx->d_gout), ctx->d_stat + 4, path_bits);
+            hipLaunchKernelGGL(k_shade_guided<true>, dim3((uint32_t)ctx->num_cu * sh.shade_bpc), dim3(256), 0,
+                               ctx->stream, sv, sp, ctx->q[cur], ext + bb, ctx->q[nxt], ext + bb + 1, ctx->sq,
+                               shd + bb, ctx->d_out_l, aov, ga, (const BatchViewDev *)b.d_views, path_bits,
+                               (const LaunchDesc *)b.d_launch);
+        } else {
+            hipLaunchKernelGGL(k_shade, dim3(sg), dim3(sh.shade_block), 0, ctx->stream, sv, sp, ctx->q[cur],
+                               ext + bb, ctx->q[nxt], ext + bb + 1, ctx->sq, shd + bb, ctx->d_out_l, aov,
+                               (const BatchViewDev *)b.d_views, path_bits);
+        }
         hipLaunchKernelGGL(shadow_kernel(false, !b.spheres.empty()), dim3(tg), dim3(kTraceBlock), 0, ctx->stream,
                            ctx->sq, bvh, shd + bb, swork + bb, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2,
                            (const uint32_t *)b.d_roots, path_bits);
@@ -1897,6 +1965,58 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
     hipLaunchKernelGGL(k_accumulate_batch, dim3((uint32_t)ctx->num_cu * 16u), dim3(256), 0, ctx->stream,
                        ctx->d_out_l, b.d_launch, b.d_views, fp64 ? 1u : 0u);
     hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters, max_depth, b.d_rays);
+}
+
+// Checks and allocations of a guided batched render: network outputs per
+// queue slot, training records for the launch width, samples for the
+// optimizer steps of the fullest training launch.
+int batch_guide_setup(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, uint32_t max_depth,
+                      BatchGuide &bg) {
+    if (!bg.infer && !bg.train) return PT_OK;
+    GuideState &gs = ctx->guide;
+    if (!gs.enabled || !gs.for_batch || gs.grids != ctx->batch.views.size()) {
+        ctx->error = "pt_batch_render: guided rendering needs pt_guide_enable_batch";
+        return PT_ERR_NO_GUIDE;
+    }
+    if (p.mis_mode != PT_MIS_CONSISTENT) {
+        ctx->error = "pt_batch_render: guided rendering needs PT_MIS_CONSISTENT";
+        return PT_ERR_INVALID_ARG;
+    }
+    bg.view = gs.view();
+    std::memset(&bg.ga, 0, sizeof(bg.ga));
+    bg.ga.fixed_c = std::min(gs.cfg.fixed_selection, 1.f);
+    bg.gdepth = gs.cfg.guided_depth ? gs.cfg.guided_depth : max_depth;
+    if (bg.infer) {
+        if (ctx->gout_cap < sh.capacity) {
+            dfree(ctx->d_gout);
+            PT_HIP(hipMalloc((void **)&ctx->d_gout, (size_t)sh.capacity * 6 * sizeof(uint4)));
+            ctx->gout_cap = sh.capacity;
+        }
+        bg.ga.gout = ctx->d_gout;
+    }
+    if (bg.train) {
+        const pt_guide_config &gc = gs.cfg;
+        const uint64_t slots = ((uint64_t)sh.capacity + gc.train_stride - 1) / gc.train_stride;
+        const uint64_t records = slots * max_depth;
+        if (records > 0xFFFFFFFFull / 2) {
+            ctx->error = "pt_batch_render: too many guide training records per launch (raise train_stride)";
+            return PT_ERR_CAPACITY;
+        }
+        uint32_t max_frames = 1;
+        for (const auto &v : ctx->batch.views)
+            max_frames = std::max(max_frames, std::min(kMaxLaunchEntries, sh.capacity / (v.width * v.height)));
+        if (!gs.ensure_records((uint32_t)records, ctx->error) ||
+            !gs.ensure_samples(guide_sample_count((uint64_t)gc.steps_per_frame * max_frames, gc.batch_size),
+                               ctx->error))
+            return PT_ERR_HIP;
+        bg.ga.rs.rec = reinterpret_cast<f4 *>(gs.rec);
+        bg.ga.rs.capacity = gs.rec_cap;
+        bg.ga.rs.stride = gc.train_stride;
+        bg.ga.rs.phase = 0;  // per launch from its LaunchDesc
+        bg.ga.rs.max_depth = max_depth;
+        bg.ga.rs.epoch = 0;
+    }
+    return PT_OK;
 }
 
 // Renders the batch's frames with one launch shape.  Returns PT_OK and the
@@ -1920,8 +2040,11 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     }
     if ((rc = ensure_queues(ctx, sh.capacity, max_depth)) != PT_OK) return rc;
     if ((rc = ensure_spill(ctx)) != PT_OK) return rc;
+    BatchGuide bg;
+    bg.infer = (p.render_flags & PT_RENDER_GUIDE) != 0;
+    bg.train = (p.render_flags & PT_RENDER_GUIDE_TRAIN) != 0;
     std::vector<LaunchDesc> plan;
-    batch_plan(b, sh.capacity, true, plan);
+    batch_plan(b, sh.capacity, true, plan, bg.train, p.guide_train_spp);
     if (plan.size() > b.h_launch_slots) {
         if (b.h_launch) (void)hipHostFree(b.h_launch);
         b.h_launch = nullptr;
@@ -1935,11 +2058,12 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     spar.mis_mode = p.mis_mode;
     spar.aov = 0;
     const bool fp64 = (p.render_flags & PT_RENDER_FP64_ACCUM) != 0;
+    if ((rc = batch_guide_setup(ctx, p, sh, max_depth, bg)) != PT_OK) return rc;
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     if (graph) {
         PT_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-        batch_enqueue_launch(ctx, sh, max_depth, spar, path_bits, fp64);
+        batch_enqueue_launch(ctx, sh, max_depth, spar, path_bits, fp64, bg);
         hipError_t e = hipStreamEndCapture(ctx->stream, &g);
         if (e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
         if (e != hipSuccess) {
@@ -1949,15 +2073,43 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
         }
     }
     rc = PT_OK;
+    GuideState &gs = ctx->guide;
     for (size_t k = 0; k < plan.size() && rc == PT_OK; ++k) {
-        hipError_t e = hipMemcpyAsync(b.d_launch, b.h_launch + k, sizeof(LaunchDesc), hipMemcpyHostToDevice,
-                                      ctx->stream);
+        LaunchDesc &hl = b.h_launch[k];
+        const bool train_here = bg.train && hl.train_frames > 0;
+        if (train_here) {
+            gs.epoch += 1u;
+            hl.epoch = gs.epoch;
+            hl.phase = gs.phase;
+        }
+        hipError_t e = hipMemcpyAsync(b.d_launch, &hl, sizeof(LaunchDesc), hipMemcpyHostToDevice, ctx->stream);
         if (e == hipSuccess) {
             if (graph) e = hipGraphLaunch(ge, ctx->stream);
             else {
-                batch_enqueue_launch(ctx, sh, max_depth, spar, path_bits, fp64);
+                batch_enqueue_launch(ctx, sh, max_depth, spar, path_bits, fp64, bg);
                 e = hipGetLastError();
             }
+        }
+        if (e == hipSuccess && train_here) {
+            // Samples from this launch's training frames, then the optimizer
+            // steps (Adam over the hash grids of the views that trained).
+            uint32_t v_lo = 0xFFFFFFFFu, v_hi = 0;
+            for (uint32_t en = 0; en < hl.n_entries; ++en)
+                if (hl.e[en].view & kEntryTrain) {
+                    const uint32_t v = hl.e[en].view & ~kEntryTrain;
+                    v_lo = std::min(v_lo, v);
+                    v_hi = std::max(v_hi, v + 1u);
+                }
+            const GuideRecords r{gs.rec, gs.rec_cap, gs.cfg.train_stride, hl.phase, max_depth, hl.epoch};
+            const uint32_t steps = gs.cfg.steps_per_frame * hl.train_frames;
+            if (!gs.ensure_samples(guide_sample_count(steps, gs.cfg.batch_size), ctx->error) ||
+                !gs.collect(ctx->d_out_l, hl.total_paths, r, ctx->error) ||
+                !gs.train(steps, 0.f, v_lo, v_hi, ctx->error)) {
+                rc = PT_ERR_HIP;
+                break;
+            }
+            gs.phase = (gs.phase + 1u) % gs.cfg.train_stride;
+            ctx->stats.guide_steps += steps;
         }
         if (e != hipSuccess) {
             ctx->error = std::string("pt_batch_render: ") + hipGetErrorString(e);
@@ -2493,7 +2645,7 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
     const bool guide_infer = (p->flags & PT_RENDER_GUIDE) != 0;
     const bool guide_train = (p->flags & PT_RENDER_GUIDE_TRAIN) != 0;
     const bool guided = guide_infer || guide_train;
-    if (guided && !ctx->guide.enabled) {
+    if (guided && (!ctx->guide.enabled || ctx->guide.for_batch)) {
         ctx->error = "pt_render: guided rendering needs pt_guide_enable";
         return PT_ERR_NO_GUIDE;
     }
@@ -2570,7 +2722,8 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
             return PT_ERR_CAPACITY;
         }
         if (!ctx->guide.ensure_records((uint32_t)records, ctx->error) ||
-            !ctx->guide.ensure_samples(gc.steps_per_frame * per_launch * gc.batch_size, ctx->error))
+            !ctx->guide.ensure_samples(guide_sample_count((uint64_t)gc.steps_per_frame * per_launch, gc.batch_size),
+                                       ctx->error))
             return PT_ERR_HIP;
     }
 
@@ -2626,14 +2779,15 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
                 EventTimer t(ctx, timing);
                 hipLaunchKernelGGL(k_guide_infer, dim3(ctx->num_cu * ctx->infer_blocks_per_cu), dim3(kInferBlock), 0,
                                    ctx->stream, gview, ctx->q[cur], ext + b, reinterpret_cast<uint2 *>(ctx->d_gout),
-                                   ctx->d_stat + 4);
+                                   ctx->d_stat + 4, 31u);
                 t.stop(kKindGuideInfer);
             }
             {
                 EventTimer t(ctx, timing);
                 if (guided)
-                    hipLaunchKernelGGL(k_shade_guided, dim3(sg), dim3(256), 0, ctx->stream, sv, sp, ctx->q[cur],
-                                       ext + b, ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov, ga);
+                    hipLaunchKernelGGL(k_shade_guided<false>, dim3(sg), dim3(256), 0, ctx->stream, sv, sp, ctx->q[cur],
+                                       ext + b, ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov, ga,
+                                       (const BatchViewDev *)nullptr, 31u, (const LaunchDesc *)nullptr);
                 else
                     hipLaunchKernelGGL(k_shade, dim3(sg), dim3(256), 0, ctx->stream, sv, sp, ctx->q[cur], ext + b,
                                        ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov,
@@ -2663,7 +2817,9 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
             EventTimer t(ctx, timing);
             GuideState &gs = ctx->guide;
             GuideRecords r{gs.rec, gs.rec_cap, ga.rs.stride, ga.rs.phase, max_depth, gs.epoch};
-            if (!gs.collect(ctx->d_out_l, total, r, ctx->error) ||
+            if (!gs.ensure_samples(guide_sample_count((uint64_t)gs.cfg.steps_per_frame * frames, gs.cfg.batch_size),
+                                   ctx->error) ||
+                !gs.collect(ctx->d_out_l, total, r, ctx->error) ||
                 !gs.train(gs.cfg.steps_per_frame * frames, 0.f, ctx->error))
                 return PT_ERR_HIP;
             gs.phase = (gs.phase + 1u) % gs.cfg.train_stride;
@@ -2865,11 +3021,63 @@ int pt_guide_enable(pt_ctx *ctx, const pt_guide_config *config) {
     pt_guide_config c;
     std::memset(&c, 0, sizeof(c));
     if (config) c = *config;
-    if (!ctx->guide.init(c, ctx->scene_lo, ctx->scene_hi, ctx->stream, ctx->error)) {
+    if (!ctx->guide.init(c, ctx->scene_lo, ctx->scene_hi, 1u, ctx->stream, ctx->error)) {
         ctx->guide.release();
         return ctx->error.rfind("pt_guide_enable", 0) == 0 ? PT_ERR_INVALID_ARG : PT_ERR_HIP;
     }
+    ctx->guide.for_batch = false;
     return PT_OK;
+}
+
+int pt_guide_enable_batch(pt_ctx *ctx, const pt_guide_config *config) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    const BatchState &b = ctx->batch;
+    if (b.views.empty()) {
+        ctx->error = "pt_guide_enable_batch: no batch views";
+        return PT_ERR_NO_SCENE;
+    }
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    pt_guide_config c;
+    std::memset(&c, 0, sizeof(c));
+    if (config) c = *config;
+    std::vector<float> lo(3 * b.views.size()), hi(3 * b.views.size());
+    for (size_t v = 0; v < b.views.size(); ++v)
+        for (int k = 0; k < 3; ++k) {
+            lo[3 * v + k] = b.views[v].lo[k];
+            hi[3 * v + k] = b.views[v].hi[k];
+        }
+    if (!ctx->guide.init(c, lo.data(), hi.data(), (uint32_t)b.views.size(), ctx->stream, ctx->error)) {
+        ctx->guide.release();
+        if (ctx->error.rfind("pt_guide_enable", 0) == 0) {
+            ctx->error.insert(std::strlen("pt_guide_enable"), "_batch");
+            return PT_ERR_INVALID_ARG;
+        }
+        return PT_ERR_HIP;
+    }
+    ctx->guide.for_batch = true;
+    return PT_OK;
+}
+
+int pt_guide_query_grid(pt_ctx *ctx, uint32_t grid, const float *positions, uint32_t n, float *outputs) {
+    int rc = guide_ready(ctx, "pt_guide_query_grid");
+    if (rc != PT_OK) return rc;
+    if (n && (!positions || !outputs)) {
+        ctx->error = "pt_guide_query_grid: null buffer";
+        return PT_ERR_INVALID_ARG;
+    }
+    if (grid >= ctx->guide.grids) {
+        ctx->error = "pt_guide_query_grid: no such hash grid";
+        return PT_ERR_INVALID_ARG;
+    }
+    return ctx->guide.query(positions, n, outputs, grid, ctx->error) ? PT_OK : PT_ERR_HIP;
+}
+
+int pt_guide_grid_bounds(pt_ctx *ctx, uint32_t grid, float lo[3], float inv[3]) {
+    int rc = guide_ready(ctx, "pt_guide_grid_bounds");
+    if (rc != PT_OK) return rc;
+    if (!lo || !inv) return PT_ERR_INVALID_ARG;
+    return ctx->guide.grid_box(grid, lo, inv, ctx->error) ? PT_OK : PT_ERR_INVALID_ARG;
 }
 
 int pt_guide_disable(pt_ctx *ctx) {
@@ -2893,7 +3101,7 @@ int pt_guide_query(pt_ctx *ctx, const float *positions, uint32_t n, float *outpu
         ctx->error = "pt_guide_query: null buffer";
         return PT_ERR_INVALID_ARG;
     }
-    return ctx->guide.query(positions, n, outputs, ctx->error) ? PT_OK : PT_ERR_HIP;
+    return ctx->guide.query(positions, n, outputs, 0u, ctx->error) ? PT_OK : PT_ERR_HIP;
 }
 
 static int guide_check_samples(pt_ctx *ctx, const char *who, const float *samples, uint32_t n) {
@@ -2969,6 +3177,7 @@ int pt_batch_clear(pt_ctx *ctx) {
     if (!ctx) return PT_ERR_INVALID_ARG;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->guide.for_batch) ctx->guide.release();
     batch_free_all(ctx);
     return PT_OK;
 }
@@ -3060,10 +3269,19 @@ int pt_batch_add_view(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uin
     b.materials.insert(b.materials.end(), d.h_materials.begin(), d.h_materials.end());
     b.emitters.insert(b.emitters.end(), d.h_emitters.begin(), d.h_emitters.end());
     b.cdf.insert(b.cdf.end(), d.h_cdf.begin(), d.h_cdf.end());
+    if (ctx->guide.for_batch) {
+        // The batch guide has one hash grid per view: a new view disables it.
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->guide.release();
+    }
     BatchView v;
     v.width = d.width;
     v.height = d.height;
     v.spp = spp;
+    for (int k = 0; k < 3; ++k) {
+        v.lo[k] = d.lo[k];
+        v.hi[k] = d.hi[k];
+    }
     v.max_depth = d.max_depth;
     v.camera = d.camera;
     v.env = d.env;
@@ -3174,7 +3392,7 @@ int pt_batch_autotune(pt_ctx *ctx, const pt_batch_params *params, pt_batch_tunin
     }
     const uint32_t path_bits = batch_path_bits(b.views.size());
     pt_batch_params p = *params;
-    p.render_flags &= ~(uint32_t)PT_RENDER_FP64_ACCUM;
+    p.render_flags &= ~(uint32_t)(PT_RENDER_FP64_ACCUM | PT_RENDER_GUIDE_TRAIN);  // probes leave the guide as is
     uint32_t candidates = 0;
     int rc = PT_OK;
     auto time_shape = [&](const BatchShape &sh, double &ns) -> int {

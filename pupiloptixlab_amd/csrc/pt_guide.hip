@@ -117,12 +117,20 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideView G, const 
     const float4 p4 = samples[3 * (size_t)me];
     const float4 d4 = samples[3 * (size_t)me + 1];
     const float4 aux = samples[3 * (size_t)me + 2];
-    const float weight = in_range ? aux.w : 0.f;
+    // With several hash grids, position.w names the sample's grid; a sample
+    // naming no grid is unused.
+    uint32_t grid = 0u;
+    bool grid_ok = true;
+    if (G.grids > 1u) {
+        grid_ok = p4.w >= 0.f && p4.w < (float)G.grids;
+        grid = grid_ok ? (uint32_t)p4.w : 0u;
+    }
+    const float weight = (in_range && grid_ok) ? aux.w : 0.f;
     const bool valid = weight > 0.f;
-    const f3 u = guide_normalize_pos(G, mk3(p4.x, p4.y, p4.z));
+    const f3 u = guide_grid_pos(G, grid, mk3(p4.x, p4.y, p4.z));
 
     // Forward (inputs of every layer kept for the backward pass).
-    const half8 in0 = guide_encode(G, u, g);
+    const half8 in0 = guide_encode(G, u, g, grid);
     half8 act[4][2];
     floatx4 out[4];
     guide_mlp_tile(G, lane, in0, out, act);
@@ -287,7 +295,7 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideView G, const 
                 const int level = guide_lane_level(g, t);
                 LevelCorners lc;
                 level_corners(G, level, u, lc);
-                float2 *tg = B.table_grad + ((size_t)level << G.log2t);
+                float2 *tg = B.table_grad + guide_level_base(G, grid, level);
 #pragma unroll
                 for (int k = 0; k < 8; ++k) {
                     unsafeAtomicAdd(&tg[lc.idx[k]].x, lc.w[k] * g0);
@@ -485,6 +493,20 @@ __global__ void k_guide_pack(const float *w, _Float16 *fwd, _Float16 *bwd) {
     bwd[t] = (_Float16)(kb < in_dim ? W[kb * 64 + kf] : 0.f);
 }
 
+// Table initialization U(-1e-4, 1e-4), a counter-based draw per entry
+// (SplitMix64 of seed and index), so tables of any size start on the device.
+__global__ void k_guide_init_table(float2 *p, size_t n, uint64_t seed) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        uint64_t z = seed ^ ((uint64_t)e * 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        const float a = (float)(uint32_t)(z >> 40) * (1.f / 16777216.f);
+        const float b = (float)(uint32_t)(z & 0xFFFFFFu) * (1.f / 16777216.f);
+        p[e] = make_float2((a * 2.f - 1.f) * 1e-4f, (b * 2.f - 1.f) * 1e-4f);
+    }
+}
+
 __global__ void k_guide_fp16_table(const float2 *p, uint32_t *p16, size_t n) {
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
         const float2 v = p[e];
@@ -492,14 +514,15 @@ __global__ void k_guide_fp16_table(const float2 *p, uint32_t *p16, size_t n) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_guide_query(GuideView G, const float *pos, uint32_t n, float *out) {
+__global__ __launch_bounds__(256) void k_guide_query(GuideView G, const float *pos, uint32_t n, float *out,
+                                                     uint32_t grid) {
     __shared__ float stage[4][64 * kGuideUsed];
     const int wave = threadIdx.x >> 6;
     const int lane = (int)__lane_id();
     const uint32_t base = (blockIdx.x * 4 + wave) * 64;
     if (base >= n) return;
     const uint32_t me = min(base + lane, n - 1);
-    guide_infer_wave(G, mk3(pos[3 * me], pos[3 * me + 1], pos[3 * me + 2]), stage[wave]);
+    guide_infer_wave(G, mk3(pos[3 * me], pos[3 * me + 1], pos[3 * me + 2]), stage[wave], grid);
     wave_sync_lds();
     if (base + lane < n)
         for (int k = 0; k < kGuideUsed; ++k)
@@ -549,7 +572,7 @@ __device__ __forceinline__ bool build_sample(const GuideRecords &r, uint32_t idx
     target = luminance(mk3(ix, iy, iz)) * r1.w;
     fpv = luminance(mk3(r3.x * ix, r3.y * iy, r3.z * iz));
     if (!isfinite(target) || !isfinite(fpv)) return false;
-    s0 = make_float4(r0.x, r0.y, r0.z, 0.f);
+    s0 = make_float4(r0.x, r0.y, r0.z, r2.w);  // r2.w: the vertex's hash grid
     s1 = make_float4(r1.x, r1.y, r1.z, 0.f);
     pb = r0.w;
     return true;
@@ -690,13 +713,21 @@ void GuideState::release() {
     gfree(dyt);
     gfree(samples);
     gfree(rec);
-    batch_cap = sample_cap = rec_cap = 0;
+    gfree(d_grid_bounds);
+    batch_cap = sample_cap = sample_fill = rec_cap = 0;
+    grids = 1;
+    grid_bounds.clear();
     enabled = false;
+    for_batch = false;
 }
 
-bool GuideState::init(const pt_guide_config &c, const float lo[3], const float hi[3], hipStream_t s,
+bool GuideState::init(const pt_guide_config &c, const float *lo, const float *hi, uint32_t n_grids, hipStream_t s,
                       std::string &err) {
     release();
+    if (n_grids == 0) {
+        err = "pt_guide_enable: no hash grids";
+        return false;
+    }
     stream = s;
     cfg = c;
     if (cfg.log2_table == 0) cfg.log2_table = 19;
@@ -727,18 +758,30 @@ bool GuideState::init(const pt_guide_config &c, const float lo[3], const float h
         const uint64_t side = levels[l].res + 1ull;
         levels[l].dense = side * side * side <= T ? 1u : 0u;
     }
-    for (int k = 0; k < 3; ++k) {
-        if (!(hi[k] >= lo[k])) {
-            err = "pt_guide_enable: invalid scene bounds";
-            return false;
+    // Per grid: the scene box padded by 1e-3 of its extent (at least 1e-4).
+    grid_bounds.assign((size_t)n_grids * 8, 0.f);
+    for (uint32_t gi = 0; gi < n_grids; ++gi)
+        for (int k = 0; k < 3; ++k) {
+            const float l = lo[3 * gi + k], h = hi[3 * gi + k];
+            if (!(h >= l) || !std::isfinite(l) || !std::isfinite(h)) {
+                err = "pt_guide_enable: invalid scene bounds";
+                return false;
+            }
+            const float ext = h - l;
+            const float pad = std::max(ext * 1e-3f, 1e-4f);
+            grid_bounds[8 * gi + k] = l - pad;
+            grid_bounds[8 * gi + 4 + k] = 1.f / (ext + 2.f * pad);
         }
-        const float ext = hi[k] - lo[k];
-        const float pad = std::max(ext * 1e-3f, 1e-4f);
-        bounds_lo[k] = lo[k] - pad;
-        bounds_inv[k] = 1.f / (ext + 2.f * pad);
+    for (int k = 0; k < 3; ++k) {
+        bounds_lo[k] = grid_bounds[k];
+        bounds_inv[k] = grid_bounds[4 + k];
     }
-    const size_t nt = (size_t)kGuideLevels * T;
+    grids = n_grids;
+    grid_entries = (size_t)kGuideLevels * T;
+    const size_t nt = grid_entries * n_grids;
     table_entries = nt;
+    GUIDE_HIP(hipMalloc((void **)&d_grid_bounds, grid_bounds.size() * sizeof(float)));
+    GUIDE_HIP(hipMemcpy(d_grid_bounds, grid_bounds.data(), grid_bounds.size() * sizeof(float), hipMemcpyHostToDevice));
     const size_t nw = 4 * 64 * 64;
     GUIDE_HIP(hipMalloc((void **)&table16, nt * sizeof(uint32_t)));
     GUIDE_HIP(hipMalloc((void **)&table32, nt * sizeof(float2)));
@@ -755,8 +798,8 @@ bool GuideState::init(const pt_guide_config &c, const float lo[3], const float h
     // Initialization: table U(-1e-4, 1e-4); weights U(+-sqrt(6 / (fan_in + fan_out)))
     // on the used part of each matrix, zero elsewhere.
     uint64_t st = (uint64_t)cfg.seed * 0x2545F4914F6CDD1Dull + 1;
-    std::vector<float> ht(nt * 2);
-    for (auto &x : ht) x = (uniform01(st) * 2.f - 1.f) * 1e-4f;
+    hipLaunchKernelGGL(k_guide_init_table, dim3(2048), dim3(256), 0, stream, table32, nt, splitmix(st));
+    GUIDE_HIP(hipGetLastError());
     std::vector<float> hw(nw, 0.f);
     for (int l = 0; l < 4; ++l) {
         const int in = l == 0 ? kGuideInputs : kGuideHidden;
@@ -765,7 +808,6 @@ bool GuideState::init(const pt_guide_config &c, const float lo[3], const float h
         for (int k = 0; k < in; ++k)
             for (int n = 0; n < out; ++n) hw[(size_t)l * 4096 + k * 64 + n] = (uniform01(st) * 2.f - 1.f) * a;
     }
-    GUIDE_HIP(hipMemcpy(table32, ht.data(), nt * sizeof(float2), hipMemcpyHostToDevice));
     GUIDE_HIP(hipMemcpy(w32, hw.data(), nw * sizeof(float), hipMemcpyHostToDevice));
     GUIDE_HIP(hipMemset(table_m, 0, nt * sizeof(float2)));
     GUIDE_HIP(hipMemset(table_v, 0, nt * sizeof(float2)));
@@ -805,6 +847,8 @@ GuideView GuideState::view() const {
     v.log2t = cfg.log2_table;
     v.lo = mk3(bounds_lo[0], bounds_lo[1], bounds_lo[2]);
     v.inv_ext = mk3(bounds_inv[0], bounds_inv[1], bounds_inv[2]);
+    v.grids = grids;
+    v.grid_bounds = reinterpret_cast<const float4 *>(d_grid_bounds);
     return v;
 }
 
@@ -822,6 +866,7 @@ bool GuideState::ensure_batch(uint32_t batch, std::string &err) {
 }
 
 bool GuideState::ensure_samples(uint32_t n, std::string &err) {
+    sample_fill = n;
     if (n <= sample_cap) return true;
     gfree(samples);
     GUIDE_HIP(hipMalloc((void **)&samples, (size_t)n * 3 * sizeof(float4)));
@@ -839,9 +884,10 @@ bool GuideState::ensure_records(uint32_t n, std::string &err) {
     return true;
 }
 
-bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool apply, std::string &err) {
+bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool apply, uint32_t grid_lo,
+                             uint32_t grid_hi, std::string &err) {
     const GuideView G = view();
-    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, first, batch, sample_cap, fp_norm);
+    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, first, batch, sample_fill, fp_norm);
     TrainBuffers B{xt, dyt, table_g};
     const uint32_t waves = batch / 16;
     hipLaunchKernelGGL(k_guide_train, dim3((waves + 3) / 4), dim3(kTrainBlock), 0, stream, G,
@@ -850,8 +896,10 @@ bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool
                        dyt, batch, ctl, w_grad);
     const AdamParams at{cfg.learning_rate, 0.9f, 0.99f, 1e-15f};
     const AdamParams aw{cfg.learning_rate, 0.9f, 0.99f, 1e-8f};
-    hipLaunchKernelGGL(k_guide_adam_table, dim3(2048), dim3(256), 0, stream, table32, table_m, table_v, table_g,
-                       table16, table_entries, at, ctl, apply ? 1u : 0u);
+    // Sparse Adam over the grids the step's samples can touch.
+    const size_t e0 = (size_t)grid_lo * grid_entries, ne = (size_t)(grid_hi - grid_lo) * grid_entries;
+    hipLaunchKernelGGL(k_guide_adam_table, dim3(2048), dim3(256), 0, stream, table32 + e0, table_m + e0,
+                       table_v + e0, table_g + e0, table16 + e0, ne, at, ctl, apply ? 1u : 0u);
     hipLaunchKernelGGL(k_guide_finish, dim3(1), dim3(kFinishBlock), 0, stream, w32, w_m, w_v, w_grad, aw, ctl,
                        apply ? 1u : 0u, (float *)nullptr, fwd_a, bwd_a);
     GUIDE_HIP(hipGetLastError());
@@ -863,19 +911,23 @@ bool GuideState::collect(const float4 *out_l, uint32_t total_paths, const GuideR
     hipLaunchKernelGGL(k_guide_collect_begin, dim3(1), dim3(64), 0, stream, ctl);
     const uint32_t grid = std::max(1u, std::min<uint32_t>((r.capacity + 255) / 256, 4096u));
     hipLaunchKernelGGL(k_guide_collect_count, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, ctl);
-    hipLaunchKernelGGL(k_guide_collect, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, samples, sample_cap,
+    hipLaunchKernelGGL(k_guide_collect, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, samples, sample_fill,
                        ctl);
-    hipLaunchKernelGGL(k_guide_collect_end, dim3(1), dim3(64), 0, stream, ctl, sample_cap);
+    hipLaunchKernelGGL(k_guide_collect_end, dim3(1), dim3(64), 0, stream, ctl, sample_fill);
     GUIDE_HIP(hipGetLastError());
     return true;
 }
 
-bool GuideState::train(uint32_t steps, float fp_norm, std::string &err) {
+bool GuideState::train(uint32_t steps, float fp_norm, std::string &err) { return train(steps, fp_norm, 0u, grids, err); }
+
+bool GuideState::train(uint32_t steps, float fp_norm, uint32_t grid_lo, uint32_t grid_hi, std::string &err) {
     const uint32_t B = cfg.batch_size;
     if (!ensure_batch(B, err)) return false;
-    if ((uint64_t)steps * B > sample_cap) steps = sample_cap / B;
+    if ((uint64_t)steps * B > sample_fill) steps = sample_fill / B;
+    grid_hi = std::min(grid_hi, grids);
+    if (grid_lo >= grid_hi) return true;
     for (uint32_t s = 0; s < steps; ++s)
-        if (!launch_step(s * B, B, fp_norm, true, err)) return false;
+        if (!launch_step(s * B, B, fp_norm, true, grid_lo, grid_hi, err)) return false;
     return true;
 }
 
@@ -891,7 +943,7 @@ bool GuideState::gradients(uint32_t n, float fp_norm, float *table_grad, float *
                            std::string &err) {
     if (!ensure_batch(n, err)) return false;
     const GuideView G = view();
-    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, 0u, n, sample_cap, fp_norm);
+    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, 0u, n, sample_fill, fp_norm);
     TrainBuffers B{xt, dyt, table_g};
     hipLaunchKernelGGL(k_guide_train, dim3((n / 16 + 3) / 4), dim3(kTrainBlock), 0, stream, G, samples, n, ctl, B,
                        LossParams{cfg.loss_flags, cfg.l2_weight});
@@ -960,14 +1012,18 @@ bool GuideState::set_params(const float *table, const float *weights, std::strin
     return true;
 }
 
-bool GuideState::query(const float *pos, uint32_t n, float *out, std::string &err) {
+bool GuideState::query(const float *pos, uint32_t n, float *out, uint32_t grid, std::string &err) {
     if (n == 0) return true;
+    if (grid >= grids) {
+        err = "pt_guide_query: no such hash grid";
+        return false;
+    }
     float *d_pos = nullptr, *d_out = nullptr;
     hipError_t e = hipMalloc((void **)&d_pos, (size_t)n * 3 * sizeof(float));
     if (e == hipSuccess) e = hipMalloc((void **)&d_out, (size_t)n * kGuideUsed * sizeof(float));
     if (e == hipSuccess) e = hipMemcpy(d_pos, pos, (size_t)n * 3 * sizeof(float), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_guide_query, dim3((n + 255) / 256), dim3(256), 0, stream, view(), d_pos, n, d_out);
+        hipLaunchKernelGGL(k_guide_query, dim3((n + 255) / 256), dim3(256), 0, stream, view(), d_pos, n, d_out, grid);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -985,7 +1041,7 @@ bool GuideState::read_samples(float *out, uint32_t max_samples, uint32_t &count,
     GUIDE_HIP(hipStreamSynchronize(stream));
     uint32_t n = 0;
     GUIDE_HIP(hipMemcpy(&n, &ctl->n_samples, sizeof(n), hipMemcpyDeviceToHost));
-    count = std::min(std::min(n, sample_cap), max_samples);
+    count = std::min(std::min(n, sample_fill), max_samples);
     if (count) GUIDE_HIP(hipMemcpy(out, samples, (size_t)count * 3 * sizeof(float4), hipMemcpyDeviceToHost));
     return true;
 }
@@ -1010,6 +1066,19 @@ bool GuideState::info(pt_guide_info &o, std::string &err) {
     for (int k = 0; k < 3; ++k) {
         o.bounds_lo[k] = bounds_lo[k];
         o.bounds_inv[k] = bounds_inv[k];
+    }
+    o.grids = grids;
+    return true;
+}
+
+bool GuideState::grid_box(uint32_t grid, float lo[3], float inv[3], std::string &err) {
+    if (grid >= grids) {
+        err = "pt_guide_grid_bounds: no such hash grid";
+        return false;
+    }
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = grid_bounds[8 * grid + k];
+        inv[k] = grid_bounds[8 * grid + 4 + k];
     }
     return true;
 }

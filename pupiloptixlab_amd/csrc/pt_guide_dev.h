@@ -24,14 +24,21 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 struct GuideView {
-    const uint32_t *table;     // [levels][T] packed FP16 feature pairs
+    const uint32_t *table;     // [grids][levels][T] packed FP16 feature pairs
     const half8 *fwd_a;        // [layer 4][rt 4][chunk 2][lane 64]
     const half8 *bwd_a;        // [layer 4][kt 4][chunk 2][lane 64]
     GuideLevel levels[kGuideLevels];
     uint32_t mask;             // T - 1
     uint32_t log2t;
-    f3 lo, inv_ext;            // scene bounds used to normalize positions
+    f3 lo, inv_ext;            // bounds of grid 0, used to normalize positions
+    uint32_t grids;            // hash grids (one per scene); the MLP is shared
+    const float4 *grid_bounds; // per grid: (lo.xyz, 0), (inv_ext.xyz, 0)
 };
+
+// First entry of grid `grid`, level `level` in the table.
+__device__ __forceinline__ size_t guide_level_base(const GuideView &G, uint32_t grid, int level) {
+    return ((size_t)grid * kGuideLevels + (size_t)level) << G.log2t;
+}
 
 __device__ __forceinline__ floatx4 mfma16(half8 a, half8 b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
@@ -39,6 +46,14 @@ __device__ __forceinline__ floatx4 mfma16(half8 a, half8 b, floatx4 c) {
 
 __device__ __forceinline__ f3 guide_normalize_pos(const GuideView &G, f3 p) {
     f3 u = (p - G.lo) * G.inv_ext;
+    return mk3(clampf(u.x, 0.f, 1.f), clampf(u.y, 0.f, 1.f), clampf(u.z, 0.f, 1.f));
+}
+
+// Position normalized by the bounds of hash grid `grid` (< G.grids).
+__device__ __forceinline__ f3 guide_grid_pos(const GuideView &G, uint32_t grid, f3 p) {
+    if (grid == 0u) return guide_normalize_pos(G, p);
+    const float4 lo = G.grid_bounds[2 * grid], inv = G.grid_bounds[2 * grid + 1];
+    const f3 u = (p - mk3(lo.x, lo.y, lo.z)) * mk3(inv.x, inv.y, inv.z);
     return mk3(clampf(u.x, 0.f, 1.f), clampf(u.y, 0.f, 1.f), clampf(u.z, 0.f, 1.f));
 }
 
@@ -74,15 +89,16 @@ __device__ __forceinline__ void level_corners(const GuideView &G, int level, f3 
     }
 }
 
-// First-layer B operand of lane group g for the query at normalized position u.
-__device__ __forceinline__ half8 guide_encode(const GuideView &G, f3 u, int g) {
+// First-layer B operand of lane group g for the query at normalized position
+// u in hash grid `grid`.
+__device__ __forceinline__ half8 guide_encode(const GuideView &G, f3 u, int g, uint32_t grid = 0u) {
     half8 b;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const int level = guide_lane_level(g, t);
         LevelCorners lc;
         level_corners(G, level, u, lc);
-        const uint32_t *tab = G.table + ((size_t)level << G.log2t);
+        const uint32_t *tab = G.table + guide_level_base(G, grid, level);
         float a0 = 0.f, a1 = 0.f;
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -161,18 +177,19 @@ __device__ __forceinline__ void guide_mlp_tile_shared(const half8 *w, int lane, 
     for (int rt = 0; rt < 4; ++rt) out[rt] = d[rt];
 }
 
-// Inference for the 64 queries of a wave: lane L supplies the position of
-// query L (any value when `active` is false) and receives its 41 outputs in
-// `stage` (LDS, 64 x 41 floats per wave).  Must be called by all 64 lanes.
-__device__ __forceinline__ void guide_infer_wave(const GuideView &G, f3 pos, float *stage) {
+// Inference for the 64 queries of a wave in hash grid `grid` (wave-uniform):
+// lane L supplies the position of query L (any value when `active` is false)
+// and receives its 41 outputs in `stage` (LDS, 64 x 41 floats per wave).
+// Must be called by all 64 lanes.
+__device__ __forceinline__ void guide_infer_wave(const GuideView &G, f3 pos, float *stage, uint32_t grid = 0u) {
     const int lane = (int)__lane_id();
     const int g = lane >> 4, i = lane & 15;
-    const f3 u = guide_normalize_pos(G, pos);
+    const f3 u = guide_grid_pos(G, grid, pos);
 #pragma unroll
     for (int qt = 0; qt < 4; ++qt) {
         const int q = qt * 16 + i;
         const f3 uq = mk3(__shfl(u.x, q), __shfl(u.y, q), __shfl(u.z, q));
-        const half8 in0 = guide_encode(G, uq, g);
+        const half8 in0 = guide_encode(G, uq, g, grid);
         floatx4 out[4];
         guide_mlp_tile(G, lane, in0, out, nullptr);
 #pragma unroll

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/pupil_hip.h"
 #include "pt_guide_dev.h"
@@ -47,7 +48,7 @@ struct GuideCtl {
 // vertex of a training path (component-major, 5 float4 per record):
 //   0: position.xyz, BSDF pdf of the sampled direction
 //   1: sampled direction.xyz, |cos| to the shading normal
-//   2: path throughput after the vertex (T_{k+1}).xyz
+//   2: path throughput after the vertex (T_{k+1}).xyz, hash grid of the vertex
 //   3: sample weight f |cos| / p.xyz
 //   4: radiance sum L_after - T_{k+1} Le of the next vertex.xyz, epoch bits
 // Record k of a path is completed by the shading of vertex k + 1 (at once,
@@ -66,8 +67,13 @@ public:
     bool enabled = false;
     pt_guide_config cfg{};
     GuideLevel levels[kGuideLevels]{};
-    float bounds_lo[3]{}, bounds_inv[3]{};
-    size_t table_entries = 0;
+    float bounds_lo[3]{}, bounds_inv[3]{};  // grid 0
+    uint32_t grids = 1;                     // hash grids, one per scene; the MLP is shared
+    std::vector<float> grid_bounds;         // per grid: lo.xyz, 0, inv_ext.xyz, 0
+    float *d_grid_bounds = nullptr;
+    size_t grid_entries = 0;                // table entries per grid (levels x T)
+    size_t table_entries = 0;               // all grids
+    bool for_batch = false;                 // one grid per batch view (pt_guide_enable_batch)
 
     // parameters and optimizer moments
     uint32_t *table16 = nullptr;  // FP16 feature pairs used by the kernels
@@ -82,7 +88,8 @@ public:
 
     // training samples (3 float4 each) and render-side records
     float4 *samples = nullptr;
-    uint32_t sample_cap = 0;
+    uint32_t sample_cap = 0;   // allocated
+    uint32_t sample_fill = 0;  // used by the next collection and steps (set by ensure_samples)
     float4 *rec = nullptr;
     uint32_t rec_cap = 0;
     uint32_t epoch = 0;
@@ -93,8 +100,12 @@ public:
 
     ~GuideState() { release(); }
     void release();
-    bool init(const pt_guide_config &c, const float lo[3], const float hi[3], hipStream_t s, std::string &err);
+    // lo / hi: 3 floats per grid (the scene boxes the grids cover).
+    bool init(const pt_guide_config &c, const float *lo, const float *hi, uint32_t n_grids, hipStream_t s,
+              std::string &err);
     GuideView view() const;
+    // Allocates room for n samples and makes n the collection size: a
+    // collection keeps a uniform subsample of at most n samples of its launch.
     bool ensure_samples(uint32_t n, std::string &err);
     bool ensure_records(uint32_t n, std::string &err);
     // Sample collection from the finished paths of one launch.
@@ -102,18 +113,22 @@ public:
     // `steps` optimizer steps over the collected samples, batch_size each;
     // steps without samples are skipped on the device.
     bool train(uint32_t steps, float fp_norm, std::string &err);
+    // ... when the samples only name grids [grid_lo, grid_hi): Adam visits those.
+    bool train(uint32_t steps, float fp_norm, uint32_t grid_lo, uint32_t grid_hi, std::string &err);
     // Host-sample entry points (tests / external trainers).
     bool upload_samples(const float *host, uint32_t n, std::string &err);
     bool gradients(uint32_t n, float fp_norm, float *table_grad, float *weight_grad, float *loss, std::string &err);
     bool get_params(float *table, float *weights, std::string &err);
     bool set_params(const float *table, const float *weights, std::string &err);
-    bool query(const float *pos, uint32_t n, float *out, std::string &err);
+    bool query(const float *pos, uint32_t n, float *out, uint32_t grid, std::string &err);
     bool info(pt_guide_info &out, std::string &err);
+    bool grid_box(uint32_t grid, float lo[3], float inv[3], std::string &err);
     bool read_samples(float *out, uint32_t max_samples, uint32_t &count, std::string &err);
 
 private:
     bool ensure_batch(uint32_t batch, std::string &err);
-    bool launch_step(uint32_t first, uint32_t batch, float fp_norm, bool apply, std::string &err);
+    bool launch_step(uint32_t first, uint32_t batch, float fp_norm, bool apply, uint32_t grid_lo, uint32_t grid_hi,
+                     std::string &err);
     bool pack(std::string &err);
 };
 

@@ -621,11 +621,27 @@ void add_wall(HostScene &s, uint32_t material, int axis, float c, float u0, floa
 }
 
 // 4 x 3 x 6 m room whose only light sits in a closet behind a 5 cm door gap.
+// Seed 0 is the canonical layout (door centered at z = 3); other seeds move
+// the door and its closet along the wall (center z in [1.5, 4.5]) and place
+// the camera at random inside the room, looking towards the far half.
 void build_door_room(HostScene &s, const pt_gen_params &p) {
     s.width = p.width ? p.width : 1920;
     s.height = p.height ? p.height : 1080;
     s.max_depth = p.max_depth ? p.max_depth : 16;
     const float cell = 0.0215f;
+    float zc = 3.f;
+    float eye[3] = {1.0f, 1.6f, 0.5f}, target[3] = {3.0f, 1.2f, 4.5f}, up[3] = {0.f, 1.f, 0.f};
+    if (p.seed != 0) {
+        SplitMix rng{p.seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull};
+        zc = rng.range(1.5f, 4.5f);
+        eye[0] = rng.range(0.4f, 1.6f);
+        eye[1] = rng.range(1.2f, 1.9f);
+        eye[2] = rng.range(0.3f, 1.2f);
+        target[0] = rng.range(2.4f, 3.6f);
+        target[1] = rng.range(0.8f, 1.6f);
+        target[2] = rng.range(3.5f, 5.5f);
+    }
+    const float d0 = zc - 0.025f, d1 = zc + 0.025f, c0 = zc - 0.5f, c1 = zc + 0.5f;
     uint32_t wall = s.add_material(make_diffuse(0.6f, 0.6f, 0.6f, true));
     uint32_t black = s.add_material(make_diffuse(0.f, 0.f, 0.f, true));
     // Room x in [0,4], y in [0,3], z in [0,6].
@@ -634,19 +650,18 @@ void build_door_room(HostScene &s, const pt_gen_params &p) {
     add_wall(s, wall, 0, 0.f, 0.f, 3.f, 0.f, 6.f, cell);   // x = 0 (u = y, v = z)
     add_wall(s, wall, 2, 0.f, 0.f, 4.f, 0.f, 3.f, cell);   // z = 0 (u = x, v = y)
     add_wall(s, wall, 2, 6.f, 0.f, 4.f, 0.f, 3.f, cell);   // z = 6
-    // x = 4 wall with a door slit z in [2.975, 3.025], y in [0, 2].
-    add_wall(s, wall, 0, 4.f, 0.f, 3.f, 0.f, 2.975f, cell);
-    add_wall(s, wall, 0, 4.f, 0.f, 3.f, 3.025f, 6.f, cell);
-    add_wall(s, wall, 0, 4.f, 2.f, 3.f, 2.975f, 3.025f, cell);
-    // Closet x in [4,5], z in [2.5,3.5].
-    add_wall(s, wall, 1, 0.f, 2.5f, 3.5f, 4.f, 5.f, cell);
-    add_wall(s, wall, 1, 3.f, 2.5f, 3.5f, 4.f, 5.f, cell);
-    add_wall(s, wall, 0, 5.f, 0.f, 3.f, 2.5f, 3.5f, cell);
-    add_wall(s, wall, 2, 2.5f, 4.f, 5.f, 0.f, 3.f, cell);
-    add_wall(s, wall, 2, 3.5f, 4.f, 5.f, 0.f, 3.f, cell);
+    // x = 4 wall with a door slit z in [zc - 0.025, zc + 0.025], y in [0, 2].
+    add_wall(s, wall, 0, 4.f, 0.f, 3.f, 0.f, d0, cell);
+    add_wall(s, wall, 0, 4.f, 0.f, 3.f, d1, 6.f, cell);
+    add_wall(s, wall, 0, 4.f, 2.f, 3.f, d0, d1, cell);
+    // Closet x in [4,5], z in [zc - 0.5, zc + 0.5].
+    add_wall(s, wall, 1, 0.f, c0, c1, 4.f, 5.f, cell);
+    add_wall(s, wall, 1, 3.f, c0, c1, 4.f, 5.f, cell);
+    add_wall(s, wall, 0, 5.f, 0.f, 3.f, c0, c1, cell);
+    add_wall(s, wall, 2, c0, 4.f, 5.f, 0.f, 3.f, cell);
+    add_wall(s, wall, 2, c1, 4.f, 5.f, 0.f, 3.f, cell);
     pt_texture_desc le = make_rgb_texture(200.f, 200.f, 200.f);
-    add_wall(s, black, 0, 4.99f, 1.0f, 2.0f, 2.75f, 3.25f, 0.25f, true, &le);
-    float eye[3] = {1.0f, 1.6f, 0.5f}, target[3] = {3.0f, 1.2f, 4.5f}, up[3] = {0.f, 1.f, 0.f};
+    add_wall(s, black, 0, 4.99f, 1.0f, 2.0f, zc - 0.25f, zc + 0.25f, 0.25f, true, &le);
     Transform cam;
     cam.look_at(eye, target, up);
     s.set_camera(60.f, cam.m);

@@ -21,6 +21,7 @@
 namespace pupil {
 
 constexpr uint32_t kPathRecorded = 1u << 11;  // the last vertex wrote a training record
+constexpr uint32_t kPathNoTrain = 1u << 12;   // the path's frame does not train the guide (batched renders)
 
 struct GuidedVertex {
     LocalGeometry g;
@@ -30,6 +31,7 @@ struct GuidedVertex {
 };
 
 // Training-record sink of the guided step (null `rec` disables recording).
+// Record component 2's w holds the hash grid of the vertex (its batch view).
 struct RecordSink {
     f4 *rec;             // component-major: 5 x capacity float4
     uint32_t capacity;
@@ -41,8 +43,8 @@ PT_HD bool guidable_material(uint32_t type) {
            type == kMatRoughPlastic;
 }
 
-PT_HD bool training_path(const RecordSink &r, uint32_t path) {
-    return r.rec != nullptr && path % r.stride == r.phase;
+PT_HD bool training_path(const RecordSink &r, uint32_t path, uint32_t flags) {
+    return r.rec != nullptr && (flags & kPathNoTrain) == 0u && path % r.stride == r.phase;
 }
 
 PT_HD uint32_t record_index(const RecordSink &r, uint32_t path, uint32_t depth) {
@@ -69,7 +71,7 @@ PT_HD bool guided_begin(const SceneView &s, const ShadeParams &prm, PathState &p
     out.shadow = false;
     out.wrote_aov = false;
     const bool primary = (ps.flags & kPathPrimary) != 0;
-    const bool train = training_path(rs, path);
+    const bool train = training_path(rs, path, ps.flags);
     uint32_t depth = ps.flags & kPathDepthMask;
 
     if (prim == kMissPrim) {
@@ -149,7 +151,8 @@ PT_HD bool guided_begin(const SceneView &s, const ShadeParams &prm, PathState &p
 // Next-event estimation and direction sampling.  `use_guide` selects the
 // mixture path; `m` is read only then.
 PT_HD void guided_scatter(const SceneView &s, PathState &ps, const GuidedVertex &v, bool use_guide,
-                          const GuideMixture &m, const RecordSink &rs, uint32_t path, ShadeOut &out) {
+                          const GuideMixture &m, const RecordSink &rs, uint32_t path, ShadeOut &out,
+                          uint32_t grid = 0u) {
     const LocalGeometry &g = v.g;
     const float c = use_guide ? m.c : 0.f;
     const f3 wo_l = to_local(-ps.d, g.normal);
@@ -215,7 +218,7 @@ PT_HD void guided_scatter(const SceneView &s, PathState &ps, const GuidedVertex 
     const bool delta = (lobe & kLobeDelta) != 0u;
     float p = pb;
     if (use_guide && !delta) p = c * guide_pdf(m, wi_w) + (1.f - c) * pb;
-    const bool record = v.guidable && !delta && training_path(rs, path);
+    const bool record = v.guidable && !delta && training_path(rs, path, ps.flags);
     const uint32_t ridx = record ? record_index(rs, path, v.depth - 1u) : 0u;
     const size_t rn = rs.capacity;
     if (is_zero(f * fabsf(wi_l.z)) || is_zero(p)) {
@@ -224,7 +227,7 @@ PT_HD void guided_scatter(const SceneView &s, PathState &ps, const GuidedVertex 
         if (record && ridx < rs.capacity) {
             rs.rec[ridx] = f4{g.position.x, g.position.y, g.position.z, pb};
             rs.rec[rn + ridx] = f4{wi_w.x, wi_w.y, wi_w.z, fabsf(wi_l.z)};
-            rs.rec[2 * rn + ridx] = f4{0.f, 0.f, 0.f, 0.f};
+            rs.rec[2 * rn + ridx] = f4{0.f, 0.f, 0.f, (float)grid};
             rs.rec[3 * rn + ridx] = f4{0.f, 0.f, 0.f, 0.f};
             rs.rec[4 * rn + ridx] = f4{0.f, 0.f, 0.f, u2f(rs.epoch)};
         }
@@ -236,13 +239,13 @@ PT_HD void guided_scatter(const SceneView &s, PathState &ps, const GuidedVertex 
     ps.o = g.position;
     ps.d = wi_w;
     ps.prev_pdf = p;
-    uint32_t fl = v.depth;
+    uint32_t fl = v.depth | (ps.flags & kPathNoTrain);
     if (delta) fl |= kPathPrevDelta;
     if (dot(g.normal, ps.d) > 0.f) fl |= kPathEnvUpper;
     if (record && ridx < rs.capacity) {
         rs.rec[ridx] = f4{g.position.x, g.position.y, g.position.z, pb};
         rs.rec[rn + ridx] = f4{wi_w.x, wi_w.y, wi_w.z, fabsf(wi_l.z)};
-        rs.rec[2 * rn + ridx] = f4{ps.T.x, ps.T.y, ps.T.z, 0.f};
+        rs.rec[2 * rn + ridx] = f4{ps.T.x, ps.T.y, ps.T.z, (float)grid};
         rs.rec[3 * rn + ridx] = f4{weight.x, weight.y, weight.z, 0.f};
         fl |= kPathRecorded;
     }

@@ -1,0 +1,246 @@
+"""Guided batched rendering: the batch guide has one hash grid per view (over
+that view's scene bounds) and one shared MLP; pt_batch_render samples it and
+trains it online inside the batched wavefront loop.
+
+* Each grid is the single-scene guide of its view: same normalization bounds
+  as pt_guide_enable on that scene alone, forward outputs against the float64
+  guide oracle (oracle/guide.py) on that grid's table slice.
+* Gradients of samples naming different grids land in their own slices; the
+  weight gradient and loss are the sample-weighted means over all grids.
+* With fixed parameters, a guided batch render gives every view exactly the
+  image a guided pt_render of that scene alone gives (bit-identical, direct
+  launches and hipGraph replay).
+* Trained online in the batch, the guide stays unbiased and beats plain BSDF
+  sampling on door-room views at equal sample count.
+"""
+import numpy as np
+import pytest
+
+import pupiloptixlab_amd as pt
+from oracle import guide as G
+
+pytestmark = pytest.mark.gpu
+
+LOG2T = 14
+
+
+def small_scenes():
+    return [
+        (pt.HostScene.generate(pt.SCENE_CORNELL, width=64, height=48), 4),
+        (pt.HostScene.generate(pt.SCENE_DOOR_ROOM, seed=3, width=80, height=45, max_depth=6), 3),
+        (pt.HostScene.generate(pt.SCENE_TERRAIN, seed=2, width=64, height=64, grid=64, spheres=8, max_depth=5), 2),
+    ]
+
+
+def random_params(seed, grids, table_sd=0.5):
+    rng = np.random.default_rng(seed)
+    table = G.fp16(rng.normal(0, table_sd, (grids, G.LEVELS, 1 << LOG2T, 2))).astype(np.float32)
+    dims = [32, 64, 64, 64, 41]
+    w = np.zeros((4, 64, 64), np.float32)
+    for l in range(4):
+        w[l, :dims[l], :dims[l + 1]] = G.fp16(rng.normal(0, 1.2 / np.sqrt(dims[l]), (dims[l], dims[l + 1])))
+    return table, w
+
+
+def oracle_weights(w):
+    dims = [32, 64, 64, 64, 64]
+    return [w[l, :dims[l], :].astype(np.float64) for l in range(4)]
+
+
+def scene_positions(scene, n, seed):
+    p = scene.positions()
+    lo, hi = p.min(0), p.max(0)
+    return (lo + np.random.default_rng(seed).random((n, 3)) * (hi - lo)).astype(np.float32)
+
+
+def normalized(lo, inv, pos):
+    return np.clip((pos - lo) * inv, 0.0, 1.0).astype(np.float64)
+
+
+def batch_with_views(tracer, views):
+    tracer.batch_clear()
+    for scene, spp in views:
+        tracer.batch_add_view(scene, spp)
+
+
+def test_batch_guide_grids_are_per_scene_guides(tracer):
+    print("case: batch guide, 3 views, grid bounds + forward vs oracle per grid", flush=True)
+    views = small_scenes()
+    single_bounds = []
+    for scene, _ in views:
+        tracer.set_scene(scene)
+        tracer.build_bvh()
+        tracer.guide_enable(log2_table=LOG2T)
+        info = tracer.guide_info()
+        single_bounds.append((np.array(info["bounds_lo"], np.float32), np.array(info["bounds_inv"], np.float32)))
+    tracer.guide_disable()
+    batch_with_views(tracer, views)
+    tracer.guide_enable_batch(log2_table=LOG2T)
+    info = tracer.guide_info()
+    assert info["enabled"] == 1 and info["grids"] == 3
+    table, w = random_params(11, 3)
+    tracer.guide_set_params(table, w)
+    tb, wb = tracer.guide_get_params()
+    assert tb.shape == table.shape and np.array_equal(tb, table) and np.array_equal(wb, w)
+    for k, (scene, _) in enumerate(views):
+        lo, inv = tracer.guide_grid_bounds(k)
+        assert np.array_equal(lo, single_bounds[k][0]) and np.array_equal(inv, single_bounds[k][1])
+        pos = scene_positions(scene, 1024, 20 + k)
+        out = tracer.guide_query(pos, grid=k)
+        u = normalized(lo, inv, pos)
+        feats, _ = G.encode(table[k].astype(np.float64), u, info["resolution"], LOG2T, quantize=True)
+        ref, _ = G.mlp_forward(feats, oracle_weights(w), quantize=True)
+        ref = ref[:, :41]
+        err = np.abs(out - ref) / (np.sqrt(np.mean(ref ** 2, axis=0)) + np.abs(ref))
+        print(f"grid {k}: max rel err {err.max():.2e} mean {err.mean():.2e}")
+        assert err.max() <= 1e-2 and err.mean() <= 5e-4
+    pos = scene_positions(views[0][0], 256, 30)
+    assert np.array_equal(tracer.guide_query(pos), tracer.guide_query(pos, grid=0))
+    with pytest.raises(pt.PupilError):
+        tracer.guide_query(pos, grid=3)
+    with pytest.raises(pt.PupilError):
+        tracer.guide_grid_bounds(3)
+    tracer.batch_clear()
+
+
+def test_batch_guide_gradients_per_grid(tracer):
+    print("case: batch guide gradients, 3072 samples over 3 grids (+ samples naming no grid)", flush=True)
+    views = small_scenes()
+    batch_with_views(tracer, views)
+    tracer.guide_enable_batch(log2_table=LOG2T, seed=4)
+    table, w = random_params(12, 3, table_sd=0.3)
+    tracer.guide_set_params(table, w)
+    n = 3072
+    rng = np.random.default_rng(13)
+    grid = rng.integers(0, 3, n).astype(np.float32)
+    grid[::17] = 3.0   # no such grid: unused
+    grid[5::19] = -1.0
+    pos = np.zeros((n, 3), np.float32)
+    for k, (scene, _) in enumerate(views):
+        m = grid == k
+        pos[m] = scene_positions(scene, int(m.sum()), 40 + k)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    target = rng.gamma(1.0, 1.0, n)
+    fs = rng.gamma(1.0, 1.0, n)
+    pb = rng.uniform(0.05, 1.0, n)
+    weight = np.where(np.arange(n) % 3 == 0, 2.0, 1.0).astype(np.float32)
+    samples = pt.PathTracer.guide_samples(pos, d, target, fs, pb, weight, grid=grid)
+    tg, wg, loss = tracer.guide_gradients(samples, fp_norm=1.1)
+    assert tg.shape == (3, G.LEVELS, 1 << LOG2T, 2)
+    info = tracer.guide_info()
+    f64 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+    total_w = float(weight[(grid >= 0) & (grid < 3)].sum())
+    dws_all = [np.zeros_like(x) for x in oracle_weights(w)]
+    loss_all = 0.0
+    for k in range(3):
+        m = grid == k
+        lo, inv = tracer.guide_grid_bounds(k)
+        dws, dtable, o, lk = G.gradients(table[k].astype(np.float64), oracle_weights(w), normalized(lo, inv, pos[m]),
+                                         f64(d[m]), f64(target[m]), f64(fs[m]), f64(pb[m]), info["resolution"],
+                                         LOG2T, norm_fp=1.1, valid=f64(weight[m]), quantize=True)
+        share = float(weight[m].sum()) / total_w
+        e = np.linalg.norm(tg[k] - dtable * share) / np.linalg.norm(dtable * share)
+        print(f"grid {k}: table-gradient rel err {e:.2e}")
+        assert e <= 2e-2
+        for l in range(4):
+            dws_all[l] += dws[l] * share
+        loss_all += lk * share
+    assert loss == pytest.approx(loss_all, rel=5e-3)
+    dims = [32, 64, 64, 64, 41]
+    for l in range(4):
+        ref = dws_all[l][:, :dims[l + 1]]
+        e = np.linalg.norm(wg[l, :dims[l], :dims[l + 1]] - ref) / np.linalg.norm(ref)
+        print(f"layer {l} weight-gradient rel err {e:.2e}")
+        assert e <= 2e-2
+    tracer.batch_clear()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_guided_matches_single_guided_renders(tracer, graph):
+    print(f"case: guided batch (fixed params) vs guided pt_render per view, graph={graph}", flush=True)
+    views = small_scenes()
+    table, w = random_params(21, 3, table_sd=0.4)
+    ref = []
+    for k, (scene, spp) in enumerate(views):
+        tracer.set_scene(scene)
+        tracer.build_bvh()
+        tracer.guide_enable(log2_table=LOG2T)
+        tracer.guide_set_params(table[k], w)
+        tracer.render(spp=spp, mis_mode=pt.MIS_CONSISTENT, aov=False, guide=True)
+        ref.append(tracer.read(pt.BUF_ACCUM))
+    tracer.guide_disable()
+    batch_with_views(tracer, views)
+    tracer.guide_enable_batch(log2_table=LOG2T)
+    tracer.guide_set_params(table, w)
+    # small launches: views split across launches and launches mix views
+    tracer.batch_render(mis_mode=pt.MIS_CONSISTENT, guide=True, graph=graph, paths_per_launch=8192)
+    for k, r in enumerate(ref):
+        img = tracer.batch_read(k)
+        assert np.all(np.isfinite(img))
+        assert np.array_equal(img, r), f"view {k} differs"
+    tracer.batch_clear()
+
+
+def relmse(img, ref):
+    return float(np.mean((img - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def test_batch_guided_training_gains_and_stays_unbiased(tracer):
+    """Two door-room variants and a terrain view in one batch, 320x180 (the
+    terrain 128x72): references of 2048 plain FP64 frames per view; plain
+    batch vs guided batch (online training on the first 48 of 192 frames of
+    each view, shared MLP) at the same sample count."""
+    print("case: guided batch training, 2 door rooms 320x180 + terrain 128x72, 192 spp", flush=True)
+    views = [(pt.HostScene.generate(pt.SCENE_DOOR_ROOM, seed=0, width=320, height=180), 192),
+             (pt.HostScene.generate(pt.SCENE_DOOR_ROOM, seed=7, width=320, height=180), 192),
+             (pt.HostScene.generate(pt.SCENE_TERRAIN, seed=5, width=128, height=72, grid=96, spheres=16), 192)]
+    refs = []
+    for scene, _ in views:
+        tracer.set_scene(scene)
+        tracer.build_bvh()
+        tracer.render(spp=2048, mis_mode=pt.MIS_CONSISTENT, fp64=True, aov=False)
+        refs.append(tracer.read(pt.BUF_ACCUM64) / 2048)
+    batch_with_views(tracer, views)
+    tracer.batch_render(mis_mode=pt.MIS_CONSISTENT, graph=True)
+    plain = [relmse(tracer.batch_read(k)[..., :3].astype(np.float64), refs[k]) for k in range(3)]
+    tracer.batch_reset()
+    tracer.guide_enable_batch(seed=2)
+    tracer.batch_render(mis_mode=pt.MIS_CONSISTENT, graph=True, guide=True, guide_train=True, guide_train_spp=48,
+                        paths_per_launch=2 * 320 * 180)
+    info = tracer.guide_info()
+    expected_steps = 8 * 48 * 3
+    print(f"steps {info['steps']} skipped {info['skipped']} (expected {expected_steps} in all)")
+    assert info["steps"] > 0 and info["steps"] + info["skipped"] <= expected_steps
+    for k in range(3):
+        img = tracer.batch_read(k)[..., :3].astype(np.float64)
+        assert np.all(np.isfinite(img))
+        g = relmse(img, refs[k])
+        bias = img.mean() / refs[k].mean() - 1
+        print(f"view {k}: relmse plain {plain[k]:.4f} guided {g:.4f} ratio {plain[k] / g:.2f} mean {bias:+.4f}")
+        assert abs(bias) < 0.02
+        if k < 2:
+            assert g < 0.6 * plain[k]
+    tracer.batch_clear()
+
+
+def test_batch_guide_errors(tracer):
+    views = small_scenes()[:2]
+    tracer.batch_clear()
+    with pytest.raises(pt.PupilError):
+        tracer.guide_enable_batch(log2_table=12)  # no views
+    batch_with_views(tracer, views)
+    with pytest.raises(pt.PupilError) as e:
+        tracer.batch_render(mis_mode=pt.MIS_CONSISTENT, guide=True)
+    assert e.value.code == -10
+    tracer.guide_enable_batch(log2_table=12)
+    with pytest.raises(pt.PupilError):
+        tracer.batch_render(mis_mode=pt.MIS_REFERENCE, guide=True)
+    tracer.set_scene(views[0][0])
+    tracer.build_bvh()
+    with pytest.raises(pt.PupilError) as e:  # the batch guide is not a guide of the single scene
+        tracer.render(spp=1, mis_mode=pt.MIS_CONSISTENT, guide=True)
+    assert e.value.code == -10
+    tracer.batch_add_view(views[0][0], 1)  # a new view disables the batch guide
+    assert tracer.guide_info()["enabled"] == 0
+    tracer.batch_clear()

@@ -40,6 +40,8 @@ def test_struct_layouts_match_header():
     assert pt.SceneDesc.bitmaps.offset == 240 and pt.SceneDesc.env_to_world.offset == 256
     assert C.sizeof(pt.SceneDesc) == 296
     assert C.sizeof(pt.SphereDesc) == 48 and C.sizeof(pt.BitmapDesc) == 24
+    assert C.sizeof(pt.BatchParams) == 36 and pt.BatchParams.guide_train_spp.offset == 32
+    assert pt.GuideInfo.grids.offset == 180 and C.sizeof(pt.GuideInfo) == 184
 
 
 def test_create_without_device_fails_cleanly():
@@ -100,6 +102,28 @@ def test_door_room_size():
     assert 400_000 < d.num_triangles < 700_000
     light = [sh for sh in s.shapes() if sh.is_emitter]
     assert len(light) == 1 and light[0].radiance.color0[0] == 200.0
+
+
+def test_door_room_variants():
+    """Seed 0 is the canonical room (door slit centered at z = 3); other seeds
+    move the door, its closet and light along the x = 4 wall and the camera."""
+    base = pt.HostScene.generate(pt.SCENE_DOOR_ROOM, width=32, height=18)
+    p0 = base.positions()
+    light = lambda s: s.positions()[-15:]  # the light (last shape): 1 m x 0.5 m in 0.25 m cells, 5 x 3 vertices
+    assert np.allclose(light(base)[:, 2].min(), 2.75) and np.allclose(light(base)[:, 2].max(), 3.25)
+    zs = set()
+    for seed in (1, 2, 3):
+        v = pt.HostScene.generate(pt.SCENE_DOOR_ROOM, seed=seed, width=32, height=18)
+        q = v.positions()
+        assert q.min(0).tolist() == p0.min(0).tolist() and q.max(0).tolist() == p0.max(0).tolist()
+        zl = light(v)[:, 2]
+        assert zl.max() - zl.min() == pytest.approx(0.5, abs=1e-5) and 1.25 <= zl.min() and zl.max() <= 4.75
+        zs.add(round(float(zl.min()), 4))
+        assert list(v.desc.camera.camera_to_world) != list(base.desc.camera.camera_to_world)
+    assert len(zs) == 3
+    again = pt.HostScene.generate(pt.SCENE_DOOR_ROOM, seed=2, width=32, height=18)
+    assert np.array_equal(again.positions(), pt.HostScene.generate(pt.SCENE_DOOR_ROOM, seed=2, width=32,
+                                                                   height=18).positions())
 
 
 # ---------------------------------------------------------------- wide BVH (host traversal) vs brute force
