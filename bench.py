@@ -210,6 +210,122 @@ def guided_experiment(pt, ref_spp, plain_spp, train_spp):
     return out
 
 
+def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp, terrain_ref_spp, train_spp,
+                              terrain_grids, seed):
+    """Guided plus batched: n_door seeded door-room variants (door and camera
+    moved) and n_terrain seeded terrain variants (heightfield grid drawn from
+    terrain_grids, 256 icospheres, all seven BSDFs) at width x height, spp per
+    view from {16, 64, 256} (each kind cycles through them), in one wavefront
+    loop.  The guide has one hash grid per view and one shared MLP, trained
+    online on the first train_spp frames of every view (round-robin over the
+    views), with optimizer steps per frame scaled to the film size (8 per
+    1920x1080 frame, at least 1).
+
+    Plain: the batch with BSDF sampling (hipGraph replay, default launch
+    width), timed; rendered again m = round(T_guided / T_plain) times in all so
+    that both sides use about the same wall-clock.  Guided: one call, training
+    included (training launches of ~2M paths, the rest at the default width).
+    relMSE per view against FP64 plain references (door_ref_spp /
+    terrain_ref_spp frames, single-view renders)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    views = []
+    for i in range(n_door):
+        views.append(("door", pt.HostScene.generate(pt.SCENE_DOOR_ROOM, seed=1 + i, width=width, height=height)))
+    for i in range(n_terrain):
+        grid = int(rng.choice(terrain_grids))
+        views.append(("terrain", pt.HostScene.generate(pt.SCENE_TERRAIN, seed=100 + i, width=width, height=height,
+                                                       grid=grid)))
+        log(f"guided batch: terrain variant {i} generated ({views[-1][1].desc.num_triangles} tris)")
+    spps = [(256, 64, 16)[i % 3] for i in range(n_door)] + [(64, 256, 16)[i % 3] for i in range(n_terrain)]
+    tris = [v[1].desc.num_triangles for v in views]
+    t = pt.PathTracer(0)
+    mis = pt.MIS_CONSISTENT
+    refs = []
+    t_ref0 = time.perf_counter()
+    for kind, scene in views:
+        t.set_scene(scene)
+        t.build_bvh()
+        n = door_ref_spp if kind == "door" else terrain_ref_spp
+        t.render(spp=n, mis_mode=mis, fp64=True, aov=False)
+        refs.append(t.read(pt.BUF_ACCUM64) / n)
+        log(f"guided batch: reference {len(refs)}/{len(views)} ({kind}, {n} spp) at {time.perf_counter() - t_ref0:.1f}s")
+    log(f"guided batch: {len(views)} views, {sum(tris)} tris, references in {time.perf_counter() - t_ref0:.1f}s")
+    for (kind, scene), spp in zip(views, spps):
+        t.batch_add_view(scene, spp)
+    t_build = time.perf_counter() - t0
+    log(f"guided batch: views added ({t_build:.1f}s in all)")
+    samples = sum(width * height * spp for spp in spps)
+
+    def errors():
+        return [relmse(t.batch_read(k)[..., :3].astype(np.float64), refs[k]) for k in range(len(views))]
+
+    t.batch_render(mis_mode=mis, graph=True)  # warm-up
+    t.batch_reset()
+    t0 = time.perf_counter()
+    t.batch_render(mis_mode=mis, graph=True)
+    t_plain1 = time.perf_counter() - t0
+    e_plain1 = errors()
+    log(f"guided batch: plain {t_plain1:.2f}s")
+    t.batch_reset()
+    spf = max(1, int(round(8 * width * height / (1920 * 1080))))
+    t.guide_enable_batch(seed=seed, steps_per_frame=spf)
+    t0 = time.perf_counter()
+    t.batch_render(mis_mode=mis, graph=True, guide=True, guide_train=True, guide_train_spp=train_spp)
+    t_guided = time.perf_counter() - t0
+    e_guided = errors()
+    info = t.guide_info()
+    log(f"guided batch: guided {t_guided:.2f}s")
+    m = max(1, int(round(t_guided / t_plain1)))
+    t.batch_reset()
+    t0 = time.perf_counter()
+    for r in range(m):
+        t.batch_render(mis_mode=mis, graph=True)
+        log(f"guided batch: plain equal-time render {r + 1}/{m}")
+    t_plain = time.perf_counter() - t0
+    e_plain = errors()
+    t.close()
+    per_view = [{"scene": kind, "triangles": tr, "spp": spp, "relmse_plain_equal_time": round(ep, 5),
+                 "relmse_plain_same_spp": round(e1, 5), "relmse_guided": round(eg, 5)}
+                for (kind, _), tr, spp, ep, e1, eg in zip(views, tris, spps, e_plain, e_plain1, e_guided)]
+    mean = lambda xs: float(np.mean(xs)) if xs else None
+    kinds = {}
+    for kind in ("door", "terrain"):
+        idx = [k for k, v in enumerate(views) if v[0] == kind]
+        if idx:
+            ep = mean([e_plain[k] for k in idx])
+            eg = mean([e_guided[k] for k in idx])
+            kinds[kind] = {"relmse_plain_equal_time": round(ep, 5), "relmse_guided": round(eg, 5),
+                           "ratio": round(ep / eg, 3)}
+    ep, eg = mean(e_plain), mean(e_guided)
+    out = {
+        "workload": (f"{n_door} door-room variants + {n_terrain} terrain variants (grids {list(terrain_grids)}), "
+                     f"{width}x{height}, spp {spps}, depth 16, one wavefront loop; guide: one hash grid per view, "
+                     f"shared MLP, trained on the first {train_spp} frames of each view, {spf} steps per frame"),
+        "views": len(views),
+        "triangles": sum(tris),
+        "samples": samples,
+        "build_seconds": round(t_build, 2),
+        "plain_seconds_same_spp": round(t_plain1, 3),
+        "plain_msamples_per_s": round(samples / t_plain1 / 1e6, 2),
+        "guided_seconds": round(t_guided, 3),
+        "guided_msamples_per_s": round(samples / t_guided / 1e6, 2),
+        "plain_renders_equal_time": m,
+        "plain_seconds_equal_time": round(t_plain, 3),
+        "relmse_plain_equal_time": round(ep, 5),
+        "relmse_guided": round(eg, 5),
+        "ratio": round(ep / eg, 3),
+        "by_scene": kinds,
+        "guide_steps": info["steps"],
+        "guide_skipped_steps": info["skipped"],
+        "per_view": per_view,
+    }
+    log(f"guided batch: {json.dumps({k: v for k, v in out.items() if k != 'per_view'})}")
+    return out
+
+
 def summarize_pmc(fetch_dir, write_dir, kernel_prefix, out_path):
     """Per-dispatch HBM bytes of one kernel from two rocprofv3 --pmc runs
     (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by the factor that
@@ -246,6 +362,12 @@ def summarize_pmc(fetch_dir, write_dir, kernel_prefix, out_path):
     print(json.dumps(out))
 
 
+def run_guided_batch(pt, full):
+    if full:
+        return batched_guided_experiment(pt, 16, 16, 1920, 1080, 1024, 256, 64, [512, 1024, 1448], 0)
+    return batched_guided_experiment(pt, 4, 4, 480, 270, 4096, 1024, 64, [256, 512], 0)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-pmc":
         # bench.py --summarize-pmc FETCH_DIR WRITE_DIR KERNEL OUT_JSON
@@ -270,6 +392,12 @@ def main():
     ap.add_argument("--batched-views", type=int, default=64)
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched experiment and print its JSON (profiling runs)")
+    ap.add_argument("--no-guided-batch", action="store_true", help="skip the guided + batched experiment")
+    ap.add_argument("--guided-batch-only", action="store_true",
+                    help="run only the guided + batched experiment and print its JSON")
+    ap.add_argument("--guided-batch-full", action="store_true",
+                    help="guided + batched at full scale: 16 + 16 views at 1920x1080, terrain grids up to "
+                         "1448^2 (4.2M tris)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -285,6 +413,9 @@ def main():
     if args.guided_only:
         print(json.dumps(guided_experiment(pt, args.guided_ref_spp, args.guided_plain_spp, args.guided_train_spp)),
               flush=True)
+        return
+    if args.guided_batch_only:
+        print(json.dumps(run_guided_batch(pt, args.guided_batch_full)), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -367,6 +498,9 @@ def main():
     batched = None
     if not args.no_batched:
         batched = batched_experiment(pt, args.batched_views, 0)
+    guided_batch = None
+    if not args.no_guided_batch:
+        guided_batch = run_guided_batch(pt, args.guided_batch_full)
 
     result = {
         "metric": "Msamples/s at 1080p on 1 GPU; relMSE at equal time, guided vs plain PT",
@@ -409,6 +543,7 @@ def main():
         "relmse_guided_vs_plain": guided["ratio"] if guided else None,
         "guided": guided,
         "batched": batched,
+        "guided_batched": guided_batch,
     }
     print(json.dumps(result), flush=True)
 

@@ -1290,7 +1290,12 @@ void trace_path(const World &w, const or_params &p, uint32_t max_depth, uint32_t
         BRec br;
         br.wo = to_local(neg(ray_d), g.n);
         bsdf_sample(bsdf, br, rng);
-        if (iszero3(mul(br.f, fabsf(br.wi.z))) || iszero(br.pdf)) break;
+        // The sample's value and pdf must be finite (a GGX reflection exactly in
+        // the tangent plane evaluates 0 / 0 in rough_dielectric.h:39-48); the
+        // reference would carry the NaN into the film, the path ends here.
+        if (iszero3(mul(br.f, fabsf(br.wi.z))) || iszero(br.pdf) || !std::isfinite(br.f.x + br.f.y + br.f.z) ||
+            !std::isfinite(br.pdf))
+            break;
         throughput = mul(throughput, divs(mul(br.f, fabsf(br.wi.z)), br.pdf));
         ray_o = g.pos;
         ray_d = to_world(br.wi, g.n);

@@ -154,6 +154,22 @@ __device__ __forceinline__ uint32_t wave_fetch(uint32_t *work, uint32_t n) {
     return __shfl(base, 0);
 }
 
+// Work distribution of the persistent traversal kernels, 64 items per wave
+// at a time: a wave's first chunk is fixed by its index, later chunks come
+// from the shared cursor (which starts at zero) past the statically assigned
+// range.  When the queue is shorter than the grid, idle waves exit without
+// touching the cursor: one cursor atomic per wave would serialize the whole
+// grid on one address even when there is nothing to trace.
+struct WaveWork {
+    uint32_t base, dyn0;
+    __device__ __forceinline__ WaveWork() {
+        const uint32_t wpb = blockDim.x >> 6;
+        base = (blockIdx.x * wpb + (threadIdx.x >> 6)) * 64u;
+        dyn0 = gridDim.x * wpb * 64u;
+    }
+    __device__ __forceinline__ void next(uint32_t *work) { base = dyn0 + wave_fetch(work, 64u); }
+};
+
 // --- closest-hit traversal ------------------------------------------------------
 // Persistent grid sized to the resident occupancy; each wave pulls 64 rays at
 // a time, so waves that drew short traversals keep working.
@@ -168,8 +184,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(PathQueue q, DeviceBvh bv
     const uint32_t count = *count_ptr;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraverseStats st{0, 0};
-    for (;;) {
-        const uint32_t base = wave_fetch(work, 64u);
+    for (WaveWork ww;; ww.next(work)) {
+        const uint32_t base = ww.base;
         if (base >= count) break;
         const uint32_t i = base + lane_id();
         if (i < count) {
@@ -199,8 +215,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_shadow(ShadowQueue sq, DeviceBv
     const uint32_t count = *count_ptr;
     const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     TraverseStats st{0, 0};
-    for (;;) {
-        const uint32_t base = wave_fetch(work, 64u);
+    for (WaveWork ww;; ww.next(work)) {
+        const uint32_t base = ww.base;
         if (base >= count) break;
         const uint32_t i = base + lane_id();
         bool occluded = true;
@@ -410,9 +426,10 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQu
                                                              uint2 *gout, unsigned long long *queries,
                                                              uint32_t path_bits) {
     __shared__ half8 wl[4 * 4 * 2 * 64];
+    const uint32_t waves = kInferBlock / 64;
+    if (blockIdx.x * waves * 64u >= *count_ptr) return;  // no slot for this block: skip the weight load
     for (uint32_t t = threadIdx.x; t < 4 * 4 * 2 * 64; t += kInferBlock) wl[t] = G.fwd_a[t];
     __syncthreads();
-    const uint32_t waves = kInferBlock / 64;
     const uint32_t wave = threadIdx.x >> 6;
     const int lane = (int)lane_id();
     const int g = lane >> 4, il = lane & 15;
@@ -1810,33 +1827,66 @@ int batch_upload(pt_ctx *ctx) {
 
 // Packs every view's frames (views in order, frames in order) into launches
 // of at most `capacity` paths.  With `train`, frame f of a view trains the
-// guide when train_spp == 0 or f < train_spp.
+// guide when train_spp == 0 or f < train_spp; training frames come first,
+// round-robin over the views (the next k frames of every view still
+// training, k chosen so that one round fills about `train_capacity` paths),
+// so that the shared network learns from all views at once, then the other
+// frames view by view.  A launch holds either training frames only, at most
+// `train_capacity` paths of them (and at least one frame), or none.
 void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::vector<LaunchDesc> &plan,
-                bool train = false, uint32_t train_spp = 0) {
+                bool train = false, uint32_t train_spp = 0, uint32_t train_capacity = 0) {
+    // (view, frame) in launch order
+    std::vector<std::pair<uint32_t, uint32_t>> order;
+    std::vector<uint32_t> trained(b.views.size(), 0);
+    if (train) {
+        for (size_t vi = 0; vi < b.views.size(); ++vi)
+            trained[vi] = train_spp == 0 ? b.views[vi].spp : std::min(train_spp, b.views[vi].spp);
+        for (uint32_t f0 = 0;;) {
+            uint64_t active = 0;
+            for (size_t vi = 0; vi < b.views.size(); ++vi)
+                if (trained[vi] > f0) active += (uint64_t)b.views[vi].width * b.views[vi].height;
+            if (active == 0) break;
+            const uint32_t k = (uint32_t)std::max<uint64_t>(1, train_capacity / active);
+            for (uint32_t vi = 0; vi < (uint32_t)b.views.size(); ++vi)
+                for (uint32_t f = f0; f < std::min(f0 + k, trained[vi]); ++f) order.emplace_back(vi, f);
+            f0 += k;
+        }
+    }
+    for (uint32_t vi = 0; vi < (uint32_t)b.views.size(); ++vi)
+        for (uint32_t f = trained[vi]; f < b.views[vi].spp; ++f) order.emplace_back(vi, f);
     plan.clear();
     LaunchDesc cur;
     std::memset(&cur, 0, sizeof(cur));
+    // A view has at most one segment per launch: its frames are folded into
+    // its film by one thread per pixel, in order.
+    std::vector<uint8_t> in_launch(b.views.size(), 0);
     auto flush = [&]() {
         if (cur.n_entries) plan.push_back(cur);
+        for (uint32_t k = 0; k < cur.n_segs; ++k) in_launch[cur.g[k].view] = 0;
         std::memset(&cur, 0, sizeof(cur));
     };
-    for (uint32_t vi = 0; vi < (uint32_t)b.views.size(); ++vi) {
+    for (const auto &vf : order) {
+        const uint32_t vi = vf.first, f = vf.second;
         const BatchView &v = b.views[vi];
         const uint32_t npix = v.width * v.height;
-        for (uint32_t f = 0; f < v.spp; ++f) {
+        {
             const bool new_seg = cur.n_segs == 0 || cur.g[cur.n_segs - 1].view != vi;
-            if ((uint64_t)cur.total_paths + npix > capacity || cur.n_entries == kMaxLaunchEntries ||
-                (new_seg && cur.n_segs == kMaxLaunchSegs))
+            const bool train_frame = f < trained[vi];
+            const bool train_launch = cur.train_frames > 0;
+            const uint32_t cap = train_frame ? std::min(capacity, train_capacity) : capacity;
+            if ((uint64_t)cur.total_paths + npix > cap || cur.n_entries == kMaxLaunchEntries ||
+                (new_seg && (cur.n_segs == kMaxLaunchSegs || in_launch[vi])) ||
+                (cur.n_entries && train_frame != train_launch))
                 flush();
             const bool seg_start = cur.n_segs == 0 || cur.g[cur.n_segs - 1].view != vi;
             LaunchEntry &e = cur.e[cur.n_entries];
-            const bool train_frame = train && (train_spp == 0 || f < train_spp);
             e.view = vi | (train_frame ? kEntryTrain : 0u);
             if (train_frame) cur.train_frames += 1;
             e.frame_seed = v.random_seed + f;
             e.path_begin = cur.total_paths;
             e.pixels = npix;
             if (seg_start) {
+                in_launch[vi] = 1;
                 LaunchSeg &g = cur.g[cur.n_segs++];
                 g.view = vi;
                 g.first_entry = cur.n_entries;
@@ -1908,7 +1958,7 @@ struct BatchGuide {
 
 // The kernels of one launch (the sequence a hipGraph captures).
 void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth, const ShadeParams &sp,
-                          uint32_t path_bits, bool fp64, const BatchGuide &bg) {
+                          uint32_t path_bits, bool fp64, const BatchGuide &bg, bool timing = false) {
     BatchState &b = ctx->batch;
     SceneView sv;
     sv.tri = b.d_tri;
@@ -1937,29 +1987,44 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
     int cur = 0;
     for (uint32_t bb = 0; bb < max_depth; ++bb) {
         const int nxt = cur ^ 1;
-        hipLaunchKernelGGL(trace_kernel(false, !b.spheres.empty()), dim3(tg), dim3(kTraceBlock), 0, ctx->stream,
-                           ctx->q[cur], bvh, ext + bb, twork + bb, ctx->d_stat, (const uint32_t *)b.d_roots,
-                           path_bits);
+        {
+            EventTimer t(ctx, timing);
+            hipLaunchKernelGGL(trace_kernel(false, !b.spheres.empty()), dim3(tg), dim3(kTraceBlock), 0, ctx->stream,
+                               ctx->q[cur], bvh, ext + bb, twork + bb, ctx->d_stat, (const uint32_t *)b.d_roots,
+                               path_bits);
+            t.stop(kKindTrace);
+        }
         if (bg.infer || bg.train) {
             GuideShadeArgs ga = bg.ga;
             const bool infer_here = bg.infer && bb < bg.gdepth && bb + 1 < max_depth;
             if (!infer_here) ga.gout = nullptr;
-            if (infer_here)
+            if (infer_here) {
+                EventTimer t(ctx, timing);
                 hipLaunchKernelGGL(k_guide_infer, dim3(ctx->num_cu * ctx->infer_blocks_per_cu), dim3(kInferBlock), 0,
                                    ctx->stream, bg.view, ctx->q[cur], ext + bb,
                                    reinterpret_cast<uint2 *>(ctx->d_gout), ctx->d_stat + 4, path_bits);
+                t.stop(kKindGuideInfer);
+            }
+            EventTimer t(ctx, timing);
             hipLaunchKernelGGL(k_shade_guided<true>, dim3((uint32_t)ctx->num_cu * sh.shade_bpc), dim3(256), 0,
                                ctx->stream, sv, sp, ctx->q[cur], ext + bb, ctx->q[nxt], ext + bb + 1, ctx->sq,
                                shd + bb, ctx->d_out_l, aov, ga, (const BatchViewDev *)b.d_views, path_bits,
                                (const LaunchDesc *)b.d_launch);
+            t.stop(kKindShade);
         } else {
+            EventTimer t(ctx, timing);
             hipLaunchKernelGGL(k_shade, dim3(sg), dim3(sh.shade_block), 0, ctx->stream, sv, sp, ctx->q[cur],
                                ext + bb, ctx->q[nxt], ext + bb + 1, ctx->sq, shd + bb, ctx->d_out_l, aov,
                                (const BatchViewDev *)b.d_views, path_bits);
+            t.stop(kKindShade);
         }
-        hipLaunchKernelGGL(shadow_kernel(false, !b.spheres.empty()), dim3(tg), dim3(kTraceBlock), 0, ctx->stream,
-                           ctx->sq, bvh, shd + bb, swork + bb, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2,
-                           (const uint32_t *)b.d_roots, path_bits);
+        {
+            EventTimer t(ctx, timing);
+            hipLaunchKernelGGL(shadow_kernel(false, !b.spheres.empty()), dim3(tg), dim3(kTraceBlock), 0, ctx->stream,
+                               ctx->sq, bvh, shd + bb, swork + bb, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2,
+                               (const uint32_t *)b.d_roots, path_bits);
+            t.stop(kKindShadow);
+        }
         cur = nxt;
     }
     hipLaunchKernelGGL(k_accumulate_batch, dim3((uint32_t)ctx->num_cu * 16u), dim3(256), 0, ctx->stream,
@@ -1967,9 +2032,19 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
     hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters, max_depth, b.d_rays);
 }
 
+// Width of a launch of guide-training frames: at most 2^21 paths (and at
+// least one frame of the largest view), the training launch width of
+// pt_render, so that the network is updated every ~2M paths.
+constexpr uint32_t kTrainLaunchPaths = 1u << 21;
+uint32_t batch_train_capacity(const BatchState &b, const BatchShape &sh) {
+    uint32_t max_npix = 1;
+    for (const auto &v : b.views) max_npix = std::max(max_npix, v.width * v.height);
+    return std::min(sh.capacity, std::max(kTrainLaunchPaths, max_npix));
+}
+
 // Checks and allocations of a guided batched render: network outputs per
-// queue slot, training records for the launch width, samples for the
-// optimizer steps of the fullest training launch.
+// queue slot, training records for the training launch width, samples for
+// the optimizer steps of the fullest training launch.
 int batch_guide_setup(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, uint32_t max_depth,
                       BatchGuide &bg) {
     if (!bg.infer && !bg.train) return PT_OK;
@@ -1996,7 +2071,8 @@ int batch_guide_setup(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &s
     }
     if (bg.train) {
         const pt_guide_config &gc = gs.cfg;
-        const uint64_t slots = ((uint64_t)sh.capacity + gc.train_stride - 1) / gc.train_stride;
+        const uint32_t train_cap = batch_train_capacity(ctx->batch, sh);
+        const uint64_t slots = ((uint64_t)train_cap + gc.train_stride - 1) / gc.train_stride;
         const uint64_t records = slots * max_depth;
         if (records > 0xFFFFFFFFull / 2) {
             ctx->error = "pt_batch_render: too many guide training records per launch (raise train_stride)";
@@ -2004,7 +2080,7 @@ int batch_guide_setup(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &s
         }
         uint32_t max_frames = 1;
         for (const auto &v : ctx->batch.views)
-            max_frames = std::max(max_frames, std::min(kMaxLaunchEntries, sh.capacity / (v.width * v.height)));
+            max_frames = std::max(max_frames, std::min(kMaxLaunchEntries, train_cap / (v.width * v.height)));
         if (!gs.ensure_records((uint32_t)records, ctx->error) ||
             !gs.ensure_samples(guide_sample_count((uint64_t)gc.steps_per_frame * max_frames, gc.batch_size),
                                ctx->error))
@@ -2044,7 +2120,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     bg.infer = (p.render_flags & PT_RENDER_GUIDE) != 0;
     bg.train = (p.render_flags & PT_RENDER_GUIDE_TRAIN) != 0;
     std::vector<LaunchDesc> plan;
-    batch_plan(b, sh.capacity, true, plan, bg.train, p.guide_train_spp);
+    batch_plan(b, sh.capacity, true, plan, bg.train, p.guide_train_spp, batch_train_capacity(b, sh));
     if (plan.size() > b.h_launch_slots) {
         if (b.h_launch) (void)hipHostFree(b.h_launch);
         b.h_launch = nullptr;
@@ -2059,6 +2135,9 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     spar.aov = 0;
     const bool fp64 = (p.render_flags & PT_RENDER_FP64_ACCUM) != 0;
     if ((rc = batch_guide_setup(ctx, p, sh, max_depth, bg)) != PT_OK) return rc;
+    // Per-kernel event timing (direct launches only; around the training
+    // steps in both modes).
+    const bool timing = (p.render_flags & PT_RENDER_TIMING) != 0;
     hipGraph_t g = nullptr;
     hipGraphExec_t ge = nullptr;
     if (graph) {
@@ -2086,11 +2165,12 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
         if (e == hipSuccess) {
             if (graph) e = hipGraphLaunch(ge, ctx->stream);
             else {
-                batch_enqueue_launch(ctx, sh, max_depth, spar, path_bits, fp64, bg);
+                batch_enqueue_launch(ctx, sh, max_depth, spar, path_bits, fp64, bg, timing);
                 e = hipGetLastError();
             }
         }
         if (e == hipSuccess && train_here) {
+            EventTimer tt(ctx, timing);
             // Samples from this launch's training frames, then the optimizer
             // steps (Adam over the hash grids of the views that trained).
             uint32_t v_lo = 0xFFFFFFFFu, v_hi = 0;
@@ -2110,6 +2190,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
             }
             gs.phase = (gs.phase + 1u) % gs.cfg.train_stride;
             ctx->stats.guide_steps += steps;
+            tt.stop(kKindGuide);
         }
         if (e != hipSuccess) {
             ctx->error = std::string("pt_batch_render: ") + hipGetErrorString(e);
@@ -2119,6 +2200,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     hipError_t e = hipStreamSynchronize(ctx->stream);
     if (ge) (void)hipGraphExecDestroy(ge);
     if (g) (void)hipGraphDestroy(g);
+    if (timing) collect_event_times(ctx);
     if (rc != PT_OK) return rc;
     if (e != hipSuccess) {
         ctx->error = std::string("pt_batch_render: ") + hipGetErrorString(e);

@@ -76,6 +76,10 @@ PT_HD float minf(float a, float b) { return fminf(a, b); }
 // IsZero / MISWeight / GetLuminance: optix/util.h:206-220, 202-204.
 PT_HD bool is_zero(float v) { return fabsf(v) < kEps; }
 PT_HD bool is_zero(f3 v) { return fabsf(v.x) < kEps && fabsf(v.y) < kEps && fabsf(v.z) < kEps; }
+// A BSDF sample the path can continue with: its value and pdf are finite.
+// (Degenerate directions, e.g. a GGX reflection exactly in the tangent plane,
+// evaluate 0 / 0.)
+PT_HD bool finite_sample(f3 f, float pdf) { return isfinite(f.x + f.y + f.z) && isfinite(pdf); }
 PT_HD float mis_weight(float x, float y) { return x / (x + y); }
 PT_HD float luminance(f3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
 

@@ -447,10 +447,15 @@ double fbm_height(double x, double z, uint64_t seed) {
 const float kAuEta[3] = {0.14282f, 0.37414f, 1.43944f};
 const float kAuK[3] = {3.97472f, 2.38066f, 1.59981f};
 
+// Opaque types are two-sided: the icospheres sink into the heightfield, so
+// paths reach their inner faces, where a one-sided conductor would evaluate
+// its Fresnel term at cos < 0 (reflectance above 1; conductor.h:27-34 with
+// fresnel.h:31-49 of the reference) and plastics would go black.
 pt_material_desc random_material(SplitMix &rng, int type) {
     pt_material_desc m;
     std::memset(&m, 0, sizeof(m));
     m.type = (uint32_t)type;
+    m.twosided = (type == PT_MAT_DIELECTRIC || type == PT_MAT_ROUGH_DIELECTRIC) ? 0u : 1u;
     m.int_ior = 1.5f;
     m.ext_ior = 1.0f;
     for (int i = 0; i < 4; ++i) m.tex[i] = make_rgb_texture(0.f, 0.f, 0.f);

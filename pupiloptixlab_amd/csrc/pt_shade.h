@@ -291,7 +291,7 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
         const f3 wo_l = to_local(-ps.d, g.normal);
         BsdfSample bs;
         bsdf_sample(bsdf, wo_l, ps.rng, bs);
-        if (is_zero(bs.f * fabsf(bs.wi.z)) || is_zero(bs.pdf)) {
+        if (is_zero(bs.f * fabsf(bs.wi.z)) || is_zero(bs.pdf) || !finite_sample(bs.f, bs.pdf)) {
             out.done = true;
             return;
         }

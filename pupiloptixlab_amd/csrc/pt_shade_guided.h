@@ -221,7 +221,7 @@ PT_HD void guided_scatter(const SceneView &s, PathState &ps, const GuidedVertex 
     const bool record = v.guidable && !delta && training_path(rs, path, ps.flags);
     const uint32_t ridx = record ? record_index(rs, path, v.depth - 1u) : 0u;
     const size_t rn = rs.capacity;
-    if (is_zero(f * fabsf(wi_l.z)) || is_zero(p)) {
+    if (is_zero(f * fabsf(wi_l.z)) || is_zero(p) || !finite_sample(f, p)) {
         // A direction that carries nothing: recorded complete with a zero
         // target, so the guide learns to avoid it.
         if (record && ridx < rs.capacity) {
