@@ -4,7 +4,11 @@
 //   OpenEXR  scanline files, compression NONE / RLE / ZIPS / ZIP, pixel types
 //            UINT / HALF / FLOAT (channels R, G, B, A or Y);
 //   Radiance RGBE (.hdr / .pic), flat and new run-length scanlines;
-//   PNG      8- and 16-bit (and 1/2/4-bit gray / palette), non-interlaced.
+//   PNG      8- and 16-bit (and 1/2/4-bit gray / palette), non-interlaced;
+//   BMP      uncompressed palette / 16 / 24 / 32-bit, bit fields;
+//   TGA      color-mapped, true-color and gray, raw or run-length encoded;
+//   PNM      binary P5 / P6;
+//   JPEG     baseline and extended sequential Huffman (pt_jpeg.cpp).
 // Output: RGBA float, row 0 = the first (top) row of the file, as both
 // loaders of the reference return it.  8-bit data is linearized with
 // (v / 255)^2.2 as in util/texture.cpp:106-111 (16-bit PNG channels keep their
@@ -541,7 +545,289 @@ bool load_png(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
     return true;
 }
 
+// 8-bit pixels (1-4 channels, row 0 = top) to linear RGBA float: color
+// channels (v / 255)^2.2, alpha v / 255; gray is expanded to RGB.
+void rgba_from_8bit(const std::vector<unsigned char> &px, int chans, uint32_t W, uint32_t H, std::vector<float> &rgba) {
+    float lut[256];
+    for (int i = 0; i < 256; ++i) lut[i] = std::pow(i * 1.f / 255.f, 2.2f);
+    rgba.assign((size_t)W * H * 4, 0.f);
+    for (size_t i = 0; i < (size_t)W * H; ++i) {
+        const unsigned char *s = &px[i * chans];
+        const unsigned char r = s[0], g = chans >= 3 ? s[1] : s[0], b = chans >= 3 ? s[2] : s[0];
+        const unsigned char a = chans == 4 ? s[3] : (chans == 2 ? s[1] : 255);
+        rgba[4 * i + 0] = lut[r];
+        rgba[4 * i + 1] = lut[g];
+        rgba[4 * i + 2] = lut[b];
+        rgba[4 * i + 3] = a * 1.f / 255.f;
+    }
+}
+
+uint32_t le16(const unsigned char *p) { return (uint32_t)p[0] | (uint32_t)p[1] << 8; }
+uint32_t le32(const unsigned char *p) { return le16(p) | le16(p + 2) << 16; }
+
+// ------------------------------------------------------------------ BMP
+// Windows bitmap: BITMAPINFOHEADER and later headers, uncompressed 1/4/8-bit
+// palette and 24/32-bit pixels, 16/32-bit BI_BITFIELDS; bottom-up or top-down.
+bool load_bmp(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std::vector<float> &rgba,
+              std::string &err) {
+    if (f.size() < 54 || f[0] != 'B' || f[1] != 'M') {
+        err = "parse: not a BMP file";
+        return false;
+    }
+    const uint32_t off = le32(&f[10]), hsize = le32(&f[14]);
+    if (hsize < 40 || 14 + (size_t)hsize > f.size()) {
+        err = "unsupported: BMP header";
+        return false;
+    }
+    const int32_t W = (int32_t)le32(&f[18]), Hs = (int32_t)le32(&f[22]);
+    const uint32_t bpp = le16(&f[28]), comp = le32(&f[30]);
+    uint32_t ncolors = le32(&f[46]);
+    const bool top_down = Hs < 0;
+    const uint32_t H = (uint32_t)(top_down ? -(int64_t)Hs : Hs);
+    if (W <= 0 || H == 0 || W > 65536 || H > 65536) {
+        err = "parse: bad BMP size";
+        return false;
+    }
+    uint32_t mask[4] = {0x00FF0000u, 0x0000FF00u, 0x000000FFu, 0u};
+    if (bpp == 16) {
+        mask[0] = 0x7C00u;
+        mask[1] = 0x03E0u;
+        mask[2] = 0x001Fu;
+    }
+    if (comp == 3 || comp == 6) {  // BI_BITFIELDS / BI_ALPHABITFIELDS
+        const size_t mp = hsize >= 52 ? 54 : 14 + hsize;
+        if (bpp != 16 && bpp != 32) {
+            err = "unsupported: BMP bit fields at this depth";
+            return false;
+        }
+        if (mp + 12 > f.size()) {
+            err = "parse: truncated BMP";
+            return false;
+        }
+        for (int k = 0; k < 3; ++k) mask[k] = le32(&f[mp + 4 * k]);
+        mask[3] = (hsize >= 56 || comp == 6) && mp + 16 <= f.size() ? le32(&f[mp + 12]) : 0u;
+    } else if (comp != 0) {
+        err = "unsupported: compressed BMP";
+        return false;
+    } else if (bpp == 32) {
+        mask[3] = 0u;  // plain 32-bit BMP: the fourth byte is not alpha
+    }
+    if (bpp != 1 && bpp != 4 && bpp != 8 && bpp != 16 && bpp != 24 && bpp != 32) {
+        err = "unsupported: BMP bit depth";
+        return false;
+    }
+    std::vector<unsigned char> pal;
+    if (bpp <= 8) {
+        if (ncolors == 0) ncolors = 1u << bpp;
+        const size_t pp = 14 + hsize;
+        if (ncolors > 256 || pp + 4 * (size_t)ncolors > f.size()) {
+            err = "parse: bad BMP palette";
+            return false;
+        }
+        pal.assign(&f[pp], &f[pp] + 4 * (size_t)ncolors);
+    }
+    const size_t stride = (((size_t)W * bpp + 31) / 32) * 4;
+    if ((size_t)off + stride * H > f.size()) {
+        err = "parse: truncated BMP";
+        return false;
+    }
+    auto field = [](uint32_t v, uint32_t m) -> unsigned char {
+        if (m == 0) return 255;
+        int shift = 0;
+        while (!((m >> shift) & 1u)) ++shift;
+        const uint32_t bits = m >> shift;
+        const uint32_t x = (v & m) >> shift;
+        return (unsigned char)((x * 255u + bits / 2) / bits);
+    };
+    const bool has_alpha = mask[3] != 0;
+    const int chans = has_alpha ? 4 : 3;
+    std::vector<unsigned char> px((size_t)W * H * chans);
+    for (uint32_t y = 0; y < H; ++y) {
+        const unsigned char *row = &f[off + stride * (top_down ? y : H - 1 - y)];
+        for (int32_t x = 0; x < W; ++x) {
+            unsigned char *d = &px[((size_t)y * W + x) * chans];
+            if (bpp <= 8) {
+                const size_t bit = (size_t)x * bpp;
+                const uint32_t idx = (row[bit / 8] >> (8 - bpp - (bit % 8))) & ((1u << bpp) - 1u);
+                if (idx >= ncolors) {
+                    err = "parse: BMP palette index out of range";
+                    return false;
+                }
+                d[0] = pal[4 * idx + 2];
+                d[1] = pal[4 * idx + 1];
+                d[2] = pal[4 * idx + 0];
+            } else if (bpp == 24) {
+                d[0] = row[3 * x + 2];
+                d[1] = row[3 * x + 1];
+                d[2] = row[3 * x + 0];
+            } else {
+                const uint32_t v = bpp == 16 ? le16(&row[2 * x]) : le32(&row[4 * x]);
+                for (int k = 0; k < 3; ++k) d[k] = field(v, mask[k]);
+                if (has_alpha) d[3] = field(v, mask[3]);
+            }
+        }
+    }
+    w = (uint32_t)W;
+    h = H;
+    rgba_from_8bit(px, chans, w, h, rgba);
+    return true;
+}
+
+// ------------------------------------------------------------------ TGA
+// Truevision TGA: color-mapped (1, 9), true-color (2, 10) and gray (3, 11)
+// images, raw or run-length encoded, 8/15/16/24/32 bits; origin bit honored.
+bool load_tga(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std::vector<float> &rgba,
+              std::string &err) {
+    if (f.size() < 18) {
+        err = "parse: not a TGA file";
+        return false;
+    }
+    const uint32_t id_len = f[0], cmap_type = f[1], type = f[2];
+    const uint32_t cmap_first = le16(&f[3]), cmap_len = le16(&f[5]), cmap_bits = f[7];
+    const uint32_t W = le16(&f[12]), H = le16(&f[14]), bpp = f[16], desc = f[17];
+    const bool rle = type >= 9;
+    const uint32_t base = rle ? type - 8 : type;
+    if ((base != 1 && base != 2 && base != 3) || W == 0 || H == 0) {
+        err = "unsupported: TGA image type";
+        return false;
+    }
+    if ((base == 1 && (bpp != 8 || cmap_type != 1)) || (base == 2 && bpp != 15 && bpp != 16 && bpp != 24 && bpp != 32) ||
+        (base == 3 && bpp != 8)) {
+        err = "unsupported: TGA pixel depth";
+        return false;
+    }
+    size_t p = 18 + id_len;
+    const uint32_t cbytes = (cmap_bits + 7) / 8;
+    std::vector<unsigned char> cmap;
+    if (cmap_type == 1) {
+        if (p + (size_t)cmap_len * cbytes > f.size()) {
+            err = "parse: truncated TGA color map";
+            return false;
+        }
+        cmap.assign(&f[p], &f[p] + (size_t)cmap_len * cbytes);
+        p += (size_t)cmap_len * cbytes;
+    }
+    const bool alpha = (base == 1 ? cmap_bits : bpp) == 32;
+    const int chans = base == 3 ? 1 : (alpha ? 4 : 3);
+    auto decode = [&](const unsigned char *s, unsigned char *d) -> bool {
+        const unsigned char *c = s;
+        uint32_t nb = (bpp + 7) / 8;
+        if (base == 1) {
+            const uint32_t idx = s[0];
+            if (idx < cmap_first || idx - cmap_first >= cmap_len) return false;
+            c = &cmap[(size_t)(idx - cmap_first) * cbytes];
+            nb = cbytes;
+        }
+        if (base == 3) {
+            d[0] = c[0];
+        } else if (nb == 2) {
+            const uint32_t v = le16(c);
+            d[0] = (unsigned char)((((v >> 10) & 31u) * 255u + 15u) / 31u);
+            d[1] = (unsigned char)((((v >> 5) & 31u) * 255u + 15u) / 31u);
+            d[2] = (unsigned char)(((v & 31u) * 255u + 15u) / 31u);
+        } else {
+            d[0] = c[2];
+            d[1] = c[1];
+            d[2] = c[0];
+            if (chans == 4) d[3] = c[3];
+        }
+        return true;
+    };
+    const uint32_t sbytes = (bpp + 7) / 8;  // bytes per stored pixel
+    std::vector<unsigned char> px((size_t)W * H * chans);
+    const size_t n = (size_t)W * H;
+    size_t i = 0;
+    while (i < n) {
+        uint32_t count = 1;
+        bool run = false;
+        if (rle) {
+            if (p >= f.size()) break;
+            const unsigned char hdr = f[p++];
+            count = (hdr & 0x7Fu) + 1u;
+            run = (hdr & 0x80u) != 0;
+        }
+        for (uint32_t k = 0; k < count && i < n; ++k, ++i) {
+            const size_t src = run ? p : p + (size_t)k * sbytes;
+            if (src + sbytes > f.size() || !decode(&f[src], &px[i * chans])) {
+                err = "parse: truncated or bad TGA pixel data";
+                return false;
+            }
+        }
+        p += run ? sbytes : (size_t)count * sbytes;
+        if (!rle) {
+            p = 18 + id_len + cmap.size() + i * sbytes;
+        }
+    }
+    if (i < n) {
+        err = "parse: truncated TGA pixel data";
+        return false;
+    }
+    // Origin: bit 5 set = top-left, else bottom-left; bit 4 = right-to-left.
+    std::vector<unsigned char> out(px.size());
+    const bool top = (desc & 0x20u) != 0, right = (desc & 0x10u) != 0;
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            const uint32_t sy = top ? y : H - 1 - y, sx = right ? W - 1 - x : x;
+            std::memcpy(&out[((size_t)y * W + x) * chans], &px[((size_t)sy * W + sx) * chans], chans);
+        }
+    w = W;
+    h = H;
+    rgba_from_8bit(out, chans, w, h, rgba);
+    return true;
+}
+
+// ------------------------------------------------------------------ PNM
+// Binary netpbm: P5 (gray) and P6 (RGB), maxval up to 65535 (high byte kept).
+bool load_pnm(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std::vector<float> &rgba,
+              std::string &err) {
+    if (f.size() < 3 || f[0] != 'P' || (f[1] != '5' && f[1] != '6')) {
+        err = "unsupported: PNM variant (P5 and P6 are read)";
+        return false;
+    }
+    const int chans = f[1] == '6' ? 3 : 1;
+    size_t p = 2;
+    uint32_t v[3];
+    for (int k = 0; k < 3; ++k) {
+        while (p < f.size()) {
+            if (f[p] == '#') {
+                while (p < f.size() && f[p] != '\n') ++p;
+            } else if (std::isspace(f[p])) {
+                ++p;
+            } else {
+                break;
+            }
+        }
+        uint64_t x = 0;
+        size_t digits = 0;
+        while (p < f.size() && std::isdigit(f[p]) && digits < 10) x = x * 10 + (f[p++] - '0'), ++digits;
+        if (digits == 0 || x == 0 || x > 65535) {
+            err = "parse: bad PNM header";
+            return false;
+        }
+        v[k] = (uint32_t)x;
+    }
+    ++p;  // the single whitespace byte before the raster
+    const uint32_t W = v[0], H = v[1], maxv = v[2];
+    const size_t bpc = maxv > 255 ? 2 : 1;
+    if (p + (size_t)W * H * chans * bpc > f.size()) {
+        err = "parse: truncated PNM raster";
+        return false;
+    }
+    std::vector<unsigned char> px((size_t)W * H * chans);
+    for (size_t i = 0; i < px.size(); ++i) {
+        const uint32_t s = bpc == 2 ? ((uint32_t)f[p + 2 * i] << 8 | f[p + 2 * i + 1]) : f[p + i];
+        px[i] = (unsigned char)(((uint64_t)s * 255u + maxv / 2) / maxv);
+    }
+    w = W;
+    h = H;
+    rgba_from_8bit(px, chans, w, h, rgba);
+    return true;
+}
+
 }  // namespace
+
+bool load_jpeg(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std::vector<unsigned char> &px,
+               int &chans, std::string &err);
 
 bool load_image(const std::string &path, uint32_t &w, uint32_t &h, std::vector<float> &rgba, std::string &err) {
     std::vector<unsigned char> f;
@@ -551,8 +837,16 @@ bool load_image(const std::string &path, uint32_t &w, uint32_t &h, std::vector<f
     if (e == "exr") ok = load_exr(f, w, h, rgba, err);
     else if (e == "hdr" || e == "pic") ok = load_hdr(f, w, h, rgba, err);
     else if (e == "png") ok = load_png(f, w, h, rgba, err);
-    else {
-        err = "unsupported: image format ." + e + " (EXR, HDR and PNG are read)";
+    else if (e == "bmp") ok = load_bmp(f, w, h, rgba, err);
+    else if (e == "tga") ok = load_tga(f, w, h, rgba, err);
+    else if (e == "ppm" || e == "pgm" || e == "pnm") ok = load_pnm(f, w, h, rgba, err);
+    else if (e == "jpg" || e == "jpeg") {
+        std::vector<unsigned char> px;
+        int chans = 0;
+        ok = load_jpeg(f, w, h, px, chans, err);
+        if (ok) rgba_from_8bit(px, chans, w, h, rgba);
+    } else {
+        err = "unsupported: image format ." + e + " (EXR, HDR, PNG, JPEG, BMP, TGA and PNM are read)";
         return false;
     }
     if (!ok && err.find(path) == std::string::npos) err += " (" + path + ")";

@@ -746,8 +746,8 @@ bool load_xml_scene(const std::string &path, HostScene &out, std::string &err) {
     return true;
 }
 
-// Wavefront OBJ: v / vt / vn / f (polygons fan-triangulated); one vertex per
-// face corner.
+// Wavefront OBJ: v / vt / vn / f (polygons triangulated as below); one vertex
+// per face corner.
 bool load_obj_mesh(const std::string &path, MeshData &mesh, std::string &err) {
     std::ifstream f(path);
     if (!f) {
@@ -801,6 +801,37 @@ bool load_obj_mesh(const std::string &path, MeshData &mesh, std::string &err) {
                     return false;
                 }
                 face.push_back(c);
+            }
+            // Quads fan out from their concave (reflex) corner, if they have
+            // one (a quad has at most one), the triangulation assimp's
+            // aiProcess_Triangulate gives the reference (resource/shape.cpp:273)
+            // for quads; convex quads keep the fan from the first corner.
+            // Reflex: the turn at the corner opposes the polygon normal
+            // (Newell).  Larger polygons are fanned from their first corner.
+            if (face.size() == 4) {
+                double P[4][3];
+                for (int a = 0; a < 4; ++a)
+                    for (int c = 0; c < 3; ++c) P[a][c] = v[3 * (size_t)(face[(size_t)a].v - 1) + c];
+                double n[3] = {0.0, 0.0, 0.0};
+                for (int a = 0; a < 4; ++a) {
+                    const double *p0 = P[a], *p1 = P[(a + 1) % 4];
+                    n[0] += (p0[1] - p1[1]) * (p0[2] + p1[2]);
+                    n[1] += (p0[2] - p1[2]) * (p0[0] + p1[0]);
+                    n[2] += (p0[0] - p1[0]) * (p0[1] + p1[1]);
+                }
+                int start = 0;
+                for (int a = 0; a < 4; ++a) {
+                    const double *pp = P[(a + 3) % 4], *pc = P[a], *pn = P[(a + 1) % 4];
+                    const double e0[3] = {pc[0] - pp[0], pc[1] - pp[1], pc[2] - pp[2]};
+                    const double e1[3] = {pn[0] - pc[0], pn[1] - pc[1], pn[2] - pc[2]};
+                    const double t[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2],
+                                         e0[0] * e1[1] - e0[1] * e1[0]};
+                    if (t[0] * n[0] + t[1] * n[1] + t[2] * n[2] < 0.0) {
+                        start = a;
+                        break;
+                    }
+                }
+                std::rotate(face.begin(), face.begin() + start, face.end());
             }
             for (size_t k = 1; k + 1 < face.size(); ++k) {
                 for (const Corner &c : {face[0], face[k], face[k + 1]}) {

@@ -287,6 +287,35 @@ def test_xml_obj_shape(tmp_path):
     assert sh.flags & pt.SHAPE_FLIP_TEXCOORDS and sh.flags & pt.SHAPE_HAS_NORMALS
 
 
+def tri_area(p):
+    return 0.5 * abs((p[1, 0] - p[0, 0]) * (p[2, 1] - p[0, 1]) - (p[2, 0] - p[0, 0]) * (p[1, 1] - p[0, 1]))
+
+
+@pytest.mark.parametrize("rot", [0, 1, 2, 3])
+def test_obj_concave_quad_fans_from_the_concave_corner(tmp_path, rot):
+    """A dart-shaped quad (corner 2 concave) written with its corners rotated:
+    the two triangles must tile the quad (areas sum to its area, 0.5), which
+    only the fan from the concave corner does.  A convex quad keeps the fan
+    from its first corner (0,1,2), (0,2,3)."""
+    dart = [(0, 0), (2, 1), (1, 1), (2, 2)]  # reflex angle at (1, 1)
+    order = [(k + rot) % 4 for k in range(4)]
+    lines = [f"v {dart[k][0]} {dart[k][1]} 0" for k in order] + ["f 1 2 3 4"]
+    (tmp_path / "dart.obj").write_text("\n".join(lines) + "\n")
+    xml = """<scene version="3.0.0"><shape type="obj"><string name="filename" value="dart.obj"/></shape></scene>"""
+    s = pt.HostScene.load_xml(write(tmp_path, "d.xml", xml))
+    p = s.positions().reshape(-1, 3)
+    idx = np.ctypeslib.as_array(s.desc.indices, shape=(s.desc.num_triangles * 3,)).reshape(-1, 3)
+    areas = [tri_area(p[t]) for t in idx]
+    assert len(areas) == 2 and sum(areas) == pytest.approx(0.5, abs=1e-6)
+    # the concave corner (1, 1) is the shared fan apex
+    apex = p[idx[0, 0]]
+    assert tuple(apex[:2]) == (1.0, 1.0) and tuple(p[idx[1, 0]][:2]) == (1.0, 1.0)
+    (tmp_path / "sq.obj").write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1 2 3 4\n")
+    s = pt.HostScene.load_xml(write(tmp_path, "q.xml", xml.replace("dart.obj", "sq.obj")))
+    q = s.positions().reshape(-1, 3)[:, :2]
+    assert q.tolist() == [[0, 0], [1, 0], [1, 1], [0, 0], [1, 1], [0, 1]]
+
+
 @pytest.mark.parametrize("text,code", [
     ("<scene><shape type='cube'>", pt.STATUS and -6),
     ("<notscene/>", -6),

@@ -240,10 +240,141 @@ def test_exr_round_trip_with_writer(tmp_path):
     assert np.array_equal(got[..., :3], buf[::-1, :, :3]) and np.all(got[..., 3] == 1.0)
 
 
+def to8(img):
+    """Linear loader output back to the 8-bit levels it came from."""
+    return np.rint(np.clip(img, 0, 1) ** (1 / 2.2) * 255).astype(np.int32)
+
+
+def pil_image(seed, mode, w=37, h=23):
+    from PIL import Image
+    rng = np.random.default_rng(seed)
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = np.stack([xx * 255 // max(1, w - 1), yy * 255 // max(1, h - 1), (xx + yy) * 3 % 256], -1)
+    px = np.clip(base + rng.integers(-20, 21, base.shape), 0, 255).astype(np.uint8)
+    if mode == "RGBA":
+        px = np.concatenate([px, rng.integers(0, 256, (h, w, 1)).astype(np.uint8)], -1)
+    img = Image.fromarray(px if mode != "L" else px[..., 0], mode if mode in ("RGB", "RGBA", "L") else "RGB")
+    return img.convert(mode) if mode == "P" else img
+
+
+@pytest.mark.parametrize("mode", ["RGB", "RGBA", "L", "P"])
+def test_bmp_written_by_pillow(tmp_path, mode):
+    img = pil_image(1, mode)
+    path = str(tmp_path / "a.bmp")
+    img.save(path)
+    from PIL import Image
+    ref = np.asarray(Image.open(path).convert("RGBA"))
+    got = pt.load_image(path)
+    assert got.shape == ref.shape
+    assert np.array_equal(to8(got[..., :3]), ref[..., :3])
+    assert np.allclose(got[..., 3], ref[..., 3] / 255.0, atol=1e-7)
+
+
+def bmp_bytes(px, bpp, top_down=False, masks=None):
+    """Minimal BMP writer: 16-bit 565 or 32-bit with bit-field masks."""
+    h, w = px.shape[:2]
+    stride = (w * bpp + 31) // 32 * 4
+    rows = []
+    for y in (range(h) if top_down else range(h - 1, -1, -1)):
+        r = bytearray()
+        for x in range(w):
+            v = 0
+            for k, m in enumerate(masks):
+                if m:
+                    shift = (m & -m).bit_length() - 1
+                    bits = m >> shift
+                    v |= (int(px[y, x, k]) * bits // 255) << shift
+            r += v.to_bytes(bpp // 8, "little")
+        rows.append(bytes(r) + b"\0" * (stride - len(r)))
+    data = b"".join(rows)
+    hdr = struct.pack("<IiiHHIIiiII", 56, w, -h if top_down else h, 1, bpp, 3, len(data), 2835, 2835, 0, 0)
+    hdr += struct.pack("<IIII", *masks)
+    off = 14 + len(hdr)
+    return b"BM" + struct.pack("<IHHI", off + len(data), 0, 0, off) + hdr + data
+
+
+@pytest.mark.parametrize("bpp,top_down", [(16, False), (32, True)])
+def test_bmp_bitfields(tmp_path, bpp, top_down):
+    rng = np.random.default_rng(bpp)
+    px = rng.integers(0, 256, (5, 7, 4)).astype(np.uint8)
+    masks = (0xF800, 0x07E0, 0x001F, 0) if bpp == 16 else (0x00FF0000, 0x0000FF00, 0x000000FF, 0xFF000000)
+    got = pt.load_image(write(tmp_path, "b.bmp", bmp_bytes(px, bpp, top_down, masks)))
+    for k, m in enumerate(masks[:3]):
+        bits = m >> ((m & -m).bit_length() - 1)
+        q = px[..., k].astype(np.int64) * bits // 255
+        assert np.array_equal(to8(got[..., k]), (q * 255 + bits // 2) // bits)
+    if bpp == 32:
+        assert np.allclose(got[..., 3], px[..., 3] / 255.0, atol=1e-7)
+
+
+@pytest.mark.parametrize("mode,rle", [("RGB", False), ("RGBA", True), ("L", True), ("P", False), ("RGB", True)])
+def test_tga_written_by_pillow(tmp_path, mode, rle):
+    from PIL import Image
+    img = pil_image(2, mode)
+    arr = np.array(img)
+    if rle:  # long runs for the run-length coder
+        arr[3:9] = arr[3:4]
+        img = Image.fromarray(arr, img.mode)
+        if mode == "P":
+            img.putpalette(pil_image(2, mode).getpalette())
+    path = str(tmp_path / "a.tga")
+    img.save(path, compression="tga_rle" if rle else None)
+    ref = np.asarray(Image.open(path).convert("RGBA"))
+    got = pt.load_image(path)
+    assert np.array_equal(to8(got[..., :3]), ref[..., :3])
+    assert np.allclose(got[..., 3], ref[..., 3] / 255.0, atol=1e-7)
+
+
+def test_pnm_8_and_16_bit(tmp_path):
+    rng = np.random.default_rng(5)
+    px = rng.integers(0, 256, (6, 9, 3)).astype(np.uint8)
+    got = pt.load_image(write(tmp_path, "a.ppm", b"P6\n# comment\n9 6\n255\n" + px.tobytes()))
+    assert np.array_equal(to8(got[..., :3]), px)
+    g16 = rng.integers(0, 65536, (4, 5)).astype(">u2")
+    got = pt.load_image(write(tmp_path, "a.pgm", b"P5 5 4 65535\n" + g16.tobytes()))
+    expect = (g16.astype(np.int64) * 255 + 32767) // 65535
+    assert np.array_equal(to8(got[..., 0]), expect) and np.array_equal(got[..., 0], got[..., 2])
+
+
+@pytest.mark.parametrize("mode,subsampling", [("L", 0), ("RGB", 0), ("RGB", 1), ("RGB", 2)])
+def test_jpeg_against_pillow(tmp_path, mode, subsampling):
+    """Baseline JPEG written by Pillow (libjpeg): our decode vs libjpeg's.  The
+    standard leaves IDCT and upsampling rounding open, so levels may differ
+    by a few; parity with stb_image's rounding is unpinned."""
+    from PIL import Image
+    img = pil_image(3, mode, 61, 45)
+    path = str(tmp_path / "a.jpg")
+    kw = {} if mode == "L" else {"subsampling": subsampling}
+    img.save(path, quality=92, **kw)
+    ref = np.asarray(Image.open(path).convert("RGB")).astype(np.int32)
+    got = to8(pt.load_image(path)[..., :3])
+    d = np.abs(got - ref)
+    print(f"{mode} subsampling {subsampling}: max diff {d.max()} mean {d.mean():.3f}")
+    assert d.max() <= (2 if subsampling == 0 else 4) and d.mean() <= (0.1 if subsampling == 0 else 0.4)
+
+
+def test_jpeg_restart_markers_and_progressive(tmp_path):
+    from PIL import Image
+    img = pil_image(4, "RGB", 80, 40)
+    path = str(tmp_path / "r.jpg")
+    img.save(path, quality=90, subsampling=2, restart_marker_blocks=3)
+    assert b"\xff\xdd" in open(path, "rb").read()
+    ref = np.asarray(Image.open(path).convert("RGB")).astype(np.int32)
+    got = to8(pt.load_image(path)[..., :3])
+    assert np.abs(got - ref).max() <= 6
+    img.save(str(tmp_path / "p.jpg"), quality=90, progressive=True)
+    with pytest.raises(pt.PupilError) as e:
+        pt.load_image(str(tmp_path / "p.jpg"))
+    assert e.value.code == -8
+
+
 def test_image_errors(tmp_path):
     with pytest.raises(pt.PupilError) as e:
-        pt.load_image(write(tmp_path, "x.jpg", b"\xff\xd8\xff"))
+        pt.load_image(write(tmp_path, "x.gif", b"GIF89a"))
     assert e.value.code == -8
+    with pytest.raises(pt.PupilError) as e:
+        pt.load_image(write(tmp_path, "x.jpg", b"\xff\xd8\xff"))
+    assert e.value.code == -6
     with pytest.raises(pt.PupilError) as e:
         pt.load_image(write(tmp_path, "t.png", png_bytes(np.zeros((4, 4, 3), np.uint8), 2)[:40]))
     assert e.value.code == -6
