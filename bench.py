@@ -28,6 +28,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# Reference renders draw frames from this index on, so they share no samples
+# with the renders they judge (frames 0, 1, ...).
+REF_SEED = 1 << 24
 # Algorithmic bytes per unit ("Roofline accounting").
 RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 28, 16, 80, 36
 SHADOW_RESULT_BYTES = 4
@@ -154,6 +157,7 @@ def guided_experiment(pt, ref_spp, plain_spp, train_spp):
     t.build_bvh()
     mis = pt.MIS_CONSISTENT
     t0 = time.perf_counter()
+    t.set_random_seed(REF_SEED)  # reference samples independent of the tested frames 0, 1, ...
     t.render(spp=ref_spp, mis_mode=mis, fp64=True, aov=False)
     ref = t.read(pt.BUF_ACCUM64) / ref_spp
     t_ref = time.perf_counter() - t0
@@ -249,6 +253,7 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
         t.set_scene(scene)
         t.build_bvh()
         n = door_ref_spp if kind == "door" else terrain_ref_spp
+        t.set_random_seed(REF_SEED)
         t.render(spp=n, mis_mode=mis, fp64=True, aov=False)
         refs.append(t.read(pt.BUF_ACCUM64) / n)
         log(f"guided batch: reference {len(refs)}/{len(views)} ({kind}, {n} spp) at {time.perf_counter() - t_ref0:.1f}s")

@@ -250,6 +250,11 @@ int pt_build_bvh(pt_ctx *ctx, const pt_bvh_opts *opts);
 int pt_set_camera(pt_ctx *ctx, const float sample_to_camera[16], const float camera_to_world[16]);
 /* m_dirty restart of PTPass::OnRun (pt_pass.cpp:40-49). */
 int pt_reset_accumulation(pt_ctx *ctx);
+/* Frame index of the next frame (launch_params.random_seed, pt_pass.cpp:45,56,
+ * which the reference restarts at 0 with the accumulation).  Setting it keeps
+ * the accumulation; a large offset draws samples independent of frames
+ * 0, 1, ... (reference renders that must not share samples with a test). */
+int pt_set_random_seed(pt_ctx *ctx, uint32_t frame);
 /* RenderObject::UpdateTransform (world/render_object.cpp:72-75) with the
  * world's RenderInstanceTransform / RenderInstanceUpdate handling
  * (world/world.cpp:45-59): the shape's geometry becomes `transform` (3x4

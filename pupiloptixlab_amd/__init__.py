@@ -206,6 +206,7 @@ _SIGNATURES = {
     "pt_guide_get_info": (C.c_int, [C.c_void_p, C.POINTER(GuideInfo)]),
     "pt_guide_query": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "pt_guide_enable_batch": (C.c_int, [C.c_void_p, C.POINTER(GuideConfig)]),
+    "pt_set_random_seed": (C.c_int, [C.c_void_p, C.c_uint32]),
     "pt_guide_query_grid": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p]),
     "pt_guide_grid_bounds": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "pt_guide_train": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.POINTER(C.c_float)]),
@@ -410,6 +411,10 @@ class PathTracer:
 
     def reset(self) -> None:
         _check(lib().pt_reset_accumulation(self._ctx), "pt_reset_accumulation", self._ctx)
+
+    def set_random_seed(self, frame: int) -> None:
+        """Frame index of the next frame (samples of frame f are drawn from key f)."""
+        _check(lib().pt_set_random_seed(self._ctx, frame), "pt_set_random_seed", self._ctx)
 
     def set_shape_transform(self, shape: int, transform) -> None:
         """RenderObject::UpdateTransform: the shape's pt_set_scene geometry under

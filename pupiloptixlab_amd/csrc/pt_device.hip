@@ -1949,11 +1949,19 @@ uint32_t guide_sample_count(uint64_t steps, uint32_t batch) {
 
 // Guided batched render: network outputs per queue slot and the record sink
 // (epoch and phase of each launch are read from its LaunchDesc).
+constexpr uint32_t kMixRegions = 8;  // training launches whose samples one optimizer step mixes
+
 struct BatchGuide {
     bool infer = false, train = false;
     GuideView view{};
     GuideShadeArgs ga{};
     uint32_t gdepth = 0;  // vertices below this depth may sample the guide
+    // Training samples of the last `regions` training launches sit in
+    // interleaved regions of `region_cap` slots, so every optimizer step
+    // mixes several launches (views) instead of fitting the shared MLP to one
+    // view after another.
+    uint32_t regions = 1, region_cap = 0, next_region = 0, cursor = 0;
+    uint32_t region_lo[kMixRegions] = {0}, region_hi[kMixRegions] = {0};  // views of each region
 };
 
 // The kernels of one launch (the sequence a hipGraph captures).
@@ -2081,9 +2089,11 @@ int batch_guide_setup(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &s
         uint32_t max_frames = 1;
         for (const auto &v : ctx->batch.views)
             max_frames = std::max(max_frames, std::min(kMaxLaunchEntries, train_cap / (v.width * v.height)));
+        bg.region_cap = guide_sample_count((uint64_t)gc.steps_per_frame * max_frames, gc.batch_size);
+        bg.regions = std::min<uint32_t>(kMixRegions, (uint32_t)ctx->batch.views.size());
+        if ((uint64_t)bg.region_cap * bg.regions > (1ull << 28)) bg.regions = 1;
         if (!gs.ensure_records((uint32_t)records, ctx->error) ||
-            !gs.ensure_samples(guide_sample_count((uint64_t)gc.steps_per_frame * max_frames, gc.batch_size),
-                               ctx->error))
+            !gs.ensure_samples(bg.region_cap * bg.regions, ctx->error))
             return PT_ERR_HIP;
         bg.ga.rs.rec = reinterpret_cast<f4 *>(gs.rec);
         bg.ga.rs.capacity = gs.rec_cap;
@@ -2182,9 +2192,29 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
                 }
             const GuideRecords r{gs.rec, gs.rec_cap, gs.cfg.train_stride, hl.phase, max_depth, hl.epoch};
             const uint32_t steps = gs.cfg.steps_per_frame * hl.train_frames;
-            if (!gs.ensure_samples(guide_sample_count(steps, gs.cfg.batch_size), ctx->error) ||
-                !gs.collect(ctx->d_out_l, hl.total_paths, r, ctx->error) ||
-                !gs.train(steps, 0.f, v_lo, v_hi, ctx->error)) {
+            bool ok;
+            if (bg.regions > 1) {
+                const uint32_t reg = bg.next_region++ % bg.regions;
+                bg.region_lo[reg] = v_lo;
+                bg.region_hi[reg] = v_hi;
+                uint32_t u_lo = v_lo, u_hi = v_hi;
+                for (uint32_t k = 0; k < bg.regions; ++k)
+                    if (bg.region_hi[k] > bg.region_lo[k]) {
+                        u_lo = std::min(u_lo, bg.region_lo[k]);
+                        u_hi = std::max(u_hi, bg.region_hi[k]);
+                    }
+                const uint32_t window = bg.region_cap * bg.regions, B = gs.cfg.batch_size;
+                const uint32_t n = std::min(steps, bg.region_cap / B);  // about one pass over a launch's samples
+                ok = gs.collect(ctx->d_out_l, hl.total_paths, r, ctx->error, reg, bg.regions,
+                                std::min(bg.region_cap, guide_sample_count(steps, B))) &&
+                     gs.train_window(n, 0.f, u_lo, u_hi, bg.cursor, window, ctx->error);
+                bg.cursor = (uint32_t)(((uint64_t)bg.cursor + (uint64_t)n * B) % window);
+            } else {
+                ok = gs.ensure_samples(guide_sample_count(steps, gs.cfg.batch_size), ctx->error) &&
+                     gs.collect(ctx->d_out_l, hl.total_paths, r, ctx->error) &&
+                     gs.train(steps, 0.f, v_lo, v_hi, ctx->error);
+            }
+            if (!ok) {
                 rc = PT_ERR_HIP;
                 break;
             }
@@ -2465,6 +2495,12 @@ int pt_set_camera(pt_ctx *ctx, const float s2c[16], const float c2w[16]) {
         ctx->camera.c2w[r] = f4{c2w[4 * r], c2w[4 * r + 1], c2w[4 * r + 2], c2w[4 * r + 3]};
     }
     return pt_reset_accumulation(ctx);
+}
+
+int pt_set_random_seed(pt_ctx *ctx, uint32_t frame) {
+    if (!ctx) return PT_ERR_INVALID_ARG;
+    ctx->random_seed = frame;
+    return PT_OK;
 }
 
 int pt_reset_accumulation(pt_ctx *ctx) {

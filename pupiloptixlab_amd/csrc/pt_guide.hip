@@ -45,9 +45,13 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__global__ void k_guide_begin(GuideCtl *ctl, uint32_t first, uint32_t batch, uint32_t capacity, float fp_explicit) {
+// all_slots: every slot of [0, capacity) holds a sample or a zero-weight
+// placeholder (interleaved sample regions), else the last collection's
+// samples are packed at [0, n_samples).
+__global__ void k_guide_begin(GuideCtl *ctl, uint32_t first, uint32_t batch, uint32_t capacity, float fp_explicit,
+                              uint32_t all_slots) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const uint32_t n = min(ctl->n_samples, capacity);
+    const uint32_t n = all_slots ? capacity : min(ctl->n_samples, capacity);
     const uint32_t valid = n > first ? min(batch, n - first) : 0u;
     ctl->batch_valid = valid;
     ctl->active = valid > 0u ? 1u : 0u;
@@ -605,8 +609,11 @@ __global__ __launch_bounds__(256) void k_guide_collect_count(const float4 *out_l
 // rest of the capacity is filled with zero-contribution samples; each class
 // is a uniform subsample, and a kept sample's weight (aux.w) is 1 / its keep
 // probability, so weighted batch means estimate means over all records.
+// Kept sample k goes to slot k * stride + offset (stride 1: packed; else one
+// of `stride` interleaved regions), k < capacity.
 __global__ __launch_bounds__(256) void k_guide_collect(const float4 *out_l, uint32_t total, GuideRecords r,
-                                                       float4 *samples, uint32_t capacity, GuideCtl *ctl) {
+                                                       float4 *samples, uint32_t capacity, GuideCtl *ctl,
+                                                       uint32_t stride, uint32_t offset) {
     __shared__ uint32_t wcount[4], wbase[4];
     const uint32_t n = r.capacity;
     const uint32_t nnz = ctl->n_nonzero, nz = ctl->n_zero;
@@ -653,9 +660,10 @@ __global__ __launch_bounds__(256) void k_guide_collect(const float4 *out_l, uint
         fs_acc += kept ? fpv * s2.w : 0.f;
         fc_acc += (kept && fpv > 0.f) ? s2.w : 0.f;
         if (ok && k < capacity) {
-            samples[3 * (size_t)k] = s0;
-            samples[3 * (size_t)k + 1] = s1;
-            samples[3 * (size_t)k + 2] = s2;
+            const size_t slot = (size_t)k * stride + offset;
+            samples[3 * slot] = s0;
+            samples[3 * slot + 1] = s1;
+            samples[3 * slot + 2] = s2;
         }
     }
 #pragma unroll
@@ -885,9 +893,10 @@ bool GuideState::ensure_records(uint32_t n, std::string &err) {
 }
 
 bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool apply, uint32_t grid_lo,
-                             uint32_t grid_hi, std::string &err) {
+                             uint32_t grid_hi, std::string &err, uint32_t window) {
     const GuideView G = view();
-    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, first, batch, sample_fill, fp_norm);
+    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, first, batch, window ? window : sample_fill,
+                       fp_norm, window ? 1u : 0u);
     TrainBuffers B{xt, dyt, table_g};
     const uint32_t waves = batch / 16;
     hipLaunchKernelGGL(k_guide_train, dim3((waves + 3) / 4), dim3(kTrainBlock), 0, stream, G,
@@ -907,14 +916,33 @@ bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool
     return true;
 }
 
-bool GuideState::collect(const float4 *out_l, uint32_t total_paths, const GuideRecords &r, std::string &err) {
+bool GuideState::collect(const float4 *out_l, uint32_t total_paths, const GuideRecords &r, std::string &err,
+                         uint32_t region, uint32_t regions, uint32_t region_cap) {
+    const uint32_t cap = regions > 1 ? region_cap : sample_fill;
+    if (regions > 1 && ((uint64_t)region_cap * regions > sample_cap || region >= regions)) {
+        err = "guide: sample regions exceed the sample buffer";
+        return false;
+    }
     hipLaunchKernelGGL(k_guide_collect_begin, dim3(1), dim3(64), 0, stream, ctl);
     const uint32_t grid = std::max(1u, std::min<uint32_t>((r.capacity + 255) / 256, 4096u));
     hipLaunchKernelGGL(k_guide_collect_count, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, ctl);
-    hipLaunchKernelGGL(k_guide_collect, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, samples, sample_fill,
-                       ctl);
-    hipLaunchKernelGGL(k_guide_collect_end, dim3(1), dim3(64), 0, stream, ctl, sample_fill);
+    hipLaunchKernelGGL(k_guide_collect, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, samples, cap, ctl,
+                       regions > 1 ? regions : 1u, regions > 1 ? region : 0u);
+    hipLaunchKernelGGL(k_guide_collect_end, dim3(1), dim3(64), 0, stream, ctl, cap);
     GUIDE_HIP(hipGetLastError());
+    return true;
+}
+
+bool GuideState::train_window(uint32_t steps, float fp_norm, uint32_t grid_lo, uint32_t grid_hi, uint32_t first,
+                              uint32_t window, std::string &err) {
+    const uint32_t B = cfg.batch_size;
+    if (!ensure_batch(B, err)) return false;
+    grid_hi = std::min(grid_hi, grids);
+    if (grid_lo >= grid_hi || window < B || window % B != 0 || window > sample_cap) return true;
+    for (uint32_t s = 0; s < steps; ++s) {
+        const uint32_t f = (uint32_t)(((uint64_t)first + (uint64_t)s * B) % window);
+        if (!launch_step(f, B, fp_norm, true, grid_lo, grid_hi, err, window)) return false;
+    }
     return true;
 }
 
@@ -943,7 +971,7 @@ bool GuideState::gradients(uint32_t n, float fp_norm, float *table_grad, float *
                            std::string &err) {
     if (!ensure_batch(n, err)) return false;
     const GuideView G = view();
-    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, 0u, n, sample_fill, fp_norm);
+    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, 0u, n, sample_fill, fp_norm, 0u);
     TrainBuffers B{xt, dyt, table_g};
     hipLaunchKernelGGL(k_guide_train, dim3((n / 16 + 3) / 4), dim3(kTrainBlock), 0, stream, G, samples, n, ctl, B,
                        LossParams{cfg.loss_flags, cfg.l2_weight});

@@ -92,6 +92,7 @@ def test_gpu_denoise_rendered_image(tracer):
     scene = pt.HostScene.generate(pt.SCENE_CORNELL, width=256, height=256)
     tracer.set_scene(scene)
     tracer.build_bvh()
+    tracer.set_random_seed(1 << 24)  # reference samples independent of frames 0, 1, ...
     tracer.render(spp=1024, mis_mode=pt.MIS_CONSISTENT, aov=False)
     ref = tracer.read(pt.BUF_ACCUM)[..., :3].astype(np.float64)
     tracer.reset()

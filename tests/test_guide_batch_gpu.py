@@ -230,6 +230,7 @@ def test_batch_guided_training_gains_and_stays_unbiased(tracer):
     for scene, _ in views:
         tracer.set_scene(scene)
         tracer.build_bvh()
+        tracer.set_random_seed(1 << 24)  # reference samples independent of frames 0, 1, ...
         tracer.render(spp=2048, mis_mode=pt.MIS_CONSISTENT, fp64=True, aov=False)
         refs.append(tracer.read(pt.BUF_ACCUM64) / 2048)
     batch_with_views(tracer, views)

@@ -232,6 +232,7 @@ def test_guided_cornell_matches_plain(tracer):
     scene = pt.HostScene.generate(pt.SCENE_CORNELL, width=128, height=128)
     tracer.set_scene(scene)
     tracer.build_bvh()
+    tracer.set_random_seed(1 << 24)  # reference samples independent of frames 0, 1, ...
     tracer.render(spp=1024, mis_mode=pt.MIS_CONSISTENT)
     ref = tracer.read(pt.BUF_ACCUM)[..., :3].astype(np.float64)
     tracer.reset()
@@ -271,6 +272,7 @@ def test_guided_door_room_gains_over_plain(tracer):
     scene = pt.HostScene.generate(pt.SCENE_DOOR_ROOM, width=480, height=270)
     tracer.set_scene(scene)
     tracer.build_bvh()
+    tracer.set_random_seed(1 << 24)  # reference samples independent of frames 0, 1, ...
     tracer.render(spp=4096, mis_mode=pt.MIS_CONSISTENT, fp64=True, aov=False)
     ref = tracer.read(pt.BUF_ACCUM64) / 4096
     tracer.reset()
