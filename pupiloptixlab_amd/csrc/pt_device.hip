@@ -419,12 +419,22 @@ __device__ __forceinline__ void unpack_half2(uint32_t v, float &a, float &b) {
 // per slot) straight from the MFMA result registers: lane (g, i) of a tile
 // holds outputs 16 rt + 4 g .. +3 of query i.  Waves stride over the queue;
 // a wave with no hit skips the network.  In a batched render the path id's
-// view bits (above `path_bits`) select the query's hash grid.
+// view bits (above `path_bits`) select the query's hash grid.  Only hits on
+// materials the guided step samples are queried (misses and delta / smooth
+// plastic surfaces gather no table entries; their slots are left unwritten).
 constexpr int kInferBlock = 512;
+
+// Scene records k_guide_infer needs to tell a guidable hit.
+struct InferScene {
+    const uint32_t *tri;  // 4 per triangle: i0, i1, i2, shape
+    const ShapeRec *shapes;
+    const SphereRec *spheres;
+    const MaterialRec *materials;
+};
 
 __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQueue q, const uint32_t *count_ptr,
                                                              uint2 *gout, unsigned long long *queries,
-                                                             uint32_t path_bits) {
+                                                             uint32_t path_bits, InferScene sc) {
     __shared__ half8 wl[4 * 4 * 2 * 64];
     const uint32_t waves = kInferBlock / 64;
     if (blockIdx.x * waves * 64u >= *count_ptr) return;  // no slot for this block: skip the weight load
@@ -443,8 +453,14 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQu
         uint32_t grid = 0u;
         if (valid) {
             const float4 r0 = q.ray0[i], r1 = q.ray1[i], h = q.hit[i];
-            valid = f2u(h.y) != kMissPrim;
-            if (valid) p = f3of(r0) + f3of(r1) * h.x;
+            const uint32_t prim = f2u(h.y);
+            valid = prim != kMissPrim;
+            if (valid) {
+                const uint32_t shape =
+                    (prim & kPrimSphere) ? sc.spheres[prim & kPrimIndexMask].shape : sc.tri[4 * (size_t)prim + 3];
+                valid = guidable_material(sc.materials[sc.shapes[shape].material].type);
+                p = f3of(r0) + f3of(r1) * h.x;
+            }
             grid = min(f2u(r0.w) >> path_bits, G.grids - 1u);
         }
         const unsigned long long vmask = __ballot(valid);
@@ -459,10 +475,13 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQu
             asm volatile("" ::: "memory");
             const int qi = qt * 16 + il;
             const f3 uq = mk3(__shfl(u.x, qi), __shfl(u.y, qi), __shfl(u.z, qi));
-            const half8 in0 = guide_encode(G, uq, g, (uint32_t)__shfl((int)grid, qi));
+            const uint32_t gq = (uint32_t)__shfl((int)grid, qi);
+            const bool vq = __shfl(vbit, qi) != 0;
+            half8 in0 = {};
+            if (vq) in0 = guide_encode(G, uq, g, gq);  // no table gathers for an unused query
             floatx4 out[4];
             guide_mlp_tile_shared(wl, lane, in0, out);
-            if (__shfl(vbit, qi)) {
+            if (vq) {
                 uint2 *dst = gout + (size_t)(base + qi) * 12u + (uint32_t)g;
 #pragma unroll
                 for (int rt = 0; rt < 3; ++rt)
@@ -2010,7 +2029,8 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
                 EventTimer t(ctx, timing);
                 hipLaunchKernelGGL(k_guide_infer, dim3(ctx->num_cu * ctx->infer_blocks_per_cu), dim3(kInferBlock), 0,
                                    ctx->stream, bg.view, ctx->q[cur], ext + bb,
-                                   reinterpret_cast<uint2 *>(ctx->d_gout), ctx->d_stat + 4, path_bits);
+                                   reinterpret_cast<uint2 *>(ctx->d_gout), ctx->d_stat + 4, path_bits,
+                                   InferScene{sv.tri, sv.shapes, sv.spheres, sv.materials});
                 t.stop(kKindGuideInfer);
             }
             EventTimer t(ctx, timing);
@@ -2897,7 +2917,7 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
                 EventTimer t(ctx, timing);
                 hipLaunchKernelGGL(k_guide_infer, dim3(ctx->num_cu * ctx->infer_blocks_per_cu), dim3(kInferBlock), 0,
                                    ctx->stream, gview, ctx->q[cur], ext + b, reinterpret_cast<uint2 *>(ctx->d_gout),
-                                   ctx->d_stat + 4, 31u);
+                                   ctx->d_stat + 4, 31u, InferScene{sv.tri, sv.shapes, sv.spheres, sv.materials});
                 t.stop(kKindGuideInfer);
             }
             {
