@@ -1968,19 +1968,19 @@ uint32_t guide_sample_count(uint64_t steps, uint32_t batch) {
 
 // Guided batched render: network outputs per queue slot and the record sink
 // (epoch and phase of each launch are read from its LaunchDesc).
-constexpr uint32_t kMixRegions = 8;  // training launches whose samples one optimizer step mixes
+constexpr uint32_t kMixLaunches = 8;  // training launches whose samples one optimizer step mixes
 
 struct BatchGuide {
     bool infer = false, train = false;
     GuideView view{};
     GuideShadeArgs ga{};
     uint32_t gdepth = 0;  // vertices below this depth may sample the guide
-    // Training samples of the last `regions` training launches sit in
-    // interleaved regions of `region_cap` slots, so every optimizer step
-    // mixes several launches (views) instead of fitting the shared MLP to one
-    // view after another.
-    uint32_t regions = 1, region_cap = 0, next_region = 0, cursor = 0;
-    uint32_t region_lo[kMixRegions] = {0}, region_hi[kMixRegions] = {0};  // views of each region
+    // With several views, training samples go to a ring that holds about the
+    // last kMixLaunches training launches, and every optimizer step draws its
+    // batch from the whole ring, so each step mixes several views instead of
+    // fitting the shared MLP to one view after another.
+    uint32_t ring_n = 0;  // slots (a multiple of the batch size); 0 = one launch at a time
+    uint32_t step = 0;    // optimizer steps taken on the ring
 };
 
 // The kernels of one launch (the sequence a hipGraph captures).
@@ -2074,7 +2074,7 @@ uint32_t batch_train_capacity(const BatchState &b, const BatchShape &sh) {
 // queue slot, training records for the training launch width, samples for
 // the optimizer steps of the fullest training launch.
 int batch_guide_setup(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, uint32_t max_depth,
-                      BatchGuide &bg) {
+                      const std::vector<LaunchDesc> &plan, BatchGuide &bg) {
     if (!bg.infer && !bg.train) return PT_OK;
     GuideState &gs = ctx->guide;
     if (!gs.enabled || !gs.for_batch || gs.grids != ctx->batch.views.size()) {
@@ -2109,12 +2109,25 @@ int batch_guide_setup(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &s
         uint32_t max_frames = 1;
         for (const auto &v : ctx->batch.views)
             max_frames = std::max(max_frames, std::min(kMaxLaunchEntries, train_cap / (v.width * v.height)));
-        bg.region_cap = guide_sample_count((uint64_t)gc.steps_per_frame * max_frames, gc.batch_size);
-        bg.regions = std::min<uint32_t>(kMixRegions, (uint32_t)ctx->batch.views.size());
-        if ((uint64_t)bg.region_cap * bg.regions > (1ull << 28)) bg.regions = 1;
         if (!gs.ensure_records((uint32_t)records, ctx->error) ||
-            !gs.ensure_samples(bg.region_cap * bg.regions, ctx->error))
+            !gs.ensure_samples(guide_sample_count((uint64_t)gc.steps_per_frame * max_frames, gc.batch_size),
+                               ctx->error))
             return PT_ERR_HIP;
+        // Ring of about kMixLaunches average training launches (their sample
+        // budgets), a multiple of the batch size.
+        uint64_t frames = 0, launches = 0;
+        for (const LaunchDesc &l : plan)
+            if (l.train_frames) {
+                frames += l.train_frames;
+                ++launches;
+            }
+        if (ctx->batch.views.size() > 1 && launches > 1) {
+            const uint64_t per = std::max<uint64_t>(1, (frames * gc.steps_per_frame + launches - 1) / launches);
+            const uint64_t batches = std::min<uint64_t>(per * std::min<uint64_t>(kMixLaunches, launches),
+                                                        (1ull << 28) / gc.batch_size);
+            bg.ring_n = (uint32_t)(batches * gc.batch_size);
+            if (!gs.ensure_samples(bg.ring_n, ctx->error)) return PT_ERR_HIP;
+        }
         bg.ga.rs.rec = reinterpret_cast<f4 *>(gs.rec);
         bg.ga.rs.capacity = gs.rec_cap;
         bg.ga.rs.stride = gc.train_stride;
@@ -2164,7 +2177,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     spar.mis_mode = p.mis_mode;
     spar.aov = 0;
     const bool fp64 = (p.render_flags & PT_RENDER_FP64_ACCUM) != 0;
-    if ((rc = batch_guide_setup(ctx, p, sh, max_depth, bg)) != PT_OK) return rc;
+    if ((rc = batch_guide_setup(ctx, p, sh, max_depth, plan, bg)) != PT_OK) return rc;
     // Per-kernel event timing (direct launches only; around the training
     // steps in both modes).
     const bool timing = (p.render_flags & PT_RENDER_TIMING) != 0;
@@ -2213,22 +2226,11 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
             const GuideRecords r{gs.rec, gs.rec_cap, gs.cfg.train_stride, hl.phase, max_depth, hl.epoch};
             const uint32_t steps = gs.cfg.steps_per_frame * hl.train_frames;
             bool ok;
-            if (bg.regions > 1) {
-                const uint32_t reg = bg.next_region++ % bg.regions;
-                bg.region_lo[reg] = v_lo;
-                bg.region_hi[reg] = v_hi;
-                uint32_t u_lo = v_lo, u_hi = v_hi;
-                for (uint32_t k = 0; k < bg.regions; ++k)
-                    if (bg.region_hi[k] > bg.region_lo[k]) {
-                        u_lo = std::min(u_lo, bg.region_lo[k]);
-                        u_hi = std::max(u_hi, bg.region_hi[k]);
-                    }
-                const uint32_t window = bg.region_cap * bg.regions, B = gs.cfg.batch_size;
-                const uint32_t n = std::min(steps, bg.region_cap / B);  // about one pass over a launch's samples
-                ok = gs.collect(ctx->d_out_l, hl.total_paths, r, ctx->error, reg, bg.regions,
-                                std::min(bg.region_cap, guide_sample_count(steps, B))) &&
-                     gs.train_window(n, 0.f, u_lo, u_hi, bg.cursor, window, ctx->error);
-                bg.cursor = (uint32_t)(((uint64_t)bg.cursor + (uint64_t)n * B) % window);
+            if (bg.ring_n) {
+                ok = gs.collect(ctx->d_out_l, hl.total_paths, r, ctx->error, bg.ring_n,
+                                guide_sample_count(steps, gs.cfg.batch_size)) &&
+                     gs.train_ring(steps, 0.f, bg.step, bg.ring_n, ctx->error);
+                bg.step += steps;
             } else {
                 ok = gs.ensure_samples(guide_sample_count(steps, gs.cfg.batch_size), ctx->error) &&
                      gs.collect(ctx->d_out_l, hl.total_paths, r, ctx->error) &&

@@ -102,8 +102,12 @@ __device__ __forceinline__ float xor_max4(float v) {
 //
 // samples: 3 float4 per sample (position, direction, {target, F/p, BSDF pdf,
 // weight}); weight 0 marks an unused slot.
+// Batch element q reads slot first + q, or with a ring of ring_n = m B
+// slots slot (first + q m) mod ring_n: every m-th slot, so one batch spans
+// the whole ring (the samples of several launches).
 __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideView G, const float4 *samples, uint32_t batch,
-                                                             GuideCtl *ctl, TrainBuffers B, LossParams lp) {
+                                                             GuideCtl *ctl, TrainBuffers B, LossParams lp,
+                                                             uint32_t first, uint32_t ring_n, uint32_t ring_m) {
     __shared__ float stage[kTrainBlock / 64][16 * kGuideUsed];
     if (!ctl->active) return;
     const uint32_t nvalid = ctl->batch_valid;
@@ -118,9 +122,10 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideView G, const 
 
     const uint32_t me = base + i;
     const bool in_range = me < nvalid;
-    const float4 p4 = samples[3 * (size_t)me];
-    const float4 d4 = samples[3 * (size_t)me + 1];
-    const float4 aux = samples[3 * (size_t)me + 2];
+    const size_t slot = ring_n ? ((size_t)first + (size_t)me * ring_m) % ring_n : (size_t)first + me;
+    const float4 p4 = samples[3 * slot];
+    const float4 d4 = samples[3 * slot + 1];
+    const float4 aux = samples[3 * slot + 2];
     // With several hash grids, position.w names the sample's grid; a sample
     // naming no grid is unused.
     uint32_t grid = 0u;
@@ -543,8 +548,12 @@ __global__ void k_guide_collect_begin(GuideCtl *ctl) {
     }
 }
 
-__global__ void k_guide_collect_end(GuideCtl *ctl, uint32_t capacity) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) ctl->total_samples += min(ctl->n_samples, capacity);
+__global__ void k_guide_collect_end(GuideCtl *ctl, uint32_t capacity, uint32_t ring_n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const uint32_t kept = min(ctl->n_samples, capacity);
+        ctl->total_samples += kept;
+        if (ring_n) ctl->ring_head = (uint32_t)(((uint64_t)ctl->ring_head + kept) % ring_n);
+    }
 }
 
 __device__ __forceinline__ uint32_t mix32(uint32_t x) {
@@ -609,11 +618,12 @@ __global__ __launch_bounds__(256) void k_guide_collect_count(const float4 *out_l
 // rest of the capacity is filled with zero-contribution samples; each class
 // is a uniform subsample, and a kept sample's weight (aux.w) is 1 / its keep
 // probability, so weighted batch means estimate means over all records.
-// Kept sample k goes to slot k * stride + offset (stride 1: packed; else one
-// of `stride` interleaved regions), k < capacity.
+// Kept sample k (k < capacity) goes to slot k, or with a ring of ring_n
+// slots to slot (ring_head + k) mod ring_n.
 __global__ __launch_bounds__(256) void k_guide_collect(const float4 *out_l, uint32_t total, GuideRecords r,
                                                        float4 *samples, uint32_t capacity, GuideCtl *ctl,
-                                                       uint32_t stride, uint32_t offset) {
+                                                       uint32_t ring_n) {
+    const uint32_t head = ring_n ? ctl->ring_head : 0u;
     __shared__ uint32_t wcount[4], wbase[4];
     const uint32_t n = r.capacity;
     const uint32_t nnz = ctl->n_nonzero, nz = ctl->n_zero;
@@ -660,7 +670,7 @@ __global__ __launch_bounds__(256) void k_guide_collect(const float4 *out_l, uint
         fs_acc += kept ? fpv * s2.w : 0.f;
         fc_acc += (kept && fpv > 0.f) ? s2.w : 0.f;
         if (ok && k < capacity) {
-            const size_t slot = (size_t)k * stride + offset;
+            const size_t slot = ring_n ? ((size_t)head + k) % ring_n : (size_t)k;
             samples[3 * slot] = s0;
             samples[3 * slot + 1] = s1;
             samples[3 * slot + 2] = s2;
@@ -893,14 +903,14 @@ bool GuideState::ensure_records(uint32_t n, std::string &err) {
 }
 
 bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool apply, uint32_t grid_lo,
-                             uint32_t grid_hi, std::string &err, uint32_t window) {
+                             uint32_t grid_hi, std::string &err, uint32_t ring_n) {
     const GuideView G = view();
-    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, first, batch, window ? window : sample_fill,
-                       fp_norm, window ? 1u : 0u);
+    hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, first, batch, ring_n ? ring_n : sample_fill,
+                       fp_norm, ring_n ? 1u : 0u);
     TrainBuffers B{xt, dyt, table_g};
     const uint32_t waves = batch / 16;
-    hipLaunchKernelGGL(k_guide_train, dim3((waves + 3) / 4), dim3(kTrainBlock), 0, stream, G,
-                       samples + 3 * (size_t)first, batch, ctl, B, LossParams{cfg.loss_flags, cfg.l2_weight});
+    hipLaunchKernelGGL(k_guide_train, dim3((waves + 3) / 4), dim3(kTrainBlock), 0, stream, G, samples, batch, ctl, B,
+                       LossParams{cfg.loss_flags, cfg.l2_weight}, first, ring_n, ring_n ? ring_n / batch : 0u);
     hipLaunchKernelGGL(k_guide_dw, dim3((batch + kGuideDwSlice - 1) / kGuideDwSlice, 4), dim3(64), 0, stream, xt,
                        dyt, batch, ctl, w_grad);
     const AdamParams at{cfg.learning_rate, 0.9f, 0.99f, 1e-15f};
@@ -917,32 +927,30 @@ bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool
 }
 
 bool GuideState::collect(const float4 *out_l, uint32_t total_paths, const GuideRecords &r, std::string &err,
-                         uint32_t region, uint32_t regions, uint32_t region_cap) {
-    const uint32_t cap = regions > 1 ? region_cap : sample_fill;
-    if (regions > 1 && ((uint64_t)region_cap * regions > sample_cap || region >= regions)) {
-        err = "guide: sample regions exceed the sample buffer";
+                         uint32_t ring_n, uint32_t ring_cap) {
+    const uint32_t cap = ring_n ? std::min(ring_cap, ring_n) : sample_fill;
+    if (ring_n > sample_cap) {
+        err = "guide: sample ring exceeds the sample buffer";
         return false;
     }
     hipLaunchKernelGGL(k_guide_collect_begin, dim3(1), dim3(64), 0, stream, ctl);
     const uint32_t grid = std::max(1u, std::min<uint32_t>((r.capacity + 255) / 256, 4096u));
     hipLaunchKernelGGL(k_guide_collect_count, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, ctl);
     hipLaunchKernelGGL(k_guide_collect, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, samples, cap, ctl,
-                       regions > 1 ? regions : 1u, regions > 1 ? region : 0u);
-    hipLaunchKernelGGL(k_guide_collect_end, dim3(1), dim3(64), 0, stream, ctl, cap);
+                       ring_n);
+    hipLaunchKernelGGL(k_guide_collect_end, dim3(1), dim3(64), 0, stream, ctl, cap, ring_n);
     GUIDE_HIP(hipGetLastError());
     return true;
 }
 
-bool GuideState::train_window(uint32_t steps, float fp_norm, uint32_t grid_lo, uint32_t grid_hi, uint32_t first,
-                              uint32_t window, std::string &err) {
+bool GuideState::train_ring(uint32_t steps, float fp_norm, uint32_t first_step, uint32_t ring_n, std::string &err) {
     const uint32_t B = cfg.batch_size;
     if (!ensure_batch(B, err)) return false;
-    grid_hi = std::min(grid_hi, grids);
-    if (grid_lo >= grid_hi || window < B || window % B != 0 || window > sample_cap) return true;
-    for (uint32_t s = 0; s < steps; ++s) {
-        const uint32_t f = (uint32_t)(((uint64_t)first + (uint64_t)s * B) % window);
-        if (!launch_step(f, B, fp_norm, true, grid_lo, grid_hi, err, window)) return false;
-    }
+    if (ring_n < B || ring_n % B != 0 || ring_n > sample_cap) return true;
+    const uint32_t m = ring_n / B;
+    for (uint32_t s = 0; s < steps; ++s)
+        if (!launch_step((uint32_t)(((uint64_t)first_step + s) % m), B, fp_norm, true, 0u, grids, err, ring_n))
+            return false;
     return true;
 }
 
@@ -974,7 +982,7 @@ bool GuideState::gradients(uint32_t n, float fp_norm, float *table_grad, float *
     hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, 0u, n, sample_fill, fp_norm, 0u);
     TrainBuffers B{xt, dyt, table_g};
     hipLaunchKernelGGL(k_guide_train, dim3((n / 16 + 3) / 4), dim3(kTrainBlock), 0, stream, G, samples, n, ctl, B,
-                       LossParams{cfg.loss_flags, cfg.l2_weight});
+                       LossParams{cfg.loss_flags, cfg.l2_weight}, 0u, 0u, 0u);
     hipLaunchKernelGGL(k_guide_dw, dim3((n + kGuideDwSlice - 1) / kGuideDwSlice, 4), dim3(64), 0, stream, xt, dyt,
                        n, ctl, w_grad);
     GUIDE_HIP(hipGetLastError());

@@ -42,6 +42,7 @@ struct GuideCtl {
     float fp_sum;          // weighted sum of F/p over collected samples
     float fp_cnt;          // weighted count of collected samples with F/p > 0
     float fp_norm;         // selection-loss normalization of the current step
+    uint32_t ring_head;    // next slot of the sample ring (collections into a ring)
 };
 
 // Training records written by the guided shading kernel, one per guided
@@ -109,20 +110,19 @@ public:
     bool ensure_samples(uint32_t n, std::string &err);
     bool ensure_records(uint32_t n, std::string &err);
     // Sample collection from the finished paths of one launch: packed at the
-    // start of the buffer, or (regions > 1) at most region_cap samples into
-    // slots k * regions + region, so that any B consecutive slots mix the
-    // last `regions` collections.
+    // start of the buffer, or (ring_n > 0) at most ring_cap samples appended
+    // to a ring of ring_n slots, which then holds the last collections.
     bool collect(const float4 *out_l, uint32_t total_paths, const GuideRecords &r, std::string &err,
-                 uint32_t region = 0, uint32_t regions = 1, uint32_t region_cap = 0);
+                 uint32_t ring_n = 0, uint32_t ring_cap = 0);
     // `steps` optimizer steps over the collected samples, batch_size each;
     // steps without samples are skipped on the device.
     bool train(uint32_t steps, float fp_norm, std::string &err);
     // ... when the samples only name grids [grid_lo, grid_hi): Adam visits those.
     bool train(uint32_t steps, float fp_norm, uint32_t grid_lo, uint32_t grid_hi, std::string &err);
-    // Steps over slots [first, first + steps B) of the interleaved regions
-    // (modulo `window` slots, a multiple of the batch size).
-    bool train_window(uint32_t steps, float fp_norm, uint32_t grid_lo, uint32_t grid_hi, uint32_t first,
-                      uint32_t window, std::string &err);
+    // Steps over the sample ring (ring_n = m B slots): step k takes every m-th
+    // slot from offset (first_step + k) mod m, so each batch spans the whole
+    // ring; Adam visits every grid.
+    bool train_ring(uint32_t steps, float fp_norm, uint32_t first_step, uint32_t ring_n, std::string &err);
     // Host-sample entry points (tests / external trainers).
     bool upload_samples(const float *host, uint32_t n, std::string &err);
     bool gradients(uint32_t n, float fp_norm, float *table_grad, float *weight_grad, float *loss, std::string &err);
@@ -136,7 +136,7 @@ public:
 private:
     bool ensure_batch(uint32_t batch, std::string &err);
     bool launch_step(uint32_t first, uint32_t batch, float fp_norm, bool apply, uint32_t grid_lo, uint32_t grid_hi,
-                     std::string &err, uint32_t window = 0);
+                     std::string &err, uint32_t ring_n = 0);
     bool pack(std::string &err);
 };
 
