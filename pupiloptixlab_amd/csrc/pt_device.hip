@@ -304,7 +304,8 @@ struct LaunchDesc {
 
 // With `views` set (batched render) the path id carries the view above bit
 // `path_bits`; the view selects its emitters, environment and depth.
-__global__ __launch_bounds__(256) void k_shade(SceneView scene, ShadeParams prm, PathQueue in,
+template <bool kBatch>
+__global__ __launch_bounds__(256, kBatch ? 1 : 4) void k_shade(SceneView scene, ShadeParams prm, PathQueue in,
                                                const uint32_t *count_in, PathQueue out, uint32_t *count_out,
                                                ShadowQueue sq, uint32_t *count_shadow, float4 *out_l,
                                                AovBuffers aov, const BatchViewDev *views, uint32_t path_bits) {
@@ -328,7 +329,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneView scene, ShadeParams prm,
             ps.rng.seed = f2u(lr.z);
             ps.flags = f2u(lr.w);
             prim = f2u(h.y);
-            if (views) {
+            if constexpr (kBatch) {
                 const uint32_t view = packed >> path_bits;
                 path = packed & ((1u << path_bits) - 1u);
                 vbits = view << path_bits;
@@ -2041,7 +2042,7 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
             t.stop(kKindShade);
         } else {
             EventTimer t(ctx, timing);
-            hipLaunchKernelGGL(k_shade, dim3(sg), dim3(sh.shade_block), 0, ctx->stream, sv, sp, ctx->q[cur],
+            hipLaunchKernelGGL(k_shade<true>, dim3(sg), dim3(sh.shade_block), 0, ctx->stream, sv, sp, ctx->q[cur],
                                ext + bb, ctx->q[nxt], ext + bb + 1, ctx->sq, shd + bb, ctx->d_out_l, aov,
                                (const BatchViewDev *)b.d_views, path_bits);
             t.stop(kKindShade);
@@ -2929,7 +2930,7 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
                                        ext + b, ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov, ga,
                                        (const BatchViewDev *)nullptr, 31u, (const LaunchDesc *)nullptr);
                 else
-                    hipLaunchKernelGGL(k_shade, dim3(sg), dim3(256), 0, ctx->stream, sv, sp, ctx->q[cur], ext + b,
+                    hipLaunchKernelGGL(k_shade<false>, dim3(sg), dim3(256), 0, ctx->stream, sv, sp, ctx->q[cur], ext + b,
                                        ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov,
                                        (const BatchViewDev *)nullptr, 31u);
                 t.stop(kKindShade);

@@ -540,6 +540,7 @@ __global__ __launch_bounds__(256) void k_guide_query(GuideView G, const float *p
 
 __global__ void k_guide_collect_begin(GuideCtl *ctl) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
+        ctl->ring_start = ctl->ring_head;
         ctl->n_samples = 0u;
         ctl->n_nonzero = 0u;
         ctl->n_zero = 0u;
@@ -553,6 +554,21 @@ __global__ void k_guide_collect_end(GuideCtl *ctl, uint32_t capacity, uint32_t r
         const uint32_t kept = min(ctl->n_samples, capacity);
         ctl->total_samples += kept;
         if (ring_n) ctl->ring_head = (uint32_t)(((uint64_t)ctl->ring_head + kept) % ring_n);
+    }
+}
+
+// Ring collections: F/p of the samples just added is divided by their own
+// collection's mean (the selection-loss normalization), so the samples of
+// different launches -- views with very different brightness -- can share
+// one optimizer step with the normalization 1.
+__global__ void k_guide_ring_normalize(float4 *samples, const GuideCtl *ctl, uint32_t capacity, uint32_t ring_n) {
+    const uint32_t kept = min(ctl->n_samples, capacity);
+    const float norm = (ctl->fp_cnt > 0.f && ctl->fp_sum > 0.f) ? ctl->fp_sum / ctl->fp_cnt : 1.f;
+    const float inv = 1.f / norm;
+    const uint32_t start = ctl->ring_start;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < kept; k += gridDim.x * blockDim.x) {
+        const size_t slot = ((size_t)start + k) % ring_n;
+        samples[3 * slot + 2].y *= inv;
     }
 }
 
@@ -938,6 +954,9 @@ bool GuideState::collect(const float4 *out_l, uint32_t total_paths, const GuideR
     hipLaunchKernelGGL(k_guide_collect_count, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, ctl);
     hipLaunchKernelGGL(k_guide_collect, dim3(grid), dim3(256), 0, stream, out_l, total_paths, r, samples, cap, ctl,
                        ring_n);
+    if (ring_n)
+        hipLaunchKernelGGL(k_guide_ring_normalize, dim3(std::max(1u, std::min((cap + 255u) / 256u, 2048u))),
+                           dim3(256), 0, stream, samples, ctl, cap, ring_n);
     hipLaunchKernelGGL(k_guide_collect_end, dim3(1), dim3(64), 0, stream, ctl, cap, ring_n);
     GUIDE_HIP(hipGetLastError());
     return true;
@@ -948,8 +967,10 @@ bool GuideState::train_ring(uint32_t steps, float fp_norm, uint32_t first_step, 
     if (!ensure_batch(B, err)) return false;
     if (ring_n < B || ring_n % B != 0 || ring_n > sample_cap) return true;
     const uint32_t m = ring_n / B;
+    // Ring samples carry F/p already divided by their collection's mean.
+    (void)fp_norm;
     for (uint32_t s = 0; s < steps; ++s)
-        if (!launch_step((uint32_t)(((uint64_t)first_step + s) % m), B, fp_norm, true, 0u, grids, err, ring_n))
+        if (!launch_step((uint32_t)(((uint64_t)first_step + s) % m), B, 1.f, true, 0u, grids, err, ring_n))
             return false;
     return true;
 }

@@ -43,6 +43,7 @@ struct GuideCtl {
     float fp_cnt;          // weighted count of collected samples with F/p > 0
     float fp_norm;         // selection-loss normalization of the current step
     uint32_t ring_head;    // next slot of the sample ring (collections into a ring)
+    uint32_t ring_start;   // first slot of the current ring collection
 };
 
 // Training records written by the guided shading kernel, one per guided
@@ -121,7 +122,8 @@ public:
     bool train(uint32_t steps, float fp_norm, uint32_t grid_lo, uint32_t grid_hi, std::string &err);
     // Steps over the sample ring (ring_n = m B slots): step k takes every m-th
     // slot from offset (first_step + k) mod m, so each batch spans the whole
-    // ring; Adam visits every grid.
+    // ring; Adam visits every grid.  Ring samples are normalized per
+    // collection, so the selection-loss normalization is 1 (fp_norm unused).
     bool train_ring(uint32_t steps, float fp_norm, uint32_t first_step, uint32_t ring_n, std::string &err);
     // Host-sample entry points (tests / external trainers).
     bool upload_samples(const float *host, uint32_t n, std::string &err);
