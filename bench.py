@@ -11,7 +11,10 @@ Prints one JSON line (rank 0).  `value` is Msamples/s with the scene and BVH
 resident on the GPU before the timed region starts.  `relmse_guided_vs_plain`
 is the second half of the metric: relMSE of the plain tracer divided by that
 of the guided tracer at equal wall-clock on the door-room scene (1080p, depth
-16, against a 4096-spp FP64 plain reference; see guided_experiment).  The roofline block uses
+16, against a 4096-spp FP64 plain reference; see guided_experiment).  `batched` is the
+multi-view batch (64 Cornell variants), `guided_batched` guided plus batched (door-room and
+terrain variants with per-view hash grids and a shared MLP; `--guided-batch-full` runs the
+16 + 16 views at 1080p).  The roofline block uses
 the kernel with the largest share of GPU time; its algorithmic bytes come from
 a counting pass over the same frames (identical rays: the tracer is
 deterministic) and its durations from HIP events recorded on the pass stream
@@ -370,7 +373,7 @@ def summarize_pmc(fetch_dir, write_dir, kernel_prefix, out_path):
 def run_guided_batch(pt, full):
     if full:
         return batched_guided_experiment(pt, 16, 16, 1920, 1080, 1024, 256, 64, [512, 1024, 1448], 0)
-    return batched_guided_experiment(pt, 4, 4, 480, 270, 4096, 1024, 64, [256, 512], 0)
+    return batched_guided_experiment(pt, 4, 4, 960, 540, 2048, 512, 64, [256, 512], 0)
 
 
 def main():
