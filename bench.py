@@ -12,9 +12,9 @@ resident on the GPU before the timed region starts.  `relmse_guided_vs_plain`
 is the second half of the metric: relMSE of the plain tracer divided by that
 of the guided tracer at equal wall-clock on the door-room scene (1080p, depth
 16, against a 4096-spp FP64 plain reference; see guided_experiment).  `batched` is the
-multi-view batch (64 Cornell variants), `guided_batched` guided plus batched (door-room and
-terrain variants with per-view hash grids and a shared MLP; `--guided-batch-full` runs the
-16 + 16 views at 1080p).  The roofline block uses
+multi-view batch (64 Cornell variants), `guided_batched` guided plus batched (2 door-room + 2
+terrain variants at 1080p with per-view hash grids and a shared MLP; `--guided-batch-full` runs
+the 16 + 16 views).  The roofline block uses
 the kernel with the largest share of GPU time; its algorithmic bytes come from
 a counting pass over the same frames (identical rays: the tracer is
 deterministic) and its durations from HIP events recorded on the pass stream
@@ -308,6 +308,9 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
             kinds[kind] = {"relmse_plain_equal_time": round(ep, 5), "relmse_guided": round(eg, 5),
                            "ratio": round(ep / eg, 3)}
     ep, eg = mean(e_plain), mean(e_guided)
+    # Per-view equal-time ratios; their geometric mean weighs every view alike
+    # (the ratio of mean relMSEs is dominated by the noisiest, lowest-spp views).
+    geo = float(np.exp(np.mean([np.log(a / b) for a, b in zip(e_plain, e_guided)])))
     out = {
         "workload": (f"{n_door} door-room variants + {n_terrain} terrain variants (grids {list(terrain_grids)}), "
                      f"{width}x{height}, spp {spps}, depth 16, one wavefront loop; guide: one hash grid per view, "
@@ -325,6 +328,7 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
         "relmse_plain_equal_time": round(ep, 5),
         "relmse_guided": round(eg, 5),
         "ratio": round(ep / eg, 3),
+        "ratio_geomean_per_view": round(geo, 3),
         "by_scene": kinds,
         "guide_steps": info["steps"],
         "guide_skipped_steps": info["skipped"],
@@ -373,7 +377,7 @@ def summarize_pmc(fetch_dir, write_dir, kernel_prefix, out_path):
 def run_guided_batch(pt, full):
     if full:
         return batched_guided_experiment(pt, 16, 16, 1920, 1080, 1024, 256, 64, [512, 1024, 1448], 0)
-    return batched_guided_experiment(pt, 4, 4, 960, 540, 2048, 512, 64, [256, 512], 0)
+    return batched_guided_experiment(pt, 2, 2, 1920, 1080, 1024, 256, 64, [512], 0)
 
 
 def main():
