@@ -32,9 +32,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # Reference renders draw frames from this index on, so they share no samples
-# with the renders they judge (frames 0, 1, ...).
+# with the renders they are compared with (frames 0, 1, ...).
 REF_SEED = 1 << 24
-# Algorithmic bytes per unit ("Roofline accounting").
+# Algorithmic bytes per unit: a ray record (origin, direction = 28 B) and a
+# hit record (16 B) per traced ray, an 80-B compressed node per node visit, a
+# 36-B triangle per triangle test, a 4-B result per shadow ray, and the ray
+# state a shading step reads and writes per path vertex.
 RAY_BYTES, HIT_BYTES, NODE_BYTES, TRI_BYTES = 28, 16, 80, 36
 SHADOW_RESULT_BYTES = 4
 SHADE_BYTES_PER_VERTEX = 280
@@ -413,8 +416,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(the multi-GPU split is described in DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (a multi-GPU run would split "
+              "the frames of every pixel across GPUs and sum the per-GPU means).", file=sys.stderr)
         sys.exit(2)
 
     import pupiloptixlab_amd as pt

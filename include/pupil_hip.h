@@ -294,7 +294,7 @@ int pt_trace_rays(pt_ctx *ctx, const float *rays, uint32_t count, int any_hit, v
  * DenoisePass (system/denoise_pass.cpp:31-93) runs optix::Denoiser
  * (optix/denoiser.h:9-38) on the noisy buffer with optional albedo / normal
  * guides (EMode UseAlbedo = 1, UseNormal = 2) and writes "final result".  The
- * OptiX model is closed; this build filters with an edge-avoiding a-trous
+ * OptiX model is closed; pt_denoise filters with an edge-avoiding a-trous
  * wavelet transform (Dammertz et al. 2010) with variance-guided luminance
  * stopping (Schied et al. 2017) on albedo-demodulated color. */
 enum pt_denoise_flags { PT_DENOISE_ALBEDO = 1, PT_DENOISE_NORMAL = 2 };
