@@ -246,17 +246,31 @@ PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float 
     // Root group: the root sits in slot 0 of a virtual parent.
     uint32_t g_base = root, g_bits = (1u << (24 + (0 ^ oct))) | 1u;
     uint32_t t_base = 0, t_bits = 0;
-    while (true) {
-        if (g_bits & 0xFF000000u) {
-            uint32_t bitpos = __builtin_ctz(g_bits >> 24);
-            g_bits &= ~(1u << (24 + bitpos));
-            uint32_t slot = bitpos ^ oct;
-            uint32_t imask = g_bits & 0xFFu;
-            uint32_t node_index = g_base + __builtin_popcount(imask & ((1u << slot) - 1u));
-            if (g_bits & 0xFF000000u) stack.push(g_base, g_bits);
-
-            WideNode n;
-            load_node(node_index, n);
+    // Next node to visit: the nearest remaining child of the current group,
+    // or of the group on top of the stack; the rest of the group is pushed.
+    // The choice never depends on hits found later, so the next node is
+    // picked -- and, for closest-hit rays, its load issued -- before the
+    // current node's triangles are tested, and the two memory latencies
+    // overlap.
+    auto next_node = [&](uint32_t &node_index) -> bool {
+        if ((g_bits & 0xFF000000u) == 0) {
+            if (stack.empty()) return false;
+            stack.pop(g_base, g_bits);
+        }
+        const uint32_t bitpos = __builtin_ctz(g_bits >> 24);
+        g_bits &= ~(1u << (24 + bitpos));
+        const uint32_t slot = bitpos ^ oct;
+        const uint32_t imask = g_bits & 0xFFu;
+        node_index = g_base + __builtin_popcount(imask & ((1u << slot) - 1u));
+        if (g_bits & 0xFF000000u) stack.push(g_base, g_bits);
+        return true;
+    };
+    uint32_t node_index = 0;
+    bool have = next_node(node_index);
+    WideNode n;
+    if (have) load_node(node_index, n);
+    while (have) {
+        {
             if (stats) stats->nodes++;
             // Slab distances of the quantized child boxes: with a = 2^e / d and
             // b = (origin - o) / d per axis, plane q is at t = q a + b (one FMA).
@@ -303,9 +317,11 @@ PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float 
             g_bits = new_bits | n.imask;
             t_base = n.tri_base;
             t_bits = tri_bits;
-        } else {
-            g_bits = 0;
         }
+        // Closest-hit rays prefetch; any-hit rays usually stop at their
+        // first triangle hit, so for them the load waits (fewer registers).
+        const bool have_next = next_node(node_index);
+        if (!kAnyHit && have_next) load_node(node_index, n);
 
         while (t_bits) {
             uint32_t i = __builtin_ctz(t_bits);
@@ -340,11 +356,8 @@ PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float 
                 }
             }
         }
-
-        if ((g_bits & 0xFF000000u) == 0) {
-            if (stack.empty()) break;
-            stack.pop(g_base, g_bits);
-        }
+        if (kAnyHit && have_next) load_node(node_index, n);
+        have = have_next;
     }
     return hit.prim != 0xFFFFFFFFu;
 }

@@ -176,8 +176,10 @@ struct WaveWork {
 // Several scenes can share the queues (batched views): the view of a path
 // sits above bit `path_bits` of its id and selects its tree's root in `roots`
 // (null for a single scene, where path_bits is 31).
+// Five waves per SIMD (at most 96 VGPRs): the closest-hit traversal keeps the
+// next node's load in flight while it tests a leaf's triangles.
 template <bool kCount, bool kSph>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(PathQueue q, DeviceBvh bvh, const uint32_t *count_ptr,
+__global__ __launch_bounds__(kTraceBlock, 5) void k_trace(PathQueue q, DeviceBvh bvh, const uint32_t *count_ptr,
                                                        uint32_t *work, unsigned long long *stat,
                                                        const uint32_t *roots, uint32_t path_bits) {
     __shared__ uint2 lds[kLdsStack * kTraceBlock];
@@ -962,8 +964,10 @@ struct pt_ctx {
     hipStream_t stream = nullptr;
     std::string error;
     int num_cu = 256;
-    uint32_t trace_blocks_per_cu = 8;      // resident traversal blocks per CU, sphere variant
-    uint32_t trace_blocks_per_cu_tri = 8;  // triangle-only variant
+    uint32_t trace_blocks_per_cu = 8;       // resident closest-hit traversal blocks per CU, sphere variant
+    uint32_t trace_blocks_per_cu_tri = 8;   // triangle-only variant
+    uint32_t shadow_blocks_per_cu = 8;      // resident any-hit (shadow) traversal blocks per CU, spheres
+    uint32_t shadow_blocks_per_cu_tri = 8;  // triangle-only variant
 
     // scene (host copies kept for the BVH build)
     bool has_scene = false;
@@ -1224,11 +1228,17 @@ int ensure_queues(pt_ctx *ctx, uint32_t paths, uint32_t max_depth) {
 uint32_t trace_grid(pt_ctx *ctx) {
     return (uint32_t)ctx->num_cu * (ctx->num_spheres > 0 ? ctx->trace_blocks_per_cu : ctx->trace_blocks_per_cu_tri);
 }
+uint32_t shadow_grid(pt_ctx *ctx, bool spheres) {
+    return (uint32_t)ctx->num_cu * (spheres ? ctx->shadow_blocks_per_cu : ctx->shadow_blocks_per_cu_tri);
+}
 uint32_t shade_grid(pt_ctx *ctx) { return (uint32_t)ctx->num_cu * 8u; }
 
 int ensure_spill(pt_ctx *ctx) {
     const uint32_t threads =
-        (uint32_t)ctx->num_cu * std::max(ctx->trace_blocks_per_cu, ctx->trace_blocks_per_cu_tri) * kTraceBlock;
+        (uint32_t)ctx->num_cu *
+        std::max(std::max(ctx->trace_blocks_per_cu, ctx->trace_blocks_per_cu_tri),
+                 std::max(ctx->shadow_blocks_per_cu, ctx->shadow_blocks_per_cu_tri)) *
+        kTraceBlock;
     if (ctx->d_spill && ctx->spill_threads >= threads) return PT_OK;
     dfree(ctx->d_spill);
     PT_HIP(hipMalloc((void **)&ctx->d_spill, (size_t)threads * kSpillEntries * sizeof(uint2)));
@@ -2049,7 +2059,8 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
         }
         {
             EventTimer t(ctx, timing);
-            hipLaunchKernelGGL(shadow_kernel(false, !b.spheres.empty()), dim3(tg), dim3(kTraceBlock), 0, ctx->stream,
+            hipLaunchKernelGGL(shadow_kernel(false, !b.spheres.empty()), dim3(shadow_grid(ctx, !b.spheres.empty())),
+                               dim3(kTraceBlock), 0, ctx->stream,
                                ctx->sq, bvh, shd + bb, swork + bb, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2,
                                (const uint32_t *)b.d_roots, path_bits);
             t.stop(kKindShadow);
@@ -2329,15 +2340,17 @@ int pt_create(int hip_device, pt_ctx **out) {
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->num_cu = prop.multiProcessorCount;
     {
-        // Persistent traversal grid: exactly the blocks that can be resident.
-        int a = 0, b = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace<false, true>, kTraceBlock, 0) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false, true>, kTraceBlock, 0) == hipSuccess)
-            ctx->trace_blocks_per_cu = (uint32_t)std::max(1, std::min(a, b));
-        a = b = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_trace<false, false>, kTraceBlock, 0) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_shadow<false, false>, kTraceBlock, 0) == hipSuccess)
-            ctx->trace_blocks_per_cu_tri = (uint32_t)std::max(1, std::min(a, b));
+        // Persistent traversal grids: exactly the blocks of each kernel that
+        // can be resident.
+        auto resident = [](const void *kernel, uint32_t &out) {
+            int n = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kernel, kTraceBlock, 0) == hipSuccess && n > 0)
+                out = (uint32_t)n;
+        };
+        resident(reinterpret_cast<const void *>(k_trace<false, true>), ctx->trace_blocks_per_cu);
+        resident(reinterpret_cast<const void *>(k_trace<false, false>), ctx->trace_blocks_per_cu_tri);
+        resident(reinterpret_cast<const void *>(k_shadow<false, true>), ctx->shadow_blocks_per_cu);
+        resident(reinterpret_cast<const void *>(k_shadow<false, false>), ctx->shadow_blocks_per_cu_tri);
         int c = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_guide_infer, kInferBlock, 0) == hipSuccess && c > 0)
             ctx->infer_blocks_per_cu = (uint32_t)c;
@@ -2937,7 +2950,8 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
             }
             {
                 EventTimer t(ctx, timing);
-                hipLaunchKernelGGL(shadow_kernel(ctx->counting, ctx->num_spheres > 0), dim3(tg), dim3(kTraceBlock), 0,
+                hipLaunchKernelGGL(shadow_kernel(ctx->counting, ctx->num_spheres > 0),
+                                   dim3(shadow_grid(ctx, ctx->num_spheres > 0)), dim3(kTraceBlock), 0,
                                    ctx->stream, ctx->sq, bvh, shd + b, swork + b, ctx->q[nxt], ctx->d_out_l,
                                    ctx->d_stat + 2, (const uint32_t *)nullptr, 31u);
                 t.stop(kKindShadow);
