@@ -227,137 +227,171 @@ struct TraverseStats {
 // `root` is the index of the tree's root node (several trees can share one
 // node array).  With kSpheres false the sphere test is compiled out (for
 // trees without sphere leaves).
-template <bool kAnyHit, bool kSpheres = true, typename Stack, typename NodeLoad, typename TriLoad>
-PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float tmin, float tmax,
-                        Stack &stack, Hit &hit, TraverseStats *stats, uint32_t root = 0u) {
-    const RayPre rp = ray_precompute(d);
-    const float idx = 1.f / (fabsf(d.x) > 1e-12f ? d.x : copysignf(1e-12f, d.x));
-    const float idy = 1.f / (fabsf(d.y) > 1e-12f ? d.y : copysignf(1e-12f, d.y));
-    const float idz = 1.f / (fabsf(d.z) > 1e-12f ? d.z : copysignf(1e-12f, d.z));
-    const uint32_t oct = (d.x < 0.f ? 1u : 0u) | (d.y < 0.f ? 2u : 0u) | (d.z < 0.f ? 4u : 0u);
-    const bool neg_x = idx < 0.f, neg_y = idy < 0.f, neg_z = idz < 0.f;  // sign of each slab slope
+//
+// The traversal of one ray is trav_init followed by trav_step until it
+// returns false; one step visits one node and tests its leaf primitives.
+// traverse_bvh runs a ray to the end; the GPU closest-hit kernel interleaves
+// rays per lane (a lane whose ray ends takes the next one).
+struct TravRay {
+    f3 o, d;
+    RayPre rp;
+    float idx, idy, idz;
+    float tmin, tmax, tbest;
+    uint32_t oct;
+    uint32_t g_base, g_bits;  // current node group
+    bool have;                // `n` holds (or is loading) the next node to visit
+    WideNode n;
+};
 
+// Next node to visit: the nearest remaining child of the current group, or of
+// the group on top of the stack; the rest of the group is pushed.  The choice
+// never depends on hits found later, so the next node is picked -- and, for
+// closest-hit rays, its load issued -- before the current node's primitives
+// are tested, and the two memory latencies overlap.
+template <typename Stack>
+PT_HD bool trav_next(TravRay &r, Stack &stack, uint32_t &node_index) {
+    if ((r.g_bits & 0xFF000000u) == 0) {
+        if (stack.empty()) return false;
+        stack.pop(r.g_base, r.g_bits);
+    }
+    const uint32_t bitpos = __builtin_ctz(r.g_bits >> 24);
+    r.g_bits &= ~(1u << (24 + bitpos));
+    const uint32_t slot = bitpos ^ r.oct;
+    const uint32_t imask = r.g_bits & 0xFFu;
+    node_index = r.g_base + __builtin_popcount(imask & ((1u << slot) - 1u));
+    if (r.g_bits & 0xFF000000u) stack.push(r.g_base, r.g_bits);
+    return true;
+}
+
+template <typename Stack, typename NodeLoad>
+PT_HD void trav_init(TravRay &r, NodeLoad load_node, f3 o, f3 d, float tmin, float tmax, Stack &stack, Hit &hit,
+                     uint32_t root) {
+    r.o = o;
+    r.d = d;
+    r.rp = ray_precompute(d);
+    r.idx = 1.f / (fabsf(d.x) > 1e-12f ? d.x : copysignf(1e-12f, d.x));
+    r.idy = 1.f / (fabsf(d.y) > 1e-12f ? d.y : copysignf(1e-12f, d.y));
+    r.idz = 1.f / (fabsf(d.z) > 1e-12f ? d.z : copysignf(1e-12f, d.z));
+    r.oct = (d.x < 0.f ? 1u : 0u) | (d.y < 0.f ? 2u : 0u) | (d.z < 0.f ? 4u : 0u);
+    r.tmin = tmin;
+    r.tmax = tmax;
+    r.tbest = tmax;
     hit.t = tmax;
     hit.prim = 0xFFFFFFFFu;
     hit.b1 = 0.f;
     hit.b2 = 0.f;
-    float tbest = tmax;
-
     // Root group: the root sits in slot 0 of a virtual parent.
-    uint32_t g_base = root, g_bits = (1u << (24 + (0 ^ oct))) | 1u;
-    uint32_t t_base = 0, t_bits = 0;
-    // Next node to visit: the nearest remaining child of the current group,
-    // or of the group on top of the stack; the rest of the group is pushed.
-    // The choice never depends on hits found later, so the next node is
-    // picked -- and, for closest-hit rays, its load issued -- before the
-    // current node's triangles are tested, and the two memory latencies
-    // overlap.
-    auto next_node = [&](uint32_t &node_index) -> bool {
-        if ((g_bits & 0xFF000000u) == 0) {
-            if (stack.empty()) return false;
-            stack.pop(g_base, g_bits);
-        }
-        const uint32_t bitpos = __builtin_ctz(g_bits >> 24);
-        g_bits &= ~(1u << (24 + bitpos));
-        const uint32_t slot = bitpos ^ oct;
-        const uint32_t imask = g_bits & 0xFFu;
-        node_index = g_base + __builtin_popcount(imask & ((1u << slot) - 1u));
-        if (g_bits & 0xFF000000u) stack.push(g_base, g_bits);
-        return true;
-    };
+    r.g_base = root;
+    r.g_bits = (1u << (24 + (0 ^ r.oct))) | 1u;
     uint32_t node_index = 0;
-    bool have = next_node(node_index);
-    WideNode n;
-    if (have) load_node(node_index, n);
-    while (have) {
-        {
-            if (stats) stats->nodes++;
-            // Slab distances of the quantized child boxes: with a = 2^e / d and
-            // b = (origin - o) / d per axis, plane q is at t = q a + b (one FMA).
-            // Rounding differs from decoding the box first by a few ulps, which
-            // the relative node margin covers; triangle tests stay exact.
-            // The near plane of an axis is the low bound when the slope a is
-            // positive and the high bound otherwise (fma is monotone in q), so
-            // the byte arrays are swapped once per node instead of taking a
-            // min and a max per child (Ylitie et al. 2017); the distances are
-            // the same as min(t0, t1) / max(t0, t1).
-            const float ax = exp2_from_biased(n.ex) * idx, bx = (n.px - o.x) * idx;
-            const float ay = exp2_from_biased(n.ey) * idy, by = (n.py - o.y) * idy;
-            const float az = exp2_from_biased(n.ez) * idz, bz = (n.pz - o.z) * idz;
-            uint32_t nqx[2], fqx[2], nqy[2], fqy[2], nqz[2], fqz[2];
-            std::memcpy(nqx, n.qlo_x, 8);
-            std::memcpy(fqx, n.qhi_x, 8);
-            std::memcpy(nqy, n.qlo_y, 8);
-            std::memcpy(fqy, n.qhi_y, 8);
-            std::memcpy(nqz, n.qlo_z, 8);
-            std::memcpy(fqz, n.qhi_z, 8);
-            if (neg_x) swap_words(nqx, fqx);
-            if (neg_y) swap_words(nqy, fqy);
-            if (neg_z) swap_words(nqz, fqz);
-            uint32_t new_bits = 0, tri_bits = 0;
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t meta = n.meta[k];
-                if (meta == 0) continue;
-                const float txn = fmaf(qbyte(nqx, k), ax, bx), txf = fmaf(qbyte(fqx, k), ax, bx);
-                const float tyn = fmaf(qbyte(nqy, k), ay, by), tyf = fmaf(qbyte(fqy, k), ay, by);
-                const float tzn = fmaf(qbyte(nqz, k), az, bz), tzf = fmaf(qbyte(fqz, k), az, bz);
-                float tn = fmaxf(fmaxf(txn, tyn), fmaxf(tzn, tmin));
-                float tf = fminf(fminf(txf, tyf), fminf(tzf, tbest));
-                if (tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin)) {
-                    if (n.imask & (1u << k)) {
-                        new_bits |= 1u << (24 + (k ^ oct));
-                    } else {
-                        uint32_t cnt_unary = meta >> 5;
-                        uint32_t off = meta & 31u;
-                        tri_bits |= cnt_unary << off;
-                    }
-                }
-            }
-            g_base = n.child_base;
-            g_bits = new_bits | n.imask;
-            t_base = n.tri_base;
-            t_bits = tri_bits;
-        }
-        // Closest-hit rays prefetch; any-hit rays usually stop at their
-        // first triangle hit, so for them the load waits (fewer registers).
-        const bool have_next = next_node(node_index);
-        if (!kAnyHit && have_next) load_node(node_index, n);
+    r.have = trav_next(r, stack, node_index);
+    if (r.have) load_node(node_index, r.n);
+}
 
-        while (t_bits) {
-            uint32_t i = __builtin_ctz(t_bits);
-            t_bits &= t_bits - 1u;
-            LeafTri tri;
-            load_tri(t_base + i, tri);
-            if (stats) stats->tris++;
-            float t, b1 = 0.f, b2 = 0.f;
-            bool hit_prim;
-            if (kSpheres && leaf_is_sphere(tri)) {
-                hit_prim = intersect_sphere(o, d, f3{tri.v0.x, tri.v0.y, tri.v0.z}, tri.v1.x, tmin, tmax, t);
-            } else {
-                hit_prim = intersect_triangle(o, rp, f3{tri.v0.x, tri.v0.y, tri.v0.z},
-                                              f3{tri.v1.x, tri.v1.y, tri.v1.z}, f3{tri.v2.x, tri.v2.y, tri.v2.z},
-                                              t, b1, b2);
-            }
-            if (hit_prim) {
-                uint32_t prim = f2u(tri.v0.w);
-                if (t >= tmin && (t < tbest || (t == tbest && prim < hit.prim))) {
-                    if (kAnyHit) {
-                        hit.t = t;
-                        hit.prim = prim;
-                        hit.b1 = b1;
-                        hit.b2 = b2;
-                        return true;
-                    }
-                    tbest = t;
-                    hit.t = t;
-                    hit.prim = prim;
-                    hit.b1 = b1;
-                    hit.b2 = b2;
+// One node of the ray's traversal and its leaf primitives.  Returns whether
+// the ray has more to visit (false also once an any-hit ray has found a hit).
+template <bool kAnyHit, bool kSpheres = true, typename Stack, typename NodeLoad, typename TriLoad>
+PT_HD bool trav_step(TravRay &r, NodeLoad load_node, TriLoad load_tri, Stack &stack, Hit &hit,
+                     TraverseStats *stats) {
+    if (!r.have) return false;
+    const WideNode &n = r.n;
+    const bool neg_x = r.idx < 0.f, neg_y = r.idy < 0.f, neg_z = r.idz < 0.f;  // sign of each slab slope
+    uint32_t t_base, t_bits;
+    {
+        if (stats) stats->nodes++;
+        // Slab distances of the quantized child boxes: with a = 2^e / d and
+        // b = (origin - o) / d per axis, plane q is at t = q a + b (one FMA).
+        // Rounding differs from decoding the box first by a few ulps, which
+        // the relative node margin covers; triangle tests stay exact.
+        // The near plane of an axis is the low bound when the slope a is
+        // positive and the high bound otherwise (fma is monotone in q), so
+        // the byte arrays are swapped once per node instead of taking a
+        // min and a max per child (Ylitie et al. 2017); the distances are
+        // the same as min(t0, t1) / max(t0, t1).
+        const float ax = exp2_from_biased(n.ex) * r.idx, bx = (n.px - r.o.x) * r.idx;
+        const float ay = exp2_from_biased(n.ey) * r.idy, by = (n.py - r.o.y) * r.idy;
+        const float az = exp2_from_biased(n.ez) * r.idz, bz = (n.pz - r.o.z) * r.idz;
+        uint32_t nqx[2], fqx[2], nqy[2], fqy[2], nqz[2], fqz[2];
+        std::memcpy(nqx, n.qlo_x, 8);
+        std::memcpy(fqx, n.qhi_x, 8);
+        std::memcpy(nqy, n.qlo_y, 8);
+        std::memcpy(fqy, n.qhi_y, 8);
+        std::memcpy(nqz, n.qlo_z, 8);
+        std::memcpy(fqz, n.qhi_z, 8);
+        if (neg_x) swap_words(nqx, fqx);
+        if (neg_y) swap_words(nqy, fqy);
+        if (neg_z) swap_words(nqz, fqz);
+        uint32_t new_bits = 0, tri_bits = 0;
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t meta = n.meta[k];
+            if (meta == 0) continue;
+            const float txn = fmaf(qbyte(nqx, k), ax, bx), txf = fmaf(qbyte(fqx, k), ax, bx);
+            const float tyn = fmaf(qbyte(nqy, k), ay, by), tyf = fmaf(qbyte(fqy, k), ay, by);
+            const float tzn = fmaf(qbyte(nqz, k), az, bz), tzf = fmaf(qbyte(fqz, k), az, bz);
+            float tn = fmaxf(fmaxf(txn, tyn), fmaxf(tzn, r.tmin));
+            float tf = fminf(fminf(txf, tyf), fminf(tzf, r.tbest));
+            if (tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin)) {
+                if (n.imask & (1u << k)) {
+                    new_bits |= 1u << (24 + (k ^ r.oct));
+                } else {
+                    uint32_t cnt_unary = meta >> 5;
+                    uint32_t off = meta & 31u;
+                    tri_bits |= cnt_unary << off;
                 }
             }
         }
-        if (kAnyHit && have_next) load_node(node_index, n);
-        have = have_next;
+        r.g_base = n.child_base;
+        r.g_bits = new_bits | n.imask;
+        t_base = n.tri_base;
+        t_bits = tri_bits;
+    }
+    // Closest-hit rays prefetch the next node; any-hit rays usually stop at
+    // their first primitive hit, so for them the load waits (fewer registers).
+    uint32_t node_index = 0;
+    const bool have_next = trav_next(r, stack, node_index);
+    if (!kAnyHit && have_next) load_node(node_index, r.n);
+
+    while (t_bits) {
+        uint32_t i = __builtin_ctz(t_bits);
+        t_bits &= t_bits - 1u;
+        LeafTri tri;
+        load_tri(t_base + i, tri);
+        if (stats) stats->tris++;
+        float t, b1 = 0.f, b2 = 0.f;
+        bool hit_prim;
+        if (kSpheres && leaf_is_sphere(tri)) {
+            hit_prim = intersect_sphere(r.o, r.d, f3{tri.v0.x, tri.v0.y, tri.v0.z}, tri.v1.x, r.tmin, r.tmax, t);
+        } else {
+            hit_prim = intersect_triangle(r.o, r.rp, f3{tri.v0.x, tri.v0.y, tri.v0.z},
+                                          f3{tri.v1.x, tri.v1.y, tri.v1.z}, f3{tri.v2.x, tri.v2.y, tri.v2.z}, t,
+                                          b1, b2);
+        }
+        if (hit_prim) {
+            uint32_t prim = f2u(tri.v0.w);
+            if (t >= r.tmin && (t < r.tbest || (t == r.tbest && prim < hit.prim))) {
+                hit.t = t;
+                hit.prim = prim;
+                hit.b1 = b1;
+                hit.b2 = b2;
+                if (kAnyHit) {
+                    r.have = false;
+                    return false;
+                }
+                r.tbest = t;
+            }
+        }
+    }
+    if (kAnyHit && have_next) load_node(node_index, r.n);
+    r.have = have_next;
+    return have_next;
+}
+
+template <bool kAnyHit, bool kSpheres = true, typename Stack, typename NodeLoad, typename TriLoad>
+PT_HD bool traverse_bvh(NodeLoad load_node, TriLoad load_tri, f3 o, f3 d, float tmin, float tmax,
+                        Stack &stack, Hit &hit, TraverseStats *stats, uint32_t root = 0u) {
+    TravRay r;
+    trav_init(r, load_node, o, d, tmin, tmax, stack, hit, root);
+    while (trav_step<kAnyHit, kSpheres>(r, load_node, load_tri, stack, hit, stats)) {
     }
     return hit.prim != 0xFFFFFFFFu;
 }
