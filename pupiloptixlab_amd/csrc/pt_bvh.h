@@ -159,13 +159,6 @@ PT_HD bool intersect_sphere(f3 o, f3 d, f3 c, float r, float tmin, float tmax, f
 
 PT_HD float exp2_from_biased(uint8_t e) { return u2f(((uint32_t)e) << 23); }
 PT_HD float qbyte(const uint32_t *w, int k) { return (float)((w[k >> 2] >> (8 * (k & 3))) & 0xFFu); }
-PT_HD void swap_words(uint32_t *a, uint32_t *b) {
-    for (int i = 0; i < 2; ++i) {
-        const uint32_t t = a[i];
-        a[i] = b[i];
-        b[i] = t;
-    }
-}
 
 // Exponent n with x = m 2^n, m in [0.5, 1), for positive finite x (frexp);
 // subnormal x reports -1022.
@@ -311,39 +304,50 @@ PT_HD bool trav_step(TravRay &r, NodeLoad load_node, TriLoad load_tri, Stack &st
         const float ax = exp2_from_biased(n.ex) * r.idx, bx = (n.px - r.o.x) * r.idx;
         const float ay = exp2_from_biased(n.ey) * r.idy, by = (n.py - r.o.y) * r.idy;
         const float az = exp2_from_biased(n.ez) * r.idz, bz = (n.pz - r.o.z) * r.idz;
+        uint32_t lo[6], hi[6];
+        std::memcpy(lo + 0, n.qlo_x, 8);
+        std::memcpy(hi + 0, n.qhi_x, 8);
+        std::memcpy(lo + 2, n.qlo_y, 8);
+        std::memcpy(hi + 2, n.qhi_y, 8);
+        std::memcpy(lo + 4, n.qlo_z, 8);
+        std::memcpy(hi + 4, n.qhi_z, 8);
         uint32_t nqx[2], fqx[2], nqy[2], fqy[2], nqz[2], fqz[2];
-        std::memcpy(nqx, n.qlo_x, 8);
-        std::memcpy(fqx, n.qhi_x, 8);
-        std::memcpy(nqy, n.qlo_y, 8);
-        std::memcpy(fqy, n.qhi_y, 8);
-        std::memcpy(nqz, n.qlo_z, 8);
-        std::memcpy(fqz, n.qhi_z, 8);
-        if (neg_x) swap_words(nqx, fqx);
-        if (neg_y) swap_words(nqy, fqy);
-        if (neg_z) swap_words(nqz, fqz);
-        uint32_t new_bits = 0, tri_bits = 0;
+        for (int i = 0; i < 2; ++i) {  // selects, not branches
+            nqx[i] = neg_x ? hi[i] : lo[i];
+            fqx[i] = neg_x ? lo[i] : hi[i];
+            nqy[i] = neg_y ? hi[2 + i] : lo[2 + i];
+            fqy[i] = neg_y ? lo[2 + i] : hi[2 + i];
+            nqz[i] = neg_z ? hi[4 + i] : lo[4 + i];
+            fqz[i] = neg_z ? lo[4 + i] : hi[4 + i];
+        }
+        // A hit internal child sets bit 24 + (slot ^ octant), a hit leaf its
+        // unary triangle count at its offset, so one word carries both.
+        // Any-hit rays test every slot without branching (an empty slot, meta
+        // 0, contributes no bit); closest-hit rays skip empty slots, which
+        // measured faster for them.  Both cull the same children.
+        uint32_t bits = 0;
         for (int k = 0; k < 8; ++k) {
             const uint32_t meta = n.meta[k];
-            if (meta == 0) continue;
+            if (!kAnyHit && meta == 0) continue;
             const float txn = fmaf(qbyte(nqx, k), ax, bx), txf = fmaf(qbyte(fqx, k), ax, bx);
             const float tyn = fmaf(qbyte(nqy, k), ay, by), tyf = fmaf(qbyte(fqy, k), ay, by);
             const float tzn = fmaf(qbyte(nqz, k), az, bz), tzf = fmaf(qbyte(fqz, k), az, bz);
             float tn = fmaxf(fmaxf(txn, tyn), fmaxf(tzn, r.tmin));
             float tf = fminf(fminf(txf, tyf), fminf(tzf, r.tbest));
-            if (tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin)) {
-                if (n.imask & (1u << k)) {
-                    new_bits |= 1u << (24 + (k ^ r.oct));
-                } else {
-                    uint32_t cnt_unary = meta >> 5;
-                    uint32_t off = meta & 31u;
-                    tri_bits |= cnt_unary << off;
-                }
+            const bool internal = (n.imask >> k) & 1u;
+            if (kAnyHit) {
+                const bool hit = meta != 0u && tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin);
+                const uint32_t child = internal ? (1u << 24) << (k ^ r.oct) : (meta >> 5) << (meta & 31u);
+                bits |= hit ? child : 0u;
+            } else if (tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin)) {
+                if (internal) bits |= (1u << 24) << (k ^ r.oct);
+                else bits |= (meta >> 5) << (meta & 31u);
             }
         }
         r.g_base = n.child_base;
-        r.g_bits = new_bits | n.imask;
+        r.g_bits = (bits & 0xFF000000u) | n.imask;
         t_base = n.tri_base;
-        t_bits = tri_bits;
+        t_bits = bits & 0x00FFFFFFu;
     }
     // Closest-hit rays prefetch the next node; any-hit rays usually stop at
     // their first primitive hit, so for them the load waits (fewer registers).
