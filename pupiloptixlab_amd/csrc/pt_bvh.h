@@ -34,6 +34,16 @@ struct WideNode {           // 80 bytes, read as five 16-byte loads
 };
 static_assert(sizeof(WideNode) == 80, "wide node must be 80 bytes");
 
+// A node as traversal holds it: its 80 bytes as twenty 32-bit words, decoded
+// with shifts so that registers hold whole words.  Words: 0-2 px, py, pz;
+// 3 ex | ey << 8 | ez << 16 | imask << 24; 4 child_base; 5 tri_base; 6-7
+// meta; 8-9 qlo_x; 10-11 qlo_y; 12-13 qlo_z; 14-15 qhi_x; 16-17 qhi_y;
+// 18-19 qhi_z (little-endian byte order within a word).
+struct NodeWords {
+    uint32_t w[20];
+};
+static_assert(sizeof(NodeWords) == sizeof(WideNode), "node words must cover the node");
+
 // Leaf primitives in traversal order, three float4 each.  Triangle: v0.xyz +
 // primitive id bits, v1.xyz, v2.xyz (v1.w = 0).  Sphere: v0 = (center,
 // primitive id bits), v1 = (radius, 0, 0, tag bits 1), v2 unused.
@@ -233,7 +243,7 @@ struct TravRay {
     uint32_t oct;
     uint32_t g_base, g_bits;  // current node group
     bool have;                // `n` holds (or is loading) the next node to visit
-    WideNode n;
+    NodeWords n;
 };
 
 // Next node to visit: the nearest remaining child of the current group, or of
@@ -287,7 +297,8 @@ template <bool kAnyHit, bool kSpheres = true, typename Stack, typename NodeLoad,
 PT_HD bool trav_step(TravRay &r, NodeLoad load_node, TriLoad load_tri, Stack &stack, Hit &hit,
                      TraverseStats *stats) {
     if (!r.have) return false;
-    const WideNode &n = r.n;
+    const uint32_t *w = r.n.w;
+    const uint32_t imask = w[3] >> 24;
     const bool neg_x = r.idx < 0.f, neg_y = r.idy < 0.f, neg_z = r.idz < 0.f;  // sign of each slab slope
     uint32_t t_base, t_bits;
     {
@@ -301,24 +312,17 @@ PT_HD bool trav_step(TravRay &r, NodeLoad load_node, TriLoad load_tri, Stack &st
         // the byte arrays are swapped once per node instead of taking a
         // min and a max per child (Ylitie et al. 2017); the distances are
         // the same as min(t0, t1) / max(t0, t1).
-        const float ax = exp2_from_biased(n.ex) * r.idx, bx = (n.px - r.o.x) * r.idx;
-        const float ay = exp2_from_biased(n.ey) * r.idy, by = (n.py - r.o.y) * r.idy;
-        const float az = exp2_from_biased(n.ez) * r.idz, bz = (n.pz - r.o.z) * r.idz;
-        uint32_t lo[6], hi[6];
-        std::memcpy(lo + 0, n.qlo_x, 8);
-        std::memcpy(hi + 0, n.qhi_x, 8);
-        std::memcpy(lo + 2, n.qlo_y, 8);
-        std::memcpy(hi + 2, n.qhi_y, 8);
-        std::memcpy(lo + 4, n.qlo_z, 8);
-        std::memcpy(hi + 4, n.qhi_z, 8);
+        const float ax = u2f((w[3] & 0xFFu) << 23) * r.idx, bx = (u2f(w[0]) - r.o.x) * r.idx;
+        const float ay = u2f(((w[3] >> 8) & 0xFFu) << 23) * r.idy, by = (u2f(w[1]) - r.o.y) * r.idy;
+        const float az = u2f(((w[3] >> 16) & 0xFFu) << 23) * r.idz, bz = (u2f(w[2]) - r.o.z) * r.idz;
         uint32_t nqx[2], fqx[2], nqy[2], fqy[2], nqz[2], fqz[2];
         for (int i = 0; i < 2; ++i) {  // selects, not branches
-            nqx[i] = neg_x ? hi[i] : lo[i];
-            fqx[i] = neg_x ? lo[i] : hi[i];
-            nqy[i] = neg_y ? hi[2 + i] : lo[2 + i];
-            fqy[i] = neg_y ? lo[2 + i] : hi[2 + i];
-            nqz[i] = neg_z ? hi[4 + i] : lo[4 + i];
-            fqz[i] = neg_z ? lo[4 + i] : hi[4 + i];
+            nqx[i] = neg_x ? w[14 + i] : w[8 + i];
+            fqx[i] = neg_x ? w[8 + i] : w[14 + i];
+            nqy[i] = neg_y ? w[16 + i] : w[10 + i];
+            fqy[i] = neg_y ? w[10 + i] : w[16 + i];
+            nqz[i] = neg_z ? w[18 + i] : w[12 + i];
+            fqz[i] = neg_z ? w[12 + i] : w[18 + i];
         }
         // A hit internal child sets bit 24 + (slot ^ octant), a hit leaf its
         // unary triangle count at its offset, so one word carries both.
@@ -327,14 +331,14 @@ PT_HD bool trav_step(TravRay &r, NodeLoad load_node, TriLoad load_tri, Stack &st
         // measured faster for them.  Both cull the same children.
         uint32_t bits = 0;
         for (int k = 0; k < 8; ++k) {
-            const uint32_t meta = n.meta[k];
+            const uint32_t meta = (w[6 + (k >> 2)] >> (8 * (k & 3))) & 0xFFu;
             if (!kAnyHit && meta == 0) continue;
             const float txn = fmaf(qbyte(nqx, k), ax, bx), txf = fmaf(qbyte(fqx, k), ax, bx);
             const float tyn = fmaf(qbyte(nqy, k), ay, by), tyf = fmaf(qbyte(fqy, k), ay, by);
             const float tzn = fmaf(qbyte(nqz, k), az, bz), tzf = fmaf(qbyte(fqz, k), az, bz);
             float tn = fmaxf(fmaxf(txn, tyn), fmaxf(tzn, r.tmin));
             float tf = fminf(fminf(txf, tyf), fminf(tzf, r.tbest));
-            const bool internal = (n.imask >> k) & 1u;
+            const bool internal = (imask >> k) & 1u;
             if (kAnyHit) {
                 const bool hit = meta != 0u && tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin);
                 const uint32_t child = internal ? (1u << 24) << (k ^ r.oct) : (meta >> 5) << (meta & 31u);
@@ -344,9 +348,9 @@ PT_HD bool trav_step(TravRay &r, NodeLoad load_node, TriLoad load_tri, Stack &st
                 else bits |= (meta >> 5) << (meta & 31u);
             }
         }
-        r.g_base = n.child_base;
-        r.g_bits = (bits & 0xFF000000u) | n.imask;
-        t_base = n.tri_base;
+        r.g_base = w[4];
+        r.g_bits = (bits & 0xFF000000u) | imask;
+        t_base = w[5];
         t_bits = bits & 0x00FFFFFFu;
     }
     // Closest-hit rays prefetch the next node; any-hit rays usually stop at

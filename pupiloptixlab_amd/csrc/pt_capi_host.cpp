@@ -130,7 +130,7 @@ int pt_host_trace_rays(const pt_scene_desc *s, const float *rays, uint32_t count
         HostStack stack;
         pupil::Hit hit;
         pupil::TraverseStats ts{0, 0};
-        auto ln = [&](uint32_t k, pupil::WideNode &n) { n = nodes[k]; };
+        auto ln = [&](uint32_t k, pupil::NodeWords &n) { std::memcpy(&n, &nodes[k], sizeof(n)); };
         auto lt = [&](uint32_t k, pupil::LeafTri &t) { t = leaf[k]; };
         if (any_hit)
             pupil::traverse_bvh<true>(ln, lt, pupil::mk3(r[0], r[1], r[2]), pupil::mk3(r[4], r[5], r[6]), r[3], r[7],

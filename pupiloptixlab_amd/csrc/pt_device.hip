@@ -57,6 +57,17 @@ struct DeviceBvh {
     uint2 *spill;
 };
 
+__device__ __forceinline__ void load_node_words(const WideNode *nodes, uint32_t i, NodeWords &n) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(nodes + i);
+    for (int k = 0; k < 5; ++k) {
+        const uint4 v = p[k];
+        n.w[4 * k + 0] = v.x;
+        n.w[4 * k + 1] = v.y;
+        n.w[4 * k + 2] = v.z;
+        n.w[4 * k + 3] = v.w;
+    }
+}
+
 __device__ __forceinline__ void load_node_dev(const WideNode *nodes, uint32_t i, WideNode &n) {
     const uint4 *p = reinterpret_cast<const uint4 *>(nodes + i);
     uint4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4];
@@ -91,7 +102,7 @@ template <bool kAnyHit, bool kCount, bool kSph>
 __device__ __forceinline__ bool trace_one(const DeviceBvh &bvh, uint2 *lds_col, uint32_t gtid, f3 o, f3 d,
                                           float tmin, float tmax, Hit &h, TraverseStats &st, uint32_t root = 0u) {
     LdsStack stack{lds_col, bvh.spill + (size_t)gtid * kSpillEntries, (uint32_t)kTraceBlock, 0u};
-    auto ln = [&](uint32_t i, WideNode &n) { load_node_dev(bvh.nodes, i, n); };
+    auto ln = [&](uint32_t i, NodeWords &n) { load_node_words(bvh.nodes, i, n); };
     auto lt = [&](uint32_t i, LeafTri &t) {
         const float4 *p = reinterpret_cast<const float4 *>(bvh.tris + i);
         float4 a = p[0], b = p[1], c = p[2];
