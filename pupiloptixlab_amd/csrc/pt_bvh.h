@@ -326,25 +326,17 @@ PT_HD bool trav_step(TravRay &r, NodeLoad load_node, TriLoad load_tri, Stack &st
         }
         // A hit internal child sets bit 24 + (slot ^ octant), a hit leaf its
         // unary triangle count at its offset, so one word carries both.
-        // Any-hit rays test every slot without branching (an empty slot, meta
-        // 0, contributes no bit); closest-hit rays skip empty slots, which
-        // measured faster for them.  Both cull the same children.
         uint32_t bits = 0;
         for (int k = 0; k < 8; ++k) {
             const uint32_t meta = (w[6 + (k >> 2)] >> (8 * (k & 3))) & 0xFFu;
-            if (!kAnyHit && meta == 0) continue;
+            if (meta == 0) continue;
             const float txn = fmaf(qbyte(nqx, k), ax, bx), txf = fmaf(qbyte(fqx, k), ax, bx);
             const float tyn = fmaf(qbyte(nqy, k), ay, by), tyf = fmaf(qbyte(fqy, k), ay, by);
             const float tzn = fmaf(qbyte(nqz, k), az, bz), tzf = fmaf(qbyte(fqz, k), az, bz);
             float tn = fmaxf(fmaxf(txn, tyn), fmaxf(tzn, r.tmin));
             float tf = fminf(fminf(txf, tyf), fminf(tzf, r.tbest));
-            const bool internal = (imask >> k) & 1u;
-            if (kAnyHit) {
-                const bool hit = meta != 0u && tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin);
-                const uint32_t child = internal ? (1u << 24) << (k ^ r.oct) : (meta >> 5) << (meta & 31u);
-                bits |= hit ? child : 0u;
-            } else if (tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin)) {
-                if (internal) bits |= (1u << 24) << (k ^ r.oct);
+            if (tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin)) {
+                if ((imask >> k) & 1u) bits |= (1u << 24) << (k ^ r.oct);
                 else bits |= (meta >> 5) << (meta & 31u);
             }
         }

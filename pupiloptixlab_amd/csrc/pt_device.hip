@@ -34,7 +34,7 @@ namespace {
 enum KernelKind { kKindRaygen = 0, kKindTrace, kKindShade, kKindShadow, kKindAccumulate, kKindGuide, kKindGuideInfer };
 
 constexpr int kTraceBlock = 128;
-constexpr int kLdsStack = 12;  // stack entries per lane kept in LDS
+constexpr int kLdsStack = 8;   // stack entries per lane kept in LDS
 constexpr uint32_t kSpillEntries = kMaxBvhDepth + 2 - kLdsStack;
 
 // --- queue layout -------------------------------------------------------------
