@@ -60,6 +60,9 @@ constexpr uint32_t kPrimIndexMask = kPrimSphere - 1u;
 PT_HD bool leaf_is_sphere(const LeafTri &l) { return f2u(l.v1.w) != 0u; }
 
 constexpr float kNodeMargin = 1e-4f;     // relative widening of node intervals
+// The widened test tn (1 - m) <= tf (1 + m) as one product: tn <= tf r with
+// r = (1 + m) / (1 - m) rounded up (tn >= tmin >= 0).
+constexpr float kNodeMarginRatio = 1.0002001f;
 constexpr uint32_t kMaxBvhDepth = 64;    // builder guarantees depth <= this
 
 struct Hit {
@@ -335,7 +338,7 @@ PT_HD bool trav_step(TravRay &r, NodeLoad load_node, TriLoad load_tri, Stack &st
             const float tzn = fmaf(qbyte(nqz, k), az, bz), tzf = fmaf(qbyte(fqz, k), az, bz);
             float tn = fmaxf(fmaxf(txn, tyn), fmaxf(tzn, r.tmin));
             float tf = fminf(fminf(txf, tyf), fminf(tzf, r.tbest));
-            if (tn * (1.f - kNodeMargin) <= tf * (1.f + kNodeMargin)) {
+            if (tn <= tf * kNodeMarginRatio) {
                 if ((imask >> k) & 1u) bits |= (1u << 24) << (k ^ r.oct);
                 else bits |= (meta >> 5) << (meta & 31u);
             }
