@@ -318,7 +318,7 @@ struct LaunchDesc {
 // With `views` set (batched render) the path id carries the view above bit
 // `path_bits`; the view selects its emitters, environment and depth.
 template <bool kBatch>
-__global__ __launch_bounds__(256, kBatch ? 1 : 4) void k_shade(SceneView scene, ShadeParams prm, PathQueue in,
+__global__ __launch_bounds__(256, kBatch ? 1 : 5) void k_shade(SceneView scene, ShadeParams prm, PathQueue in,
                                                const uint32_t *count_in, PathQueue out, uint32_t *count_out,
                                                ShadowQueue sq, uint32_t *count_shadow, float4 *out_l,
                                                AovBuffers aov, const BatchViewDev *views, uint32_t path_bits) {
