@@ -3604,7 +3604,7 @@ int pt_batch_autotune(pt_ctx *ctx, const pt_batch_params *params, pt_batch_tunin
             best = sh;
         }
     }
-    const uint32_t shades[][2] = {{256, 4}, {256, 16}, {128, 8}, {128, 16}, {64, 16}};
+    const uint32_t shades[][2] = {{256, 4}, {256, 16}, {256, 64}, {128, 8}, {128, 16}, {128, 128}, {64, 16}};
     for (auto &sb : shades) {
         if (rc != PT_OK) break;
         BatchShape sh = best;
