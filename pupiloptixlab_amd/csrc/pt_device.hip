@@ -2843,12 +2843,14 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
         return PT_ERR_INVALID_ARG;
     }
     uint32_t per_launch = p->spp_per_launch;
-    // Default launch width: ~64M paths in flight (about 15 GB of queues), which
-    // amortizes the latency-bound tail of the last bounces over many frames.
+    // Default launch width: ~128M paths in flight (about 30 GB of queues, a
+    // tenth of the HBM), which amortizes the latency-bound tail of the last
+    // bounces over many frames (terrain 1080p, 64 spp: 748, 794 and 820
+    // Msamples/s at 16, 32 and 64 frames per launch).
     // While training the guide, launches are kept short (~2M paths, one
     // 1080p frame) so the network is updated after every launch.
     if (per_launch == 0) {
-        const uint32_t paths = guide_train ? 2u * 1024u * 1024u : 64u * 1024u * 1024u;
+        const uint32_t paths = guide_train ? 2u * 1024u * 1024u : 128u * 1024u * 1024u;
         per_launch = std::max(1u, std::min(64u, paths / std::max(1u, npix)));
     }
     per_launch = std::min(per_launch, std::max(1u, p->spp));
