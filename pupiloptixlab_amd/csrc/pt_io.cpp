@@ -8,6 +8,7 @@
 // scale -> rotate -> translate regardless of element order
 // (resource/xml/util_loader.cpp:128-192).
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -746,6 +747,66 @@ bool load_xml_scene(const std::string &path, HostScene &out, std::string &err) {
     return true;
 }
 
+// Ear clipping of a polygon with five or more corners (assimp's
+// aiProcess_Triangulate clips ears of larger polygons for the reference,
+// resource/shape.cpp:273).  The corners are projected onto the coordinate
+// plane most perpendicular to the Newell normal; an ear is a convex corner
+// whose triangle with its two neighbours holds no other remaining corner.
+// The search starts at corner 1 and resumes at the corner after the one
+// clipped, so a convex polygon gives the fan from corner 0 -- (0, k, k + 1)
+// -- and every triangle keeps the polygon's winding.  When no ear is left
+// (degenerate or self-intersecting input) the remaining corners are fanned.
+static std::vector<std::array<int, 3>> triangulate_face(const std::vector<std::array<double, 3>> &P) {
+    const size_t n = P.size();
+    double nrm[3] = {0.0, 0.0, 0.0};
+    for (size_t a = 0; a < n; ++a) {
+        const std::array<double, 3> &p0 = P[a], &p1 = P[(a + 1) % n];
+        nrm[0] += (p0[1] - p1[1]) * (p0[2] + p1[2]);
+        nrm[1] += (p0[2] - p1[2]) * (p0[0] + p1[0]);
+        nrm[2] += (p0[0] - p1[0]) * (p0[1] + p1[1]);
+    }
+    int k = 0;
+    for (int c = 1; c < 3; ++c)
+        if (std::fabs(nrm[c]) > std::fabs(nrm[k])) k = c;
+    const int iu = (k + 1) % 3, iv = (k + 2) % 3;  // cyclic: 2D winding sign = sign of nrm[k]
+    const double orient = nrm[k] >= 0.0 ? 1.0 : -1.0;
+    auto cross2 = [&](size_t a, size_t b, size_t c) {
+        return ((P[b][iu] - P[a][iu]) * (P[c][iv] - P[a][iv]) - (P[b][iv] - P[a][iv]) * (P[c][iu] - P[a][iu])) *
+               orient;
+    };
+    std::vector<size_t> rem(n);
+    for (size_t a = 0; a < n; ++a) rem[a] = a;
+    std::vector<std::array<int, 3>> out;
+    size_t i = 1;
+    while (rem.size() > 3) {
+        const size_t m = rem.size();
+        bool clipped = false;
+        for (size_t tries = 0; tries < m; ++tries, ++i) {
+            const size_t c = i % m, pr = (c + m - 1) % m, nx = (c + 1) % m;
+            const size_t a = rem[pr], b = rem[c], d = rem[nx];
+            if (!(cross2(a, b, d) > 0.0)) continue;  // reflex or degenerate corner
+            bool inside = false;
+            for (size_t q = 0; q < m && !inside; ++q) {
+                if (q == pr || q == c || q == nx) continue;
+                const size_t e = rem[q];
+                inside = cross2(a, b, e) >= 0.0 && cross2(b, d, e) >= 0.0 && cross2(d, a, e) >= 0.0;
+            }
+            if (inside) continue;
+            out.push_back({(int)a, (int)b, (int)d});
+            rem.erase(rem.begin() + (std::ptrdiff_t)c);
+            i = c;  // the corner after the clipped one now sits at position c
+            clipped = true;
+            break;
+        }
+        if (!clipped) {
+            for (size_t q = 1; q + 1 < rem.size(); ++q) out.push_back({(int)rem[0], (int)rem[q], (int)rem[q + 1]});
+            return out;
+        }
+    }
+    out.push_back({(int)rem[0], (int)rem[1], (int)rem[2]});
+    return out;
+}
+
 // Wavefront OBJ: v / vt / vn / f (polygons triangulated as below); one vertex
 // per face corner.
 bool load_obj_mesh(const std::string &path, MeshData &mesh, std::string &err) {
@@ -807,7 +868,22 @@ bool load_obj_mesh(const std::string &path, MeshData &mesh, std::string &err) {
             // aiProcess_Triangulate gives the reference (resource/shape.cpp:273)
             // for quads; convex quads keep the fan from the first corner.
             // Reflex: the turn at the corner opposes the polygon normal
-            // (Newell).  Larger polygons are fanned from their first corner.
+            // (Newell).  Larger polygons are ear-clipped (triangulate_face).
+            if (face.size() >= 5) {
+                std::vector<std::array<double, 3>> P(face.size());
+                for (size_t a = 0; a < face.size(); ++a)
+                    for (int c = 0; c < 3; ++c) P[a][(size_t)c] = v[3 * (size_t)(face[a].v - 1) + c];
+                for (const std::array<int, 3> &tri : triangulate_face(P)) {
+                    for (int k : tri) {
+                        const Corner &c = face[(size_t)k];
+                        idx.push_back((uint32_t)corners.size());
+                        corners.push_back(c);
+                        any_n = any_n || c.n > 0;
+                        any_t = any_t || c.t > 0;
+                    }
+                }
+                continue;
+            }
             if (face.size() == 4) {
                 double P[4][3];
                 for (int a = 0; a < 4; ++a)

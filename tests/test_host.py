@@ -316,6 +316,75 @@ def test_obj_concave_quad_fans_from_the_concave_corner(tmp_path, rot):
     assert q.tolist() == [[0, 0], [1, 0], [1, 1], [0, 0], [1, 1], [0, 1]]
 
 
+def _inside(poly, pt2):
+    """Even-odd rule for a 2D point against a polygon (independent of the loader)."""
+    x, y = pt2
+    c = False
+    for k in range(len(poly)):
+        (x0, y0), (x1, y1) = poly[k], poly[(k + 1) % len(poly)]
+        if (y0 > y) != (y1 > y) and x < x0 + (y - y0) * (x1 - x0) / (y1 - y0):
+            c = not c
+    return c
+
+
+def _signed_area(poly):
+    return 0.5 * sum(poly[k][0] * poly[(k + 1) % len(poly)][1] - poly[(k + 1) % len(poly)][0] * poly[k][1]
+                     for k in range(len(poly)))
+
+
+CONCAVE_POLYGONS = {
+    "L": [(0, 0), (3, 0), (3, 1), (1, 1), (1, 3), (0, 3)],
+    "comb": [(0, 0), (5, 0), (5, 3), (4, 3), (4, 1), (3, 1), (3, 3), (2, 3), (2, 1), (1, 1), (1, 3), (0, 3)],
+    "star": [(np.cos(a) * r, np.sin(a) * r) for a, r in
+             zip(np.linspace(0, 2 * np.pi, 10, endpoint=False), [2.0, 0.8] * 5)],
+}
+
+
+@pytest.mark.parametrize("name", sorted(CONCAVE_POLYGONS))
+@pytest.mark.parametrize("rot", [0, 1, 3])
+@pytest.mark.parametrize("plane,flip", [("xy", False), ("xy", True), ("yz", False), ("zx", True)])
+def test_obj_concave_polygon_is_ear_clipped(tmp_path, name, rot, plane, flip):
+    """Faces with five or more corners are ear-clipped on their own plane:
+    n - 2 triangles that keep the polygon's winding, tile it (areas sum to
+    the shoelace area) and lie inside it (every centroid passes the even-odd
+    test), whatever corner the face starts at, its winding and its plane."""
+    poly = CONCAVE_POLYGONS[name]
+    if flip:
+        poly = poly[::-1]
+    poly = poly[rot:] + poly[:rot]
+    axes = {"xy": (0, 1, 2), "yz": (1, 2, 0), "zx": (2, 0, 1)}[plane]
+    lines = []
+    for u, v in poly:
+        xyz = [0.0, 0.0, 0.0]
+        xyz[axes[0]], xyz[axes[1]], xyz[axes[2]] = float(u), float(v), 0.25
+        lines.append("v %.9g %.9g %.9g" % tuple(xyz))
+    lines.append("f " + " ".join(str(k + 1) for k in range(len(poly))))
+    (tmp_path / "p.obj").write_text("\n".join(lines) + "\n")
+    xml = """<scene version="3.0.0"><shape type="obj"><string name="filename" value="p.obj"/></shape></scene>"""
+    s = pt.HostScene.load_xml(write(tmp_path, "p.xml", xml))
+    p = s.positions().reshape(-1, 3)[:, [axes[0], axes[1]]].astype(np.float64)
+    idx = np.ctypeslib.as_array(s.desc.indices, shape=(s.desc.num_triangles * 3,)).reshape(-1, 3)
+    assert len(idx) == len(poly) - 2
+    area = _signed_area(poly)
+    tris = [p[t] for t in idx]
+    signed = [_signed_area([tuple(q) for q in t]) for t in tris]
+    assert all(a * area > 0 for a in signed)  # winding kept, no degenerate triangle
+    assert sum(abs(a) for a in signed) == pytest.approx(abs(area), rel=1e-6)
+    assert all(_inside(poly, t.mean(axis=0)) for t in tris)
+
+
+def test_obj_convex_polygon_fans_from_first_corner(tmp_path):
+    """A convex pentagon keeps the fan (0, 1, 2), (0, 2, 3), (0, 3, 4)."""
+    pent = [(np.cos(a), np.sin(a)) for a in np.linspace(0, 2 * np.pi, 5, endpoint=False)]
+    lines = ["v %.9g %.9g 0" % q for q in pent] + ["f 1 2 3 4 5"]
+    (tmp_path / "pent.obj").write_text("\n".join(lines) + "\n")
+    xml = """<scene version="3.0.0"><shape type="obj"><string name="filename" value="pent.obj"/></shape></scene>"""
+    s = pt.HostScene.load_xml(write(tmp_path, "pent.xml", xml))
+    p = s.positions().reshape(-1, 3)[:, :2]
+    want = np.array([pent[k] for t in ((0, 1, 2), (0, 2, 3), (0, 3, 4)) for k in t], np.float32)
+    assert np.array_equal(p, want)
+
+
 @pytest.mark.parametrize("text,code", [
     ("<scene><shape type='cube'>", pt.STATUS and -6),
     ("<notscene/>", -6),
