@@ -1,17 +1,22 @@
 // JPEG reader for bitmap textures (util/texture.cpp:87-116 reads JPEG files
-// through stb_image).  Written from ITU-T T.81 / JFIF 1.02: sequential DCT
-// with Huffman coding (SOF0 / SOF1), 8-bit samples, 1 (gray) or 3 (YCbCr)
-// components with sampling factors up to 4, restart intervals.  Progressive
-// and arithmetic-coded files are reported as unsupported.
+// through stb_image).  Written from ITU-T T.81 / JFIF 1.02: sequential
+// (SOF0 / SOF1) and progressive (SOF2, Annex G: spectral selection and
+// successive approximation) DCT with Huffman coding, 8-bit samples, 1 (gray)
+// or 3 (YCbCr) components with sampling factors up to 4, restart intervals,
+// scans holding any subset of the components.  Lossless and arithmetic-coded
+// files are reported as unsupported.
 //
-// Decoding steps: Huffman decode of each 8x8 block (DC difference + AC
-// run/size pairs, zig-zag order), dequantization, a separable float IDCT
+// Decoding steps: Huffman decode of each scan into per-component quantized
+// coefficients (DC difference + AC run/size pairs in zig-zag order; the
+// progressive scans add bands and bit planes to them), then per 8x8 block
+// dequantization, a separable float IDCT
 // rounded to 8 bits, chroma upsampling by linear ("triangle") interpolation
 // at the JFIF sample positions, and the JFIF YCbCr -> RGB matrix.  Outputs of
 // different decoders of the same file may differ by a level or two (IDCT and
 // upsampling rounding are not specified by the standard).
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -36,7 +41,9 @@ struct Huffman {
 struct Component {
     int id = 0, h = 1, v = 1, tq = 0, td = 0, ta = 0;
     int bw = 0, bh = 0;  // blocks per line / column (padded to whole MCUs)
+    int cbw = 0, cbh = 0;  // blocks covering the component itself (non-interleaved scans)
     int dc_pred = 0;
+    std::vector<int16_t> coef;         // bw x bh blocks of 64 quantized coefficients, natural order
     std::vector<unsigned char> plane;  // bw * 8 x bh * 8 samples
 };
 
@@ -170,13 +177,15 @@ bool load_jpeg(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, st
     Huffman dc[4], ac[4];
     std::vector<Component> comp;
     int W = 0, H = 0, hmax = 1, vmax = 1, restart_interval = 0;
-    bool frame = false, done = false, adobe_rgb = false;
+    bool frame = false, progressive = false, adobe_rgb = false;
+    int scans = 0;
     size_t p = 2;
     auto seg_len = [&](size_t at) -> size_t { return (size_t)f[at] << 8 | f[at + 1]; };
-    while (!done) {
+    while (true) {
         while (p < f.size() && f[p] != 0xFF) ++p;  // skip fill up to a marker
         while (p < f.size() && f[p] == 0xFF) ++p;
         if (p >= f.size()) {
+            if (scans > 0) break;  // image data without EOI: keep what was decoded
             err = "parse: JPEG ends before its image data";
             return false;
         }
@@ -236,11 +245,16 @@ bool load_jpeg(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, st
                 hf.defined = true;
                 q += 17 + total;
             }
-        } else if (m == 0xC0 || m == 0xC1) {  // SOF0 / SOF1
+        } else if (m == 0xC0 || m == 0xC1 || m == 0xC2) {  // SOF0 / SOF1 / SOF2
+            if (frame) {
+                err = "parse: JPEG with two frame headers";
+                return false;
+            }
             if (n < 6 || d[0] != 8) {
                 err = "unsupported: JPEG sample precision";
                 return false;
             }
+            progressive = m == 0xC2;
             H = d[1] << 8 | d[2];
             W = d[3] << 8 | d[4];
             const int nc = d[5];
@@ -261,9 +275,19 @@ bool load_jpeg(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, st
                 hmax = std::max(hmax, comp[k].h);
                 vmax = std::max(vmax, comp[k].v);
             }
+            // Coefficient storage padded to whole MCUs; a single-component
+            // frame is never interleaved, its blocks cover the image itself.
+            const int mcux = (W + 8 * hmax - 1) / (8 * hmax), mcuy = (H + 8 * vmax - 1) / (8 * vmax);
+            for (auto &c : comp) {
+                c.cbw = ((W * c.h + hmax - 1) / hmax + 7) / 8;
+                c.cbh = ((H * c.v + vmax - 1) / vmax + 7) / 8;
+                c.bw = nc == 1 ? c.cbw : mcux * c.h;
+                c.bh = nc == 1 ? c.cbh : mcuy * c.v;
+                c.coef.assign((size_t)c.bw * c.bh * 64, 0);
+            }
             frame = true;
-        } else if ((m >= 0xC2 && m <= 0xC3) || (m >= 0xC5 && m <= 0xCF && m != 0xC8 && m != 0xCC)) {
-            err = "unsupported: progressive, lossless or arithmetic-coded JPEG";
+        } else if (m == 0xC3 || (m >= 0xC5 && m <= 0xCF && m != 0xC8 && m != 0xCC)) {
+            err = "unsupported: lossless or arithmetic-coded JPEG";
             return false;
         } else if (m == 0xDD) {  // DRI
             if (n < 2) {
@@ -273,93 +297,149 @@ bool load_jpeg(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, st
             restart_interval = d[0] << 8 | d[1];
         } else if (m == 0xEE && n >= 12 && std::memcmp(d, "Adobe", 5) == 0) {
             adobe_rgb = d[11] == 0;  // transform 0: components are RGB, not YCbCr
-        } else if (m == 0xDA) {  // SOS: one scan with every component (sequential)
+        } else if (m == 0xDA) {  // SOS
             if (!frame) {
                 err = "parse: JPEG scan before frame header";
                 return false;
             }
-            const int ns = d[0];
-            if (ns != (int)comp.size() || n < 1 + 2 * (size_t)ns + 3) {
-                err = "unsupported: JPEG with several scans";
+            const int ns = n >= 1 ? d[0] : 0;
+            if (ns < 1 || ns > (int)comp.size() || n < 1 + 2 * (size_t)ns + 3) {
+                err = "parse: bad JPEG scan header";
                 return false;
             }
+            const int ss = d[1 + 2 * ns], se = d[2 + 2 * ns], ah = d[3 + 2 * ns] >> 4, al = d[3 + 2 * ns] & 15;
+            // Scan kinds (T.81 G.1.1.1.1): sequential scans carry the whole
+            // band 0..63 at full precision; progressive scans carry the DC
+            // (Ss = 0 = Se, any components) or one AC band of one component.
+            if (progressive ? (ss > se || se > 63 || (ss == 0 && se != 0) || (ss > 0 && ns != 1) || al > 13 ||
+                               (ah != 0 && ah != al + 1))
+                            : (ss != 0 || se != 63 || ah != 0 || al != 0)) {
+                err = "parse: bad JPEG scan parameters";
+                return false;
+            }
+            std::vector<Component *> sc;
             for (int k = 0; k < ns; ++k) {
                 const int cid = d[1 + 2 * k];
-                int ci = -1;
-                for (size_t c = 0; c < comp.size(); ++c)
-                    if (comp[c].id == cid) ci = (int)c;
-                if (ci < 0) {
+                Component *c = nullptr;
+                for (auto &cc : comp)
+                    if (cc.id == cid) c = &cc;
+                if (!c || std::find(sc.begin(), sc.end(), c) != sc.end()) {
                     err = "parse: JPEG scan names an unknown component";
                     return false;
                 }
-                comp[ci].td = d[2 + 2 * k] >> 4;
-                comp[ci].ta = d[2 + 2 * k] & 15;
-                if (comp[ci].td > 3 || comp[ci].ta > 3 || !dc[comp[ci].td].defined || !ac[comp[ci].ta].defined ||
-                    !qt_defined[comp[ci].tq]) {
+                c->td = d[2 + 2 * k] >> 4;
+                c->ta = d[2 + 2 * k] & 15;
+                const bool need_dc = ss == 0 && ah == 0, need_ac = se > 0;
+                if (c->td > 3 || c->ta > 3 || (need_dc && !dc[c->td].defined) || (need_ac && !ac[c->ta].defined) ||
+                    !qt_defined[c->tq]) {
                     err = "parse: JPEG scan without its tables";
                     return false;
                 }
+                sc.push_back(c);
             }
             p += len;
-            // Entropy-coded data.
-            const int mcux = (W + 8 * hmax - 1) / (8 * hmax), mcuy = (H + 8 * vmax - 1) / (8 * vmax);
-            for (auto &c : comp) {
-                c.bw = mcux * (comp.size() == 1 ? 1 : c.h);
-                c.bh = mcuy * (comp.size() == 1 ? 1 : c.v);
-                if (comp.size() == 1) {  // non-interleaved: blocks cover the component itself
-                    c.bw = (W * c.h / hmax + 7) / 8;
-                    c.bh = (H * c.v / vmax + 7) / 8;
-                }
-                c.plane.assign((size_t)c.bw * 8 * c.bh * 8, 0);
-                c.dc_pred = 0;
-            }
+            for (auto *c : sc) c->dc_pred = 0;
             Reader r(f);
             r.p = p;
-            float blk[64];
+            int eobrun = 0;  // blocks left in an end-of-band run (progressive AC scans)
+            const int p1 = 1 << al, m1 = -(1 << al);
+            // Correction bit of an already nonzero coefficient (G.1.2.3).
+            auto refine = [&](int16_t &c) {
+                if (r.bit() && (c & p1) == 0) c = (int16_t)(c >= 0 ? c + p1 : c + m1);
+            };
             auto decode_block = [&](Component &c, int bx, int by) -> bool {
-                int coef[64] = {0};
-                int s;
-                if (!r.decode(dc[c.td], s) || s > 11) return false;
-                c.dc_pred += extend(r.receive(s), s);
-                coef[0] = c.dc_pred;
-                for (int k = 1; k < 64;) {
-                    int rs;
-                    if (!r.decode(ac[c.ta], rs)) return false;
-                    const int run = rs >> 4, size = rs & 15;
-                    if (size == 0) {
-                        if (run != 15) break;  // EOB
-                        k += 16;
-                        continue;
+                int16_t *co = &c.coef[((size_t)by * c.bw + bx) * 64];
+                int sym;
+                if (ss == 0) {  // DC
+                    if (ah == 0) {
+                        if (!r.decode(dc[c.td], sym) || sym > 11) return false;
+                        c.dc_pred += extend(r.receive(sym), sym);
+                        co[0] = (int16_t)(c.dc_pred * (1 << al));
+                    } else if (r.bit()) {
+                        co[0] = (int16_t)(co[0] | p1);
                     }
-                    k += run;
-                    if (k > 63) return false;
-                    coef[kZigzag[k]] = extend(r.receive(size), size);
-                    ++k;
+                    if (se == 0) return true;
                 }
-                const uint16_t *q = qt[c.tq];
-                for (int k = 0; k < 64; ++k) blk[k] = (float)(coef[k] * (int)q[k]);
-                if (bx < c.bw && by < c.bh)
-                    idct8x8(blk, &c.plane[(size_t)by * 8 * c.bw * 8 + (size_t)bx * 8], c.bw * 8);
+                const int k0 = std::max(ss, 1);
+                if (!progressive || ah == 0) {  // AC band, first pass (or sequential)
+                    if (eobrun > 0) {
+                        --eobrun;
+                        return true;
+                    }
+                    for (int k = k0; k <= se;) {
+                        if (!r.decode(ac[c.ta], sym)) return false;
+                        const int run = sym >> 4, size = sym & 15;
+                        if (size == 0) {
+                            if (run != 15) {  // end of band, here or for the next 2^run + bits - 1 blocks
+                                if (progressive) eobrun = (1 << run) - 1 + (run ? r.receive(run) : 0);
+                                break;
+                            }
+                            k += 16;
+                            continue;
+                        }
+                        k += run;
+                        if (k > se) return false;
+                        co[kZigzag[k]] = (int16_t)(extend(r.receive(size), size) * (1 << al));
+                        ++k;
+                    }
+                    return true;
+                }
+                // AC band, refinement pass: new coefficients of magnitude 1 << al,
+                // each after `run` zero-history positions; every nonzero
+                // coefficient passed on the way receives a correction bit.
+                int k = k0;
+                if (eobrun == 0) {
+                    for (; k <= se; ++k) {
+                        if (!r.decode(ac[c.ta], sym)) return false;
+                        int run = sym >> 4, size = sym & 15, val = 0;
+                        if (size != 0) {
+                            if (size != 1) return false;
+                            val = r.bit() ? p1 : m1;
+                        } else if (run != 15) {
+                            eobrun = (1 << run) + (run ? r.receive(run) : 0);
+                            break;
+                        }
+                        for (; k <= se; ++k) {
+                            int16_t &cz = co[kZigzag[k]];
+                            if (cz != 0) refine(cz);
+                            else if (run-- == 0) break;
+                        }
+                        if (val != 0) {
+                            if (k > se) return false;
+                            co[kZigzag[k]] = (int16_t)val;
+                        }
+                    }
+                }
+                if (eobrun > 0) {  // the rest of the band: correction bits only
+                    for (; k <= se; ++k) {
+                        int16_t &cz = co[kZigzag[k]];
+                        if (cz != 0) refine(cz);
+                    }
+                    --eobrun;
+                }
                 return true;
             };
-            const int units = comp.size() == 1 ? comp[0].bw * comp[0].bh : mcux * mcuy;
+            const int mcux = (W + 8 * hmax - 1) / (8 * hmax), mcuy = (H + 8 * vmax - 1) / (8 * vmax);
+            const bool interleaved = ns > 1;
+            const int units = interleaved ? mcux * mcuy : sc[0]->cbw * sc[0]->cbh;
             for (int u = 0; u < units; ++u) {
                 if (restart_interval && u > 0 && u % restart_interval == 0) {
                     if (!r.restart()) {
                         err = "parse: JPEG restart marker missing";
                         return false;
                     }
-                    for (auto &c : comp) c.dc_pred = 0;
+                    for (auto *c : sc) c->dc_pred = 0;
+                    eobrun = 0;
                 }
                 bool ok = true;
-                if (comp.size() == 1) {
-                    ok = decode_block(comp[0], u % comp[0].bw, u / comp[0].bw);
+                if (!interleaved) {
+                    ok = decode_block(*sc[0], u % sc[0]->cbw, u / sc[0]->cbw);
                 } else {
                     const int mx = u % mcux, my = u / mcux;
-                    for (auto &c : comp)
-                        for (int yy = 0; yy < c.v && ok; ++yy)
-                            for (int xx = 0; xx < c.h && ok; ++xx)
-                                ok = decode_block(c, mx * c.h + xx, my * c.v + yy);
+                    for (auto *c : sc)
+                        for (int yy = 0; yy < c->v && ok; ++yy)
+                            for (int xx = 0; xx < c->h && ok; ++xx)
+                                ok = decode_block(*c, mx * c->h + xx, my * c->v + yy);
                 }
                 if (!ok) {
                     err = "parse: bad JPEG entropy-coded data";
@@ -367,14 +447,28 @@ bool load_jpeg(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, st
                 }
             }
             p = r.p;
-            done = true;
-            break;
+            ++scans;
+            continue;
         }
-        if (!done) p += len;
+        p += len;
     }
-    if (!done) {
+    if (!frame || scans == 0) {
         err = "parse: JPEG without image data";
         return false;
+    }
+    // Dequantize and transform every block of every component.
+    for (auto &c : comp) {
+        c.plane.assign((size_t)c.bw * 8 * c.bh * 8, 0);
+        const uint16_t *q = qt[c.tq];
+        float blk[64];
+        for (int by = 0; by < c.bh; ++by)
+            for (int bx = 0; bx < c.bw; ++bx) {
+                const int16_t *co = &c.coef[((size_t)by * c.bw + bx) * 64];
+                for (int k = 0; k < 64; ++k) blk[k] = (float)((int)co[k] * (int)q[k]);
+                idct8x8(blk, &c.plane[(size_t)by * 8 * c.bw * 8 + (size_t)bx * 8], c.bw * 8);
+            }
+        c.coef.clear();
+        c.coef.shrink_to_fit();
     }
     w = (uint32_t)W;
     h = (uint32_t)H;

@@ -353,7 +353,7 @@ def test_jpeg_against_pillow(tmp_path, mode, subsampling):
     assert d.max() <= (2 if subsampling == 0 else 4) and d.mean() <= (0.1 if subsampling == 0 else 0.4)
 
 
-def test_jpeg_restart_markers_and_progressive(tmp_path):
+def test_jpeg_restart_markers_and_lossless_rejected(tmp_path):
     from PIL import Image
     img = pil_image(4, "RGB", 80, 40)
     path = str(tmp_path / "r.jpg")
@@ -362,10 +362,40 @@ def test_jpeg_restart_markers_and_progressive(tmp_path):
     ref = np.asarray(Image.open(path).convert("RGB")).astype(np.int32)
     got = to8(pt.load_image(path)[..., :3])
     assert np.abs(got - ref).max() <= 6
-    img.save(str(tmp_path / "p.jpg"), quality=90, progressive=True)
+    # SOF3 (lossless) is not read: the baseline file with its frame marker changed
+    data = bytearray(open(path, "rb").read())
+    at = data.find(b"\xff\xc0")
+    data[at + 1] = 0xC3
     with pytest.raises(pt.PupilError) as e:
-        pt.load_image(str(tmp_path / "p.jpg"))
+        pt.load_image(write(tmp_path, "l.jpg", bytes(data)))
     assert e.value.code == -8
+
+
+@pytest.mark.parametrize("mode,subsampling,restart", [("L", 0, 0), ("RGB", 0, 0), ("RGB", 2, 0), ("RGB", 1, 0),
+                                                      ("RGB", 2, 2), ("L", 0, 5)])
+def test_progressive_jpeg(tmp_path, mode, subsampling, restart):
+    """Progressive JPEG (libjpeg's default script: DC first + refinement scans,
+    spectral bands, successive approximation) re-encodes the same quantized
+    coefficients as the baseline file Pillow writes with the same settings,
+    so our decode of the progressive file must equal our decode of the
+    baseline file exactly; against libjpeg's own decode the usual IDCT /
+    upsampling rounding tolerance applies."""
+    from PIL import Image
+    img = pil_image(7, mode, 77, 53)
+    kw = {} if mode == "L" else {"subsampling": subsampling}
+    if restart:
+        kw["restart_marker_blocks"] = restart
+    base, prog = str(tmp_path / "b.jpg"), str(tmp_path / "p.jpg")
+    img.save(base, quality=88, **kw)
+    img.save(prog, quality=88, progressive=True, **kw)
+    data = open(prog, "rb").read()
+    assert b"\xff\xc2" in data and data.count(b"\xff\xda") > 3  # SOF2, several scans
+    got_p = pt.load_image(prog)
+    got_b = pt.load_image(base)
+    assert np.array_equal(got_p, got_b)
+    ref = np.asarray(Image.open(prog).convert("RGB")).astype(np.int32)
+    d = np.abs(to8(got_p[..., :3]) - ref)
+    assert d.max() <= (2 if subsampling == 0 else 4) and d.mean() <= (0.1 if subsampling == 0 else 0.4)
 
 
 def test_image_errors(tmp_path):
