@@ -8,7 +8,7 @@
 //   BMP      uncompressed palette / 16 / 24 / 32-bit, bit fields;
 //   TGA      color-mapped, true-color and gray, raw or run-length encoded;
 //   PNM      binary P5 / P6;
-//   JPEG     baseline and extended sequential Huffman (pt_jpeg.cpp).
+//   JPEG     baseline, extended sequential and progressive Huffman (pt_jpeg.cpp).
 // Output: RGBA float, row 0 = the first (top) row of the file, as both
 // loaders of the reference return it.  8-bit data is linearized with
 // (v / 255)^2.2 as in util/texture.cpp:106-111 (16-bit PNG channels keep their
