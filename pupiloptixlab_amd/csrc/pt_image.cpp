@@ -4,7 +4,7 @@
 //   OpenEXR  scanline files, compression NONE / RLE / ZIPS / ZIP, pixel types
 //            UINT / HALF / FLOAT (channels R, G, B, A or Y);
 //   Radiance RGBE (.hdr / .pic), flat and new run-length scanlines;
-//   PNG      8- and 16-bit (and 1/2/4-bit gray / palette), non-interlaced;
+//   PNG      8- and 16-bit (and 1/2/4-bit gray / palette), plain or Adam7-interlaced;
 //   BMP      uncompressed palette / 16 / 24 / 32-bit, bit fields;
 //   TGA      color-mapped, true-color and gray, raw or run-length encoded;
 //   PNM      binary P5 / P6;
@@ -445,8 +445,8 @@ bool load_png(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
         err = "parse: PNG without a valid IHDR";
         return false;
     }
-    if (interlace != 0) {
-        err = "unsupported: interlaced PNG";
+    if (interlace > 1) {
+        err = "parse: bad PNG interlace method";
         return false;
     }
     int chans;
@@ -466,81 +466,105 @@ bool load_png(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
         err = "parse: bad PNG bit depth or palette";
         return false;
     }
+    // Passes: one for a plain image; Adam7 (PNG spec 8.2) splits the image
+    // into seven sub-images (x0, y0, dx, dy), each filtered on its own, stored
+    // one after the other; an empty pass has no bytes at all.
+    struct Pass { uint32_t x0, y0, dx, dy, w, h; };
+    static const uint32_t adam7[7][4] = {{0, 0, 8, 8}, {4, 0, 8, 8}, {0, 4, 4, 8}, {2, 0, 4, 4},
+                                         {0, 2, 2, 4}, {1, 0, 2, 2}, {0, 1, 1, 2}};
+    std::vector<Pass> passes;
+    for (int k = 0; k < (interlace ? 7 : 1); ++k) {
+        Pass ps{0, 0, 1, 1, W, H};
+        if (interlace) {
+            ps = Pass{adam7[k][0], adam7[k][1], adam7[k][2], adam7[k][3], 0, 0};
+            ps.w = W > ps.x0 ? (W - ps.x0 + ps.dx - 1) / ps.dx : 0;
+            ps.h = H > ps.y0 ? (H - ps.y0 + ps.dy - 1) / ps.dy : 0;
+        }
+        if (ps.w && ps.h) passes.push_back(ps);
+    }
     const size_t bits_pp = (size_t)chans * depth;
-    const size_t row_bytes = ((size_t)W * bits_pp + 7) / 8;
     const size_t bpp = std::max<size_t>(1, bits_pp / 8);
-    std::vector<unsigned char> raw((row_bytes + 1) * H);
+    auto row_bytes_of = [&](uint32_t pw) { return ((size_t)pw * bits_pp + 7) / 8; };
+    size_t total = 0;
+    for (const Pass &ps : passes) total += (row_bytes_of(ps.w) + 1) * ps.h;
+    std::vector<unsigned char> raw(total);
     uLongf dl = (uLongf)raw.size();
     if (idat.empty() || uncompress(raw.data(), &dl, idat.data(), (uLong)idat.size()) != Z_OK || dl != raw.size()) {
         err = "parse: bad PNG image data";
         return false;
     }
-    std::vector<unsigned char> prev(row_bytes, 0), cur(row_bytes);
     w = W;
     h = H;
     rgba.assign((size_t)W * H * 4, 0.f);
     float lut[256];
     for (int i = 0; i < 256; ++i) lut[i] = std::pow(i * 1.f / 255.f, 2.2f);
-    for (uint32_t y = 0; y < H; ++y) {
-        const unsigned char *src = &raw[(row_bytes + 1) * y];
-        const int ft = src[0];
-        for (size_t i = 0; i < row_bytes; ++i) {
-            const int a = i >= bpp ? cur[i - bpp] : 0, b = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
-            int v = src[1 + i];
-            switch (ft) {
-                case 0: break;
-                case 1: v += a; break;
-                case 2: v += b; break;
-                case 3: v += (a + b) / 2; break;
-                case 4: {
-                    const int pp = a + b - c, pa = std::abs(pp - a), pb = std::abs(pp - b), pc = std::abs(pp - c);
-                    v += (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
-                } break;
-                default:
-                    err = "parse: bad PNG filter";
-                    return false;
-            }
-            cur[i] = (unsigned char)v;
-        }
-        float *row = &rgba[(size_t)y * W * 4];
-        for (uint32_t x = 0; x < W; ++x) {
-            unsigned char px[4] = {0, 0, 0, 255};
-            if (depth < 8) {
-                const size_t bit = (size_t)x * depth;
-                const int v = (cur[bit / 8] >> (8 - depth - (int)(bit % 8))) & ((1 << depth) - 1);
-                if (ctype == 3) {
-                    if ((size_t)v * 3 + 2 >= plte.size()) {
-                        err = "parse: PNG palette index out of range";
+    size_t off = 0;
+    for (const Pass &ps : passes) {
+        const size_t row_bytes = row_bytes_of(ps.w);
+        std::vector<unsigned char> prev(row_bytes, 0), cur(row_bytes);
+        for (uint32_t y = 0; y < ps.h; ++y) {
+            const unsigned char *src = &raw[off + (row_bytes + 1) * y];
+            const int ft = src[0];
+            for (size_t i = 0; i < row_bytes; ++i) {
+                const int a = i >= bpp ? cur[i - bpp] : 0, b = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
+                int v = src[1 + i];
+                switch (ft) {
+                    case 0: break;
+                    case 1: v += a; break;
+                    case 2: v += b; break;
+                    case 3: v += (a + b) / 2; break;
+                    case 4: {
+                        const int pp = a + b - c, pa = std::abs(pp - a), pb = std::abs(pp - b), pc = std::abs(pp - c);
+                        v += (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
+                    } break;
+                    default:
+                        err = "parse: bad PNG filter";
                         return false;
-                    }
-                    for (int k = 0; k < 3; ++k) px[k] = plte[3 * v + k];
-                } else {
-                    const int scale = depth == 1 ? 255 : (depth == 2 ? 0x55 : 0x11);
-                    px[0] = px[1] = px[2] = (unsigned char)(v * scale);
                 }
-            } else {
-                const size_t step = depth / 8;
-                unsigned char s[4];
-                for (int k = 0; k < chans; ++k) s[k] = cur[((size_t)x * chans + k) * step];  // high byte
-                if (ctype == 3) {
-                    if ((size_t)s[0] * 3 + 2 >= plte.size()) {
-                        err = "parse: PNG palette index out of range";
-                        return false;
-                    }
-                    for (int k = 0; k < 3; ++k) px[k] = plte[3 * (size_t)s[0] + k];
-                } else if (chans <= 2) {
-                    px[0] = px[1] = px[2] = s[0];
-                    if (chans == 2) px[3] = s[1];
-                } else {
-                    for (int k = 0; k < chans; ++k) px[k] = s[k];
-                }
+                cur[i] = (unsigned char)v;
             }
-            row[4 * x + 0] = lut[px[0]];
-            row[4 * x + 1] = lut[px[1]];
-            row[4 * x + 2] = lut[px[2]];
-            row[4 * x + 3] = px[3] * 1.f / 255.f;
+            float *row = &rgba[(size_t)(ps.y0 + y * ps.dy) * W * 4];
+            for (uint32_t x = 0; x < ps.w; ++x) {
+                unsigned char px[4] = {0, 0, 0, 255};
+                if (depth < 8) {
+                    const size_t bit = (size_t)x * depth;
+                    const int v = (cur[bit / 8] >> (8 - depth - (int)(bit % 8))) & ((1 << depth) - 1);
+                    if (ctype == 3) {
+                        if ((size_t)v * 3 + 2 >= plte.size()) {
+                            err = "parse: PNG palette index out of range";
+                            return false;
+                        }
+                        for (int k = 0; k < 3; ++k) px[k] = plte[3 * v + k];
+                    } else {
+                        const int scale = depth == 1 ? 255 : (depth == 2 ? 0x55 : 0x11);
+                        px[0] = px[1] = px[2] = (unsigned char)(v * scale);
+                    }
+                } else {
+                    const size_t step = depth / 8;
+                    unsigned char sm[4];
+                    for (int k = 0; k < chans; ++k) sm[k] = cur[((size_t)x * chans + k) * step];  // high byte
+                    if (ctype == 3) {
+                        if ((size_t)sm[0] * 3 + 2 >= plte.size()) {
+                            err = "parse: PNG palette index out of range";
+                            return false;
+                        }
+                        for (int k = 0; k < 3; ++k) px[k] = plte[3 * (size_t)sm[0] + k];
+                    } else if (chans <= 2) {
+                        px[0] = px[1] = px[2] = sm[0];
+                        if (chans == 2) px[3] = sm[1];
+                    } else {
+                        for (int k = 0; k < chans; ++k) px[k] = sm[k];
+                    }
+                }
+                float *o = row + 4 * (size_t)(ps.x0 + x * ps.dx);
+                o[0] = lut[px[0]];
+                o[1] = lut[px[1]];
+                o[2] = lut[px[2]];
+                o[3] = px[3] * 1.f / 255.f;
+            }
+            prev.swap(cur);
         }
-        prev.swap(cur);
+        off += (row_bytes + 1) * ps.h;
     }
     return true;
 }

@@ -22,46 +22,65 @@ import pupiloptixlab_amd as pt
 
 
 # ---------------------------------------------------------------- encoders
-def png_bytes(pixels, color_type, depth=8, palette=None, filters=(0, 1, 2, 3, 4)):
-    """pixels: (h, w, c) uint array in the PNG sample layout."""
+ADAM7 = ((0, 0, 8, 8), (4, 0, 8, 8), (0, 4, 4, 8), (2, 0, 4, 4), (0, 2, 2, 4), (1, 0, 2, 2), (0, 1, 1, 2))
+
+
+def png_bytes(pixels, color_type, depth=8, palette=None, filters=(0, 1, 2, 3, 4), interlace=False):
+    """pixels: (h, w, c) uint array in the PNG sample layout.  With
+    interlace, the seven Adam7 sub-images are filtered and stored in turn
+    (empty ones are left out)."""
     h, w = pixels.shape[:2]
-    raw = bytearray()
     dt = ">u2" if depth == 16 else "u1"
     bpp = max(1, pixels.shape[2] * depth // 8) if pixels.ndim == 3 else 1
-    prev = bytes((w * bpp) if depth >= 8 else ((w * depth + 7) // 8))
-    for y in range(h):
-        if depth >= 8:
-            row = pixels[y].astype(dt).tobytes()
-        else:  # packed gray / palette indices
-            bits = "".join(format(int(v), f"0{depth}b") for v in pixels[y].reshape(-1))
-            bits += "0" * (-len(bits) % 8)
-            row = bytes(int(bits[i:i + 8], 2) for i in range(0, len(bits), 8))
-        ft = filters[y % len(filters)]
-        out = bytearray()
-        for i, v in enumerate(row):
-            a = row[i - bpp] if i >= bpp else 0
-            b = prev[i]
-            c = prev[i - bpp] if i >= bpp else 0
-            if ft == 0:
-                pred = 0
-            elif ft == 1:
-                pred = a
-            elif ft == 2:
-                pred = b
-            elif ft == 3:
-                pred = (a + b) // 2
-            else:
-                p = a + b - c
-                pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
-                pred = a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
-            out.append((v - pred) & 0xFF)
-        raw += bytes([ft]) + out
-        prev = row
+
+    def filtered(pix):
+        raw = bytearray()
+        ph, pw = pix.shape[:2]
+        prev = bytes((pw * bpp) if depth >= 8 else ((pw * depth + 7) // 8))
+        for y in range(ph):
+            if depth >= 8:
+                row = pix[y].astype(dt).tobytes()
+            else:  # packed gray / palette indices
+                bits = "".join(format(int(v), f"0{depth}b") for v in pix[y].reshape(-1))
+                bits += "0" * (-len(bits) % 8)
+                row = bytes(int(bits[i:i + 8], 2) for i in range(0, len(bits), 8))
+            ft = filters[y % len(filters)]
+            out = bytearray()
+            for i, v in enumerate(row):
+                a = row[i - bpp] if i >= bpp else 0
+                b = prev[i]
+                c = prev[i - bpp] if i >= bpp else 0
+                if ft == 0:
+                    pred = 0
+                elif ft == 1:
+                    pred = a
+                elif ft == 2:
+                    pred = b
+                elif ft == 3:
+                    pred = (a + b) // 2
+                else:
+                    p = a + b - c
+                    pa, pb, pc = abs(p - a), abs(p - b), abs(p - c)
+                    pred = a if (pa <= pb and pa <= pc) else (b if pb <= pc else c)
+                out.append((v - pred) & 0xFF)
+            raw += bytes([ft]) + out
+            prev = row
+        return raw
+
+    if interlace:
+        raw = bytearray()
+        for x0, y0, dx, dy in ADAM7:
+            sub = pixels[y0::dy, x0::dx]
+            if sub.shape[0] and sub.shape[1]:
+                raw += filtered(sub)
+    else:
+        raw = filtered(pixels)
 
     def chunk(t, d):
         return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
 
-    data = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, color_type, 0, 0, 0))
+    data = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, depth, color_type, 0, 0,
+                                                                 1 if interlace else 0))
     if palette is not None:
         data += chunk(b"PLTE", bytes(np.asarray(palette, np.uint8).reshape(-1)))
     return data + chunk(b"IDAT", zlib.compress(bytes(raw))) + chunk(b"IEND", b"")
@@ -205,6 +224,45 @@ def test_png_palette_16bit_and_low_depth(tmp_path):
     px16 = np.random.default_rng(9).integers(0, 65536, (3, 4, 3)).astype(np.uint16)
     img = pt.load_image(write(tmp_path, "s.png", png_bytes(px16, 2, 16)))
     assert np.allclose(img[..., :3], lin8(px16 >> 8), atol=1e-7)  # high byte, as an 8-bit loader keeps it
+
+
+@pytest.mark.parametrize("h,w", [(1, 1), (3, 2), (9, 13), (16, 17)])
+@pytest.mark.parametrize("kind", ["rgb8", "rgba16", "gray1", "gray4", "pal2", "ga8"])
+def test_png_adam7_interlaced(tmp_path, kind, h, w):
+    """Adam7-interlaced files (including sizes where some of the seven passes
+    are empty) decode to the same image as the plain file of the same pixels,
+    and to the expected values."""
+    rng = np.random.default_rng(h * 31 + w)
+    pal = rng.integers(0, 256, (4, 3)).astype(np.uint8)
+    ctype, depth, px = {
+        "rgb8": (2, 8, lambda: rng.integers(0, 256, (h, w, 3)).astype(np.uint8)),
+        "rgba16": (6, 16, lambda: rng.integers(0, 65536, (h, w, 4)).astype(np.uint16)),
+        "gray1": (0, 1, lambda: rng.integers(0, 2, (h, w)).astype(np.uint8)),
+        "gray4": (0, 4, lambda: rng.integers(0, 16, (h, w)).astype(np.uint8)),
+        "pal2": (3, 2, lambda: rng.integers(0, 4, (h, w)).astype(np.uint8)),
+        "ga8": (4, 8, lambda: rng.integers(0, 256, (h, w, 2)).astype(np.uint8)),
+    }[kind]
+    px = px()
+    palette = pal if ctype == 3 else None
+    plain = pt.load_image(write(tmp_path, "a.png", png_bytes(px, ctype, depth, palette)))
+    inter = pt.load_image(write(tmp_path, "i.png", png_bytes(px, ctype, depth, palette, interlace=True)))
+    assert inter.shape == (h, w, 4) and np.array_equal(inter, plain)
+    if kind == "rgb8":
+        from PIL import Image  # an independent reader of the interlaced test file
+        assert np.array_equal(np.asarray(Image.open(str(tmp_path / "i.png")).convert("RGB")), px)
+        assert np.allclose(inter[..., :3], lin8(px), atol=1e-7)
+    elif kind == "rgba16":
+        assert np.allclose(inter[..., :3], lin8(px[..., :3] >> 8), atol=1e-7)
+        assert np.allclose(inter[..., 3], (px[..., 3] >> 8) / 255.0, atol=1e-7)
+    elif kind == "gray1":
+        assert np.allclose(inter[..., 0], lin8(px * 255), atol=1e-7)
+    elif kind == "gray4":
+        assert np.allclose(inter[..., 1], lin8(px * 17), atol=1e-7)
+    elif kind == "pal2":
+        assert np.allclose(inter[..., :3], lin8(pal[px]), atol=1e-7)
+    else:
+        assert np.allclose(inter[..., 2], lin8(px[..., 0]), atol=1e-7)
+        assert np.allclose(inter[..., 3], px[..., 1] / 255.0, atol=1e-7)
 
 
 @pytest.mark.parametrize("rle", [False, True])
