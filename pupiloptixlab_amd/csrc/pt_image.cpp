@@ -8,7 +8,8 @@
 //   BMP      uncompressed palette / 16 / 24 / 32-bit, bit fields;
 //   TGA      color-mapped, true-color and gray, raw or run-length encoded;
 //   PNM      binary P5 / P6;
-//   JPEG     baseline, extended sequential and progressive Huffman (pt_jpeg.cpp).
+//   JPEG     baseline, extended sequential and progressive Huffman (pt_jpeg.cpp);
+//   GIF      the first image (87a / 89a), transparency, interlacing.
 // Output: RGBA float, row 0 = the first (top) row of the file, as both
 // loaders of the reference return it.  8-bit data is linearized with
 // (v / 255)^2.2 as in util/texture.cpp:106-111 (16-bit PNG channels keep their
@@ -589,6 +590,183 @@ void rgba_from_8bit(const std::vector<unsigned char> &px, int chans, uint32_t W,
 uint32_t le16(const unsigned char *p) { return (uint32_t)p[0] | (uint32_t)p[1] << 8; }
 uint32_t le32(const unsigned char *p) { return le16(p) | le16(p + 2) << 16; }
 
+// ------------------------------------------------------------------ GIF
+// First image of a GIF87a / GIF89a file, as stb_image returns it: a canvas of
+// the logical screen size, transparent black where the image does not cover
+// it or where the graphic control extension marks the transparent index;
+// colors from the local table, else the global one.  LZW (variable code
+// width from min code size + 1 up to 12 bits, clear and end codes, the table
+// kept full once 4096 codes exist) and the four-pass row order of an
+// interlaced image follow the GIF89a specification.
+bool load_gif(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std::vector<unsigned char> &px,
+              std::string &err) {
+    if (f.size() < 13 || (std::memcmp(f.data(), "GIF87a", 6) != 0 && std::memcmp(f.data(), "GIF89a", 6) != 0)) {
+        err = "parse: not a GIF file";
+        return false;
+    }
+    const uint32_t W = f[6] | f[7] << 8, H = f[8] | f[9] << 8;
+    if (W == 0 || H == 0) {
+        err = "parse: GIF with an empty screen";
+        return false;
+    }
+    size_t p = 13;
+    std::vector<unsigned char> gct;
+    if (f[10] & 0x80) {
+        const size_t n = (size_t)3 << ((f[10] & 7) + 1);
+        if (p + n > f.size()) {
+            err = "parse: truncated GIF color table";
+            return false;
+        }
+        gct.assign(f.begin() + (std::ptrdiff_t)p, f.begin() + (std::ptrdiff_t)(p + n));
+        p += n;
+    }
+    int transparent = -1;
+    auto skip_sub_blocks = [&]() -> bool {
+        while (p < f.size()) {
+            const size_t n = f[p++];
+            if (n == 0) return true;
+            p += n;
+        }
+        return false;
+    };
+    while (p < f.size()) {
+        const unsigned char b = f[p++];
+        if (b == 0x3B) break;  // trailer
+        if (b == 0x21) {       // extension
+            if (p >= f.size()) break;
+            const unsigned char label = f[p++];
+            if (label == 0xF9 && p + 5 <= f.size() && f[p] >= 4) {  // graphic control
+                transparent = (f[p + 1] & 1) ? f[p + 4] : -1;
+            }
+            if (!skip_sub_blocks()) break;
+            continue;
+        }
+        if (b != 0x2C) {
+            err = "parse: bad GIF block";
+            return false;
+        }
+        if (p + 9 > f.size()) break;
+        const uint32_t ix = f[p] | f[p + 1] << 8, iy = f[p + 2] | f[p + 3] << 8;
+        const uint32_t iw = f[p + 4] | f[p + 5] << 8, ih = f[p + 6] | f[p + 7] << 8;
+        const unsigned char flags = f[p + 8];
+        p += 9;
+        std::vector<unsigned char> lct;
+        if (flags & 0x80) {
+            const size_t n = (size_t)3 << ((flags & 7) + 1);
+            if (p + n > f.size()) break;
+            lct.assign(f.begin() + (std::ptrdiff_t)p, f.begin() + (std::ptrdiff_t)(p + n));
+            p += n;
+        }
+        const std::vector<unsigned char> &ct = lct.empty() ? gct : lct;
+        if (ct.empty() || p >= f.size()) {
+            err = "parse: GIF image without a color table";
+            return false;
+        }
+        const int min_size = f[p++];
+        if (min_size < 2 || min_size > 11) {
+            err = "parse: bad GIF LZW code size";
+            return false;
+        }
+        // Data sub-blocks joined into one stream.
+        std::vector<unsigned char> data;
+        while (p < f.size()) {
+            const size_t n = f[p++];
+            if (n == 0) break;
+            if (p + n > f.size()) {
+                err = "parse: truncated GIF image data";
+                return false;
+            }
+            data.insert(data.end(), f.begin() + (std::ptrdiff_t)p, f.begin() + (std::ptrdiff_t)(p + n));
+            p += n;
+        }
+        // LZW: code k < clear is the literal k; each new code is an earlier
+        // code's string plus one byte.
+        const int clear = 1 << min_size, end = clear + 1;
+        std::vector<uint16_t> prefix(4096);
+        std::vector<unsigned char> suffix(4096), first(4096), stack;
+        for (int k = 0; k < clear; ++k) {
+            suffix[(size_t)k] = (unsigned char)k;
+            first[(size_t)k] = (unsigned char)k;
+        }
+        std::vector<unsigned char> idx;
+        idx.reserve((size_t)iw * ih);
+        int width = min_size + 1, next = end + 1, prev = -1;
+        uint32_t acc = 0;
+        int nacc = 0;
+        size_t q = 0;
+        while (idx.size() < (size_t)iw * ih) {
+            while (nacc < width && q < data.size()) {
+                acc |= (uint32_t)data[q++] << nacc;
+                nacc += 8;
+            }
+            if (nacc < width) break;  // data ended early: the rest stays uncovered
+            const int code = (int)(acc & ((1u << width) - 1u));
+            acc >>= width;
+            nacc -= width;
+            if (code == clear) {
+                width = min_size + 1;
+                next = end + 1;
+                prev = -1;
+                continue;
+            }
+            if (code == end) break;
+            int c = code;
+            if (code > next || (code == next && prev < 0)) {
+                err = "parse: bad GIF LZW code";
+                return false;
+            }
+            stack.clear();
+            if (code == next) {  // the string being defined: prev's string + its first byte
+                stack.push_back(first[(size_t)prev]);
+                c = prev;
+            }
+            while (c >= clear) {
+                stack.push_back(suffix[(size_t)c]);
+                c = prefix[(size_t)c];
+            }
+            stack.push_back((unsigned char)c);
+            for (auto it = stack.rbegin(); it != stack.rend() && idx.size() < (size_t)iw * ih; ++it) idx.push_back(*it);
+            if (prev >= 0 && next < 4096) {
+                prefix[(size_t)next] = (uint16_t)prev;
+                suffix[(size_t)next] = (unsigned char)c;  // first byte of this code's string
+                first[(size_t)next] = first[(size_t)prev];
+                ++next;
+                if (next == (1 << width) && width < 12) ++width;
+            }
+            prev = code;
+        }
+        // Rows in file order: top to bottom, or the four interlace passes.
+        std::vector<uint32_t> rows;
+        if (flags & 0x40) {
+            static const uint32_t start[4] = {0, 4, 2, 1}, step[4] = {8, 8, 4, 2};
+            for (int ps = 0; ps < 4; ++ps)
+                for (uint32_t y = start[ps]; y < ih; y += step[ps]) rows.push_back(y);
+        } else {
+            for (uint32_t y = 0; y < ih; ++y) rows.push_back(y);
+        }
+        w = W;
+        h = H;
+        px.assign((size_t)W * H * 4, 0);
+        for (size_t r = 0; r < rows.size(); ++r) {
+            const uint32_t y = iy + rows[r];
+            for (uint32_t x = 0; x < iw; ++x) {
+                const size_t k = r * iw + x;
+                if (k >= idx.size()) break;
+                const int v = idx[k];
+                if (v == transparent || y >= H || ix + x >= W || (size_t)v * 3 + 2 >= ct.size()) continue;
+                unsigned char *o = &px[((size_t)y * W + ix + x) * 4];
+                o[0] = ct[(size_t)v * 3];
+                o[1] = ct[(size_t)v * 3 + 1];
+                o[2] = ct[(size_t)v * 3 + 2];
+                o[3] = 255;
+            }
+        }
+        return true;
+    }
+    err = "parse: GIF without an image";
+    return false;
+}
+
 // ------------------------------------------------------------------ BMP
 // Windows bitmap: BITMAPINFOHEADER and later headers, uncompressed 1/4/8-bit
 // palette and 24/32-bit pixels, 16/32-bit BI_BITFIELDS; bottom-up or top-down.
@@ -864,13 +1042,17 @@ bool load_image(const std::string &path, uint32_t &w, uint32_t &h, std::vector<f
     else if (e == "bmp") ok = load_bmp(f, w, h, rgba, err);
     else if (e == "tga") ok = load_tga(f, w, h, rgba, err);
     else if (e == "ppm" || e == "pgm" || e == "pnm") ok = load_pnm(f, w, h, rgba, err);
-    else if (e == "jpg" || e == "jpeg") {
+    else if (e == "gif") {
+        std::vector<unsigned char> px;
+        ok = load_gif(f, w, h, px, err);
+        if (ok) rgba_from_8bit(px, 4, w, h, rgba);
+    } else if (e == "jpg" || e == "jpeg") {
         std::vector<unsigned char> px;
         int chans = 0;
         ok = load_jpeg(f, w, h, px, chans, err);
         if (ok) rgba_from_8bit(px, chans, w, h, rgba);
     } else {
-        err = "unsupported: image format ." + e + " (EXR, HDR, PNG, JPEG, BMP, TGA and PNM are read)";
+        err = "unsupported: image format ." + e + " (EXR, HDR, PNG, JPEG, GIF, BMP, TGA and PNM are read)";
         return false;
     }
     if (!ok && err.find(path) == std::string::npos) err += " (" + path + ")";

@@ -456,10 +456,48 @@ def test_progressive_jpeg(tmp_path, mode, subsampling, restart):
     assert d.max() <= (2 if subsampling == 0 else 4) and d.mean() <= (0.1 if subsampling == 0 else 0.4)
 
 
+@pytest.mark.parametrize("interlace,transparent,size", [(False, None, (37, 23)), (True, None, (37, 23)),
+                                                        (True, 5, (64, 48)), (False, 0, (300, 200))])
+def test_gif_first_image(tmp_path, interlace, transparent, size):
+    """GIF written by Pillow: colors equal Pillow's decode exactly (alpha 0 on
+    the transparent index, as stb_image's canvas); the 300x200 low-entropy
+    image fills the 4096-entry LZW table, so codes of every width and the
+    full-table case are decoded."""
+    from PIL import Image
+    rng = np.random.default_rng(size[0] + (transparent or 0))
+    w, h = size
+    if w > 100:  # smooth bands: long repeats, many table entries
+        idx = ((np.arange(w)[None, :] // 3 + np.arange(h)[:, None] // 5) % 64).astype(np.uint8)
+        idx[rng.random((h, w)) < 0.02] = rng.integers(0, 64)
+    else:
+        idx = rng.integers(0, 64, (h, w)).astype(np.uint8)
+    pal = rng.integers(0, 256, (64, 3)).astype(np.uint8)
+    img = Image.fromarray(idx, "P")
+    img.putpalette(pal.reshape(-1).tolist())
+    path = str(tmp_path / "a.gif")
+    kw = {"interlace": interlace}
+    if transparent is not None:
+        kw["transparency"] = transparent
+    img.save(path, **kw)
+    ref = np.asarray(Image.open(path).convert("RGBA"))
+    got = pt.load_image(path)
+    assert got.shape == (h, w, 4)
+    alpha = np.round(got[..., 3] * 255).astype(np.int32)
+    assert np.array_equal(alpha, ref[..., 3].astype(np.int32))
+    opaque = ref[..., 3] == 255
+    assert np.array_equal(to8(got[..., :3])[opaque], ref[..., :3][opaque])
+    assert np.all(got[~opaque] == 0)
+    if transparent is not None:
+        assert (~opaque).any() and np.array_equal(~opaque, idx == transparent)
+
+
 def test_image_errors(tmp_path):
     with pytest.raises(pt.PupilError) as e:
-        pt.load_image(write(tmp_path, "x.gif", b"GIF89a"))
+        pt.load_image(write(tmp_path, "x.psd", b"8BPS"))
     assert e.value.code == -8
+    with pytest.raises(pt.PupilError) as e:
+        pt.load_image(write(tmp_path, "x.gif", b"GIF89a"))
+    assert e.value.code == -6
     with pytest.raises(pt.PupilError) as e:
         pt.load_image(write(tmp_path, "x.jpg", b"\xff\xd8\xff"))
     assert e.value.code == -6
