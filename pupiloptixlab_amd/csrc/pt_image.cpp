@@ -9,7 +9,8 @@
 //   TGA      color-mapped, true-color and gray, raw or run-length encoded;
 //   PNM      binary P5 / P6;
 //   JPEG     baseline, extended sequential and progressive Huffman (pt_jpeg.cpp);
-//   GIF      the first image (87a / 89a), transparency, interlacing.
+//   GIF      the first image (87a / 89a), transparency, interlacing;
+//   PSD      the composite image, 8/16-bit RGB, raw or PackBits.
 // Output: RGBA float, row 0 = the first (top) row of the file, as both
 // loaders of the reference return it.  8-bit data is linearized with
 // (v / 255)^2.2 as in util/texture.cpp:106-111 (16-bit PNG channels keep their
@@ -590,6 +591,103 @@ void rgba_from_8bit(const std::vector<unsigned char> &px, int chans, uint32_t W,
 uint32_t le16(const unsigned char *p) { return (uint32_t)p[0] | (uint32_t)p[1] << 8; }
 uint32_t le32(const unsigned char *p) { return le16(p) | le16(p + 2) << 16; }
 
+// ------------------------------------------------------------------ PSD
+// The composite (merged) image of a Photoshop file, as stb_image reads it:
+// RGB color mode, 8 or 16 bits per channel (16-bit samples keep their high
+// byte), raw or PackBits (RLE) compressed, channel-planar; a fourth channel
+// is alpha, a file with fewer than four channels is opaque.  Layers, masks
+// and image resources are skipped.
+bool load_psd(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std::vector<unsigned char> &px,
+              std::string &err) {
+    auto u16 = [&](size_t at) { return (uint32_t)f[at] << 8 | f[at + 1]; };
+    auto u32 = [&](size_t at) { return u16(at) << 16 | u16(at + 2); };
+    if (f.size() < 26 || std::memcmp(f.data(), "8BPS", 4) != 0 || u16(4) != 1) {
+        err = "parse: not a PSD file";
+        return false;
+    }
+    const uint32_t nch = u16(12), H = u32(14), W = u32(18), depth = u16(22), mode = u16(24);
+    if (nch < 1 || nch > 56 || W == 0 || H == 0 || W > 30000 || H > 30000) {
+        err = "parse: bad PSD header";
+        return false;
+    }
+    if (mode != 3 || (depth != 8 && depth != 16)) {
+        err = "unsupported: PSD color mode or depth (8/16-bit RGB is read)";
+        return false;
+    }
+    size_t p = 26;
+    for (int section = 0; section < 3; ++section) {  // color mode data, image resources, layers and masks
+        if (p + 4 > f.size()) {
+            err = "parse: truncated PSD file";
+            return false;
+        }
+        p += 4 + (size_t)u32(p);
+    }
+    if (p + 2 > f.size()) {
+        err = "parse: truncated PSD file";
+        return false;
+    }
+    const uint32_t comp = u16(p);
+    p += 2;
+    if (comp > 1) {
+        err = "unsupported: PSD compression (raw and PackBits are read)";
+        return false;
+    }
+    const size_t bps = depth / 8, row = (size_t)W * bps, plane = row * H;
+    const uint32_t used = std::min<uint32_t>(nch, 4);
+    std::vector<unsigned char> planes((size_t)used * plane);
+    if (comp == 0) {
+        if (p + (size_t)used * plane > f.size()) {
+            err = "parse: truncated PSD image data";
+            return false;
+        }
+        std::memcpy(planes.data(), &f[p], planes.size());
+    } else {
+        // Byte counts of every row of every channel, then the PackBits rows.
+        size_t q = p + (size_t)2 * nch * H;
+        if (q > f.size()) {
+            err = "parse: truncated PSD image data";
+            return false;
+        }
+        for (uint32_t c = 0; c < used; ++c)
+            for (uint32_t y = 0; y < H; ++y) {
+                const size_t n = u16(p + 2 * ((size_t)c * H + y)), stop = q + n;
+                if (stop > f.size()) {
+                    err = "parse: truncated PSD image data";
+                    return false;
+                }
+                unsigned char *dst = &planes[(size_t)c * plane + (size_t)y * row];
+                size_t o = 0;
+                while (q < stop && o < row) {
+                    const int hdr = (signed char)f[q++];
+                    if (hdr >= 0) {  // hdr + 1 literal bytes
+                        const size_t m = std::min<size_t>((size_t)hdr + 1, std::min(row - o, stop - q));
+                        std::memcpy(dst + o, &f[q], m);
+                        o += m;
+                        q += (size_t)hdr + 1;
+                    } else if (hdr != -128) {  // one byte repeated 1 - hdr times
+                        if (q >= stop) break;
+                        const size_t m = std::min<size_t>((size_t)(1 - hdr), row - o);
+                        std::memset(dst + o, f[q++], m);
+                        o += m;
+                    }
+                }
+                if (o != row) {
+                    err = "parse: bad PSD PackBits row";
+                    return false;
+                }
+                q = stop;
+            }
+    }
+    w = W;
+    h = H;
+    px.assign((size_t)W * H * 4, 255);
+    for (uint32_t c = 0; c < used; ++c)
+        for (size_t i = 0; i < (size_t)W * H; ++i) px[4 * i + c] = planes[(size_t)c * plane + i * bps];  // high byte
+    if (used < 3)
+        for (size_t i = 0; i < (size_t)W * H; ++i) px[4 * i + 1] = px[4 * i + 2] = px[4 * i];
+    return true;
+}
+
 // ------------------------------------------------------------------ GIF
 // First image of a GIF87a / GIF89a file, as stb_image returns it: a canvas of
 // the logical screen size, transparent black where the image does not cover
@@ -1042,7 +1140,11 @@ bool load_image(const std::string &path, uint32_t &w, uint32_t &h, std::vector<f
     else if (e == "bmp") ok = load_bmp(f, w, h, rgba, err);
     else if (e == "tga") ok = load_tga(f, w, h, rgba, err);
     else if (e == "ppm" || e == "pgm" || e == "pnm") ok = load_pnm(f, w, h, rgba, err);
-    else if (e == "gif") {
+    else if (e == "psd") {
+        std::vector<unsigned char> px;
+        ok = load_psd(f, w, h, px, err);
+        if (ok) rgba_from_8bit(px, 4, w, h, rgba);
+    } else if (e == "gif") {
         std::vector<unsigned char> px;
         ok = load_gif(f, w, h, px, err);
         if (ok) rgba_from_8bit(px, 4, w, h, rgba);
@@ -1052,7 +1154,7 @@ bool load_image(const std::string &path, uint32_t &w, uint32_t &h, std::vector<f
         ok = load_jpeg(f, w, h, px, chans, err);
         if (ok) rgba_from_8bit(px, chans, w, h, rgba);
     } else {
-        err = "unsupported: image format ." + e + " (EXR, HDR, PNG, JPEG, GIF, BMP, TGA and PNM are read)";
+        err = "unsupported: image format ." + e + " (EXR, HDR, PNG, JPEG, GIF, PSD, BMP, TGA and PNM are read)";
         return false;
     }
     if (!ok && err.find(path) == std::string::npos) err += " (" + path + ")";

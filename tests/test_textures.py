@@ -491,9 +491,69 @@ def test_gif_first_image(tmp_path, interlace, transparent, size):
         assert (~opaque).any() and np.array_equal(~opaque, idx == transparent)
 
 
+def packbits(row):
+    """PackBits: runs of 3..128 equal bytes as (1 - n, byte), the rest as literal chunks."""
+    out, i, lit = bytearray(), 0, bytearray()
+
+    def flush():
+        while lit:
+            chunk = lit[:128]
+            out.extend([len(chunk) - 1]); out.extend(chunk)
+            del lit[:128]
+
+    while i < len(row):
+        j = i
+        while j < len(row) and j - i < 128 and row[j] == row[i]:
+            j += 1
+        if j - i >= 3:
+            flush()
+            out.extend([(1 - (j - i)) & 0xFF, row[i]])
+            i = j
+        else:
+            lit.append(row[i])
+            i += 1
+    flush()
+    return bytes(out)
+
+
+def psd_bytes(planes, depth, rle):
+    """planes: (channels, h, w) samples of an RGB(A) document."""
+    nch, h, w = planes.shape
+    dt = ">u2" if depth == 16 else "u1"
+    head = b"8BPS" + struct.pack(">H6xHIIHH", 1, nch, h, w, depth, 3)
+    body = struct.pack(">I", 0) * 3  # color mode data, image resources, layers and masks: empty
+    if not rle:
+        return head + body + struct.pack(">H", 0) + planes.astype(dt).tobytes()
+    rows = [packbits(planes[c, y].astype(dt).tobytes()) for c in range(nch) for y in range(h)]
+    return head + body + struct.pack(">H", 1) + b"".join(struct.pack(">H", len(r)) for r in rows) + b"".join(rows)
+
+
+@pytest.mark.parametrize("depth,rle,nch", [(8, False, 3), (8, True, 4), (16, False, 4), (16, True, 3)])
+def test_psd_composite(tmp_path, depth, rle, nch):
+    rng = np.random.default_rng(depth + nch + rle)
+    h, w = 9, 300  # wide rows: literal chunks longer than 128 bytes
+    planes = rng.integers(0, 1 << depth, (nch, h, w)).astype(np.uint16 if depth == 16 else np.uint8)
+    planes[:, 2, 10:200] = planes[:, 2, 10:11]  # runs for PackBits (longer than 128)
+    path = write(tmp_path, "a.psd", psd_bytes(planes, depth, rle))
+    got = pt.load_image(path)
+    hi = (planes >> 8) if depth == 16 else planes
+    assert got.shape == (h, w, 4)
+    for c in range(3):
+        assert np.array_equal(to8(got[..., c]), hi[c].astype(np.int32))
+    alpha = hi[3] if nch == 4 else np.full((h, w), 255)
+    assert np.allclose(got[..., 3], alpha / 255.0, atol=1e-7)
+    if depth == 8:
+        from PIL import Image  # an independent reader of the test file
+        ref = np.asarray(Image.open(path).convert("RGBA" if nch == 4 else "RGB"))
+        assert np.array_equal(ref[..., :3].transpose(2, 0, 1), planes[:3])
+
+
 def test_image_errors(tmp_path):
     with pytest.raises(pt.PupilError) as e:
-        pt.load_image(write(tmp_path, "x.psd", b"8BPS"))
+        pt.load_image(write(tmp_path, "x.pic", b"\x53\x80\xf6\x34"))  # not Radiance
+    assert e.value.code == -6
+    with pytest.raises(pt.PupilError) as e:  # CMYK document
+        pt.load_image(write(tmp_path, "c.psd", b"8BPS" + struct.pack(">H6xHIIHH", 1, 4, 2, 2, 8, 4)))
     assert e.value.code == -8
     with pytest.raises(pt.PupilError) as e:
         pt.load_image(write(tmp_path, "x.gif", b"GIF89a"))
