@@ -634,12 +634,14 @@ bool load_psd(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
     }
     const size_t bps = depth / 8, row = (size_t)W * bps, plane = row * H;
     const uint32_t used = std::min<uint32_t>(nch, 4);
+    // Every row costs at least one stored byte (raw: all of it), so a file
+    // shorter than that is truncated whatever its header claims.
+    if (p + (comp == 0 ? (size_t)used * plane : (size_t)2 * nch * H + (size_t)used * H) > f.size()) {
+        err = "parse: truncated PSD image data";
+        return false;
+    }
     std::vector<unsigned char> planes((size_t)used * plane);
     if (comp == 0) {
-        if (p + (size_t)used * plane > f.size()) {
-            err = "parse: truncated PSD image data";
-            return false;
-        }
         std::memcpy(planes.data(), &f[p], planes.size());
     } else {
         // Byte counts of every row of every channel, then the PackBits rows.
@@ -748,6 +750,10 @@ bool load_gif(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
         const uint32_t iw = f[p + 4] | f[p + 5] << 8, ih = f[p + 6] | f[p + 7] << 8;
         const unsigned char flags = f[p + 8];
         p += 9;
+        if ((uint64_t)iw * ih > (uint64_t)W * H || iw == 0 || ih == 0) {
+            err = "parse: GIF image larger than its screen";
+            return false;
+        }
         std::vector<unsigned char> lct;
         if (flags & 0x80) {
             const size_t n = (size_t)3 << ((flags & 7) + 1);
