@@ -555,6 +555,15 @@ def test_image_errors(tmp_path):
     with pytest.raises(pt.PupilError) as e:  # CMYK document
         pt.load_image(write(tmp_path, "c.psd", b"8BPS" + struct.pack(">H6xHIIHH", 1, 4, 2, 2, 8, 4)))
     assert e.value.code == -8
+    with pytest.raises(pt.PupilError) as e:  # header claims 30000 x 30000 RGB, no data follows
+        pt.load_image(write(tmp_path, "t.psd", b"8BPS" + struct.pack(">H6xHIIHH", 1, 3, 30000, 30000, 8, 3)
+                            + struct.pack(">I", 0) * 3 + struct.pack(">H", 0)))
+    assert e.value.code == -6
+    gif = bytearray(b"GIF89a" + struct.pack("<HHBBB", 4, 4, 0x80, 0, 0) + bytes(6))  # 4x4 screen, 2 colors
+    gif += b"\x2c" + struct.pack("<HHHHB", 0, 0, 60000, 60000, 0) + b"\x02\x02\x4c\x01\x00\x3b"
+    with pytest.raises(pt.PupilError) as e:  # image descriptor far larger than the screen
+        pt.load_image(write(tmp_path, "big.gif", bytes(gif)))
+    assert e.value.code == -6
     with pytest.raises(pt.PupilError) as e:
         pt.load_image(write(tmp_path, "x.gif", b"GIF89a"))
     assert e.value.code == -6
