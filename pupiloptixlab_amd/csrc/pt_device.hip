@@ -979,6 +979,7 @@ struct pt_ctx {
     uint32_t trace_blocks_per_cu = 8;       // resident closest-hit traversal blocks per CU, sphere variant
     uint32_t trace_blocks_per_cu_tri = 8;   // triangle-only variant
     uint32_t shade_blocks_per_cu = 8;       // resident k_shade<false> blocks (256 threads) per CU
+    uint32_t shade_guided_blocks_per_cu = 2;  // resident k_shade_guided<false> blocks per CU
     uint32_t shadow_blocks_per_cu = 8;      // resident any-hit (shadow) traversal blocks per CU, spheres
     uint32_t shadow_blocks_per_cu_tri = 8;  // triangle-only variant
 
@@ -1249,11 +1250,12 @@ uint32_t shadow_grid(pt_ctx *ctx, bool spheres) {
 // blocks than slots the dispatcher balances uneven shading work across CUs,
 // and blocks that find no work exit at once.  Terrain 1080p, 64 spp,
 // k_shade<false> per render at 1, 4, 16 and 64 rounds: 49.2, 48.0, 46.9 and
-// 46.3 ms.  k_shade_guided<false> keeps 8 blocks per CU: its training
-// records are appended in block order until the record buffer is full, and
-// larger grids measured no faster.
-uint32_t shade_grid(pt_ctx *ctx, bool guided) {
-    return (uint32_t)ctx->num_cu * (guided ? 8u : kShadeRounds * ctx->shade_blocks_per_cu);
+// 46.3 ms.  k_shade_guided<false> uses the same rounds when it only samples;
+// while training it keeps 8 blocks per CU, since its training records are
+// appended in block order until the record buffer is full.
+uint32_t shade_grid(pt_ctx *ctx, bool guided, bool training) {
+    if (guided && training) return (uint32_t)ctx->num_cu * 8u;
+    return (uint32_t)ctx->num_cu * kShadeRounds * (guided ? ctx->shade_guided_blocks_per_cu : ctx->shade_blocks_per_cu);
 }
 
 int ensure_spill(pt_ctx *ctx) {
@@ -2380,6 +2382,9 @@ int pt_create(int hip_device, pt_ctx **out) {
         c = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_shade<false>, 256, 0) == hipSuccess && c > 0)
             ctx->shade_blocks_per_cu = (uint32_t)c;
+        c = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_shade_guided<false>, 256, 0) == hipSuccess && c > 0)
+            ctx->shade_guided_blocks_per_cu = (uint32_t)c;
 
     }
     if (hipMalloc((void **)&ctx->d_stat, 5 * sizeof(unsigned long long)) != hipSuccess ||
@@ -2870,7 +2875,7 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
     sp.rr_depth = p->rr_depth;
     sp.mis_mode = p->mis_mode;
     sp.aov = (p->flags & PT_RENDER_AOV) ? 1u : 0u;
-    const uint32_t tg = trace_grid(ctx), sg = shade_grid(ctx, guided);
+    const uint32_t tg = trace_grid(ctx), sg = shade_grid(ctx, guided, guide_train);
 
     const uint32_t batches = (p->spp + per_launch - 1) / per_launch;
     const uint32_t hist_elems = std::max(1u, batches) * ctx->num_counters;
