@@ -313,6 +313,125 @@ def test_conductor_and_dielectric_delta_samples():
     assert refl / n == pytest.approx(F, abs=0.02)
 
 
+# ---------------------------------------------------------------- BSDF albedo closed forms
+# Directional albedo A(wo) = E[f |cos| / pdf] of the oracle's own sampler
+# against float64 closed forms built from textbook Fresnel equations.
+def fresnel_conductor_textbook(eta, k, ci):
+    n = complex(eta, k)
+    ct = np.sqrt(1 - (1 - ci * ci) / (n * n))
+    rs = (ci - n * ct) / (ci + n * ct)
+    rp = (n * ci - ct) / (n * ci + ct)
+    return 0.5 * (abs(rs) ** 2 + abs(rp) ** 2)
+
+
+def fdr_exact(eta):
+    """Hemispherical average of the Fresnel reflectance, 2 * integral of F(mu)
+    mu over mu in [0, 1], for light arriving from the side where the relative
+    index is eta (eta < 1: from inside, with total internal reflection)."""
+    m = 200000
+    mu = (np.arange(m) + 0.5) / m
+    f = np.array([fresnel_dielectric_textbook(eta, float(c)) for c in mu[::100]])
+    return float(2 * np.mean(f * mu[::100]))
+
+
+def albedo(m, wo, n, seed=4242):
+    acc = np.zeros(3)
+    for _ in range(n):
+        s = oracle.bsdf_sample(m, wo, seed)
+        seed = s["seed"]
+        if s["pdf"] > 0:
+            acc += s["f"] * abs(s["wi"][2]) / s["pdf"]
+    return acc / n
+
+
+def wo_at(cos_theta):
+    st = math.sqrt(1 - cos_theta * cos_theta)
+    return np.array([st * 0.6, st * 0.8, cos_theta], np.float32)
+
+
+def plastic_albedo(rho, eta, ci, int_fdr):
+    """Smooth plastic (plastic.h:32-80, linear mode): specular F_o plus the
+    substrate rho / (1 - Fdr_int) * (1 - F_o) * (1 - F_i) / (pi eta^2)
+    integrated over wi: (1 - F_o) rho (1 - Fdr_ext) / (eta^2 (1 - Fdr_int))."""
+    fo = fresnel_dielectric_textbook(eta, ci)
+    return fo + (1 - fo) * rho * (1 - fdr_exact(eta)) / (eta * eta * (1 - int_fdr))
+
+
+@pytest.mark.parametrize("kind,alpha", [(pt.MAT_PLASTIC, 0.3), (pt.MAT_ROUGH_PLASTIC, 0.02)])
+def test_plastic_albedo_closed_form(kind, alpha):
+    """Plastic with a substrate term built on Fdr(1/eta) (optix_material.cpp:95-99)
+    conserves energy: since 1 - Fdr(eta) = eta^2 (1 - Fdr(1/eta)), the albedo is
+    F_o + (1 - F_o) rho per channel (1 for white).  Built on Fdr(eta) instead, a
+    white plastic's albedo would drop to about 0.47."""
+    eta = 1.5
+    rho = np.array([1.0, 0.6, 0.3])
+    m = material(kind, alpha, rgb=tuple(rho))
+    wrong = plastic_albedo(1.0, eta, 0.95, fdr_exact(eta))
+    assert wrong < 0.6
+    for ci in (0.95, 0.6, 0.3):
+        want = np.array([plastic_albedo(r, eta, ci, fdr_exact(1 / eta)) for r in rho])
+        fo = fresnel_dielectric_textbook(eta, ci)
+        assert want == pytest.approx(fo + (1 - fo) * rho, rel=1e-3)
+        got = albedo(m, wo_at(ci), 20000)
+        # the reference's Fdr is a fit (Egan-Hilgeman, 0.1% in 1-1.5); the
+        # rough specular lobe at alpha 0.02 loses little to masking
+        assert got == pytest.approx(want, rel=6e-3), (ci, got, want)
+
+
+def test_rough_conductor_albedo_is_fresnel_at_small_alpha():
+    """GGX conductor (rough_conductor.h:21-46) at alpha 0.01 reflects F_cond(cos)
+    per channel: the microfacets are nearly all aligned with the normal."""
+    eta, k = (0.14282, 0.37414, 1.43944), (3.97472, 2.38066, 1.59981)
+    m = material(pt.MAT_ROUGH_CONDUCTOR, 0.01)
+    for ci in (0.95, 0.6, 0.3):
+        want = np.array([fresnel_conductor_textbook(e, kk, ci) for e, kk in zip(eta, k)])
+        got = albedo(m, wo_at(ci), 6000)
+        assert got == pytest.approx(want, rel=1e-2), (ci, got, want)
+
+
+def test_rough_dielectric_albedo_at_small_alpha():
+    """Walter et al. GGX dielectric (rough_dielectric.h:21-96) at alpha 0.01 from
+    outside: reflection F plus transmission (1 - F) / eta^2 (the sampled weight of
+    a refracted direction is G1(wi) / eta^2, the radiance scaling of the
+    smooth case)."""
+    eta = 1.5
+    m = material(pt.MAT_ROUGH_DIELECTRIC, 0.01)
+    for ci in (0.95, 0.6, 0.3):
+        f = fresnel_dielectric_textbook(eta, ci)
+        want = f + (1 - f) / eta ** 2
+        got = albedo(m, wo_at(ci), 20000)
+        assert got == pytest.approx(np.full(3, want), rel=1e-2), (ci, got, want)
+
+
+def test_smooth_dielectric_sample_weights_exact():
+    """Smooth dielectric (dielectric.h:28-44): a reflected sample weighs 1 and a
+    refracted one 1/eta^2 entering (eta^2 leaving), with Snell's directions, and
+    the reflection frequency is the textbook F."""
+    eta = 1.5
+    m = material(pt.MAT_DIELECTRIC)
+    for ci in (0.9, 0.5, -0.9, -0.5):
+        wo = wo_at(abs(ci)) * np.array([1, 1, np.sign(ci)], np.float32)
+        seed, refl, n = 7, 0, 3000
+        for _ in range(n):
+            s = oracle.bsdf_sample(m, wo, seed)
+            seed = s["seed"]
+            w = s["f"] * abs(s["wi"][2]) / s["pdf"]
+            if s["wi"][2] * wo[2] > 0:
+                refl += 1
+                assert np.allclose(s["wi"], [-wo[0], -wo[1], wo[2]], atol=1e-6)
+                assert np.allclose(w, 1.0, rtol=1e-5)
+            else:
+                scale = 1 / eta ** 2 if ci > 0 else eta ** 2
+                assert np.allclose(w, scale, rtol=1e-5), (ci, w)
+                sin_i = math.sqrt(1 - ci * ci)
+                sin_t = math.sqrt(max(0.0, 1 - float(s["wi"][2]) ** 2))
+                assert sin_t == pytest.approx(sin_i / eta if ci > 0 else sin_i * eta, abs=2e-5)
+                # refraction stays in the plane of incidence, on the far side
+                assert np.dot(s["wi"][:2], wo[:2]) <= 0
+        f_want = fresnel_dielectric_textbook(eta, ci)
+        assert refl / n == pytest.approx(f_want, abs=4 * math.sqrt(f_want * (1 - f_want) / n) + 1e-3)
+
+
 # ---------------------------------------------------------------- watertight intersection
 def test_intersection_closed_form():
     v0, v1, v2 = np.array([0, 0, 0], np.float32), np.array([1, 0, 0], np.float32), np.array([0, 1, 0], np.float32)
@@ -424,6 +543,163 @@ def test_reference_mis_is_biased_as_documented():
     ref = float(oracle.render(scene.desc, spp=96)["accum"][..., :3].mean())
     expected = sum(0.5 ** k for k in range(4))
     assert abs(ref - expected) / expected > 0.05
+
+
+# ---------------------------------------------------------------- reference-mode MIS (main.cu:106-185)
+# A diffuse floor (reflectance rho) under a downward-facing quad light (two
+# triangle emitters) and a constant environment, seen by a 2x2 camera with a
+# 0.1 degree field of view, max_depth 2: one NEE and one BSDF sample per path.
+# Three emitters, so every selection probability is 1/3 and the reference
+# weights do not sum to one.  The expectation of each estimator is computed
+# below in float64 from the weights alone (no oracle code): quadrature over
+# the light's area for the light directions, over cos(theta) for the rest of
+# the hemisphere.
+MIS_XML = """<scene version="3.0.0">
+  <integrator type="path"><integer name="max_depth" value="2"/></integrator>
+  <sensor type="perspective"><float name="fov" value="0.1"/>
+    <transform name="to_world"><lookat origin="0, 3, 0.3" target="0, 0, 0" up="0, 1, 0"/></transform>
+    <film type="hdrfilm"><integer name="width" value="2"/><integer name="height" value="2"/></film>
+  </sensor>
+  <shape type="rectangle">
+    <transform name="to_world"><scale x="50" y="50"/><rotate x="1" angle="-90"/></transform>
+    <bsdf type="{floor}">{floor_params}</bsdf>
+  </shape>
+  <shape type="rectangle">
+    <transform name="to_world"><scale x="{half}" y="{half}"/><rotate x="1" angle="90"/>
+      <translate value="{cx}, {cy}, {cz}"/></transform>
+    <bsdf type="diffuse"><rgb name="reflectance" value="0"/></bsdf>
+    <emitter type="area"><rgb name="radiance" value="{le}"/></emitter>
+  </shape>
+  <emitter type="constant"><rgb name="radiance" value="{env}"/></emitter>
+</scene>"""
+MIS_SCENE = dict(rho=0.5, half=0.6, cx=1.0, cy=0.8, cz=-0.5, le=10.0, env=3.0)
+
+
+def mis_scene(tmp_path, floor="diffuse", floor_params=None, **kw):
+    prm = dict(MIS_SCENE, **kw)
+    if floor_params is None:
+        floor_params = f'<rgb name="reflectance" value="{prm["rho"]}"/>'
+    path = tmp_path / "mis.xml"
+    path.write_text(MIS_XML.format(floor=floor, floor_params=floor_params, **prm))
+    return pt.HostScene.load_xml(str(path)), prm
+
+
+def floor_points(desc):
+    """Floor points seen through the pixel centers (the footprint is ~5 mm)."""
+    pts = []
+    for y in range(desc.height):
+        for x in range(desc.width):
+            o, d = oracle.camera_ray(desc, x, y, 0.5, 0.5)
+            o, d = o.astype(np.float64), d.astype(np.float64)
+            pts.append(o - o[1] / d[1] * d)
+    return pts
+
+
+def mis_expectation(x, prm, sel_light, sel_env, variant, n=800):
+    """E[L] at floor point x for one NEE + one BSDF sample, float64.
+
+    Light directions (quad area, two triangles of area A): Le f cos_x cos_l / d^2
+    x (w_nee + w_bsdf); the others: f cos Le_env x (w_nee_env + w_bsdf_env).
+    p_l = d^2 / (cos_l A) is the triangle's solid-angle density, p_b = cos / pi
+    the diffuse BSDF's; the environment is sampled on the hemisphere
+    (1 / 2pi, env.h:70-80) and evaluated as 1 / 4pi (env.h:82-85)."""
+    rho, le, env, h = prm["rho"], prm["le"], prm["env"], prm["half"]
+    u = (np.arange(n) + 0.5) / n * 2 - 1
+    X, Z = np.meshgrid(prm["cx"] + u * h, prm["cz"] + u * h, indexing="ij")
+    dA = (2 * h / n) ** 2
+    vx, vy, vz = X - x[0], prm["cy"] - x[1], Z - x[2]
+    d2 = vx * vx + vy * vy + vz * vz
+    cos_x = vy / np.sqrt(d2)  # floor normal +y
+    cos_l = cos_x             # light normal -y
+    p_l = d2 / (cos_l * (2 * h * h))
+    p_b = cos_x / np.pi
+    a, b = 1 / (2 * np.pi), 1 / (4 * np.pi)
+    if variant == "reference":  # main.cu:138-140, :180-182, :168-171
+        wl, wb = p_l / (p_l + p_b), p_b / (p_b + sel_light * p_l)
+        wle, wbe = (lambda q: a / (a + q)), (lambda q: q / (q + b))
+    elif variant == "selection_swapped":  # selection probability on the NEE side only
+        wl, wb = sel_light * p_l / (sel_light * p_l + p_b), p_b / (p_b + p_l)
+        wle, wbe = (lambda q: a / (a + q)), (lambda q: q / (q + b))
+    elif variant == "env_pdf_quarter":  # 1 / 4pi on both sides of the environment
+        wl, wb = p_l / (p_l + p_b), p_b / (p_b + sel_light * p_l)
+        wle, wbe = (lambda q: b / (b + q)), (lambda q: q / (q + b))
+    elif variant == "env_selection_on_miss":  # env selection probability on the BSDF side too
+        wl, wb = p_l / (p_l + p_b), p_b / (p_b + sel_light * p_l)
+        wle, wbe = (lambda q: a / (a + q)), (lambda q: q / (q + sel_env * b))
+    elif variant == "consistent":  # selection probability on both sides, hemisphere pdf for the env
+        wl, wb = sel_light * p_l / (sel_light * p_l + p_b), p_b / (p_b + sel_light * p_l)
+        wle, wbe = (lambda q: sel_env * a / (sel_env * a + q)), (lambda q: q / (q + sel_env * a))
+    else:
+        raise ValueError(variant)
+    light = np.sum(le * rho / np.pi * cos_x * cos_l / d2 * (wl + wb)) * dA
+    g = lambda c: rho / np.pi * c * env * (wle(c / np.pi) + wbe(c / np.pi))
+    m = 200000
+    mu = (np.arange(m) + 0.5) / m
+    hemisphere = 2 * np.pi * np.sum(g(mu)) / m
+    behind_light = np.sum(g(cos_x) * cos_l / d2) * dA  # env directions the light occludes
+    return light + hemisphere - behind_light
+
+
+MIS_VARIANTS = ["reference", "selection_swapped", "env_pdf_quarter", "env_selection_on_miss", "consistent"]
+
+
+def test_reference_mis_matches_float64_expectation(tmp_path):
+    """Reference-mode NEE weight formed before the selection probability
+    (Q1), the environment's BSDF-side weight against 1/(4 pi) while NEE uses
+    1/(2 pi) (Q3): the oracle's mean equals the float64 expectation of exactly
+    these weights within 0.5%, and every plausible misplacement of a
+    selection probability or pdf moves the expectation by more than 3%."""
+    scene, prm = mis_scene(tmp_path)
+    d = scene.desc
+    sel, _, env_sel = oracle.emitter_table(d)
+    assert sel == pytest.approx([1 / 3, 1 / 3]) and env_sel == pytest.approx(1 / 3)
+    pts = floor_points(d)
+    expect = {v: float(np.mean([mis_expectation(x, prm, sel[0], env_sel, v) for x in pts])) for v in MIS_VARIANTS}
+    r = oracle.render(d, spp=1 << 20, mis_mode=pt.MIS_REFERENCE)
+    assert np.all(r["prim"] != pt.MISS)
+    got = float(r["accum"][..., :3].mean())
+    assert got == pytest.approx(expect["reference"], rel=5e-3)
+    for v in MIS_VARIANTS[1:]:
+        assert abs(expect[v] / expect["reference"] - 1) > 0.03, (v, expect)
+        assert abs(got / expect[v] - 1) > 0.025, (v, got, expect)
+
+
+def test_consistent_mis_is_unbiased_closed_form(tmp_path):
+    """Consistent mode: the weights sum to one, so the mean is the reflected
+    radiance (rho/pi) * integral of L_in cos over the hemisphere."""
+    scene, prm = mis_scene(tmp_path)
+    d = scene.desc
+    sel, _, env_sel = oracle.emitter_table(d)
+    pts = floor_points(d)
+    expect = float(np.mean([mis_expectation(x, prm, sel[0], env_sel, "consistent") for x in pts]))
+    # the same value without any MIS weight: light + env irradiance
+    h, n = prm["half"], 800
+    u = (np.arange(n) + 0.5) / n * 2 - 1
+    direct = []
+    for x in pts:
+        X, Z = np.meshgrid(prm["cx"] + u * h, prm["cz"] + u * h, indexing="ij")
+        vy = prm["cy"] - x[1]
+        d2 = (X - x[0]) ** 2 + vy ** 2 + (Z - x[2]) ** 2
+        ccos = vy * vy / d2  # cos_x * cos_l
+        light_irr = np.sum(ccos / d2) * (2 * h / n) ** 2  # integral of cos over the light's solid angle
+        direct.append(prm["rho"] / np.pi * (prm["le"] * light_irr + prm["env"] * (np.pi - light_irr)))
+    assert expect == pytest.approx(float(np.mean(direct)), rel=1e-6)
+    r = oracle.render(d, spp=1 << 20, mis_mode=pt.MIS_CONSISTENT)
+    assert float(r["accum"][..., :3].mean()) == pytest.approx(expect, rel=5e-3)
+
+
+def test_reference_env_mis_after_delta_bounce(tmp_path):
+    """A perfect mirror floor (conductor 'none': eta 0, k 1, F = 1): NEE gives
+    nothing (f = 0), the reflected ray escapes, and the reference weights the
+    environment against 1/(4 pi) even after the delta bounce (Q2): the pixel
+    is Le_env / (1 + 1/(4 pi)) exactly.  The consistent mode gives weight 1."""
+    scene, prm = mis_scene(tmp_path, floor="conductor", floor_params='<string name="material" value="none"/>')
+    d = scene.desc
+    env = prm["env"]
+    r = oracle.render(d, spp=64, mis_mode=pt.MIS_REFERENCE)
+    assert np.allclose(r["accum"][..., :3], env / (1 + 1 / (4 * np.pi)), rtol=1e-6)
+    r = oracle.render(d, spp=64, mis_mode=pt.MIS_CONSISTENT)
+    assert np.allclose(r["accum"][..., :3], env, rtol=1e-6)
 
 
 def test_environment_only_is_exact():
