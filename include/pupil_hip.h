@@ -481,6 +481,13 @@ typedef struct pt_gen_params {
 /* Scene::LoadFromXML (resource/scene.cpp:27-56) for the mitsuba subset. */
 int pt_scene_load_xml(const char *path, pt_host_scene **out, char *err, size_t err_len);
 int pt_scene_generate(int kind, const pt_gen_params *params, pt_host_scene **out);
+/* Loader warnings, one per line, for input the reference accepts with a
+ * fallback and a log line (unknown bsdf and shape types, resource/material.cpp
+ * and resource/shape.cpp; an unknown conductor name, which material.cpp:72-80
+ * replaces by eta 0, k 1 silently).  Returns the number of warnings (< 0 on a
+ * bad argument); buf (len bytes, may be NULL) gets them newline-separated,
+ * truncated to fit. */
+int pt_scene_get_warnings(const pt_host_scene *scene, char *buf, size_t len);
 /* Pointers in the returned desc stay valid until pt_scene_destroy. */
 int pt_scene_get_desc(const pt_host_scene *scene, pt_scene_desc *out);
 void pt_scene_destroy(pt_host_scene *scene);

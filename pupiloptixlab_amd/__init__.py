@@ -193,6 +193,7 @@ _SIGNATURES = {
     "pt_trace_rays": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.c_void_p]),
     "pt_bandwidth_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_double)]),
     "pt_scene_load_xml": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]),
+    "pt_scene_get_warnings": (C.c_int, [C.c_void_p, C.c_char_p, C.c_size_t]),
     "pt_scene_generate": (C.c_int, [C.c_int, C.POINTER(GenParams), C.POINTER(C.c_void_p)]),
     "pt_scene_get_desc": (C.c_int, [C.c_void_p, C.POINTER(SceneDesc)]),
     "pt_scene_destroy": (None, [C.c_void_p]),
@@ -280,6 +281,15 @@ class HostScene:
         if code != PT_OK:
             raise PupilError(code, f"pt_scene_load_xml({path}): {err.value.decode()}")
         return cls(h)
+
+    def warnings(self) -> list:
+        """Loader warnings (unknown conductor names, BSDF and shape types)."""
+        n = lib().pt_scene_get_warnings(self._h, None, 0)
+        if n <= 0:
+            return []
+        buf = C.create_string_buffer(256 * n + 1)
+        lib().pt_scene_get_warnings(self._h, buf, len(buf))
+        return [w for w in buf.value.decode().split("\n") if w]
 
     def close(self):
         if self._h:

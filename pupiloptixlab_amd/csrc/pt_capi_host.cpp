@@ -36,9 +36,28 @@ void copy_err(const std::string &s, char *err, size_t len) {
 }
 }  // namespace
 
+namespace {
+// A host allocation that fails inside an entry point is reported as
+// PT_ERR_OOM instead of escaping the C interface.
+template <typename F>
+int guard_oom(std::nullptr_t, const char *, F &&body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        return PT_ERR_OOM;
+    }
+}
+}  // namespace
+
 extern "C" {
 
+static int pt_scene_load_xml_unguarded(const char *path, pt_host_scene **out, char *err, size_t err_len);
 int pt_scene_load_xml(const char *path, pt_host_scene **out, char *err, size_t err_len) {
+    return guard_oom(nullptr, "pt_scene_load_xml", [&] {
+        return pt_scene_load_xml_unguarded(path, out, err, err_len);
+    });
+}
+static int pt_scene_load_xml_unguarded(const char *path, pt_host_scene **out, char *err, size_t err_len) {
     if (!path || !out) return PT_ERR_INVALID_ARG;
     *out = nullptr;
     pt_host_scene *hs = new (std::nothrow) pt_host_scene();
@@ -53,7 +72,19 @@ int pt_scene_load_xml(const char *path, pt_host_scene **out, char *err, size_t e
     return PT_OK;
 }
 
+int pt_scene_get_warnings(const pt_host_scene *scene, char *buf, size_t len) {
+    if (!scene) return PT_ERR_INVALID_ARG;
+    std::string all;
+    for (const auto &w : scene->scene.warnings) all += w + "\n";
+    copy_err(all, buf, len);
+    return (int)scene->scene.warnings.size();
+}
+
+static int pt_scene_generate_unguarded(int kind, const pt_gen_params *params, pt_host_scene **out);
 int pt_scene_generate(int kind, const pt_gen_params *params, pt_host_scene **out) {
+    return guard_oom(nullptr, "pt_scene_generate", [&] { return pt_scene_generate_unguarded(kind, params, out); });
+}
+static int pt_scene_generate_unguarded(int kind, const pt_gen_params *params, pt_host_scene **out) {
     if (!out) return PT_ERR_INVALID_ARG;
     *out = nullptr;
     pt_gen_params p;
@@ -78,7 +109,15 @@ int pt_scene_get_desc(const pt_host_scene *scene, pt_scene_desc *out) {
 
 void pt_scene_destroy(pt_host_scene *scene) { delete scene; }
 
+static int pt_load_image_unguarded(const char *path, uint32_t *width, uint32_t *height, float *rgba, size_t capacity, char *err,
+                  size_t err_len);
 int pt_load_image(const char *path, uint32_t *width, uint32_t *height, float *rgba, size_t capacity, char *err,
+                  size_t err_len) {
+    return guard_oom(nullptr, "pt_load_image", [&] {
+        return pt_load_image_unguarded(path, width, height, rgba, capacity, err, err_len);
+    });
+}
+static int pt_load_image_unguarded(const char *path, uint32_t *width, uint32_t *height, float *rgba, size_t capacity, char *err,
                   size_t err_len) {
     if (!path || !width || !height) return PT_ERR_INVALID_ARG;
     uint32_t w = 0, h = 0;
@@ -97,13 +136,25 @@ int pt_load_image(const char *path, uint32_t *width, uint32_t *height, float *rg
     return PT_OK;
 }
 
+static int pt_write_exr_unguarded(const char *path, const float *rgba, uint32_t width, uint32_t height);
 int pt_write_exr(const char *path, const float *rgba, uint32_t width, uint32_t height) {
+    return guard_oom(nullptr, "pt_write_exr", [&] { return pt_write_exr_unguarded(path, rgba, width, height); });
+}
+static int pt_write_exr_unguarded(const char *path, const float *rgba, uint32_t width, uint32_t height) {
     if (!path || !rgba || width == 0 || height == 0) return PT_ERR_INVALID_ARG;
     std::string e;
     return pupil::write_exr_rgb(path, rgba, width, height, e) ? PT_OK : PT_ERR_IO;
 }
 
+static int pt_host_trace_rays_unguarded(const pt_scene_desc *s, const float *rays, uint32_t count, int any_hit, void *hits,
+                       uint64_t *node_visits, uint64_t *tri_tests);
 int pt_host_trace_rays(const pt_scene_desc *s, const float *rays, uint32_t count, int any_hit, void *hits,
+                       uint64_t *node_visits, uint64_t *tri_tests) {
+    return guard_oom(nullptr, "pt_host_trace_rays", [&] {
+        return pt_host_trace_rays_unguarded(s, rays, count, any_hit, hits, node_visits, tri_tests);
+    });
+}
+static int pt_host_trace_rays_unguarded(const pt_scene_desc *s, const float *rays, uint32_t count, int any_hit, void *hits,
                        uint64_t *node_visits, uint64_t *tri_tests) {
     if (!s || (!rays && count) || (!hits && count)) return PT_ERR_INVALID_ARG;
     if (s->num_triangles && (!s->positions || !s->indices)) return PT_ERR_INVALID_ARG;

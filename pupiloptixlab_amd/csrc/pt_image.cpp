@@ -28,6 +28,19 @@
 #include "pt_scene.h"
 
 namespace pupil {
+
+// One size limit for every reader, applied to the header's dimensions before
+// any pixel buffer is allocated (as stb_image limits each side): at most
+// 2^24 pixels per side and 2^28 pixels in all (4 GiB of float RGBA).
+bool image_size_ok(uint64_t W, uint64_t H, std::string &err) {
+    if (W == 0 || H == 0 || W > kMaxImageSide || H > kMaxImageSide || W * H > kMaxImagePixels) {
+        err = "unsupported: image of " + std::to_string(W) + "x" + std::to_string(H) +
+              " pixels (at most 2^24 per side and 2^28 in all)";
+        return false;
+    }
+    return true;
+}
+
 namespace {
 
 bool read_file(const std::string &path, std::vector<unsigned char> &out, std::string &err) {
@@ -193,6 +206,7 @@ bool load_exr(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
         err = "unsupported: EXR image larger than 65536 pixels per side";
         return false;
     }
+    if (!image_size_ok((uint64_t)W, (uint64_t)H, err)) return false;
     int lines = 0;
     switch (compression) {
         case 0: case 1: case 2: lines = 1; break;
@@ -343,6 +357,7 @@ bool load_hdr(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
         err = "unsupported: HDR orientation other than -Y H +X W / +Y H +X W";
         return false;
     }
+    if (!image_size_ok((uint64_t)W, (uint64_t)H, err)) return false;
     const bool bottom_up = std::string(ya) == "+Y";
     w = (uint32_t)W;
     h = (uint32_t)H;
@@ -447,6 +462,7 @@ bool load_png(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
         err = "parse: PNG without a valid IHDR";
         return false;
     }
+    if (!image_size_ok(W, H, err)) return false;
     if (interlace > 1) {
         err = "parse: bad PNG interlace method";
         return false;
@@ -614,6 +630,7 @@ bool load_psd(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
         err = "unsupported: PSD color mode or depth (8/16-bit RGB is read)";
         return false;
     }
+    if (!image_size_ok(W, H, err)) return false;
     size_t p = 26;
     for (int section = 0; section < 3; ++section) {  // color mode data, image resources, layers and masks
         if (p + 4 > f.size()) {
@@ -639,6 +656,27 @@ bool load_psd(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
     if (p + (comp == 0 ? (size_t)used * plane : (size_t)2 * nch * H + (size_t)used * H) > f.size()) {
         err = "parse: truncated PSD image data";
         return false;
+    }
+    if (comp == 1) {
+        // PackBits stores at most 128 bytes per two-byte packet, so a row of
+        // `row` bytes needs at least 2 ceil(row / 128) stored bytes; the
+        // declared counts of the rows read must all be that large and fit in
+        // the file before the planes are allocated.
+        const size_t min_row = 2 * ((row + 127) / 128);
+        size_t total = 0;
+        for (uint32_t c = 0; c < used; ++c)
+            for (uint32_t y = 0; y < H; ++y) {
+                const size_t n = u16(p + 2 * ((size_t)c * H + y));
+                if (n < min_row) {
+                    err = "parse: PSD PackBits row shorter than its width allows";
+                    return false;
+                }
+                total += n;
+            }
+        if (p + (size_t)2 * nch * H + total > f.size()) {
+            err = "parse: truncated PSD image data";
+            return false;
+        }
     }
     std::vector<unsigned char> planes((size_t)used * plane);
     if (comp == 0) {
@@ -709,6 +747,7 @@ bool load_gif(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
         err = "parse: GIF with an empty screen";
         return false;
     }
+    if (!image_size_ok(W, H, err)) return false;
     size_t p = 13;
     std::vector<unsigned char> gct;
     if (f[10] & 0x80) {
@@ -947,6 +986,7 @@ bool load_bmp(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
     };
     const bool has_alpha = mask[3] != 0;
     const int chans = has_alpha ? 4 : 3;
+    if (!image_size_ok((uint64_t)W, (uint64_t)H, err)) return false;
     std::vector<unsigned char> px((size_t)W * H * chans);
     for (uint32_t y = 0; y < H; ++y) {
         const unsigned char *row = &f[off + stride * (top_down ? y : H - 1 - y)];
@@ -1040,6 +1080,7 @@ bool load_tga(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
         return true;
     };
     const uint32_t sbytes = (bpp + 7) / 8;  // bytes per stored pixel
+    if (!image_size_ok((uint64_t)W, (uint64_t)H, err)) return false;
     std::vector<unsigned char> px((size_t)W * H * chans);
     const size_t n = (size_t)W * H;
     size_t i = 0;
@@ -1119,6 +1160,7 @@ bool load_pnm(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, std
         err = "parse: truncated PNM raster";
         return false;
     }
+    if (!image_size_ok((uint64_t)W, (uint64_t)H, err)) return false;
     std::vector<unsigned char> px((size_t)W * H * chans);
     for (size_t i = 0; i < px.size(); ++i) {
         const uint32_t s = bpc == 2 ? ((uint32_t)f[p + 2 * i] << 8 | f[p + 2 * i + 1]) : f[p + i];

@@ -544,9 +544,14 @@ struct Loader {
         if (t == "conductor" || t == "roughconductor") {
             m.type = t == "conductor" ? PT_MAT_CONDUCTOR : PT_MAT_ROUGH_CONDUCTOR;
             float eta[3] = {0.f, 0.f, 0.f}, k[3] = {1.f, 1.f, 1.f};
-            if (!lookup_conductor_ior(o->prop("material"), eta, k)) {
+            const std::string metal = o->prop("material");
+            if (!lookup_conductor_ior(metal, eta, k)) {
+                // material.cpp:72-80, :92-96: an absent or unknown name is the
+                // perfect mirror (eta 0, k 1).
                 eta[0] = eta[1] = eta[2] = 0.f;
                 k[0] = k[1] = k[2] = 1.f;
+                if (!metal.empty())
+                    scene.warnings.push_back("unknown conductor material '" + metal + "': using eta 0, k 1");
             }
             if (!load_texture(o, "specular_reflectance", m.tex[0], 1.f, 1.f, 1.f)) return false;
             if (!load_texture(o, "eta", m.tex[1], eta[0], eta[1], eta[2])) return false;
@@ -566,6 +571,7 @@ struct Loader {
         }
         // Unknown bsdf: the reference warns and uses an Unknown material that
         // shades black; map it to a black diffuse surface.
+        scene.warnings.push_back("unknown bsdf type '" + t + "': shaded black");
         m.type = PT_MAT_DIFFUSE;
         m.tex[0] = make_rgb_texture(0.f, 0.f, 0.f);
         return true;
@@ -598,7 +604,8 @@ struct Loader {
             shared_kind = 2;
             shared_flip[2] = load_bool(o, "flip_normals", false);
         } else {
-            return true;  // unknown shape type: skipped with a warning in the reference
+            scene.warnings.push_back("unknown shape type '" + o->type + "': skipped");
+            return true;  // skipped with a warning in the reference
         }
         pt_material_desc mat;
         if (!load_material(o->sub("bsdf"), mat)) return false;

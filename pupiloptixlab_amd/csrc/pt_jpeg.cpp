@@ -21,6 +21,8 @@
 #include <string>
 #include <vector>
 
+#include "pt_scene.h"
+
 namespace pupil {
 
 namespace {
@@ -262,6 +264,7 @@ bool load_jpeg(const std::vector<unsigned char> &f, uint32_t &w, uint32_t &h, st
                 err = "unsupported: JPEG frame (1 or 3 components are read)";
                 return false;
             }
+            if (!image_size_ok((uint64_t)W, (uint64_t)H, err)) return false;
             comp.resize(nc);
             for (int k = 0; k < nc; ++k) {
                 comp[k].id = d[6 + 3 * k];

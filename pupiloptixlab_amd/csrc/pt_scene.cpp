@@ -804,12 +804,82 @@ const DielectricIor kDielectricIor[] = {
     {"amber", 1.55f}, {"pet", 1.5750f}, {"diamond", 2.419f},
 };
 struct ConductorIor { const char *name; float eta[3]; float k[3]; };
+// Conductor eta / k at the RGB primaries, 73 named metals and compounds
+// (render/material/ior.h:59-174; physical data, so the values are the same).
 const ConductorIor kConductorIor[] = {
+    {"a-C", {2.93785f, 2.22242f, 1.96400f}, {0.88555f, 0.79763f, 0.81356f}},
     {"Ag", {0.15494f, 0.11648f, 0.13809f}, {4.81810f, 3.11562f, 2.14240f}},
+    {"Al", {1.65394f, 0.87850f, 0.52012f}, {9.20430f, 6.25621f, 4.82675f}},
+    {"Al2O3", {2.12296f, 1.67556f, 1.61569f}, {2.11325f, 1.66785f, 1.60811f}},
+    {"AlAs", {3.59737f, 3.22615f, 2.21235f}, {0.00067f, -0.00050f, 0.00741f}},
+    {"AlAs_palik", {3.59737f, 3.22615f, 2.21235f}, {0.00067f, -0.00050f, 0.00741f}},
+    {"AlSb", {4.70830f, 4.05196f, 4.57419f}, {-0.02918f, 0.09345f, 1.29784f}},
+    {"AlSb_palik", {4.70830f, 4.05196f, 4.57419f}, {-0.02918f, 0.09345f, 1.29784f}},
     {"Au", {0.14282f, 0.37414f, 1.43944f}, {3.97472f, 2.38066f, 1.59981f}},
+    {"Be", {4.17618f, 3.17830f, 2.77819f}, {3.82730f, 3.00374f, 2.86293f}},
+    {"Be_palik", {4.17618f, 3.17830f, 2.77819f}, {3.82730f, 3.00374f, 2.86293f}},
+    {"Cr", {4.36041f, 2.91052f, 1.65119f}, {5.19538f, 4.22239f, 3.74700f}},
+    {"CsI", {2.14035f, 1.69870f, 1.65890f}, {0.00000f, 0.00000f, 0.00000f}},
+    {"CsI_palik", {2.14035f, 1.69870f, 1.65890f}, {0.00000f, 0.00000f, 0.00000f}},
     {"Cu", {0.19999f, 0.92209f, 1.09988f}, {3.90464f, 2.44763f, 2.13765f}},
     {"Cu_palik", {0.21221f, 0.91805f, 1.10007f}, {3.91324f, 2.45193f, 2.13213f}},
-    {"none", {0.f, 0.f, 0.f}, {1.f, 1.f, 1.f}},
+    {"Cu2O", {3.54426f, 2.94365f, 2.74619f}, {0.11395f, 0.19341f, 0.60475f}},
+    {"Cu2O_palik", {3.54426f, 2.94365f, 2.74619f}, {0.11395f, 0.19341f, 0.60475f}},
+    {"CuO", {3.24062f, 2.44261f, 2.20556f}, {0.51999f, 0.56882f, 0.72064f}},
+    {"CuO_palik", {3.24062f, 2.44261f, 2.20556f}, {0.51999f, 0.56882f, 0.72064f}},
+    {"d-C", {2.90008f, 2.29322f, 2.22275f}, {0.00000f, 0.00000f, 0.00000f}},
+    {"d-C_palik", {2.90008f, 2.29322f, 2.22275f}, {0.00000f, 0.00000f, 0.00000f}},
+    {"Fe", {2.76404f, 1.95417f, 1.62766f}, {3.83077f, 2.73841f, 2.31812f}},
+    {"GaN", {2.86583f, 2.29184f, 2.29848f}, {0.00020f, -0.00017f, 0.00128f}},
+    {"Ge", {7.05832f, 4.78868f, 3.57814f}, {0.50903f, 2.19196f, 2.05716f}},
+    {"Hg", {2.39384f, 1.43697f, 0.90762f}, {6.31420f, 4.36266f, 3.41454f}},
+    {"Hg_palik", {2.39384f, 1.43697f, 0.90762f}, {6.31420f, 4.36266f, 3.41454f}},
+    {"HgTe", {4.76940f, 3.22413f, 2.65439f}, {1.62853f, 1.57746f, 1.72592f}},
+    {"HgTe_palik", {4.76940f, 3.22413f, 2.65439f}, {1.62853f, 1.57746f, 1.72592f}},
+    {"Ir", {3.07986f, 2.07777f, 1.61446f}, {5.58028f, 4.05855f, 3.26033f}},
+    {"Ir_palik", {3.07986f, 2.07777f, 1.61446f}, {5.58028f, 4.05855f, 3.26033f}},
+    {"K", {0.06391f, 0.04631f, 0.03810f}, {2.09975f, 1.34607f, 0.91128f}},
+    {"K_palik", {0.06391f, 0.04631f, 0.03810f}, {2.09975f, 1.34607f, 0.91128f}},
+    {"Li", {0.26525f, 0.19519f, 0.22045f}, {3.53305f, 2.30618f, 1.66505f}},
+    {"Li_palik", {0.26525f, 0.19519f, 0.22045f}, {3.53305f, 2.30618f, 1.66505f}},
+    {"MgO", {2.08521f, 1.64721f, 1.59150f}, {0.00000f, -0.00000f, 0.00000f}},
+    {"MgO_palik", {2.08521f, 1.64721f, 1.59150f}, {0.00000f, -0.00000f, 0.00000f}},
+    {"Mo", {4.47417f, 3.51799f, 2.77018f}, {4.10240f, 3.41361f, 3.14393f}},
+    {"Mo_palik", {4.47417f, 3.51799f, 2.77018f}, {4.10240f, 3.41361f, 3.14393f}},
+    {"Na", {0.06014f, 0.05602f, 0.06186f}, {3.17254f, 2.10800f, 1.57575f}},
+    {"Na_palik", {0.06014f, 0.05602f, 0.06186f}, {3.17254f, 2.10800f, 1.57575f}},
+    {"Nb", {3.41288f, 2.78427f, 2.39051f}, {3.43408f, 2.73183f, 2.57445f}},
+    {"Nb_palik", {3.41288f, 2.78427f, 2.39051f}, {3.43408f, 2.73183f, 2.57445f}},
+    {"Ni", {2.36225f, 1.65983f, 1.46395f}, {4.48929f, 3.04369f, 2.34046f}},
+    {"Ni_palik", {2.36225f, 1.65983f, 1.46395f}, {4.48929f, 3.04369f, 2.34046f}},
+    {"Rh", {2.58031f, 1.85624f, 1.55114f}, {6.76790f, 4.69297f, 3.96766f}},
+    {"Rh_palik", {2.58031f, 1.85624f, 1.55114f}, {6.76790f, 4.69297f, 3.96766f}},
+    {"Se", {4.08037f, 2.83343f, 2.81458f}, {0.75378f, 0.63705f, 0.52047f}},
+    {"Se_palik", {4.08037f, 2.83343f, 2.81458f}, {0.75378f, 0.63705f, 0.52047f}},
+    {"SiC", {3.16562f, 2.52061f, 2.47411f}, {0.00000f, -0.00000f, 0.00001f}},
+    {"SiC_palik", {3.16562f, 2.52061f, 2.47411f}, {0.00000f, -0.00000f, 0.00001f}},
+    {"SnTe", {4.51562f, 1.97692f, 1.27897f}, {0.00000f, 0.00000f, 0.00000f}},
+    {"SnTe_palik", {4.51562f, 1.97692f, 1.27897f}, {0.00000f, 0.00000f, 0.00000f}},
+    {"Ta", {2.05820f, 2.38802f, 2.62250f}, {2.40293f, 1.73767f, 1.94291f}},
+    {"Ta_palik", {2.05820f, 2.38802f, 2.62250f}, {2.40293f, 1.73767f, 1.94291f}},
+    {"Te", {7.37519f, 4.47257f, 2.63149f}, {3.24919f, 3.51992f, 3.28521f}},
+    {"Te_palik", {7.37519f, 4.47257f, 2.63149f}, {3.24919f, 3.51992f, 3.28521f}},
+    {"ThF4", {1.82684f, 1.43917f, 1.38471f}, {0.00000f, 0.00000f, 0.00000f}},
+    {"ThF4_palik", {1.82684f, 1.43917f, 1.38471f}, {0.00000f, 0.00000f, 0.00000f}},
+    {"TiC", {3.69261f, 2.83141f, 2.57683f}, {3.25876f, 2.34656f, 2.16818f}},
+    {"TiC_palik", {3.69261f, 2.83141f, 2.57683f}, {3.25876f, 2.34656f, 2.16818f}},
+    {"TiN", {1.64497f, 1.14800f, 1.37685f}, {3.36132f, 1.93936f, 1.09967f}},
+    {"TiN_palik", {1.64497f, 1.14800f, 1.37685f}, {3.36132f, 1.93936f, 1.09967f}},
+    {"TiO2", {3.44929f, 2.79576f, 2.89899f}, {0.00010f, -0.00009f, 0.00063f}},
+    {"TiO2_palik", {3.44929f, 2.79576f, 2.89899f}, {0.00010f, -0.00009f, 0.00063f}},
+    {"VC", {3.64981f, 2.74689f, 2.52731f}, {3.06184f, 2.19400f, 1.95902f}},
+    {"VC_palik", {3.64981f, 2.74689f, 2.52731f}, {3.06184f, 2.19400f, 1.95902f}},
+    {"V", {4.26843f, 3.50571f, 2.75528f}, {3.48377f, 2.88322f, 3.10511f}},
+    {"V_palik", {4.26843f, 3.50571f, 2.75528f}, {3.48377f, 2.88322f, 3.10511f}},
+    {"VN", {2.85952f, 2.11468f, 1.93597f}, {3.02590f, 2.05174f, 1.61287f}},
+    {"VN_palik", {2.85952f, 2.11468f, 1.93597f}, {3.02590f, 2.05174f, 1.61287f}},
+    {"W", {4.36142f, 3.29330f, 2.99191f}, {3.49325f, 2.59934f, 2.26838f}},
+    {"none", {0.0f, 0.0f, 0.0f}, {1.0f, 1.0f, 1.0f}},
 };
 }  // namespace
 

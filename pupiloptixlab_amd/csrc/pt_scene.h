@@ -75,6 +75,9 @@ public:
     pt_camera_desc camera{};
     uint32_t width = 768, height = 576, max_depth = 1;
     bool any_normals = false, any_texcoords = false;
+    // Loader notes for input the reference accepts with a fallback (unknown
+    // conductor names, BSDF types and shape types).
+    std::vector<std::string> warnings;
 
     uint32_t add_material(const pt_material_desc &m);
     // Flattens one instance: positions by `to_world`, normals by its inverse
@@ -100,6 +103,10 @@ bool load_obj_mesh(const std::string &path, MeshData &mesh, std::string &err);
 // util::BitmapTexture::Load: EXR / Radiance HDR / PNG as RGBA float, row 0 =
 // top row; `err` starts with "io:" for file errors.
 bool load_image(const std::string &path, uint32_t &w, uint32_t &h, std::vector<float> &rgba, std::string &err);
+// Size limit shared by the image readers, checked before any pixel buffer is
+// allocated; false (with `err`) above 2^24 pixels per side or 2^28 in all.
+constexpr uint64_t kMaxImageSide = 1ull << 24, kMaxImagePixels = 1ull << 28;
+bool image_size_ok(uint64_t width, uint64_t height, std::string &err);
 bool write_exr_rgb(const std::string &path, const float *rgba, uint32_t width, uint32_t height, std::string &err);
 
 // Dielectric IOR names (Hecht, Optics, 4th ed.) and conductor eta/k tables

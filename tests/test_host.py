@@ -256,6 +256,58 @@ def test_xml_scene_semantics(tmp_path):
     assert v[0] / v[1] == pytest.approx(2.0, rel=1e-5)
 
 
+CONDUCTOR_XML = """<scene version="3.0.0">
+  <sensor type="perspective"><float name="fov" value="40"/>
+    <film type="hdrfilm"><integer name="width" value="8"/><integer name="height" value="8"/></film></sensor>
+  {shapes}
+</scene>
+"""
+
+
+def test_xml_named_conductors(tmp_path):
+    """render/material/ior.h:59-174: every named metal loads its eta / k (Al,
+    Cr, Fe and Ni among them); an unknown name falls back to the mirror (eta 0,
+    k 1) as material.cpp:72-80 does, and the loader records a warning."""
+    named = {"Al": ((1.65394, 0.87850, 0.52012), (9.20430, 6.25621, 4.82675)),
+             "Cr": ((4.36041, 2.91052, 1.65119), (5.19538, 4.22239, 3.74700)),
+             "Au": ((0.14282, 0.37414, 1.43944), (3.97472, 2.38066, 1.59981)),
+             "none": ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))}
+    shapes = "".join(f'<shape type="rectangle"><bsdf type="{"roughconductor" if i % 2 else "conductor"}">'
+                     f'<string name="material" value="{n}"/></bsdf></shape>'
+                     for i, n in enumerate(list(named) + ["Unobtainium"]))
+    s = pt.HostScene.load_xml(write(tmp_path, "metals.xml", CONDUCTOR_XML.format(shapes=shapes)))
+    m = s.materials()
+    for i, (name, (eta, k)) in enumerate(named.items()):
+        assert tuple(m[i].tex[1].color0) == pytest.approx(eta, abs=1e-6), name
+        assert tuple(m[i].tex[2].color0) == pytest.approx(k, abs=1e-6), name
+    assert tuple(m[4].tex[1].color0) == (0.0, 0.0, 0.0) and tuple(m[4].tex[2].color0) == (1.0, 1.0, 1.0)
+    w = s.warnings()
+    assert len(w) == 1 and "Unobtainium" in w[0]
+
+
+def test_xml_conductor_table_complete():
+    """All 73 conductor entries of ior.h load by name (none of them falls back
+    to the mirror, which only "none" is)."""
+    names = ["a-C", "Ag", "Al", "Al2O3", "AlAs", "AlAs_palik", "AlSb", "AlSb_palik", "Au", "Be", "Be_palik", "Cr",
+             "CsI", "CsI_palik", "Cu", "Cu_palik", "Cu2O", "Cu2O_palik", "CuO", "CuO_palik", "d-C", "d-C_palik",
+             "Fe", "GaN", "Ge", "Hg", "Hg_palik", "HgTe", "HgTe_palik", "Ir", "Ir_palik", "K", "K_palik", "Li",
+             "Li_palik", "MgO", "MgO_palik", "Mo", "Mo_palik", "Na", "Na_palik", "Nb", "Nb_palik", "Ni",
+             "Ni_palik", "Rh", "Rh_palik", "Se", "Se_palik", "SiC", "SiC_palik", "SnTe", "SnTe_palik", "Ta",
+             "Ta_palik", "Te", "Te_palik", "ThF4", "ThF4_palik", "TiC", "TiC_palik", "TiN", "TiN_palik", "TiO2",
+             "TiO2_palik", "VC", "VC_palik", "V", "V_palik", "VN", "VN_palik", "W", "none"]
+    import tempfile
+    import pathlib
+    d = pathlib.Path(tempfile.mkdtemp())
+    shapes = "".join(f'<shape type="rectangle"><bsdf type="conductor"><string name="material" value="{n}"/>'
+                      f'</bsdf></shape>' for n in names)
+    s = pt.HostScene.load_xml(write(d, "all.xml", CONDUCTOR_XML.format(shapes=shapes)))
+    assert s.warnings() == []
+    m = s.materials()
+    assert len(m) == len(names)
+    mirror = [n for n, mm in zip(names, m) if tuple(mm.tex[1].color0) == (0.0, 0.0, 0.0)]
+    assert mirror == ["none"]
+
+
 def test_xml_fov_axis_x_and_lookat(tmp_path):
     xml = """<scene version="3.0.0">
       <sensor type="perspective"><float name="fov" value="90"/>

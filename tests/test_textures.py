@@ -11,6 +11,7 @@ float64 over the bilinearly filtered radiance; the reference's own texel-
 corner sampler (MIS_REFERENCE) is a discrete quadrature, so only the exact
 sampler (MIS_CONSISTENT) is held to the closed form.
 """
+import os
 import struct
 import zlib
 
@@ -555,8 +556,8 @@ def test_image_errors(tmp_path):
     with pytest.raises(pt.PupilError) as e:  # CMYK document
         pt.load_image(write(tmp_path, "c.psd", b"8BPS" + struct.pack(">H6xHIIHH", 1, 4, 2, 2, 8, 4)))
     assert e.value.code == -8
-    with pytest.raises(pt.PupilError) as e:  # header claims 30000 x 30000 RGB, no data follows
-        pt.load_image(write(tmp_path, "t.psd", b"8BPS" + struct.pack(">H6xHIIHH", 1, 3, 30000, 30000, 8, 3)
+    with pytest.raises(pt.PupilError) as e:  # header claims 16000 x 16000 RGB, no data follows
+        pt.load_image(write(tmp_path, "t.psd", b"8BPS" + struct.pack(">H6xHIIHH", 1, 3, 16000, 16000, 8, 3)
                             + struct.pack(">I", 0) * 3 + struct.pack(">H", 0)))
     assert e.value.code == -6
     gif = bytearray(b"GIF89a" + struct.pack("<HHBBB", 4, 4, 0x80, 0, 0) + bytes(6))  # 4x4 screen, 2 colors
@@ -576,6 +577,72 @@ def test_image_errors(tmp_path):
     with pytest.raises(pt.PupilError) as e:
         pt.load_image(str(tmp_path / "missing.exr"))
     assert e.value.code == -5
+
+
+def test_image_size_limits_before_allocation(tmp_path):
+    """Every reader checks the header's size against one limit (2^24 pixels per
+    side, 2^28 in all) before allocating: a 29-byte GIF with a 65535 x 65535
+    screen and a 360 KB PackBits PSD claiming 30000 x 30000 x 4 x 16 bit are
+    rejected at once, and PackBits row counts too short for the row width (or
+    summing past the end of the file) are rejected before the planes exist."""
+    gif = b"GIF89a" + struct.pack("<HHBBB", 65535, 65535, 0, 0, 0) + b"\x2c" + struct.pack("<HHHHB", 0, 0, 1, 1, 0)
+    gif += b"\x02\x02\x4c\x01\x00\x3b"
+    assert len(gif) == 29
+    with pytest.raises(pt.PupilError) as e:
+        pt.load_image(write(tmp_path, "screen.gif", gif))
+    assert e.value.code == -8 and "2^28" in str(e.value)
+
+    def psd_rle(w, h, nch, counts, depth=16, data=b""):
+        head = b"8BPS" + struct.pack(">H6xHIIHH", 1, nch, h, w, depth, 3) + struct.pack(">I", 0) * 3
+        return head + struct.pack(">H", 1) + struct.pack(f">{nch * h}H", *counts) + data
+
+    big = psd_rle(30000, 30000, 4, [1] * (4 * 30000), data=bytes(4 * 30000))
+    assert 350_000 < len(big) < 370_000
+    with pytest.raises(pt.PupilError) as e:
+        pt.load_image(write(tmp_path, "big.psd", big))
+    assert e.value.code == -8
+    # within the size limit: 1-byte rows cannot hold 2 x 4000 bytes each
+    with pytest.raises(pt.PupilError) as e:
+        pt.load_image(write(tmp_path, "short.psd", psd_rle(4000, 4000, 4, [1] * 16000, data=bytes(16000))))
+    assert e.value.code == -6 and "PackBits" in str(e.value)
+    # plausible counts whose sum runs past the end of the file
+    with pytest.raises(pt.PupilError) as e:
+        pt.load_image(write(tmp_path, "trunc.psd", psd_rle(4000, 4000, 3, [126] * 12000, data=bytes(1000))))
+    assert e.value.code == -6 and "truncated" in str(e.value)
+
+
+OOM_SCRIPT = """
+import resource, sys
+sys.path.insert(0, {root!r})
+import pupiloptixlab_amd as pt
+pt.lib()  # map the library before the address space is limited
+vm = [int(l.split()[1]) * 1024 for l in open("/proc/self/status") if l.startswith("VmSize")][0]
+resource.setrlimit(resource.RLIMIT_AS, (vm + (512 << 20), resource.RLIM_INFINITY))
+try:
+    pt.load_image({path!r})
+except pt.PupilError as e:
+    print("code", e.code)
+"""
+
+
+def test_host_allocation_failure_is_reported(tmp_path):
+    """A reader whose pixel buffer cannot be allocated (address space limited
+    to 512 MB above the interpreter's) returns PT_ERR_OOM through the C
+    interface instead of terminating the process."""
+    import subprocess
+    import sys
+    import zlib
+    w = h = 16384  # 2^28 pixels: within the size limit, 4 GiB as float RGBA
+    ihdr = struct.pack(">IIBBBBB", w, h, 8, 2, 0, 0, 0)
+    raw = zlib.compress(b"")
+    chunk = lambda t, d: struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+    png = b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", ihdr) + chunk(b"IDAT", raw) + chunk(b"IEND", b"")
+    path = write(tmp_path, "huge.png", png)
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", OOM_SCRIPT.format(root=root, path=path)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.strip() == "code -7", (r.stdout, r.stderr[-2000:])
 
 
 # ---------------------------------------------------------------- bitmap sampling
