@@ -47,32 +47,60 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene_desc, target_seconds, threads):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_sample(scene_desc, target_seconds, threads):
+    """The oracle on a band of rows through the middle of the terrain frame,
+    sized (rows x spp) to about target_seconds of work on `threads` threads."""
     import oracle
 
     w, h = scene_desc.width, scene_desc.height
-    # Calibrate on a band of rows through the middle of the frame, then size
-    # the sample (rows x spp) to the time budget.
-    rows = 32
+    rows = 8 if threads == 1 else 32
     y0 = h // 2 - rows // 2
     r = oracle.render(scene_desc, spp=1, rows=(y0, y0 + rows), threads=threads)
     per_row_ms = max(r["counts"]["render_ms"], 1e-3) / rows
     budget_rows = target_seconds * 1000.0 / per_row_ms
     spp = int(max(1, min(64, budget_rows // h)))
-    rows = int(max(32, min(h, budget_rows / spp)))
+    rows = int(max(4, min(h, budget_rows / spp)))
     y0 = max(0, h // 2 - rows // 2)
     r = oracle.render(scene_desc, spp=spp, rows=(y0, y0 + rows), threads=threads)
     c = r["counts"]
     samples = rows * w * spp
     secs = c["render_ms"] / 1000.0
+    return {"msamples_per_s": samples / secs / 1e6, "mrays_per_s": (c["rays_extension"] + c["rays_shadow"]) / secs / 1e6,
+            "threads": int(c["threads"]), "rows": (y0, y0 + rows - 1), "spp": spp, "samples": samples,
+            "seconds": secs}
+
+
+def cpu_baseline(scene_desc, target_seconds, threads):
+    """The oracle (scalar C++, the same BVH traversal rules and integrator) on
+    `threads` threads and on one thread, on bounded samples of the same frame."""
+    many = cpu_sample(scene_desc, target_seconds * 0.6, threads)
+    one = cpu_sample(scene_desc, target_seconds * 0.4, 1)
+    visible = os.cpu_count() or 1
     return {
-        "value": samples / secs / 1e6,
+        "value": many["msamples_per_s"],
         "unit": "Msamples/s",
-        "cores": int(c["threads"]),
+        "cores": many["threads"],
         "kind": "port",
-        "sample": (f"oracle (scalar C++, {int(c['threads'])} threads) on the same terrain frame, rows {y0}-"
-                   f"{y0 + rows - 1} of 1080, {spp} spp, depth 16: {samples} samples in {secs:.2f} s, "
-                   f"{(c['rays_extension'] + c['rays_shadow']) / secs / 1e6:.3f} Mrays/s; BVH build excluded"),
+        "sample": (f"oracle (scalar C++, {many['threads']} threads on {visible} cores visible, {cpu_model()}) on "
+                   f"the same terrain frame, rows {many['rows'][0]}-"
+                   f"{many['rows'][1]} of 1080, {many['spp']} spp, depth 16: {many['samples']} samples in "
+                   f"{many['seconds']:.2f} s, {many['mrays_per_s']:.3f} Mrays/s; 1 thread: rows {one['rows'][0]}-"
+                   f"{one['rows'][1]}, {one['spp']} spp, {one['msamples_per_s']:.4f} Msamples/s, "
+                   f"{one['mrays_per_s']:.3f} Mrays/s; BVH build excluded"),
+        "single_core_msamples_per_s": round(one["msamples_per_s"], 5),
+        "cores_visible": visible,
+        "cpu_model": cpu_model(),
     }
 
 
@@ -341,10 +369,11 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
     return out
 
 
-def summarize_pmc(fetch_dir, write_dir, kernel_prefix, out_path):
-    """Per-dispatch HBM bytes of one kernel from two rocprofv3 --pmc runs
+def summarize_pmc(fetch_dir, write_dir, frames_per_launch, out_path, kernels):
+    """Per-dispatch HBM bytes of each kernel from two rocprofv3 --pmc runs
     (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by the factor that
-    maps the counter onto the known bytes of the copy-probe kernel."""
+    maps the counter onto the known bytes of the copy-probe kernel.  The runs
+    render frames_per_launch frames per launch (the bench's launch shape)."""
     import csv
     import glob
 
@@ -358,23 +387,44 @@ def summarize_pmc(fetch_dir, write_dir, kernel_prefix, out_path):
                     vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
         return vals
 
-    out = {"kernel": kernel_prefix}
+    out = {"frames_per_launch": int(frames_per_launch), "kernels": {}}
     for counter, d, key in (("FETCH_SIZE", fetch_dir, "fetch"), ("WRITE_SIZE", write_dir, "write")):
         vals = per_kernel(d, counter)
         copy = [v for k, vs in vals.items() if "k_copy" in k for v in vs]
-        kern = [v for k, vs in vals.items() if kernel_prefix in k for v in vs]
-        if not copy or not kern:
-            raise SystemExit(f"missing {counter} samples for k_copy or {kernel_prefix} in {d}")
-        copy_kib = sum(copy) / len(copy)
-        factor = (COPY_PROBE_BYTES / 1024.0) / copy_kib
-        out[f"{key}_kib_per_dispatch_raw"] = sum(kern) / len(kern)
+        if not copy:
+            raise SystemExit(f"missing {counter} samples for k_copy in {d}")
+        factor = (COPY_PROBE_BYTES / 1024.0) / (sum(copy) / len(copy))
         out[f"{key}_copy_factor"] = factor
-        out[f"{key}_bytes_per_dispatch"] = out[f"{key}_kib_per_dispatch_raw"] * 1024.0 * factor
-        out[f"{key}_dispatches"] = len(kern)
-    out["hbm_bytes_per_launch"] = int(out["fetch_bytes_per_dispatch"] + out["write_bytes_per_dispatch"])
+        for name, prefix in kernels.items():
+            kern = [v for k, vs in vals.items() if prefix in k for v in vs]
+            if not kern:
+                raise SystemExit(f"missing {counter} samples for {prefix} in {d}")
+            e = out["kernels"].setdefault(name, {"kernel": prefix})
+            e[f"{key}_kib_per_dispatch_raw"] = sum(kern) / len(kern)
+            e[f"{key}_bytes_per_dispatch"] = e[f"{key}_kib_per_dispatch_raw"] * 1024.0 * factor
+            e[f"{key}_dispatches"] = len(kern)
+    for e in out["kernels"].values():
+        e["hbm_bytes_per_dispatch"] = int(e["fetch_bytes_per_dispatch"] + e["write_bytes_per_dispatch"])
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_terrain_1080p.json")
+TRAFFIC_KERNELS = {"trace": "k_trace<false, false>", "shade": "k_shade<false>", "shadow": "k_shadow<false, false>"}
+
+
+def measured_traffic(frames_per_launch):
+    """HBM bytes per dispatch of each kernel from the committed counter
+    summary, for launches of frames_per_launch frames: measured at that shape,
+    or scaled linearly in frames from the measured shape (marked so)."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return {}, None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    scale = frames_per_launch / t["frames_per_launch"]
+    note = None if scale == 1 else f"scaled from {t['frames_per_launch']}-frame launches x {scale:.3f}"
+    return {k: v["hbm_bytes_per_dispatch"] * scale for k, v in t["kernels"].items()}, note
 
 
 def run_guided_batch(pt, full):
@@ -385,8 +435,8 @@ def run_guided_batch(pt, full):
 
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-pmc":
-        # bench.py --summarize-pmc FETCH_DIR WRITE_DIR KERNEL OUT_JSON
-        summarize_pmc(*sys.argv[2:6])
+        # bench.py --summarize-pmc FETCH_DIR WRITE_DIR FRAMES_PER_LAUNCH OUT_JSON
+        summarize_pmc(sys.argv[2], sys.argv[3], int(sys.argv[4]), sys.argv[5], TRAFFIC_KERNELS)
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -448,13 +498,15 @@ def main():
     log(f"device copy bandwidth: {copy_gbps:.0f} GB/s (read + write)")
 
     npix = d.width * d.height
+    # Every render writes the albedo / normal / test / primitive-id buffers of
+    # its last frame, as each launch of the reference does (main.cu:97-104).
     if args.warmup:
-        tracer.render(spp=args.warmup, aov=False, spp_per_launch=args.spp_per_launch)
+        tracer.render(spp=args.warmup, aov=True, spp_per_launch=args.spp_per_launch)
     # Counting pass over exactly the frames of the timed run (deterministic rays).
     tracer.reset()
     tracer.set_counting(True)
     before = tracer.stats()
-    tracer.render(spp=args.steps, aov=False, spp_per_launch=args.spp_per_launch)
+    tracer.render(spp=args.steps, aov=True, spp_per_launch=args.spp_per_launch)
     after = tracer.stats()
     counted = {k: after[k] - before[k] for k in ("rays_extension", "rays_shadow", "node_visits", "tri_tests",
                                                   "shadow_node_visits", "shadow_tri_tests")}
@@ -466,7 +518,7 @@ def main():
     tracer.reset()
     base = tracer.stats()
     t_start = time.perf_counter()
-    tracer.render(spp=args.steps, aov=False, timing=True, spp_per_launch=args.spp_per_launch)
+    tracer.render(spp=args.steps, aov=True, timing=True, spp_per_launch=args.spp_per_launch)
     elapsed = time.perf_counter() - t_start
     s = tracer.stats()
     ms = {k: s[k] - base[k] for k in ("ms_raygen", "ms_trace", "ms_shade", "ms_shadow", "ms_accumulate")}
@@ -489,19 +541,37 @@ def main():
     dominant = max(kernels, key=lambda k: kernels[k][0])
     dms, dlaunch, dbytes = kernels[dominant]
     achieved = dbytes / (dms / 1000.0) / 1e9 if dms > 0 else 0.0
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_terrain_{dominant}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    per_kernel = {k: {"ms_total": round(v[0], 3), "launches": v[1],
-                      "ms_per_launch": round(v[0] / max(1, v[1]), 4),
-                      "algorithmic_GBps": round(v[2] / (v[0] / 1000.0) / 1e9, 1) if v[0] > 0 else None}
-                  for k, v in kernels.items()}
+    # Frames per launch of the timed render (pt_render's default width: up to
+    # 64 frames and 128M paths per launch).
+    frames_per_launch = args.spp_per_launch or max(1, min(64, (128 << 20) // npix))
+    frames_per_launch = min(frames_per_launch, args.steps)
+    traffic_by_kernel, traffic_note = measured_traffic(frames_per_launch)
+    per_kernel = {}
+    for k, (kms, kl, kb) in kernels.items():
+        e = {"ms_total": round(kms, 3), "launches": kl, "ms_per_launch": round(kms / max(1, kl), 4),
+             "algorithmic_GBps": round(kb / (kms / 1000.0) / 1e9, 1) if kms > 0 else None,
+             "algorithmic_bytes_per_launch": int(kb / max(1, kl))}
+        if k in traffic_by_kernel and kms > 0:
+            e["hbm_bytes_per_launch"] = int(traffic_by_kernel[k])
+            e["hbm_GBps"] = round(traffic_by_kernel[k] / (kms / max(1, kl) / 1000.0) / 1e9, 1)
+        per_kernel[k] = e
+    traffic = per_kernel[dominant].get("hbm_bytes_per_launch")
+    hbm_gbps = per_kernel[dominant].get("hbm_GBps")
+    # What the counters say bounds the kernel: HBM bandwidth if its measured
+    # traffic runs near the device's copy bandwidth, else latency and issue.
+    if hbm_gbps is None:
+        limiter = "unmeasured (no counter summary)"
+    elif hbm_gbps >= 0.6 * copy_gbps:
+        limiter = f"hbm bandwidth (counter traffic {hbm_gbps:.0f} GB/s)"
+    else:
+        limiter = (f"latency and instruction issue: counter traffic {hbm_gbps:.0f} GB/s is "
+                   f"{hbm_gbps / HBM_PEAK_GBS:.1%} of HBM peak; node fetches are served by L2 / Infinity Cache")
     log(f"timed: {elapsed:.3f}s, {value:.2f} Msamples/s, kernels {json.dumps(per_kernel)}")
 
     cpu = None
     if not args.no_cpu_baseline:
+        # Every core the process sees, at most 16 (a one-GPU job's share of a
+        # host), plus a one-thread figure.
         threads = min(16, os.cpu_count() or 1)
         cpu = cpu_baseline(d, args.cpu_seconds, threads)
         log(f"cpu baseline: {cpu}")
@@ -550,10 +620,15 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_shape": traffic_note or f"measured at this launch shape ({frames_per_launch} frames per launch)",
+            "traffic_GBps": hbm_gbps,
+            "limiter": limiter,
             "algorithmic_bytes_per_launch": int(dbytes / max(1, dlaunch)),
             "avg_launch_ms": round(dms / max(1, dlaunch), 4),
+            "frames_per_launch": frames_per_launch,
             "measured_copy_GBps": round(copy_gbps, 1),
         },
+        "aov_written": True,
         "cpu_baseline": cpu,
         "relmse_guided_vs_plain": guided["ratio"] if guided else None,
         "guided": guided,

@@ -318,7 +318,9 @@ struct LaunchDesc {
 
 // With `views` set (batched render) the path id carries the view above bit
 // `path_bits`; the view selects its emitters, environment and depth.
-template <bool kBatch>
+// kFeat: the scene features compiled in (kFeat*): 0 for triangle scenes with
+// RGB textures and no environment map, kFeatAll otherwise.
+template <bool kBatch, uint32_t kFeat>
 __global__ __launch_bounds__(256, kBatch ? 1 : 5) void k_shade(SceneView scene, ShadeParams prm, PathQueue in,
                                                const uint32_t *count_in, PathQueue out, uint32_t *count_out,
                                                ShadowQueue sq, uint32_t *count_shadow, float4 *out_l,
@@ -355,9 +357,9 @@ __global__ __launch_bounds__(256, kBatch ? 1 : 5) void k_shade(SceneView scene, 
                 sv.env = bv.env;
                 ShadeParams pv = prm;
                 pv.max_depth = bv.max_depth;
-                shade_path(sv, pv, ps, prim, h.x, h.z, h.w, so);
+                shade_path(sv, pv, ps, prim, h.x, h.z, h.w, so, kFeat);
             } else {
-                shade_path(scene, prm, ps, prim, h.x, h.z, h.w, so);
+                shade_path(scene, prm, ps, prim, h.x, h.z, h.w, so, kFeat);
             }
         }
         const uint32_t j = wave_alloc(valid && so.cont, count_out);
@@ -788,6 +790,13 @@ ShadowKernel shadow_kernel(bool count, bool sph) {
     return count ? (sph ? k_shadow<true, true> : k_shadow<true, false>)
                  : (sph ? k_shadow<false, true> : k_shadow<false, false>);
 }
+using ShadeKernel = void (*)(SceneView, ShadeParams, PathQueue, const uint32_t *, PathQueue, uint32_t *, ShadowQueue,
+                            uint32_t *, float4 *, AovBuffers, const BatchViewDev *, uint32_t);
+// The shading kernel for a scene using the kFeat* features `feat`.
+ShadeKernel shade_kernel(bool batch, uint32_t feat) {
+    if (batch) return feat ? k_shade<true, kFeatAll> : k_shade<true, 0u>;
+    return feat ? k_shade<false, kFeatAll> : k_shade<false, 0u>;
+}
 TraceRaysKernel trace_rays_kernel(bool any_hit, bool sph) {
     return any_hit ? (sph ? k_trace_rays<true, true> : k_trace_rays<true, false>)
                    : (sph ? k_trace_rays<false, true> : k_trace_rays<false, false>);
@@ -936,6 +945,7 @@ struct BatchState {
     std::vector<LeafTri> leaf;
     bool any_normals = false, any_texcoords = false;
     bool dirty = true;
+    uint32_t feat = 0;  // kFeat* used by any view
     uint32_t *d_tri = nullptr;
     float4 *d_pos = nullptr, *d_nrm = nullptr;
     float2 *d_uv = nullptr;
@@ -998,6 +1008,7 @@ struct pt_ctx {
     CameraRec camera{};
     uint32_t width = 0, height = 0, scene_max_depth = 1;
     uint32_t num_tris = 0, num_spheres = 0;
+    uint32_t shade_feat = kFeatAll;  // kFeat* of the scene: picks the shading kernel
     // dynamic updates: geometry as given to pt_set_scene, per-shape facts
     std::vector<float> h_pos0, h_nrm0, h_sph_m0;  // 3 / 3 per vertex, 12 per sphere
     std::vector<ShapeHost> shape_host;
@@ -1360,7 +1371,26 @@ struct SceneHostData {
     uint32_t width = 0, height = 0, max_depth = 1, num_tris = 0, num_spheres = 0;
     float lo[3]{}, hi[3]{};
     bool has_normals = false, has_texcoords = false;
+    uint32_t feat = 0;  // kFeat* the scene uses (shading kernel choice)
 };
+
+// The kFeat* features a scene's shading needs.
+uint32_t scene_features(const SceneHostData &d) {
+    uint32_t f = 0;
+    auto tex = [&](const TextureRec &t) {
+        if (t.type == kTexChecker) f |= kFeatChecker;
+        if (t.type == kTexBitmap) f |= kFeatBitmap;
+    };
+    for (const MaterialRec &m : d.h_materials)
+        for (const TextureRec &t : m.tex) tex(t);
+    for (const EmitterRec &e : d.h_emitters) {
+        tex(e.radiance);
+        if (e.type == kEmitterSphere) f |= kFeatSphere;
+    }
+    if (!d.h_spheres.empty()) f |= kFeatSphere;
+    if (d.env.type == kEmitterEnvMap) f |= kFeatEnvMap | kFeatBitmap;
+    return f;
+}
 
 // EnvMapEmitter set-up (world/emitter.cpp:107-149 BuildEnvMapCdfTable and
 // :293-312): per row the running luminance sums normalized to a column cdf,
@@ -1776,6 +1806,7 @@ int convert_scene(const pt_scene_desc *s, const char *who, SceneHostData &out, s
 
     out.has_normals = s->normals != nullptr;
     out.has_texcoords = s->texcoords != nullptr;
+    out.feat = scene_features(out);
     return PT_OK;
 }
 
@@ -2077,8 +2108,8 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
             t.stop(kKindShade);
         } else {
             EventTimer t(ctx, timing);
-            hipLaunchKernelGGL(k_shade<true>, dim3(sg), dim3(sh.shade_block), 0, ctx->stream, sv, sp, ctx->q[cur],
-                               ext + bb, ctx->q[nxt], ext + bb + 1, ctx->sq, shd + bb, ctx->d_out_l, aov,
+            hipLaunchKernelGGL(shade_kernel(true, b.feat), dim3(sg), dim3(sh.shade_block), 0, ctx->stream, sv, sp,
+                               ctx->q[cur], ext + bb, ctx->q[nxt], ext + bb + 1, ctx->sq, shd + bb, ctx->d_out_l, aov,
                                (const BatchViewDev *)b.d_views, path_bits);
             t.stop(kKindShade);
         }
@@ -2338,6 +2369,21 @@ BatchShape batch_shape(pt_ctx *ctx, const pt_batch_params &p) {
 
 }  // namespace
 
+namespace {
+// A host allocation that fails inside an entry point is reported as
+// PT_ERR_OOM (with the entry point named in pt_last_error) instead of
+// escaping the C interface.
+template <typename F>
+int guard_oom(pt_ctx *ctx, const char *who, F &&body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        if (ctx) ctx->error = std::string(who) + ": out of host memory";
+        return PT_ERR_OOM;
+    }
+}
+}  // namespace
+
 extern "C" {
 
 int pt_create(int hip_device, pt_ctx **out) {
@@ -2380,7 +2426,7 @@ int pt_create(int hip_device, pt_ctx **out) {
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_guide_infer, kInferBlock, 0) == hipSuccess && c > 0)
             ctx->infer_blocks_per_cu = (uint32_t)c;
         c = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_shade<false>, 256, 0) == hipSuccess && c > 0)
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_shade<false, kFeatAll>, 256, 0) == hipSuccess && c > 0)
             ctx->shade_blocks_per_cu = (uint32_t)c;
         c = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&c, k_shade_guided<false>, 256, 0) == hipSuccess && c > 0)
@@ -2418,7 +2464,11 @@ void pt_destroy(pt_ctx *ctx) {
 
 const char *pt_last_error(const pt_ctx *ctx) { return ctx ? ctx->error.c_str() : g_create_error.c_str(); }
 
+static int pt_set_scene_unguarded(pt_ctx *ctx, const pt_scene_desc *s);
 int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
+    return guard_oom(ctx, "pt_set_scene", [&] { return pt_set_scene_unguarded(ctx, s); });
+}
+static int pt_set_scene_unguarded(pt_ctx *ctx, const pt_scene_desc *s) {
     if (!ctx) return PT_ERR_INVALID_ARG;
     SceneHostData d;
     int rc = convert_scene(s, "pt_set_scene", d, ctx->error);
@@ -2455,6 +2505,7 @@ int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
     ctx->scene_max_depth = d.max_depth;
     ctx->num_tris = d.num_tris;
     ctx->num_spheres = d.num_spheres;
+    ctx->shade_feat = d.feat;
     for (int k = 0; k < 3; ++k) {
         ctx->scene_lo[k] = d.lo[k];
         ctx->scene_hi[k] = d.hi[k];
@@ -2504,10 +2555,15 @@ int pt_set_scene(pt_ctx *ctx, const pt_scene_desc *s) {
     ctx->sample_cnt = 0;
     ctx->random_seed = 0;
     ctx->stats = pt_stats{};
+    PT_HIP(hipMemset(ctx->d_stat, 0, 5 * sizeof(unsigned long long)));
     return PT_OK;
 }
 
+static int pt_build_bvh_unguarded(pt_ctx *ctx, const pt_bvh_opts *opts);
 int pt_build_bvh(pt_ctx *ctx, const pt_bvh_opts *opts) {
+    return guard_oom(ctx, "pt_build_bvh", [&] { return pt_build_bvh_unguarded(ctx, opts); });
+}
+static int pt_build_bvh_unguarded(pt_ctx *ctx, const pt_bvh_opts *opts) {
     if (!ctx) return PT_ERR_INVALID_ARG;
     if (!ctx->has_scene) {
         ctx->error = "pt_build_bvh: no scene";
@@ -2584,7 +2640,11 @@ int pt_reset_accumulation(pt_ctx *ctx) {
     return PT_OK;
 }
 
+static int pt_set_shape_transform_unguarded(pt_ctx *ctx, uint32_t shape, const float m[12]);
 int pt_set_shape_transform(pt_ctx *ctx, uint32_t shape, const float m[12]) {
+    return guard_oom(ctx, "pt_set_shape_transform", [&] { return pt_set_shape_transform_unguarded(ctx, shape, m); });
+}
+static int pt_set_shape_transform_unguarded(pt_ctx *ctx, uint32_t shape, const float m[12]) {
     if (!ctx || !m) return PT_ERR_INVALID_ARG;
     if (!ctx->has_scene) {
         ctx->error = "pt_set_shape_transform: no scene";
@@ -2757,7 +2817,15 @@ int pt_denoise(pt_ctx *ctx, const pt_denoise_params *params) {
     return PT_OK;
 }
 
+static int pt_denoise_image_unguarded(pt_ctx *ctx, uint32_t w, uint32_t h, const float *rgba, const float *albedo, const float *normal,
+                     float *out, const pt_denoise_params *params);
 int pt_denoise_image(pt_ctx *ctx, uint32_t w, uint32_t h, const float *rgba, const float *albedo, const float *normal,
+                     float *out, const pt_denoise_params *params) {
+    return guard_oom(ctx, "pt_denoise_image", [&] {
+        return pt_denoise_image_unguarded(ctx, w, h, rgba, albedo, normal, out, params);
+    });
+}
+static int pt_denoise_image_unguarded(pt_ctx *ctx, uint32_t w, uint32_t h, const float *rgba, const float *albedo, const float *normal,
                      float *out, const pt_denoise_params *params) {
     if (!ctx || !rgba || !out || w == 0 || h == 0 || (uint64_t)w * h > (1ull << 28)) return PT_ERR_INVALID_ARG;
     DenoiseParams prm;
@@ -2810,7 +2878,11 @@ int pt_set_counting(pt_ctx *ctx, int enable) {
     return PT_OK;
 }
 
+static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p);
 int pt_render(pt_ctx *ctx, const pt_render_params *p) {
+    return guard_oom(ctx, "pt_render", [&] { return pt_render_unguarded(ctx, p); });
+}
+static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
     if (!ctx || !p) return PT_ERR_INVALID_ARG;
     if (!ctx->has_scene) {
         ctx->error = "pt_render: no scene";
@@ -2866,6 +2938,9 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
     int rc = ensure_queues(ctx, per_launch * npix, max_depth);
     if (rc != PT_OK) return rc;
     if (ctx->counting) PT_HIP(hipMemsetAsync(ctx->d_stat, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    // Network queries of this call only (batched renders count into the same
+    // device word).
+    if (guide_infer) PT_HIP(hipMemsetAsync(ctx->d_stat + 4, 0, sizeof(unsigned long long), ctx->stream));
 
     const bool timing = (p->flags & PT_RENDER_TIMING) != 0;
     const SceneView sv = scene_view(ctx);
@@ -2977,9 +3052,9 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
                                        ext + b, ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov, ga,
                                        (const BatchViewDev *)nullptr, 31u, (const LaunchDesc *)nullptr);
                 else
-                    hipLaunchKernelGGL(k_shade<false>, dim3(sg), dim3(256), 0, ctx->stream, sv, sp, ctx->q[cur], ext + b,
-                                       ctx->q[nxt], ext + b + 1, ctx->sq, shd + b, ctx->d_out_l, aov,
-                                       (const BatchViewDev *)nullptr, 31u);
+                    hipLaunchKernelGGL(shade_kernel(false, ctx->shade_feat), dim3(sg), dim3(256), 0, ctx->stream, sv,
+                                       sp, ctx->q[cur], ext + b, ctx->q[nxt], ext + b + 1, ctx->sq, shd + b,
+                                       ctx->d_out_l, aov, (const BatchViewDev *)nullptr, 31u);
                 t.stop(kKindShade);
             }
             {
@@ -3038,7 +3113,7 @@ int pt_render(pt_ctx *ctx, const pt_render_params *p) {
     if (guide_infer) {
         unsigned long long nq = 0;
         PT_HIP(hipMemcpy(&nq, ctx->d_stat + 4, sizeof(nq), hipMemcpyDeviceToHost));
-        ctx->stats.guide_queries = nq;
+        ctx->stats.guide_queries += nq;
     }
     if (guide_train) {
         unsigned long long kept = 0;
@@ -3141,7 +3216,11 @@ int pt_bandwidth_probe(pt_ctx *ctx, uint64_t bytes, uint32_t iters, double *gbps
     return rc;
 }
 
+static int pt_trace_rays_unguarded(pt_ctx *ctx, const float *rays, uint32_t count, int any_hit, void *hits);
 int pt_trace_rays(pt_ctx *ctx, const float *rays, uint32_t count, int any_hit, void *hits) {
+    return guard_oom(ctx, "pt_trace_rays", [&] { return pt_trace_rays_unguarded(ctx, rays, count, any_hit, hits); });
+}
+static int pt_trace_rays_unguarded(pt_ctx *ctx, const float *rays, uint32_t count, int any_hit, void *hits) {
     if (!ctx || (!rays && count) || (!hits && count)) return PT_ERR_INVALID_ARG;
     if (!ctx->has_bvh) {
         ctx->error = "pt_trace_rays: BVH not built";
@@ -3199,7 +3278,11 @@ int guide_ready(pt_ctx *ctx, const char *who) {
 
 }  // namespace
 
+static int pt_guide_enable_unguarded(pt_ctx *ctx, const pt_guide_config *config);
 int pt_guide_enable(pt_ctx *ctx, const pt_guide_config *config) {
+    return guard_oom(ctx, "pt_guide_enable", [&] { return pt_guide_enable_unguarded(ctx, config); });
+}
+static int pt_guide_enable_unguarded(pt_ctx *ctx, const pt_guide_config *config) {
     if (!ctx) return PT_ERR_INVALID_ARG;
     if (!ctx->has_scene) {
         ctx->error = "pt_guide_enable: no scene";
@@ -3218,7 +3301,11 @@ int pt_guide_enable(pt_ctx *ctx, const pt_guide_config *config) {
     return PT_OK;
 }
 
+static int pt_guide_enable_batch_unguarded(pt_ctx *ctx, const pt_guide_config *config);
 int pt_guide_enable_batch(pt_ctx *ctx, const pt_guide_config *config) {
+    return guard_oom(ctx, "pt_guide_enable_batch", [&] { return pt_guide_enable_batch_unguarded(ctx, config); });
+}
+static int pt_guide_enable_batch_unguarded(pt_ctx *ctx, const pt_guide_config *config) {
     if (!ctx) return PT_ERR_INVALID_ARG;
     const BatchState &b = ctx->batch;
     if (b.views.empty()) {
@@ -3371,7 +3458,13 @@ int pt_batch_clear(pt_ctx *ctx) {
     return PT_OK;
 }
 
+static int pt_batch_add_view_unguarded(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uint32_t *view_index);
 int pt_batch_add_view(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uint32_t *view_index) {
+    return guard_oom(ctx, "pt_batch_add_view", [&] {
+        return pt_batch_add_view_unguarded(ctx, scene, spp, view_index);
+    });
+}
+static int pt_batch_add_view_unguarded(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uint32_t *view_index) {
     if (!ctx) return PT_ERR_INVALID_ARG;
     if (spp == 0) {
         ctx->error = "pt_batch_add_view: spp must be positive";
@@ -3415,6 +3508,7 @@ int pt_batch_add_view(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uin
         t.v0.w = u2f(f2u(t.v0.w) + (uint32_t)(leaf_is_sphere(t) ? nsph0 : nt0));
     b.nodes.insert(b.nodes.end(), nodes.begin(), nodes.end());
     b.leaf.insert(b.leaf.end(), leaf.begin(), leaf.end());
+    b.feat |= d.feat;
     b.pos4.insert(b.pos4.end(), d.h_pos4.begin(), d.h_pos4.end());
     if (d.has_normals && !b.any_normals) b.nrm4.assign(nv0 * 4, 0.f);
     if (d.has_texcoords && !b.any_texcoords) b.uv2.assign(nv0 * 2, 0.f);
@@ -3495,7 +3589,11 @@ int pt_batch_add_view(pt_ctx *ctx, const pt_scene_desc *scene, uint32_t spp, uin
     return PT_OK;
 }
 
+static int pt_batch_render_unguarded(pt_ctx *ctx, const pt_batch_params *params);
 int pt_batch_render(pt_ctx *ctx, const pt_batch_params *params) {
+    return guard_oom(ctx, "pt_batch_render", [&] { return pt_batch_render_unguarded(ctx, params); });
+}
+static int pt_batch_render_unguarded(pt_ctx *ctx, const pt_batch_params *params) {
     if (!ctx || !params) return PT_ERR_INVALID_ARG;
     BatchState &b = ctx->batch;
     if (b.views.empty()) {
@@ -3556,7 +3654,11 @@ int pt_batch_read(pt_ctx *ctx, uint32_t view, int which, void *dst, size_t bytes
     return PT_OK;
 }
 
+static int pt_batch_autotune_unguarded(pt_ctx *ctx, const pt_batch_params *params, pt_batch_tuning *out);
 int pt_batch_autotune(pt_ctx *ctx, const pt_batch_params *params, pt_batch_tuning *out) {
+    return guard_oom(ctx, "pt_batch_autotune", [&] { return pt_batch_autotune_unguarded(ctx, params, out); });
+}
+static int pt_batch_autotune_unguarded(pt_ctx *ctx, const pt_batch_params *params, pt_batch_tuning *out) {
     if (!ctx || !params) return PT_ERR_INVALID_ARG;
     BatchState &b = ctx->batch;
     if (b.views.empty()) {

@@ -122,7 +122,10 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideView G, const 
 
     const uint32_t me = base + i;
     const bool in_range = me < nvalid;
-    const size_t slot = ring_n ? ((size_t)first + (size_t)me * ring_m) % ring_n : (size_t)first + me;
+    // Lanes past the batch's valid samples read slot `first` (always filled)
+    // and carry weight 0, so no lane reads past the collected samples.
+    const size_t slot = ring_n ? ((size_t)first + (size_t)(in_range ? me : 0u) * ring_m) % ring_n
+                               : (size_t)first + (in_range ? me : 0u);
     const float4 p4 = samples[3 * slot];
     const float4 d4 = samples[3 * slot + 1];
     const float4 aux = samples[3 * slot + 2];

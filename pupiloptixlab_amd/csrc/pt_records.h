@@ -196,13 +196,24 @@ PT_HD f3 bitmap_sample(const TexArrays &tx, uint32_t index, float u, float v) {
                w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z);
 }
 
+// Scene features a shading kernel is compiled for.  A kernel compiled
+// without a feature drops its code (a scene without bitmap textures, an
+// environment map or spheres shades with fewer registers); the host picks a
+// kernel whose features cover the scene's, so the dropped branches are never
+// taken and the arithmetic of the others is unchanged.
+constexpr uint32_t kFeatBitmap = 1u;   // bitmap textures (materials, emitters, environment map)
+constexpr uint32_t kFeatChecker = 2u;  // checkerboard textures
+constexpr uint32_t kFeatEnvMap = 4u;   // environment map emitter
+constexpr uint32_t kFeatSphere = 8u;   // analytic spheres (shapes and emitters)
+constexpr uint32_t kFeatAll = 15u;
+
 // Texture::Sample (cuda/texture.h:33-57).
-PT_HD f3 texture_sample(const TextureRec &t, f2 uv, const TexArrays &tex) {
-    if (t.type == kTexRgb) return mk3(t.c0.x, t.c0.y, t.c0.z);
+PT_HD f3 texture_sample(const TextureRec &t, f2 uv, const TexArrays &tex, uint32_t feat = kFeatAll) {
+    if (t.type == kTexRgb || !(feat & (kFeatBitmap | kFeatChecker))) return mk3(t.c0.x, t.c0.y, t.c0.z);
     const f4 tc = f4{uv.x, uv.y, 0.f, 1.f};
     float tx = dot(t.r0, tc);
     float ty = dot(t.r1, tc);
-    if (t.type == kTexChecker) {
+    if ((feat & kFeatChecker) && t.type == kTexChecker) {
         tx = tx - (tx > 0.f ? floorf(tx) : ceilf(tx));
         ty = ty - (ty > 0.f ? floorf(ty) : ceilf(ty));
         if (tx < 0.f) tx += 1.f;
@@ -211,6 +222,7 @@ PT_HD f3 texture_sample(const TextureRec &t, f2 uv, const TexArrays &tex) {
         if (tx > 0.5f) return ty > 0.5f ? p1 : p2;
         return ty > 0.5f ? p2 : p1;
     }
+    if (!(feat & kFeatBitmap)) return mk3(t.c0.x, t.c0.y, t.c0.z);
     return bitmap_sample(tex, t.bitmap, tx, ty);
 }
 
@@ -219,7 +231,7 @@ PT_HD bool material_uses_texcoords(const MaterialRec &m) {
 }
 
 // Material::GetLocalBsdf (optix_material.h:119-131) with the per-type GetLocal.
-PT_HD LocalBsdf material_local_bsdf(const MaterialRec &m, f2 uv, const TexArrays &tx) {
+PT_HD LocalBsdf material_local_bsdf(const MaterialRec &m, f2 uv, const TexArrays &tx, uint32_t feat = kFeatAll) {
     LocalBsdf b;
     b.type = m.type;
     b.nonlinear = (m.flags & kMatNonlinear) ? 1u : 0u;
@@ -227,27 +239,27 @@ PT_HD LocalBsdf material_local_bsdf(const MaterialRec &m, f2 uv, const TexArrays
     b.eta = m.eta;
     b.int_fdr = m.int_fdr;
     b.spec_weight = m.spec_weight;
-    b.c0 = texture_sample(m.tex[0], uv, tx);
+    b.c0 = texture_sample(m.tex[0], uv, tx, feat);
     b.c1 = mk3(0.f);
     b.c2 = mk3(0.f);
     switch (m.type) {
         case kMatDielectric:
         case kMatPlastic:
-            b.c1 = texture_sample(m.tex[1], uv, tx);
+            b.c1 = texture_sample(m.tex[1], uv, tx, feat);
             break;
         case kMatRoughDielectric:
         case kMatRoughPlastic:
-            b.c1 = texture_sample(m.tex[1], uv, tx);
-            b.alpha = texture_sample(m.tex[2], uv, tx).x;
+            b.c1 = texture_sample(m.tex[1], uv, tx, feat);
+            b.alpha = texture_sample(m.tex[2], uv, tx, feat).x;
             break;
         case kMatConductor:
-            b.c1 = texture_sample(m.tex[1], uv, tx);
-            b.c2 = texture_sample(m.tex[2], uv, tx);
+            b.c1 = texture_sample(m.tex[1], uv, tx, feat);
+            b.c2 = texture_sample(m.tex[2], uv, tx, feat);
             break;
         case kMatRoughConductor:
-            b.c1 = texture_sample(m.tex[1], uv, tx);
-            b.c2 = texture_sample(m.tex[2], uv, tx);
-            b.alpha = texture_sample(m.tex[3], uv, tx).x;
+            b.c1 = texture_sample(m.tex[1], uv, tx, feat);
+            b.c2 = texture_sample(m.tex[2], uv, tx, feat);
+            b.alpha = texture_sample(m.tex[3], uv, tx, feat).x;
             break;
         default:
             break;
@@ -266,7 +278,7 @@ struct EmitterSample {
 
 // TriAreaEmitter::SampleDirect (area.h:17-35).
 PT_HD void area_sample_direct(const EmitterRec &e, const LocalGeometry &hit, f2 xi, EmitterSample &ret,
-                              const TexArrays &tx) {
+                              const TexArrays &tx, uint32_t feat = kFeatAll) {
     f3 t = uniform_sample_triangle(xi.x, xi.y);
     f3 p0 = mk3(e.p0.x, e.p0.y, e.p0.z), p1 = mk3(e.p1.x, e.p1.y, e.p1.z), p2 = mk3(e.p2.x, e.p2.y, e.p2.z);
     f3 n0 = mk3(e.n0.x, e.n0.y, e.n0.z), n1 = mk3(e.n1.x, e.n1.y, e.n1.z), n2 = mk3(e.n2.x, e.n2.y, e.n2.z);
@@ -274,7 +286,7 @@ PT_HD void area_sample_direct(const EmitterRec &e, const LocalGeometry &hit, f2 
     f3 normal = normalize(n0 * t.x + n1 * t.y + n2 * t.z);
     f2 uv0 = mk2(e.uv01.x, e.uv01.y), uv1 = mk2(e.uv01.z, e.uv01.w), uv2 = mk2(e.uv2.x, e.uv2.y);
     f2 tex = uv0 * t.x + uv1 * t.y + uv2 * t.z;
-    ret.radiance = texture_sample(e.radiance, tex, tx);
+    ret.radiance = texture_sample(e.radiance, tex, tx, feat);
     ret.wi = normalize(position - hit.position);
     float nol = dot(hit.normal, ret.wi);
     float lnol = dot(normal, -ret.wi);
@@ -290,11 +302,11 @@ PT_HD void area_sample_direct(const EmitterRec &e, const LocalGeometry &hit, f2 
 // SphereEmitter::SampleDirect (sphere.h:14-33): a uniform point on the whole
 // sphere, texcoord from the world-frame direction.
 PT_HD void sphere_sample_direct(const EmitterRec &e, const LocalGeometry &hit, f2 xi, EmitterSample &ret,
-                                const TexArrays &tx) {
+                                const TexArrays &tx, uint32_t feat = kFeatAll) {
     const f3 t = uniform_sample_sphere(xi.x, xi.y);
     const f3 position = t * e.p0.w + mk3(e.p0.x, e.p0.y, e.p0.z);
     const f3 normal = normalize(t);
-    ret.radiance = texture_sample(e.radiance, sphere_texcoord(t), tx);
+    ret.radiance = texture_sample(e.radiance, sphere_texcoord(t), tx, feat);
     ret.wi = normalize(position - hit.position);
     const float nol = dot(hit.normal, ret.wi);
     const float lnol = dot(normal, -ret.wi);
@@ -309,17 +321,17 @@ PT_HD void sphere_sample_direct(const EmitterRec &e, const LocalGeometry &hit, f
 
 // Emitter::SampleDirect (render/emitter.h:74-90) for the area kinds.
 PT_HD void emitter_sample_direct(const EmitterRec &e, const LocalGeometry &hit, f2 xi, EmitterSample &ret,
-                                 const TexArrays &tx) {
-    if (e.type == kEmitterSphere) sphere_sample_direct(e, hit, xi, ret, tx);
-    else area_sample_direct(e, hit, xi, ret, tx);
+                                 const TexArrays &tx, uint32_t feat = kFeatAll) {
+    if ((feat & kFeatSphere) && e.type == kEmitterSphere) sphere_sample_direct(e, hit, xi, ret, tx, feat);
+    else area_sample_direct(e, hit, xi, ret, tx, feat);
 }
 
 // Solid-angle density with which emitter_sample_direct produces the emitter
 // point `p` (barycentrics b1, b2 on a triangle) seen from `o`: the emitter's
 // own outward normal, as in SampleDirect, not the shading normal of the hit.
-PT_HD float emitter_nee_pdf(const EmitterRec &e, f3 p, float b1, float b2, f3 o) {
+PT_HD float emitter_nee_pdf(const EmitterRec &e, f3 p, float b1, float b2, f3 o, uint32_t feat = kFeatAll) {
     f3 en;
-    if (e.type == kEmitterSphere) {
+    if ((feat & kFeatSphere) && e.type == kEmitterSphere) {
         en = normalize(p - mk3(e.p0.x, e.p0.y, e.p0.z));
     } else {
         const float wv = 1.f - b1 - b2;
@@ -336,7 +348,7 @@ PT_HD float emitter_nee_pdf(const EmitterRec &e, f3 p, float b1, float b2, f3 o)
 // 35-43), which are the same function of the hit geometry; pdf stays 0 when
 // the hit faces away.
 PT_HD void area_eval(const EmitterRec &e, const LocalGeometry &emit_geo, f3 scatter_pos, f3 &radiance, float &pdf,
-                     const TexArrays &tx) {
+                     const TexArrays &tx, uint32_t feat = kFeatAll) {
     f3 dir = normalize(scatter_pos - emit_geo.position);
     float lnol = dot(emit_geo.normal, dir);
     pdf = 0.f;
@@ -344,7 +356,7 @@ PT_HD void area_eval(const EmitterRec &e, const LocalGeometry &emit_geo, f3 scat
     if (lnol > 0.f) {
         float distance = length(scatter_pos - emit_geo.position);
         pdf = distance * distance / (lnol * e.area);
-        radiance = texture_sample(e.radiance, emit_geo.texcoord, tx);
+        radiance = texture_sample(e.radiance, emit_geo.texcoord, tx, feat);
     }
 }
 
@@ -471,8 +483,8 @@ PT_HD void env_map_sample_exact(const EnvRec &e, f2 xi, EmitterSample &ret, cons
 
 // Next-event sample of the environment (ConstEnv or EnvMap).
 PT_HD void env_sample_direct(const EnvRec &env, const LocalGeometry &hit, f2 xi, bool exact, EmitterSample &ret,
-                             const TexArrays &tx) {
-    if (env.type == kEmitterEnvMap) {
+                             const TexArrays &tx, uint32_t feat = kFeatAll) {
+    if ((feat & kFeatEnvMap) && env.type == kEmitterEnvMap) {
         if (exact) env_map_sample_exact(env, xi, ret, tx);
         else env_map_sample_reference(env, xi, ret, tx);
     } else {

@@ -95,8 +95,8 @@ PT_HD void sphere_hit_geometry(const SceneView &s, uint32_t k, float t, f3 ray_o
 // Reconstructs the hit (render/geometry.h:60-100, 176-180) in world space.
 // Triangles use the barycentrics (b1, b2), spheres the ray and its t.
 PT_HD void hit_geometry(const SceneView &s, uint32_t prim, float t_hit, float b1, float b2, f3 ray_o, f3 ray_dir,
-                        LocalGeometry &g, uint32_t &material, int32_t &emitter_index) {
-    if (prim & kPrimSphere) {
+                        LocalGeometry &g, uint32_t &material, int32_t &emitter_index, uint32_t feat = kFeatAll) {
+    if ((feat & kFeatSphere) && (prim & kPrimSphere)) {
         uint32_t flags;
         sphere_hit_geometry(s, prim & kPrimIndexMask, t_hit, ray_o, ray_dir, g, material, emitter_index, flags);
         if (flags & kShapeFlipNormals) g.normal = g.normal * -1.f;
@@ -150,8 +150,10 @@ PT_HD int32_t select_emitter(const SceneView &s, float p) {
     return -2;
 }
 
+// `feat`: the scene features compiled in (kFeat*; a constant of the calling
+// kernel, so the code of absent features is dropped).
 PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps, uint32_t prim, float t_hit,
-                      float b1, float b2, ShadeOut &out) {
+                      float b1, float b2, ShadeOut &out, uint32_t feat = kFeatAll) {
     out.cont = false;
     out.done = false;
     out.shadow = false;
@@ -169,7 +171,7 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
             env_rad = mk3(s.env.color.x, s.env.color.y, s.env.color.z);
             env_pdf = 0.25f * kInvPi;
             light_pdf = (ps.flags & kPathEnvUpper) ? s.env.select_prob * (kInvPi * 0.5f) : 0.f;
-        } else if (s.env.type == kEmitterEnvMap) {
+        } else if ((feat & kFeatEnvMap) && s.env.type == kEmitterEnvMap) {
             if (consistent) {
                 float p;
                 env_map_eval(s.env, ps.d, true, s.tex, env_rad, p);
@@ -203,12 +205,13 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
     LocalGeometry g;
     uint32_t mat_index;
     int32_t emitter_index;
-    hit_geometry(s, prim, t_hit, b1, b2, ps.o, ps.d, g, mat_index, emitter_index);
+    hit_geometry(s, prim, t_hit, b1, b2, ps.o, ps.d, g, mat_index, emitter_index, feat);
     const MaterialRec &mat = s.materials[mat_index];
-    const LocalBsdf bsdf = material_local_bsdf(mat, g.texcoord, s.tex);
+    const LocalBsdf bsdf = material_local_bsdf(mat, g.texcoord, s.tex, feat);
 
     if (primary) {
-        if (emitter_index >= 0) ps.L = ps.L + texture_sample(s.emitters[emitter_index].radiance, g.texcoord, s.tex);
+        if (emitter_index >= 0)
+            ps.L = ps.L + texture_sample(s.emitters[emitter_index].radiance, g.texcoord, s.tex, feat);
         if (prm.aov) {
             out.wrote_aov = true;
             out.albedo = bsdf.c0;
@@ -219,11 +222,11 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
         const EmitterRec &e = s.emitters[emitter_index];
         f3 rad;
         float epdf;
-        area_eval(e, g, ps.o, rad, epdf, s.tex);
+        area_eval(e, g, ps.o, rad, epdf, s.tex, feat);
         if (!is_zero(epdf)) {
             // Consistent mode weighs against the pdf next-event estimation
             // would assign to this point, which uses the emitter's own normals.
-            const float lpdf = consistent ? emitter_nee_pdf(e, g.position, b1, b2, ps.o) : epdf;
+            const float lpdf = consistent ? emitter_nee_pdf(e, g.position, b1, b2, ps.o, feat) : epdf;
             float mis = (ps.flags & kPathPrevDelta) ? 1.f : mis_weight(ps.prev_pdf, lpdf * e.select_prob);
             ps.L = ps.L + ps.T * rad * mis;
         }
@@ -252,10 +255,10 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
         es.pdf = 0.f;
         float sel_prob = 0.f;
         if (sel >= 0) {
-            emitter_sample_direct(s.emitters[sel], g, mk2(x, y), es, s.tex);
+            emitter_sample_direct(s.emitters[sel], g, mk2(x, y), es, s.tex, feat);
             sel_prob = s.emitters[sel].select_prob;
         } else if (sel == -1) {
-            env_sample_direct(s.env, g, mk2(x, y), consistent, es, s.tex);
+            env_sample_direct(s.env, g, mk2(x, y), consistent, es, s.tex, feat);
             sel_prob = s.env.select_prob;
         }
         if (es.pdf != 0.f) {
