@@ -272,6 +272,24 @@ int pt_refit_bvh(pt_ctx *ctx);
 /* PTPass::OnRun x spp (pt_pass.cpp:39-57) + optix::Pass::Run
  * (optix/pass.h:71-89): renders `spp` frames; blocks until done. */
 int pt_render(pt_ctx *ctx, const pt_render_params *params);
+/* Allocation-free variant of pt_render.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It renders
+ * like pt_render (same kernels, identical images for the same frames) but
+ * takes its path queues, queue counters and traversal stack spill area from a
+ * 640 MiB array that the library holds in device memory per GPU for the life
+ * of the process (resolved once per device in pt_create), calls no hipMalloc,
+ * hipFree or other allocating routine, and does not synchronize: it returns
+ * once the work is enqueued on the context's stream (pt_read_buffer waits for
+ * it).  Frames per launch = spp_per_launch, or as many as the array holds;
+ * one frame must fit (paths of 224 bytes: about 2.4M pixels; 1920x1080 fits),
+ * else PT_ERR_CAPACITY with the limit in pt_last_error.  Guided rendering and
+ * PT_RENDER_TIMING give PT_ERR_UNSUPPORTED; pending pt_set_shape_transform
+ * updates must be refit first (pt_refit_bvh).  Ray counts in pt_stats are not
+ * updated.  Every call on one GPU shares that GPU's array: calls that overlap
+ * on one GPU (several contexts or threads) race and must be serialized by the
+ * caller; each call clears the counters it reads. */
+int pt_render_noalloc(pt_ctx *ctx, const pt_render_params *params);
 /* BufferManager::GetBuffer + device-to-host copy (system/buffer.cpp:34-99). */
 int pt_read_buffer(pt_ctx *ctx, int which, void *host_dst, size_t bytes);
 int pt_get_stats(pt_ctx *ctx, pt_stats *out);

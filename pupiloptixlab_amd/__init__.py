@@ -186,6 +186,7 @@ _SIGNATURES = {
                                    C.c_void_p, C.c_void_p]),
     "pt_refit_bvh": (C.c_int, [C.c_void_p]),
     "pt_render": (C.c_int, [C.c_void_p, C.POINTER(RenderParams)]),
+    "pt_render_noalloc": (C.c_int, [C.c_void_p, C.POINTER(RenderParams)]),
     "pt_read_buffer": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t]),
     "pt_get_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "pt_set_counting": (C.c_int, [C.c_void_p, C.c_int]),

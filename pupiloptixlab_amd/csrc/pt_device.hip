@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1090,6 +1091,31 @@ struct pt_ctx {
 namespace {
 
 thread_local std::string g_create_error;
+
+// Device memory of pt_render_noalloc: path queues, shadow queue, radiance
+// slots, queue counters and the traversal stack spill area of one render call,
+// carved out of a file-scope array instead of hipMalloc.  Each GPU that loads
+// the library holds its own copy until the process ends.  In host code the
+// array's name is not a device address: its per-device address comes from
+// hipGetSymbolAddress (kept in g_scratch_ptr) and only that is handed to
+// kernels.
+constexpr size_t kNoAllocScratchBytes = 640ull << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Resolves this device's copy of `scratch` once (the device must be current).
+hipError_t resolve_scratch(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<unsigned char *>(ptr);
+    return hipSuccess;
+}
 
 #define PT_HIP(call)                                                                         \
     do {                                                                                    \
@@ -2433,6 +2459,15 @@ int pt_create(int hip_device, pt_ctx **out) {
             ctx->shade_guided_blocks_per_cu = (uint32_t)c;
 
     }
+    {
+        const hipError_t e = resolve_scratch(hip_device);
+        if (e != hipSuccess) {
+            g_create_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            (void)hipStreamDestroy(ctx->stream);
+            delete ctx;
+            return PT_ERR_HIP;
+        }
+    }
     if (hipMalloc((void **)&ctx->d_stat, 5 * sizeof(unsigned long long)) != hipSuccess ||
         hipMemset(ctx->d_stat, 0, 5 * sizeof(unsigned long long)) != hipSuccess) {
         g_create_error = "hipMalloc failed";
@@ -3133,6 +3168,159 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
         ctx->stats.shadow_tri_tests += st[3];
     }
     ctx->stats.ms_total += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    return PT_OK;
+}
+
+// Carves one call's buffers out of the scratch array: counters, the
+// traversal spill area for `spill_threads` lanes, then two path queues, the
+// shadow queue and the radiance slots for as many paths as fit.
+struct NoAllocLayout {
+    PathQueue q[2];
+    ShadowQueue sq;
+    float4 *out_l;
+    uint32_t *counters;
+    uint2 *spill;
+    uint32_t capacity;  // paths
+};
+
+bool noalloc_layout(unsigned char *base, uint32_t num_counters, uint32_t spill_threads, NoAllocLayout &L) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char *p = base + off;
+        off += (bytes + 255) & ~(size_t)255;
+        return p;
+    };
+    L.counters = reinterpret_cast<uint32_t *>(take((size_t)num_counters * sizeof(uint32_t)));
+    L.spill = reinterpret_cast<uint2 *>(take((size_t)spill_threads * kSpillEntries * sizeof(uint2)));
+    if (off >= kNoAllocScratchBytes) return false;
+    // 10 path-queue arrays, 3 shadow arrays and the radiance slots: 14
+    // float4 per path, each array rounded up to 256 bytes.
+    const size_t left = kNoAllocScratchBytes - off - 14 * 256;
+    L.capacity = (uint32_t)std::min<size_t>(left / (14 * sizeof(float4)), 0x7FFFFFFFull);
+    const size_t n = L.capacity;
+    for (auto &q : L.q) {
+        q.ray0 = reinterpret_cast<float4 *>(take(n * sizeof(float4)));
+        q.ray1 = reinterpret_cast<float4 *>(take(n * sizeof(float4)));
+        q.tl = reinterpret_cast<float4 *>(take(n * sizeof(float4)));
+        q.lr = reinterpret_cast<float4 *>(take(n * sizeof(float4)));
+        q.hit = reinterpret_cast<float4 *>(take(n * sizeof(float4)));
+    }
+    L.sq.s0 = reinterpret_cast<float4 *>(take(n * sizeof(float4)));
+    L.sq.s1 = reinterpret_cast<float4 *>(take(n * sizeof(float4)));
+    L.sq.s2 = reinterpret_cast<float4 *>(take(n * sizeof(float4)));
+    L.out_l = reinterpret_cast<float4 *>(take(n * sizeof(float4)));
+    return off <= kNoAllocScratchBytes;
+}
+
+// Traversal blocks per CU of the allocation-free render (its spill area is
+// sized for this grid).
+constexpr uint32_t kNoAllocTraceBlocksPerCu = 8;
+
+int pt_render_noalloc(pt_ctx *ctx, const pt_render_params *p) {
+    if (!ctx || !p) return PT_ERR_INVALID_ARG;
+    if (!ctx->has_scene) {
+        ctx->error = "pt_render_noalloc: no scene";
+        return PT_ERR_NO_SCENE;
+    }
+    if (!ctx->has_bvh) {
+        ctx->error = "pt_render_noalloc: BVH not built";
+        return PT_ERR_NO_BVH;
+    }
+    if (ctx->bvh_dirty) {
+        ctx->error = "pt_render_noalloc: shape updates pending (call pt_refit_bvh first)";
+        return PT_ERR_INVALID_ARG;
+    }
+    if (p->mis_mode > PT_MIS_CONSISTENT) {
+        ctx->error = "pt_render_noalloc: unknown MIS mode";
+        return PT_ERR_INVALID_ARG;
+    }
+    if (p->flags & (PT_RENDER_GUIDE | PT_RENDER_GUIDE_TRAIN | PT_RENDER_TIMING)) {
+        ctx->error = "pt_render_noalloc: guided rendering and timing are not available without allocation";
+        return PT_ERR_UNSUPPORTED;
+    }
+    const int dev = ctx->device;
+    unsigned char *base = (dev >= 0 && dev < kMaxScratchDevices) ? g_scratch_ptr[dev] : nullptr;
+    if (!base) {
+        ctx->error = "pt_render_noalloc: scratch memory of this device was not resolved";
+        return PT_ERR_HIP;
+    }
+    const uint32_t npix = ctx->width * ctx->height;
+    const uint32_t max_depth = p->max_depth ? p->max_depth : ctx->scene_max_depth;
+    if (max_depth > 250) {
+        ctx->error = "pt_render_noalloc: max_depth above 250";
+        return PT_ERR_INVALID_ARG;
+    }
+    const uint32_t num_counters = 4 * (max_depth + 2);
+    const uint32_t tbpc = std::min(kNoAllocTraceBlocksPerCu, ctx->num_spheres > 0 ? ctx->trace_blocks_per_cu
+                                                                                 : ctx->trace_blocks_per_cu_tri);
+    const uint32_t sbpc = std::min(kNoAllocTraceBlocksPerCu, ctx->num_spheres > 0 ? ctx->shadow_blocks_per_cu
+                                                                                 : ctx->shadow_blocks_per_cu_tri);
+    const uint32_t tg = (uint32_t)ctx->num_cu * tbpc, shg = (uint32_t)ctx->num_cu * sbpc;
+    NoAllocLayout L;
+    if (!noalloc_layout(base, num_counters, std::max(tg, shg) * kTraceBlock, L) || L.capacity < npix) {
+        ctx->error = "pt_render_noalloc: one frame of " + std::to_string(npix) + " paths exceeds the " +
+                     std::to_string(kNoAllocScratchBytes >> 20) + " MiB scratch (" + std::to_string(L.capacity) +
+                     " paths of 224 bytes)";
+        return PT_ERR_CAPACITY;
+    }
+    uint32_t per_launch = p->spp_per_launch ? p->spp_per_launch : L.capacity / npix;
+    per_launch = std::max(1u, std::min(per_launch, std::max(1u, p->spp)));
+    if ((uint64_t)per_launch * npix > L.capacity) {
+        ctx->error = "pt_render_noalloc: spp_per_launch * pixels exceeds the scratch capacity of " +
+                     std::to_string(L.capacity) + " paths";
+        return PT_ERR_CAPACITY;
+    }
+    (void)hipSetDevice(ctx->device);
+    const SceneView sv = scene_view(ctx);
+    const DeviceBvh bvh{ctx->d_nodes, ctx->d_leaf, L.spill};
+    ShadeParams sp;
+    sp.max_depth = max_depth;
+    sp.rr_depth = p->rr_depth;
+    sp.mis_mode = p->mis_mode;
+    sp.aov = (p->flags & PT_RENDER_AOV) ? 1u : 0u;
+    const uint32_t sg = shade_grid(ctx, false, false);
+    uint32_t done = 0, sample_cnt = ctx->sample_cnt, seed = ctx->random_seed;
+    while (done < p->spp) {
+        const uint32_t frames = std::min(per_launch, p->spp - done);
+        const uint32_t total = frames * npix;
+        uint32_t *ext = L.counters, *shd = L.counters + (max_depth + 2);
+        uint32_t *twork = L.counters + 2 * (max_depth + 2), *swork = L.counters + 3 * (max_depth + 2);
+        // k_raygen clears every counter before the first traversal reads them.
+        const uint32_t rgrid = std::min<uint32_t>((total + 255) / 256, (uint32_t)ctx->num_cu * 32u);
+        hipLaunchKernelGGL(k_raygen, dim3(rgrid), dim3(256), 0, ctx->stream, L.q[0], ctx->camera, ctx->width,
+                           ctx->height, frames, seed, L.counters, num_counters);
+        AovBuffers aov{nullptr, nullptr, nullptr, nullptr, frames - 1, npix};
+        if (sp.aov) {
+            aov.albedo = ctx->d_albedo;
+            aov.normal = ctx->d_normal;
+            aov.test = ctx->d_test;
+            aov.prim = ctx->d_prim;
+        }
+        int cur = 0;
+        for (uint32_t b = 0; b < max_depth; ++b) {
+            const int nxt = cur ^ 1;
+            hipLaunchKernelGGL(trace_kernel(false, ctx->num_spheres > 0), dim3(tg), dim3(kTraceBlock), 0, ctx->stream,
+                               L.q[cur], bvh, ext + b, twork + b, ctx->d_stat, (const uint32_t *)nullptr, 31u);
+            hipLaunchKernelGGL(shade_kernel(false, ctx->shade_feat), dim3(sg), dim3(256), 0, ctx->stream, sv, sp,
+                               L.q[cur], ext + b, L.q[nxt], ext + b + 1, L.sq, shd + b, L.out_l, aov,
+                               (const BatchViewDev *)nullptr, 31u);
+            hipLaunchKernelGGL(shadow_kernel(false, ctx->num_spheres > 0), dim3(shg), dim3(kTraceBlock), 0,
+                               ctx->stream, L.sq, bvh, shd + b, swork + b, L.q[nxt], L.out_l, ctx->d_stat + 2,
+                               (const uint32_t *)nullptr, 31u);
+            cur = nxt;
+        }
+        const uint32_t agrid = std::min<uint32_t>((npix + 255) / 256, (uint32_t)ctx->num_cu * 16u);
+        hipLaunchKernelGGL(k_accumulate, dim3(agrid), dim3(256), 0, ctx->stream, L.out_l, ctx->d_accum, ctx->d_frame,
+                           (p->flags & PT_RENDER_FP64_ACCUM) ? ctx->d_accum64 : nullptr, npix, frames, sample_cnt,
+                           p->accumulate);
+        PT_HIP(hipGetLastError());
+        if (p->accumulate) sample_cnt += frames;
+        seed += frames;
+        done += frames;
+    }
+    ctx->sample_cnt = sample_cnt;
+    ctx->random_seed = seed;
+    ctx->stats.samples += (uint64_t)p->spp * npix;
     return PT_OK;
 }
 
