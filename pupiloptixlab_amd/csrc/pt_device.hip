@@ -135,6 +135,25 @@ __device__ __forceinline__ f3 f3of(float4 v) { return mk3(v.x, v.y, v.z); }
 
 // --- raygen ---------------------------------------------------------------------
 // main.cu:39-78 for every (frame, pixel) of the launch.
+// Queue order of a frame's camera paths: slot j of the frame traces pixel
+// tiled_pixel(j).  Full 8x8 tiles come first (one tile per 64-lane wave, so a
+// wave's rays leave from one compact patch of the film and stay together
+// through the compaction of later bounces), then the pixels of a partial
+// right strip and bottom strip in row order.  The path id keeps the pixel
+// (frame * pixels + pixel), so films, AOVs and random streams do not depend
+// on the order.
+__device__ __forceinline__ uint32_t tiled_pixel(uint32_t j, uint32_t w, uint32_t h) {
+    const uint32_t w8 = w & ~7u, h8 = h & ~7u, tiled = w8 * h8;
+    if (j < tiled) {
+        const uint32_t t = j >> 6, k = j & 63u, tiles_per_row = w8 >> 3;
+        return ((t / tiles_per_row) * 8u + (k >> 3)) * w + (t % tiles_per_row) * 8u + (k & 7u);
+    }
+    j -= tiled;
+    const uint32_t rw = w - w8;
+    if (j < rw * h8) return (j / rw) * w + w8 + j % rw;
+    return h8 * w + (j - rw * h8);
+}
+
 __global__ void k_raygen(PathQueue q, CameraRec cam, uint32_t width, uint32_t height, uint32_t frames,
                          uint32_t frame_seed0, uint32_t *counters, uint32_t num_counters) {
     const uint32_t npix = width * height;
@@ -146,14 +165,14 @@ __global__ void k_raygen(PathQueue q, CameraRec cam, uint32_t width, uint32_t he
     }
     for (uint32_t i = tid; i < total; i += gridDim.x * blockDim.x) {
         const uint32_t f = i / npix;
-        const uint32_t pixel = i - f * npix;
+        const uint32_t pixel = tiled_pixel(i - f * npix, width, height);
         const uint32_t x = pixel % width, y = pixel / width;
         Rng rng = rng_init(pixel, frame_seed0 + f);
         const float jx = rng_next(rng);
         const float jy = rng_next(rng);
         f3 o, d;
         camera_ray(cam, x, y, width, height, jx, jy, o, d);
-        q.ray0[i] = make_float4(o.x, o.y, o.z, u2f(i));
+        q.ray0[i] = make_float4(o.x, o.y, o.z, u2f(f * npix + pixel));
         q.ray1[i] = make_float4(d.x, d.y, d.z, 0.f);
         q.tl[i] = make_float4(1.f, 1.f, 1.f, 0.f);
         q.lr[i] = make_float4(0.f, 0.f, u2f(rng.seed), u2f(kPathPrimary));
@@ -287,7 +306,9 @@ struct BatchViewDev {
     CameraRec camera;
     EnvRec env;
     uint32_t width, height, max_depth, root;
-    uint32_t emitter_begin, emitter_count, pad0, pad1;
+    uint32_t emitter_begin, emitter_count;
+    uint32_t guide_off;  // guided batches: the view's guide was gated off (its paths sample the BSDF)
+    uint32_t pad1;
     float4 *accum, *frame;
     double *accum64;
     void *pad2;
@@ -298,6 +319,9 @@ struct BatchViewDev {
 constexpr uint32_t kMaxLaunchEntries = 1024;  // frames per launch
 constexpr uint32_t kMaxLaunchSegs = 256;      // runs of consecutive frames of one view per launch
 constexpr uint32_t kEntryTrain = 0x80000000u; // LaunchEntry::view flag: the frame trains the guide
+constexpr uint32_t kEntryPlain = 0x40000000u; // LaunchEntry::view flag: the frame samples the BSDF only
+constexpr uint32_t kEntryEval = 0x20000000u;  // LaunchEntry::view flag: the frame measures its view's error
+constexpr uint32_t kEntryViewMask = 0x1FFFFFFFu;
 
 // Frame `frame_seed` (RNG frame index) of a view occupies the launch's paths
 // [path_begin, path_begin + pixels).
@@ -471,8 +495,9 @@ __global__ __launch_bounds__(kInferBlock) void k_guide_infer(GuideView G, PathQu
         uint32_t grid = 0u;
         if (valid) {
             const float4 r0 = q.ray0[i], r1 = q.ray1[i], h = q.hit[i];
+            const uint32_t pflags = f2u(reinterpret_cast<const float *>(q.lr + i)[3]);
             const uint32_t prim = f2u(h.y);
-            valid = prim != kMissPrim;
+            valid = prim != kMissPrim && !(pflags & kPathNoGuide);
             if (valid) {
                 const uint32_t shape =
                     (prim & kPrimSphere) ? sc.spheres[prim & kPrimIndexMask].shape : sc.tri[4 * (size_t)prim + 3];
@@ -565,7 +590,7 @@ __global__ __launch_bounds__(256) void k_shade_guided(SceneView scene, ShadePara
             }
             scatter = guided_begin(sv, pv, ps, prim, h.x, h.z, h.w, rs, path, v, so);
         }
-        bool use_guide = ga.gout != nullptr && scatter && v.guidable;
+        bool use_guide = ga.gout != nullptr && scatter && v.guidable && !(ps.flags & kPathNoGuide);
         GuideMixture m;
         if (use_guide) {
             float o[48];
@@ -673,17 +698,18 @@ __global__ void k_raygen_batch(PathQueue q, const LaunchDesc *ld, const BatchVie
             else hi = mid - 1u;
         }
         const LaunchEntry en = ld->e[lo];
-        const uint32_t view = en.view & ~kEntryTrain;
+        const uint32_t view = en.view & kEntryViewMask;
         const BatchViewDev &v = views[view];
-        const uint32_t pixel = i - en.path_begin;
+        const uint32_t pixel = tiled_pixel(i - en.path_begin, v.width, v.height);
         const uint32_t x = pixel % v.width, y = pixel / v.width;
         Rng rng = rng_init(pixel, en.frame_seed);
         const float jx = rng_next(rng);
         const float jy = rng_next(rng);
         f3 o, d;
         camera_ray(v.camera, x, y, v.width, v.height, jx, jy, o, d);
-        const uint32_t flags = kPathPrimary | ((en.view & kEntryTrain) ? 0u : kPathNoTrain);
-        q.ray0[i] = make_float4(o.x, o.y, o.z, u2f((view << path_bits) | i));
+        uint32_t flags = kPathPrimary | ((en.view & kEntryTrain) ? 0u : kPathNoTrain);
+        if ((en.view & kEntryPlain) || v.guide_off) flags |= kPathNoGuide;
+        q.ray0[i] = make_float4(o.x, o.y, o.z, u2f((view << path_bits) | (en.path_begin + pixel)));
         q.ray1[i] = make_float4(d.x, d.y, d.z, 0.f);
         q.tl[i] = make_float4(1.f, 1.f, 1.f, 0.f);
         q.lr[i] = make_float4(0.f, 0.f, u2f(rng.seed), u2f(flags));
@@ -741,6 +767,64 @@ __global__ void k_accumulate_batch(const float4 *out_l, const LaunchDesc *ld, co
 
 // Adds one launch's queue sizes to running ray counts (graph-capturable; no
 // host read-back per launch).
+// Per-frame error of the evaluation frames of a guided batch's launch: for
+// each frame flagged kEntryEval, the mean over pixels and channels of
+// (L - mu)^2 / (mu^2 + 0.01), with mu the view's running mean after this
+// launch, is added to acc[4 view] (guided frames) or acc[4 view + 2] (plain
+// frames), and the frame counted in acc[4 view + 1] / acc[4 view + 3].
+// blockIdx.y = launch entry.
+__global__ __launch_bounds__(256) void k_eval_error(const float4 *out_l, const LaunchDesc *ld,
+                                                    const BatchViewDev *views, double *acc) {
+    __shared__ float part[4];
+    const uint32_t en = blockIdx.y;
+    if (en >= ld->n_entries) return;
+    const LaunchEntry e = ld->e[en];
+    if (!(e.view & kEntryEval)) return;
+    const uint32_t v = e.view & kEntryViewMask;
+    const float4 *mu = views[v].accum;
+    float sum = 0.f;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < e.pixels; p += gridDim.x * blockDim.x) {
+        const float4 l = out_l[e.path_begin + p], m = mu[p];
+        const float dx = l.x - m.x, dy = l.y - m.y, dz = l.z - m.z;
+        sum += (dx * dx / (m.x * m.x + 0.01f) + dy * dy / (m.y * m.y + 0.01f) + dz * dz / (m.z * m.z + 0.01f)) *
+               (1.f / 3.f);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if (lane_id() == 0) part[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double t = (double)part[0] + (double)part[1] + (double)part[2] + (double)part[3];
+        const uint32_t k = 4u * v + ((e.view & kEntryPlain) ? 2u : 0u);
+        atomicAdd(acc + k, t / (double)e.pixels);
+        if (blockIdx.x == 0) atomicAdd(acc + k + 1, 1.0);
+    }
+}
+
+// Gate of a guided batch after its training frames: a view keeps its guide
+// when its guided evaluation frames' mean error is below 1/kGateCost of its
+// plain ones (a guided frame costs more: network queries); otherwise its
+// remaining frames sample the BSDF only.  Views without evaluation frames
+// keep the guide.  Clears the accumulators for the next gated render.
+constexpr double kGateCost = 1.25;
+__global__ void k_guide_gate(BatchViewDev *views, uint32_t n, double *acc, uint32_t *gated_off) {
+    if (blockIdx.x != 0) return;
+    for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) {
+        const double ng = acc[4 * v + 1], np = acc[4 * v + 3];
+        const bool off = ng > 0.0 && np > 0.0 && (acc[4 * v] / ng) * kGateCost >= acc[4 * v + 2] / np;
+        views[v].guide_off = off ? 1u : 0u;
+        if (off) atomicAdd(gated_off, 1u);
+    }
+}
+
+__global__ void k_guide_gate_reset(BatchViewDev *views, uint32_t n, double *acc, uint32_t *gated_off) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        views[v].guide_off = 0u;
+        acc[4 * v] = acc[4 * v + 1] = acc[4 * v + 2] = acc[4 * v + 3] = 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *gated_off = 0u;
+}
+
 __global__ void k_batch_count(const uint32_t *counters, uint32_t max_depth, unsigned long long *rays) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     unsigned long long e = 0, sh = 0;
@@ -966,6 +1050,9 @@ struct BatchState {
     LaunchDesc *h_launch = nullptr;  // pinned ring, one slot per launch of a render
     uint32_t h_launch_slots = 0;
     unsigned long long *d_rays = nullptr;  // extension, shadow
+    double *d_gate_acc = nullptr;          // guide gate: 4 per view (guided / plain error sums and frames)
+    uint32_t *d_gate_off = nullptr;        // guide gate: views switched off by the last gated render
+    uint32_t gate_views = 0;
     bool tuned = false;
     pt_batch_tuning tuning{};
     pt_batch_stats stats{};
@@ -1876,6 +1963,8 @@ void batch_free_all(pt_ctx *ctx) {
     }
     dfree(b.d_launch);
     dfree(b.d_rays);
+    dfree(b.d_gate_acc);
+    dfree(b.d_gate_off);
     if (b.h_launch) (void)hipHostFree(b.h_launch);
     b.h_launch = nullptr;
     b.h_launch_slots = 0;
@@ -1945,8 +2034,12 @@ int batch_upload(pt_ctx *ctx) {
 // so that the shared network learns from all views at once, then the other
 // frames view by view.  A launch holds either training frames only, at most
 // `train_capacity` paths of them (and at least one frame), or none.
+// With `gate`, the second half of each view's training frames (views training
+// on at least kGateMinFrames frames) measure the view's error (kEntryEval),
+// every other one of them without the guide (kEntryPlain).
+constexpr uint32_t kGateMinFrames = 4;
 void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::vector<LaunchDesc> &plan,
-                bool train = false, uint32_t train_spp = 0, uint32_t train_capacity = 0) {
+                bool train = false, uint32_t train_spp = 0, uint32_t train_capacity = 0, bool gate = false) {
     // (view, frame) in launch order
     std::vector<std::pair<uint32_t, uint32_t>> order;
     std::vector<uint32_t> trained(b.views.size(), 0);
@@ -1993,6 +2086,10 @@ void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::ve
             const bool seg_start = cur.n_segs == 0 || cur.g[cur.n_segs - 1].view != vi;
             LaunchEntry &e = cur.e[cur.n_entries];
             e.view = vi | (train_frame ? kEntryTrain : 0u);
+            if (gate && train_frame && trained[vi] >= kGateMinFrames && f >= trained[vi] / 2) {
+                e.view |= kEntryEval;
+                if ((f - trained[vi] / 2) & 1u) e.view |= kEntryPlain;
+            }
             if (train_frame) cur.train_frames += 1;
             e.frame_seed = v.random_seed + f;
             e.path_begin = cur.total_paths;
@@ -2257,7 +2354,15 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     bg.infer = (p.render_flags & PT_RENDER_GUIDE) != 0;
     bg.train = (p.render_flags & PT_RENDER_GUIDE_TRAIN) != 0;
     std::vector<LaunchDesc> plan;
-    batch_plan(b, sh.capacity, true, plan, bg.train, p.guide_train_spp, batch_train_capacity(b, sh));
+    const bool gate = bg.train && (p.batch_flags & PT_BATCH_GUIDE_GATE) != 0;
+    batch_plan(b, sh.capacity, true, plan, bg.train, p.guide_train_spp, batch_train_capacity(b, sh), gate);
+    if (bg.train && b.gate_views < b.views.size()) {
+        dfree(b.d_gate_acc);
+        dfree(b.d_gate_off);
+        PT_HIP(hipMalloc((void **)&b.d_gate_acc, b.views.size() * 4 * sizeof(double)));
+        PT_HIP(hipMalloc((void **)&b.d_gate_off, sizeof(uint32_t)));
+        b.gate_views = (uint32_t)b.views.size();
+    }
     if (plan.size() > b.h_launch_slots) {
         if (b.h_launch) (void)hipHostFree(b.h_launch);
         b.h_launch = nullptr;
@@ -2290,6 +2395,13 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     }
     rc = PT_OK;
     GuideState &gs = ctx->guide;
+    size_t last_train = plan.size();
+    for (size_t k = 0; k < plan.size(); ++k)
+        if (plan[k].train_frames) last_train = k;
+    const uint32_t nviews = (uint32_t)b.views.size();
+    if (bg.train)  // a training render decides the gate afresh (views start guided)
+        hipLaunchKernelGGL(k_guide_gate_reset, dim3((nviews + 255) / 256), dim3(256), 0, ctx->stream, b.d_views,
+                           nviews, b.d_gate_acc, b.d_gate_off);
     for (size_t k = 0; k < plan.size() && rc == PT_OK; ++k) {
         LaunchDesc &hl = b.h_launch[k];
         const bool train_here = bg.train && hl.train_frames > 0;
@@ -2306,6 +2418,20 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
                 e = hipGetLastError();
             }
         }
+        if (e == hipSuccess && gate && train_here) {
+            uint32_t eval_px = 0;
+            for (uint32_t en = 0; en < hl.n_entries; ++en)
+                if (hl.e[en].view & kEntryEval) eval_px = std::max(eval_px, hl.e[en].pixels);
+            if (eval_px) {
+                const uint32_t bx = std::min((eval_px + 255u) / 256u, 64u);
+                hipLaunchKernelGGL(k_eval_error, dim3(bx, hl.n_entries), dim3(256), 0, ctx->stream, ctx->d_out_l,
+                                   (const LaunchDesc *)b.d_launch, (const BatchViewDev *)b.d_views, b.d_gate_acc);
+            }
+            if (k == last_train)
+                hipLaunchKernelGGL(k_guide_gate, dim3(1), dim3(256), 0, ctx->stream, b.d_views, nviews,
+                                   b.d_gate_acc, b.d_gate_off);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess && train_here) {
             EventTimer tt(ctx, timing);
             // Samples from this launch's training frames, then the optimizer
@@ -2313,7 +2439,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
             uint32_t v_lo = 0xFFFFFFFFu, v_hi = 0;
             for (uint32_t en = 0; en < hl.n_entries; ++en)
                 if (hl.e[en].view & kEntryTrain) {
-                    const uint32_t v = hl.e[en].view & ~kEntryTrain;
+                    const uint32_t v = hl.e[en].view & kEntryViewMask;
                     v_lo = std::min(v_lo, v);
                     v_hi = std::max(v_hi, v + 1u);
                 }
@@ -2352,6 +2478,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
         ctx->error = std::string("pt_batch_render: ") + hipGetErrorString(e);
         return PT_ERR_HIP;
     }
+    if (gate) PT_HIP(hipMemcpy(&b.stats.guide_gated_off, b.d_gate_off, sizeof(uint32_t), hipMemcpyDeviceToHost));
     uint64_t paths = 0;
     for (auto &v : b.views) {
         paths += (uint64_t)v.width * v.height * v.spp;
