@@ -249,7 +249,7 @@ def guided_experiment(pt, ref_spp, plain_spp, train_spp):
 
 
 def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp, terrain_ref_spp, train_spp,
-                              terrain_grids, seed):
+                              terrain_grids, seed, gate=True):
     """Guided plus batched: n_door seeded door-room variants (door and camera
     moved) and n_terrain seeded terrain variants (heightfield grid drawn from
     terrain_grids, 256 icospheres, all seven BSDFs) at width x height, spp per
@@ -312,10 +312,12 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
     spf = max(1, int(round(8 * width * height / (1920 * 1080))))
     t.guide_enable_batch(seed=seed, steps_per_frame=spf)
     t0 = time.perf_counter()
-    t.batch_render(mis_mode=mis, graph=True, guide=True, guide_train=True, guide_train_spp=train_spp)
+    t.batch_render(mis_mode=mis, graph=True, guide=True, guide_train=True, guide_train_spp=train_spp,
+                   guide_gate=gate)
     t_guided = time.perf_counter() - t0
     e_guided = errors()
     info = t.guide_info()
+    gated_off = t.batch_stats()["guide_gated_off"] if gate else 0
     log(f"guided batch: guided {t_guided:.2f}s")
     m = max(1, int(round(t_guided / t_plain1)))
     t.batch_reset()
@@ -337,8 +339,14 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
             ep = mean([e_plain[k] for k in idx])
             eg = mean([e_guided[k] for k in idx])
             kinds[kind] = {"relmse_plain_equal_time": round(ep, 5), "relmse_guided": round(eg, 5),
-                           "ratio": round(ep / eg, 3)}
+                           "ratio": round(ep / eg, 3),
+                           "ratio_time_scaled": round(mean([ep_scaled[k] for k in idx]) / eg, 3),
+                           "min_view_ratio_time_scaled": round(min(ep_scaled[k] / e_guided[k] for k in idx), 3)}
     ep, eg = mean(e_plain), mean(e_guided)
+    # The plain side ran m whole batches (m T_plain, not exactly T_guided):
+    # scaled to T_guided with relMSE proportional to 1 / time.
+    scale = m * t_plain1 / t_guided
+    ep_scaled = [e * scale for e in e_plain]
     # Per-view equal-time ratios; their geometric mean weighs every view alike
     # (the ratio of mean relMSEs is dominated by the noisiest, lowest-spp views).
     geo = float(np.exp(np.mean([np.log(a / b) for a, b in zip(e_plain, e_guided)])))
@@ -359,8 +367,11 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
         "relmse_plain_equal_time": round(ep, 5),
         "relmse_guided": round(eg, 5),
         "ratio": round(ep / eg, 3),
+        "ratio_time_scaled": round(mean(ep_scaled) / eg, 3),
         "ratio_geomean_per_view": round(geo, 3),
         "by_scene": kinds,
+        "guide_gate": gate,
+        "views_gated_off": gated_off,
         "guide_steps": info["steps"],
         "guide_skipped_steps": info["skipped"],
         "per_view": per_view,
@@ -411,7 +422,7 @@ def summarize_pmc(fetch_dir, write_dir, frames_per_launch, out_path, kernels):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_terrain_1080p.json")
-TRAFFIC_KERNELS = {"trace": "k_trace<false, false>", "shade": "k_shade<false>", "shadow": "k_shadow<false, false>"}
+TRAFFIC_KERNELS = {"trace": "k_trace<false, false>", "shade": "k_shade<false, ", "shadow": "k_shadow<false, false>"}
 
 
 def measured_traffic(frames_per_launch):

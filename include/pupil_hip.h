@@ -425,7 +425,12 @@ int pt_guide_read_samples(pt_ctx *ctx, float *samples, uint32_t max_samples, uin
  * (same seeds, same bits). */
 enum pt_batch_flags {
     PT_BATCH_GRAPH = 1,    /* capture one launch's kernels in a hipGraph and replay it */
-    PT_BATCH_AUTOTUNE = 2  /* pick the launch shape by timing candidates first (pt_batch_autotune) */
+    PT_BATCH_AUTOTUNE = 2, /* pick the launch shape by timing candidates first (pt_batch_autotune) */
+    PT_BATCH_GUIDE_GATE = 4 /* with PT_RENDER_GUIDE_TRAIN: in the second half of each view's training
+                               frames every other frame samples the BSDF only; both halves measure the
+                               view's per-frame error against its running mean, and a view whose guided
+                               frames do not reach 1/1.25 of the plain frames' error renders its
+                               remaining frames without the guide (no network queries) */
 };
 
 typedef struct pt_batch_params {
@@ -454,6 +459,8 @@ typedef struct pt_batch_stats {
     uint32_t views, launches;
     uint64_t paths, rays_extension, rays_shadow, triangles;
     double ms_total, build_ms;
+    uint32_t guide_gated_off; /* views whose guide the last gated render switched off */
+    uint32_t pad;
 } pt_batch_stats;
 
 /* Removes all views and frees their device data. */

@@ -43,7 +43,7 @@ RENDER_AOV, RENDER_FP64_ACCUM, RENDER_TIMING, RENDER_GUIDE, RENDER_GUIDE_TRAIN =
 DENOISE_ALBEDO, DENOISE_NORMAL = 1, 2
 GUIDE_OUTPUTS, GUIDE_LEVELS = 41, 16
 GUIDE_NO_RADIANCE_L2, GUIDE_NO_GUIDE_CE, GUIDE_NO_MIXTURE_CE = 1, 2, 4
-BATCH_GRAPH, BATCH_AUTOTUNE = 1, 2
+BATCH_GRAPH, BATCH_AUTOTUNE, BATCH_GUIDE_GATE = 1, 2, 4
 BUF_FRAME, BUF_ACCUM, BUF_ALBEDO, BUF_NORMAL, BUF_TEST, BUF_PRIM_ID, BUF_ACCUM64 = range(7)
 SCENE_CORNELL, SCENE_TERRAIN, SCENE_DOOR_ROOM, SCENE_FURNACE, SCENE_CORNELL_VARIANT = 1, 2, 3, 4, 5
 MISS = 0xFFFFFFFF
@@ -160,7 +160,8 @@ class BatchTuning(C.Structure):
 class BatchStats(C.Structure):
     _fields_ = [("views", C.c_uint32), ("launches", C.c_uint32), ("paths", C.c_uint64),
                 ("rays_extension", C.c_uint64), ("rays_shadow", C.c_uint64), ("triangles", C.c_uint64),
-                ("ms_total", C.c_double), ("build_ms", C.c_double)]
+                ("ms_total", C.c_double), ("build_ms", C.c_double), ("guide_gated_off", C.c_uint32),
+                ("pad", C.c_uint32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -627,22 +628,27 @@ class PathTracer:
 
     @staticmethod
     def _batch_params(rr_depth, mis_mode, fp64, timing, graph, autotune, paths_per_launch, shade_block,
-                      shade_blocks_per_cu, trace_blocks_per_cu, guide=False, guide_train=False, guide_train_spp=0):
+                      shade_blocks_per_cu, trace_blocks_per_cu, guide=False, guide_train=False, guide_train_spp=0,
+                      guide_gate=False):
         rf = (RENDER_FP64_ACCUM if fp64 else 0) | (RENDER_TIMING if timing else 0)
         rf |= (RENDER_GUIDE if guide else 0) | (RENDER_GUIDE_TRAIN if guide_train else 0)
-        bf = (BATCH_GRAPH if graph else 0) | (BATCH_AUTOTUNE if autotune else 0)
+        bf = (BATCH_GRAPH if graph else 0) | (BATCH_AUTOTUNE if autotune else 0) | (BATCH_GUIDE_GATE if guide_gate else 0)
         return BatchParams(rr_depth, mis_mode, rf, bf, paths_per_launch, shade_block, shade_blocks_per_cu,
                            trace_blocks_per_cu, guide_train_spp)
 
     def batch_render(self, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE, fp64: bool = False,
                      timing: bool = False, graph: bool = False, autotune: bool = False, paths_per_launch: int = 0,
                      shade_block: int = 0, shade_blocks_per_cu: int = 0, trace_blocks_per_cu: int = 0,
-                     guide: bool = False, guide_train: bool = False, guide_train_spp: int = 0) -> None:
+                     guide: bool = False, guide_train: bool = False, guide_train_spp: int = 0,
+                     guide_gate: bool = False) -> None:
         """Renders every view's frames.  guide / guide_train (after
         guide_enable_batch, MIS_CONSISTENT): sample the batch guide / train it
-        on the first guide_train_spp frames of each view (0 = all)."""
+        on the first guide_train_spp frames of each view (0 = all).
+        guide_gate (with guide_train): switch the guide off for views whose
+        guided training frames show no lower error than their plain ones."""
         p = self._batch_params(rr_depth, mis_mode, fp64, timing, graph, autotune, paths_per_launch, shade_block,
-                               shade_blocks_per_cu, trace_blocks_per_cu, guide, guide_train, guide_train_spp)
+                               shade_blocks_per_cu, trace_blocks_per_cu, guide, guide_train, guide_train_spp,
+                               guide_gate)
         _check(lib().pt_batch_render(self._ctx, C.byref(p)), "pt_batch_render", self._ctx)
 
     def batch_autotune(self, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE, graph: bool = False,

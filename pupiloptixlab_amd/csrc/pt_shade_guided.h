@@ -22,6 +22,8 @@ namespace pupil {
 
 constexpr uint32_t kPathRecorded = 1u << 11;  // the last vertex wrote a training record
 constexpr uint32_t kPathNoTrain = 1u << 12;   // the path's frame does not train the guide (batched renders)
+constexpr uint32_t kPathNoGuide = 1u << 13;   // the path samples the BSDF only (batched renders: a plain
+                                              // evaluation frame, or a view whose guide was gated off)
 
 struct GuidedVertex {
     LocalGeometry g;
@@ -239,7 +241,7 @@ PT_HD void guided_scatter(const SceneView &s, PathState &ps, const GuidedVertex 
     ps.o = g.position;
     ps.d = wi_w;
     ps.prev_pdf = p;
-    uint32_t fl = v.depth | (ps.flags & kPathNoTrain);
+    uint32_t fl = v.depth | (ps.flags & (kPathNoTrain | kPathNoGuide));
     if (delta) fl |= kPathPrevDelta;
     if (dot(g.normal, ps.d) > 0.f) fl |= kPathEnvUpper;
     if (record && ridx < rs.capacity) {
