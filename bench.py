@@ -332,6 +332,10 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
                  "relmse_plain_same_spp": round(e1, 5), "relmse_guided": round(eg, 5)}
                 for (kind, _), tr, spp, ep, e1, eg in zip(views, tris, spps, e_plain, e_plain1, e_guided)]
     mean = lambda xs: float(np.mean(xs)) if xs else None
+    # The plain side ran m whole batches (m T_plain, not exactly T_guided):
+    # scaled to T_guided with relMSE proportional to 1 / time.
+    scale = m * t_plain1 / t_guided
+    ep_scaled = [e * scale for e in e_plain]
     kinds = {}
     for kind in ("door", "terrain"):
         idx = [k for k, v in enumerate(views) if v[0] == kind]
@@ -343,10 +347,6 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
                            "ratio_time_scaled": round(mean([ep_scaled[k] for k in idx]) / eg, 3),
                            "min_view_ratio_time_scaled": round(min(ep_scaled[k] / e_guided[k] for k in idx), 3)}
     ep, eg = mean(e_plain), mean(e_guided)
-    # The plain side ran m whole batches (m T_plain, not exactly T_guided):
-    # scaled to T_guided with relMSE proportional to 1 / time.
-    scale = m * t_plain1 / t_guided
-    ep_scaled = [e * scale for e in e_plain]
     # Per-view equal-time ratios; their geometric mean weighs every view alike
     # (the ratio of mean relMSEs is dominated by the noisiest, lowest-spp views).
     geo = float(np.exp(np.mean([np.log(a / b) for a, b in zip(e_plain, e_guided)])))
