@@ -263,8 +263,9 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
     width), timed; rendered again m = round(T_guided / T_plain) times in all so
     that both sides use about the same wall-clock.  Guided: one call, training
     included (training launches of ~2M paths, the rest at the default width).
-    relMSE per view against FP64 plain references (door_ref_spp /
-    terrain_ref_spp frames, single-view renders)."""
+    relMSE per view against FP64 references (door_ref_spp guided frames after
+    128 training frames / terrain_ref_spp plain frames, single-view renders,
+    frames disjoint from the tested ones)."""
     import numpy as np
 
     rng = np.random.default_rng(seed)
@@ -283,12 +284,26 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
     mis = pt.MIS_CONSISTENT
     refs = []
     t_ref0 = time.perf_counter()
-    for kind, scene in views:
+    for i, (kind, scene) in enumerate(views):
         t.set_scene(scene)
         t.build_bvh()
         n = door_ref_spp if kind == "door" else terrain_ref_spp
-        t.set_random_seed(REF_SEED)
-        t.render(spp=n, mis_mode=mis, fp64=True, aov=False)
+        if kind == "door":
+            # A guided reference: unbiased like a plain one, but its own error
+            # at 1024 frames is about a tenth of a plain 1024-frame image's,
+            # which would otherwise dominate both sides' relMSE.  Its network
+            # is trained on frames 2^23 on and it renders frames 2^24 on, so
+            # it shares no samples with the renders it judges.
+            t.guide_enable(seed=1000 + i)
+            t.set_random_seed(REF_SEED >> 1)
+            t.render(spp=128, mis_mode=mis, aov=False, guide=True, guide_train=True)
+            t.reset()
+            t.set_random_seed(REF_SEED)
+            t.render(spp=n, mis_mode=mis, fp64=True, aov=False, guide=True)
+            t.guide_disable()
+        else:
+            t.set_random_seed(REF_SEED)
+            t.render(spp=n, mis_mode=mis, fp64=True, aov=False)
         refs.append(t.read(pt.BUF_ACCUM64) / n)
         log(f"guided batch: reference {len(refs)}/{len(views)} ({kind}, {n} spp) at {time.perf_counter() - t_ref0:.1f}s")
     log(f"guided batch: {len(views)} views, {sum(tris)} tris, references in {time.perf_counter() - t_ref0:.1f}s")
@@ -353,7 +368,9 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
     out = {
         "workload": (f"{n_door} door-room variants + {n_terrain} terrain variants (grids {list(terrain_grids)}), "
                      f"{width}x{height}, spp {spps}, depth 16, one wavefront loop; guide: one hash grid per view, "
-                     f"shared MLP, trained on the first {train_spp} frames of each view, {spf} steps per frame"),
+                     f"shared MLP, trained on the first {train_spp} frames of each view, {spf} steps per frame"
+                     f"{', per-view gate' if gate else ''}; references: door views 1024 guided FP64 frames, "
+                     f"terrain views {terrain_ref_spp} plain FP64 frames"),
         "views": len(views),
         "triangles": sum(tris),
         "samples": samples,
@@ -438,10 +455,10 @@ def measured_traffic(frames_per_launch):
     return {k: v["hbm_bytes_per_dispatch"] * scale for k, v in t["kernels"].items()}, note
 
 
-def run_guided_batch(pt, full):
+def run_guided_batch(pt, full, train_spp=32):
     if full:
-        return batched_guided_experiment(pt, 16, 16, 1920, 1080, 1024, 256, 64, [512, 1024, 1448], 0)
-    return batched_guided_experiment(pt, 2, 2, 1920, 1080, 1024, 256, 64, [512], 0)
+        return batched_guided_experiment(pt, 16, 16, 1920, 1080, 1024, 256, train_spp, [512, 1024, 1448], 0)
+    return batched_guided_experiment(pt, 2, 2, 1920, 1080, 1024, 256, train_spp, [512], 0)
 
 
 def main():
@@ -471,6 +488,8 @@ def main():
     ap.add_argument("--no-guided-batch", action="store_true", help="skip the guided + batched experiment")
     ap.add_argument("--guided-batch-only", action="store_true",
                     help="run only the guided + batched experiment and print its JSON")
+    ap.add_argument("--guided-batch-train-spp", type=int, default=32,
+                    help="guided + batched: frames of each view that train the guide")
     ap.add_argument("--guided-batch-full", action="store_true",
                     help="guided + batched at full scale: 16 + 16 views at 1920x1080, terrain grids up to "
                          "1448^2 (4.2M tris)")
@@ -491,7 +510,7 @@ def main():
               flush=True)
         return
     if args.guided_batch_only:
-        print(json.dumps(run_guided_batch(pt, args.guided_batch_full)), flush=True)
+        print(json.dumps(run_guided_batch(pt, args.guided_batch_full, args.guided_batch_train_spp)), flush=True)
         return
 
     t0 = time.perf_counter()
@@ -596,7 +615,7 @@ def main():
         batched = batched_experiment(pt, args.batched_views, 0)
     guided_batch = None
     if not args.no_guided_batch:
-        guided_batch = run_guided_batch(pt, args.guided_batch_full)
+        guided_batch = run_guided_batch(pt, args.guided_batch_full, args.guided_batch_train_spp)
 
     result = {
         "metric": "Msamples/s at 1080p on 1 GPU; relMSE at equal time, guided vs plain PT",

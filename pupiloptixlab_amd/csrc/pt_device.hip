@@ -332,7 +332,11 @@ struct LaunchEntry {
 // into its film in frame order; its pixels are threads [pix_begin, +pixels).
 struct LaunchSeg {
     uint32_t view, first_entry, frames, pix_begin;
-    uint32_t sample_cnt0, accumulate, pad0, pad1;
+    uint32_t sample_cnt0, accumulate;
+    // Gated guided renders: the view restarts its accumulation with its first
+    // frame after training if it keeps the guide (BatchViewDev::guide_off 0),
+    // counting frames from restart_cnt0 instead of sample_cnt0.
+    uint32_t restart, restart_cnt0;
 };
 struct LaunchDesc {
     uint32_t total_paths, n_entries, n_segs, seg_pixels;
@@ -734,9 +738,10 @@ __global__ void k_accumulate_batch(const float4 *out_l, const LaunchDesc *ld, co
         const uint32_t p = t - sg.pix_begin;
         const float4 acc = v.accum[p];
         f3 a = mk3(acc.x, acc.y, acc.z);
-        uint32_t n = sg.sample_cnt0;
+        const bool restart = sg.restart && !v.guide_off;
+        uint32_t n = restart ? sg.restart_cnt0 : sg.sample_cnt0;
         double sx = 0, sy = 0, sz = 0;
-        if (fp64) {
+        if (fp64 && !(restart && n == 0)) {
             sx = v.accum64[3 * (size_t)p];
             sy = v.accum64[3 * (size_t)p + 1];
             sz = v.accum64[3 * (size_t)p + 2];
@@ -767,24 +772,49 @@ __global__ void k_accumulate_batch(const float4 *out_l, const LaunchDesc *ld, co
 
 // Adds one launch's queue sizes to running ray counts (graph-capturable; no
 // host read-back per launch).
+// Guide gate accumulators: per view, per kind (guided, plain), per frame slot
+// (frame index mod kGateSlots), the frame's error and a frame count.
+constexpr uint32_t kGateSlots = 128;
+constexpr size_t kGateDoublesPerView = 2 * kGateSlots * 2;
+__device__ __forceinline__ size_t gate_slot(uint32_t view, uint32_t plain, uint32_t slot) {
+    return (((size_t)view * 2 + plain) * kGateSlots + slot) * 2;
+}
+
 // Per-frame error of the evaluation frames of a guided batch's launch: for
 // each frame flagged kEntryEval, the mean over pixels and channels of
-// (L - mu)^2 / (mu^2 + 0.01), with mu the view's running mean after this
-// launch, is added to acc[4 view] (guided frames) or acc[4 view + 2] (plain
-// frames), and the frame counted in acc[4 view + 1] / acc[4 view + 3].
+// (L - mu)^2 / (mu^2 + 0.01) -- the relMSE one frame would have against the
+// view's mean -- goes to the frame's slot of its view and kind.
+// mu is the leave-one-out mean of the view's N frames so far,
+// (N mean - L) / (N - 1): the mean after this launch includes the frame
+// itself, which would shrink a firefly's deviation to at most (N - 1)^2 of
+// its own share and hide exactly the variance that guiding removes.
 // blockIdx.y = launch entry.
 __global__ __launch_bounds__(256) void k_eval_error(const float4 *out_l, const LaunchDesc *ld,
                                                     const BatchViewDev *views, double *acc) {
     __shared__ float part[4];
+    __shared__ uint32_t frames_so_far;
     const uint32_t en = blockIdx.y;
     if (en >= ld->n_entries) return;
     const LaunchEntry e = ld->e[en];
     if (!(e.view & kEntryEval)) return;
+    if (threadIdx.x == 0) {
+        uint32_t n = 0;
+        for (uint32_t k = 0; k < ld->n_segs; ++k) {
+            const LaunchSeg &sg = ld->g[k];
+            if (en >= sg.first_entry && en < sg.first_entry + sg.frames) n = sg.sample_cnt0 + sg.frames;
+        }
+        frames_so_far = n;
+    }
+    __syncthreads();
+    const uint32_t n = frames_so_far;
+    if (n < 2u) return;
+    const float fn = (float)n, inv = 1.f / (float)(n - 1u);
     const uint32_t v = e.view & kEntryViewMask;
     const float4 *mu = views[v].accum;
     float sum = 0.f;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < e.pixels; p += gridDim.x * blockDim.x) {
-        const float4 l = out_l[e.path_begin + p], m = mu[p];
+        const float4 l = out_l[e.path_begin + p], ma = mu[p];
+        const float4 m = make_float4((fn * ma.x - l.x) * inv, (fn * ma.y - l.y) * inv, (fn * ma.z - l.z) * inv, 0.f);
         const float dx = l.x - m.x, dy = l.y - m.y, dz = l.z - m.z;
         sum += (dx * dx / (m.x * m.x + 0.01f) + dy * dy / (m.y * m.y + 0.01f) + dz * dz / (m.z * m.z + 0.01f)) *
                (1.f / 3.f);
@@ -795,32 +825,59 @@ __global__ __launch_bounds__(256) void k_eval_error(const float4 *out_l, const L
     __syncthreads();
     if (threadIdx.x == 0) {
         const double t = (double)part[0] + (double)part[1] + (double)part[2] + (double)part[3];
-        const uint32_t k = 4u * v + ((e.view & kEntryPlain) ? 2u : 0u);
+        const size_t k = gate_slot(v, (e.view & kEntryPlain) ? 1u : 0u, e.frame_seed % kGateSlots);
         atomicAdd(acc + k, t / (double)e.pixels);
         if (blockIdx.x == 0) atomicAdd(acc + k + 1, 1.0);
     }
 }
 
-// Gate of a guided batch after its training frames: a view keeps its guide
-// when its guided evaluation frames' mean error is below 1/kGateCost of its
-// plain ones (a guided frame costs more: network queries); otherwise its
-// remaining frames sample the BSDF only.  Views without evaluation frames
-// keep the guide.  Clears the accumulators for the next gated render.
+// Mean and median of the per-frame errors of one view and kind; false
+// without frames.
+__device__ bool gate_stats(const double *acc, uint32_t view, uint32_t plain, double &mean, double &median) {
+    double x[kGateSlots];
+    uint32_t n = 0;
+    for (uint32_t s = 0; s < kGateSlots; ++s) {
+        const size_t k = gate_slot(view, plain, s);
+        if (acc[k + 1] > 0.0) {
+            double val = acc[k] / acc[k + 1];
+            uint32_t j = n++;  // insertion sort
+            while (j > 0 && x[j - 1] > val) {
+                x[j] = x[j - 1];
+                --j;
+            }
+            x[j] = val;
+        }
+    }
+    if (n == 0) return false;
+    double sum = 0.0;
+    for (uint32_t i = 0; i < n; ++i) sum += x[i];
+    mean = sum / n;
+    median = (n & 1u) ? x[n / 2] : 0.5 * (x[n / 2 - 1] + x[n / 2]);
+    return true;
+}
+
+// Gate of a guided batch after its training frames: a view's remaining
+// frames sample the BSDF only when neither the mean nor the median of its
+// guided evaluation frames' errors is below 1/kGateCost of its plain ones (a
+// guided frame costs more: network queries).  The median keeps one
+// heavy-tailed frame from deciding; the mean keeps the tail in the picture.
+// Views without evaluation frames keep the guide.
 constexpr double kGateCost = 1.25;
 __global__ void k_guide_gate(BatchViewDev *views, uint32_t n, double *acc, uint32_t *gated_off) {
-    if (blockIdx.x != 0) return;
-    for (uint32_t v = threadIdx.x; v < n; v += blockDim.x) {
-        const double ng = acc[4 * v + 1], np = acc[4 * v + 3];
-        const bool off = ng > 0.0 && np > 0.0 && (acc[4 * v] / ng) * kGateCost >= acc[4 * v + 2] / np;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+        double gm, gmed, pm, pmed;
+        const bool off = gate_stats(acc, v, 0u, gm, gmed) && gate_stats(acc, v, 1u, pm, pmed) &&
+                         gm * kGateCost >= pm && gmed * kGateCost >= pmed;
         views[v].guide_off = off ? 1u : 0u;
         if (off) atomicAdd(gated_off, 1u);
     }
 }
 
 __global__ void k_guide_gate_reset(BatchViewDev *views, uint32_t n, double *acc, uint32_t *gated_off) {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-        views[v].guide_off = 0u;
-        acc[4 * v] = acc[4 * v + 1] = acc[4 * v + 2] = acc[4 * v + 3] = 0.0;
+    for (size_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (size_t)n * kGateDoublesPerView;
+         i += gridDim.x * blockDim.x) {
+        acc[i] = 0.0;
+        if (i % kGateDoublesPerView == 0) views[i / kGateDoublesPerView].guide_off = 0u;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *gated_off = 0u;
 }
@@ -2034,10 +2091,13 @@ int batch_upload(pt_ctx *ctx) {
 // so that the shared network learns from all views at once, then the other
 // frames view by view.  A launch holds either training frames only, at most
 // `train_capacity` paths of them (and at least one frame), or none.
-// With `gate`, the second half of each view's training frames (views training
-// on at least kGateMinFrames frames) measure the view's error (kEntryEval),
-// every other one of them without the guide (kEntryPlain).
+// With `gate`, the second half of the training frames of each view that has
+// frames after its training (and trains on at least kGateMinFrames frames)
+// measure the view's error (kEntryEval), every other one of them without the
+// guide (kEntryPlain); a view rendered entirely by training frames has
+// nothing to gate and samples with the guide throughout.
 constexpr uint32_t kGateMinFrames = 4;
+constexpr uint32_t kGateWindow = 128;  // at most this many evaluation frames (one slot each, mod 128)
 void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::vector<LaunchDesc> &plan,
                 bool train = false, uint32_t train_spp = 0, uint32_t train_capacity = 0, bool gate = false) {
     // (view, frame) in launch order
@@ -2086,7 +2146,8 @@ void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::ve
             const bool seg_start = cur.n_segs == 0 || cur.g[cur.n_segs - 1].view != vi;
             LaunchEntry &e = cur.e[cur.n_entries];
             e.view = vi | (train_frame ? kEntryTrain : 0u);
-            if (gate && train_frame && trained[vi] >= kGateMinFrames && f >= trained[vi] / 2) {
+            if (gate && train_frame && trained[vi] >= kGateMinFrames && trained[vi] < v.spp &&
+                f >= std::max(trained[vi] / 2, trained[vi] - std::min(trained[vi], kGateWindow))) {
                 e.view |= kEntryEval;
                 if ((f - trained[vi] / 2) & 1u) e.view |= kEntryPlain;
             }
@@ -2103,6 +2164,8 @@ void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::ve
                 g.pix_begin = cur.seg_pixels;
                 g.sample_cnt0 = accumulate ? v.sample_cnt + f : 0u;
                 g.accumulate = accumulate ? 1u : 0u;
+                g.restart = (gate && accumulate && !train_frame && trained[vi] >= kGateMinFrames) ? 1u : 0u;
+                g.restart_cnt0 = g.restart ? f - trained[vi] : 0u;
                 cur.seg_pixels += npix;
             }
             cur.g[cur.n_segs - 1].frames += 1;
@@ -2359,7 +2422,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     if (bg.train && b.gate_views < b.views.size()) {
         dfree(b.d_gate_acc);
         dfree(b.d_gate_off);
-        PT_HIP(hipMalloc((void **)&b.d_gate_acc, b.views.size() * 4 * sizeof(double)));
+        PT_HIP(hipMalloc((void **)&b.d_gate_acc, b.views.size() * kGateDoublesPerView * sizeof(double)));
         PT_HIP(hipMalloc((void **)&b.d_gate_off, sizeof(uint32_t)));
         b.gate_views = (uint32_t)b.views.size();
     }
@@ -2400,8 +2463,9 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
         if (plan[k].train_frames) last_train = k;
     const uint32_t nviews = (uint32_t)b.views.size();
     if (bg.train)  // a training render decides the gate afresh (views start guided)
-        hipLaunchKernelGGL(k_guide_gate_reset, dim3((nviews + 255) / 256), dim3(256), 0, ctx->stream, b.d_views,
-                           nviews, b.d_gate_acc, b.d_gate_off);
+        hipLaunchKernelGGL(k_guide_gate_reset, dim3((uint32_t)std::min<size_t>(
+                               (nviews * kGateDoublesPerView + 255) / 256, 1024)), dim3(256), 0, ctx->stream,
+                           b.d_views, nviews, b.d_gate_acc, b.d_gate_off);
     for (size_t k = 0; k < plan.size() && rc == PT_OK; ++k) {
         LaunchDesc &hl = b.h_launch[k];
         const bool train_here = bg.train && hl.train_frames > 0;
@@ -2428,8 +2492,8 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
                                    (const LaunchDesc *)b.d_launch, (const BatchViewDev *)b.d_views, b.d_gate_acc);
             }
             if (k == last_train)
-                hipLaunchKernelGGL(k_guide_gate, dim3(1), dim3(256), 0, ctx->stream, b.d_views, nviews,
-                                   b.d_gate_acc, b.d_gate_off);
+                hipLaunchKernelGGL(k_guide_gate, dim3((nviews + 63) / 64), dim3(64), 0, ctx->stream, b.d_views,
+                                   nviews, b.d_gate_acc, b.d_gate_off);
             e = hipGetLastError();
         }
         if (e == hipSuccess && train_here) {
@@ -2478,11 +2542,24 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
         ctx->error = std::string("pt_batch_render: ") + hipGetErrorString(e);
         return PT_ERR_HIP;
     }
-    if (gate) PT_HIP(hipMemcpy(&b.stats.guide_gated_off, b.d_gate_off, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<BatchViewDev> gated_views;
+    if (gate) {
+        PT_HIP(hipMemcpy(&b.stats.guide_gated_off, b.d_gate_off, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        gated_views.resize(b.views.size());
+        PT_HIP(hipMemcpy(gated_views.data(), b.d_views, gated_views.size() * sizeof(BatchViewDev),
+                         hipMemcpyDeviceToHost));
+    }
     uint64_t paths = 0;
-    for (auto &v : b.views) {
+    for (size_t vi = 0; vi < b.views.size(); ++vi) {
+        BatchView &v = b.views[vi];
         paths += (uint64_t)v.width * v.height * v.spp;
-        v.sample_cnt += v.spp;
+        const uint32_t trained = p.guide_train_spp == 0 ? v.spp : std::min(p.guide_train_spp, v.spp);
+        // A view that kept its guide restarted its accumulation after training
+        // (when it had frames after training).
+        if (gate && trained >= kGateMinFrames && trained < v.spp && !gated_views[vi].guide_off)
+            v.sample_cnt = v.spp - trained;
+        else
+            v.sample_cnt += v.spp;
         v.random_seed += v.spp;
     }
     *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

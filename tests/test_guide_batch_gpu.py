@@ -275,6 +275,47 @@ def test_batch_guided_training_gains_and_stays_unbiased(tracer):
     tracer.batch_clear()
 
 
+def test_batch_guide_gate(tracer):
+    """PT_BATCH_GUIDE_GATE: a door-room view (320x180, 160 spp) and a terrain
+    view (128x72, 160 spp) train on their first 48 frames; in the second half
+    of training every other frame samples the BSDF only, and each view keeps
+    the guide only if its guided frames show clearly lower per-frame relMSE.
+    The door view keeps it (and its image holds only frames after training),
+    the terrain view, lit directly, does not; both images stay unbiased
+    against 2048-frame FP64 plain references, and the door view's relMSE is
+    below the plain batch's at the same samples."""
+    print("case: guided batch with the gate, door 320x180 + terrain 128x72, 160 spp, train 48", flush=True)
+    views = [(pt.HostScene.generate(pt.SCENE_DOOR_ROOM, seed=0, width=320, height=180), 160),
+             (pt.HostScene.generate(pt.SCENE_TERRAIN, seed=5, width=128, height=72, grid=96, spheres=16), 160)]
+    refs = []
+    for scene, _ in views:
+        tracer.set_scene(scene)
+        tracer.build_bvh()
+        tracer.set_random_seed(1 << 24)
+        tracer.render(spp=2048, mis_mode=pt.MIS_CONSISTENT, fp64=True, aov=False)
+        refs.append(tracer.read(pt.BUF_ACCUM64) / 2048)
+    batch_with_views(tracer, views)
+    tracer.batch_render(mis_mode=pt.MIS_CONSISTENT, graph=True)
+    plain = [relmse(tracer.batch_read(k)[..., :3].astype(np.float64), refs[k]) for k in range(2)]
+    tracer.batch_reset()
+    tracer.guide_enable_batch(seed=5)
+    tracer.batch_render(mis_mode=pt.MIS_CONSISTENT, graph=True, guide=True, guide_train=True, guide_train_spp=48,
+                        guide_gate=True)
+    gated = tracer.batch_stats()["guide_gated_off"]
+    for k in range(2):
+        img = tracer.batch_read(k)[..., :3].astype(np.float64)
+        assert np.all(np.isfinite(img))
+        g = relmse(img, refs[k])
+        bias = img.mean() / refs[k].mean() - 1
+        print(f"view {k}: relmse plain {plain[k]:.4f} guided {g:.4f} ratio {plain[k] / g:.2f}; mean {bias:+.4f}")
+        assert abs(bias) < 0.03
+        if k == 0:
+            assert g < plain[k]
+    print(f"views gated off: {gated}")
+    assert gated == 1
+    tracer.batch_clear()
+
+
 def test_batch_guide_errors(tracer):
     views = small_scenes()[:2]
     tracer.batch_clear()
