@@ -143,9 +143,12 @@ def batched_experiment(pt, views, seed):
     log(f"batched: tuning {tuning}")
     t.batch_render(graph=True)  # warm-up
     t.batch_reset()
+    st0 = t.batch_stats()
     t0 = time.perf_counter()
     t.batch_render(graph=True)
     t_graph = time.perf_counter() - t0
+    st1 = t.batch_stats()
+    rays = (st1["rays_extension"] - st0["rays_extension"]) + (st1["rays_shadow"] - st0["rays_shadow"])
     t.batch_reset()
     t0 = time.perf_counter()
     t.batch_render(graph=False)
@@ -166,6 +169,11 @@ def batched_experiment(pt, views, seed):
         "seconds_graph": round(t_graph, 3),
         "seconds_direct": round(t_direct, 3),
         "launches_per_render": st["launches"] // 3 if st["launches"] else None,
+        # Rays per sample explain the gap to the single terrain view: closed
+        # Cornell boxes keep every path bouncing to depth 8 (extension and
+        # shadow rays at each vertex); terrain paths escape to the sky early.
+        "rays_per_sample": round(rays / samples, 3),
+        "mrays_per_s_graph": round(rays / t_graph / 1e6, 1),
         "tuning": tuning,
         "all_finite": finite,
     }

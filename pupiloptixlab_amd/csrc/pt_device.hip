@@ -374,6 +374,21 @@ __global__ __launch_bounds__(256, kBatch ? 1 : 5) void k_shade(SceneView scene, 
             ps.rng.seed = f2u(lr.z);
             ps.flags = f2u(lr.w);
             prim = f2u(h.y);
+            // AOVs of the launch's last frame, stored where shading finds them.
+            const auto aov_write = [&](f3 albedo, f3 normal, float test) {
+                if (!aov.albedo) return;
+                const uint32_t f = path / aov.npix;
+                if (f != aov.last_frame) return;
+                const uint32_t px = path - f * aov.npix;
+                aov.albedo[3 * px + 0] = albedo.x;
+                aov.albedo[3 * px + 1] = albedo.y;
+                aov.albedo[3 * px + 2] = albedo.z;
+                aov.normal[3 * px + 0] = normal.x;
+                aov.normal[3 * px + 1] = normal.y;
+                aov.normal[3 * px + 2] = normal.z;
+                aov.test[px] = test;
+                aov.prim[px] = prim;
+            };
             if constexpr (kBatch) {
                 const uint32_t view = packed >> path_bits;
                 path = packed & ((1u << path_bits) - 1u);
@@ -386,28 +401,14 @@ __global__ __launch_bounds__(256, kBatch ? 1 : 5) void k_shade(SceneView scene, 
                 sv.env = bv.env;
                 ShadeParams pv = prm;
                 pv.max_depth = bv.max_depth;
-                shade_path(sv, pv, ps, prim, h.x, h.z, h.w, so, kFeat);
+                shade_path(sv, pv, ps, prim, h.x, h.z, h.w, so, kFeat, aov_write);
             } else {
-                shade_path(scene, prm, ps, prim, h.x, h.z, h.w, so, kFeat);
+                shade_path(scene, prm, ps, prim, h.x, h.z, h.w, so, kFeat, aov_write);
             }
         }
         const uint32_t j = wave_alloc(valid && so.cont, count_out);
         const uint32_t k = wave_alloc(valid && so.shadow, count_shadow);
         if (!valid) continue;
-        if (so.wrote_aov && aov.albedo) {
-            const uint32_t f = path / aov.npix;
-            if (f == aov.last_frame) {
-                const uint32_t px = path - f * aov.npix;
-                aov.albedo[3 * px + 0] = so.albedo.x;
-                aov.albedo[3 * px + 1] = so.albedo.y;
-                aov.albedo[3 * px + 2] = so.albedo.z;
-                aov.normal[3 * px + 0] = so.normal.x;
-                aov.normal[3 * px + 1] = so.normal.y;
-                aov.normal[3 * px + 2] = so.normal.z;
-                aov.test[px] = so.test;
-                aov.prim[px] = prim;
-            }
-        }
         if (so.cont) {
             out.ray0[j] = make_float4(ps.o.x, ps.o.y, ps.o.z, u2f(packed));
             out.ray1[j] = make_float4(ps.d.x, ps.d.y, ps.d.z, ps.prev_pdf);
@@ -2091,13 +2092,24 @@ int batch_upload(pt_ctx *ctx) {
 // so that the shared network learns from all views at once, then the other
 // frames view by view.  A launch holds either training frames only, at most
 // `train_capacity` paths of them (and at least one frame), or none.
+constexpr uint32_t kGateMinFrames = 4;
+constexpr uint32_t kGateWindow = 128;  // at most this many evaluation frames (one slot each, mod 128)
+
+// Training frames of a view of `spp` frames in a guided batch render: the
+// first train_spp (0 = all); with the gate at most half of them, so that
+// every view with at least 2 kGateMinFrames frames has frames after its
+// training for the gate to decide and, if it keeps the guide, to restart on.
+uint32_t training_frames(uint32_t spp, uint32_t train_spp, bool gate) {
+    uint32_t t = train_spp == 0 ? spp : std::min(train_spp, spp);
+    if (gate && spp >= 2 * kGateMinFrames) t = std::min(t, spp / 2);
+    return t;
+}
+
 // With `gate`, the second half of the training frames of each view that has
 // frames after its training (and trains on at least kGateMinFrames frames)
 // measure the view's error (kEntryEval), every other one of them without the
 // guide (kEntryPlain); a view rendered entirely by training frames has
 // nothing to gate and samples with the guide throughout.
-constexpr uint32_t kGateMinFrames = 4;
-constexpr uint32_t kGateWindow = 128;  // at most this many evaluation frames (one slot each, mod 128)
 void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::vector<LaunchDesc> &plan,
                 bool train = false, uint32_t train_spp = 0, uint32_t train_capacity = 0, bool gate = false) {
     // (view, frame) in launch order
@@ -2105,7 +2117,7 @@ void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::ve
     std::vector<uint32_t> trained(b.views.size(), 0);
     if (train) {
         for (size_t vi = 0; vi < b.views.size(); ++vi)
-            trained[vi] = train_spp == 0 ? b.views[vi].spp : std::min(train_spp, b.views[vi].spp);
+            trained[vi] = training_frames(b.views[vi].spp, train_spp, gate);
         for (uint32_t f0 = 0;;) {
             uint64_t active = 0;
             for (size_t vi = 0; vi < b.views.size(); ++vi)
@@ -2553,7 +2565,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     for (size_t vi = 0; vi < b.views.size(); ++vi) {
         BatchView &v = b.views[vi];
         paths += (uint64_t)v.width * v.height * v.spp;
-        const uint32_t trained = p.guide_train_spp == 0 ? v.spp : std::min(p.guide_train_spp, v.spp);
+        const uint32_t trained = training_frames(v.spp, p.guide_train_spp, gate);
         // A view that kept its guide restarted its accumulation after training
         // (when it had frames after training).
         if (gate && trained >= kGateMinFrames && trained < v.spp && !gated_views[vi].guide_off)

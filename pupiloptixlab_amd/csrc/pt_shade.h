@@ -150,10 +150,20 @@ PT_HD int32_t select_emitter(const SceneView &s, float p) {
     return -2;
 }
 
+// Receives the AOVs of a camera path's first vertex (albedo, shading normal,
+// the `test` draw) when ShadeParams::aov is set, at the point they are
+// known, so a kernel can store them there instead of carrying them through
+// the rest of the step; the default keeps them in ShadeOut only.
+struct NoAovWrite {
+    PT_HD void operator()(f3, f3, float) const {}
+};
+
 // `feat`: the scene features compiled in (kFeat*; a constant of the calling
 // kernel, so the code of absent features is dropped).
+template <typename AovWrite = NoAovWrite>
 PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps, uint32_t prim, float t_hit,
-                      float b1, float b2, ShadeOut &out, uint32_t feat = kFeatAll) {
+                      float b1, float b2, ShadeOut &out, uint32_t feat = kFeatAll,
+                      const AovWrite &aov_write = AovWrite()) {
     out.cont = false;
     out.done = false;
     out.shadow = false;
@@ -182,12 +192,13 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
             }
         }
         if (primary) {
+            out.test = rng_next(ps.rng);
             if (prm.aov) {
                 out.wrote_aov = true;
                 out.albedo = mk3(0.f);
                 out.normal = mk3(0.f);
+                aov_write(out.albedo, out.normal, out.test);
             }
-            out.test = rng_next(ps.rng);
         } else {
             float mis;
             if (consistent) {
@@ -212,12 +223,13 @@ PT_HD void shade_path(const SceneView &s, const ShadeParams &prm, PathState &ps,
     if (primary) {
         if (emitter_index >= 0)
             ps.L = ps.L + texture_sample(s.emitters[emitter_index].radiance, g.texcoord, s.tex, feat);
+        out.test = rng_next(ps.rng);
         if (prm.aov) {
             out.wrote_aov = true;
             out.albedo = bsdf.c0;
             out.normal = g.normal;
+            aov_write(out.albedo, out.normal, out.test);
         }
-        out.test = rng_next(ps.rng);
     } else if (emitter_index >= 0) {
         const EmitterRec &e = s.emitters[emitter_index];
         f3 rad;
