@@ -426,9 +426,8 @@ int pt_guide_read_samples(pt_ctx *ctx, float *samples, uint32_t max_samples, uin
 enum pt_batch_flags {
     PT_BATCH_GRAPH = 1,    /* capture one launch's kernels in a hipGraph and replay it */
     PT_BATCH_AUTOTUNE = 2, /* pick the launch shape by timing candidates first (pt_batch_autotune) */
-    PT_BATCH_GUIDE_GATE = 4 /* with PT_RENDER_GUIDE_TRAIN: a view of at least 8 frames trains on at most
-                               half of them; for each view with frames after its training (at least 4
-                               training frames), in the second half of its training frames
+    PT_BATCH_GUIDE_GATE = 4 /* with PT_RENDER_GUIDE_TRAIN: for each view with frames after its training
+                               (at least 4 training frames), in the second half of its training frames
                                every other frame samples the BSDF only, and every
                                one of them measures its relMSE against the view's leave-one-out mean.
                                A view whose guided frames reach neither in mean nor in median 1/1.25 of

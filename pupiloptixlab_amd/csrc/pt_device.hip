@@ -2096,13 +2096,12 @@ constexpr uint32_t kGateMinFrames = 4;
 constexpr uint32_t kGateWindow = 128;  // at most this many evaluation frames (one slot each, mod 128)
 
 // Training frames of a view of `spp` frames in a guided batch render: the
-// first train_spp (0 = all); with the gate at most half of them, so that
-// every view with at least 2 kGateMinFrames frames has frames after its
-// training for the gate to decide and, if it keeps the guide, to restart on.
-uint32_t training_frames(uint32_t spp, uint32_t train_spp, bool gate) {
-    uint32_t t = train_spp == 0 ? spp : std::min(train_spp, spp);
-    if (gate && spp >= 2 * kGateMinFrames) t = std::min(t, spp / 2);
-    return t;
+// first train_spp (0 = all).  (Capping them at half of a view's frames, so
+// that every view could be gated, was measured worse: a 16-frame door view
+// trained on 8 frames learned too little to keep the guide, and rendered
+// like the plain tracer, where training on all 16 halves its error.)
+uint32_t training_frames(uint32_t spp, uint32_t train_spp, bool /*gate*/) {
+    return train_spp == 0 ? spp : std::min(train_spp, spp);
 }
 
 // With `gate`, the second half of the training frames of each view that has
