@@ -426,15 +426,15 @@ int pt_guide_read_samples(pt_ctx *ctx, float *samples, uint32_t max_samples, uin
 enum pt_batch_flags {
     PT_BATCH_GRAPH = 1,    /* capture one launch's kernels in a hipGraph and replay it */
     PT_BATCH_AUTOTUNE = 2, /* pick the launch shape by timing candidates first (pt_batch_autotune) */
-    PT_BATCH_GUIDE_GATE = 4 /* with PT_RENDER_GUIDE_TRAIN: for each view with frames after its training
-                               (at least 4 training frames), in the second half of its training frames
-                               every other frame samples the BSDF only, and every
-                               one of them measures its relMSE against the view's leave-one-out mean.
-                               A view whose guided frames reach neither in mean nor in median 1/1.25 of
-                               its plain frames' error renders its remaining frames without the guide
-                               (no network queries); a view that keeps the guide restarts its
-                               accumulation with its first frame after training, so the image holds
-                               only frames of the trained guide */
+    PT_BATCH_GUIDE_GATE = 4 /* with PT_RENDER_GUIDE_TRAIN: each view training on T >= 16 frames measures,
+                               on its training frames [T/4, T/2), every other one sampling the BSDF
+                               only, each frame's relMSE against the view's leave-one-out mean.  A view
+                               whose guided frames reach neither in mean nor in median 1/1.25 of its
+                               plain frames' error stops training there and renders its remaining
+                               frames without the guide (no network queries); a view that keeps the
+                               guide trains on to T and, with frames after T, restarts its
+                               accumulation there, so the image holds only frames of the trained
+                               guide */
 };
 
 typedef struct pt_batch_params {

@@ -22,6 +22,7 @@
 
 #include "../../include/pupil_hip.h"
 #include "pt_bvh.h"
+#include "pt_batch_plan.h"
 #include "pt_bvh_build.h"
 #include "pt_denoise.h"
 #include "pt_guide_state.h"
@@ -312,37 +313,6 @@ struct BatchViewDev {
     float4 *accum, *frame;
     double *accum64;
     void *pad2;
-};
-
-// Launch layout of a batched render (one LaunchDesc per launch, copied to
-// the device before the launch's kernels run).
-constexpr uint32_t kMaxLaunchEntries = 1024;  // frames per launch
-constexpr uint32_t kMaxLaunchSegs = 256;      // runs of consecutive frames of one view per launch
-constexpr uint32_t kEntryTrain = 0x80000000u; // LaunchEntry::view flag: the frame trains the guide
-constexpr uint32_t kEntryPlain = 0x40000000u; // LaunchEntry::view flag: the frame samples the BSDF only
-constexpr uint32_t kEntryEval = 0x20000000u;  // LaunchEntry::view flag: the frame measures its view's error
-constexpr uint32_t kEntryViewMask = 0x1FFFFFFFu;
-
-// Frame `frame_seed` (RNG frame index) of a view occupies the launch's paths
-// [path_begin, path_begin + pixels).
-struct LaunchEntry {
-    uint32_t view, frame_seed, path_begin, pixels;
-};
-// Consecutive frames [first_entry, first_entry + frames) of one view, folded
-// into its film in frame order; its pixels are threads [pix_begin, +pixels).
-struct LaunchSeg {
-    uint32_t view, first_entry, frames, pix_begin;
-    uint32_t sample_cnt0, accumulate;
-    // Gated guided renders: the view restarts its accumulation with its first
-    // frame after training if it keeps the guide (BatchViewDev::guide_off 0),
-    // counting frames from restart_cnt0 instead of sample_cnt0.
-    uint32_t restart, restart_cnt0;
-};
-struct LaunchDesc {
-    uint32_t total_paths, n_entries, n_segs, seg_pixels;
-    uint32_t epoch, phase, train_frames, pad;  // guide training records of this launch
-    LaunchEntry e[kMaxLaunchEntries];
-    LaunchSeg g[kMaxLaunchSegs];
 };
 
 // With `views` set (batched render) the path id carries the view above bit
@@ -857,21 +827,23 @@ __device__ bool gate_stats(const double *acc, uint32_t view, uint32_t plain, dou
     return true;
 }
 
-// Gate of a guided batch after its training frames: a view's remaining
-// frames sample the BSDF only when neither the mean nor the median of its
-// guided evaluation frames' errors is below 1/kGateCost of its plain ones (a
-// guided frame costs more: network queries).  The median keeps one
-// heavy-tailed frame from deciding; the mean keeps the tail in the picture.
-// Views without evaluation frames keep the guide.
+// Gate of a guided batch, for each view whose last evaluation frame is in
+// this launch (kEntryDecide): the view's remaining frames sample the BSDF
+// only when neither the mean nor the median of its guided evaluation frames'
+// errors is below 1/kGateCost of its plain ones (a guided frame costs more:
+// network queries).  The median keeps one heavy-tailed frame from deciding;
+// the mean keeps the tail in the picture.  Views without evaluation frames
+// keep the guide.  One thread per launch entry.
 constexpr double kGateCost = 1.25;
-__global__ void k_guide_gate(BatchViewDev *views, uint32_t n, double *acc, uint32_t *gated_off) {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
-        double gm, gmed, pm, pmed;
-        const bool off = gate_stats(acc, v, 0u, gm, gmed) && gate_stats(acc, v, 1u, pm, pmed) &&
-                         gm * kGateCost >= pm && gmed * kGateCost >= pmed;
-        views[v].guide_off = off ? 1u : 0u;
-        if (off) atomicAdd(gated_off, 1u);
-    }
+__global__ void k_guide_gate(BatchViewDev *views, const LaunchDesc *ld, double *acc, uint32_t *gated_off) {
+    const uint32_t en = blockIdx.x * blockDim.x + threadIdx.x;
+    if (en >= ld->n_entries || !(ld->e[en].view & kEntryDecide)) return;
+    const uint32_t v = ld->e[en].view & kEntryViewMask;
+    double gm, gmed, pm, pmed;
+    const bool off = gate_stats(acc, v, 0u, gm, gmed) && gate_stats(acc, v, 1u, pm, pmed) &&
+                     gm * kGateCost >= pm && gmed * kGateCost >= pmed;
+    views[v].guide_off = off ? 1u : 0u;
+    if (off) atomicAdd(gated_off, 1u);
 }
 
 __global__ void k_guide_gate_reset(BatchViewDev *views, uint32_t n, double *acc, uint32_t *gated_off) {
@@ -2084,109 +2056,6 @@ int batch_upload(pt_ctx *ctx) {
     return PT_OK;
 }
 
-// Packs every view's frames (views in order, frames in order) into launches
-// of at most `capacity` paths.  With `train`, frame f of a view trains the
-// guide when train_spp == 0 or f < train_spp; training frames come first,
-// round-robin over the views (the next k frames of every view still
-// training, k chosen so that one round fills about `train_capacity` paths),
-// so that the shared network learns from all views at once, then the other
-// frames view by view.  A launch holds either training frames only, at most
-// `train_capacity` paths of them (and at least one frame), or none.
-constexpr uint32_t kGateMinFrames = 4;
-constexpr uint32_t kGateWindow = 128;  // at most this many evaluation frames (one slot each, mod 128)
-
-// Training frames of a view of `spp` frames in a guided batch render: the
-// first train_spp (0 = all).  (Capping them at half of a view's frames, so
-// that every view could be gated, was measured worse: a 16-frame door view
-// trained on 8 frames learned too little to keep the guide, and rendered
-// like the plain tracer, where training on all 16 halves its error.)
-uint32_t training_frames(uint32_t spp, uint32_t train_spp, bool /*gate*/) {
-    return train_spp == 0 ? spp : std::min(train_spp, spp);
-}
-
-// With `gate`, the second half of the training frames of each view that has
-// frames after its training (and trains on at least kGateMinFrames frames)
-// measure the view's error (kEntryEval), every other one of them without the
-// guide (kEntryPlain); a view rendered entirely by training frames has
-// nothing to gate and samples with the guide throughout.
-void batch_plan(const BatchState &b, uint32_t capacity, bool accumulate, std::vector<LaunchDesc> &plan,
-                bool train = false, uint32_t train_spp = 0, uint32_t train_capacity = 0, bool gate = false) {
-    // (view, frame) in launch order
-    std::vector<std::pair<uint32_t, uint32_t>> order;
-    std::vector<uint32_t> trained(b.views.size(), 0);
-    if (train) {
-        for (size_t vi = 0; vi < b.views.size(); ++vi)
-            trained[vi] = training_frames(b.views[vi].spp, train_spp, gate);
-        for (uint32_t f0 = 0;;) {
-            uint64_t active = 0;
-            for (size_t vi = 0; vi < b.views.size(); ++vi)
-                if (trained[vi] > f0) active += (uint64_t)b.views[vi].width * b.views[vi].height;
-            if (active == 0) break;
-            const uint32_t k = (uint32_t)std::max<uint64_t>(1, train_capacity / active);
-            for (uint32_t vi = 0; vi < (uint32_t)b.views.size(); ++vi)
-                for (uint32_t f = f0; f < std::min(f0 + k, trained[vi]); ++f) order.emplace_back(vi, f);
-            f0 += k;
-        }
-    }
-    for (uint32_t vi = 0; vi < (uint32_t)b.views.size(); ++vi)
-        for (uint32_t f = trained[vi]; f < b.views[vi].spp; ++f) order.emplace_back(vi, f);
-    plan.clear();
-    LaunchDesc cur;
-    std::memset(&cur, 0, sizeof(cur));
-    // A view has at most one segment per launch: its frames are folded into
-    // its film by one thread per pixel, in order.
-    std::vector<uint8_t> in_launch(b.views.size(), 0);
-    auto flush = [&]() {
-        if (cur.n_entries) plan.push_back(cur);
-        for (uint32_t k = 0; k < cur.n_segs; ++k) in_launch[cur.g[k].view] = 0;
-        std::memset(&cur, 0, sizeof(cur));
-    };
-    for (const auto &vf : order) {
-        const uint32_t vi = vf.first, f = vf.second;
-        const BatchView &v = b.views[vi];
-        const uint32_t npix = v.width * v.height;
-        {
-            const bool new_seg = cur.n_segs == 0 || cur.g[cur.n_segs - 1].view != vi;
-            const bool train_frame = f < trained[vi];
-            const bool train_launch = cur.train_frames > 0;
-            const uint32_t cap = train_frame ? std::min(capacity, train_capacity) : capacity;
-            if ((uint64_t)cur.total_paths + npix > cap || cur.n_entries == kMaxLaunchEntries ||
-                (new_seg && (cur.n_segs == kMaxLaunchSegs || in_launch[vi])) ||
-                (cur.n_entries && train_frame != train_launch))
-                flush();
-            const bool seg_start = cur.n_segs == 0 || cur.g[cur.n_segs - 1].view != vi;
-            LaunchEntry &e = cur.e[cur.n_entries];
-            e.view = vi | (train_frame ? kEntryTrain : 0u);
-            if (gate && train_frame && trained[vi] >= kGateMinFrames && trained[vi] < v.spp &&
-                f >= std::max(trained[vi] / 2, trained[vi] - std::min(trained[vi], kGateWindow))) {
-                e.view |= kEntryEval;
-                if ((f - trained[vi] / 2) & 1u) e.view |= kEntryPlain;
-            }
-            if (train_frame) cur.train_frames += 1;
-            e.frame_seed = v.random_seed + f;
-            e.path_begin = cur.total_paths;
-            e.pixels = npix;
-            if (seg_start) {
-                in_launch[vi] = 1;
-                LaunchSeg &g = cur.g[cur.n_segs++];
-                g.view = vi;
-                g.first_entry = cur.n_entries;
-                g.frames = 0;
-                g.pix_begin = cur.seg_pixels;
-                g.sample_cnt0 = accumulate ? v.sample_cnt + f : 0u;
-                g.accumulate = accumulate ? 1u : 0u;
-                g.restart = (gate && accumulate && !train_frame && trained[vi] >= kGateMinFrames) ? 1u : 0u;
-                g.restart_cnt0 = g.restart ? f - trained[vi] : 0u;
-                cur.seg_pixels += npix;
-            }
-            cur.g[cur.n_segs - 1].frames += 1;
-            cur.n_entries += 1;
-            cur.total_paths += npix;
-        }
-    }
-    flush();
-}
-
 // Brings the BVH up to date with pending shape updates: leaf records from
 // the current geometry, then node boxes level by level from the deepest.
 int bvh_refit(pt_ctx *ctx) {
@@ -2427,9 +2296,17 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     BatchGuide bg;
     bg.infer = (p.render_flags & PT_RENDER_GUIDE) != 0;
     bg.train = (p.render_flags & PT_RENDER_GUIDE_TRAIN) != 0;
-    std::vector<LaunchDesc> plan;
     const bool gate = bg.train && (p.batch_flags & PT_BATCH_GUIDE_GATE) != 0;
-    batch_plan(b, sh.capacity, true, plan, bg.train, p.guide_train_spp, batch_train_capacity(b, sh), gate);
+    const uint32_t nviews = (uint32_t)b.views.size();
+    PlanState ps;
+    ps.next.assign(nviews, 0u);
+    ps.T.assign(nviews, 0u);
+    if (bg.train)
+        for (uint32_t vi = 0; vi < nviews; ++vi) ps.T[vi] = training_frames(b.views[vi].spp, p.guide_train_spp);
+    ps.train_end = ps.T;
+    const uint32_t train_cap = batch_train_capacity(b, sh);
+    std::vector<LaunchDesc> plan;
+    batch_plan(b.views, sh.capacity, true, plan, ps, train_cap, gate);
     if (bg.train && b.gate_views < b.views.size()) {
         dfree(b.d_gate_acc);
         dfree(b.d_gate_off);
@@ -2437,13 +2314,21 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
         PT_HIP(hipMalloc((void **)&b.d_gate_off, sizeof(uint32_t)));
         b.gate_views = (uint32_t)b.views.size();
     }
-    if (plan.size() > b.h_launch_slots) {
-        if (b.h_launch) (void)hipHostFree(b.h_launch);
-        b.h_launch = nullptr;
-        PT_HIP(hipHostMalloc((void **)&b.h_launch, plan.size() * sizeof(LaunchDesc), hipHostMallocDefault));
-        b.h_launch_slots = (uint32_t)plan.size();
-    }
-    std::memcpy(b.h_launch, plan.data(), plan.size() * sizeof(LaunchDesc));
+    // Launch descriptors are copied to the device from pinned slots, slot k
+    // for the plan's launch k; (re)staged only while no copy is in flight (at
+    // the start of a render or after a stream synchronization).
+    auto stage_plan = [&]() -> int {
+        if (plan.size() > b.h_launch_slots) {
+            if (b.h_launch) (void)hipHostFree(b.h_launch);
+            b.h_launch = nullptr;
+            b.h_launch_slots = 0;
+            PT_HIP(hipHostMalloc((void **)&b.h_launch, plan.size() * sizeof(LaunchDesc), hipHostMallocDefault));
+            b.h_launch_slots = (uint32_t)plan.size();
+        }
+        std::memcpy(b.h_launch, plan.data(), plan.size() * sizeof(LaunchDesc));
+        return PT_OK;
+    };
+    if ((rc = stage_plan()) != PT_OK) return rc;
     ShadeParams spar;
     spar.max_depth = max_depth;
     spar.rr_depth = p.rr_depth;
@@ -2469,14 +2354,12 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     }
     rc = PT_OK;
     GuideState &gs = ctx->guide;
-    size_t last_train = plan.size();
-    for (size_t k = 0; k < plan.size(); ++k)
-        if (plan[k].train_frames) last_train = k;
-    const uint32_t nviews = (uint32_t)b.views.size();
     if (bg.train)  // a training render decides the gate afresh (views start guided)
         hipLaunchKernelGGL(k_guide_gate_reset, dim3((uint32_t)std::min<size_t>(
                                (nviews * kGateDoublesPerView + 255) / 256, 1024)), dim3(256), 0, ctx->stream,
                            b.d_views, nviews, b.d_gate_acc, b.d_gate_off);
+    uint32_t executed = 0;
+    std::vector<BatchViewDev> gated_views;
     for (size_t k = 0; k < plan.size() && rc == PT_OK; ++k) {
         LaunchDesc &hl = b.h_launch[k];
         const bool train_here = bg.train && hl.train_frames > 0;
@@ -2493,18 +2376,21 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
                 e = hipGetLastError();
             }
         }
+        bool decide = false;
         if (e == hipSuccess && gate && train_here) {
             uint32_t eval_px = 0;
-            for (uint32_t en = 0; en < hl.n_entries; ++en)
+            for (uint32_t en = 0; en < hl.n_entries; ++en) {
                 if (hl.e[en].view & kEntryEval) eval_px = std::max(eval_px, hl.e[en].pixels);
+                if (hl.e[en].view & kEntryDecide) decide = true;
+            }
             if (eval_px) {
                 const uint32_t bx = std::min((eval_px + 255u) / 256u, 64u);
                 hipLaunchKernelGGL(k_eval_error, dim3(bx, hl.n_entries), dim3(256), 0, ctx->stream, ctx->d_out_l,
                                    (const LaunchDesc *)b.d_launch, (const BatchViewDev *)b.d_views, b.d_gate_acc);
             }
-            if (k == last_train)
-                hipLaunchKernelGGL(k_guide_gate, dim3((nviews + 63) / 64), dim3(64), 0, ctx->stream, b.d_views,
-                                   nviews, b.d_gate_acc, b.d_gate_off);
+            if (decide)
+                hipLaunchKernelGGL(k_guide_gate, dim3((hl.n_entries + 63) / 64), dim3(64), 0, ctx->stream,
+                                   b.d_views, (const LaunchDesc *)b.d_launch, b.d_gate_acc, b.d_gate_off);
             e = hipGetLastError();
         }
         if (e == hipSuccess && train_here) {
@@ -2542,6 +2428,30 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
         if (e != hipSuccess) {
             ctx->error = std::string("pt_batch_render: ") + hipGetErrorString(e);
             rc = PT_ERR_HIP;
+            break;
+        }
+        ++executed;
+        plan_advance(ps, hl, b.views);
+        if (decide && k + 1 < plan.size()) {
+            // Views the gate switched off stop training now: read the
+            // decisions and plan the remaining frames again.
+            gated_views.resize(nviews);
+            e = hipStreamSynchronize(ctx->stream);
+            if (e == hipSuccess)
+                e = hipMemcpy(gated_views.data(), b.d_views, nviews * sizeof(BatchViewDev), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) {
+                ctx->error = std::string("pt_batch_render: ") + hipGetErrorString(e);
+                rc = PT_ERR_HIP;
+                break;
+            }
+            bool changed = false;
+            for (uint32_t vi = 0; vi < nviews; ++vi)
+                if (gated_views[vi].guide_off) changed |= plan_gate_off(ps, vi);
+            if (changed) {
+                batch_plan(b.views, sh.capacity, true, plan, ps, train_cap, gate);
+                if ((rc = stage_plan()) != PT_OK) break;
+                k = (size_t)-1;  // continue with the new plan's first launch
+            }
         }
     }
     hipError_t e = hipStreamSynchronize(ctx->stream);
@@ -2553,7 +2463,6 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
         ctx->error = std::string("pt_batch_render: ") + hipGetErrorString(e);
         return PT_ERR_HIP;
     }
-    std::vector<BatchViewDev> gated_views;
     if (gate) {
         PT_HIP(hipMemcpy(&b.stats.guide_gated_off, b.d_gate_off, sizeof(uint32_t), hipMemcpyDeviceToHost));
         gated_views.resize(b.views.size());
@@ -2564,7 +2473,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     for (size_t vi = 0; vi < b.views.size(); ++vi) {
         BatchView &v = b.views[vi];
         paths += (uint64_t)v.width * v.height * v.spp;
-        const uint32_t trained = training_frames(v.spp, p.guide_train_spp, gate);
+        const uint32_t trained = training_frames(v.spp, p.guide_train_spp);
         // A view that kept its guide restarted its accumulation after training
         // (when it had frames after training).
         if (gate && trained >= kGateMinFrames && trained < v.spp && !gated_views[vi].guide_off)
@@ -2574,7 +2483,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
         v.random_seed += v.spp;
     }
     *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (launches) *launches = (uint32_t)plan.size();
+    if (launches) *launches = executed;
     b.stats.paths += paths;
     return PT_OK;
 }

@@ -68,6 +68,7 @@ struct TrainBuffers {
     uint16_t *xt;       // [layer][k 64][batch] layer inputs (FP16 bits)
     uint16_t *dyt;      // [layer][n 64][batch] scaled layer output gradients (FP16 bits)
     float2 *table_grad; // [level][T]
+    uint32_t *touched;  // [grid]: set to 1 by every step sample of the grid (Adam visits only those grids)
 };
 
 struct LossParams {
@@ -139,6 +140,7 @@ __global__ __launch_bounds__(kTrainBlock) void k_guide_train(GuideView G, const 
     }
     const float weight = (in_range && grid_ok) ? aux.w : 0.f;
     const bool valid = weight > 0.f;
+    if (valid) B.touched[grid] = 1u;  // same value from every writer
     const f3 u = guide_grid_pos(G, grid, mk3(p4.x, p4.y, p4.z));
 
     // Forward (inputs of every layer kept for the backward pass).
@@ -398,30 +400,39 @@ __device__ __forceinline__ bool adam_step_of(const GuideCtl *ctl, uint32_t apply
 // batch-summed gradient divided by the sample-weight total: entries no sample
 // touched (gradient exactly zero) keep their values and moments.  Gradients
 // are cleared.
-__global__ void k_guide_adam_table(float2 *p, float2 *m, float2 *v, float2 *g, uint32_t *p16, size_t n, AdamParams a,
+// Grids [grid_lo, grid_hi) of grid_entries entries each; grids that no
+// sample of the step touched have an all-zero gradient and are skipped whole
+// (with one grid per batch view, a step's samples name a few of them).
+__global__ void k_guide_adam_table(float2 *p, float2 *m, float2 *v, float2 *g, uint32_t *p16, size_t grid_entries,
+                                   uint32_t grid_lo, uint32_t grid_hi, const uint32_t *touched, AdamParams a,
                                    const GuideCtl *ctl, uint32_t apply) {
     if (!ctl->active) return;
     float c1, c2;
     const bool step = adam_step_of(ctl, apply, a, c1, c2);
     const float inv = inv_valid(ctl);
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-        float2 gg = g[e];
-        if (gg.x == 0.f && gg.y == 0.f) continue;
-        g[e] = make_float2(0.f, 0.f);
-        if (!step) continue;
-        gg.x *= inv;
-        gg.y *= inv;
-        float2 mm = m[e], vv = v[e], pp = p[e];
-        mm.x = a.b1 * mm.x + (1.f - a.b1) * gg.x;
-        mm.y = a.b1 * mm.y + (1.f - a.b1) * gg.y;
-        vv.x = a.b2 * vv.x + (1.f - a.b2) * gg.x * gg.x;
-        vv.y = a.b2 * vv.y + (1.f - a.b2) * gg.y * gg.y;
-        pp.x -= a.lr * (mm.x / c1) / (sqrtf(vv.x / c2) + a.eps);
-        pp.y -= a.lr * (mm.y / c1) / (sqrtf(vv.y / c2) + a.eps);
-        m[e] = mm;
-        v[e] = vv;
-        p[e] = pp;
-        p16[e] = (uint32_t)hbits((_Float16)pp.x) | ((uint32_t)hbits((_Float16)pp.y) << 16);
+    for (uint32_t gi = grid_lo; gi < grid_hi; ++gi) {
+        if (!touched[gi]) continue;
+        const size_t e0 = (size_t)gi * grid_entries, e1 = e0 + grid_entries;
+        const size_t stride = (size_t)gridDim.x * blockDim.x;
+        for (size_t e = e0 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < e1; e += stride) {
+            float2 gg = g[e];
+            if (gg.x == 0.f && gg.y == 0.f) continue;
+            g[e] = make_float2(0.f, 0.f);
+            if (!step) continue;
+            gg.x *= inv;
+            gg.y *= inv;
+            float2 mm = m[e], vv = v[e], pp = p[e];
+            mm.x = a.b1 * mm.x + (1.f - a.b1) * gg.x;
+            mm.y = a.b1 * mm.y + (1.f - a.b1) * gg.y;
+            vv.x = a.b2 * vv.x + (1.f - a.b2) * gg.x * gg.x;
+            vv.y = a.b2 * vv.y + (1.f - a.b2) * gg.y * gg.y;
+            pp.x -= a.lr * (mm.x / c1) / (sqrtf(vv.x / c2) + a.eps);
+            pp.y -= a.lr * (mm.y / c1) / (sqrtf(vv.y / c2) + a.eps);
+            m[e] = mm;
+            v[e] = vv;
+            p[e] = pp;
+            p16[e] = (uint32_t)hbits((_Float16)pp.x) | ((uint32_t)hbits((_Float16)pp.y) << 16);
+        }
     }
 }
 
@@ -429,13 +440,16 @@ constexpr int kFinishBlock = 1024;
 
 // Single block: Adam on the MLP weights (gradient = w_grad / weight total;
 // w_grad is cleared), then the FP16 MFMA operands are re-packed and the step
-// is closed (step count, dynamic loss scale, loss bookkeeping).
+// is closed (step count, dynamic loss scale, loss bookkeeping, the grids'
+// touched flags cleared).
 //   fwd[layer][rt][chunk][lane][s] = W[perm(chunk, g, s)][16 rt + i]
 //   bwd[layer][kt][chunk][lane][s] = W[16 kt + i][perm(chunk, g, s)]
 // apply = 0 only reports the gradient (grad_out) and closes the step.
 __global__ __launch_bounds__(kFinishBlock) void k_guide_finish(float *p, float *m, float *v, float *w_grad,
                                                               AdamParams a, GuideCtl *ctl, uint32_t apply,
-                                                              float *grad_out, _Float16 *fwd, _Float16 *bwd) {
+                                                              float *grad_out, _Float16 *fwd, _Float16 *bwd,
+                                                              uint32_t *touched, uint32_t grids) {
+    for (uint32_t gi = threadIdx.x; gi < grids; gi += kFinishBlock) touched[gi] = 0u;
     const bool active = ctl->active != 0u;
     float c1, c2;
     const bool step = active && adam_step_of(ctl, apply, a, c1, c2);
@@ -739,6 +753,7 @@ void GuideState::release() {
     gfree(table_m);
     gfree(table_v);
     gfree(table_g);
+    gfree(grid_touched);
     gfree(w32);
     gfree(w_m);
     gfree(w_v);
@@ -825,6 +840,7 @@ bool GuideState::init(const pt_guide_config &c, const float *lo, const float *hi
     GUIDE_HIP(hipMalloc((void **)&table_m, nt * sizeof(float2)));
     GUIDE_HIP(hipMalloc((void **)&table_v, nt * sizeof(float2)));
     GUIDE_HIP(hipMalloc((void **)&table_g, nt * sizeof(float2)));
+    GUIDE_HIP(hipMalloc((void **)&grid_touched, n_grids * sizeof(uint32_t)));
     GUIDE_HIP(hipMalloc((void **)&w32, nw * sizeof(float)));
     GUIDE_HIP(hipMalloc((void **)&w_m, nw * sizeof(float)));
     GUIDE_HIP(hipMalloc((void **)&w_v, nw * sizeof(float)));
@@ -849,6 +865,7 @@ bool GuideState::init(const pt_guide_config &c, const float *lo, const float *hi
     GUIDE_HIP(hipMemset(table_m, 0, nt * sizeof(float2)));
     GUIDE_HIP(hipMemset(table_v, 0, nt * sizeof(float2)));
     GUIDE_HIP(hipMemset(table_g, 0, nt * sizeof(float2)));
+    GUIDE_HIP(hipMemset(grid_touched, 0, n_grids * sizeof(uint32_t)));
     GUIDE_HIP(hipMemset(w_m, 0, nw * sizeof(float)));
     GUIDE_HIP(hipMemset(w_v, 0, nw * sizeof(float)));
     GUIDE_HIP(hipMemset(w_grad, 0, nw * sizeof(float)));
@@ -926,7 +943,7 @@ bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool
     const GuideView G = view();
     hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, first, batch, ring_n ? ring_n : sample_fill,
                        fp_norm, ring_n ? 1u : 0u);
-    TrainBuffers B{xt, dyt, table_g};
+    TrainBuffers B{xt, dyt, table_g, grid_touched};
     const uint32_t waves = batch / 16;
     hipLaunchKernelGGL(k_guide_train, dim3((waves + 3) / 4), dim3(kTrainBlock), 0, stream, G, samples, batch, ctl, B,
                        LossParams{cfg.loss_flags, cfg.l2_weight}, first, ring_n, ring_n ? ring_n / batch : 0u);
@@ -935,11 +952,11 @@ bool GuideState::launch_step(uint32_t first, uint32_t batch, float fp_norm, bool
     const AdamParams at{cfg.learning_rate, 0.9f, 0.99f, 1e-15f};
     const AdamParams aw{cfg.learning_rate, 0.9f, 0.99f, 1e-8f};
     // Sparse Adam over the grids the step's samples can touch.
-    const size_t e0 = (size_t)grid_lo * grid_entries, ne = (size_t)(grid_hi - grid_lo) * grid_entries;
-    hipLaunchKernelGGL(k_guide_adam_table, dim3(2048), dim3(256), 0, stream, table32 + e0, table_m + e0,
-                       table_v + e0, table_g + e0, table16 + e0, ne, at, ctl, apply ? 1u : 0u);
+    hipLaunchKernelGGL(k_guide_adam_table, dim3(2048), dim3(256), 0, stream, table32, table_m, table_v, table_g,
+                       table16, grid_entries, grid_lo, grid_hi, (const uint32_t *)grid_touched, at, ctl,
+                       apply ? 1u : 0u);
     hipLaunchKernelGGL(k_guide_finish, dim3(1), dim3(kFinishBlock), 0, stream, w32, w_m, w_v, w_grad, aw, ctl,
-                       apply ? 1u : 0u, (float *)nullptr, fwd_a, bwd_a);
+                       apply ? 1u : 0u, (float *)nullptr, fwd_a, bwd_a, grid_touched, grids);
     GUIDE_HIP(hipGetLastError());
     ++steps_launched;
     return true;
@@ -1004,7 +1021,7 @@ bool GuideState::gradients(uint32_t n, float fp_norm, float *table_grad, float *
     if (!ensure_batch(n, err)) return false;
     const GuideView G = view();
     hipLaunchKernelGGL(k_guide_begin, dim3(1), dim3(64), 0, stream, ctl, 0u, n, sample_fill, fp_norm, 0u);
-    TrainBuffers B{xt, dyt, table_g};
+    TrainBuffers B{xt, dyt, table_g, grid_touched};
     hipLaunchKernelGGL(k_guide_train, dim3((n / 16 + 3) / 4), dim3(kTrainBlock), 0, stream, G, samples, n, ctl, B,
                        LossParams{cfg.loss_flags, cfg.l2_weight}, 0u, 0u, 0u);
     hipLaunchKernelGGL(k_guide_dw, dim3((n + kGuideDwSlice - 1) / kGuideDwSlice, 4), dim3(64), 0, stream, xt, dyt,
@@ -1025,7 +1042,7 @@ bool GuideState::gradients(uint32_t n, float fp_norm, float *table_grad, float *
     GUIDE_HIP(hipMalloc((void **)&d_wg, 4 * 64 * 64 * sizeof(float)));
     const AdamParams aw{cfg.learning_rate, 0.9f, 0.99f, 1e-8f};
     hipLaunchKernelGGL(k_guide_finish, dim3(1), dim3(kFinishBlock), 0, stream, w32, w_m, w_v, w_grad, aw, ctl, 0u,
-                       d_wg, fwd_a, bwd_a);
+                       d_wg, fwd_a, bwd_a, grid_touched, grids);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemsetAsync(table_g, 0, table_entries * sizeof(float2), stream);
     if (e == hipSuccess) e = hipStreamSynchronize(stream);

@@ -80,6 +80,7 @@ public:
     // parameters and optimizer moments
     uint32_t *table16 = nullptr;  // FP16 feature pairs used by the kernels
     float2 *table32 = nullptr, *table_m = nullptr, *table_v = nullptr, *table_g = nullptr;
+    uint32_t *grid_touched = nullptr;  // per grid: a sample of the current step names it
     float *w32 = nullptr, *w_m = nullptr, *w_v = nullptr, *w_grad = nullptr;
     _Float16 *fwd_a = nullptr, *bwd_a = nullptr;
     GuideCtl *ctl = nullptr;
@@ -122,7 +123,7 @@ public:
     bool train(uint32_t steps, float fp_norm, uint32_t grid_lo, uint32_t grid_hi, std::string &err);
     // Steps over the sample ring (ring_n = m B slots): step k takes every m-th
     // slot from offset (first_step + k) mod m, so each batch spans the whole
-    // ring; Adam visits every grid.  Ring samples are normalized per
+    // ring; Adam visits the grids the batch's samples name.  Ring samples are normalized per
     // collection, so the selection-loss normalization is 1 (fp_norm unused).
     bool train_ring(uint32_t steps, float fp_norm, uint32_t first_step, uint32_t ring_n, std::string &err);
     // Host-sample entry points (tests / external trainers).
