@@ -485,6 +485,17 @@ int pt_batch_reset(pt_ctx *ctx);
 int pt_batch_autotune(pt_ctx *ctx, const pt_batch_params *params, pt_batch_tuning *out);
 int pt_batch_get_stats(pt_ctx *ctx, pt_batch_stats *out);
 
+/* What the gate of the last gated guided render measured for one view
+ * (PT_BATCH_GUIDE_GATE): mean and median per-frame relMSE of its guided and
+ * of its BSDF-only evaluation frames, their counts, and whether the view's
+ * guide was switched off.  No reference counterpart: the gate belongs to the
+ * learned guide; this reports why a view kept or lost it. */
+typedef struct pt_batch_gate_info {
+    double guided_mean, guided_median, plain_mean, plain_median;
+    uint32_t guided_frames, plain_frames, guide_off, pad;
+} pt_batch_gate_info;
+int pt_batch_get_gate(pt_ctx *ctx, uint32_t view, pt_batch_gate_info *out);
+
 /* ---- host scene helpers (resource::Scene / World, headless) ----------- */
 typedef struct pt_host_scene pt_host_scene;
 

@@ -167,6 +167,15 @@ class BatchStats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchGateInfo(C.Structure):
+    _fields_ = [("guided_mean", C.c_double), ("guided_median", C.c_double), ("plain_mean", C.c_double),
+                ("plain_median", C.c_double), ("guided_frames", C.c_uint32), ("plain_frames", C.c_uint32),
+                ("guide_off", C.c_uint32), ("pad", C.c_uint32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
 class GenParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("width", C.c_uint32), ("height", C.c_uint32), ("grid", C.c_uint32),
                 ("spheres", C.c_uint32), ("max_depth", C.c_uint32), ("albedo", C.c_float), ("radiance", C.c_float),
@@ -225,6 +234,7 @@ _SIGNATURES = {
     "pt_batch_reset": (C.c_int, [C.c_void_p]),
     "pt_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(BatchParams), C.POINTER(BatchTuning)]),
     "pt_batch_get_stats": (C.c_int, [C.c_void_p, C.POINTER(BatchStats)]),
+    "pt_batch_get_gate": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(BatchGateInfo)]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -674,3 +684,11 @@ class PathTracer:
         s = BatchStats()
         _check(lib().pt_batch_get_stats(self._ctx, C.byref(s)), "pt_batch_get_stats", self._ctx)
         return s.as_dict()
+
+    def batch_gate(self, view: int) -> dict:
+        """What the last gated guided render measured for `view`: mean and
+        median per-frame relMSE of its guided and BSDF-only evaluation frames,
+        their counts, and whether its guide was switched off."""
+        g = BatchGateInfo()
+        _check(lib().pt_batch_get_gate(self._ctx, view, C.byref(g)), "pt_batch_get_gate", self._ctx)
+        return g.as_dict()

@@ -747,7 +747,7 @@ __global__ void k_accumulate_batch(const float4 *out_l, const LaunchDesc *ld, co
 // (frame index mod kGateSlots), the frame's error and a frame count.
 constexpr uint32_t kGateSlots = 128;
 constexpr size_t kGateDoublesPerView = 2 * kGateSlots * 2;
-__device__ __forceinline__ size_t gate_slot(uint32_t view, uint32_t plain, uint32_t slot) {
+__host__ __device__ __forceinline__ size_t gate_slot(uint32_t view, uint32_t plain, uint32_t slot) {
     return (((size_t)view * 2 + plain) * kGateSlots + slot) * 2;
 }
 
@@ -4062,6 +4062,44 @@ static int pt_batch_autotune_unguarded(pt_ctx *ctx, const pt_batch_params *param
     b.tuning.best_ns_per_path = best_ns;
     b.tuning.default_ns_per_path = default_ns;
     if (out) *out = b.tuning;
+    return PT_OK;
+}
+
+int pt_batch_get_gate(pt_ctx *ctx, uint32_t view, pt_batch_gate_info *out) {
+    if (!ctx || !out) return PT_ERR_INVALID_ARG;
+    BatchState &b = ctx->batch;
+    if (view >= b.views.size()) {
+        ctx->error = "pt_batch_get_gate: view out of range";
+        return PT_ERR_INVALID_ARG;
+    }
+    std::memset(out, 0, sizeof(*out));
+    if (!b.d_gate_acc || view >= b.gate_views) return PT_OK;  // no gated render yet
+    (void)hipSetDevice(ctx->device);
+    std::vector<double> acc(kGateDoublesPerView);
+    BatchViewDev dv;
+    PT_HIP(hipStreamSynchronize(ctx->stream));
+    PT_HIP(hipMemcpy(acc.data(), b.d_gate_acc + (size_t)view * kGateDoublesPerView,
+                     kGateDoublesPerView * sizeof(double), hipMemcpyDeviceToHost));
+    PT_HIP(hipMemcpy(&dv, (const BatchViewDev *)b.d_views + view, sizeof(dv), hipMemcpyDeviceToHost));
+    for (uint32_t plain = 0; plain < 2; ++plain) {
+        std::vector<double> x;  // per-frame errors, as k_guide_gate forms them
+        for (uint32_t slot = 0; slot < kGateSlots; ++slot) {
+            const size_t k = gate_slot(0u, plain, slot);
+            if (acc[k + 1] > 0.0) x.push_back(acc[k] / acc[k + 1]);
+        }
+        double mean = 0.0, median = 0.0;
+        if (!x.empty()) {
+            std::sort(x.begin(), x.end());
+            for (double e : x) mean += e;
+            mean /= (double)x.size();
+            const size_t n = x.size();
+            median = (n & 1u) ? x[n / 2] : 0.5 * (x[n / 2 - 1] + x[n / 2]);
+        }
+        (plain ? out->plain_mean : out->guided_mean) = mean;
+        (plain ? out->plain_median : out->guided_median) = median;
+        (plain ? out->plain_frames : out->guided_frames) = (uint32_t)x.size();
+    }
+    out->guide_off = dv.guide_off;
     return PT_OK;
 }
 

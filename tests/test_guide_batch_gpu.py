@@ -276,17 +276,19 @@ def test_batch_guided_training_gains_and_stays_unbiased(tracer):
 
 
 def test_batch_guide_gate(tracer):
-    """PT_BATCH_GUIDE_GATE: a door-room view (320x180, 160 spp) and a terrain
-    view (128x72, 160 spp) train on their first 48 frames; in the second half
-    of training every other frame samples the BSDF only, and each view keeps
-    the guide only if its guided frames show clearly lower per-frame relMSE.
+    """PT_BATCH_GUIDE_GATE: a door-room view (320x180, 256 spp) and a terrain
+    view (128x72, 256 spp) train on their first 192 frames; frames 48..95
+    alternate between guided and BSDF-only sampling, and each view keeps the
+    guide only if its guided frames show clearly lower per-frame relMSE
+    (at 320x180 the door view's guide needs about 48 frames, ~2.8M paths, to
+    pay: measured 240 vs 260 on frames 24..47, 30 vs 190 on frames 48..95).
     The door view keeps it (and its image holds only frames after training),
-    the terrain view, lit directly, does not; both images stay unbiased
-    against 2048-frame FP64 plain references, and the door view's relMSE is
-    below the plain batch's at the same samples."""
-    print("case: guided batch with the gate, door 320x180 + terrain 128x72, 160 spp, train 48", flush=True)
-    views = [(pt.HostScene.generate(pt.SCENE_DOOR_ROOM, seed=0, width=320, height=180), 160),
-             (pt.HostScene.generate(pt.SCENE_TERRAIN, seed=5, width=128, height=72, grid=96, spheres=16), 160)]
+    the terrain view, lit directly, does not and stops training at frame 96;
+    both images stay unbiased against 2048-frame FP64 plain references, and
+    the door view's relMSE is below the plain batch's at the same samples."""
+    print("case: guided batch with the gate, door 320x180 + terrain 128x72, 256 spp, train 192", flush=True)
+    views = [(pt.HostScene.generate(pt.SCENE_DOOR_ROOM, seed=0, width=320, height=180), 256),
+             (pt.HostScene.generate(pt.SCENE_TERRAIN, seed=5, width=128, height=72, grid=96, spheres=16), 256)]
     refs = []
     for scene, _ in views:
         tracer.set_scene(scene)
@@ -301,10 +303,16 @@ def test_batch_guide_gate(tracer):
     tracer.guide_enable_batch(seed=5)
     # narrow launches (about two door frames), so the guide is updated between
     # training frames as it is for 1080p views
-    tracer.batch_render(mis_mode=pt.MIS_CONSISTENT, graph=True, guide=True, guide_train=True, guide_train_spp=48,
+    tracer.batch_render(mis_mode=pt.MIS_CONSISTENT, graph=True, guide=True, guide_train=True, guide_train_spp=192,
                         guide_gate=True, paths_per_launch=2 * 320 * 180)
     gated = tracer.batch_stats()["guide_gated_off"]
     print(f"views gated off: {gated}")
+    info = [tracer.batch_gate(k) for k in range(2)]
+    for k in range(2):
+        print(f"view {k} gate: {info[k]}")
+    assert [i["guide_off"] for i in info] == [0, 1]
+    assert all(i["guided_frames"] == 24 and i["plain_frames"] == 24 for i in info)
+    assert info[0]["guided_mean"] * 1.25 < info[0]["plain_mean"]
     for k in range(2):
         img = tracer.batch_read(k)[..., :3].astype(np.float64)
         assert np.all(np.isfinite(img))
