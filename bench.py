@@ -341,6 +341,7 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
     e_guided = errors()
     info = t.guide_info()
     gated_off = t.batch_stats()["guide_gated_off"] if gate else 0
+    gates = [t.batch_gate(k) for k in range(len(views))] if gate else [None] * len(views)
     log(f"guided batch: guided {t_guided:.2f}s")
     m = max(1, int(round(t_guided / t_plain1)))
     t.batch_reset()
@@ -352,8 +353,12 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
     e_plain = errors()
     t.close()
     per_view = [{"scene": kind, "triangles": tr, "spp": spp, "relmse_plain_equal_time": round(ep, 5),
-                 "relmse_plain_same_spp": round(e1, 5), "relmse_guided": round(eg, 5)}
-                for (kind, _), tr, spp, ep, e1, eg in zip(views, tris, spps, e_plain, e_plain1, e_guided)]
+                 "relmse_plain_same_spp": round(e1, 5), "relmse_guided": round(eg, 5),
+                 **({"gate": {"guide_off": g["guide_off"],
+                              "frame_relmse_guided": [round(g["guided_mean"], 4), round(g["guided_median"], 4)],
+                              "frame_relmse_plain": [round(g["plain_mean"], 4), round(g["plain_median"], 4)],
+                              "frames": [g["guided_frames"], g["plain_frames"]]}} if g else {})}
+                for (kind, _), tr, spp, ep, e1, eg, g in zip(views, tris, spps, e_plain, e_plain1, e_guided, gates)]
     mean = lambda xs: float(np.mean(xs)) if xs else None
     # The plain side ran m whole batches (m T_plain, not exactly T_guided):
     # scaled to T_guided with relMSE proportional to 1 / time.
