@@ -449,6 +449,9 @@ typedef struct pt_batch_params {
     uint32_t trace_blocks_per_cu;/* traversal blocks per CU, 0 = tuned or resident maximum */
     uint32_t guide_train_spp;    /* with PT_RENDER_GUIDE_TRAIN: each view trains the guide on its first
                                     guide_train_spp frames of the call (0 = all frames) */
+    uint32_t train_paths_per_launch; /* with PT_RENDER_GUIDE_TRAIN: paths per training launch (at least
+                                    one frame of the largest view), 0 = 8M; the network is updated
+                                    after every training launch */
 } pt_batch_params;
 
 typedef struct pt_batch_tuning {

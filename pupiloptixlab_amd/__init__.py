@@ -145,7 +145,7 @@ class BatchParams(C.Structure):
     _fields_ = [("rr_depth", C.c_uint32), ("mis_mode", C.c_uint32), ("render_flags", C.c_uint32),
                 ("batch_flags", C.c_uint32), ("paths_per_launch", C.c_uint32), ("shade_block", C.c_uint32),
                 ("shade_blocks_per_cu", C.c_uint32), ("trace_blocks_per_cu", C.c_uint32),
-                ("guide_train_spp", C.c_uint32)]
+                ("guide_train_spp", C.c_uint32), ("train_paths_per_launch", C.c_uint32)]
 
 
 class BatchTuning(C.Structure):
@@ -639,26 +639,27 @@ class PathTracer:
     @staticmethod
     def _batch_params(rr_depth, mis_mode, fp64, timing, graph, autotune, paths_per_launch, shade_block,
                       shade_blocks_per_cu, trace_blocks_per_cu, guide=False, guide_train=False, guide_train_spp=0,
-                      guide_gate=False):
+                      guide_gate=False, train_paths_per_launch=0):
         rf = (RENDER_FP64_ACCUM if fp64 else 0) | (RENDER_TIMING if timing else 0)
         rf |= (RENDER_GUIDE if guide else 0) | (RENDER_GUIDE_TRAIN if guide_train else 0)
         bf = (BATCH_GRAPH if graph else 0) | (BATCH_AUTOTUNE if autotune else 0) | (BATCH_GUIDE_GATE if guide_gate else 0)
         return BatchParams(rr_depth, mis_mode, rf, bf, paths_per_launch, shade_block, shade_blocks_per_cu,
-                           trace_blocks_per_cu, guide_train_spp)
+                           trace_blocks_per_cu, guide_train_spp, train_paths_per_launch)
 
     def batch_render(self, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE, fp64: bool = False,
                      timing: bool = False, graph: bool = False, autotune: bool = False, paths_per_launch: int = 0,
                      shade_block: int = 0, shade_blocks_per_cu: int = 0, trace_blocks_per_cu: int = 0,
                      guide: bool = False, guide_train: bool = False, guide_train_spp: int = 0,
-                     guide_gate: bool = False) -> None:
+                     guide_gate: bool = False, train_paths_per_launch: int = 0) -> None:
         """Renders every view's frames.  guide / guide_train (after
         guide_enable_batch, MIS_CONSISTENT): sample the batch guide / train it
         on the first guide_train_spp frames of each view (0 = all).
         guide_gate (with guide_train): switch the guide off for views whose
-        guided training frames show no lower error than their plain ones."""
+        guided training frames show no lower error than their plain ones.
+        train_paths_per_launch: width of a training launch (0 = 8M paths)."""
         p = self._batch_params(rr_depth, mis_mode, fp64, timing, graph, autotune, paths_per_launch, shade_block,
                                shade_blocks_per_cu, trace_blocks_per_cu, guide, guide_train, guide_train_spp,
-                               guide_gate)
+                               guide_gate, train_paths_per_launch)
         _check(lib().pt_batch_render(self._ctx, C.byref(p)), "pt_batch_render", self._ctx)
 
     def batch_autotune(self, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE, graph: bool = False,

@@ -40,7 +40,8 @@ def test_struct_layouts_match_header():
     assert pt.SceneDesc.bitmaps.offset == 240 and pt.SceneDesc.env_to_world.offset == 256
     assert C.sizeof(pt.SceneDesc) == 296
     assert C.sizeof(pt.SphereDesc) == 48 and C.sizeof(pt.BitmapDesc) == 24
-    assert C.sizeof(pt.BatchParams) == 36 and pt.BatchParams.guide_train_spp.offset == 32
+    assert C.sizeof(pt.BatchParams) == 40 and pt.BatchParams.guide_train_spp.offset == 32
+    assert pt.BatchParams.train_paths_per_launch.offset == 36
     assert pt.GuideInfo.grids.offset == 180 and C.sizeof(pt.GuideInfo) == 184
 
 
