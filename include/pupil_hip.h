@@ -429,7 +429,7 @@ enum pt_batch_flags {
     PT_BATCH_GUIDE_GATE = 4 /* with PT_RENDER_GUIDE_TRAIN: each view training on T >= 16 frames measures,
                                on its training frames [T/4, T/2), every other one sampling the BSDF
                                only, each frame's relMSE against the view's leave-one-out mean.  A view
-                               whose guided frames reach neither in mean nor in median 1/1.25 of its
+                               whose guided frames reach neither in mean nor in median 1/1.1 of its
                                plain frames' error stops training there and renders its remaining
                                frames without the guide (no network queries); a view that keeps the
                                guide trains on to T and, with frames after T, restarts its
