@@ -188,8 +188,8 @@ def guided_experiment(pt, ref_spp, plain_spp, train_spp, ref_g_spp=1024):
     Reference: ref_spp plain frames accumulated in FP64 (the headline ratio);
     both renders are also judged against a guided FP64 reference of ref_g_spp
     frames (vs_guided_reference).  Plain: plain_spp frames, timed.  Guided: trained online on its own paths during the first
-    train_spp frames, then rendering continues in 8-frame calls until the plain
-    run's time is used up; the guided time includes training.  Both use
+    train_spp frames, then rendering continues, in calls sized to the time left,
+    until the plain run's time is used up; the guided time includes training.  Both use
     balance-heuristic MIS with the emitter selection probability on both sides."""
     import numpy as np
 
@@ -235,10 +235,21 @@ def guided_experiment(pt, ref_spp, plain_spp, train_spp, ref_g_spp=1024):
     t0 = time.perf_counter()
     t.render(spp=train_spp, mis_mode=mis, aov=False, guide=True, guide_train=True, timing=True)
     t_train = time.perf_counter() - t0
+    # Then as many guided frames as fit in the rest of the plain render's
+    # time, in calls sized from the measured frame time (at most 64 frames, a
+    # launch of the library's default width; the plain render's calls use the
+    # same), so the guided side stops within a frame of the plain time.
     n = train_spp
-    while time.perf_counter() - t0 < t_plain:
-        t.render(spp=8, mis_mode=mis, aov=False, guide=True, timing=True)
-        n += 8
+    per_frame = None
+    while True:
+        remaining = t_plain - (time.perf_counter() - t0)
+        k = 8 if per_frame is None else min(64, int(remaining / per_frame))
+        if remaining <= 0.0 or k < 1:
+            break
+        t1 = time.perf_counter()
+        t.render(spp=k, mis_mode=mis, aov=False, guide=True, timing=True)
+        per_frame = (time.perf_counter() - t1) / k
+        n += k
     t_guided = time.perf_counter() - t0
     img = t.read(pt.BUF_ACCUM)[..., :3].astype(np.float64)
     e_guided = relmse(img, ref)
