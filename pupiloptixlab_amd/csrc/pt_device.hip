@@ -136,13 +136,16 @@ __device__ __forceinline__ f3 f3of(float4 v) { return mk3(v.x, v.y, v.z); }
 
 // --- raygen ---------------------------------------------------------------------
 // main.cu:39-78 for every (frame, pixel) of the launch.
-// Queue order of a frame's camera paths: slot j of the frame traces pixel
-// tiled_pixel(j).  Full 8x8 tiles come first (one tile per 64-lane wave, so a
-// wave's rays leave from one compact patch of the film and stay together
-// through the compaction of later bounces), then the pixels of a partial
-// right strip and bottom strip in row order.  The path id keeps the pixel
-// (frame * pixels + pixel), so films, AOVs and random streams do not depend
-// on the order.
+// Queue order of a launch's camera paths: slot i traces frame i % frames of
+// the pixel at place i / frames of the film's tile order.  The lanes of a
+// wave thus carry a few neighbouring pixels in all the launch's frames: their
+// camera rays nearly coincide, their later rays leave from nearly the same
+// points, and compaction keeps them together at later bounces (terrain
+// 1080p, against one frame after another: traversal 2.6% faster, launches
+// 1.7% faster at 20 frames and 2.1% at 64).  Tile order (tiled_pixel): full
+// 8x8 tiles first, then the pixels of a partial right strip and bottom strip
+// in row order.  The path id keeps frame and pixel (frame * pixels + pixel),
+// so films, AOVs and random streams do not depend on the order.
 __device__ __forceinline__ uint32_t tiled_pixel(uint32_t j, uint32_t w, uint32_t h) {
     const uint32_t w8 = w & ~7u, h8 = h & ~7u, tiled = w8 * h8;
     if (j < tiled) {
@@ -165,8 +168,8 @@ __global__ void k_raygen(PathQueue q, CameraRec cam, uint32_t width, uint32_t he
         for (uint32_t i = 1; i < num_counters; ++i) counters[i] = 0u;
     }
     for (uint32_t i = tid; i < total; i += gridDim.x * blockDim.x) {
-        const uint32_t f = i / npix;
-        const uint32_t pixel = tiled_pixel(i - f * npix, width, height);
+        const uint32_t f = i % frames;
+        const uint32_t pixel = tiled_pixel(i / frames, width, height);
         const uint32_t x = pixel % width, y = pixel / width;
         Rng rng = rng_init(pixel, frame_seed0 + f);
         const float jx = rng_next(rng);
@@ -659,23 +662,27 @@ __global__ void k_accumulate(const float4 *out_l, float4 *accum, float4 *frame, 
 __global__ void k_raygen_batch(PathQueue q, const LaunchDesc *ld, const BatchViewDev *views, uint32_t path_bits,
                                uint32_t *counters, uint32_t num_counters) {
     const uint32_t total = ld->total_paths;
-    const uint32_t ne = ld->n_entries;
+    const uint32_t ns = ld->n_segs;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
         counters[0] = total;
         for (uint32_t i = 1; i < num_counters; ++i) counters[i] = 0u;
     }
     for (uint32_t i = tid; i < total; i += gridDim.x * blockDim.x) {
-        uint32_t lo = 0, hi = ne - 1;
+        // The segment holding slot i; within a segment (consecutive frames of
+        // one view) frames are interleaved as in k_raygen.
+        uint32_t lo = 0, hi = ns - 1;
         while (lo < hi) {
             const uint32_t mid = (lo + hi + 1u) >> 1;
-            if (ld->e[mid].path_begin <= i) lo = mid;
+            if (ld->e[ld->g[mid].first_entry].path_begin <= i) lo = mid;
             else hi = mid - 1u;
         }
-        const LaunchEntry en = ld->e[lo];
+        const LaunchSeg &sg = ld->g[lo];
+        const uint32_t local = i - ld->e[sg.first_entry].path_begin;
+        const LaunchEntry en = ld->e[sg.first_entry + local % sg.frames];
         const uint32_t view = en.view & kEntryViewMask;
         const BatchViewDev &v = views[view];
-        const uint32_t pixel = tiled_pixel(i - en.path_begin, v.width, v.height);
+        const uint32_t pixel = tiled_pixel(local / sg.frames, v.width, v.height);
         const uint32_t x = pixel % v.width, y = pixel / v.width;
         Rng rng = rng_init(pixel, en.frame_seed);
         const float jx = rng_next(rng);
