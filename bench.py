@@ -614,7 +614,11 @@ def main():
                    + counted["shadow_tri_tests"] * TRI_BYTES),
         "shade": (ms["ms_shade"], launches["launches_shade"], rays_ext * SHADE_BYTES_PER_VERTEX),
     }
-    dominant = max(kernels, key=lambda k: kernels[k][0])
+    # Shadow rays run on the pass's side stream, overlapping the next bounce's
+    # traversal: their event time spans the overlap (their blocks wait for CUs
+    # that traversal holds), so the dominant kernel is taken among the kernels
+    # of the pass stream.
+    dominant = max(("trace", "shade"), key=lambda k: kernels[k][0])
     dms, dlaunch, dbytes = kernels[dominant]
     achieved = dbytes / (dms / 1000.0) / 1e9 if dms > 0 else 0.0
     # Frames per launch of the timed render (pt_render's default width: up to
@@ -630,6 +634,8 @@ def main():
         if k in traffic_by_kernel and kms > 0:
             e["hbm_bytes_per_launch"] = int(traffic_by_kernel[k])
             e["hbm_GBps"] = round(traffic_by_kernel[k] / (kms / max(1, kl) / 1000.0) / 1e9, 1)
+        if k == "shadow":
+            e["stream"] = "side stream, overlapping the next bounce's traversal (event time spans the overlap)"
         per_kernel[k] = e
     traffic = per_kernel[dominant].get("hbm_bytes_per_launch")
     hbm_gbps = per_kernel[dominant].get("hbm_GBps")

@@ -1112,6 +1112,10 @@ struct ShapeHost {
 struct pt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Side stream of pt_render: the shadow rays of bounce b run on it while
+    // the pass stream traces bounce b + 1 (dep_events order the two).
+    hipStream_t stream2 = nullptr;
+    std::vector<hipEvent_t> dep_events;
     std::string error;
     int num_cu = 256;
     uint32_t trace_blocks_per_cu = 8;       // resident closest-hit traversal blocks per CU, sphere variant
@@ -1430,12 +1434,29 @@ int ensure_spill(pt_ctx *ctx) {
         kTraceBlock;
     if (ctx->d_spill && ctx->spill_threads >= threads) return PT_OK;
     dfree(ctx->d_spill);
-    PT_HIP(hipMalloc((void **)&ctx->d_spill, (size_t)threads * kSpillEntries * sizeof(uint2)));
+    // Two areas: the second serves shadow rays traced on the side stream
+    // while the pass stream traverses.
+    PT_HIP(hipMalloc((void **)&ctx->d_spill, 2 * (size_t)threads * kSpillEntries * sizeof(uint2)));
     ctx->spill_threads = threads;
     return PT_OK;
 }
 
 DeviceBvh device_bvh(pt_ctx *ctx) { return DeviceBvh{ctx->d_nodes, ctx->d_leaf, ctx->d_spill}; }
+// The tree with the spill area of kernels on the side stream.
+DeviceBvh device_bvh_side(pt_ctx *ctx) {
+    return DeviceBvh{ctx->d_nodes, ctx->d_leaf, ctx->d_spill + (size_t)ctx->spill_threads * kSpillEntries};
+}
+
+// Events ordering the pass stream and the side stream within a render, two
+// per bounce (shading done, shadow rays done).
+int ensure_dep_events(pt_ctx *ctx, uint32_t n) {
+    while (ctx->dep_events.size() < n) {
+        hipEvent_t e = nullptr;
+        PT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ctx->dep_events.push_back(e);
+    }
+    return PT_OK;
+}
 
 SceneView scene_view(pt_ctx *ctx) {
     SceneView v;
@@ -1470,18 +1491,19 @@ hipEvent_t next_event(pt_ctx *ctx) {
 struct EventTimer {
     pt_ctx *ctx;
     bool on;
+    hipStream_t stream;
     hipEvent_t a = nullptr;
-    EventTimer(pt_ctx *c, bool enabled) : ctx(c), on(enabled) {
+    EventTimer(pt_ctx *c, bool enabled, hipStream_t s = nullptr) : ctx(c), on(enabled), stream(s ? s : c->stream) {
         if (on) {
             a = next_event(ctx);
-            if (a) (void)hipEventRecord(a, ctx->stream);
+            if (a) (void)hipEventRecord(a, stream);
         }
     }
     void stop(int kind) {
         if (!on || !a) return;
         hipEvent_t b = next_event(ctx);
         if (!b) return;
-        (void)hipEventRecord(b, ctx->stream);
+        (void)hipEventRecord(b, stream);
         ctx->pending.push_back(pt_ctx::Pending{kind, a, b});
     }
 };
@@ -2575,6 +2597,12 @@ int pt_create(int hip_device, pt_ctx **out) {
         delete ctx;
         return PT_ERR_HIP;
     }
+    if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "failed to create the side stream";
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return PT_ERR_HIP;
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, hip_device) == hipSuccess && prop.multiProcessorCount > 0)
         ctx->num_cu = prop.multiProcessorCount;
@@ -2625,6 +2653,7 @@ void pt_destroy(pt_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream2) (void)hipStreamSynchronize(ctx->stream2);
     ctx->guide.release();
     batch_free_all(ctx);
     free_scene(ctx);
@@ -2635,6 +2664,8 @@ void pt_destroy(pt_ctx *ctx) {
     dfree(ctx->d_refit_err);
     dfree(ctx->d_count_hist);
     for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->dep_events) (void)hipEventDestroy(e);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -3127,7 +3158,8 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
 
     const bool timing = (p->flags & PT_RENDER_TIMING) != 0;
     const SceneView sv = scene_view(ctx);
-    const DeviceBvh bvh = device_bvh(ctx);
+    const DeviceBvh bvh = device_bvh(ctx), bvh_side = device_bvh_side(ctx);
+    if ((rc = ensure_dep_events(ctx, 2 * max_depth)) != PT_OK) return rc;
     ShadeParams sp;
     sp.max_depth = max_depth;
     sp.rr_depth = p->rr_depth;
@@ -3206,6 +3238,11 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
             ga.rs.max_depth = max_depth;
             ga.rs.epoch = gs.epoch;
         }
+        // Bounce b: trace and shade on the pass stream, then the shadow rays
+        // on the side stream, where they overlap the traversal of bounce
+        // b + 1 (which reads only the queued rays and writes their hits).
+        // Bounce b + 1 waits for them before it reads its paths' radiance
+        // (the shadow rays add to it) and refills the shadow queue.
         int cur = 0;
         for (uint32_t b = 0; b < max_depth; ++b) {
             const int nxt = cur ^ 1;
@@ -3216,6 +3253,7 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
                                    (const uint32_t *)nullptr, 31u);
                 t.stop(kKindTrace);
             }
+            if (b > 0) PT_HIP(hipStreamWaitEvent(ctx->stream, ctx->dep_events[2 * (b - 1) + 1], 0));
             // The guide is sampled at vertices b < guided_depth; the vertex of the
             // last bounce never samples a direction.
             const uint32_t gdepth = ctx->guide.cfg.guided_depth ? ctx->guide.cfg.guided_depth : max_depth;
@@ -3240,16 +3278,20 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
                                        ctx->d_out_l, aov, (const BatchViewDev *)nullptr, 31u);
                 t.stop(kKindShade);
             }
+            PT_HIP(hipEventRecord(ctx->dep_events[2 * b], ctx->stream));
+            PT_HIP(hipStreamWaitEvent(ctx->stream2, ctx->dep_events[2 * b], 0));
             {
-                EventTimer t(ctx, timing);
+                EventTimer t(ctx, timing, ctx->stream2);
                 hipLaunchKernelGGL(shadow_kernel(ctx->counting, ctx->num_spheres > 0),
                                    dim3(shadow_grid(ctx, ctx->num_spheres > 0)), dim3(kTraceBlock), 0,
-                                   ctx->stream, ctx->sq, bvh, shd + b, swork + b, ctx->q[nxt], ctx->d_out_l,
+                                   ctx->stream2, ctx->sq, bvh_side, shd + b, swork + b, ctx->q[nxt], ctx->d_out_l,
                                    ctx->d_stat + 2, (const uint32_t *)nullptr, 31u);
                 t.stop(kKindShadow);
             }
+            PT_HIP(hipEventRecord(ctx->dep_events[2 * b + 1], ctx->stream2));
             cur = nxt;
         }
+        PT_HIP(hipStreamWaitEvent(ctx->stream, ctx->dep_events[2 * (max_depth - 1) + 1], 0));
         {
             EventTimer t(ctx, timing);
             const uint32_t grid = std::min<uint32_t>((npix + 255) / 256, (uint32_t)ctx->num_cu * 16u);
