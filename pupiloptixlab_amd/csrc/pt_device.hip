@@ -2149,7 +2149,11 @@ struct BatchGuide {
     uint32_t step = 0;    // optimizer steps taken on the ring
 };
 
-// The kernels of one launch (the sequence a hipGraph captures).
+// The kernels of one launch (the sequence a hipGraph captures).  As in
+// pt_render, the shadow rays of bounce b run on the side stream and overlap
+// the traversal of bounce b + 1; under capture the side stream joins the graph
+// through the events and rejoins the pass stream before the accumulation.
+// Needs ensure_dep_events(ctx, 2 * max_depth).
 void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth, const ShadeParams &sp,
                           uint32_t path_bits, bool fp64, const BatchGuide &bg, bool timing = false) {
     BatchState &b = ctx->batch;
@@ -2168,6 +2172,7 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
     sv.tex = TexArrays{b.d_bitmaps, reinterpret_cast<const f4 *>(b.d_texels)};
     std::memset(&sv.env, 0, sizeof(sv.env));
     const DeviceBvh bvh{b.d_nodes, b.d_leaf, ctx->d_spill};
+    const DeviceBvh bvh_side{b.d_nodes, b.d_leaf, ctx->d_spill + (size_t)ctx->spill_threads * kSpillEntries};
     AovBuffers aov{nullptr, nullptr, nullptr, nullptr, 0u, 1u};
     uint32_t *ext = ctx->d_counters;
     uint32_t *shd = ctx->d_counters + (max_depth + 2);
@@ -2187,6 +2192,7 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
                                path_bits);
             t.stop(kKindTrace);
         }
+        if (bb > 0) (void)hipStreamWaitEvent(ctx->stream, ctx->dep_events[2 * (bb - 1) + 1], 0);
         if (bg.infer || bg.train) {
             GuideShadeArgs ga = bg.ga;
             const bool infer_here = bg.infer && bb < bg.gdepth && bb + 1 < max_depth;
@@ -2212,16 +2218,20 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
                                (const BatchViewDev *)b.d_views, path_bits);
             t.stop(kKindShade);
         }
+        (void)hipEventRecord(ctx->dep_events[2 * bb], ctx->stream);
+        (void)hipStreamWaitEvent(ctx->stream2, ctx->dep_events[2 * bb], 0);
         {
-            EventTimer t(ctx, timing);
+            EventTimer t(ctx, timing, ctx->stream2);
             hipLaunchKernelGGL(shadow_kernel(false, !b.spheres.empty()), dim3(shadow_grid(ctx, !b.spheres.empty())),
-                               dim3(kTraceBlock), 0, ctx->stream,
-                               ctx->sq, bvh, shd + bb, swork + bb, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2,
+                               dim3(kTraceBlock), 0, ctx->stream2,
+                               ctx->sq, bvh_side, shd + bb, swork + bb, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2,
                                (const uint32_t *)b.d_roots, path_bits);
             t.stop(kKindShadow);
         }
+        (void)hipEventRecord(ctx->dep_events[2 * bb + 1], ctx->stream2);
         cur = nxt;
     }
+    (void)hipStreamWaitEvent(ctx->stream, ctx->dep_events[2 * (max_depth - 1) + 1], 0);
     hipLaunchKernelGGL(k_accumulate_batch, dim3((uint32_t)ctx->num_cu * 16u), dim3(256), 0, ctx->stream,
                        ctx->d_out_l, b.d_launch, b.d_views, fp64 ? 1u : 0u);
     hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters, max_depth, b.d_rays);
@@ -2375,6 +2385,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     spar.aov = 0;
     const bool fp64 = (p.render_flags & PT_RENDER_FP64_ACCUM) != 0;
     if ((rc = batch_guide_setup(ctx, p, sh, max_depth, plan, bg)) != PT_OK) return rc;
+    if ((rc = ensure_dep_events(ctx, 2 * max_depth)) != PT_OK) return rc;
     // Per-kernel event timing (direct launches only; around the training
     // steps in both modes).
     const bool timing = (p.render_flags & PT_RENDER_TIMING) != 0;
