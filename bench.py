@@ -443,11 +443,13 @@ def batched_guided_experiment(pt, n_door, n_terrain, width, height, door_ref_spp
     return out
 
 
-def summarize_pmc(fetch_dir, write_dir, frames_per_launch, out_path, kernels):
+def summarize_pmc(fetch_dir, write_dir, frames_per_launch, out_path, kernels, chains=1):
     """Per-dispatch HBM bytes of each kernel from two rocprofv3 --pmc runs
     (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by the factor that
     maps the counter onto the known bytes of the copy-probe kernel.  The runs
-    render frames_per_launch frames per launch (the bench's launch shape)."""
+    render frames_per_launch frames per launch (the bench's launch shape) in
+    `chains` concurrent chains, so one bounce (a logical launch, the unit of
+    the bench's per-kernel figures) is `chains` dispatches of each kernel."""
     import csv
     import glob
 
@@ -461,7 +463,7 @@ def summarize_pmc(fetch_dir, write_dir, frames_per_launch, out_path, kernels):
                     vals.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
         return vals
 
-    out = {"frames_per_launch": int(frames_per_launch), "kernels": {}}
+    out = {"frames_per_launch": int(frames_per_launch), "chains": int(chains), "kernels": {}}
     for counter, d, key in (("FETCH_SIZE", fetch_dir, "fetch"), ("WRITE_SIZE", write_dir, "write")):
         vals = per_kernel(d, counter)
         copy = [v for k, vs in vals.items() if "k_copy" in k for v in vs]
@@ -479,6 +481,7 @@ def summarize_pmc(fetch_dir, write_dir, frames_per_launch, out_path, kernels):
             e[f"{key}_dispatches"] = len(kern)
     for e in out["kernels"].values():
         e["hbm_bytes_per_dispatch"] = int(e["fetch_bytes_per_dispatch"] + e["write_bytes_per_dispatch"])
+        e["hbm_bytes_per_launch"] = e["hbm_bytes_per_dispatch"] * int(chains)
     with open(out_path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))
@@ -489,16 +492,18 @@ TRAFFIC_KERNELS = {"trace": "k_trace<false, false>", "shade": "k_shade<false, ",
 
 
 def measured_traffic(frames_per_launch):
-    """HBM bytes per dispatch of each kernel from the committed counter
-    summary, for launches of frames_per_launch frames: measured at that shape,
-    or scaled linearly in frames from the measured shape (marked so)."""
+    """HBM bytes per logical launch (one bounce, all chains) of each kernel
+    from the committed counter summary, for launches of frames_per_launch
+    frames: measured at that shape, or scaled linearly in frames from the
+    measured shape (marked so)."""
     if not os.path.exists(TRAFFIC_FILE):
         return {}, None
     with open(TRAFFIC_FILE) as f:
         t = json.load(f)
     scale = frames_per_launch / t["frames_per_launch"]
     note = None if scale == 1 else f"scaled from {t['frames_per_launch']}-frame launches x {scale:.3f}"
-    return {k: v["hbm_bytes_per_dispatch"] * scale for k, v in t["kernels"].items()}, note
+    return {k: v.get("hbm_bytes_per_launch", v["hbm_bytes_per_dispatch"]) * scale
+            for k, v in t["kernels"].items()}, note
 
 
 def run_guided_batch(pt, full, train_spp=32):
@@ -509,8 +514,9 @@ def run_guided_batch(pt, full, train_spp=32):
 
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize-pmc":
-        # bench.py --summarize-pmc FETCH_DIR WRITE_DIR FRAMES_PER_LAUNCH OUT_JSON
-        summarize_pmc(sys.argv[2], sys.argv[3], int(sys.argv[4]), sys.argv[5], TRAFFIC_KERNELS)
+        # bench.py --summarize-pmc FETCH_DIR WRITE_DIR FRAMES_PER_LAUNCH OUT_JSON [CHAINS]
+        summarize_pmc(sys.argv[2], sys.argv[3], int(sys.argv[4]), sys.argv[5], TRAFFIC_KERNELS,
+                      int(sys.argv[6]) if len(sys.argv) > 6 else 1)
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -519,6 +525,8 @@ def main():
     ap.add_argument("--spp-per-launch", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-single-chain", action="store_true",
+                    help="skip the one-chain pass that times the dominant kernel alone (counter runs use this)")
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--spheres", type=int, default=256)
     ap.add_argument("--no-guided", action="store_true", help="skip the guided-vs-plain relMSE experiment")
@@ -603,6 +611,19 @@ def main():
     rays_shadow = s["rays_shadow"] - base["rays_shadow"]
     assert rays_ext == counted["rays_extension"], "timed and counting passes diverged"
 
+    # The same frames once more (untimed for `value`) with each launch as one
+    # chain of kernels: the dominant kernel's event time when it runs alone.
+    alone, elapsed1 = None, None
+    if not args.no_single_chain:
+        tracer.reset()
+        base1 = tracer.stats()
+        t1 = time.perf_counter()
+        tracer.render(spp=args.steps, aov=True, timing=True, spp_per_launch=args.spp_per_launch, one_chain=True)
+        elapsed1 = time.perf_counter() - t1
+        s1 = tracer.stats()
+        alone = {k: (s1["ms_" + k] - base1["ms_" + k], s1["launches_" + k] - base1["launches_" + k])
+                 for k in ("trace", "shade", "shadow")}
+
     samples = args.steps * npix
     value = samples / elapsed / 1e6
     kernels = {
@@ -617,10 +638,21 @@ def main():
     # Shadow rays run on the pass's side stream, overlapping the next bounce's
     # traversal: their event time spans the overlap (their blocks wait for CUs
     # that traversal holds), so the dominant kernel is taken among the kernels
-    # of the pass stream.
+    # of the pass stream.  A launch here is one bounce of the pass: with two
+    # concurrent chains, both chains' dispatches of the kernel, timed from the
+    # first start to the last end (they share the GPU with the other chain's
+    # shading meanwhile); the one-chain pass gives the kernel's time alone.
     dominant = max(("trace", "shade"), key=lambda k: kernels[k][0])
     dms, dlaunch, dbytes = kernels[dominant]
     achieved = dbytes / (dms / 1000.0) / 1e9 if dms > 0 else 0.0
+    single_chain = None
+    if alone:
+        ams, alaunch = alone[dominant]
+        achieved_alone = dbytes / (ams / 1000.0) / 1e9 if ams > 0 else 0.0
+        single_chain = {"avg_launch_ms": round(ams / max(1, alaunch), 4), "achieved": round(achieved_alone, 1),
+                        "frac": round(achieved_alone / HBM_PEAK_GBS, 4),
+                        "msamples_per_s": round(samples / elapsed1 / 1e6, 3),
+                        "pass": "the same frames again with every launch as one chain (not part of value)"}
     # Frames per launch of the timed render (pt_render's default width: up to
     # 64 frames and 128M paths per launch).
     frames_per_launch = args.spp_per_launch or max(1, min(64, (128 << 20) // npix))
@@ -705,8 +737,11 @@ def main():
             "traffic_shape": traffic_note or f"measured at this launch shape ({frames_per_launch} frames per launch)",
             "traffic_GBps": hbm_gbps,
             "limiter": limiter,
+            "single_chain": single_chain,
             "algorithmic_bytes_per_launch": int(dbytes / max(1, dlaunch)),
             "avg_launch_ms": round(dms / max(1, dlaunch), 4),
+            "launch": ("one bounce of the pass: the dominant kernel's dispatches of all concurrent chains, timed "
+                       "from the first start to the last end"),
             "frames_per_launch": frames_per_launch,
             "measured_copy_GBps": round(copy_gbps, 1),
         },

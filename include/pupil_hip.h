@@ -182,7 +182,11 @@ enum pt_render_flags {
     PT_RENDER_TIMING = 4,     /* record per-kernel HIP event times */
     PT_RENDER_GUIDE = 8,      /* sample with the learned guiding proposal (needs
                                  pt_guide_enable and PT_MIS_CONSISTENT) */
-    PT_RENDER_GUIDE_TRAIN = 16 /* train the guide online on this render's paths */
+    PT_RENDER_GUIDE_TRAIN = 16, /* train the guide online on this render's paths */
+    PT_RENDER_ONE_CHAIN = 32    /* run every launch as one chain of kernels (by default a launch of
+                                   at least two frames and 2^22 paths runs its two halves as
+                                   concurrent chains; the image is the same either way, and with
+                                   one chain each kernel's event time is that of the kernel alone) */
 };
 
 /* Launch parameters (example/path_tracer/type.h:9-33 config + pass knobs). */

@@ -40,6 +40,7 @@ FILTER_POINT, FILTER_LINEAR = 0, 1
 WRAP_REPEAT, WRAP_CLAMP, WRAP_MIRROR, WRAP_BORDER = 0, 1, 2, 3
 MIS_REFERENCE, MIS_CONSISTENT = 0, 1
 RENDER_AOV, RENDER_FP64_ACCUM, RENDER_TIMING, RENDER_GUIDE, RENDER_GUIDE_TRAIN = 1, 2, 4, 8, 16
+RENDER_ONE_CHAIN = 32
 DENOISE_ALBEDO, DENOISE_NORMAL = 1, 2
 GUIDE_OUTPUTS, GUIDE_LEVELS = 41, 16
 GUIDE_NO_RADIANCE_L2, GUIDE_NO_GUIDE_CE, GUIDE_NO_MIXTURE_CE = 1, 2, 4
@@ -472,12 +473,15 @@ class PathTracer:
 
     def render(self, spp: int = 1, max_depth: int = 0, rr_depth: int = 2, mis_mode: int = MIS_REFERENCE,
                accumulate: bool = True, aov: bool = True, fp64: bool = False, timing: bool = False,
-               spp_per_launch: int = 0, guide: bool = False, guide_train: bool = False) -> None:
+               spp_per_launch: int = 0, guide: bool = False, guide_train: bool = False,
+               one_chain: bool = False) -> None:
         """Renders `spp` frames.  guide: sample with the learned proposal;
         guide_train: train it on this render's paths (both need
-        guide_enable() and mis_mode=MIS_CONSISTENT)."""
+        guide_enable() and mis_mode=MIS_CONSISTENT); one_chain: run each
+        launch as one chain of kernels (same image)."""
         flags = (RENDER_AOV if aov else 0) | (RENDER_FP64_ACCUM if fp64 else 0) | (RENDER_TIMING if timing else 0)
         flags |= (RENDER_GUIDE if guide else 0) | (RENDER_GUIDE_TRAIN if guide_train else 0)
+        flags |= RENDER_ONE_CHAIN if one_chain else 0
         p = RenderParams(spp, max_depth, rr_depth, mis_mode, int(accumulate), flags, spp_per_launch, 0)
         _check(lib().pt_render(self._ctx, C.byref(p)), "pt_render", self._ctx)
 

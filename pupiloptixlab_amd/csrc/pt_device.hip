@@ -1201,7 +1201,9 @@ struct pt_ctx {
     struct Pending {
         int kind;
         hipEvent_t a, b;
+        uint64_t group;  // timers of one group time one logical launch
     };
+    uint64_t timer_groups = 0;  // source of single-timer groups
     std::vector<hipEvent_t> event_pool;
     size_t event_next = 0;
     std::vector<Pending> pending;
@@ -1499,12 +1501,18 @@ hipEvent_t next_event(pt_ctx *ctx) {
 // Brackets one kernel launch with events on the pass stream; durations are
 // read after the render's final synchronization, so timing adds no host
 // synchronization inside the timed region.
+// Timers sharing a nonzero `group` (the same bounce of the concurrent chains
+// of one pt_render launch) time one logical launch: from the earliest start
+// to the latest end among them.
+constexpr uint64_t kGroupSingle = 1ull << 63;  // groups of single timers
 struct EventTimer {
     pt_ctx *ctx;
     bool on;
     hipStream_t stream;
+    uint64_t group;
     hipEvent_t a = nullptr;
-    EventTimer(pt_ctx *c, bool enabled, hipStream_t s = nullptr) : ctx(c), on(enabled), stream(s ? s : c->stream) {
+    EventTimer(pt_ctx *c, bool enabled, hipStream_t s = nullptr, uint64_t g = 0)
+        : ctx(c), on(enabled), stream(s ? s : c->stream), group(g ? g : (kGroupSingle | ++c->timer_groups)) {
         if (on) {
             a = next_event(ctx);
             if (a) (void)hipEventRecord(a, stream);
@@ -1515,15 +1523,26 @@ struct EventTimer {
         hipEvent_t b = next_event(ctx);
         if (!b) return;
         (void)hipEventRecord(b, stream);
-        ctx->pending.push_back(pt_ctx::Pending{kind, a, b});
+        ctx->pending.push_back(pt_ctx::Pending{kind, a, b, group});
     }
 };
 
 void collect_event_times(pt_ctx *ctx) {
+    // Span of every (kind, group): times relative to the first timer's start.
+    std::map<std::pair<int, uint64_t>, std::pair<float, float>> span;
+    const hipEvent_t ref = ctx->pending.empty() ? nullptr : ctx->pending.front().a;
     for (const auto &p : ctx->pending) {
-        float ms = 0.f;
-        if (hipEventElapsedTime(&ms, p.a, p.b) != hipSuccess) continue;
-        switch (p.kind) {
+        float t0 = 0.f, t1 = 0.f;
+        if (hipEventElapsedTime(&t0, ref, p.a) != hipSuccess || hipEventElapsedTime(&t1, ref, p.b) != hipSuccess)
+            continue;
+        const auto key = std::make_pair(p.kind, p.group);
+        auto it = span.find(key);
+        if (it == span.end()) span.emplace(key, std::make_pair(t0, t1));
+        else it->second = std::make_pair(std::min(it->second.first, t0), std::max(it->second.second, t1));
+    }
+    for (const auto &e : span) {
+        const float ms = e.second.second - e.second.first;
+        switch (e.first.first) {
             case kKindRaygen: ctx->stats.ms_raygen += ms; break;
             case kKindTrace: ctx->stats.ms_trace += ms; ctx->stats.launches_trace++; break;
             case kKindShade: ctx->stats.ms_shade += ms; ctx->stats.launches_shade++; break;
@@ -3230,15 +3249,16 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
             return PT_ERR_HIP;
     }
 
-    // A launch of at least kTwoChainPaths paths (not training the guide) runs
-    // as two chains: the first half of its frames on the pass stream and its
-    // side stream, the second half on two more streams, each chain with its
-    // own part of the queues, counters, spill areas and events.  The chains
-    // fill each other's idle CUs while one drains a bounce (terrain 1080p,
-    // 64 frames per launch: 856 -> 940 Msamples/s for two concurrent halves;
-    // 20 frames: +2%, so smaller launches keep one chain).  Every path's work
-    // is unchanged, so is the image.
-    constexpr uint64_t kTwoChainPaths = 1ull << 26;
+    // A launch of at least two frames and kTwoChainPaths paths (not training
+    // the guide) runs as two chains: the first half of its frames on the pass
+    // stream and its side stream, the second half on two more streams, each
+    // chain with its own part of the queues, counters, spill areas and
+    // events.  A chain's bounces serialize and every kernel drains before the
+    // next starts; the other chain fills the idle CUs meanwhile (terrain
+    // 1080p: +8-9% at 4, 8 and 20 frames per launch, +12% at 64).  Training
+    // launches keep one chain (no gain measured).  Every path's work is
+    // unchanged, so is the image.
+    constexpr uint64_t kTwoChainPaths = 1ull << 22;
     uint32_t done = 0, batch = 0;
     uint32_t sample_cnt = ctx->sample_cnt, seed = ctx->random_seed;
     std::vector<uint32_t> chains_per_launch;
@@ -3246,7 +3266,10 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
     while (done < p->spp) {
         const uint32_t frames = std::min(per_launch, p->spp - done);
         const uint32_t total = frames * npix;
-        const uint32_t nchain = (!guide_train && frames >= 2 && (uint64_t)total >= kTwoChainPaths) ? 2u : 1u;
+        const uint32_t nchain =
+            (!guide_train && !(p->flags & PT_RENDER_ONE_CHAIN) && frames >= 2 && (uint64_t)total >= kTwoChainPaths)
+                ? 2u
+                : 1u;
         AovBuffers aov{nullptr, nullptr, nullptr, nullptr, frames - 1, npix};
         if (sp.aov) {
             aov.albedo = ctx->d_albedo;
@@ -3296,6 +3319,8 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
             k.gout = ctx->d_gout ? ctx->d_gout + 6 * k.off : nullptr;
         }
         hipEvent_t fork = ctx->dep_events[4 * max_depth], join = ctx->dep_events[4 * max_depth + 1];
+        // Timer groups: a bounce's kernels of both chains time as one launch.
+        const uint64_t launch_group = (uint64_t)(batch + 1) << 16;
         if (nchain == 2) {
             // The second chain starts after everything before on the pass stream.
             PT_HIP(hipEventRecord(fork, ctx->stream));
@@ -3303,7 +3328,7 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
         }
         for (uint32_t c = 0; c < nchain; ++c) {
             const Chain &k = ch[c];
-            EventTimer t(ctx, timing, k.main);
+            EventTimer t(ctx, timing, k.main, launch_group | 0xFFFFu);
             const uint32_t paths = k.frames * npix;
             const uint32_t grid = std::min<uint32_t>((paths + 255) / 256, (uint32_t)ctx->num_cu * 32u);
             hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, k.main, k.q[0], ctx->camera, ctx->width,
@@ -3322,7 +3347,7 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
             for (uint32_t c = 0; c < nchain; ++c) {
                 const Chain &k = ch[c];
                 {
-                    EventTimer t(ctx, timing, k.main);
+                    EventTimer t(ctx, timing, k.main, launch_group | (b + 1));
                     hipLaunchKernelGGL(trace_kernel(ctx->counting, ctx->num_spheres > 0), dim3(tg), dim3(kTraceBlock),
                                        0, k.main, k.q[cur], k.bvh, k.ext + b, k.twork + b, ctx->d_stat,
                                        (const uint32_t *)nullptr, 31u);
@@ -3334,14 +3359,14 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
                 GuideShadeArgs gac = ga;
                 gac.gout = infer_here ? k.gout : nullptr;
                 if (infer_here) {
-                    EventTimer t(ctx, timing, k.main);
+                    EventTimer t(ctx, timing, k.main, launch_group | (b + 1));
                     hipLaunchKernelGGL(k_guide_infer, dim3(ctx->num_cu * ctx->infer_blocks_per_cu), dim3(kInferBlock),
                                        0, k.main, gview, k.q[cur], k.ext + b, reinterpret_cast<uint2 *>(k.gout),
                                        ctx->d_stat + 4, 31u, InferScene{sv.tri, sv.shapes, sv.spheres, sv.materials});
                     t.stop(kKindGuideInfer);
                 }
                 {
-                    EventTimer t(ctx, timing, k.main);
+                    EventTimer t(ctx, timing, k.main, launch_group | (b + 1));
                     if (guided)
                         hipLaunchKernelGGL(k_shade_guided<false>, dim3(sg), dim3(256), 0, k.main, sv, sp, k.q[cur],
                                            k.ext + b, k.q[nxt], k.ext + b + 1, k.sq, k.shd + b, ctx->d_out_l, aov, gac,
@@ -3355,7 +3380,7 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
                 PT_HIP(hipEventRecord(k.ev[2 * b], k.main));
                 PT_HIP(hipStreamWaitEvent(k.side, k.ev[2 * b], 0));
                 {
-                    EventTimer t(ctx, timing, k.side);
+                    EventTimer t(ctx, timing, k.side, launch_group | (b + 1));
                     hipLaunchKernelGGL(shadow_kernel(ctx->counting, ctx->num_spheres > 0),
                                        dim3(shadow_grid(ctx, ctx->num_spheres > 0)), dim3(kTraceBlock), 0, k.side,
                                        k.sq, k.bvh_side, k.shd + b, k.swork + b, k.q[nxt], ctx->d_out_l,
