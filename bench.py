@@ -635,14 +635,12 @@ def main():
                    + counted["shadow_tri_tests"] * TRI_BYTES),
         "shade": (ms["ms_shade"], launches["launches_shade"], rays_ext * SHADE_BYTES_PER_VERTEX),
     }
-    # Shadow rays run on the pass's side stream, overlapping the next bounce's
-    # traversal: their event time spans the overlap (their blocks wait for CUs
-    # that traversal holds), so the dominant kernel is taken among the kernels
-    # of the pass stream.  A launch here is one bounce of the pass: with two
-    # concurrent chains, both chains' dispatches of the kernel, timed from the
-    # first start to the last end (they share the GPU with the other chain's
-    # shading meanwhile); the one-chain pass gives the kernel's time alone.
-    dominant = max(("trace", "shade"), key=lambda k: kernels[k][0])
+    # A launch here is one bounce of the pass (all concurrent chains), and a
+    # kernel's time is its active time: the union of its dispatches' intervals
+    # (they share the GPU with the other chains' kernels meanwhile).  The
+    # one-chain pass gives each kernel's time alone; the dominant kernel is
+    # the one that takes longest alone.
+    dominant = max(kernels, key=lambda k: (alone or {k: kernels[k]})[k][0])
     dms, dlaunch, dbytes = kernels[dominant]
     achieved = dbytes / (dms / 1000.0) / 1e9 if dms > 0 else 0.0
     single_chain = None
@@ -666,8 +664,6 @@ def main():
         if k in traffic_by_kernel and kms > 0:
             e["hbm_bytes_per_launch"] = int(traffic_by_kernel[k])
             e["hbm_GBps"] = round(traffic_by_kernel[k] / (kms / max(1, kl) / 1000.0) / 1e9, 1)
-        if k == "shadow":
-            e["stream"] = "side stream, overlapping the next bounce's traversal (event time spans the overlap)"
         per_kernel[k] = e
     traffic = per_kernel[dominant].get("hbm_bytes_per_launch")
     hbm_gbps = per_kernel[dominant].get("hbm_GBps")
@@ -740,8 +736,8 @@ def main():
             "single_chain": single_chain,
             "algorithmic_bytes_per_launch": int(dbytes / max(1, dlaunch)),
             "avg_launch_ms": round(dms / max(1, dlaunch), 4),
-            "launch": ("one bounce of the pass: the dominant kernel's dispatches of all concurrent chains, timed "
-                       "from the first start to the last end"),
+            "launch": ("one bounce of the pass (all concurrent chains); avg_launch_ms is the kernel's active time "
+                       "(union of its dispatch intervals) per bounce"),
             "frames_per_launch": frames_per_launch,
             "measured_copy_GBps": round(copy_gbps, 1),
         },

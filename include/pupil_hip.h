@@ -183,10 +183,11 @@ enum pt_render_flags {
     PT_RENDER_GUIDE = 8,      /* sample with the learned guiding proposal (needs
                                  pt_guide_enable and PT_MIS_CONSISTENT) */
     PT_RENDER_GUIDE_TRAIN = 16, /* train the guide online on this render's paths */
-    PT_RENDER_ONE_CHAIN = 32    /* run every launch as one chain of kernels (by default a launch of
-                                   at least two frames and 2^22 paths runs its two halves as
-                                   concurrent chains; the image is the same either way, and with
-                                   one chain each kernel's event time is that of the kernel alone) */
+    PT_RENDER_ONE_CHAIN = 32    /* run every launch as one chain of kernels (by default a launch's
+                                   paths are split into up to four chains of at least 10^6 paths,
+                                   run concurrently on streams of their own; the image is the same
+                                   either way, and with one chain each kernel's event time is that
+                                   of the kernel alone) */
 };
 
 /* Launch parameters (example/path_tracer/type.h:9-33 config + pass knobs). */
@@ -220,8 +221,10 @@ typedef struct pt_stats {
     uint64_t shadow_node_visits; /* any-hit traversal; only with counting enabled */
     uint64_t shadow_tri_tests;
     double ms_total;         /* wall time of pt_render calls */
-    /* Sums of per-launch HIP event times and the number of launches timed,
-     * for launches made with PT_RENDER_TIMING. */
+    /* Per kernel kind, for launches made with PT_RENDER_TIMING: the time during
+     * which at least one of its dispatches ran (HIP events; the kernels of a
+     * launch's concurrent chains overlap) and the number of launches timed (a
+     * bounce of all chains counts once). */
     double ms_raygen, ms_trace, ms_shade, ms_shadow, ms_accumulate;
     uint64_t launches_trace, launches_shade, launches_shadow;
     uint32_t bvh_nodes, bvh_leaf_tris, bvh_depth;

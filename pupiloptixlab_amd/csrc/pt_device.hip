@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <map>
+#include <set>
 #include <mutex>
 #include <cmath>
 #include <cstdio>
@@ -158,27 +159,29 @@ __device__ __forceinline__ uint32_t tiled_pixel(uint32_t j, uint32_t w, uint32_t
     return h8 * w + (j - rw * h8);
 }
 
-// frame0: launch frame of the first frame here (path ids count frames from
-// the launch's first); frame_seed0: random-stream key of the first frame.
+// One chain of a launch: frames [frame0, frame0 + frames) of the pixels at
+// places [pix0, pix0 + pixels) of the tile order; frame f of the launch
+// draws from random-stream key frame_seed0 + f.
 __global__ void k_raygen(PathQueue q, CameraRec cam, uint32_t width, uint32_t height, uint32_t frames,
-                         uint32_t frame_seed0, uint32_t *counters, uint32_t num_counters, uint32_t frame0) {
+                         uint32_t frame_seed0, uint32_t *counters, uint32_t num_counters, uint32_t frame0,
+                         uint32_t pix0, uint32_t pixels) {
     const uint32_t npix = width * height;
-    const uint32_t total = npix * frames;
+    const uint32_t total = pixels * frames;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
         counters[0] = total;
         for (uint32_t i = 1; i < num_counters; ++i) counters[i] = 0u;
     }
     for (uint32_t i = tid; i < total; i += gridDim.x * blockDim.x) {
-        const uint32_t f = i % frames;
-        const uint32_t pixel = tiled_pixel(i / frames, width, height);
+        const uint32_t f = frame0 + i % frames;
+        const uint32_t pixel = tiled_pixel(pix0 + i / frames, width, height);
         const uint32_t x = pixel % width, y = pixel / width;
         Rng rng = rng_init(pixel, frame_seed0 + f);
         const float jx = rng_next(rng);
         const float jy = rng_next(rng);
         f3 o, d;
         camera_ray(cam, x, y, width, height, jx, jy, o, d);
-        q.ray0[i] = make_float4(o.x, o.y, o.z, u2f((frame0 + f) * npix + pixel));
+        q.ray0[i] = make_float4(o.x, o.y, o.z, u2f(f * npix + pixel));
         q.ray1[i] = make_float4(d.x, d.y, d.z, 0.f);
         q.tl[i] = make_float4(1.f, 1.f, 1.f, 0.f);
         q.lr[i] = make_float4(0.f, 0.f, u2f(rng.seed), u2f(kPathPrimary));
@@ -1114,12 +1117,10 @@ struct ShapeHost {
 struct pt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // Side stream of pt_render: the shadow rays of bounce b run on it while
-    // the pass stream traces bounce b + 1 (dep_events order the two).
-    hipStream_t stream2 = nullptr;
-    // Pass and side streams of a second chain of frames (large launches of
-    // pt_render run as two chains, below).
-    hipStream_t stream3 = nullptr, stream4 = nullptr;
+    // Streams of the second to fourth chains of a pt_render launch; stream2
+    // is also the side stream of batched launches, where the shadow rays of
+    // bounce b overlap the traversal of bounce b + 1.  dep_events order them.
+    hipStream_t stream2 = nullptr, stream3 = nullptr, stream4 = nullptr;
     std::vector<hipEvent_t> dep_events;
     std::string error;
     int num_cu = 256;
@@ -1408,8 +1409,8 @@ int ensure_queues(pt_ctx *ctx, uint32_t paths, uint32_t max_depth) {
     PT_HIP(hipMalloc((void **)&ctx->sq.s1, n * sizeof(float4)));
     PT_HIP(hipMalloc((void **)&ctx->sq.s2, n * sizeof(float4)));
     PT_HIP(hipMalloc((void **)&ctx->d_out_l, n * sizeof(float4)));
-    // One set of counters per chain of a pt_render launch.
-    PT_HIP(hipMalloc((void **)&ctx->d_counters, 2 * need_counters * sizeof(uint32_t)));
+    // One set of counters per chain of a pt_render launch (four at most).
+    PT_HIP(hipMalloc((void **)&ctx->d_counters, 4 * need_counters * sizeof(uint32_t)));
     ctx->capacity = paths;
     ctx->num_counters = need_counters;
     return PT_OK;
@@ -1442,16 +1443,16 @@ int ensure_spill(pt_ctx *ctx) {
         kTraceBlock;
     if (ctx->d_spill && ctx->spill_threads >= threads) return PT_OK;
     dfree(ctx->d_spill);
-    // Four areas: traversal and shadow rays (side stream) of each of the two
-    // chains a pt_render launch may run concurrently.
+    // Four areas: one per chain of a pt_render launch (batched launches use
+    // the first for the pass stream and the second for their side stream).
     PT_HIP(hipMalloc((void **)&ctx->d_spill, 4 * (size_t)threads * kSpillEntries * sizeof(uint2)));
     ctx->spill_threads = threads;
     return PT_OK;
 }
 
 DeviceBvh device_bvh(pt_ctx *ctx) { return DeviceBvh{ctx->d_nodes, ctx->d_leaf, ctx->d_spill}; }
-// The tree with spill area `area` (0: pass stream, 1: side stream of the
-// first chain; 2, 3: those of the second).
+// The tree with spill area `area` (chain `area` of a pt_render launch; in
+// batched launches 0 is the pass stream's, 1 the side stream's).
 DeviceBvh device_bvh_area(pt_ctx *ctx, uint32_t area) {
     return DeviceBvh{ctx->d_nodes, ctx->d_leaf, ctx->d_spill + (size_t)area * ctx->spill_threads * kSpillEntries};
 }
@@ -1460,8 +1461,9 @@ PathQueue queue_at(const PathQueue &q, size_t off) {
 }
 ShadowQueue shadow_queue_at(const ShadowQueue &q, size_t off) { return ShadowQueue{q.s0 + off, q.s1 + off, q.s2 + off}; }
 
-// Events ordering the pass stream and the side stream within a render, two
-// per bounce (shading done, shadow rays done).
+// Events ordering the streams of a render: the fork and joins of a pt_render
+// launch's chains; in batched launches two per bounce (shading done, shadow
+// rays done) between the pass stream and the side stream.
 int ensure_dep_events(pt_ctx *ctx, uint32_t n) {
     while (ctx->dep_events.size() < n) {
         hipEvent_t e = nullptr;
@@ -1501,9 +1503,11 @@ hipEvent_t next_event(pt_ctx *ctx) {
 // Brackets one kernel launch with events on the pass stream; durations are
 // read after the render's final synchronization, so timing adds no host
 // synchronization inside the timed region.
-// Timers sharing a nonzero `group` (the same bounce of the concurrent chains
-// of one pt_render launch) time one logical launch: from the earliest start
-// to the latest end among them.
+// A kernel kind's time is the union of its timers' intervals (the time
+// during which at least one of its dispatches ran; kernels of concurrent
+// chains overlap), and its launch count the number of distinct groups: timers
+// sharing a nonzero `group` (the same bounce of the chains of one pt_render
+// launch) count as one launch.
 constexpr uint64_t kGroupSingle = 1ull << 63;  // groups of single timers
 struct EventTimer {
     pt_ctx *ctx;
@@ -1528,25 +1532,38 @@ struct EventTimer {
 };
 
 void collect_event_times(pt_ctx *ctx) {
-    // Span of every (kind, group): times relative to the first timer's start.
-    std::map<std::pair<int, uint64_t>, std::pair<float, float>> span;
+    // Intervals relative to the first timer's start, per kind.
+    std::map<int, std::vector<std::pair<float, float>>> spans;
+    std::map<int, std::set<uint64_t>> groups;
     const hipEvent_t ref = ctx->pending.empty() ? nullptr : ctx->pending.front().a;
     for (const auto &p : ctx->pending) {
         float t0 = 0.f, t1 = 0.f;
         if (hipEventElapsedTime(&t0, ref, p.a) != hipSuccess || hipEventElapsedTime(&t1, ref, p.b) != hipSuccess)
             continue;
-        const auto key = std::make_pair(p.kind, p.group);
-        auto it = span.find(key);
-        if (it == span.end()) span.emplace(key, std::make_pair(t0, t1));
-        else it->second = std::make_pair(std::min(it->second.first, t0), std::max(it->second.second, t1));
+        spans[p.kind].emplace_back(t0, t1);
+        groups[p.kind].insert(p.group);
     }
-    for (const auto &e : span) {
-        const float ms = e.second.second - e.second.first;
-        switch (e.first.first) {
+    for (auto &e : spans) {
+        auto &iv = e.second;
+        std::sort(iv.begin(), iv.end());
+        double ms = 0.0;
+        float lo = iv.front().first, hi = iv.front().second;
+        for (const auto &x : iv) {
+            if (x.first > hi) {
+                ms += hi - lo;
+                lo = x.first;
+                hi = x.second;
+            } else {
+                hi = std::max(hi, x.second);
+            }
+        }
+        ms += hi - lo;
+        const uint64_t n = groups[e.first].size();
+        switch (e.first) {
             case kKindRaygen: ctx->stats.ms_raygen += ms; break;
-            case kKindTrace: ctx->stats.ms_trace += ms; ctx->stats.launches_trace++; break;
-            case kKindShade: ctx->stats.ms_shade += ms; ctx->stats.launches_shade++; break;
-            case kKindShadow: ctx->stats.ms_shadow += ms; ctx->stats.launches_shadow++; break;
+            case kKindTrace: ctx->stats.ms_trace += ms; ctx->stats.launches_trace += n; break;
+            case kKindShade: ctx->stats.ms_shade += ms; ctx->stats.launches_shade += n; break;
+            case kKindShadow: ctx->stats.ms_shadow += ms; ctx->stats.launches_shadow += n; break;
             case kKindGuide: ctx->stats.ms_guide += ms; break;
             case kKindGuideInfer: ctx->stats.ms_guide_infer += ms; break;
             default: ctx->stats.ms_accumulate += ms; break;
@@ -2179,10 +2196,13 @@ struct BatchGuide {
     uint32_t step = 0;    // optimizer steps taken on the ring
 };
 
-// The kernels of one launch (the sequence a hipGraph captures).  As in
-// pt_render, the shadow rays of bounce b run on the side stream and overlap
-// the traversal of bounce b + 1; under capture the side stream joins the graph
-// through the events and rejoins the pass stream before the accumulation.
+// The kernels of one launch (the sequence a hipGraph captures).  The shadow
+// rays of bounce b run on the side stream (stream2) and overlap the
+// traversal of bounce b + 1 (which reads only the queued rays and writes
+// their hits); bounce b + 1 waits for them before it reads its paths'
+// radiance and refills the shadow queue.  Under capture the side stream joins
+// the graph through the events and rejoins the pass stream before the
+// accumulation.
 // Needs ensure_dep_events(ctx, 2 * max_depth).
 void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth, const ShadeParams &sp,
                           uint32_t path_bits, bool fp64, const BatchGuide &bg, bool timing = false) {
@@ -3203,7 +3223,7 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
 
     const bool timing = (p->flags & PT_RENDER_TIMING) != 0;
     const SceneView sv = scene_view(ctx);
-    if ((rc = ensure_dep_events(ctx, 4 * max_depth + 2)) != PT_OK) return rc;
+    if ((rc = ensure_dep_events(ctx, 4)) != PT_OK) return rc;  // fork and the joins of the chains
     ShadeParams sp;
     sp.max_depth = max_depth;
     sp.rr_depth = p->rr_depth;
@@ -3212,7 +3232,7 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
     const uint32_t tg = trace_grid(ctx), sg = shade_grid(ctx, guided, guide_train);
 
     const uint32_t batches = (p->spp + per_launch - 1) / per_launch;
-    const uint32_t hist_elems = std::max(1u, batches) * 2u * ctx->num_counters;
+    const uint32_t hist_elems = std::max(1u, batches) * 4u * ctx->num_counters;  // one set per chain
     if (hist_elems > ctx->count_hist_slots || !ctx->d_count_hist) {
         dfree(ctx->d_count_hist);
         PT_HIP(hipMalloc((void **)&ctx->d_count_hist, (size_t)hist_elems * sizeof(uint32_t)));
@@ -3249,27 +3269,34 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
             return PT_ERR_HIP;
     }
 
-    // A launch of at least two frames and kTwoChainPaths paths (not training
-    // the guide) runs as two chains: the first half of its frames on the pass
-    // stream and its side stream, the second half on two more streams, each
-    // chain with its own part of the queues, counters, spill areas and
-    // events.  A chain's bounces serialize and every kernel drains before the
-    // next starts; the other chain fills the idle CUs meanwhile (terrain
-    // 1080p: +8-9% at 4, 8 and 20 frames per launch, +12% at 64).  Training
-    // launches keep one chain (no gain measured).  Every path's work is
-    // unchanged, so is the image.
-    constexpr uint64_t kTwoChainPaths = 1ull << 22;
+    // Chains: a launch's queue slots are split into up to kMaxChains
+    // contiguous ranges of at least kChainPaths (10^6) paths, each traced, shaded and
+    // shadow-tested on its own stream with its own part of the queues, its
+    // own counters and spill area.  A chain's bounces serialize and every
+    // kernel drains before the next starts; the other chains fill the idle
+    // CUs meanwhile.  Terrain 1080p, one chain against four: 815 -> 970
+    // Msamples/s at 20 frames per launch, 861 -> 1058-1071 at 64.  More
+    // streams than the hardware queues (four) cost: two chains with a second
+    // stream each for their shadow rays reached 885 and 967, three such 743.
+    // Every path's work is unchanged, so is the image.
+    constexpr uint32_t kMaxChains = 4;
+    constexpr uint32_t kChainPaths = 1000000u;
     uint32_t done = 0, batch = 0;
     uint32_t sample_cnt = ctx->sample_cnt, seed = ctx->random_seed;
     std::vector<uint32_t> chains_per_launch;
     const uint32_t gdepth = ctx->guide.cfg.guided_depth ? ctx->guide.cfg.guided_depth : max_depth;
+    const hipStream_t chain_stream[kMaxChains] = {ctx->stream, ctx->stream2, ctx->stream3, ctx->stream4};
     while (done < p->spp) {
         const uint32_t frames = std::min(per_launch, p->spp - done);
         const uint32_t total = frames * npix;
         const uint32_t nchain =
-            (!guide_train && !(p->flags & PT_RENDER_ONE_CHAIN) && frames >= 2 && (uint64_t)total >= kTwoChainPaths)
-                ? 2u
-                : 1u;
+            (p->flags & PT_RENDER_ONE_CHAIN) ? 1u : std::max(1u, std::min(kMaxChains, total / kChainPaths));
+        // With at least as many frames as chains, a chain takes a share of the
+        // frames of every pixel, so the chains run through the same pixels at
+        // the same time and share L2 (terrain 1080p, four chains of 5 frames:
+        // 968 Msamples/s at 20 frames per launch, against 914 for four bands
+        // of pixels in all 20 frames); otherwise a band of pixels.
+        const bool by_frames = frames >= nchain;
         AovBuffers aov{nullptr, nullptr, nullptr, nullptr, frames - 1, npix};
         if (sp.aov) {
             aov.albedo = ctx->d_albedo;
@@ -3288,114 +3315,106 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
             ga.rs.epoch = gs.epoch;
         }
         struct Chain {
-            uint32_t frame0, frames;
-            size_t off;  // first queue slot
-            hipStream_t main, side;
+            uint32_t frame0, frames, pix0, pixels;  // frames x tile-order pixel places of the launch
+            uint32_t slot0, paths;                  // its slots [slot0, slot0 + paths) of the queues
+            hipStream_t stream;
             uint32_t *counters, *ext, *shd, *twork, *swork;
             PathQueue q[2];
             ShadowQueue sq;
-            DeviceBvh bvh, bvh_side;
-            hipEvent_t *ev;  // per bounce: shading done, shadow rays done
+            DeviceBvh bvh;
             uint4 *gout;
-        } ch[2];
+        } ch[kMaxChains];
         for (uint32_t c = 0; c < nchain; ++c) {
             Chain &k = ch[c];
-            k.frame0 = c ? (frames + 1) / 2 : 0u;
-            k.frames = nchain == 1 ? frames : (c ? frames / 2 : (frames + 1) / 2);
-            k.off = (size_t)k.frame0 * npix;
-            k.main = c ? ctx->stream3 : ctx->stream;
-            k.side = c ? ctx->stream4 : ctx->stream2;
+            if (by_frames) {
+                k.frame0 = frames * c / nchain;
+                k.frames = frames * (c + 1) / nchain - k.frame0;
+                k.pix0 = 0;
+                k.pixels = npix;
+            } else {
+                k.frame0 = 0;
+                k.frames = frames;
+                k.pix0 = (uint32_t)((uint64_t)npix * c / nchain);
+                k.pixels = (uint32_t)((uint64_t)npix * (c + 1) / nchain) - k.pix0;
+            }
+            k.slot0 = c ? ch[c - 1].slot0 + ch[c - 1].paths : 0u;
+            k.paths = k.frames * k.pixels;
+            k.stream = chain_stream[c];
             k.counters = ctx->d_counters + (size_t)c * ctx->num_counters;
             k.ext = k.counters;                          // extension queue sizes
             k.shd = k.counters + (max_depth + 2);        // shadow queue sizes
             k.twork = k.counters + 2 * (max_depth + 2);  // trace work cursors
             k.swork = k.counters + 3 * (max_depth + 2);  // shadow work cursors
-            k.q[0] = queue_at(ctx->q[0], k.off);
-            k.q[1] = queue_at(ctx->q[1], k.off);
-            k.sq = shadow_queue_at(ctx->sq, k.off);
-            k.bvh = device_bvh_area(ctx, 2 * c);
-            k.bvh_side = device_bvh_area(ctx, 2 * c + 1);
-            k.ev = ctx->dep_events.data() + (size_t)c * 2 * max_depth;
-            k.gout = ctx->d_gout ? ctx->d_gout + 6 * k.off : nullptr;
+            k.q[0] = queue_at(ctx->q[0], k.slot0);
+            k.q[1] = queue_at(ctx->q[1], k.slot0);
+            k.sq = shadow_queue_at(ctx->sq, k.slot0);
+            k.bvh = device_bvh_area(ctx, c);
+            k.gout = ctx->d_gout ? ctx->d_gout + 6 * (size_t)k.slot0 : nullptr;
         }
-        hipEvent_t fork = ctx->dep_events[4 * max_depth], join = ctx->dep_events[4 * max_depth + 1];
-        // Timer groups: a bounce's kernels of both chains time as one launch.
+        // The other chains start after everything before on the pass stream.
+        if (nchain > 1) PT_HIP(hipEventRecord(ctx->dep_events[0], ctx->stream));
+        for (uint32_t c = 1; c < nchain; ++c) PT_HIP(hipStreamWaitEvent(ch[c].stream, ctx->dep_events[0], 0));
+        // Timer groups: a bounce's kernels of all chains time as one launch.
         const uint64_t launch_group = (uint64_t)(batch + 1) << 16;
-        if (nchain == 2) {
-            // The second chain starts after everything before on the pass stream.
-            PT_HIP(hipEventRecord(fork, ctx->stream));
-            PT_HIP(hipStreamWaitEvent(ctx->stream3, fork, 0));
-        }
         for (uint32_t c = 0; c < nchain; ++c) {
             const Chain &k = ch[c];
-            EventTimer t(ctx, timing, k.main, launch_group | 0xFFFFu);
-            const uint32_t paths = k.frames * npix;
-            const uint32_t grid = std::min<uint32_t>((paths + 255) / 256, (uint32_t)ctx->num_cu * 32u);
-            hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, k.main, k.q[0], ctx->camera, ctx->width,
-                               ctx->height, k.frames, seed + k.frame0, k.counters, ctx->num_counters, k.frame0);
+            EventTimer t(ctx, timing, k.stream, launch_group | 0xFFFFu);
+            const uint32_t grid = std::min<uint32_t>((k.paths + 255) / 256, (uint32_t)ctx->num_cu * 32u);
+            hipLaunchKernelGGL(k_raygen, dim3(grid), dim3(256), 0, k.stream, k.q[0], ctx->camera, ctx->width,
+                               ctx->height, k.frames, seed, k.counters, ctx->num_counters, k.frame0, k.pix0, k.pixels);
             t.stop(kKindRaygen);
         }
-        // Bounce b of a chain: trace and shade on its pass stream, then the
-        // shadow rays on its side stream, where they overlap the traversal of
-        // bounce b + 1 (which reads only the queued rays and writes their
-        // hits).  Bounce b + 1 waits for them before it reads its paths'
-        // radiance (the shadow rays add to it) and refills the shadow queue.
         int cur = 0;
         for (uint32_t b = 0; b < max_depth; ++b) {
             const int nxt = cur ^ 1;
+            // The guide is sampled at vertices b < guided_depth; the vertex of
+            // the last bounce never samples a direction.
             const bool infer_here = guide_infer && b < gdepth && b + 1 < max_depth;
+            const uint64_t group = launch_group | (b + 1);
             for (uint32_t c = 0; c < nchain; ++c) {
                 const Chain &k = ch[c];
                 {
-                    EventTimer t(ctx, timing, k.main, launch_group | (b + 1));
+                    EventTimer t(ctx, timing, k.stream, group);
                     hipLaunchKernelGGL(trace_kernel(ctx->counting, ctx->num_spheres > 0), dim3(tg), dim3(kTraceBlock),
-                                       0, k.main, k.q[cur], k.bvh, k.ext + b, k.twork + b, ctx->d_stat,
+                                       0, k.stream, k.q[cur], k.bvh, k.ext + b, k.twork + b, ctx->d_stat,
                                        (const uint32_t *)nullptr, 31u);
                     t.stop(kKindTrace);
                 }
-                if (b > 0) PT_HIP(hipStreamWaitEvent(k.main, k.ev[2 * (b - 1) + 1], 0));
-                // The guide is sampled at vertices b < guided_depth; the vertex
-                // of the last bounce never samples a direction.
                 GuideShadeArgs gac = ga;
                 gac.gout = infer_here ? k.gout : nullptr;
                 if (infer_here) {
-                    EventTimer t(ctx, timing, k.main, launch_group | (b + 1));
+                    EventTimer t(ctx, timing, k.stream, group);
                     hipLaunchKernelGGL(k_guide_infer, dim3(ctx->num_cu * ctx->infer_blocks_per_cu), dim3(kInferBlock),
-                                       0, k.main, gview, k.q[cur], k.ext + b, reinterpret_cast<uint2 *>(k.gout),
+                                       0, k.stream, gview, k.q[cur], k.ext + b, reinterpret_cast<uint2 *>(k.gout),
                                        ctx->d_stat + 4, 31u, InferScene{sv.tri, sv.shapes, sv.spheres, sv.materials});
                     t.stop(kKindGuideInfer);
                 }
                 {
-                    EventTimer t(ctx, timing, k.main, launch_group | (b + 1));
+                    EventTimer t(ctx, timing, k.stream, group);
                     if (guided)
-                        hipLaunchKernelGGL(k_shade_guided<false>, dim3(sg), dim3(256), 0, k.main, sv, sp, k.q[cur],
+                        hipLaunchKernelGGL(k_shade_guided<false>, dim3(sg), dim3(256), 0, k.stream, sv, sp, k.q[cur],
                                            k.ext + b, k.q[nxt], k.ext + b + 1, k.sq, k.shd + b, ctx->d_out_l, aov, gac,
                                            (const BatchViewDev *)nullptr, 31u, (const LaunchDesc *)nullptr);
                     else
-                        hipLaunchKernelGGL(shade_kernel(false, ctx->shade_feat), dim3(sg), dim3(256), 0, k.main, sv,
+                        hipLaunchKernelGGL(shade_kernel(false, ctx->shade_feat), dim3(sg), dim3(256), 0, k.stream, sv,
                                            sp, k.q[cur], k.ext + b, k.q[nxt], k.ext + b + 1, k.sq, k.shd + b,
                                            ctx->d_out_l, aov, (const BatchViewDev *)nullptr, 31u);
                     t.stop(kKindShade);
                 }
-                PT_HIP(hipEventRecord(k.ev[2 * b], k.main));
-                PT_HIP(hipStreamWaitEvent(k.side, k.ev[2 * b], 0));
                 {
-                    EventTimer t(ctx, timing, k.side, launch_group | (b + 1));
+                    EventTimer t(ctx, timing, k.stream, group);
                     hipLaunchKernelGGL(shadow_kernel(ctx->counting, ctx->num_spheres > 0),
-                                       dim3(shadow_grid(ctx, ctx->num_spheres > 0)), dim3(kTraceBlock), 0, k.side,
-                                       k.sq, k.bvh_side, k.shd + b, k.swork + b, k.q[nxt], ctx->d_out_l,
-                                       ctx->d_stat + 2, (const uint32_t *)nullptr, 31u);
+                                       dim3(shadow_grid(ctx, ctx->num_spheres > 0)), dim3(kTraceBlock), 0, k.stream,
+                                       k.sq, k.bvh, k.shd + b, k.swork + b, k.q[nxt], ctx->d_out_l, ctx->d_stat + 2,
+                                       (const uint32_t *)nullptr, 31u);
                     t.stop(kKindShadow);
                 }
-                PT_HIP(hipEventRecord(k.ev[2 * b + 1], k.side));
             }
             cur = nxt;
         }
-        for (uint32_t c = 0; c < nchain; ++c)
-            PT_HIP(hipStreamWaitEvent(ch[c].main, ch[c].ev[2 * (max_depth - 1) + 1], 0));
-        if (nchain == 2) {
-            PT_HIP(hipEventRecord(join, ctx->stream3));
-            PT_HIP(hipStreamWaitEvent(ctx->stream, join, 0));
+        for (uint32_t c = 1; c < nchain; ++c) {
+            PT_HIP(hipEventRecord(ctx->dep_events[c], ch[c].stream));
+            PT_HIP(hipStreamWaitEvent(ctx->stream, ctx->dep_events[c], 0));
         }
         {
             EventTimer t(ctx, timing);
@@ -3423,7 +3442,7 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
         PT_HIP(hipGetLastError());
         // Keep this launch's queue counters (both chains' sets); read back once
         // after the last launch.
-        PT_HIP(hipMemcpyAsync(ctx->d_count_hist + (size_t)batch * 2 * ctx->num_counters, ctx->d_counters,
+        PT_HIP(hipMemcpyAsync(ctx->d_count_hist + (size_t)batch * kMaxChains * ctx->num_counters, ctx->d_counters,
                               (size_t)nchain * ctx->num_counters * sizeof(uint32_t), hipMemcpyDeviceToDevice,
                               ctx->stream));
         chains_per_launch.push_back(nchain);
@@ -3432,13 +3451,13 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
         done += frames;
         ++batch;
     }
-    std::vector<uint32_t> hist((size_t)batch * 2 * ctx->num_counters);
+    std::vector<uint32_t> hist((size_t)batch * kMaxChains * ctx->num_counters);
     PT_HIP(hipMemcpyAsync(hist.data(), ctx->d_count_hist, hist.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
                           ctx->stream));
     PT_HIP(hipStreamSynchronize(ctx->stream));
     for (uint32_t k = 0; k < batch; ++k)
         for (uint32_t c = 0; c < chains_per_launch[k]; ++c) {
-            const uint32_t *cn = hist.data() + ((size_t)k * 2 + c) * ctx->num_counters;
+            const uint32_t *cn = hist.data() + ((size_t)k * kMaxChains + c) * ctx->num_counters;
             for (uint32_t b = 0; b < max_depth; ++b) {
                 ctx->stats.rays_extension += cn[b];
                 ctx->stats.rays_shadow += cn[max_depth + 2 + b];
@@ -3587,7 +3606,7 @@ int pt_render_noalloc(pt_ctx *ctx, const pt_render_params *p) {
         // k_raygen clears every counter before the first traversal reads them.
         const uint32_t rgrid = std::min<uint32_t>((total + 255) / 256, (uint32_t)ctx->num_cu * 32u);
         hipLaunchKernelGGL(k_raygen, dim3(rgrid), dim3(256), 0, ctx->stream, L.q[0], ctx->camera, ctx->width,
-                           ctx->height, frames, seed, L.counters, num_counters, 0u);
+                           ctx->height, frames, seed, L.counters, num_counters, 0u, 0u, npix);
         AovBuffers aov{nullptr, nullptr, nullptr, nullptr, frames - 1, npix};
         if (sp.aov) {
             aov.albedo = ctx->d_albedo;

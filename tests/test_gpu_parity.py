@@ -143,17 +143,18 @@ def test_determinism(tracer):
 
 
 @pytest.mark.parametrize("guided", [False, True])
-def test_two_chain_launch_invariance(tracer, guided):
-    """A launch of at least two frames and 2^22 paths runs its two halves as
-    concurrent chains of kernels (own queue parts, counters, streams and spill
-    areas): 34 frames at 1920x1080 (one launch, two chains; then launches of 5
-    frames, two chains each, and one of 4) equal the same frames rendered with
-    every launch as one chain, bit for bit, AOVs and ray counts included;
-    guided renders (network inference, per-chain output buffers) too."""
+def test_chained_launch_invariance(tracer, guided):
+    """A launch's paths run as up to four concurrent chains of kernels of at
+    least 10^6 paths (own queue parts, counters, streams and spill areas):
+    34 frames at 1920x1080 (one launch of four chains; then launches of 5
+    frames and one of 4, four chains each) and 3 single-frame launches (two
+    chains each) equal the same frames rendered with every launch as one
+    chain, bit for bit, AOVs and ray counts included; guided renders (network
+    inference, per-chain output buffers) too."""
     kind = pt.SCENE_DOOR_ROOM if guided else pt.SCENE_TERRAIN
-    print(f"case: {'door room' if guided else 'terrain'} 1920x1080 34 frames, two chains vs one", flush=True)
+    print(f"case: {'door room' if guided else 'terrain'} 1920x1080, chained launches vs one chain", flush=True)
     scene = pt.HostScene.generate(kind, seed=2)
-    assert scene.desc.width * scene.desc.height * 4 >= 1 << 22  # the smallest launch below: 4 frames
+    assert scene.desc.width * scene.desc.height >= 2 * 10 ** 6  # a single 1080p frame splits into two chains
     tracer.set_scene(scene)
     tracer.build_bvh()
     kw = {"mis_mode": pt.MIS_CONSISTENT, "guide": True} if guided else {}
@@ -161,16 +162,16 @@ def test_two_chain_launch_invariance(tracer, guided):
         tracer.guide_enable(seed=3)
         tracer.render(spp=4, mis_mode=pt.MIS_CONSISTENT, aov=False, guide=True, guide_train=True)
     bufs = (pt.BUF_ACCUM, pt.BUF_PRIM_ID, pt.BUF_ALBEDO, pt.BUF_NORMAL, pt.BUF_TEST)
-    for launch in (0, 5):
+    for spp, launch in ((34, 0), (34, 5), (3, 1)):
         out = []
         for one in (True, False):
             tracer.reset()
             before = tracer.stats()
-            tracer.render(spp=34, spp_per_launch=launch, one_chain=one, **kw)
+            tracer.render(spp=spp, spp_per_launch=launch, one_chain=one, **kw)
             after = tracer.stats()
             rays = {k: after[k] - before[k] for k in ("rays_extension", "rays_shadow", "guide_queries")}
             out.append(({b: tracer.read(b) for b in bufs}, rays))
         assert out[0][1] == out[1][1]
-        assert out[0][1]["rays_extension"] > 34 * scene.desc.width * scene.desc.height
+        assert out[0][1]["rays_extension"] > spp * scene.desc.width * scene.desc.height
         for b in bufs:
             assert np.array_equal(out[0][0][b], out[1][0][b]), (launch, b)
