@@ -38,7 +38,13 @@ enum KernelKind { kKindRaygen = 0, kKindTrace, kKindShade, kKindShadow, kKindAcc
 
 constexpr int kTraceBlock = 128;
 constexpr int kLdsStack = 8;   // stack entries per lane kept in LDS
-constexpr uint32_t kShadeRounds = 64;  // shading grid = this many rounds of resident blocks
+constexpr uint32_t kShadeRounds = 64;
+// Concurrent chains of a launch (pt_render and batched launches): at most
+// this many, of at least kChainPaths paths each; batched queues hold a
+// multiple of kChainLcm slots, so every chain count divides them.
+constexpr uint32_t kMaxChains = 4;
+constexpr uint32_t kChainPaths = 1000000u;
+constexpr uint32_t kChainLcm = 12;  // shading grid = this many rounds of resident blocks
 constexpr uint32_t kSpillEntries = kMaxBvhDepth + 2 - kLdsStack;
 
 // --- queue layout -------------------------------------------------------------
@@ -664,9 +670,13 @@ __global__ void k_accumulate(const float4 *out_l, float4 *accum, float4 *frame, 
 
 // --- batched views ----------------------------------------------------------------
 // main.cu:39-78 for every (view, frame, pixel) of a batched launch.
+// Chain `chain` of `chains` takes the launch's slots [lo, hi) with
+// lo = total * chain / chains, into its own queue part.
 __global__ void k_raygen_batch(PathQueue q, const LaunchDesc *ld, const BatchViewDev *views, uint32_t path_bits,
-                               uint32_t *counters, uint32_t num_counters) {
-    const uint32_t total = ld->total_paths;
+                               uint32_t *counters, uint32_t num_counters, uint32_t chain, uint32_t chains) {
+    const uint32_t all = ld->total_paths;
+    const uint32_t lo = (uint32_t)((uint64_t)all * chain / chains);
+    const uint32_t total = (uint32_t)((uint64_t)all * (chain + 1) / chains) - lo;
     const uint32_t ns = ld->n_segs;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
@@ -674,16 +684,17 @@ __global__ void k_raygen_batch(PathQueue q, const LaunchDesc *ld, const BatchVie
         for (uint32_t i = 1; i < num_counters; ++i) counters[i] = 0u;
     }
     for (uint32_t i = tid; i < total; i += gridDim.x * blockDim.x) {
-        // The segment holding slot i; within a segment (consecutive frames of
-        // one view) frames are interleaved as in k_raygen.
-        uint32_t lo = 0, hi = ns - 1;
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1u) >> 1;
-            if (ld->e[ld->g[mid].first_entry].path_begin <= i) lo = mid;
-            else hi = mid - 1u;
+        // The segment holding launch slot g; within a segment (consecutive
+        // frames of one view) frames are interleaved as in k_raygen.
+        const uint32_t g = lo + i;
+        uint32_t a = 0, z = ns - 1;
+        while (a < z) {
+            const uint32_t mid = (a + z + 1u) >> 1;
+            if (ld->e[ld->g[mid].first_entry].path_begin <= g) a = mid;
+            else z = mid - 1u;
         }
-        const LaunchSeg &sg = ld->g[lo];
-        const uint32_t local = i - ld->e[sg.first_entry].path_begin;
+        const LaunchSeg &sg = ld->g[a];
+        const uint32_t local = g - ld->e[sg.first_entry].path_begin;
         const LaunchEntry en = ld->e[sg.first_entry + local % sg.frames];
         const uint32_t view = en.view & kEntryViewMask;
         const BatchViewDev &v = views[view];
@@ -2196,16 +2207,18 @@ struct BatchGuide {
     uint32_t step = 0;    // optimizer steps taken on the ring
 };
 
-// The kernels of one launch (the sequence a hipGraph captures).  The shadow
-// rays of bounce b run on the side stream (stream2) and overlap the
-// traversal of bounce b + 1 (which reads only the queued rays and writes
-// their hits); bounce b + 1 waits for them before it reads its paths'
-// radiance and refills the shadow queue.  Under capture the side stream joins
-// the graph through the events and rejoins the pass stream before the
-// accumulation.
-// Needs ensure_dep_events(ctx, 2 * max_depth).
+// The kernels of one launch (the sequence a hipGraph captures).  As in
+// pt_render, the launch's paths run as concurrent chains, each on its own
+// stream with its own queue part (a fixed share of the queues: slot ranges
+// come from the device launch descriptor), counters and spill area; under
+// capture the chains fork from and join the pass stream through events, so
+// the graph holds them as parallel branches.  Accumulation follows the join.
+// Needs ensure_dep_events(ctx, kMaxChains) and queues of batch_queue_paths.
+uint32_t batch_chains(const BatchShape &sh) { return std::max(1u, std::min(kMaxChains, sh.capacity / kChainPaths)); }
+uint32_t batch_queue_paths(const BatchShape &sh) { return (sh.capacity + kChainLcm - 1) / kChainLcm * kChainLcm; }
 void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth, const ShadeParams &sp,
-                          uint32_t path_bits, bool fp64, const BatchGuide &bg, bool timing = false) {
+                          uint32_t path_bits, bool fp64, const BatchGuide &bg, bool timing = false,
+                          uint64_t launch_group = 0) {
     BatchState &b = ctx->batch;
     SceneView sv;
     sv.tri = b.d_tri;
@@ -2221,70 +2234,96 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
     sv.pad = 0;
     sv.tex = TexArrays{b.d_bitmaps, reinterpret_cast<const f4 *>(b.d_texels)};
     std::memset(&sv.env, 0, sizeof(sv.env));
-    const DeviceBvh bvh{b.d_nodes, b.d_leaf, ctx->d_spill};
-    const DeviceBvh bvh_side{b.d_nodes, b.d_leaf, ctx->d_spill + (size_t)ctx->spill_threads * kSpillEntries};
     AovBuffers aov{nullptr, nullptr, nullptr, nullptr, 0u, 1u};
-    uint32_t *ext = ctx->d_counters;
-    uint32_t *shd = ctx->d_counters + (max_depth + 2);
-    uint32_t *twork = ctx->d_counters + 2 * (max_depth + 2);
-    uint32_t *swork = ctx->d_counters + 3 * (max_depth + 2);
     const uint32_t tg = (uint32_t)ctx->num_cu * sh.trace_bpc;
     const uint32_t sg = (uint32_t)ctx->num_cu * sh.shade_bpc;
-    hipLaunchKernelGGL(k_raygen_batch, dim3((uint32_t)ctx->num_cu * 32u), dim3(256), 0, ctx->stream, ctx->q[0],
-                       b.d_launch, b.d_views, path_bits, ctx->d_counters, ctx->num_counters);
+    const uint32_t nchain = batch_chains(sh);
+    const size_t share = batch_queue_paths(sh) / nchain;  // queue slots per chain
+    const hipStream_t streams[kMaxChains] = {ctx->stream, ctx->stream2, ctx->stream3, ctx->stream4};
+    struct Chain {
+        hipStream_t stream;
+        uint32_t *counters, *ext, *shd, *twork, *swork;
+        PathQueue q[2];
+        ShadowQueue sq;
+        DeviceBvh bvh;
+        GuideShadeArgs ga;
+        uint4 *gout;
+    } ch[kMaxChains];
+    for (uint32_t c = 0; c < nchain; ++c) {
+        Chain &k = ch[c];
+        k.stream = streams[c];
+        k.counters = ctx->d_counters + (size_t)c * ctx->num_counters;
+        k.ext = k.counters;
+        k.shd = k.counters + (max_depth + 2);
+        k.twork = k.counters + 2 * (max_depth + 2);
+        k.swork = k.counters + 3 * (max_depth + 2);
+        k.q[0] = queue_at(ctx->q[0], c * share);
+        k.q[1] = queue_at(ctx->q[1], c * share);
+        k.sq = shadow_queue_at(ctx->sq, c * share);
+        k.bvh = DeviceBvh{b.d_nodes, b.d_leaf, ctx->d_spill + (size_t)c * ctx->spill_threads * kSpillEntries};
+        k.gout = ctx->d_gout ? ctx->d_gout + 6 * c * share : nullptr;
+        k.ga = bg.ga;
+    }
+    if (nchain > 1) (void)hipEventRecord(ctx->dep_events[0], ctx->stream);
+    for (uint32_t c = 1; c < nchain; ++c) (void)hipStreamWaitEvent(ch[c].stream, ctx->dep_events[0], 0);
+    for (uint32_t c = 0; c < nchain; ++c)
+        hipLaunchKernelGGL(k_raygen_batch, dim3((uint32_t)ctx->num_cu * 32u), dim3(256), 0, ch[c].stream, ch[c].q[0],
+                           b.d_launch, b.d_views, path_bits, ch[c].counters, ctx->num_counters, c, nchain);
     int cur = 0;
     for (uint32_t bb = 0; bb < max_depth; ++bb) {
         const int nxt = cur ^ 1;
-        {
-            EventTimer t(ctx, timing);
-            hipLaunchKernelGGL(trace_kernel(false, !b.spheres.empty()), dim3(tg), dim3(kTraceBlock), 0, ctx->stream,
-                               ctx->q[cur], bvh, ext + bb, twork + bb, ctx->d_stat, (const uint32_t *)b.d_roots,
-                               path_bits);
-            t.stop(kKindTrace);
-        }
-        if (bb > 0) (void)hipStreamWaitEvent(ctx->stream, ctx->dep_events[2 * (bb - 1) + 1], 0);
-        if (bg.infer || bg.train) {
-            GuideShadeArgs ga = bg.ga;
-            const bool infer_here = bg.infer && bb < bg.gdepth && bb + 1 < max_depth;
-            if (!infer_here) ga.gout = nullptr;
-            if (infer_here) {
-                EventTimer t(ctx, timing);
-                hipLaunchKernelGGL(k_guide_infer, dim3(ctx->num_cu * ctx->infer_blocks_per_cu), dim3(kInferBlock), 0,
-                                   ctx->stream, bg.view, ctx->q[cur], ext + bb,
-                                   reinterpret_cast<uint2 *>(ctx->d_gout), ctx->d_stat + 4, path_bits,
-                                   InferScene{sv.tri, sv.shapes, sv.spheres, sv.materials});
-                t.stop(kKindGuideInfer);
+        const uint64_t group = launch_group ? (launch_group | (bb + 1)) : 0u;
+        const bool infer_here = bg.infer && bb < bg.gdepth && bb + 1 < max_depth;
+        for (uint32_t c = 0; c < nchain; ++c) {
+            const Chain &k = ch[c];
+            {
+                EventTimer t(ctx, timing, k.stream, group);
+                hipLaunchKernelGGL(trace_kernel(false, !b.spheres.empty()), dim3(tg), dim3(kTraceBlock), 0, k.stream,
+                                   k.q[cur], k.bvh, k.ext + bb, k.twork + bb, ctx->d_stat, (const uint32_t *)b.d_roots,
+                                   path_bits);
+                t.stop(kKindTrace);
             }
-            EventTimer t(ctx, timing);
-            hipLaunchKernelGGL(k_shade_guided<true>, dim3((uint32_t)ctx->num_cu * sh.shade_bpc), dim3(256), 0,
-                               ctx->stream, sv, sp, ctx->q[cur], ext + bb, ctx->q[nxt], ext + bb + 1, ctx->sq,
-                               shd + bb, ctx->d_out_l, aov, ga, (const BatchViewDev *)b.d_views, path_bits,
-                               (const LaunchDesc *)b.d_launch);
-            t.stop(kKindShade);
-        } else {
-            EventTimer t(ctx, timing);
-            hipLaunchKernelGGL(shade_kernel(true, b.feat), dim3(sg), dim3(sh.shade_block), 0, ctx->stream, sv, sp,
-                               ctx->q[cur], ext + bb, ctx->q[nxt], ext + bb + 1, ctx->sq, shd + bb, ctx->d_out_l, aov,
-                               (const BatchViewDev *)b.d_views, path_bits);
-            t.stop(kKindShade);
+            if (bg.infer || bg.train) {
+                GuideShadeArgs ga = k.ga;
+                ga.gout = infer_here ? k.gout : nullptr;
+                if (infer_here) {
+                    EventTimer t(ctx, timing, k.stream, group);
+                    hipLaunchKernelGGL(k_guide_infer, dim3(ctx->num_cu * ctx->infer_blocks_per_cu), dim3(kInferBlock),
+                                       0, k.stream, bg.view, k.q[cur], k.ext + bb, reinterpret_cast<uint2 *>(k.gout),
+                                       ctx->d_stat + 4, path_bits, InferScene{sv.tri, sv.shapes, sv.spheres, sv.materials});
+                    t.stop(kKindGuideInfer);
+                }
+                EventTimer t(ctx, timing, k.stream, group);
+                hipLaunchKernelGGL(k_shade_guided<true>, dim3((uint32_t)ctx->num_cu * sh.shade_bpc), dim3(256), 0,
+                                   k.stream, sv, sp, k.q[cur], k.ext + bb, k.q[nxt], k.ext + bb + 1, k.sq, k.shd + bb,
+                                   ctx->d_out_l, aov, ga, (const BatchViewDev *)b.d_views, path_bits,
+                                   (const LaunchDesc *)b.d_launch);
+                t.stop(kKindShade);
+            } else {
+                EventTimer t(ctx, timing, k.stream, group);
+                hipLaunchKernelGGL(shade_kernel(true, b.feat), dim3(sg), dim3(sh.shade_block), 0, k.stream, sv, sp,
+                                   k.q[cur], k.ext + bb, k.q[nxt], k.ext + bb + 1, k.sq, k.shd + bb, ctx->d_out_l, aov,
+                                   (const BatchViewDev *)b.d_views, path_bits);
+                t.stop(kKindShade);
+            }
+            {
+                EventTimer t(ctx, timing, k.stream, group);
+                hipLaunchKernelGGL(shadow_kernel(false, !b.spheres.empty()), dim3(shadow_grid(ctx, !b.spheres.empty())),
+                                   dim3(kTraceBlock), 0, k.stream, k.sq, k.bvh, k.shd + bb, k.swork + bb, k.q[nxt],
+                                   ctx->d_out_l, ctx->d_stat + 2, (const uint32_t *)b.d_roots, path_bits);
+                t.stop(kKindShadow);
+            }
         }
-        (void)hipEventRecord(ctx->dep_events[2 * bb], ctx->stream);
-        (void)hipStreamWaitEvent(ctx->stream2, ctx->dep_events[2 * bb], 0);
-        {
-            EventTimer t(ctx, timing, ctx->stream2);
-            hipLaunchKernelGGL(shadow_kernel(false, !b.spheres.empty()), dim3(shadow_grid(ctx, !b.spheres.empty())),
-                               dim3(kTraceBlock), 0, ctx->stream2,
-                               ctx->sq, bvh_side, shd + bb, swork + bb, ctx->q[nxt], ctx->d_out_l, ctx->d_stat + 2,
-                               (const uint32_t *)b.d_roots, path_bits);
-            t.stop(kKindShadow);
-        }
-        (void)hipEventRecord(ctx->dep_events[2 * bb + 1], ctx->stream2);
         cur = nxt;
     }
-    (void)hipStreamWaitEvent(ctx->stream, ctx->dep_events[2 * (max_depth - 1) + 1], 0);
+    for (uint32_t c = 1; c < nchain; ++c) {
+        (void)hipEventRecord(ctx->dep_events[c], ch[c].stream);
+        (void)hipStreamWaitEvent(ctx->stream, ctx->dep_events[c], 0);
+    }
     hipLaunchKernelGGL(k_accumulate_batch, dim3((uint32_t)ctx->num_cu * 16u), dim3(256), 0, ctx->stream,
                        ctx->d_out_l, b.d_launch, b.d_views, fp64 ? 1u : 0u);
-    hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(64), 0, ctx->stream, ctx->d_counters, max_depth, b.d_rays);
+    for (uint32_t c = 0; c < nchain; ++c)
+        hipLaunchKernelGGL(k_batch_count, dim3(1), dim3(64), 0, ctx->stream, ch[c].counters, max_depth, b.d_rays);
 }
 
 // Width of a launch of guide-training frames: by default at most 2^23 paths
@@ -2323,10 +2362,10 @@ int batch_guide_setup(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &s
     bg.ga.fixed_c = std::min(gs.cfg.fixed_selection, 1.f);
     bg.gdepth = gs.cfg.guided_depth ? gs.cfg.guided_depth : max_depth;
     if (bg.infer) {
-        if (ctx->gout_cap < sh.capacity) {
+        if (ctx->gout_cap < batch_queue_paths(sh)) {
             dfree(ctx->d_gout);
-            PT_HIP(hipMalloc((void **)&ctx->d_gout, (size_t)sh.capacity * 6 * sizeof(uint4)));
-            ctx->gout_cap = sh.capacity;
+            PT_HIP(hipMalloc((void **)&ctx->d_gout, (size_t)batch_queue_paths(sh) * 6 * sizeof(uint4)));
+            ctx->gout_cap = batch_queue_paths(sh);
         }
         bg.ga.gout = ctx->d_gout;
     }
@@ -2390,7 +2429,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
                      "path-id range";
         return PT_ERR_CAPACITY;
     }
-    if ((rc = ensure_queues(ctx, sh.capacity, max_depth)) != PT_OK) return rc;
+    if ((rc = ensure_queues(ctx, batch_queue_paths(sh), max_depth)) != PT_OK) return rc;
     if ((rc = ensure_spill(ctx)) != PT_OK) return rc;
     BatchGuide bg;
     bg.infer = (p.render_flags & PT_RENDER_GUIDE) != 0;
@@ -2435,7 +2474,7 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
     spar.aov = 0;
     const bool fp64 = (p.render_flags & PT_RENDER_FP64_ACCUM) != 0;
     if ((rc = batch_guide_setup(ctx, p, sh, max_depth, plan, bg)) != PT_OK) return rc;
-    if ((rc = ensure_dep_events(ctx, 2 * max_depth)) != PT_OK) return rc;
+    if ((rc = ensure_dep_events(ctx, kMaxChains)) != PT_OK) return rc;
     // Per-kernel event timing (direct launches only; around the training
     // steps in both modes).
     const bool timing = (p.render_flags & PT_RENDER_TIMING) != 0;
@@ -2472,7 +2511,8 @@ int batch_run(pt_ctx *ctx, const pt_batch_params &p, const BatchShape &sh, bool 
         if (e == hipSuccess) {
             if (graph) e = hipGraphLaunch(ge, ctx->stream);
             else {
-                batch_enqueue_launch(ctx, sh, max_depth, spar, path_bits, fp64, bg, timing);
+                batch_enqueue_launch(ctx, sh, max_depth, spar, path_bits, fp64, bg, timing,
+                                     (uint64_t)(executed + 1) << 16);
                 e = hipGetLastError();
             }
         }
@@ -3279,8 +3319,6 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
     // streams than the hardware queues (four) cost: two chains with a second
     // stream each for their shadow rays reached 885 and 967, three such 743.
     // Every path's work is unchanged, so is the image.
-    constexpr uint32_t kMaxChains = 4;
-    constexpr uint32_t kChainPaths = 1000000u;
     uint32_t done = 0, batch = 0;
     uint32_t sample_cnt = ctx->sample_cnt, seed = ctx->random_seed;
     std::vector<uint32_t> chains_per_launch;
