@@ -1128,9 +1128,9 @@ struct ShapeHost {
 struct pt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // Streams of the second to fourth chains of a pt_render launch; stream2
-    // is also the side stream of batched launches, where the shadow rays of
-    // bounce b overlap the traversal of bounce b + 1.  dep_events order them.
+    // Streams of the second to fourth chains of a launch (pt_render and
+    // batched launches); dep_events fork them from and join them to the pass
+    // stream.
     hipStream_t stream2 = nullptr, stream3 = nullptr, stream4 = nullptr;
     std::vector<hipEvent_t> dep_events;
     std::string error;
@@ -1454,16 +1454,14 @@ int ensure_spill(pt_ctx *ctx) {
         kTraceBlock;
     if (ctx->d_spill && ctx->spill_threads >= threads) return PT_OK;
     dfree(ctx->d_spill);
-    // Four areas: one per chain of a pt_render launch (batched launches use
-    // the first for the pass stream and the second for their side stream).
+    // Four areas: one per chain of a launch.
     PT_HIP(hipMalloc((void **)&ctx->d_spill, 4 * (size_t)threads * kSpillEntries * sizeof(uint2)));
     ctx->spill_threads = threads;
     return PT_OK;
 }
 
 DeviceBvh device_bvh(pt_ctx *ctx) { return DeviceBvh{ctx->d_nodes, ctx->d_leaf, ctx->d_spill}; }
-// The tree with spill area `area` (chain `area` of a pt_render launch; in
-// batched launches 0 is the pass stream's, 1 the side stream's).
+// The tree with the spill area of chain `area` of a launch.
 DeviceBvh device_bvh_area(pt_ctx *ctx, uint32_t area) {
     return DeviceBvh{ctx->d_nodes, ctx->d_leaf, ctx->d_spill + (size_t)area * ctx->spill_threads * kSpillEntries};
 }
@@ -1472,9 +1470,8 @@ PathQueue queue_at(const PathQueue &q, size_t off) {
 }
 ShadowQueue shadow_queue_at(const ShadowQueue &q, size_t off) { return ShadowQueue{q.s0 + off, q.s1 + off, q.s2 + off}; }
 
-// Events ordering the streams of a render: the fork and joins of a pt_render
-// launch's chains; in batched launches two per bounce (shading done, shadow
-// rays done) between the pass stream and the side stream.
+// Events ordering the streams of a render: the fork and the joins of a
+// launch's chains.
 int ensure_dep_events(pt_ctx *ctx, uint32_t n) {
     while (ctx->dep_events.size() < n) {
         hipEvent_t e = nullptr;
@@ -2701,7 +2698,7 @@ int pt_create(int hip_device, pt_ctx **out) {
     if (hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream3, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream4, hipStreamNonBlocking) != hipSuccess) {
-        g_create_error = "failed to create the side streams";
+        g_create_error = "failed to create the chain streams";
         for (hipStream_t st : {ctx->stream, ctx->stream2, ctx->stream3})
             if (st) (void)hipStreamDestroy(st);
         delete ctx;
