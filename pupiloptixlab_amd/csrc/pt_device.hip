@@ -24,6 +24,7 @@
 #include "../../include/pupil_hip.h"
 #include "pt_bvh.h"
 #include "pt_batch_plan.h"
+#include "pt_chains.h"
 #include "pt_bvh_build.h"
 #include "pt_denoise.h"
 #include "pt_guide_state.h"
@@ -38,13 +39,7 @@ enum KernelKind { kKindRaygen = 0, kKindTrace, kKindShade, kKindShadow, kKindAcc
 
 constexpr int kTraceBlock = 128;
 constexpr int kLdsStack = 8;   // stack entries per lane kept in LDS
-constexpr uint32_t kShadeRounds = 64;
-// Concurrent chains of a launch (pt_render and batched launches): at most
-// this many, of at least kChainPaths paths each; batched queues hold a
-// multiple of kChainLcm slots, so every chain count divides them.
-constexpr uint32_t kMaxChains = 4;
-constexpr uint32_t kChainPaths = 1000000u;
-constexpr uint32_t kChainLcm = 12;  // shading grid = this many rounds of resident blocks
+constexpr uint32_t kShadeRounds = 64;  // shading grid = this many rounds of resident blocks
 constexpr uint32_t kSpillEntries = kMaxBvhDepth + 2 - kLdsStack;
 
 // --- queue layout -------------------------------------------------------------
@@ -179,8 +174,9 @@ __global__ void k_raygen(PathQueue q, CameraRec cam, uint32_t width, uint32_t he
         for (uint32_t i = 1; i < num_counters; ++i) counters[i] = 0u;
     }
     for (uint32_t i = tid; i < total; i += gridDim.x * blockDim.x) {
-        const uint32_t f = frame0 + i % frames;
-        const uint32_t pixel = tiled_pixel(pix0 + i / frames, width, height);
+        uint32_t f, place;
+        chain_slot(frame0, frames, pix0, i, f, place);
+        const uint32_t pixel = tiled_pixel(place, width, height);
         const uint32_t x = pixel % width, y = pixel / width;
         Rng rng = rng_init(pixel, frame_seed0 + f);
         const float jx = rng_next(rng);
@@ -675,8 +671,8 @@ __global__ void k_accumulate(const float4 *out_l, float4 *accum, float4 *frame, 
 __global__ void k_raygen_batch(PathQueue q, const LaunchDesc *ld, const BatchViewDev *views, uint32_t path_bits,
                                uint32_t *counters, uint32_t num_counters, uint32_t chain, uint32_t chains) {
     const uint32_t all = ld->total_paths;
-    const uint32_t lo = (uint32_t)((uint64_t)all * chain / chains);
-    const uint32_t total = (uint32_t)((uint64_t)all * (chain + 1) / chains) - lo;
+    const uint32_t lo = batch_chain_begin(all, chain, chains);
+    const uint32_t total = batch_chain_begin(all, chain + 1, chains) - lo;
     const uint32_t ns = ld->n_segs;
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     if (tid == 0) {
@@ -2211,8 +2207,8 @@ struct BatchGuide {
 // capture the chains fork from and join the pass stream through events, so
 // the graph holds them as parallel branches.  Accumulation follows the join.
 // Needs ensure_dep_events(ctx, kMaxChains) and queues of batch_queue_paths.
-uint32_t batch_chains(const BatchShape &sh) { return std::max(1u, std::min(kMaxChains, sh.capacity / kChainPaths)); }
-uint32_t batch_queue_paths(const BatchShape &sh) { return (sh.capacity + kChainLcm - 1) / kChainLcm * kChainLcm; }
+uint32_t batch_chains(const BatchShape &sh) { return chain_count(sh.capacity, false); }
+uint32_t batch_queue_paths(const BatchShape &sh) { return batch_queue_slots(sh.capacity); }
 void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth, const ShadeParams &sp,
                           uint32_t path_bits, bool fp64, const BatchGuide &bg, bool timing = false,
                           uint64_t launch_group = 0) {
@@ -2235,7 +2231,7 @@ void batch_enqueue_launch(pt_ctx *ctx, const BatchShape &sh, uint32_t max_depth,
     const uint32_t tg = (uint32_t)ctx->num_cu * sh.trace_bpc;
     const uint32_t sg = (uint32_t)ctx->num_cu * sh.shade_bpc;
     const uint32_t nchain = batch_chains(sh);
-    const size_t share = batch_queue_paths(sh) / nchain;  // queue slots per chain
+    const size_t share = batch_share(batch_queue_paths(sh), nchain);  // queue slots per chain
     const hipStream_t streams[kMaxChains] = {ctx->stream, ctx->stream2, ctx->stream3, ctx->stream4};
     struct Chain {
         hipStream_t stream;
@@ -3324,14 +3320,14 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
     while (done < p->spp) {
         const uint32_t frames = std::min(per_launch, p->spp - done);
         const uint32_t total = frames * npix;
-        const uint32_t nchain =
-            (p->flags & PT_RENDER_ONE_CHAIN) ? 1u : std::max(1u, std::min(kMaxChains, total / kChainPaths));
+        const uint32_t nchain = chain_count(total, (p->flags & PT_RENDER_ONE_CHAIN) != 0);
         // With at least as many frames as chains, a chain takes a share of the
         // frames of every pixel, so the chains run through the same pixels at
         // the same time and share L2 (terrain 1080p, four chains of 5 frames:
         // 968 Msamples/s at 20 frames per launch, against 914 for four bands
         // of pixels in all 20 frames); otherwise a band of pixels.
-        const bool by_frames = frames >= nchain;
+        ChainPart part[kMaxChains];
+        chain_parts(frames, npix, nchain, part);
         AovBuffers aov{nullptr, nullptr, nullptr, nullptr, frames - 1, npix};
         if (sp.aov) {
             aov.albedo = ctx->d_albedo;
@@ -3361,19 +3357,12 @@ static int pt_render_unguarded(pt_ctx *ctx, const pt_render_params *p) {
         } ch[kMaxChains];
         for (uint32_t c = 0; c < nchain; ++c) {
             Chain &k = ch[c];
-            if (by_frames) {
-                k.frame0 = frames * c / nchain;
-                k.frames = frames * (c + 1) / nchain - k.frame0;
-                k.pix0 = 0;
-                k.pixels = npix;
-            } else {
-                k.frame0 = 0;
-                k.frames = frames;
-                k.pix0 = (uint32_t)((uint64_t)npix * c / nchain);
-                k.pixels = (uint32_t)((uint64_t)npix * (c + 1) / nchain) - k.pix0;
-            }
-            k.slot0 = c ? ch[c - 1].slot0 + ch[c - 1].paths : 0u;
-            k.paths = k.frames * k.pixels;
+            k.frame0 = part[c].frame0;
+            k.frames = part[c].frames;
+            k.pix0 = part[c].pix0;
+            k.pixels = part[c].pixels;
+            k.slot0 = part[c].slot0;
+            k.paths = part[c].paths;
             k.stream = chain_stream[c];
             k.counters = ctx->d_counters + (size_t)c * ctx->num_counters;
             k.ext = k.counters;                          // extension queue sizes
