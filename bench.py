@@ -15,10 +15,12 @@ of the guided tracer at equal wall-clock on the door-room scene (1080p, depth
 multi-view batch (64 Cornell variants), `guided_batched` guided plus batched (2 door-room + 2
 terrain variants at 1080p with per-view hash grids and a shared MLP; `--guided-batch-full` runs
 the 16 + 16 views).  The roofline block uses
-the kernel with the largest share of GPU time; its algorithmic bytes come from
-a counting pass over the same frames (identical rays: the tracer is
-deterministic) and its durations from HIP events recorded on the pass stream
-during the timed region.  The CPU baseline is the oracle (scalar C++,
+the kernel that takes longest alone; its algorithmic bytes come from a counting
+pass over the same frames (identical rays: the tracer is deterministic) and its
+time from HIP events recorded during the timed region on the streams of the
+launch's concurrent chains (its active time: the union of its dispatches'
+intervals); `roofline.single_chain` repeats the frames with every launch as one
+chain and prices the kernel alone.  The CPU baseline is the oracle (scalar C++,
 multithreaded) on a bounded band of rows of the same frame.
 """
 import argparse
