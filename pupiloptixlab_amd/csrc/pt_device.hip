@@ -1435,8 +1435,8 @@ uint32_t shadow_grid(pt_ctx *ctx, bool spheres) {
 // and blocks that find no work exit at once.  Terrain 1080p, 64 spp,
 // k_shade<false> per render at 1, 4, 16 and 64 rounds: 49.2, 48.0, 46.9 and
 // 46.3 ms.  k_shade_guided<false> uses the same rounds when it only samples;
-// while training it keeps 8 blocks per CU, since its training records are
-// appended in block order until the record buffer is full.
+// while training it keeps a grid of 8 blocks per CU (its training records are
+// indexed by path and vertex, so the grid does not change them).
 uint32_t shade_grid(pt_ctx *ctx, bool guided, bool training) {
     if (guided && training) return (uint32_t)ctx->num_cu * 8u;
     return (uint32_t)ctx->num_cu * kShadeRounds * (guided ? ctx->shade_guided_blocks_per_cu : ctx->shade_blocks_per_cu);
