@@ -25,6 +25,7 @@
 #include "pt_bvh.h"
 #include "pt_batch_plan.h"
 #include "pt_chains.h"
+#include "pt_timing.h"
 #include "pt_bvh_build.h"
 #include "pt_denoise.h"
 #include "pt_guide_state.h"
@@ -1548,20 +1549,7 @@ void collect_event_times(pt_ctx *ctx) {
         groups[p.kind].insert(p.group);
     }
     for (auto &e : spans) {
-        auto &iv = e.second;
-        std::sort(iv.begin(), iv.end());
-        double ms = 0.0;
-        float lo = iv.front().first, hi = iv.front().second;
-        for (const auto &x : iv) {
-            if (x.first > hi) {
-                ms += hi - lo;
-                lo = x.first;
-                hi = x.second;
-            } else {
-                hi = std::max(hi, x.second);
-            }
-        }
-        ms += hi - lo;
+        const double ms = interval_union(e.second);
         const uint64_t n = groups[e.first].size();
         switch (e.first) {
             case kKindRaygen: ctx->stats.ms_raygen += ms; break;
