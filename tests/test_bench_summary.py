@@ -59,3 +59,15 @@ def test_summary_scales_and_counts_chains(tmp_path, monkeypatch):
     assert note is None and same["trace"] == tr["hbm_bytes_per_launch"]
     half, note = bench.measured_traffic(10)
     assert note is not None and abs(half["trace"] - tr["hbm_bytes_per_launch"] / 2) < 1e-6
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N with N > 1 exits with a message
+    before touching a device."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                         text=True, timeout=60)
+    assert out.returncode == 2
+    assert "measures one GPU" in out.stderr and out.stdout == ""
