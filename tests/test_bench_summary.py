@@ -1,8 +1,9 @@
-"""bench.py's counter summary (--summarize-pmc) and its reading back for the
-bench line's `traffic` field, on synthetic rocprofv3 counter files: each
-counter is scaled by the factor that maps the copy probe's reading onto its
-known bytes, per-dispatch figures are averaged over a kernel's dispatches,
-and a bounce of a launch run as n concurrent chains is n dispatches."""
+"""Host-side parts of bench.py: the counter summary (--summarize-pmc) and its
+reading back for the bench line's `traffic` field, on synthetic rocprofv3
+counter files (each counter is scaled by the factor that maps the copy
+probe's reading onto its known bytes, per-dispatch figures are averaged over
+a kernel's dispatches, and a bounce of a launch run as n concurrent chains is
+n dispatches); and the refusal of runs on more than one GPU."""
 import csv
 import json
 import os
